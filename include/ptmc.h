@@ -1,0 +1,141 @@
+/*
+ * ptmc.h — C ABI of the MI355X parallel-tempering engine (libptmc.so).
+ *
+ * Drop-in boundary for the hot path of ParallelTemperingMonteCarlo.jl:
+ * single-atom displacement moves with incremental energy updates, Metropolis
+ * acceptance and temperature-ladder replica exchange.  A handle holds any
+ * number of independent PT runs (one temperature ladder each) that share one
+ * potential and one ensemble; all of them advance together on one GPU.
+ *
+ * Conventions: plain pointers and sizes, caller-owned host buffers that are
+ * copied in or out, integer status codes (PTMC_OK = 0), no exceptions across
+ * the ABI.  A handle is used from one host thread at a time; different
+ * handles are independent.  Arrays indexed by temperature follow the
+ * reference's MCState vector order (temperature position 1..R, here 0..R-1).
+ */
+#ifndef PTMC_H
+#define PTMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTMC_OK 0
+#define PTMC_E_ARG 1         /* invalid argument */
+#define PTMC_E_STATE 2       /* call not valid in the handle's state */
+#define PTMC_E_DEVICE 3      /* HIP runtime error */
+#define PTMC_E_SPHERE 4      /* "too many moves out of binding sphere" (src/MCMoves.jl:132) */
+#define PTMC_E_UNSUPPORTED 5 /* combination not built */
+#define PTMC_E_NO_DEVICE 6   /* no usable GPU */
+
+typedef struct ptmc_engine *ptmc_handle;
+
+/* Engine-wide settings.  Counterparts: MCParams (src/InputParams.jl:35-63),
+ * the ensemble (src/Ensembles.jl:42-86) and the boundary condition type
+ * (src/BoundaryConditions.jl:44-116). */
+typedef struct {
+    int32_t precision;         /* 0: FP64 everywhere; 1: binary32 positions and dE, FP64 totals */
+    int32_t ensemble;          /* 0: NVT, 1: NPT (uniform volume moves) */
+    int32_t boundary;          /* 0: SphericalBC, 1: CubicBC */
+    int32_t exchange_mode;     /* 0: reference (p = 0.1 per cycle, one random adjacent pair,
+                                      src/MCRun.jl:203, src/Exchange.jl:135);
+                                  1: one random adjacent pair every cycle;
+                                  2: all even/odd adjacent pairs, alternating cycles */
+    int32_t pair_form;         /* 0: Horner in 1/r^2 (fast); 1: the reference's loop with
+                                  repeated division (src/EnergyEvaluation.jl:923-942), FP64 only */
+    int32_t n_adjust;          /* MCParams.n_adjust (default 100) */
+    int32_t mc_sample;         /* MCParams.mc_sample (default 1) */
+    int32_t n_bin;             /* MCParams.n_bin (default 100) */
+    double min_acc, max_acc;   /* MCParams.min_acc/max_acc (0.4, 0.6) */
+    int32_t recompute_every;   /* full FP64 energy recompute every this many cycles (0 = never) */
+    int32_t lanes_per_replica; /* 0: chosen per atom count; else 1, 2, 4, 8 or 16 */
+    int32_t cycles_per_launch; /* cycles per kernel launch (0 = 256) */
+    int32_t device;            /* HIP device ordinal */
+    uint64_t seed;             /* Philox key */
+} ptmc_config;
+
+/* One PT run.  Counterparts: TempGrid (src/InputParams.jl:77-95), the start
+ * configuration(s) given to ptmc_run! (src/MCRun.jl:363-376,
+ * src/Initialization.jl:87-95 takes start_config[mod1(i, n)]), and the
+ * boundary condition / pressure. */
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_temps;
+    double radius;             /* SphericalBC radius (bohr) */
+    double box_length;         /* CubicBC side length (bohr) */
+    double pressure;           /* NPT pressure (Hartree / bohr^3) */
+    const double *beta;        /* n_temps inverse temperatures (1/Hartree) */
+    const double *x0;          /* n_configs * n_atoms * 3 start positions (bohr) */
+    int32_t n_configs;         /* start configuration of temperature i is i mod n_configs */
+    double max_displ_atom;     /* initial atom step (reference default 0.1, src/MCStates.jl:104) */
+    double max_displ_vol;      /* initial volume step (reference default 0.1) */
+    int32_t lanes_per_replica; /* 0: engine default */
+    uint32_t run_key;          /* Philox counter word for this run; 0xffffffff = run index */
+} ptmc_run_desc;
+
+/* Lifecycle. */
+int ptmc_create(const ptmc_config *cfg, ptmc_handle *out);
+int ptmc_destroy(ptmc_handle h);
+int ptmc_last_error(ptmc_handle h, char *buf, size_t len);
+
+/* Potentials — the plugin API of src/EnergyEvaluation.jl:70-76:
+ * ELJPotentialEven / ELJPotential (:138-153, :891-908) and
+ * EmbeddedAtomPotential (:1059-1073, coefficients as stored there). */
+int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, const double *coeff);
+int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam);
+
+/* Runs.  Replaces initialisation (src/Initialization.jl:70-101) and
+ * MCState construction (src/MCStates.jl:137-162): energies of the start
+ * configurations are computed on the device in FP64. */
+int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *desc, int32_t *run_index);
+int ptmc_prepare(ptmc_handle h);
+
+/* Cycles.  ptmc_cycles enqueues n_cycles Monte-Carlo cycles of every run
+ * (mc_cycle!, src/MCRun.jl:192-253) on the engine's stream and waits for
+ * them; device-side errors surface at ptmc_sync.
+ * ptmc_start_phase restarts the phase-local cycle index used by n_adjust and
+ * mc_sample (equilibration cycles count from 1, src/MCRun.jl:296-298, and so
+ * do production cycles, :397); production = 1 turns on sampling_step!. */
+int ptmc_start_phase(ptmc_handle h, int32_t production);
+int ptmc_cycles(ptmc_handle h, int64_t n_cycles);
+int ptmc_sync(ptmc_handle h);
+
+/* Histogram range of a run (initialise_histograms!, src/Sampling.jl:141-250):
+ * delta_en = (en_max - en_min) / (n_bin - 1), delta_v = (v_max - v_min) / n_bin.
+ * Clears the run's histograms. */
+int ptmc_set_histogram(ptmc_handle h, int32_t run, double en_min, double en_max, double v_min,
+                       double v_max);
+/* reset_counters (src/MCRun.jl:270-276) for every run. */
+int ptmc_reset_counters(ptmc_handle h);
+
+/* Read-back, temperature order. */
+int ptmc_get_energies(ptmc_handle h, int32_t run, double *en);
+int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *box);
+int ptmc_get_counters(ptmc_handle h, int32_t run, double *max_displ, int64_t *count_atom,
+                      int64_t *count_vol, int64_t *count_exc, double *ham);
+int ptmc_get_histograms(ptmc_handle h, int32_t run, double *hist); /* n_temps * hist_stride */
+int ptmc_hist_stride(ptmc_handle h);                                /* n_bin+2 or (n_bin+2)^2 */
+int ptmc_get_drift(ptmc_handle h, int32_t run, double *drift);     /* max |E_run - E_FP64| */
+
+/* Write-back of a run's state (checkpoint restart, src/ReadSave.jl:436-481). */
+int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *box,
+                   const double *max_displ, const int64_t *count_atom, const int64_t *count_vol,
+                   const int64_t *count_exc, const double *ham);
+
+/* Per-move trace of the next n_cycles cycles (parity tests). */
+int ptmc_set_trace(ptmc_handle h, int32_t n_cycles);
+int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t *idx);
+
+/* Introspection for benchmarks: number of workgroups, kernel time (HIP
+ * events on the engine's stream) accumulated since the last reset. */
+int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, int64_t *n_lanes);
+int ptmc_run_lanes(ptmc_handle h, int32_t run);
+int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

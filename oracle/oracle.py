@@ -1,0 +1,263 @@
+"""ORACLE — test infrastructure only.
+
+Python driver of the CPU restatement in ptmc_oracle.c (liboracle.so) plus a
+restatement of the ptmc_run! orchestration (src/MCRun.jl:282-432,
+src/Sampling.jl:141-158, :377-399).  Only tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg import this module; the shipped engine never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+kB = 3.16681196e-6
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class orc_params(C.Structure):
+    _fields_ = [
+        ("n_atoms", C.c_int32), ("n_temps", C.c_int32), ("ensemble", C.c_int32), ("bc", C.c_int32),
+        ("pair_form", C.c_int32), ("dist_cache", C.c_int32), ("lanes", C.c_int32), ("fp32", C.c_int32),
+        ("exchange_mode", C.c_int32), ("n_adjust", C.c_int32), ("mc_sample", C.c_int32), ("n_bin", C.c_int32),
+        ("sampling", C.c_int32), ("math_lib", C.c_int32),
+        ("min_acc", C.c_double), ("max_acc", C.c_double), ("radius2", C.c_double), ("pressure", C.c_double),
+        ("seed", C.c_uint64), ("run", C.c_uint32),
+        ("sweep0", C.c_int64), ("cycle0", C.c_int64),
+        ("en_min", C.c_double), ("delta_en_hist", C.c_double), ("v_min", C.c_double), ("delta_v_hist", C.c_double),
+    ]
+
+
+class orc_pot(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ncoeff", C.c_int32), ("coeff", C.c_double * 16),
+                ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+class orc_state(C.Structure):
+    _fields_ = [
+        ("x", _dp), ("box", _dp), ("max_box", _dp), ("r_cut", _dp), ("en_tot", _dp), ("beta", _dp),
+        ("max_displ", _dp), ("count_atom", _i64p), ("count_vol", _i64p), ("count_exc", _i64p),
+        ("ham", _dp), ("hist", _dp), ("d2", _dp), ("comp", _dp), ("error", C.POINTER(C.c_int32)),
+        ("trace_de", _dp), ("trace_acc", C.POINTER(C.c_int8)), ("trace_idx", C.POINTER(C.c_int32)),
+        ("replay_acc", C.POINTER(C.c_int8)),
+    ]
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not LIB.exists():
+            build()
+        _L = C.CDLL(str(LIB))
+        _L.orc_init_states.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), C.POINTER(orc_state)]
+        _L.orc_run_cycles.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), C.POINTER(orc_state), C.c_int64]
+        _L.orc_dimer_energy.argtypes = [C.POINTER(orc_pot), C.c_double, C.c_int]
+        _L.orc_dimer_energy.restype = C.c_double
+        _L.orc_distance2.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_int]
+        _L.orc_distance2.restype = C.c_double
+        _L.orc_energy_config.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), _dp, C.c_double, C.c_double, _dp]
+        _L.orc_energy_config.restype = C.c_double
+        _L.orc_lrc.argtypes = [C.c_int, C.c_double, C.POINTER(orc_pot)]
+        _L.orc_lrc.restype = C.c_double
+        _L.orc_exp_det.argtypes = [C.c_double]
+        _L.orc_exp_det.restype = C.c_double
+        _L.orc_log_det.argtypes = [C.c_double]
+        _L.orc_log_det.restype = C.c_double
+        _L.orc_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    return _L
+
+
+def philox4x32(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, o)
+    return list(o)
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def make_pot(kind: str, coeff=(), eam=None) -> orc_pot:
+    p = orc_pot()
+    p.kind = {"elj_even": 0, "elj": 1, "eam": 2}[kind]
+    p.ncoeff = len(coeff)
+    for i, c in enumerate(coeff):
+        p.coeff[i] = c
+    if eam is not None:
+        p.eam_n, p.eam_m, p.eam_ean, p.eam_ecam = eam
+    return p
+
+
+REFERENCE = dict(pair_form=0, dist_cache=1, lanes=1, math_lib=1)
+
+
+def engine_mode(lanes=1):
+    return dict(pair_form=1, dist_cache=0, lanes=lanes, math_lib=0)
+
+
+class OracleRun:
+    """One PT run advanced by the CPU restatement."""
+
+    def __init__(self, n_atoms, beta, x0, *, pot: orc_pot, bc="sphere", radius=0.0, box=0.0, ensemble="nvt",
+                 pressure=0.0, mode=None, fp32=False, exchange_mode=0, n_adjust=100, mc_sample=1, n_bin=100,
+                 min_acc=0.4, max_acc=0.6, seed=1234, run_key=0, max_displ=(0.1, 0.1), trace_cycles=0):
+        mode = dict(REFERENCE if mode is None else mode)
+        R = len(beta)
+        N = n_atoms
+        self.N, self.R = N, R
+        p = orc_params()
+        p.n_atoms, p.n_temps = N, R
+        p.ensemble = 1 if ensemble == "npt" else 0
+        p.bc = 1 if bc == "cubic" else 0
+        p.pair_form, p.dist_cache, p.lanes, p.math_lib = (mode["pair_form"], mode["dist_cache"], mode["lanes"],
+                                                          mode["math_lib"])
+        p.fp32 = 1 if fp32 else 0
+        p.exchange_mode = exchange_mode
+        p.n_adjust, p.mc_sample, p.n_bin = n_adjust, mc_sample, n_bin
+        p.min_acc, p.max_acc = min_acc, max_acc
+        p.radius2 = radius * radius
+        p.pressure = pressure
+        p.seed = seed
+        p.run = run_key
+        p.en_min, p.delta_en_hist, p.v_min, p.delta_v_hist = 0.0, 1.0, 0.0, 1.0
+        self.p = p
+        self.pot = pot
+        x0 = np.asarray(x0, dtype=np.float64).reshape(-1, N, 3)
+        self.x = np.ascontiguousarray(np.stack([x0[t % len(x0)] for t in range(R)]))
+        if fp32:
+            self.x = self.x.astype(np.float32).astype(np.float64)
+        self.box = np.full(R, float(box))
+        self.max_box = np.full(R, float(box) * 1.8 if bc == "cubic" else 30.0)
+        self.r_cut = np.zeros(R)
+        self.en_tot = np.zeros(R)
+        self.beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self.max_displ = np.tile(np.array(max_displ, dtype=np.float64), (R, 1))
+        self.count_atom = np.zeros((R, 2), dtype=np.int64)
+        self.count_vol = np.zeros((R, 2), dtype=np.int64)
+        self.count_exc = np.zeros((R, 2), dtype=np.int64)
+        self.ham = np.zeros((R, 2))
+        nb2 = n_bin + 2
+        self.hist = np.zeros((R, nb2 * nb2 if ensemble == "npt" else nb2))
+        self.d2 = np.zeros((R, N, N)) if mode["dist_cache"] else None
+        self.comp = np.zeros((R, N, 2)) if pot.kind == 2 else None
+        self.error = np.zeros(1, dtype=np.int32)
+        self.trace_cycles = trace_cycles
+        self.sweep = 0
+        self.phase = 0
+        self.replay = None
+        self._state_struct()
+        rc = lib().orc_init_states(C.byref(self.p), C.byref(self.pot), C.byref(self.s))
+        if rc:
+            raise RuntimeError(f"orc_init_states failed: {rc}")
+
+    def _state_struct(self):
+        s = orc_state()
+        s.x, s.box, s.max_box, s.r_cut = _p(self.x), _p(self.box), _p(self.max_box), _p(self.r_cut)
+        s.en_tot, s.beta, s.max_displ = _p(self.en_tot), _p(self.beta), _p(self.max_displ)
+        s.count_atom, s.count_vol, s.count_exc = (_p(self.count_atom, C.c_int64), _p(self.count_vol, C.c_int64),
+                                                  _p(self.count_exc, C.c_int64))
+        s.ham, s.hist, s.d2, s.comp = _p(self.ham), _p(self.hist), _p(self.d2), _p(self.comp)
+        s.error = _p(self.error, C.c_int32)
+        self.s = s
+
+    def start_phase(self, production: bool):
+        self.phase = 0
+        self.p.sampling = 1 if production else 0
+
+    def cycles(self, n: int, trace: bool = False, replay_acc=None):
+        if n <= 0:
+            return None
+        self.p.sweep0 = self.sweep
+        self.p.cycle0 = self.phase
+        ns = self.N + (1 if self.p.ensemble == 1 else 0)
+        tr = None
+        if trace or replay_acc is not None:
+            tr = dict(de=np.zeros((self.R, n, ns)), acc=np.zeros((self.R, n, ns), dtype=np.int8),
+                      idx=np.zeros((self.R, n, ns), dtype=np.int32))
+            self.s.trace_de = _p(tr["de"])
+            self.s.trace_acc = _p(tr["acc"], C.c_int8)
+            self.s.trace_idx = _p(tr["idx"], C.c_int32)
+        if replay_acc is not None:
+            self._replay = np.ascontiguousarray(replay_acc, dtype=np.int8)
+            self.s.replay_acc = _p(self._replay, C.c_int8)
+        rc = lib().orc_run_cycles(C.byref(self.p), C.byref(self.pot), C.byref(self.s), int(n))
+        self.s.trace_de = None
+        self.s.trace_acc = None
+        self.s.trace_idx = None
+        self.s.replay_acc = None
+        if rc == 1:
+            raise RuntimeError("Error: too many moves out of binding sphere")
+        if rc:
+            raise RuntimeError(f"orc_run_cycles failed: {rc}")
+        self.sweep += n
+        self.phase += n
+        return tr
+
+    def set_histogram(self, en_min, en_max, v_min=0.0, v_max=1.0):
+        nb = self.p.n_bin
+        self.p.en_min = en_min
+        self.p.delta_en_hist = (en_max - en_min) / (nb - 1)
+        self.p.v_min = v_min
+        self.p.delta_v_hist = (v_max - v_min) / nb
+        self.hist[:] = 0.0
+
+    def reset_counters(self):
+        self.count_atom[:] = 0
+        self.count_vol[:] = 0
+        self.count_exc[:] = 0
+
+
+def check_e_bounds(energy, ebounds):
+    """src/MCRun.jl:258-265."""
+    if energy < ebounds[0]:
+        ebounds[0] = energy
+    elif energy > ebounds[1]:
+        ebounds[1] = energy
+    return ebounds
+
+
+def run_ptmc(run: OracleRun, mc_cycles: int, eq_cycles: int, n_atoms: int):
+    """ptmc_run! (src/MCRun.jl:363-432) on an OracleRun: equilibration,
+    histogram set-up, production, finalise_results."""
+    run.start_phase(False)
+    run.cycles(eq_cycles // 2)
+    eb = [100.0, -100.0]
+    for _ in range(eq_cycles - eq_cycles // 2):
+        for e in run.en_tot:
+            eb = check_e_bounds(float(e), eb)
+    v_min = v_max = 0.0
+    if run.p.bc == 1:
+        v_min = run.box[0] ** 3 * 0.8
+        v_max = run.box[0] ** 3 * 2.0
+    run.reset_counters()
+    run.set_histogram(eb[0], eb[1], v_min, v_max)
+    run.start_phase(True)
+    run.cycles(mc_cycles)
+    n_sample = mc_cycles / run.p.mc_sample
+    en_avg = run.ham[:, 0] / n_sample
+    en2_avg = run.ham[:, 1] / n_sample
+    heat_cap = (en2_avg - en_avg**2) * run.beta**2
+    count_stat_atom = run.count_atom[:, 0] / (n_atoms * mc_cycles)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        count_stat_exc = run.count_exc[:, 1] / run.count_exc[:, 0]
+    return dict(en_min=eb[0], en_max=eb[1], v_min=v_min, v_max=v_max, en_avg=en_avg, heat_cap=heat_cap,
+                count_stat_atom=count_stat_atom, count_stat_exc=count_stat_exc)

@@ -1,0 +1,98 @@
+/*
+ * ORACLE — test infrastructure only (see orc_math.h).  CPU FP64 restatement
+ * of the reference's parallel-tempering sweep for pair and EAM potentials.
+ *
+ * One call advances one ladder (one PT run) by n_sweeps Monte-Carlo cycles
+ * exactly as src/MCRun.jl:192-253 does: N (NVT) or N+1 (NPT) moves per
+ * replica, then at most one exchange attempt, then step-size adaptation,
+ * then sampling.  All random numbers come from the Philox stream defined in
+ * orc_math.h.
+ */
+#ifndef PTMC_ORACLE_H
+#define PTMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_NVT = 0, ORC_NPT = 1 };
+enum { ORC_SPHERE = 0, ORC_CUBIC = 1 };
+enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2 };
+/* exchange_mode: 0 = reference (p = 0.1 per cycle, one random adjacent pair),
+ *                1 = one random adjacent pair every cycle,
+ *                2 = all even (cycle even) / odd (cycle odd) pairs every cycle */
+enum { ORC_EXC_REFERENCE = 0, ORC_EXC_EVERY = 1, ORC_EXC_EVEN_ODD = 2 };
+
+typedef struct {
+    int32_t n_atoms, n_temps;
+    int32_t ensemble, bc;
+    int32_t pair_form;    /* 0: reference form (r^-6 then repeated division), 1: Horner in 1/r^2 */
+    int32_t dist_cache;   /* 1: cached r^2 matrix as the reference; 0: recompute from positions */
+    int32_t lanes;        /* summation tree width of the device engine (1 = sequential) */
+    int32_t fp32;         /* 1: positions and trial moves in binary32, energies in FP64 */
+    int32_t exchange_mode;
+    int32_t n_adjust, mc_sample, n_bin;
+    int32_t sampling;     /* 1: production cycles (sampling_step! runs) */
+    int32_t math_lib;     /* 1: use libm exp/log/pow (reference form), 0: deterministic exp/log */
+    double  min_acc, max_acc;
+    double  radius2;      /* spherical container radius^2 */
+    double  pressure;
+    uint64_t seed;
+    uint32_t run;
+    int64_t sweep0;       /* global cycle counter of the first cycle (RNG counter) */
+    int64_t cycle0;       /* phase-local index of the first cycle minus one */
+    double  en_min, delta_en_hist, v_min, delta_v_hist;
+} orc_params;
+
+typedef struct {
+    int32_t kind;
+    int32_t ncoeff;
+    double  coeff[16];
+    double  eam_n, eam_m, eam_ean, eam_ecam;
+} orc_pot;
+
+typedef struct {
+    double  *x;            /* R*N*3 positions, temperature-position order */
+    double  *box;          /* R box lengths (cubic) */
+    double  *max_box;      /* R max box lengths (NPT) */
+    double  *r_cut;        /* R squared cut-off (NPT) */
+    double  *en_tot;       /* R */
+    double  *beta;         /* R */
+    double  *max_displ;    /* R*2 : atom, volume */
+    int64_t *count_atom;   /* R*2 : total, window */
+    int64_t *count_vol;    /* R*2 */
+    int64_t *count_exc;    /* R*2 : attempted, accepted */
+    double  *ham;          /* R*2 */
+    double  *hist;         /* R*(n_bin+2) NVT energy histograms, or R*(n_bin+2)^2 NPT (E,V) */
+    double  *d2;           /* R*N*N cached squared distances (dist_cache = 1) */
+    double  *comp;         /* R*N*2 EAM phi/rho components */
+    int32_t *error;        /* out: 0 ok, 1 too many moves out of binding sphere */
+    /* optional per-move trace, indexed [r][cycle][move] */
+    double  *trace_de;     /* new_en - en_tot as used by the Metropolis test */
+    int8_t  *trace_acc;    /* 1 accepted */
+    int32_t *trace_idx;    /* moved atom (N = volume move) */
+    const int8_t *replay_acc; /* if non-null: take these accept decisions (fp32 replay) */
+} orc_state;
+
+/* Build the r^2 matrices / EAM components and total energies from positions
+ * (src/MCStates.jl:137-162 -> src/EnergyEvaluation.jl:1662-1734,1778-1811). */
+int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s);
+
+/* Advance n_sweeps cycles. */
+int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_t n_sweeps);
+
+/* Helpers exported for direct known-answer tests. */
+double orc_dimer_energy(const orc_pot *pot, double r2, int pair_form);
+double orc_distance2(const double *a, const double *b, int bc, double box, int cached_form);
+double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *x,
+                         double box, double r_cut, double *en_atom);
+double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot);
+double orc_exp_det(double x);
+double orc_log_det(double x);
+void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

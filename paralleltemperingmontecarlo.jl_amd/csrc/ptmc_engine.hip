@@ -1,0 +1,916 @@
+// Host engine behind include/ptmc.h: run bookkeeping, workgroup planning,
+// device buffers, kernel launches.  Every entry point returns a status code;
+// the message of the last failure is kept per handle (ptmc_last_error).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ptmc.h"
+#include "ptmc_kernels.hpp"
+#include "ptmc_sweep.hpp"
+
+using namespace ptmc;
+
+namespace {
+
+struct RunHost {
+    ptmc_run_desc d;
+    std::vector<double> beta;
+    std::vector<double> x0;  // n_configs * N * 3
+    int kl = 1;
+    RunInfo info{};
+    int tile = -1;
+};
+
+struct Launch {
+    SweepFn fn = nullptr;
+    int tile0 = 0, ntiles = 0;
+    int wgs = 256;
+    size_t lds = 0;
+};
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void free_()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct ptmc_engine {
+    ptmc_config cfg{};
+    PotParams pot{};
+    bool pot_set = false;
+    std::vector<RunHost> runs;
+    bool prepared = false;
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0.0;
+    long long n_launch = 0;
+
+    // plan
+    std::vector<TileInfo> tiles;
+    std::vector<Launch> launches;
+    int pk = PK_EVEN_H, kc = 8;
+    int n_slots = 0, n_temps = 0;
+    int64_t n_coord = 0;
+    int hist_stride = 0;
+
+    // device state
+    DevBuf<TileInfo> d_tiles;
+    DevBuf<RunInfo> d_runs;
+    DevBuf<int32_t> d_slot_run, d_slot_temp, d_run_error;
+    DevBuf<double> d_slot_en, d_slot_box, d_slot_drift;
+    DevBuf<unsigned char> d_coords;
+    DevBuf<double> d_beta, d_displ, d_maxbox, d_ham;
+    DevBuf<long long> d_count_atom, d_count_vol, d_count_exc;
+    DevBuf<uint32_t> d_hist;
+    DevBuf<double> d_trace_de;
+    DevBuf<int8_t> d_trace_acc;
+    DevBuf<int32_t> d_trace_idx;
+    int trace_cycles = 0;
+    int trace_pending = 0;
+    int trace_nsteps = 0;
+
+    int64_t sweep_counter = 0;  // global cycle counter (Philox word 1)
+    int64_t phase_counter = 0;  // phase-local cycle index
+    int production = 0;
+};
+
+namespace {
+
+int fail(ptmc_engine *h, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(h, PTMC_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+template <typename T>
+int dalloc(ptmc_engine *h, DevBuf<T> &b, size_t n)
+{
+    b.free_();
+    if (n == 0) n = 1;
+    HIPCHK(h, hipMalloc((void **)&b.p, n * sizeof(T)));
+    b.n = n;
+    HIPCHK(h, hipMemsetAsync(b.p, 0, n * sizeof(T), h->stream));
+    return PTMC_OK;
+}
+
+template <typename T>
+int upload(ptmc_engine *h, DevBuf<T> &b, const std::vector<T> &v)
+{
+    int rc = dalloc(h, b, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(h, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    return PTMC_OK;
+}
+
+int default_lanes(int n_atoms, int precision)
+{
+    // lanes per configuration: keep ~13-16 atoms per lane in binary32,
+    // ~8-13 in FP64 (twice the LDS per atom)
+    int per = precision == 1 ? 16 : 13;
+    int kl = 1;
+    while (kl < 16 && (n_atoms + kl - 1) / kl > per) kl *= 2;
+    return kl;
+}
+
+SweepFn lookup(int precision, int bc, int ens, int pk, int kc, int kl)
+{
+    if (precision == 0) {
+        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl);
+        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl);
+        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f64_cubic_npt(pk, kc, kl);
+    } else {
+        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f32_sphere_nvt(pk, kc, kl);
+        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f32_cubic_nvt(pk, kc, kl);
+        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f32_cubic_npt(pk, kc, kl);
+    }
+    return nullptr;
+}
+
+int check_run(ptmc_engine *h, int32_t run)
+{
+    if (!h) return PTMC_E_ARG;
+    if (run < 0 || run >= (int)h->runs.size()) return fail(h, PTMC_E_ARG, "run index %d out of range", run);
+    if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
+    return PTMC_OK;
+}
+
+// Select pair evaluation and coefficient padding (trailing zero
+// coefficients leave Horner and the reference loop bit-identical).
+int choose_pair(ptmc_engine *h)
+{
+    const PotParams &p = h->pot;
+    const int K = p.ncoeff;
+    if (h->cfg.pair_form == 1) {
+        if (h->cfg.precision != 0) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form requires FP64");
+        if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form supports up to 8 coefficients");
+        h->pk = p.kind == POT_ELJ_EVEN ? PK_EVEN_R : PK_ODD_R;
+        h->kc = 8;
+        return PTMC_OK;
+    }
+    if (p.kind == POT_ELJ) {
+        if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "at most 8 ELJ coefficients");
+        h->pk = PK_ODD_H;
+        h->kc = 8;
+        return PTMC_OK;
+    }
+    if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "at most 8 ELJ coefficients");
+    if (h->cfg.precision == 1 && K == 4 && p.coeff[1] == 0.0 && p.coeff[2] == 0.0) {
+        h->pk = PK_LJ;  // Lennard-Jones written as ELJPotentialEven{4}([c6, 0, 0, c12])
+        h->kc = 4;
+        return PTMC_OK;
+    }
+    h->pk = PK_EVEN_H;
+    h->kc = K <= 6 ? 6 : 8;
+    return PTMC_OK;
+}
+
+int plan(ptmc_engine *h)
+{
+    const int prec = h->cfg.precision;
+    const size_t rsize = prec == 0 ? sizeof(double) : sizeof(float);
+    int dev = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    int max_lds = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+
+    // group runs by (lanes per configuration, N)
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (int r = 0; r < (int)h->runs.size(); ++r) {
+        RunHost &rh = h->runs[r];
+        int kl = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
+                 : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
+                                                : default_lanes(rh.d.n_atoms, prec);
+        while (kl > 1 && rh.d.n_temps * kl > 256) kl /= 2;
+        if (rh.d.n_temps * kl > 256)
+            return fail(h, PTMC_E_UNSUPPORTED, "run %d: %d temperatures exceed one workgroup", r, rh.d.n_temps);
+        rh.kl = kl;
+        groups[{kl, rh.d.n_atoms}].push_back(r);
+    }
+
+    h->tiles.clear();
+    h->launches.clear();
+    int slot = 0;
+    int64_t coord = 0;
+    // one launch per lanes-per-configuration value
+    std::map<int, std::vector<int>> tiles_by_kl;
+    std::map<int, int> wgs_by_kl;
+    for (auto &g : groups) {
+        const int kl = g.first.first, N = g.first.second;
+        std::vector<int> &rs = g.second;
+        std::stable_sort(rs.begin(), rs.end(),
+                         [&](int a, int b) { return h->runs[a].d.n_temps > h->runs[b].d.n_temps; });
+        int maxR = 0;
+        for (int r : rs) maxR = std::max(maxR, h->runs[r].d.n_temps);
+        int wgs = 256;  // __launch_bounds__(256); whole ladders fit (R * KL <= 256)
+        wgs = std::max(wgs, wgs_by_kl.count(kl) ? wgs_by_kl[kl] : 0);
+        wgs_by_kl[kl] = wgs;
+        const int cap = wgs / kl;
+        // first-fit of whole ladders into tiles of cap slots
+        std::vector<std::vector<int>> packs;
+        std::vector<int> used;
+        for (int r : rs) {
+            const int R = h->runs[r].d.n_temps;
+            int t = -1;
+            for (size_t k = 0; k < packs.size(); ++k)
+                if (used[k] + R <= cap) { t = (int)k; break; }
+            if (t < 0) {
+                packs.push_back({});
+                used.push_back(0);
+                t = (int)packs.size() - 1;
+            }
+            packs[t].push_back(r);
+            used[t] += R;
+        }
+        for (size_t k = 0; k < packs.size(); ++k) {
+            TileInfo ti{};
+            ti.slot0 = slot;
+            ti.n_atoms = N;
+            ti.n_slab = (N + kl - 1) / kl;
+            ti.coord_off = 0;  // filled below (needs the launch's wgs)
+            int local = 0;
+            for (int r : packs[k]) {
+                RunHost &rh = h->runs[r];
+                rh.info.slot_base = slot + local;
+                rh.info.local_base = local;
+                rh.tile = (int)h->tiles.size();
+                local += rh.d.n_temps;
+            }
+            ti.n_slots = local;
+            slot += local;
+            tiles_by_kl[kl].push_back((int)h->tiles.size());
+            h->tiles.push_back(ti);
+        }
+    }
+    h->n_slots = slot;
+
+    // reorder tiles so each launch is contiguous
+    std::vector<TileInfo> ordered;
+    std::vector<int> old_to_new(h->tiles.size());
+    for (auto &kv : tiles_by_kl) {
+        const int kl = kv.first;
+        Launch L;
+        L.tile0 = (int)ordered.size();
+        L.ntiles = (int)kv.second.size();
+        L.wgs = wgs_by_kl[kl];
+        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
+        if (!L.fn)
+            return fail(h, PTMC_E_UNSUPPORTED, "no kernel for precision %d boundary %d ensemble %d pair %d/%d lanes %d",
+                        prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
+        size_t maxlds = 0;
+        for (int t : kv.second) {
+            TileInfo ti = h->tiles[t];
+            ti.coord_off = coord;
+            coord += (int64_t)3 * ti.n_slab * L.wgs;
+            size_t lds = rsize * 3 * ti.n_slab * L.wgs + (size_t)(L.wgs / kl) * (8 + 8 + 4 + 4);
+            maxlds = std::max(maxlds, lds);
+            old_to_new[t] = (int)ordered.size();
+            ordered.push_back(ti);
+        }
+        if ((long long)maxlds > max_lds)
+            return fail(h, PTMC_E_UNSUPPORTED, "tile needs %zu B of LDS (device limit %d); use more lanes per configuration",
+                        maxlds, max_lds);
+        L.lds = maxlds;
+        HIPCHK(h, hipFuncSetAttribute((const void *)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+        h->launches.push_back(L);
+    }
+    for (auto &rh : h->runs) rh.tile = old_to_new[rh.tile];
+    h->tiles = ordered;
+    h->n_coord = coord;
+    // slot0 of reordered tiles keeps the values assigned above (slots stay with
+    // their runs); coordinates are addressed by tile.coord_off.
+    return PTMC_OK;
+}
+
+template <typename Real>
+void scatter_coords(ptmc_engine *h, std::vector<Real> &buf)
+{
+    buf.assign((size_t)h->n_coord, (Real)0);
+    for (auto &rh : h->runs) {
+        const TileInfo &ti = h->tiles[rh.tile];
+        int wgs = 256;
+        for (auto &L : h->launches)
+            if (rh.tile >= L.tile0 && rh.tile < L.tile0 + L.ntiles) wgs = L.wgs;
+        const int N = rh.d.n_atoms, A = ti.n_slab, kl = rh.kl;
+        for (int t = 0; t < rh.d.n_temps; ++t) {
+            const int lg = rh.info.local_base + t;  // slot t holds temperature t at start
+            const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
+            for (int j = 0; j < N; ++j) {
+                const int s = j / kl, lane = lg * kl + j % kl;
+                for (int c = 0; c < 3; ++c)
+                    buf[(size_t)ti.coord_off + ((size_t)c * A + s) * wgs + lane] = (Real)x[3 * j + c];
+            }
+        }
+    }
+}
+
+int launch_wgs(ptmc_engine *h, int tile)
+{
+    for (auto &L : h->launches)
+        if (tile >= L.tile0 && tile < L.tile0 + L.ntiles) return L.wgs;
+    return 256;
+}
+
+SweepArgs base_args(ptmc_engine *h)
+{
+    SweepArgs a{};
+    a.tiles = h->d_tiles.p;
+    a.runs = h->d_runs.p;
+    a.slot_run = h->d_slot_run.p;
+    a.slot_temp = h->d_slot_temp.p;
+    a.slot_en = h->d_slot_en.p;
+    a.slot_box = h->d_slot_box.p;
+    a.slot_drift = h->d_slot_drift.p;
+    a.coords = h->d_coords.p;
+    a.t_beta = h->d_beta.p;
+    a.t_displ = h->d_displ.p;
+    a.t_maxbox = h->d_maxbox.p;
+    a.t_count_atom = h->d_count_atom.p;
+    a.t_count_vol = h->d_count_vol.p;
+    a.t_count_exc = h->d_count_exc.p;
+    a.t_ham = h->d_ham.p;
+    a.t_hist = h->d_hist.p;
+    a.run_error = h->d_run_error.p;
+    a.pot = h->pot;
+    a.seed = h->cfg.seed;
+    a.ensemble = h->cfg.ensemble;
+    a.exchange_mode = h->cfg.exchange_mode;
+    a.n_adjust = h->cfg.n_adjust;
+    a.mc_sample = h->cfg.mc_sample;
+    a.n_bin = h->cfg.n_bin;
+    a.hist_stride = h->hist_stride;
+    a.recompute_every = h->cfg.recompute_every;
+    a.min_acc = h->cfg.min_acc;
+    a.max_acc = h->cfg.max_acc;
+    return a;
+}
+
+int launch_all(ptmc_engine *h, SweepArgs a)
+{
+    for (auto &L : h->launches) {
+        SweepArgs b = a;
+        b.tiles = h->d_tiles.p + L.tile0;
+        void *args[] = {&b};
+        HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, h->stream));
+        h->n_launch += 1;
+    }
+    return PTMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptmc_create(const ptmc_config *cfg, ptmc_handle *out)
+{
+    if (!cfg || !out) return PTMC_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PTMC_E_NO_DEVICE;
+    if (cfg->device < 0 || cfg->device >= ndev) return PTMC_E_ARG;
+    if (cfg->precision < 0 || cfg->precision > 1 || cfg->ensemble < 0 || cfg->ensemble > 1 ||
+        cfg->boundary < 0 || cfg->boundary > 1 || cfg->exchange_mode < 0 || cfg->exchange_mode > 2 ||
+        cfg->n_adjust <= 0 || cfg->mc_sample <= 0 || cfg->n_bin < 2)
+        return PTMC_E_ARG;
+    if (cfg->ensemble == ENS_NPT && cfg->boundary == BC_SPHERE) return PTMC_E_ARG;  // src/Ensembles.jl:191-193
+    ptmc_engine *h = new ptmc_engine();
+    h->cfg = *cfg;
+    if (h->cfg.cycles_per_launch <= 0) h->cfg.cycles_per_launch = 256;
+    if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        delete h;
+        return PTMC_E_DEVICE;
+    }
+    *out = h;
+    return PTMC_OK;
+}
+
+int ptmc_destroy(ptmc_handle h)
+{
+    if (!h) return PTMC_E_ARG;
+    (void)hipSetDevice(h->cfg.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->d_tiles.free_();
+    h->d_runs.free_();
+    h->d_slot_run.free_();
+    h->d_slot_temp.free_();
+    h->d_run_error.free_();
+    h->d_slot_en.free_();
+    h->d_slot_box.free_();
+    h->d_slot_drift.free_();
+    h->d_coords.free_();
+    h->d_beta.free_();
+    h->d_displ.free_();
+    h->d_maxbox.free_();
+    h->d_ham.free_();
+    h->d_count_atom.free_();
+    h->d_count_vol.free_();
+    h->d_count_exc.free_();
+    h->d_hist.free_();
+    h->d_trace_de.free_();
+    h->d_trace_acc.free_();
+    h->d_trace_idx.free_();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PTMC_OK;
+}
+
+int ptmc_last_error(ptmc_handle h, char *buf, size_t len)
+{
+    if (!h || !buf || len == 0) return PTMC_E_ARG;
+    snprintf(buf, len, "%s", h->err.c_str());
+    return PTMC_OK;
+}
+
+int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, const double *coeff)
+{
+    if (!h || !coeff || n_coeff <= 0 || n_coeff > MAX_COEFF) return h ? fail(h, PTMC_E_ARG, "bad ELJ coefficients") : PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    PotParams p{};
+    p.kind = even_powers ? POT_ELJ_EVEN : POT_ELJ;
+    p.ncoeff = n_coeff;
+    for (int i = 0; i < n_coeff; ++i) p.coeff[i] = coeff[i];
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
+}
+
+int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam)
+{
+    if (!h) return PTMC_E_ARG;
+    (void)n; (void)m; (void)ean; (void)ecam;
+    return fail(h, PTMC_E_UNSUPPORTED, "EAM sweeps are provided by ptmc_eam_* entry points");
+}
+
+int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *d, int32_t *run_index)
+{
+    if (!h || !d) return PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "runs must be added before ptmc_prepare");
+    if (d->n_atoms < 2 || d->n_temps < 1 || !d->beta || !d->x0 || d->n_configs < 1)
+        return fail(h, PTMC_E_ARG, "bad run description");
+    if (h->cfg.boundary == BC_SPHERE && !(d->radius > 0)) return fail(h, PTMC_E_ARG, "radius must be positive");
+    if (h->cfg.boundary == BC_CUBIC && !(d->box_length > 0)) return fail(h, PTMC_E_ARG, "box length must be positive");
+    if (d->n_temps > (1 << 20)) return fail(h, PTMC_E_ARG, "too many temperatures");
+    if (d->lanes_per_replica != 0 && d->lanes_per_replica != 1 && d->lanes_per_replica != 2 &&
+        d->lanes_per_replica != 4 && d->lanes_per_replica != 8 && d->lanes_per_replica != 16)
+        return fail(h, PTMC_E_ARG, "lanes_per_replica must be 1, 2, 4, 8 or 16");
+    RunHost rh;
+    rh.d = *d;
+    rh.beta.assign(d->beta, d->beta + d->n_temps);
+    rh.x0.assign(d->x0, d->x0 + (size_t)d->n_configs * d->n_atoms * 3);
+    rh.d.beta = nullptr;
+    rh.d.x0 = nullptr;
+    if (run_index) *run_index = (int32_t)h->runs.size();
+    h->runs.push_back(std::move(rh));
+    return PTMC_OK;
+}
+
+int ptmc_prepare(ptmc_handle h)
+{
+    if (!h) return PTMC_E_ARG;
+    if (h->prepared) return PTMC_OK;
+    if (!h->pot_set) return fail(h, PTMC_E_STATE, "no potential set");
+    if (h->runs.empty()) return fail(h, PTMC_E_STATE, "no runs added");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = choose_pair(h);
+    if (rc) return rc;
+    rc = plan(h);
+    if (rc) return rc;
+
+    const int nb2 = h->cfg.n_bin + 2;
+    h->hist_stride = h->cfg.ensemble == ENS_NPT ? nb2 * nb2 : nb2;
+
+    std::vector<RunInfo> infos;
+    std::vector<int32_t> slot_run(h->n_slots), slot_temp(h->n_slots);
+    std::vector<double> slot_en(h->n_slots, 0.0), slot_box(h->n_slots, 1.0);
+    std::vector<double> beta, displ, maxbox;
+    int tb = 0;
+    for (int r = 0; r < (int)h->runs.size(); ++r) {
+        RunHost &rh = h->runs[r];
+        RunInfo &ri = rh.info;
+        ri.n_atoms = rh.d.n_atoms;
+        ri.n_temps = rh.d.n_temps;
+        ri.temp_base = tb;
+        ri.run_key = rh.d.run_key == 0xffffffffu ? (uint32_t)r : rh.d.run_key;
+        ri.radius2 = rh.d.radius * rh.d.radius;  // SphericalBC stores radius^2, src/BoundaryConditions.jl:44-47
+        ri.pressure = rh.d.pressure;
+        ri.en_min = 0.0;
+        ri.delta_en = 1.0;
+        ri.v_min = 0.0;
+        ri.delta_v = 1.0;
+        infos.push_back(ri);
+        for (int t = 0; t < rh.d.n_temps; ++t) {
+            slot_run[ri.slot_base + t] = r;
+            slot_temp[ri.slot_base + t] = t;
+            slot_box[ri.slot_base + t] = h->cfg.boundary == BC_CUBIC ? rh.d.box_length : 1.0;
+            beta.push_back(rh.beta[t]);
+            displ.push_back(rh.d.max_displ_atom);
+            displ.push_back(rh.d.max_displ_vol);
+            // max_length(CubicBC) = 1.8 L, src/MCStates.jl:71-73
+            maxbox.push_back(h->cfg.boundary == BC_CUBIC ? rh.d.box_length * 1.8 : 30.0);
+        }
+        tb += rh.d.n_temps;
+    }
+    h->n_temps = tb;
+
+    if ((rc = upload(h, h->d_tiles, h->tiles))) return rc;
+    if ((rc = upload(h, h->d_runs, infos))) return rc;
+    if ((rc = upload(h, h->d_slot_run, slot_run))) return rc;
+    if ((rc = upload(h, h->d_slot_temp, slot_temp))) return rc;
+    if ((rc = upload(h, h->d_slot_en, slot_en))) return rc;
+    if ((rc = upload(h, h->d_slot_box, slot_box))) return rc;
+    if ((rc = dalloc(h, h->d_slot_drift, h->n_slots))) return rc;
+    if ((rc = dalloc(h, h->d_run_error, h->runs.size()))) return rc;
+    if ((rc = upload(h, h->d_beta, beta))) return rc;
+    if ((rc = upload(h, h->d_displ, displ))) return rc;
+    if ((rc = upload(h, h->d_maxbox, maxbox))) return rc;
+    if ((rc = dalloc(h, h->d_ham, (size_t)2 * tb))) return rc;
+    if ((rc = dalloc(h, h->d_count_atom, (size_t)2 * tb))) return rc;
+    if ((rc = dalloc(h, h->d_count_vol, (size_t)2 * tb))) return rc;
+    if ((rc = dalloc(h, h->d_count_exc, (size_t)2 * tb))) return rc;
+    if ((rc = dalloc(h, h->d_hist, (size_t)h->hist_stride * tb))) return rc;
+    if (h->cfg.precision == 0) {
+        std::vector<double> buf;
+        scatter_coords<double>(h, buf);
+        std::vector<unsigned char> raw(buf.size() * sizeof(double));
+        memcpy(raw.data(), buf.data(), raw.size());
+        if ((rc = upload(h, h->d_coords, raw))) return rc;
+    } else {
+        std::vector<float> buf;
+        scatter_coords<float>(h, buf);
+        std::vector<unsigned char> raw(buf.size() * sizeof(float));
+        memcpy(raw.data(), buf.data(), raw.size());
+        if ((rc = upload(h, h->d_coords, raw))) return rc;
+    }
+    h->prepared = true;
+
+    // initial energies (initialise_energy) on the device in FP64
+    SweepArgs a = base_args(h);
+    a.n_cycles = 0;
+    a.init_energy = 1;
+    a.init_run = -1;
+    if ((rc = launch_all(h, a))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return PTMC_OK;
+}
+
+int ptmc_start_phase(ptmc_handle h, int32_t production)
+{
+    if (!h) return PTMC_E_ARG;
+    h->phase_counter = 0;
+    h->production = production ? 1 : 0;
+    return PTMC_OK;
+}
+
+int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
+{
+    if (!h) return PTMC_E_ARG;
+    if (n_cycles < 0) return fail(h, PTMC_E_ARG, "negative cycle count");
+    int rc = ptmc_prepare(h);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int64_t done = 0;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    while (done < n_cycles) {
+        const int64_t chunk = std::min<int64_t>(n_cycles - done, h->cfg.cycles_per_launch);
+        SweepArgs a = base_args(h);
+        a.sweep0 = h->sweep_counter;
+        a.cycle0 = h->phase_counter;
+        a.n_cycles = (int32_t)chunk;
+        a.sampling = h->production;
+        a.init_run = -1;
+        if (h->trace_pending > 0) {
+            if (chunk != n_cycles || chunk > h->trace_cycles)
+                return fail(h, PTMC_E_STATE, "traced cycles must be run in one ptmc_cycles call of at most %d cycles",
+                            h->trace_cycles);
+            a.trace_de = h->d_trace_de.p;
+            a.trace_acc = h->d_trace_acc.p;
+            a.trace_idx = h->d_trace_idx.p;
+            a.trace_cycles = h->trace_cycles;
+            h->trace_pending = 0;
+        }
+        if ((rc = launch_all(h, a))) return rc;
+        h->sweep_counter += chunk;
+        h->phase_counter += chunk;
+        done += chunk;
+    }
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->kernel_ms += ms;
+    return PTMC_OK;
+}
+
+int ptmc_sync(ptmc_handle h)
+{
+    if (!h) return PTMC_E_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->prepared && !h->runs.empty()) {
+        std::vector<int32_t> e(h->runs.size());
+        HIPCHK(h, hipMemcpy(e.data(), h->d_run_error.p, e.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < e.size(); ++r)
+            if (e[r]) return fail(h, PTMC_E_SPHERE, "Error: too many moves out of binding sphere (run %zu)", r);
+    }
+    return PTMC_OK;
+}
+
+int ptmc_set_histogram(ptmc_handle h, int32_t run, double en_min, double en_max, double v_min, double v_max)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    RunHost &rh = h->runs[run];
+    rh.info.en_min = en_min;
+    rh.info.delta_en = (en_max - en_min) / (double)(h->cfg.n_bin - 1);
+    rh.info.v_min = v_min;
+    rh.info.delta_v = (v_max - v_min) / (double)h->cfg.n_bin;
+    HIPCHK(h, hipMemcpyAsync(h->d_runs.p + run, &rh.info, sizeof(RunInfo), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_hist.p + (size_t)rh.info.temp_base * h->hist_stride, 0,
+                             sizeof(uint32_t) * (size_t)rh.d.n_temps * h->hist_stride, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return PTMC_OK;
+}
+
+int ptmc_reset_counters(ptmc_handle h)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipMemsetAsync(h->d_count_atom.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_count_vol.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_count_exc.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return PTMC_OK;
+}
+
+static int fetch_slots(ptmc_engine *h, int run, std::vector<int32_t> &temp, std::vector<double> &en,
+                       std::vector<double> &box)
+{
+    const RunHost &rh = h->runs[run];
+    const int R = rh.d.n_temps, s0 = rh.info.slot_base;
+    temp.resize(R);
+    en.resize(R);
+    box.resize(R);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(temp.data(), h->d_slot_temp.p + s0, R * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(en.data(), h->d_slot_en.p + s0, R * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(box.data(), h->d_slot_box.p + s0, R * sizeof(double), hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_get_energies(ptmc_handle h, int32_t run, double *en)
+{
+    return ptmc_get_state(h, run, nullptr, en, nullptr);
+}
+
+int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *box)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    std::vector<int32_t> temp;
+    std::vector<double> e, b;
+    if ((rc = fetch_slots(h, run, temp, e, b))) return rc;
+    const RunHost &rh = h->runs[run];
+    const int R = rh.d.n_temps, N = rh.d.n_atoms;
+    for (int s = 0; s < R; ++s) {
+        const int t = temp[s];
+        if (t < 0 || t >= R) return fail(h, PTMC_E_DEVICE, "corrupt temperature label");
+        if (en) en[t] = e[s];
+        if (box) box[t] = b[s];
+    }
+    if (x) {
+        const TileInfo &ti = h->tiles[rh.tile];
+        const int wgs = launch_wgs(h, rh.tile), A = ti.n_slab, kl = rh.kl;
+        const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
+        const size_t nblk = (size_t)3 * A * wgs;
+        std::vector<unsigned char> blk(nblk * rsize);
+        HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
+        for (int s = 0; s < R; ++s) {
+            const int t = temp[s], lg = rh.info.local_base + s;
+            for (int j = 0; j < N; ++j) {
+                const int sl = j / kl, lane = lg * kl + j % kl;
+                for (int c = 0; c < 3; ++c) {
+                    const size_t e_ = ((size_t)c * A + sl) * wgs + lane;
+                    double v;
+                    if (rsize == 8) memcpy(&v, &blk[e_ * 8], 8);
+                    else {
+                        float f;
+                        memcpy(&f, &blk[e_ * 4], 4);
+                        v = f;
+                    }
+                    x[((size_t)t * N + j) * 3 + c] = v;
+                }
+            }
+        }
+    }
+    return PTMC_OK;
+}
+
+int ptmc_get_counters(ptmc_handle h, int32_t run, double *max_displ, int64_t *count_atom, int64_t *count_vol,
+                      int64_t *count_exc, double *ham)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t off = (size_t)2 * rh.info.temp_base, n = (size_t)2 * rh.d.n_temps;
+    if (max_displ) HIPCHK(h, hipMemcpy(max_displ, h->d_displ.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (count_atom) HIPCHK(h, hipMemcpy(count_atom, h->d_count_atom.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (count_vol) HIPCHK(h, hipMemcpy(count_vol, h->d_count_vol.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (count_exc) HIPCHK(h, hipMemcpy(count_exc, h->d_count_exc.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (ham) HIPCHK(h, hipMemcpy(ham, h->d_ham.p + off, n * 8, hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_hist_stride(ptmc_handle h) { return h ? h->hist_stride : -1; }
+
+int ptmc_get_histograms(ptmc_handle h, int32_t run, double *hist)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    if (!hist) return fail(h, PTMC_E_ARG, "null histogram buffer");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t n = (size_t)rh.d.n_temps * h->hist_stride;
+    std::vector<uint32_t> tmp(n);
+    HIPCHK(h, hipMemcpy(tmp.data(), h->d_hist.p + (size_t)rh.info.temp_base * h->hist_stride, n * 4,
+                        hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) hist[i] = (double)tmp[i];
+    return PTMC_OK;
+}
+
+int ptmc_get_drift(ptmc_handle h, int32_t run, double *drift)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    std::vector<int32_t> temp;
+    std::vector<double> e, b;
+    if ((rc = fetch_slots(h, run, temp, e, b))) return rc;
+    const RunHost &rh = h->runs[run];
+    std::vector<double> dr(rh.d.n_temps);
+    HIPCHK(h, hipMemcpy(dr.data(), h->d_slot_drift.p + rh.info.slot_base, dr.size() * 8, hipMemcpyDeviceToHost));
+    for (int s = 0; s < rh.d.n_temps; ++s) drift[s] = dr[s];  // by slot
+    return PTMC_OK;
+}
+
+int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *box, const double *max_displ,
+                   const int64_t *count_atom, const int64_t *count_vol, const int64_t *count_exc, const double *ham)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const int R = rh.d.n_temps, N = rh.d.n_atoms;
+    const size_t off = (size_t)2 * rh.info.temp_base, n = (size_t)2 * R;
+    if (max_displ) HIPCHK(h, hipMemcpy(h->d_displ.p + off, max_displ, n * 8, hipMemcpyHostToDevice));
+    if (count_atom) HIPCHK(h, hipMemcpy(h->d_count_atom.p + off, count_atom, n * 8, hipMemcpyHostToDevice));
+    if (count_vol) HIPCHK(h, hipMemcpy(h->d_count_vol.p + off, count_vol, n * 8, hipMemcpyHostToDevice));
+    if (count_exc) HIPCHK(h, hipMemcpy(h->d_count_exc.p + off, count_exc, n * 8, hipMemcpyHostToDevice));
+    if (ham) HIPCHK(h, hipMemcpy(h->d_ham.p + off, ham, n * 8, hipMemcpyHostToDevice));
+    if (x || box) {
+        // slot s <- temperature s; positions rewritten, energies recomputed
+        std::vector<int32_t> temp(R);
+        for (int s = 0; s < R; ++s) temp[s] = s;
+        HIPCHK(h, hipMemcpy(h->d_slot_temp.p + rh.info.slot_base, temp.data(), R * 4, hipMemcpyHostToDevice));
+        if (box) HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, box, R * 8, hipMemcpyHostToDevice));
+        if (x) {
+            const TileInfo &ti = h->tiles[rh.tile];
+            const int wgs = launch_wgs(h, rh.tile), A = ti.n_slab, kl = rh.kl;
+            const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
+            const size_t nblk = (size_t)3 * A * wgs;
+            std::vector<unsigned char> blk(nblk * rsize);
+            HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
+            for (int s = 0; s < R; ++s) {
+                const int lg = rh.info.local_base + s;
+                for (int j = 0; j < N; ++j) {
+                    const int sl = j / kl, lane = lg * kl + j % kl;
+                    for (int c = 0; c < 3; ++c) {
+                        const size_t e_ = ((size_t)c * A + sl) * wgs + lane;
+                        const double v = x[((size_t)s * N + j) * 3 + c];
+                        if (rsize == 8) memcpy(&blk[e_ * 8], &v, 8);
+                        else {
+                            float f = (float)v;
+                            memcpy(&blk[e_ * 4], &f, 4);
+                        }
+                    }
+                }
+            }
+            HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)ti.coord_off * rsize, blk.data(), blk.size(), hipMemcpyHostToDevice));
+        }
+        SweepArgs a = base_args(h);
+        a.n_cycles = 0;
+        a.init_energy = 1;
+        a.init_run = run;
+        if ((rc = launch_all(h, a))) return rc;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return PTMC_OK;
+}
+
+int ptmc_set_trace(ptmc_handle h, int32_t n_cycles)
+{
+    if (!h) return PTMC_E_ARG;
+    int rc = ptmc_prepare(h);
+    if (rc) return rc;
+    if (n_cycles <= 0) return fail(h, PTMC_E_ARG, "trace length must be positive");
+    int maxN = 0;
+    for (auto &rh : h->runs) maxN = std::max(maxN, rh.d.n_atoms);
+    const int nsteps = h->cfg.ensemble == ENS_NPT ? maxN + 1 : maxN;
+    const size_t n = (size_t)h->n_temps * n_cycles * nsteps;
+    for (auto &rh : h->runs)
+        if (rh.d.n_atoms != maxN) return fail(h, PTMC_E_UNSUPPORTED, "trace needs equal atom counts");
+    if ((rc = dalloc(h, h->d_trace_de, n))) return rc;
+    if ((rc = dalloc(h, h->d_trace_acc, n))) return rc;
+    if ((rc = dalloc(h, h->d_trace_idx, n))) return rc;
+    h->trace_cycles = n_cycles;
+    h->trace_pending = 1;
+    h->trace_nsteps = nsteps;
+    return PTMC_OK;
+}
+
+int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t *idx)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    if (!h->d_trace_de.p) return fail(h, PTMC_E_STATE, "no trace recorded");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t per = (size_t)h->trace_cycles * h->trace_nsteps;
+    const size_t off = (size_t)rh.info.temp_base * per, n = (size_t)rh.d.n_temps * per;
+    if (de) HIPCHK(h, hipMemcpy(de, h->d_trace_de.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (acc) HIPCHK(h, hipMemcpy(acc, h->d_trace_acc.p + off, n, hipMemcpyDeviceToHost));
+    if (idx) HIPCHK(h, hipMemcpy(idx, h->d_trace_idx.p + off, n * 4, hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, int64_t *n_lanes)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
+    if (n_tiles) *n_tiles = (int32_t)h->tiles.size();
+    if (n_launch_groups) *n_launch_groups = (int32_t)h->launches.size();
+    if (n_lanes) {
+        int64_t s = 0;
+        for (auto &L : h->launches) s += (int64_t)L.ntiles * L.wgs;
+        *n_lanes = s;
+    }
+    return PTMC_OK;
+}
+
+int ptmc_run_lanes(ptmc_handle h, int32_t run)
+{
+    if (!h || run < 0 || run >= (int)h->runs.size() || !h->prepared) return -1;
+    return h->runs[run].kl;
+}
+
+int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset)
+{
+    if (!h) return PTMC_E_ARG;
+    if (total_ms) *total_ms = h->kernel_ms;
+    if (n_launches) *n_launches = h->n_launch;
+    if (reset) {
+        h->kernel_ms = 0.0;
+        h->n_launch = 0;
+    }
+    return PTMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,4 @@
+// Sweep kernel instantiations: f64, cubic_npt.
+#include "ptmc_sweep.hpp"
+#include "ptmc_kernels.hpp"
+PTMC_DEFINE_LOOKUP_F64(lookup_f64_cubic_npt, BC_CUBIC, ENS_NPT)

@@ -1,0 +1,615 @@
+// Persistent PT sweep kernel for pair potentials (extended Lennard-Jones).
+//
+// Layout: one workgroup holds whole temperature ladders of runs that share
+// the atom count N.  A configuration ("slot") is owned by a cluster of KL
+// lanes; lane q of the cluster keeps atoms j = s*KL + q (s < n_slab) in LDS,
+// laid out [xyz][s][lane] so the partner loop reads one bank per lane.
+// Exchange swaps temperature labels between slots, not coordinates; the
+// per-temperature data (beta, step sizes, counters, energy sums, histograms)
+// stay with the temperature exactly as the reference keeps them in its
+// MCState vector (src/Exchange.jl:115-124 swaps config, r^2, energy and
+// potential variables only).
+//
+// One cycle (src/MCRun.jl:192-253):
+//   every slot makes N (NVT) or N+1 (NPT) moves           src/MCRun.jl:169-182
+//     move = draw index, trial, dE, Metropolis, commit      src/MCRun.jl:142-162
+//   exchange attempt(s)                                     src/MCRun.jl:203-205
+//   step-size adaptation every n_adjust cycles              src/MCRun.jl:206-212
+//   sampling every mc_sample cycles (production)            src/MCRun.jl:248-250
+#pragma once
+#include "ptmc_device.hpp"
+#include "ptmc_types.hpp"
+
+namespace ptmc {
+
+enum PairKind : int { PK_EVEN_H = 0, PK_ODD_H = 1, PK_EVEN_R = 2, PK_ODD_R = 3, PK_LJ = 4 };
+
+template <typename Real>
+__device__ __forceinline__ Real rcp_(Real x);
+template <>
+__device__ __forceinline__ float rcp_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ double rcp_<double>(double x) { return 1.0 / x; }
+
+template <typename Real>
+__device__ __forceinline__ Real fma_(Real a, Real b, Real c);
+template <>
+__device__ __forceinline__ float fma_<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fma_<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Pair energy e(r^2).  Coefficient array c has KC entries (zero padded:
+// trailing zeros leave Horner and the reference loop bit-identical).
+template <typename Real, int PK, int KC>
+__device__ __forceinline__ Real pair_energy(Real r2, const Real (&c)[KC])
+{
+    if constexpr (PK == PK_LJ) {
+        Real u = rcp_<Real>(r2);
+        Real u3 = u * u * u;
+        return u3 * fma_<Real>(c[1], u3, c[0]);
+    } else if constexpr (PK == PK_EVEN_H) {
+        Real u = rcp_<Real>(r2);
+        Real q = c[KC - 1];
+#pragma unroll
+        for (int k = KC - 2; k >= 0; --k) q = fma_<Real>(q, u, c[k]);
+        Real u3 = u * u * u;
+        return q * u3;
+    } else if constexpr (PK == PK_ODD_H) {
+        Real v;
+        if constexpr (sizeof(Real) == 4) v = __builtin_amdgcn_rsqf(r2);
+        else v = 1.0 / __builtin_sqrt(r2);
+        Real q = c[KC - 1];
+#pragma unroll
+        for (int k = KC - 2; k >= 0; --k) q = fma_<Real>(q, v, c[k]);
+        Real u = rcp_<Real>(r2);
+        Real u3 = u * u * u;
+        return q * u3;
+    } else if constexpr (PK == PK_EVEN_R) {
+        // src/EnergyEvaluation.jl:934-942
+        Real r6inv = Real(1) / (r2 * r2 * r2);
+        Real sum1 = 0;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            sum1 += c[k] * r6inv;
+            r6inv /= r2;
+        }
+        return sum1;
+    } else {
+        // PK_ODD_R, src/EnergyEvaluation.jl:923-932
+        Real r = __builtin_sqrt(r2);
+        Real r6inv = Real(1) / (r2 * r2 * r2);
+        Real sum1 = 0;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            sum1 += c[k] * r6inv;
+            r6inv /= r;
+        }
+        return sum1;
+    }
+}
+
+// FP64 pair energy used by full recomputes and volume moves.
+template <int PK, int KC>
+__device__ __forceinline__ double pair_energy64(double r2, const double (&c)[KC])
+{
+    if constexpr (PK == PK_EVEN_R || PK == PK_ODD_R || PK == PK_ODD_H)
+        return pair_energy<double, PK, KC>(r2, c);
+    else
+        return pair_energy<double, PK_EVEN_H, KC>(r2, c);
+}
+
+// Squared distance between a and b (src/Configurations.jl:101-109).
+// Cubic minimum image: FP64 moves b by L*round((a-b)/L) like the reference
+// (division for the reference pair form, multiplication by 1/L otherwise);
+// binary32 folds the image into the difference.
+template <typename Real, int BC, bool DIV>
+__device__ __forceinline__ Real dist2(Real ax, Real ay, Real az, Real bx, Real by, Real bz, Real L,
+                                      Real invL)
+{
+    if constexpr (sizeof(Real) == 8) {
+        if constexpr (BC == BC_CUBIC) {
+            Real qx = DIV ? (ax - bx) / L : (ax - bx) * invL;
+            Real qy = DIV ? (ay - by) / L : (ay - by) * invL;
+            Real qz = DIV ? (az - bz) / L : (az - bz) * invL;
+            bx = bx + L * __builtin_rint(qx);
+            by = by + L * __builtin_rint(qy);
+            bz = bz + L * __builtin_rint(qz);
+        }
+        Real dx = ax - bx, dy = ay - by, dz = az - bz;
+        return dx * dx + dy * dy + dz * dz;
+    } else {
+        Real dx = ax - bx, dy = ay - by, dz = az - bz;
+        if constexpr (BC == BC_CUBIC) {
+            dx = __builtin_fmaf(-L, __builtin_rintf(dx * invL), dx);
+            dy = __builtin_fmaf(-L, __builtin_rintf(dy * invL), dy);
+            dz = __builtin_fmaf(-L, __builtin_rintf(dz * invL), dz);
+        }
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    }
+}
+
+__device__ __forceinline__ double dist2_64(double ax, double ay, double az, double bx, double by,
+                                           double bz, int bc, double L, double invL, bool div)
+{
+    if (bc == BC_CUBIC) {
+        double qx = div ? (ax - bx) / L : (ax - bx) * invL;
+        double qy = div ? (ay - by) / L : (ay - by) * invL;
+        double qz = div ? (az - bz) / L : (az - bz) * invL;
+        bx = bx + L * __builtin_rint(qx);
+        by = by + L * __builtin_rint(qy);
+        bz = bz + L * __builtin_rint(qz);
+    }
+    double dx = ax - bx, dy = ay - by, dz = az - bz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Long-range correction, src/EnergyEvaluation.jl:968-982.
+template <int KC>
+__device__ __forceinline__ double lrc64(int n_atoms, double r_cut, const double (&c)[KC])
+{
+    if (r_cut <= 50.0) return 0.0;
+    double rcs = __builtin_sqrt(r_cut);
+    double rc3 = r_cut * rcs;
+    double e = 0.0;
+#pragma unroll
+    for (int i = 1; i <= KC; ++i) {
+        e += c[i - 1] / rc3 / (double)(2 * i + 1);
+        rc3 *= r_cut;
+    }
+    double n2 = (double)((long long)n_atoms * n_atoms);
+    e *= 3.14159265358979323846 * n2 / 4.0 / (rcs * rcs * rcs);
+    return e;
+}
+
+struct MoveDraw {
+    uint32_t index;
+    double ux, uy, uz;
+    double umet;
+};
+
+// Draws of one move attempt; the FP64 layout spends two Philox blocks per
+// attempt, the binary32 layout one (24-bit uniforms).
+template <typename Real>
+__device__ __forceinline__ MoveDraw move_draw(uint64_t seed, uint32_t run, uint32_t temp, uint32_t sweep,
+                                              uint32_t m, uint32_t attempt, uint32_t nsteps)
+{
+    MoveDraw d;
+    if constexpr (sizeof(Real) == 8) {
+        u32x4 A = draw(seed, run, RNG_MOVE, 2u * attempt, temp, sweep, m);
+        u32x4 B = draw(seed, run, RNG_MOVE, 2u * attempt + 1u, temp, sweep, m);
+        d.ux = u53(A.x, A.y);
+        d.uy = u53(A.z, A.w);
+        d.uz = u53(B.x, B.y);
+        d.umet = u53(B.z, B.w);
+        uint32_t w = (A.y & 0x7ffu) | ((A.w & 0x7ffu) << 11) | ((B.y & 0x3ffu) << 22);
+        d.index = below(w, nsteps);
+    } else {
+        u32x4 C = draw(seed, run, RNG_MOVE, attempt, temp, sweep, m);
+        d.index = below(C.x, nsteps);
+        d.ux = (double)u24(C.y);
+        d.uy = (double)u24(C.z);
+        d.uz = (double)u24(C.w);
+        uint32_t w = ((C.y & 0xffu) << 16) | ((C.z & 0xffu) << 8) | (C.w & 0xffu);
+        d.umet = (double)((float)w * 0x1.0p-24f);
+    }
+    return d;
+}
+
+template <typename Real, int BC, bool DIV>
+__device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const MoveDraw &d, Real delta,
+                                           Real L, Real invL, Real &tx, Real &ty, Real &tz)
+{
+    if constexpr (sizeof(Real) == 8) {
+        tx = xi + (d.ux - 0.5) * delta;
+        ty = yi + (d.uy - 0.5) * delta;
+        tz = zi + (d.uz - 0.5) * delta;
+        if constexpr (BC == BC_CUBIC) {
+            tx = tx - L * __builtin_rint(DIV ? tx / L : tx * invL);
+            ty = ty - L * __builtin_rint(DIV ? ty / L : ty * invL);
+            tz = tz - L * __builtin_rint(DIV ? tz / L : tz * invL);
+        }
+    } else {
+        tx = xi + ((float)d.ux - 0.5f) * delta;
+        ty = yi + ((float)d.uy - 0.5f) * delta;
+        tz = zi + ((float)d.uz - 0.5f) * delta;
+        if constexpr (BC == BC_CUBIC) {
+            float q;
+            q = __builtin_rintf(tx * invL);
+            tx = tx - L * q;
+            q = __builtin_rintf(ty * invL);
+            ty = ty - L * q;
+            q = __builtin_rintf(tz * invL);
+            tz = tz - L * q;
+        }
+    }
+}
+
+template <typename Real>
+__device__ __forceinline__ bool outside_sphere(Real x, Real y, Real z, double r2)
+{
+    if constexpr (sizeof(Real) == 8) {
+        return x * x + y * y + z * z > r2;
+    } else {
+        float s = x * x + y * y;
+        s = s + z * z;
+        return s > (float)r2;
+    }
+}
+
+// Full FP64 energy of a slot's configuration (src/EnergyEvaluation.jl:378-420):
+// rows j of this lane, partners l > j, summed by the cluster butterfly.
+template <typename Real, int PK, int KC, int BC, int ENS, int KL>
+__device__ double full_energy64(const Real *X, int A, int WGS, int gb, int q, int N, double box, double s2,
+                                double rc, const double (&c64)[KC])
+{
+    constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
+    const double invL = BC == BC_CUBIC ? 1.0 / box : 0.0;
+    double part = 0.0;
+    for (int s = 0; s < A; ++s) {
+        const int j = s * KL + q;
+        if (j >= N) break;
+        const double xj = (double)X[(0 * A + s) * WGS + gb + q];
+        const double yj = (double)X[(1 * A + s) * WGS + gb + q];
+        const double zj = (double)X[(2 * A + s) * WGS + gb + q];
+        for (int l = j + 1; l < N; ++l) {
+            const int sl = l / KL, ll = gb + (l % KL);
+            const double xl = (double)X[(0 * A + sl) * WGS + ll];
+            const double yl = (double)X[(1 * A + sl) * WGS + ll];
+            const double zl = (double)X[(2 * A + sl) * WGS + ll];
+            double r2 = dist2_64(xj, yj, zj, xl, yl, zl, BC, box, invL, DIV);
+            r2 = r2 * s2;
+            if (ENS == ENS_NPT) {
+                if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+            } else {
+                part += pair_energy64<PK, KC>(r2, c64);
+            }
+        }
+    }
+    part = group_sum<KL>(part);
+    if (ENS == ENS_NPT) part += lrc64<KC>(N, rc, c64);
+    return part;
+}
+
+template <typename Real, int PK, int KC, int BC, int ENS, int KL>
+__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+{
+    constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const TileInfo tile = a.tiles[blockIdx.x];
+    const int WGS = blockDim.x;
+    const int tid = threadIdx.x;
+    const int lg = tid / KL;
+    const int q = tid % KL;
+    const int gb = lg * KL;
+    const int nslots_max = WGS / KL;
+    const bool active = lg < tile.n_slots;
+    const int N = tile.n_atoms;
+    const int A = tile.n_slab;
+    const uint32_t nsteps = (uint32_t)(ENS == ENS_NPT ? N + 1 : N);
+
+    Real *X = reinterpret_cast<Real *>(lds_raw);
+    double *exE = reinterpret_cast<double *>(lds_raw + sizeof(Real) * 3 * A * WGS);
+    double *exV = exE + nslots_max;
+    int *exT = reinterpret_cast<int *>(exV + nslots_max);
+    int *exS = exT + nslots_max;
+
+    // coefficients: Real copy for the move path, FP64 copy for totals
+    Real c[KC];
+    double c64[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        double v = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+        c64[k] = v;
+        c[k] = (Real)v;
+    }
+    if constexpr (PK == PK_LJ) {
+        c[0] = (Real)a.pot.coeff[0];
+        c[1] = (Real)a.pot.coeff[3];
+    }
+
+    Real *gX = reinterpret_cast<Real *>(a.coords) + tile.coord_off;
+    for (int e = tid; e < 3 * A * WGS; e += WGS) X[e] = gX[e];
+
+    const int slot = tile.slot0 + lg;
+    int run = 0, t = 0, R = 1, tb = 0, lb = 0;
+    uint32_t run_key = 0;
+    double E = 0.0, box = 1.0, radius2 = 0.0, pressure = 0.0;
+    double en_min = 0.0, den = 1.0, v_min = 0.0, dvh = 1.0;
+    if (active) {
+        run = a.slot_run[slot];
+        const RunInfo ri = a.runs[run];
+        R = ri.n_temps;
+        tb = ri.temp_base;
+        lb = ri.local_base;
+        run_key = ri.run_key;
+        radius2 = ri.radius2;
+        pressure = ri.pressure;
+        en_min = ri.en_min;
+        den = ri.delta_en;
+        v_min = ri.v_min;
+        dvh = ri.delta_v;
+        t = a.slot_temp[slot];
+        E = a.slot_en[slot];
+        box = a.slot_box[slot];
+        if (q == 0) {
+            exT[lg] = t;
+            exS[lb + t] = lg;
+        }
+    }
+    __syncthreads();
+    if (a.init_energy && active && (a.init_run < 0 || run == a.init_run)) {
+        // initialise_energy, src/EnergyEvaluation.jl:1662-1692
+        E = full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, 1.0, box * box / 4.0, c64);
+    }
+
+    for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
+        const uint32_t sweep = (uint32_t)(a.sweep0 + cyc);
+        const long long pidx = a.cycle0 + cyc + 1;
+        if (active) {
+            const double beta = a.t_beta[tb + t];
+            const double delta = a.t_displ[2 * (tb + t)];
+            const double delta_v = a.t_displ[2 * (tb + t) + 1];
+            const Real delta_r = (Real)delta;
+            const Real Lr = (Real)box;
+            const Real invLr = (Real)(BC == BC_CUBIC ? 1.0 / box : 0.0);
+            Real rc = (Real)(box * box / 4.0);
+            int nacc = 0, nvacc = 0;
+            for (uint32_t m = 0; m < nsteps; ++m) {
+                MoveDraw d0 = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, 0u, nsteps);
+                const int i = (int)d0.index;
+                bool accept = false;
+                double dE_used = 0.0;
+                if (i < N) {
+                    const int si = i / KL, li = gb + (i % KL);
+                    Real xi = X[(0 * A + si) * WGS + li];
+                    Real yi = X[(1 * A + si) * WGS + li];
+                    Real zi = X[(2 * A + si) * WGS + li];
+                    Real tx, ty, tz;
+                    make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Lr, invLr, tx, ty, tz);
+                    if constexpr (BC == BC_SPHERE) {
+                        // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
+                        int count = 0;
+                        uint32_t attempt = 0;
+                        while (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                            count += 1;
+                            if (count == 50) {
+                                double sx = 0.0, sy = 0.0, sz = 0.0;
+                                for (int s = 0; s < A; ++s) {
+                                    if (s * KL + q >= N) break;
+                                    sx += (double)X[(0 * A + s) * WGS + gb + q];
+                                    sy += (double)X[(1 * A + s) * WGS + gb + q];
+                                    sz += (double)X[(2 * A + s) * WGS + gb + q];
+                                }
+                                sx = group_sum<KL>(sx) / (double)N;
+                                sy = group_sum<KL>(sy) / (double)N;
+                                sz = group_sum<KL>(sz) / (double)N;
+                                for (int s = 0; s < A; ++s) {
+                                    if (s * KL + q >= N) break;
+                                    Real *px = &X[(0 * A + s) * WGS + gb + q];
+                                    Real *py = &X[(1 * A + s) * WGS + gb + q];
+                                    Real *pz = &X[(2 * A + s) * WGS + gb + q];
+                                    *px = (Real)((double)*px - sx);
+                                    *py = (Real)((double)*py - sy);
+                                    *pz = (Real)((double)*pz - sz);
+                                }
+                                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                            } else {
+                                attempt += 1;
+                                MoveDraw dd = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, attempt, nsteps);
+                                xi = X[(0 * A + si) * WGS + li];
+                                yi = X[(1 * A + si) * WGS + li];
+                                zi = X[(2 * A + si) * WGS + li];
+                                make_trial<Real, BC, DIV>(xi, yi, zi, dd, delta_r, Lr, invLr, tx, ty, tz);
+                                if (count == 100) {
+                                    if (q == 0) a.run_error[run] = 1;
+                                    break;
+                                }
+                            }
+                        }
+                        xi = X[(0 * A + si) * WGS + li];
+                        yi = X[(1 * A + si) * WGS + li];
+                        zi = X[(2 * A + si) * WGS + li];
+                    }
+                    // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
+                    Real sn = 0, so = 0;
+#pragma unroll 2
+                    for (int s = 0; s < A; ++s) {
+                        const int j = s * KL + q;
+                        const Real xj = X[(0 * A + s) * WGS + tid];
+                        const Real yj = X[(1 * A + s) * WGS + tid];
+                        const Real zj = X[(2 * A + s) * WGS + tid];
+                        const Real r2n = dist2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, Lr, invLr);
+                        const Real r2o = dist2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, Lr, invLr);
+                        Real en = pair_energy<Real, PK, KC>(r2n, c);
+                        Real eo = pair_energy<Real, PK, KC>(r2o, c);
+                        bool ok = (j < N) & (j != i);
+                        bool okn = ok, oko = ok;
+                        if constexpr (ENS == ENS_NPT) {
+                            okn = ok & (r2n <= rc);
+                            oko = ok & (r2o <= rc);
+                        }
+                        sn += okn ? en : Real(0);
+                        so += oko ? eo : Real(0);
+                    }
+                    sn = group_sum<KL>(sn);
+                    so = group_sum<KL>(so);
+                    const Real delta_en = sn - so;
+                    const double new_en = (double)delta_en + E;
+                    const double dd = new_en - E;
+                    dE_used = dd;
+                    // Metropolis, src/Exchange.jl:53-57 and src/MCRun.jl:122-132
+                    if constexpr (sizeof(Real) == 8) {
+                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                    } else {
+                        float pr = __builtin_amdgcn_exp2f((float)(-dd * beta) * 1.4426950408889634f);
+                        accept = (dd <= 0.0) || (pr >= (float)d0.umet);
+                    }
+                    if (accept) {
+                        if (tid == li) {
+                            X[(0 * A + si) * WGS + li] = tx;
+                            X[(1 * A + si) * WGS + li] = ty;
+                            X[(2 * A + si) * WGS + li] = tz;
+                        }
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        E = new_en;
+                        nacc += 1;
+                    }
+                } else {
+                    // uniform volume move, src/MCMoves.jl:169-177, :277-287; energy
+                    // src/EnergyEvaluation.jl:397-420; acceptance src/Exchange.jl:58-71
+                    if constexpr (ENS == ENS_NPT) {
+                        const double y = (d0.ux - 0.5) * delta_v;
+                        double scale = det_exp(y / 3.0);
+                        if (box >= a.t_maxbox[tb + t] && scale > 1.0) scale = 1.0;
+                        const double nbox = box * scale;
+                        const double nrc = nbox * nbox / 4.0;
+                        const double s2 = scale * scale;
+                        const double new_en =
+                            full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, s2, nrc, c64);
+                        const double dd = new_en - E;
+                        dE_used = dd;
+                        const double vn = nbox * nbox * nbox, vo = box * box * box;
+                        const double dh = dd + pressure * (vn - vo);
+                        const double arg = -dh * beta + (double)N * det_log(vn / vo);
+                        double prob = det_exp(arg);
+                        if (prob > 1.0) prob = 1.0;
+                        accept = prob >= d0.umet;
+                        if (accept) {
+                            for (int s = 0; s < A; ++s) {
+                                Real *px = &X[(0 * A + s) * WGS + tid];
+                                Real *py = &X[(1 * A + s) * WGS + tid];
+                                Real *pz = &X[(2 * A + s) * WGS + tid];
+                                *px = (Real)((double)*px * scale);
+                                *py = (Real)((double)*py * scale);
+                                *pz = (Real)((double)*pz * scale);
+                            }
+                            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                            box = nbox;
+                            E = new_en;
+                            nvacc += 1;
+                        }
+                    }
+                }
+                if (a.trace_de != nullptr && q == 0 && cyc < a.trace_cycles) {
+                    const size_t ti = ((size_t)(tb + t) * a.trace_cycles + cyc) * nsteps + m;
+                    a.trace_de[ti] = dE_used;
+                    a.trace_acc[ti] = (int8_t)accept;
+                    a.trace_idx[ti] = i;
+                }
+                if constexpr (ENS == ENS_NPT) {
+                    // box may change on a volume move
+                    rc = (Real)(box * box / 4.0);
+                }
+            }
+            if constexpr (ENS == ENS_NPT) {
+                (void)Lr;
+            }
+            if (q == 0) {
+                a.t_count_atom[2 * (tb + t)] += nacc;
+                a.t_count_atom[2 * (tb + t) + 1] += nacc;
+                if (ENS == ENS_NPT) {
+                    a.t_count_vol[2 * (tb + t)] += nvacc;
+                    a.t_count_vol[2 * (tb + t) + 1] += nvacc;
+                }
+                exE[lg] = E;
+                exV[lg] = box * box * box;
+            }
+        }
+        __syncthreads();
+
+        // replica exchange, src/Exchange.jl:103-186 and src/MCRun.jl:203-205
+        if (active && q == 0 && R > 1) {
+            int n = -1;
+            if (a.exchange_mode == EXC_EVEN_ODD) {
+                if ((t & 1) == (int)(sweep & 1u) && t + 1 < R) n = t;
+            } else if (t == 0) {
+                u32x4 X0 = draw(a.seed, run_key, RNG_EXCH, 0u, 0u, sweep, 0u);
+                double u = u53(X0.x, X0.y);
+                if (a.exchange_mode == EXC_EVERY || u < 0.1) n = (int)below(X0.z, (uint32_t)(R - 1));
+            }
+            if (n >= 0) {
+                u32x4 X1 = draw(a.seed, run_key, RNG_EXCH, 0u, 0u, sweep, 1u + (uint32_t)n);
+                const double u = u53(X1.x, X1.y);
+                const int sa = exS[lb + n], sb = exS[lb + n + 1];
+                double e1 = exE[sa], e2 = exE[sb];
+                if (a.ensemble == ENS_NPT) {
+                    e1 = e1 + pressure * exV[sa];
+                    e2 = e2 + pressure * exV[sb];
+                }
+                const double de = e1 - e2;
+                const double db = a.t_beta[tb + n] - a.t_beta[tb + n + 1];
+                double acc = det_exp(db * de);
+                if (acc > 1.0) acc = 1.0;
+                a.t_count_exc[2 * (tb + n)] += 1;
+                a.t_count_exc[2 * (tb + n + 1)] += 1;
+                if (acc > u) {
+                    a.t_count_exc[2 * (tb + n) + 1] += 1;
+                    a.t_count_exc[2 * (tb + n + 1) + 1] += 1;
+                    exT[sa] = n + 1;
+                    exT[sb] = n;
+                    exS[lb + n] = sb;
+                    exS[lb + n + 1] = sa;
+                }
+            }
+        }
+        __syncthreads();
+
+        if (active) {
+            t = exT[lg];
+            if (q == 0) {
+                const int T = tb + t;
+                // update_max_stepsize!, src/Exchange.jl:198-263
+                if (pidx % a.n_adjust == 0) {
+                    const double ar = (double)a.t_count_atom[2 * T + 1] / (double)((long long)a.n_adjust * N);
+                    if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
+                    else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
+                    a.t_count_atom[2 * T + 1] = 0;
+                    if (ENS == ENS_NPT) {
+                        const double av = (double)a.t_count_vol[2 * T + 1] / (double)a.n_adjust;
+                        if (av < a.min_acc) a.t_displ[2 * T + 1] *= 0.9;
+                        else if (av > a.max_acc) a.t_displ[2 * T + 1] *= 1.1;
+                        a.t_count_vol[2 * T + 1] = 0;
+                    }
+                }
+                // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
+                if (a.sampling && pidx % a.mc_sample == 0) {
+                    if (ENS == ENS_NPT) {
+                        const double v = box * box * box;
+                        const double h = E + pressure * v;
+                        a.t_ham[2 * T] += h;
+                        a.t_ham[2 * T + 1] += h * h;
+                        const double ie = __builtin_floor((E - en_min) / den) + 1.0;
+                        const double iv = __builtin_floor((v - v_min) / dvh) + 1.0;
+                        const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
+                        const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
+                        a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)] += 1u;
+                    } else {
+                        a.t_ham[2 * T] += E;
+                        a.t_ham[2 * T + 1] += E * E;
+                        const double hi = (E - en_min) / den + 1.0;
+                        const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
+                        a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+                    }
+                }
+            }
+            // periodic FP64 recompute bounding binary32 drift
+            if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
+                const double ef = full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, 1.0,
+                                                                          box * box / 4.0, c64);
+                const double dr = __builtin_fabs(E - ef);
+                if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
+                E = ef;
+            }
+        }
+    }
+
+    __syncthreads();
+    for (int e = tid; e < 3 * A * WGS; e += WGS) gX[e] = X[e];
+    if (active && q == 0) {
+        a.slot_temp[slot] = t;
+        a.slot_en[slot] = E;
+        a.slot_box[slot] = box;
+    }
+}
+
+}  // namespace ptmc

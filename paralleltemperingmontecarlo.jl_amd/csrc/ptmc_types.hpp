@@ -1,0 +1,94 @@
+// Plain structs shared by the host engine and the sweep kernels.
+#pragma once
+#include <stdint.h>
+
+namespace ptmc {
+
+enum Ensemble : int32_t { ENS_NVT = 0, ENS_NPT = 1 };
+enum Boundary : int32_t { BC_SPHERE = 0, BC_CUBIC = 1 };
+enum PotKind : int32_t {
+    POT_ELJ_EVEN = 0,  // sum_k c_k r^-(4+2k), src/EnergyEvaluation.jl:138-153, :934-942
+    POT_ELJ = 1,       // sum_k c_k r^-(5+k),  src/EnergyEvaluation.jl:891-933
+    POT_EAM = 2,       // Sutton-Chen,          src/EnergyEvaluation.jl:1059-1162
+};
+// How a pair energy is evaluated on the device.
+enum PairForm : int32_t {
+    FORM_HORNER = 0,     // Horner in 1/r^2 (fast path)
+    FORM_REFERENCE = 1,  // r^-6 then repeated division, as src/EnergyEvaluation.jl:934-942
+    FORM_LJ = 2,         // two-term c6 u^3 + c12 u^6 (binary32 only)
+};
+enum ExchangeMode : int32_t { EXC_REFERENCE = 0, EXC_EVERY = 1, EXC_EVEN_ODD = 2 };
+
+constexpr int MAX_COEFF = 16;
+
+struct PotParams {
+    int32_t kind;
+    int32_t ncoeff;
+    double coeff[MAX_COEFF];
+    double eam_n, eam_m, eam_ean, eam_ecam;
+};
+
+// One PT run (one temperature ladder).
+struct RunInfo {
+    int32_t n_atoms;
+    int32_t n_temps;
+    int32_t temp_base;   // first index into the per-temperature arrays
+    int32_t slot_base;   // first global slot (a slot holds one configuration)
+    int32_t local_base;  // first slot inside its workgroup
+    uint32_t run_key;    // Philox counter word 3
+    double radius2;      // spherical container radius^2
+    double pressure;
+    double en_min, delta_en;  // energy histogram (src/Sampling.jl:141-158)
+    double v_min, delta_v;    // volume histogram (NPT)
+};
+
+// One workgroup of the sweep kernel: slots of several runs with equal N.
+struct TileInfo {
+    int32_t slot0;      // first global slot
+    int32_t n_slots;    // slots used (groups of KL lanes)
+    int32_t n_atoms;
+    int32_t n_slab;     // atoms per lane = ceil(N / KL)
+    int64_t coord_off;  // element offset of the tile's [3][n_slab][WGS] block
+};
+
+struct SweepArgs {
+    const TileInfo *tiles;
+    const RunInfo *runs;
+    const int32_t *slot_run;   // run of each slot
+    int32_t *slot_temp;        // temperature index held by each slot
+    double *slot_en;           // total energy (FP64) per slot
+    double *slot_box;          // cubic box length per slot
+    double *slot_drift;        // max |E_running - E_recomputed| per slot
+    void *coords;              // per-tile [3][n_slab][WGS] (Real)
+    const double *t_beta;      // per temperature
+    double *t_displ;           // per temperature x2 (atom, volume)
+    const double *t_maxbox;    // per temperature
+    long long *t_count_atom;   // per temperature x2 (total, window)
+    long long *t_count_vol;    // per temperature x2
+    long long *t_count_exc;    // per temperature x2 (attempted, accepted)
+    double *t_ham;             // per temperature x2
+    uint32_t *t_hist;          // per temperature x hist_stride
+    int32_t *run_error;        // per run
+    double *trace_de;          // optional [temp][cycle][move]
+    int8_t *trace_acc;
+    int32_t *trace_idx;
+    PotParams pot;
+    uint64_t seed;
+    int64_t sweep0;            // global cycle counter of the first cycle
+    int64_t cycle0;            // phase-local cycle index minus one
+    int32_t n_cycles;
+    int32_t ensemble;
+    int32_t exchange_mode;
+    int32_t n_adjust;
+    int32_t mc_sample;
+    int32_t n_bin;
+    int32_t hist_stride;
+    int32_t sampling;
+    int32_t recompute_every;
+    int32_t trace_cycles;
+    int32_t init_energy;       // compute slot energies from positions before cycling
+    int32_t init_run;          // run whose energies are initialised (-1: all)
+    double min_acc, max_acc;
+};
+
+}  // namespace ptmc

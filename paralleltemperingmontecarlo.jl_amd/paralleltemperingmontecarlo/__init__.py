@@ -1,0 +1,30 @@
+"""MI355X-native parallel-tempering Monte Carlo engine with the
+ParallelTemperingMonteCarlo.jl API for the PT hot path.
+
+The Monte-Carlo cycles (displacement moves, incremental energies,
+Metropolis, replica exchange, adaptation, sampling) run in HIP kernels in
+lib/libptmc.so; this package is the host-side mirror of the reference's
+Julia types and driver.
+"""
+from .api import (  # noqa: F401
+    CubicBC,
+    Config,
+    ELJPotential,
+    ELJPotentialEven,
+    EmbeddedAtomPotential,
+    EngineOptions,
+    MCParams,
+    MCState,
+    NPT,
+    NVT,
+    Output,
+    PTEngine,
+    SphericalBC,
+    TempGrid,
+    check_e_bounds,
+    kB,
+    ptmc_run,
+)
+from ._lib import PTMCError, load as load_library  # noqa: F401
+
+AtoBohr = 1.8897259886

@@ -1,0 +1,546 @@
+"""Host-side mirror of the ParallelTemperingMonteCarlo.jl API for the PT hot path.
+
+Names, argument meaning and defaults follow the reference's Julia types so a
+script ports line by line; the cycles themselves run in the HIP engine
+(libptmc.so) and nothing here computes energies or moves.
+
+    temp = TempGrid(4.0, 16.0, 32)                      # src/InputParams.jl:77-95
+    mc_params = MCParams(10_000, 32, 13)                # src/InputParams.jl:35-63
+    pot = ELJPotentialEven(c)                           # src/EnergyEvaluation.jl:138-153
+    start_config = Config(pos, SphericalBC(radius=r))   # src/Configurations.jl:42-51
+    mc_states, results = ptmc_run(mc_params, temp, start_config, pot, NVT(13))
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+
+# Hartree / K, src/InputParams.jl:19
+kB = 3.16681196e-6
+# second constant used by the multihistogram analysis, src/multihist_NVT.jl:18
+kB_multihist = 3.166811429 / 10**6
+
+
+# ----------------------------------------------------------------- inputs
+
+
+@dataclasses.dataclass(frozen=True)
+class MCParams:
+    """src/InputParams.jl:35-63; eq_cycles = round(eq_percentage * cycles)."""
+
+    mc_cycles: int
+    eq_cycles: int
+    mc_sample: int
+    n_traj: int
+    n_atoms: int
+    n_adjust: int
+    n_bin: int
+    min_acc: float
+    max_acc: float
+
+    def __init__(self, cycles: int, n_traj: int, n_atoms: int, *, eq_percentage: float = 0.2,
+                 mc_sample: int = 1, n_adjust: int = 100, n_bin: int = 100,
+                 min_acc: float = 0.4, max_acc: float = 0.6):
+        mc_cycles = int(cycles)
+        # Julia round(Int, x) rounds half to even, as Python's round does
+        eq = int(round(eq_percentage * mc_cycles))
+        for k, v in dict(mc_cycles=mc_cycles, eq_cycles=eq, mc_sample=int(mc_sample), n_traj=int(n_traj),
+                         n_atoms=int(n_atoms), n_adjust=int(n_adjust), n_bin=int(n_bin),
+                         min_acc=float(min_acc), max_acc=float(max_acc)).items():
+            object.__setattr__(self, k, v)
+
+
+class TempGrid:
+    """src/InputParams.jl:77-95: geometric (default) or equally spaced grid,
+    beta = 1 / (kB T)."""
+
+    def __init__(self, ti: float, tf: float, n: int, *, tdistr: str = "geometric"):
+        n = int(n)
+        if tdistr == "equally_spaced":
+            delta = (tf - ti) / (n - 1)
+            t = [ti + i * delta for i in range(n)]
+        elif tdistr == "geometric":
+            t = [ti * (tf / ti) ** (i / (n - 1)) for i in range(n)]
+        else:
+            raise ValueError(f"chosen temperature distribution {tdistr} does not exist")
+        self.t_grid = np.array(t, dtype=np.float64)
+        self.beta_grid = 1.0 / (kB * self.t_grid)
+
+    @classmethod
+    def from_beta(cls, beta: Sequence[float]) -> "TempGrid":
+        g = cls.__new__(cls)
+        g.beta_grid = np.asarray(beta, dtype=np.float64).copy()
+        g.t_grid = 1.0 / (kB * g.beta_grid)
+        return g
+
+    def __len__(self):
+        return len(self.t_grid)
+
+
+@dataclasses.dataclass
+class SphericalBC:
+    """src/BoundaryConditions.jl:44-47 (stores radius^2)."""
+
+    radius: float
+
+    @property
+    def radius2(self) -> float:
+        return self.radius * self.radius
+
+
+@dataclasses.dataclass
+class CubicBC:
+    """src/BoundaryConditions.jl:72-77."""
+
+    box_length: float
+
+
+class Config:
+    """src/Configurations.jl:42-51: positions (N x 3, bohr) and a boundary condition."""
+
+    def __init__(self, positions, boundary_condition):
+        self.positions = np.array(positions, dtype=np.float64).reshape(-1, 3)
+        self.boundary_condition = boundary_condition
+
+    def __len__(self):
+        return len(self.positions)
+
+    def __getitem__(self, i):
+        return self.positions[i]
+
+
+@dataclasses.dataclass(frozen=True)
+class NVT:
+    """src/Ensembles.jl:42-50."""
+
+    n_atoms: int
+
+    @property
+    def n_atom_moves(self) -> int:
+        return self.n_atoms
+
+
+@dataclasses.dataclass(frozen=True)
+class NPT:
+    """src/Ensembles.jl:75-86 (uniform volume moves; separated moves are not built)."""
+
+    n_atoms: int
+    pressure: float
+    separated_volume: bool = False
+
+    @property
+    def n_atom_moves(self) -> int:
+        return self.n_atoms
+
+    @property
+    def n_volume_moves(self) -> int:
+        return 1
+
+
+@dataclasses.dataclass(frozen=True)
+class ELJPotentialEven:
+    """sum_k c_k r^-(4+2k), k = 1..K; src/EnergyEvaluation.jl:138-153."""
+
+    coeff: tuple
+
+    def __init__(self, c):
+        object.__setattr__(self, "coeff", tuple(float(x) for x in c))
+
+
+@dataclasses.dataclass(frozen=True)
+class ELJPotential:
+    """sum_k c_k r^-(5+k), k = 1..K; src/EnergyEvaluation.jl:891-908."""
+
+    coeff: tuple
+
+    def __init__(self, c):
+        object.__setattr__(self, "coeff", tuple(float(x) for x in c))
+
+
+@dataclasses.dataclass(frozen=True)
+class EmbeddedAtomPotential:
+    """Sutton-Chen EAM, src/EnergyEvaluation.jl:1059-1073.  Built from the
+    paper constants as the 5-argument constructor does (:1078-1082)."""
+
+    n: float
+    m: float
+    ean: float
+    eCam: float
+
+    @classmethod
+    def from_constants(cls, n, m, eps, C_, a):
+        return cls(float(n), float(m), eps * a**n / 2, eps * C_ * a ** (m / 2))
+
+
+@dataclasses.dataclass
+class EngineOptions:
+    """Settings with no counterpart in the reference.
+
+    precision: "fp64" (everything FP64) or "fp32" (binary32 positions and
+    pair sums, FP64 totals).  exchange: "reference" (p = 0.1 per cycle, one
+    random adjacent pair, src/MCRun.jl:203), "every" (one random pair each
+    cycle) or "even_odd".  pair_form: "horner" or "reference"
+    (the reference's division loop, FP64 only)."""
+
+    precision: str = "fp64"
+    exchange: str = "reference"
+    pair_form: str = "horner"
+    seed: int = 1234
+    recompute_every: int = 0
+    lanes_per_replica: int = 0
+    cycles_per_launch: int = 256
+    device: int = 0
+    max_displ: tuple = (0.1, 0.1)
+
+
+_EXCHANGE = {"reference": 0, "every": 1, "even_odd": 2}
+
+
+# ---------------------------------------------------------------- outputs
+
+
+@dataclasses.dataclass
+class MCState:
+    """Read-back of one temperature position (src/MCStates.jl:33-53)."""
+
+    temp: float
+    beta: float
+    config: Config
+    en_tot: float
+    max_displ: np.ndarray
+    count_atom: np.ndarray
+    count_vol: np.ndarray
+    count_exc: np.ndarray
+    ham: np.ndarray
+
+
+@dataclasses.dataclass
+class Output:
+    """src/InputParams.jl:114-176."""
+
+    n_bin: int
+    en_min: float = 0.0
+    en_max: float = 0.0
+    v_min: float = 0.0
+    v_max: float = 0.0
+    delta_en_hist: float = 0.0
+    delta_v_hist: float = 0.0
+    delta_r2: float = 0.0
+    max_displ: list = dataclasses.field(default_factory=list)
+    en_avg: list = dataclasses.field(default_factory=list)
+    heat_cap: list = dataclasses.field(default_factory=list)
+    en_histogram: list = dataclasses.field(default_factory=list)
+    ev_histogram: list = dataclasses.field(default_factory=list)
+    rdf: list = dataclasses.field(default_factory=list)
+    lh_histogram: list = dataclasses.field(default_factory=list)
+    count_stat_atom: list = dataclasses.field(default_factory=list)
+    count_stat_vol: list = dataclasses.field(default_factory=list)
+    count_stat_rot: list = dataclasses.field(default_factory=list)
+    count_stat_exc: list = dataclasses.field(default_factory=list)
+
+
+# ---------------------------------------------------------------- helpers
+
+
+def check_e_bounds(energy: float, ebounds: list) -> list:
+    """src/MCRun.jl:258-265 (note the elseif: a new minimum never updates the maximum)."""
+    if energy < ebounds[0]:
+        ebounds[0] = energy
+    elif energy > ebounds[1]:
+        ebounds[1] = energy
+    return ebounds
+
+
+def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _lib.ptmc_config:
+    cfg = _lib.ptmc_config()
+    cfg.precision = {"fp64": 0, "fp32": 1}[opts.precision]
+    cfg.ensemble = 1 if isinstance(ensemble, NPT) else 0
+    if isinstance(ensemble, NPT) and ensemble.separated_volume:
+        raise NotImplementedError("separated volume moves are not part of the engine")
+    cfg.boundary = 1 if isinstance(bc, CubicBC) else 0
+    cfg.exchange_mode = _EXCHANGE[opts.exchange]
+    cfg.pair_form = {"horner": 0, "reference": 1}[opts.pair_form]
+    cfg.n_adjust = mc_params.n_adjust
+    cfg.mc_sample = mc_params.mc_sample
+    cfg.n_bin = mc_params.n_bin
+    cfg.min_acc = mc_params.min_acc
+    cfg.max_acc = mc_params.max_acc
+    cfg.recompute_every = opts.recompute_every
+    cfg.lanes_per_replica = opts.lanes_per_replica
+    cfg.cycles_per_launch = opts.cycles_per_launch
+    cfg.device = opts.device
+    cfg.seed = opts.seed & 0xFFFFFFFFFFFFFFFF
+    return cfg
+
+
+def _set_potential(eng: _lib.Engine, pot):
+    if isinstance(pot, (ELJPotentialEven, ELJPotential)):
+        c = np.array(pot.coeff, dtype=np.float64)
+        eng.call("ptmc_set_potential_elj", 1 if isinstance(pot, ELJPotentialEven) else 0, len(c), _lib.ptr(c))
+    elif isinstance(pot, EmbeddedAtomPotential):
+        eng.call("ptmc_set_potential_eam", pot.n, pot.m, pot.ean, pot.eCam)
+    else:
+        raise TypeError(f"potential {type(pot).__name__} is not handled by the pair engine")
+
+
+def _run_desc(n_atoms, beta, x0, bc, pressure, opts: EngineOptions, lanes=0, run_key=0xFFFFFFFF):
+    d = _lib.ptmc_run_desc()
+    d.n_atoms = n_atoms
+    d.n_temps = len(beta)
+    d.radius = bc.radius if isinstance(bc, SphericalBC) else 0.0
+    d.box_length = bc.box_length if isinstance(bc, CubicBC) else 0.0
+    d.pressure = pressure
+    d._beta = np.ascontiguousarray(beta, dtype=np.float64)
+    d._x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    d.beta = _lib.ptr(d._beta)
+    d.x0 = _lib.ptr(d._x0)
+    d.n_configs = d._x0.size // (3 * n_atoms)
+    d.max_displ_atom = opts.max_displ[0]
+    d.max_displ_vol = opts.max_displ[1]
+    d.lanes_per_replica = lanes
+    d.run_key = run_key
+    return d
+
+
+class PTEngine:
+    """A batch of PT runs that advance together on the GPU."""
+
+    def __init__(self, pot, ensemble, bc_kind, mc_params: MCParams, opts: EngineOptions | None = None):
+        self.opts = opts or EngineOptions()
+        self.mc_params = mc_params
+        self.ensemble = ensemble
+        self.pot = pot
+        self.eng = _lib.Engine(_engine_config(mc_params, ensemble, bc_kind, self.opts))
+        _set_potential(self.eng, pot)
+        self.runs = []
+
+    def add_run(self, n_atoms: int, beta, x0, bc, pressure: float = 0.0, lanes: int = 0,
+                run_key: int = 0xFFFFFFFF) -> int:
+        d = _run_desc(n_atoms, beta, x0, bc, pressure, self.opts, lanes, run_key)
+        idx = C.c_int32()
+        self.eng.call("ptmc_add_run", C.byref(d), C.byref(idx))
+        self.runs.append(dict(n_atoms=n_atoms, n_temps=len(beta), bc=bc, beta=np.asarray(beta, float),
+                              pressure=pressure))
+        return idx.value
+
+    def prepare(self):
+        self.eng.call("ptmc_prepare")
+
+    def start_phase(self, production: bool):
+        self.eng.call("ptmc_start_phase", 1 if production else 0)
+
+    def cycles(self, n: int):
+        if n > 0:
+            self.eng.call("ptmc_cycles", int(n))
+
+    def sync(self):
+        self.eng.call("ptmc_sync")
+
+    def energies(self, run: int) -> np.ndarray:
+        out = np.zeros(self.runs[run]["n_temps"])
+        self.eng.call("ptmc_get_energies", run, _lib.ptr(out))
+        return out
+
+    def state(self, run: int):
+        r = self.runs[run]
+        x = np.zeros((r["n_temps"], r["n_atoms"], 3))
+        en = np.zeros(r["n_temps"])
+        box = np.zeros(r["n_temps"])
+        self.eng.call("ptmc_get_state", run, _lib.ptr(x), _lib.ptr(en), _lib.ptr(box))
+        return x, en, box
+
+    def counters(self, run: int):
+        R = self.runs[run]["n_temps"]
+        md = np.zeros((R, 2))
+        ca = np.zeros((R, 2), dtype=np.int64)
+        cv = np.zeros((R, 2), dtype=np.int64)
+        ce = np.zeros((R, 2), dtype=np.int64)
+        ham = np.zeros((R, 2))
+        self.eng.call("ptmc_get_counters", run, _lib.ptr(md), _lib.ptr(ca, C.c_int64), _lib.ptr(cv, C.c_int64),
+                      _lib.ptr(ce, C.c_int64), _lib.ptr(ham))
+        return dict(max_displ=md, count_atom=ca, count_vol=cv, count_exc=ce, ham=ham)
+
+    def histograms(self, run: int) -> np.ndarray:
+        R = self.runs[run]["n_temps"]
+        stride = self.eng.lib.ptmc_hist_stride(self.eng.h)
+        h = np.zeros((R, stride))
+        self.eng.call("ptmc_get_histograms", run, _lib.ptr(h))
+        return h
+
+    def drift(self, run: int) -> np.ndarray:
+        d = np.zeros(self.runs[run]["n_temps"])
+        self.eng.call("ptmc_get_drift", run, _lib.ptr(d))
+        return d
+
+    def set_histogram(self, run: int, en_min, en_max, v_min=0.0, v_max=1.0):
+        self.eng.call("ptmc_set_histogram", run, float(en_min), float(en_max), float(v_min), float(v_max))
+
+    def reset_counters(self):
+        self.eng.call("ptmc_reset_counters")
+
+    def set_state(self, run: int, x=None, box=None, max_displ=None, count_atom=None, count_vol=None,
+                  count_exc=None, ham=None):
+        f = lambda a, t=np.float64: None if a is None else np.ascontiguousarray(a, dtype=t)
+        x, box, max_displ, ham = f(x), f(box), f(max_displ), f(ham)
+        ca, cv, ce = f(count_atom, np.int64), f(count_vol, np.int64), f(count_exc, np.int64)
+        self.eng.call("ptmc_set_state", run, _lib.ptr(x), _lib.ptr(box), _lib.ptr(max_displ),
+                      _lib.ptr(ca, C.c_int64), _lib.ptr(cv, C.c_int64), _lib.ptr(ce, C.c_int64), _lib.ptr(ham))
+
+    def set_trace(self, n_cycles: int):
+        self.eng.call("ptmc_set_trace", int(n_cycles))
+
+    def trace(self, run: int, n_cycles: int):
+        r = self.runs[run]
+        ns = r["n_atoms"] + (1 if isinstance(self.ensemble, NPT) else 0)
+        shape = (r["n_temps"], n_cycles, ns)
+        de = np.zeros(shape)
+        acc = np.zeros(shape, dtype=np.int8)
+        idx = np.zeros(shape, dtype=np.int32)
+        self.eng.call("ptmc_get_trace", run, _lib.ptr(de), _lib.ptr(acc, C.c_int8), _lib.ptr(idx, C.c_int32))
+        return de, acc, idx
+
+    def kernel_time(self, reset: bool = False):
+        ms = C.c_double()
+        n = C.c_int64()
+        self.eng.call("ptmc_kernel_time", C.byref(ms), C.byref(n), 1 if reset else 0)
+        return ms.value, n.value
+
+    def plan_info(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int64()
+        self.eng.call("ptmc_plan_info", C.byref(a), C.byref(b), C.byref(c))
+        return dict(tiles=a.value, launch_groups=b.value, lanes=c.value)
+
+    def lanes_of(self, run: int) -> int:
+        return self.eng.lib.ptmc_run_lanes(self.eng.h, run)
+
+    def close(self):
+        self.eng.close()
+
+
+# --------------------------------------------------------------- ptmc_run
+
+
+def _start_positions(start_config):
+    cfgs = start_config if isinstance(start_config, (list, tuple)) else [start_config]
+    x0 = np.stack([c.positions for c in cfgs])
+    return cfgs[0].boundary_condition, x0
+
+
+def equilibrate(engine: PTEngine, run_ids, mc_params: MCParams):
+    """equilibration / equilibration_cycle!, src/MCRun.jl:282-344: eq_cycles/2
+    moving cycles, then the energy bounds are scanned eq_cycles - eq_cycles/2
+    times over the (unchanged) states, counters are reset and the histograms
+    are initialised (src/Sampling.jl:141-250)."""
+    engine.start_phase(False)
+    engine.cycles(mc_params.eq_cycles // 2)
+    engine.sync()
+    bounds = []
+    for r in run_ids:
+        en = engine.energies(r)
+        eb = [100.0, -100.0]
+        for _ in range(mc_params.eq_cycles - mc_params.eq_cycles // 2):
+            for e in en:
+                eb = check_e_bounds(float(e), eb)
+        v_min = v_max = 0.0
+        bc = engine.runs[r]["bc"]
+        if isinstance(bc, CubicBC):
+            # the first state's current box, src/Sampling.jl:170-171
+            _, _, box = engine.state(r)
+            v_min = box[0] ** 3 * 0.8
+            v_max = box[0] ** 3 * 2.0
+        engine.set_histogram(r, eb[0], eb[1], v_min, v_max)
+        bounds.append((eb[0], eb[1], v_min, v_max))
+    engine.reset_counters()
+    return bounds
+
+
+def finalise(engine: PTEngine, run: int, mc_params: MCParams, temp: TempGrid, bounds, ensemble) -> tuple:
+    """finalise_results, src/Sampling.jl:377-399, plus the state read-back."""
+    r = engine.runs[run]
+    x, en, box = engine.state(run)
+    ctr = engine.counters(run)
+    hist = engine.histograms(run)
+    nb = mc_params.n_bin
+    out = Output(n_bin=nb)
+    out.en_min, out.en_max, out.v_min, out.v_max = bounds
+    out.delta_en_hist = (out.en_max - out.en_min) / (nb - 1)
+    out.delta_v_hist = (out.v_max - out.v_min) / nb
+    n_sample = mc_params.mc_cycles / mc_params.mc_sample
+    ham = ctr["ham"]
+    en_avg = ham[:, 0] / n_sample
+    en2_avg = ham[:, 1] / n_sample
+    beta = r["beta"]
+    out.en_avg = list(en_avg)
+    out.heat_cap = list((en2_avg - en_avg**2) * beta**2)
+    out.count_stat_atom = list(ctr["count_atom"][:, 0] / (mc_params.n_atoms * mc_params.mc_cycles))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        out.count_stat_exc = list(ctr["count_exc"][:, 1] / ctr["count_exc"][:, 0])
+    out.max_displ = list(ctr["max_displ"][:, 0])
+    if isinstance(ensemble, NPT):
+        out.ev_histogram = [h.reshape(nb + 2, nb + 2) for h in hist]
+    else:
+        out.en_histogram = [h for h in hist]
+    states = []
+    for t in range(r["n_temps"]):
+        bc = r["bc"]
+        bct = CubicBC(box[t]) if isinstance(bc, CubicBC) else bc
+        states.append(MCState(temp=float(temp.t_grid[t]), beta=float(beta[t]), config=Config(x[t], bct),
+                              en_tot=float(en[t]), max_displ=ctr["max_displ"][t].copy(),
+                              count_atom=ctr["count_atom"][t].copy(), count_vol=ctr["count_vol"][t].copy(),
+                              count_exc=ctr["count_exc"][t].copy(), ham=ham[t].copy()))
+    return states, out
+
+
+def ptmc_run(mc_params: MCParams, temp: TempGrid, start_config, potential, ensemble, *,
+             save=False, saveconfigs=False, configsname="configuration", workingdirectory=None,
+             engine: EngineOptions | None = None, on_checkpoint=None):
+    """ptmc_run!, src/MCRun.jl:363-432.  Returns (mc_states, results)."""
+    from . import readsave
+
+    opts = engine or EngineOptions()
+    bc, x0 = _start_positions(start_config)
+    if len(temp) != mc_params.n_traj:
+        raise ValueError("temperature grid and n_traj differ")
+    if x0.shape[1] != mc_params.n_atoms:
+        raise ValueError("number of atoms and positions not the same - check starting config")
+    pressure = ensemble.pressure if isinstance(ensemble, NPT) else 0.0
+    wd = workingdirectory
+    if save is not False:
+        readsave.save_init(potential, ensemble, mc_params, temp, wd)
+    eng = PTEngine(potential, ensemble, bc, mc_params, opts)
+    try:
+        run = eng.add_run(mc_params.n_atoms, temp.beta_grid, x0, bc, pressure)
+        eng.prepare()
+        bounds = equilibrate(eng, [run], mc_params)[0]
+        if save is not False:
+            readsave.save_histparams(bounds, mc_params, wd)
+        eng.start_phase(True)
+        done = 0
+        marks = []
+        if save is not False:
+            marks += list(range(int(save), mc_params.mc_cycles + 1, int(save)))
+        if saveconfigs is not False:
+            marks += list(range(int(saveconfigs), mc_params.mc_cycles + 1, int(saveconfigs)))
+        marks = sorted(set(m for m in marks if m <= mc_params.mc_cycles)) + [mc_params.mc_cycles]
+        for mark in marks:
+            eng.cycles(mark - done)
+            done = mark
+            eng.sync()
+            if save is not False and mark % int(save) == 0:
+                readsave.checkpoint(mark, eng, run, mc_params, temp, bounds, ensemble, wd)
+            if saveconfigs is not False and mark % int(saveconfigs) == 0:
+                readsave.save_configs(eng, run, f"{configsname}{mark}", wd)
+            if on_checkpoint:
+                on_checkpoint(mark, eng)
+        if save is not False and mc_params.mc_cycles % int(save) != 0:
+            readsave.checkpoint(mc_params.mc_cycles, eng, run, mc_params, temp, bounds, ensemble, wd)
+        eng.sync()
+        return finalise(eng, run, mc_params, temp, bounds, ensemble)
+    finally:
+        eng.close()

@@ -1,0 +1,127 @@
+"""Write the golden fixtures under tests/golden/ from data held in the
+reference tree (run in a container where /root/reference exists):
+
+    python tests/golden/make_fixtures.py [/root/reference]
+
+Only numbers are extracted (positions, coefficients, histogram counts and the
+expected values of the reference's own tests); no source text is copied.
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+import re
+import sys
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+def _floats(text: str):
+    return [float(x) for x in re.findall(r"-?\d+\.\d+(?:[eE][-+]?\d+)?|-?\d+(?:[eE][-+]?\d+)", text)]
+
+
+def _split_top(body: str):
+    depth = 0
+    args, cur = [], ""
+    for ch in body:
+        if ch in "[(":
+            depth += 1
+        if ch in "])":
+            depth -= 1
+            if depth < 0:
+                break
+        if ch == "," and depth == 0:
+            args.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    args.append(cur.strip())
+    return args
+
+
+def ne13_setup(ref: pathlib.Path):
+    src = (ref / "scripts" / "ne13.jl").read_text()
+    c_block = src[src.index("c = ["): src.index("]", src.index("c = ["))]
+    coeff = _floats(c_block)
+    p_block = src[src.index("pos_ne13 = ["): src.index("\n]", src.index("pos_ne13 = ["))]
+    pos = _floats(p_block)
+    ex = (ref / "examples" / "ne13.jl").read_text()
+    a2b = float(re.search(r"AtoBohr\s*=\s*([0-9.]+)", ex).group(1))
+    rad = float(re.search(r"radius\s*=\s*([0-9.]+)\s*\*\s*AtoBohr", ex).group(1))
+    out = dict(
+        source="scripts/ne13.jl:39-46 (coefficients), :60-74 (positions, Angstrom); examples/ne13.jl:56,66",
+        coeff=coeff,
+        pos_angstrom=[pos[i: i + 3] for i in range(0, len(pos), 3)],
+        AtoBohr=a2b,
+        radius_angstrom=rad,
+    )
+    assert len(out["pos_angstrom"]) == 13 and len(coeff) == 6
+    (HERE / "ne13_setup.json").write_text(json.dumps(out, indent=1))
+
+
+def multihist_ne13(ref: pathlib.Path):
+    s = (ref / "test" / "multihist_test.jl").read_text()
+    i = s.index("results=Output{Float64}(")
+    line = s[i: s.index("\n", i)]
+    args = _split_top(line[len("results=Output{Float64}("):])
+    hist = [_floats(h) for h in re.findall(r"\[([^\[\]]*)\]", args[8])]
+    temp = re.search(r"TempGrid\{(\d+)\}\((\d+),\s*(\d+)\)", s[: s.index("results=")])
+    cv = {int(k): float(v) for k, v in re.findall(r"cv\[(\d+)\]==([-0-9.eE]+)", s)}
+    out = dict(
+        source="test/multihist_test.jl:5-15 (reference Ne13 NVT Output and expected multihistogram_NVT values)",
+        n_temps=int(temp.group(1)),
+        t_min=float(temp.group(2)),
+        t_max=float(temp.group(3)),
+        n_bin=int(args[0]),
+        en_min=float(args[1]),
+        en_max=float(args[2]),
+        en_avg=_floats(args[6]),
+        heat_cap=_floats(args[7]),
+        en_histogram=hist,
+        count_stat_atom=_floats(args[11]),
+        count_stat_exc=_floats(args[14]),
+        conv_threshold=1e-3,
+        expected_cv=cv,
+        expected_len=int(re.search(r"length\(cv\)==(\d+)", s).group(1)),
+    )
+    assert len(hist) == out["n_temps"] and all(len(h) == out["n_bin"] + 2 for h in hist)
+    (HERE / "multihist_ne13.json").write_text(json.dumps(out))
+
+
+def known_answers(ref: pathlib.Path):
+    """Deterministic known answers of the reference's tests, with the lines
+    they come from.  Each value is read from the file it cites."""
+    tp = (ref / "test" / "test_potentials.jl").read_text()
+    rt = (ref / "test" / "runtests.jl").read_text()
+    eam = float(re.search(r"E ≈ (-?[0-9.]+)", tp).group(1))
+    out = dict(
+        eam_three_atoms=dict(source="test/test_potentials.jl:44-56", positions=[[1.0, 2.0, 3.0], [2.0, 4.0, 6.0],
+                                                                                [0.0, 1.0, 0.0]],
+                             n=1.0, m=1.0, eps=1.0, C=1.0, a=1.0, ean=0.5, energy=eam),
+        elj=dict(source="test/test_potentials.jl:3-15",
+                 cases=[dict(kind="elj", coeff=[-2.0, 0.0, 1.0], r2=1.0, energy=-1.0),
+                        dict(kind="elj_even", coeff=[-2.0, 1.0], r2=1.0, energy=-1.0),
+                        dict(kind="elj", coeff=[-1.0, 2.0, 3.0, -4.0, 5.0], r2=1.0, energy=5.0)]),
+        distance=dict(source="test/runtests.jl:74-100",
+                      d2mat_13=float(re.search(r"d2mat\[1, 3\] == ([0-9.]+)", rt).group(1)),
+                      v1=[1.0, 2.0, 3.0], v2=[2.0, 4.0, 6.0], v3_sphere=[0.0, 1.0, 0.0],
+                      v3_cubic=[3.0, 6.0, 9.0], box=10.0, cubic_13=36.0, plain_13=56.0, plain_12=14.0),
+        volume=dict(source="test/runtests.jl:198-202", box=10.0, volume=1000.0),
+        tempgrid=dict(source="test/runtests.jl:216-235", kB=3.16681196e-6),
+        checkpoint_r2=dict(source="test/testing_data/checkpoint/config.1:2",
+                           r2=float(re.search(r"r2:\s*([0-9.]+)", (ref / "test/testing_data/checkpoint/config.1")
+                                              .read_text()).group(1))),
+    )
+    (HERE / "known_answers.json").write_text(json.dumps(out, indent=1))
+
+
+def main():
+    ref = pathlib.Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    ne13_setup(ref)
+    multihist_ne13(ref)
+    known_answers(ref)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,254 @@
+"""Parity of the HIP engine with the CPU oracle, called through the C ABI.
+
+FP64 runs are compared bit for bit: the reference pair form against the
+oracle's restatement of the reference arithmetic, the Horner form against
+the oracle's restatement of the engine arithmetic (same summation tree).
+binary32 runs are replayed: the oracle re-derives every trial move from the
+same Philox stream, evaluates dE in FP64 on the same binary32 positions and
+must agree with the device dE to the stated tolerance; decisions may differ
+only at near-ties.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+import paralleltemperingmontecarlo as pt  # noqa: E402
+from paralleltemperingmontecarlo.api import PTEngine  # noqa: E402
+
+P_ATM = 101325 / 2.9421015697e13  # Pa -> Hartree/bohr^3 (scripts/pbc32-short.jl:47 uses the same)
+
+
+def fcc(n_cells, a):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    pts = [(np.array([i, j, k]) + b) * a for i in range(n_cells) for j in range(n_cells) for k in range(n_cells)
+           for b in base]
+    L = n_cells * a
+    x = np.array(pts) - L / 2 + a / 4
+    return x, L
+
+
+def _engine(pot, ensemble, bc, mc_params, **opts):
+    return PTEngine(pot, ensemble, bc, mc_params, pt.EngineOptions(**opts))
+
+
+def _compare_exact(eng, run, orc):
+    x, en, box = eng.state(run)
+    ctr = eng.counters(run)
+    assert np.array_equal(x, orc.x.reshape(x.shape)), np.max(np.abs(x - orc.x.reshape(x.shape)))
+    assert np.array_equal(en, orc.en_tot), np.max(np.abs(en - orc.en_tot))
+    assert np.array_equal(ctr["max_displ"], orc.max_displ)
+    assert np.array_equal(ctr["count_atom"], orc.count_atom)
+    assert np.array_equal(ctr["count_exc"], orc.count_exc)
+    assert np.array_equal(ctr["ham"], orc.ham)
+
+
+def _ne13_pair(oracle, ne13, *, form, lanes, exchange, n_temps=16, n_adjust=50, precision="fp64"):
+    temp = pt.TempGrid(4.0, 16.0, n_temps)
+    mc = pt.MCParams(300, n_temps, 13, n_adjust=n_adjust)
+    bc = pt.SphericalBC(radius=ne13["radius"])
+    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), pt.NVT(13), bc, mc, precision=precision,
+                  pair_form=form, lanes_per_replica=lanes, exchange=exchange, seed=1234)
+    run = eng.add_run(13, temp.beta_grid, ne13["pos"][None], bc, run_key=0)
+    eng.prepare()
+    mode = oracle.REFERENCE if form == "reference" else oracle.engine_mode(lanes)
+    orc = oracle.OracleRun(13, temp.beta_grid, ne13["pos"], pot=oracle.make_pot("elj_even", ne13["coeff"]),
+                           radius=ne13["radius"], mode=mode, fp32=precision == "fp32",
+                           exchange_mode={"reference": 0, "every": 1, "even_odd": 2}[exchange],
+                           n_adjust=n_adjust, seed=1234, run_key=0)
+    return eng, run, orc, mc
+
+
+def test_initial_energy_bit_exact(oracle, ne13):
+    eng, run, orc, _ = _ne13_pair(oracle, ne13, form="horner", lanes=1, exchange="reference")
+    assert np.array_equal(eng.energies(run), orc.en_tot)
+    eng.close()
+
+
+def test_fp64_reference_form_matches_reference_arithmetic(oracle, ne13):
+    """Ne13, reference pair form, one lane per configuration, reference exchange
+    (p = 0.1): equilibration + production reproduce the oracle's restatement
+    of the Julia arithmetic bit for bit (positions, energies, step sizes,
+    counters, energy sums, histograms) and the per-move trace."""
+    eng, run, orc, mc = _ne13_pair(oracle, ne13, form="reference", lanes=1, exchange="reference")
+    eng.set_trace(40)
+    eng.start_phase(False)
+    eng.cycles(40)
+    eng.sync()
+    de, acc, idx = eng.trace(run, 40)
+    orc.start_phase(False)
+    tr = orc.cycles(40, trace=True)
+    assert np.array_equal(idx, tr["idx"])
+    assert np.array_equal(acc, tr["acc"])
+    assert np.array_equal(de, tr["de"])
+    _compare_exact(eng, run, orc)
+
+    # equilibration tail, histogram set-up and production with sampling
+    eng.cycles(60)
+    orc.cycles(60)
+    e = eng.energies(run)
+    lo, hi = float(e.min()), float(e.max())
+    eng.set_histogram(run, lo, hi)
+    orc.set_histogram(lo, hi)
+    eng.reset_counters()
+    orc.reset_counters()
+    eng.start_phase(True)
+    orc.start_phase(True)
+    eng.cycles(300)
+    eng.sync()
+    orc.cycles(300)
+    _compare_exact(eng, run, orc)
+    assert np.array_equal(eng.histograms(run), orc.hist)
+    eng.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_fp64_horner_bit_exact(oracle, ne13, lanes):
+    eng, run, orc, _ = _ne13_pair(oracle, ne13, form="horner", lanes=lanes, exchange="even_odd")
+    eng.set_trace(30)
+    eng.start_phase(False)
+    eng.cycles(30)
+    eng.sync()
+    de, acc, idx = eng.trace(run, 30)
+    orc.start_phase(False)
+    tr = orc.cycles(30, trace=True)
+    assert np.array_equal(idx, tr["idx"])
+    assert np.array_equal(acc, tr["acc"])
+    assert np.array_equal(de, tr["de"])
+    eng.cycles(270)
+    eng.sync()
+    orc.cycles(270)
+    _compare_exact(eng, run, orc)
+    eng.close()
+
+
+@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
+def test_fp64_cubic_bit_exact(oracle, ne13, ensemble):
+    """Periodic neon (fcc 32, scripts/pbc32-short.jl density) with minimum
+    image; NPT adds uniform volume moves, cut-off L^2/4 and the tail correction."""
+    a = 8.7674 * ne13["AtoBohr"] / 2
+    x, L = fcc(2, a)
+    N, R = len(x), 8
+    temp = pt.TempGrid(10.0, 40.0, R)
+    mc = pt.MCParams(200, R, N, n_adjust=25)
+    bc = pt.CubicBC(L)
+    ens = pt.NPT(N, P_ATM) if ensemble == "npt" else pt.NVT(N)
+    for lanes in (1, 4):
+        eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), ens, bc, mc, lanes_per_replica=lanes,
+                      exchange="even_odd", seed=99)
+        run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=P_ATM if ensemble == "npt" else 0.0, run_key=3)
+        eng.prepare()
+        orc = oracle.OracleRun(N, temp.beta_grid, x, pot=oracle.make_pot("elj_even", ne13["coeff"]), bc="cubic",
+                               box=L, ensemble=ensemble, pressure=P_ATM if ensemble == "npt" else 0.0,
+                               mode=oracle.engine_mode(lanes), exchange_mode=2, n_adjust=25, seed=99, run_key=3)
+        assert np.array_equal(eng.energies(run), orc.en_tot)
+        eng.start_phase(False)
+        orc.start_phase(False)
+        eng.cycles(60)
+        eng.sync()
+        orc.cycles(60)
+        _compare_exact(eng, run, orc)
+        _, _, box = eng.state(run)
+        assert np.array_equal(box, orc.box)
+        ctr = eng.counters(run)
+        assert np.array_equal(ctr["count_vol"], orc.count_vol)
+        if ensemble == "npt":
+            assert orc.count_vol[:, 0].sum() > 0
+        eng.close()
+
+
+def _lj_runs(n_atoms, n_temps, seed=7):
+    """LJ cluster start: uniform in a sphere of radius 1.2 N^(1/3) sigma with
+    minimum pair distance 0.8 sigma (sigma = epsilon = 1)."""
+    rng = np.random.default_rng(seed)
+    rad = 1.2 * n_atoms ** (1 / 3)
+    pts = []
+    while len(pts) < n_atoms:
+        p = rng.uniform(-rad, rad, 3)
+        if p @ p <= (0.95 * rad) ** 2 and all(np.sum((p - q) ** 2) >= 0.64 for q in pts):
+            pts.append(p)
+    beta = 1.0 / (0.05 * (0.5 / 0.05) ** (np.arange(n_temps) / (n_temps - 1)))
+    return np.array(pts), rad, beta
+
+
+LJ_COEFF = [-4.0, 0.0, 0.0, 4.0]  # ELJPotentialEven{4}([-4 eps sigma^6, 0, 0, 4 eps sigma^12])
+
+
+@pytest.mark.parametrize("n_atoms,lanes", [(13, 1), (38, 2), (55, 4)])
+def test_fp32_lj_replay(oracle, n_atoms, lanes):
+    """binary32 positions and pair sums: every device dE agrees with the FP64
+    evaluation on the same positions to 2e-5 of the atom's interaction scale,
+    decisions differ only where |e^{-beta dE} - u| is within that error, and
+    replaying the device decisions reproduces the device positions bit for bit."""
+    x, rad, beta = _lj_runs(n_atoms, 12)
+    R = len(beta)
+    mc = pt.MCParams(100, R, n_atoms, n_adjust=20)
+    bc = pt.SphericalBC(radius=rad)
+    eng = _engine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(n_atoms), bc, mc, precision="fp32",
+                  lanes_per_replica=lanes, exchange="even_odd", seed=5)
+    run = eng.add_run(n_atoms, beta, x[None], bc, run_key=11)
+    eng.prepare()
+    ncyc = 25
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc = oracle.OracleRun(n_atoms, beta, x, pot=oracle.make_pot("elj_even", LJ_COEFF), radius=rad,
+                           mode=oracle.engine_mode(1), fp32=True, exchange_mode=2, n_adjust=20, seed=5, run_key=11)
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, replay_acc=acc)
+    assert np.array_equal(idx, tr["idx"])
+    scale = 6.0 * n_atoms ** 0  # pair well depth 1, ~12 neighbours at most: |sum e_ij| <= ~6
+    err = np.abs(de - tr["de"])
+    assert err.max() <= 2e-5 * scale, err.max()
+    x_dev, en_dev, _ = eng.state(run)
+    assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
+    # FP64 running totals of both sides agree to the accumulated binary32 error
+    assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-4
+    eng.close()
+
+
+def test_batch_packing_is_invisible(oracle):
+    """Runs of different N and ladder length packed into shared workgroups give
+    exactly the results each run gives alone."""
+    specs = [(13, 24, 1), (38, 40, 2), (13, 33, 3), (55, 28, 4), (38, 17, 5)]
+    mc = pt.MCParams(50, 1, 1, n_adjust=10)
+    results = []
+    eng = None
+    for solo in (False, True):
+        out = []
+        if not solo:
+            eng = _engine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, precision="fp32",
+                          exchange="reference", seed=77)
+        for (n, R, key) in specs:
+            x, rad, beta = _lj_runs(n, R, seed=key)
+            if solo:
+                e = _engine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(n), pt.SphericalBC(rad), mc, precision="fp32",
+                            exchange="reference", seed=77)
+                r = e.add_run(n, beta, x[None], pt.SphericalBC(rad), run_key=key)
+                e.prepare()
+                e.start_phase(False)
+                e.cycles(50)
+                e.sync()
+                out.append(e.state(r))
+                e.close()
+            else:
+                eng.add_run(n, beta, x[None], pt.SphericalBC(rad), run_key=key)
+        if not solo:
+            eng.prepare()
+            assert eng.plan_info()["launch_groups"] >= 2
+            eng.start_phase(False)
+            eng.cycles(50)
+            eng.sync()
+            out = [eng.state(i) for i in range(len(specs))]
+            eng.close()
+        results.append(out)
+    for a, b in zip(*results):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
