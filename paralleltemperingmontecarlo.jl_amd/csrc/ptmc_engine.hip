@@ -220,9 +220,8 @@ int plan(ptmc_engine *h)
     h->launches.clear();
     int slot = 0;
     int64_t coord = 0;
-    // one launch per lanes-per-configuration value
-    std::map<int, std::vector<int>> tiles_by_kl;
-    std::map<int, int> wgs_by_kl;
+    // one launch per (lanes per configuration, workgroup size)
+    std::map<std::pair<int, int>, std::vector<int>> tiles_by_kl;
     for (auto &g : groups) {
         const int kl = g.first.first, N = g.first.second;
         std::vector<int> &rs = g.second;
@@ -230,9 +229,17 @@ int plan(ptmc_engine *h)
                          [&](int a, int b) { return h->runs[a].d.n_temps > h->runs[b].d.n_temps; });
         int maxR = 0;
         for (int r : rs) maxR = std::max(maxR, h->runs[r].d.n_temps);
-        int wgs = 256;  // __launch_bounds__(256); whole ladders fit (R * KL <= 256)
-        wgs = std::max(wgs, wgs_by_kl.count(kl) ? wgs_by_kl[kl] : 0);
-        wgs_by_kl[kl] = wgs;
+        // largest workgroup (<= 256 lanes, __launch_bounds__) whose LDS fits
+        const int A = (N + kl - 1) / kl;
+        const size_t per_lane = rsize * 3 * A + (8 + 8 + 4 + 4 + kl - 1) / kl;
+        int wgs = 256;
+        while (wgs > 64 && (size_t)wgs * per_lane > (size_t)max_lds) wgs -= 64;
+        const int need = ((maxR * kl + 63) / 64) * 64;
+        if ((size_t)wgs * per_lane > (size_t)max_lds || wgs < need)
+            return fail(h, PTMC_E_UNSUPPORTED,
+                        "N = %d with %d lanes per configuration needs %zu B of LDS per lane; a %d-temperature ladder "
+                        "does not fit one workgroup (use more lanes per configuration)",
+                        N, kl, per_lane, maxR);
         const int cap = wgs / kl;
         // first-fit of whole ladders into tiles of cap slots
         std::vector<std::vector<int>> packs;
@@ -266,7 +273,7 @@ int plan(ptmc_engine *h)
             }
             ti.n_slots = local;
             slot += local;
-            tiles_by_kl[kl].push_back((int)h->tiles.size());
+            tiles_by_kl[{kl, wgs}].push_back((int)h->tiles.size());
             h->tiles.push_back(ti);
         }
     }
@@ -276,11 +283,11 @@ int plan(ptmc_engine *h)
     std::vector<TileInfo> ordered;
     std::vector<int> old_to_new(h->tiles.size());
     for (auto &kv : tiles_by_kl) {
-        const int kl = kv.first;
+        const int kl = kv.first.first;
         Launch L;
         L.tile0 = (int)ordered.size();
         L.ntiles = (int)kv.second.size();
-        L.wgs = wgs_by_kl[kl];
+        L.wgs = kv.first.second;
         L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
         if (!L.fn)
             return fail(h, PTMC_E_UNSUPPORTED, "no kernel for precision %d boundary %d ensemble %d pair %d/%d lanes %d",

@@ -350,8 +350,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
             const double delta = a.t_displ[2 * (tb + t)];
             const double delta_v = a.t_displ[2 * (tb + t) + 1];
             const Real delta_r = (Real)delta;
-            const Real Lr = (Real)box;
-            const Real invLr = (Real)(BC == BC_CUBIC ? 1.0 / box : 0.0);
+            Real Lr = (Real)box;
+            Real invLr = (Real)(BC == BC_CUBIC ? 1.0 / box : 0.0);
             Real rc = (Real)(box * box / 4.0);
             int nacc = 0, nvacc = 0;
             for (uint32_t m = 0; m < nsteps; ++m) {
@@ -497,12 +497,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
                     a.trace_idx[ti] = i;
                 }
                 if constexpr (ENS == ENS_NPT) {
-                    // box may change on a volume move
+                    // an accepted volume move changes the box for the remaining moves
                     rc = (Real)(box * box / 4.0);
+                    Lr = (Real)box;
+                    invLr = (Real)(1.0 / box);
                 }
-            }
-            if constexpr (ENS == ENS_NPT) {
-                (void)Lr;
             }
             if (q == 0) {
                 a.t_count_atom[2 * (tb + t)] += nacc;
