@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lanes-table", type=str, default="",
+                    help='JSON {"N": lanes per configuration}; default: the tuned table, else the engine default')
     ap.add_argument("--cycles-per-launch", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -130,8 +132,10 @@ def main():
     opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
                             lanes_per_replica=args.lanes, cycles_per_launch=args.cycles_per_launch)
     eng = pt.PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
+    table = {int(k): int(v) for k, v in json.loads(args.lanes_table).items()} if args.lanes_table else {}
     for r in runs:
-        eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], pt.SphericalBC(r["radius"]), run_key=r["key"])
+        eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], pt.SphericalBC(r["radius"]), run_key=r["key"],
+                    lanes=table.get(r["n_atoms"], 0))
     eng.prepare()
     plan = eng.plan_info()
 

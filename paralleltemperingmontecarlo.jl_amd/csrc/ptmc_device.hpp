@@ -131,14 +131,39 @@ __device__ __forceinline__ double det_log(double x)
     return fma(ed, ln2_hi, fma(ed, ln2_lo, lm));
 }
 
-// Sum over the KL lanes of a lane cluster with an xor butterfly.  Every lane
-// ends with the same bits because each step adds the same two operands.
+// DPP lane permutations (VALU, no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ float dpp_perm(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the KL lanes of a lane cluster (KL <= 16: one DPP row).  Step k
+// adds the partner whose index differs in bit k after steps 0..k-1 have made
+// every lane of a 2^k block hold the block sum: quad_perm [1,0,3,2] (xor 1),
+// quad_perm [2,3,0,1] (xor 2), row_half_mirror and row_mirror pair blocks of
+// 4 and 8.  Each step adds the same two operands on every lane, so all lanes
+// end with identical bits, equal to an xor butterfly.
 template <int KL, typename T>
 __device__ __forceinline__ T group_sum(T v)
 {
-#pragma unroll
-    for (int m = 1; m < KL; m <<= 1) v = v + __shfl_xor(v, m, 64);
+    static_assert(KL == 1 || KL == 2 || KL == 4 || KL == 8 || KL == 16, "cluster width");
+    if constexpr (KL >= 2) v = v + dpp_perm<0xB1>(v);
+    if constexpr (KL >= 4) v = v + dpp_perm<0x4E>(v);
+    if constexpr (KL >= 8) v = v + dpp_perm<0x141>(v);
+    if constexpr (KL >= 16) v = v + dpp_perm<0x140>(v);
     return v;
 }
+
+// Value of lane `src` (absolute lane index within the wave) of a 32-bit word.
+__device__ __forceinline__ uint32_t lane_get(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
 
 }  // namespace ptmc

@@ -25,6 +25,7 @@ struct RunHost {
     std::vector<double> beta;
     std::vector<double> x0;  // n_configs * N * 3
     int kl = 1;
+    int lanes = 256;  // workgroup width of the run's tile
     RunInfo info{};
     int tile = -1;
 };
@@ -59,6 +60,10 @@ struct ptmc_engine {
     std::string err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side streams (default priority) that run launch groups concurrently
+    std::vector<hipStream_t> side;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
     double kernel_ms = 0.0;
     long long n_launch = 0;
 
@@ -141,18 +146,24 @@ int default_lanes(int n_atoms, int precision)
     return kl;
 }
 
-SweepFn lookup(int precision, int bc, int ens, int pk, int kc, int kl)
+SweepFn lookup(int precision, int bc, int ens, int pk, int kc, int kl, int lanes)
 {
     if (precision == 0) {
-        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl);
-        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl);
-        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f64_cubic_npt(pk, kc, kl);
+        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl, lanes);
+        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl, lanes);
+        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f64_cubic_npt(pk, kc, kl, lanes);
     } else {
-        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f32_sphere_nvt(pk, kc, kl);
-        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f32_cubic_nvt(pk, kc, kl);
-        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f32_cubic_npt(pk, kc, kl);
+        if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f32_sphere_nvt(pk, kc, kl, lanes);
+        if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f32_cubic_nvt(pk, kc, kl, lanes);
+        if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f32_cubic_npt(pk, kc, kl, lanes);
     }
     return nullptr;
+}
+
+size_t tile_lds(size_t rsize, int n_atoms, int kl, int lanes)
+{
+    const int A = (n_atoms + kl - 1) / kl;
+    return rsize * 3 * A * lanes + (size_t)(lanes / kl) * (8 + 8 + 4 + 4);
 }
 
 int check_run(ptmc_engine *h, int32_t run)
@@ -202,44 +213,46 @@ int plan(ptmc_engine *h)
     int max_lds = 0;
     HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 
-    // group runs by (lanes per configuration, N)
-    std::map<std::pair<int, int>, std::vector<int>> groups;
+    // group runs by (lanes per configuration, workgroup width, N)
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     for (int r = 0; r < (int)h->runs.size(); ++r) {
         RunHost &rh = h->runs[r];
-        int kl = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
-                 : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
-                                                : default_lanes(rh.d.n_atoms, prec);
-        while (kl > 1 && rh.d.n_temps * kl > 256) kl /= 2;
-        if (rh.d.n_temps * kl > 256)
-            return fail(h, PTMC_E_UNSUPPORTED, "run %d: %d temperatures exceed one workgroup", r, rh.d.n_temps);
+        const int N = rh.d.n_atoms, R = rh.d.n_temps;
+        const int req = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
+                        : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
+                                                       : default_lanes(N, prec);
+        // smallest workgroup width that holds the whole ladder; more lanes per
+        // configuration when the slab does not fit LDS
+        int kl = -1, lanes = 0;
+        for (int w : {LANES_SMALL, LANES_LARGE}) {
+            int k = req;
+            if (w == LANES_LARGE && k < 2) k = 2;
+            while (k < 16 && tile_lds(rsize, N, k, w) > (size_t)max_lds) k *= 2;
+            if (R * k <= w && tile_lds(rsize, N, k, w) <= (size_t)max_lds) {
+                kl = k;
+                lanes = w;
+                break;
+            }
+        }
+        if (kl < 0)
+            return fail(h, PTMC_E_UNSUPPORTED, "run %d: %d temperatures x %d lanes per configuration do not fit one "
+                        "workgroup", r, R, req);
         rh.kl = kl;
-        groups[{kl, rh.d.n_atoms}].push_back(r);
+        rh.lanes = lanes;
+        groups[{kl, lanes, N}].push_back(r);
     }
 
     h->tiles.clear();
     h->launches.clear();
     int slot = 0;
     int64_t coord = 0;
-    // one launch per (lanes per configuration, workgroup size)
+    // one launch per (lanes per configuration, workgroup width)
     std::map<std::pair<int, int>, std::vector<int>> tiles_by_kl;
     for (auto &g : groups) {
-        const int kl = g.first.first, N = g.first.second;
+        const int kl = std::get<0>(g.first), wgs = std::get<1>(g.first), N = std::get<2>(g.first);
         std::vector<int> &rs = g.second;
         std::stable_sort(rs.begin(), rs.end(),
                          [&](int a, int b) { return h->runs[a].d.n_temps > h->runs[b].d.n_temps; });
-        int maxR = 0;
-        for (int r : rs) maxR = std::max(maxR, h->runs[r].d.n_temps);
-        // largest workgroup (<= 256 lanes, __launch_bounds__) whose LDS fits
-        const int A = (N + kl - 1) / kl;
-        const size_t per_lane = rsize * 3 * A + (8 + 8 + 4 + 4 + kl - 1) / kl;
-        int wgs = 256;
-        while (wgs > 64 && (size_t)wgs * per_lane > (size_t)max_lds) wgs -= 64;
-        const int need = ((maxR * kl + 63) / 64) * 64;
-        if ((size_t)wgs * per_lane > (size_t)max_lds || wgs < need)
-            return fail(h, PTMC_E_UNSUPPORTED,
-                        "N = %d with %d lanes per configuration needs %zu B of LDS per lane; a %d-temperature ladder "
-                        "does not fit one workgroup (use more lanes per configuration)",
-                        N, kl, per_lane, maxR);
         const int cap = wgs / kl;
         // first-fit of whole ladders into tiles of cap slots
         std::vector<std::vector<int>> packs;
@@ -288,7 +301,7 @@ int plan(ptmc_engine *h)
         L.tile0 = (int)ordered.size();
         L.ntiles = (int)kv.second.size();
         L.wgs = kv.first.second;
-        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
+        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl, L.wgs);
         if (!L.fn)
             return fail(h, PTMC_E_UNSUPPORTED, "no kernel for precision %d boundary %d ensemble %d pair %d/%d lanes %d",
                         prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
@@ -297,7 +310,7 @@ int plan(ptmc_engine *h)
             TileInfo ti = h->tiles[t];
             ti.coord_off = coord;
             coord += (int64_t)3 * ti.n_slab * L.wgs;
-            size_t lds = rsize * 3 * ti.n_slab * L.wgs + (size_t)(L.wgs / kl) * (8 + 8 + 4 + 4);
+            size_t lds = tile_lds(rsize, ti.n_atoms, kl, L.wgs);
             maxlds = std::max(maxlds, lds);
             old_to_new[t] = (int)ordered.size();
             ordered.push_back(ti);
@@ -309,6 +322,16 @@ int plan(ptmc_engine *h)
         HIPCHK(h, hipFuncSetAttribute((const void *)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
         h->launches.push_back(L);
     }
+    // heaviest launch group first (per-cycle work of its largest tile ~ N * slab)
+    std::stable_sort(h->launches.begin(), h->launches.end(), [&](const Launch &x, const Launch &y) {
+        auto work = [&](const Launch &L) {
+            long long w = 0;
+            for (int t = L.tile0; t < L.tile0 + L.ntiles; ++t)
+                w = std::max<long long>(w, (long long)ordered[t].n_atoms * ordered[t].n_slab);
+            return w;
+        };
+        return work(x) > work(y);
+    });
     for (auto &rh : h->runs) rh.tile = old_to_new[rh.tile];
     h->tiles = ordered;
     h->n_coord = coord;
@@ -321,19 +344,17 @@ template <typename Real>
 void scatter_coords(ptmc_engine *h, std::vector<Real> &buf)
 {
     buf.assign((size_t)h->n_coord, (Real)0);
+    const Real far = h->cfg.boundary == BC_SPHERE ? (Real)(sizeof(Real) == 4 ? 1.0e18 : 1.0e100) : (Real)0;
     for (auto &rh : h->runs) {
         const TileInfo &ti = h->tiles[rh.tile];
-        int wgs = 256;
-        for (auto &L : h->launches)
-            if (rh.tile >= L.tile0 && rh.tile < L.tile0 + L.ntiles) wgs = L.wgs;
         const int N = rh.d.n_atoms, A = ti.n_slab, kl = rh.kl;
         for (int t = 0; t < rh.d.n_temps; ++t) {
             const int lg = rh.info.local_base + t;  // slot t holds temperature t at start
             const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
-            for (int j = 0; j < N; ++j) {
+            for (int j = 0; j < A * kl; ++j) {
                 const int s = j / kl, lane = lg * kl + j % kl;
                 for (int c = 0; c < 3; ++c)
-                    buf[(size_t)ti.coord_off + ((size_t)c * A + s) * wgs + lane] = (Real)x[3 * j + c];
+                    buf[(size_t)ti.coord_off + ((size_t)s * 3 + c) * rh.lanes + lane] = j < N ? (Real)x[3 * j + c] : far;
             }
         }
     }
@@ -382,13 +403,30 @@ SweepArgs base_args(ptmc_engine *h)
 
 int launch_all(ptmc_engine *h, SweepArgs a)
 {
-    for (auto &L : h->launches) {
+    const size_t nl = h->launches.size();
+    if (nl == 1) {
+        const Launch &L = h->launches[0];
         SweepArgs b = a;
         b.tiles = h->d_tiles.p + L.tile0;
         void *args[] = {&b};
         HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, h->stream));
         h->n_launch += 1;
+        return PTMC_OK;
     }
+    // launch groups are independent: fork them onto side streams, join back
+    HIPCHK(h, hipEventRecord(h->fork, h->stream));
+    for (size_t k = 0; k < nl; ++k) {
+        const Launch &L = h->launches[k];
+        hipStream_t st = h->side[k % h->side.size()];
+        HIPCHK(h, hipStreamWaitEvent(st, h->fork, 0));
+        SweepArgs b = a;
+        b.tiles = h->d_tiles.p + L.tile0;
+        void *args[] = {&b};
+        HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, st));
+        HIPCHK(h, hipEventRecord(h->join[k], st));
+        h->n_launch += 1;
+    }
+    for (size_t k = 0; k < nl; ++k) HIPCHK(h, hipStreamWaitEvent(h->stream, h->join[k], 0));
     return PTMC_OK;
 }
 
@@ -445,6 +483,10 @@ int ptmc_destroy(ptmc_handle h)
     h->d_trace_de.free_();
     h->d_trace_acc.free_();
     h->d_trace_idx.free_();
+    for (auto st : h->side) (void)hipStreamSynchronize(st);
+    for (auto ev : h->join) (void)hipEventDestroy(ev);
+    if (h->fork) (void)hipEventDestroy(h->fork);
+    for (auto st : h->side) (void)hipStreamDestroy(st);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -513,6 +555,20 @@ int ptmc_prepare(ptmc_handle h)
     if (rc) return rc;
     rc = plan(h);
     if (rc) return rc;
+    if (h->launches.size() > 1) {
+        const size_t ns = std::min<size_t>(h->launches.size(), 4);
+        while (h->side.size() < ns) {
+            hipStream_t st;
+            HIPCHK(h, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            h->side.push_back(st);
+        }
+        if (!h->fork) HIPCHK(h, hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+        while (h->join.size() < h->launches.size()) {
+            hipEvent_t ev;
+            HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            h->join.push_back(ev);
+        }
+    }
 
     const int nb2 = h->cfg.n_bin + 2;
     h->hist_stride = h->cfg.ensemble == ENS_NPT ? nb2 * nb2 : nb2;
@@ -720,9 +776,9 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
     }
     if (x) {
         const TileInfo &ti = h->tiles[rh.tile];
-        const int wgs = launch_wgs(h, rh.tile), A = ti.n_slab, kl = rh.kl;
+        const int A = ti.n_slab, kl = rh.kl;
         const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
-        const size_t nblk = (size_t)3 * A * wgs;
+        const size_t nblk = (size_t)3 * A * rh.lanes;
         std::vector<unsigned char> blk(nblk * rsize);
         HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
         for (int s = 0; s < R; ++s) {
@@ -730,7 +786,7 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
             for (int j = 0; j < N; ++j) {
                 const int sl = j / kl, lane = lg * kl + j % kl;
                 for (int c = 0; c < 3; ++c) {
-                    const size_t e_ = ((size_t)c * A + sl) * wgs + lane;
+                    const size_t e_ = ((size_t)sl * 3 + c) * rh.lanes + lane;
                     double v;
                     if (rsize == 8) memcpy(&v, &blk[e_ * 8], 8);
                     else {
@@ -819,9 +875,9 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
         if (box) HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, box, R * 8, hipMemcpyHostToDevice));
         if (x) {
             const TileInfo &ti = h->tiles[rh.tile];
-            const int wgs = launch_wgs(h, rh.tile), A = ti.n_slab, kl = rh.kl;
+            const int A = ti.n_slab, kl = rh.kl;
             const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
-            const size_t nblk = (size_t)3 * A * wgs;
+            const size_t nblk = (size_t)3 * A * rh.lanes;
             std::vector<unsigned char> blk(nblk * rsize);
             HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
             for (int s = 0; s < R; ++s) {
@@ -829,7 +885,7 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
                 for (int j = 0; j < N; ++j) {
                     const int sl = j / kl, lane = lg * kl + j % kl;
                     for (int c = 0; c < 3; ++c) {
-                        const size_t e_ = ((size_t)c * A + sl) * wgs + lane;
+                        const size_t e_ = ((size_t)sl * 3 + c) * rh.lanes + lane;
                         const double v = x[((size_t)s * N + j) * 3 + c];
                         if (rsize == 8) memcpy(&blk[e_ * 8], &v, 8);
                         else {

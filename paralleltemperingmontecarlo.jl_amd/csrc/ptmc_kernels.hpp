@@ -2,62 +2,94 @@
 // ptmc_inst_<precision>_<bc>.hip instantiates one precision / boundary /
 // ensemble family; the engine resolves a pointer at plan time.
 #pragma once
+#include "ptmc_sweep.hpp"
 #include "ptmc_types.hpp"
 
 namespace ptmc {
 
 using SweepFn = void (*)(SweepArgs);
 
-// precision: 0 FP64, 1 binary32.  Returns nullptr if not built.
-SweepFn lookup_f64_sphere_nvt(int pk, int kc, int kl);
-SweepFn lookup_f64_cubic_nvt(int pk, int kc, int kl);
-SweepFn lookup_f64_cubic_npt(int pk, int kc, int kl);
-SweepFn lookup_f32_sphere_nvt(int pk, int kc, int kl);
-SweepFn lookup_f32_cubic_nvt(int pk, int kc, int kl);
-SweepFn lookup_f32_cubic_npt(int pk, int kc, int kl);
+// Workgroup widths the kernels are built for (lanes = threads per workgroup).
+constexpr int LANES_SMALL = 256;
+constexpr int LANES_LARGE = 1024;
+
+// Returns nullptr if the combination is not built.
+SweepFn lookup_f64_sphere_nvt(int pk, int kc, int kl, int lanes);
+SweepFn lookup_f64_cubic_nvt(int pk, int kc, int kl, int lanes);
+SweepFn lookup_f64_cubic_npt(int pk, int kc, int kl, int lanes);
+SweepFn lookup_f32_sphere_nvt(int pk, int kc, int kl, int lanes);
+SweepFn lookup_f32_cubic_nvt(int pk, int kc, int kl, int lanes);
+SweepFn lookup_f32_cubic_npt(int pk, int kc, int kl, int lanes);
 
 }  // namespace ptmc
 
-// Instantiation helper used by the ptmc_inst_*.hip files.
-#define PTMC_KL_SWITCH(REAL, PK, KC, BC, ENS, kl)                                 \
-    switch (kl) {                                                                 \
-    case 1: return &sweep_kernel<REAL, PK, KC, BC, ENS, 1>;                       \
-    case 2: return &sweep_kernel<REAL, PK, KC, BC, ENS, 2>;                       \
-    case 4: return &sweep_kernel<REAL, PK, KC, BC, ENS, 4>;                       \
-    case 8: return &sweep_kernel<REAL, PK, KC, BC, ENS, 8>;                       \
-    case 16: return &sweep_kernel<REAL, PK, KC, BC, ENS, 16>;                     \
-    default: return nullptr;                                                      \
+namespace ptmc {
+// 1024-lane workgroups only where the kernel fits 128 VGPRs without spilling
+// (binary32, spherical container, NVT).
+template <typename Real, int PK, int KC, int BC, int ENS>
+SweepFn kl_switch(int kl, int lanes)
+{
+    if (lanes == 256) {
+        switch (kl) {
+        case 1: return &sweep_kernel<Real, PK, KC, BC, ENS, 1, 256>;
+        case 2: return &sweep_kernel<Real, PK, KC, BC, ENS, 2, 256>;
+        case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 256>;
+        case 8: return &sweep_kernel<Real, PK, KC, BC, ENS, 8, 256>;
+        case 16: return &sweep_kernel<Real, PK, KC, BC, ENS, 16, 256>;
+        default: return nullptr;
+        }
+    }
+    if constexpr (sizeof(Real) == 4 && BC == BC_SPHERE && ENS == ENS_NVT) {
+        if (lanes == 1024) {
+            switch (kl) {
+            case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 1024>;
+            case 8: return &sweep_kernel<Real, PK, KC, BC, ENS, 8, 1024>;
+            case 16: return &sweep_kernel<Real, PK, KC, BC, ENS, 16, 1024>;
+            default: return nullptr;
+            }
+        }
+    }
+    return nullptr;
+}
+
+// reference pair forms (parity runs): one to four lanes, 256-lane workgroups
+template <typename Real, int PK, int KC, int BC, int ENS>
+SweepFn kl_switch_small(int kl, int lanes)
+{
+    if (lanes != 256) return nullptr;
+    switch (kl) {
+    case 1: return &sweep_kernel<Real, PK, KC, BC, ENS, 1, 256>;
+    case 2: return &sweep_kernel<Real, PK, KC, BC, ENS, 2, 256>;
+    case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 256>;
+    default: return nullptr;
+    }
+}
+}  // namespace ptmc
+
+#define PTMC_KL_SWITCH(REAL, PK, KC, BC, ENS, kl, lanes) return kl_switch<REAL, PK, KC, BC, ENS>(kl, lanes);
+#define PTMC_KL_SWITCH_SMALL(REAL, PK, KC, BC, ENS, kl, lanes) return kl_switch_small<REAL, PK, KC, BC, ENS>(kl, lanes);
+
+#define PTMC_DEFINE_LOOKUP_F64(NAME, BC, ENS)                                                        \
+    namespace ptmc {                                                                                 \
+    SweepFn NAME(int pk, int kc, int kl, int lanes)                                                  \
+    {                                                                                                \
+        if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(double, PK_EVEN_H, 6, BC, ENS, kl, lanes) }   \
+        if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(double, PK_EVEN_H, 8, BC, ENS, kl, lanes) }   \
+        if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(double, PK_ODD_H, 8, BC, ENS, kl, lanes) }     \
+        if (pk == PK_EVEN_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_EVEN_R, 8, BC, ENS, kl, lanes) } \
+        if (pk == PK_ODD_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_ODD_R, 8, BC, ENS, kl, lanes) }   \
+        return nullptr;                                                                              \
+    }                                                                                                \
     }
 
-#define PTMC_KL_SWITCH_SMALL(REAL, PK, KC, BC, ENS, kl)                           \
-    switch (kl) {                                                                 \
-    case 1: return &sweep_kernel<REAL, PK, KC, BC, ENS, 1>;                       \
-    case 2: return &sweep_kernel<REAL, PK, KC, BC, ENS, 2>;                       \
-    case 4: return &sweep_kernel<REAL, PK, KC, BC, ENS, 4>;                       \
-    default: return nullptr;                                                      \
-    }
-
-#define PTMC_DEFINE_LOOKUP_F64(NAME, BC, ENS)                                     \
-    namespace ptmc {                                                              \
-    SweepFn NAME(int pk, int kc, int kl)                                          \
-    {                                                                             \
-        if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(double, PK_EVEN_H, 6, BC, ENS, kl) } \
-        if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(double, PK_EVEN_H, 8, BC, ENS, kl) } \
-        if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(double, PK_ODD_H, 8, BC, ENS, kl) }   \
-        if (pk == PK_EVEN_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_EVEN_R, 8, BC, ENS, kl) } \
-        if (pk == PK_ODD_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_ODD_R, 8, BC, ENS, kl) }   \
-        return nullptr;                                                           \
-    }                                                                             \
-    }
-
-#define PTMC_DEFINE_LOOKUP_F32(NAME, BC, ENS)                                     \
-    namespace ptmc {                                                              \
-    SweepFn NAME(int pk, int kc, int kl)                                          \
-    {                                                                             \
-        if (pk == PK_LJ && kc == 4) { PTMC_KL_SWITCH(float, PK_LJ, 4, BC, ENS, kl) }         \
-        if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(float, PK_EVEN_H, 6, BC, ENS, kl) } \
-        if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(float, PK_EVEN_H, 8, BC, ENS, kl) } \
-        if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(float, PK_ODD_H, 8, BC, ENS, kl) }   \
-        return nullptr;                                                           \
-    }                                                                             \
+#define PTMC_DEFINE_LOOKUP_F32(NAME, BC, ENS)                                                        \
+    namespace ptmc {                                                                                 \
+    SweepFn NAME(int pk, int kc, int kl, int lanes)                                                  \
+    {                                                                                                \
+        if (pk == PK_LJ && kc == 4) { PTMC_KL_SWITCH(float, PK_LJ, 4, BC, ENS, kl, lanes) }            \
+        if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(float, PK_EVEN_H, 6, BC, ENS, kl, lanes) }    \
+        if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(float, PK_EVEN_H, 8, BC, ENS, kl, lanes) }    \
+        if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(float, PK_ODD_H, 8, BC, ENS, kl, lanes) }      \
+        return nullptr;                                                                              \
+    }                                                                                                \
     }

@@ -88,6 +88,22 @@ __device__ __forceinline__ Real pair_energy(Real r2, const Real (&c)[KC])
     }
 }
 
+// Pair energy from u = 1/r^2 (binary32 even-power forms).
+template <typename Real, int PK, int KC>
+__device__ __forceinline__ Real pair_energy_u(Real u, const Real (&c)[KC])
+{
+    if constexpr (PK == PK_LJ) {
+        Real u3 = u * u * u;
+        return u3 * fma_<Real>(c[1], u3, c[0]);
+    } else {
+        Real q = c[KC - 1];
+#pragma unroll
+        for (int k = KC - 2; k >= 0; --k) q = fma_<Real>(q, u, c[k]);
+        Real u3 = u * u * u;
+        return q * u3;
+    }
+}
+
 // FP64 pair energy used by full recomputes and volume moves.
 template <int PK, int KC>
 __device__ __forceinline__ double pair_energy64(double r2, const double (&c)[KC])
@@ -161,42 +177,56 @@ __device__ __forceinline__ double lrc64(int n_atoms, double r_cut, const double 
     return e;
 }
 
+template <typename Real>
 struct MoveDraw {
     uint32_t index;
-    double ux, uy, uz;
-    double umet;
+    Real ux, uy, uz;
+    Real umet;
 };
 
-// Draws of one move attempt; the FP64 layout spends two Philox blocks per
-// attempt, the binary32 layout one (24-bit uniforms).
+// Decode the words of one move attempt.  FP64 spends two Philox blocks per
+// attempt (53-bit uniforms, the atom index assembled from their spare low
+// bits); binary32 one block (24-bit uniforms, the Metropolis uniform built
+// from the three spare low bytes).
 template <typename Real>
-__device__ __forceinline__ MoveDraw move_draw(uint64_t seed, uint32_t run, uint32_t temp, uint32_t sweep,
-                                              uint32_t m, uint32_t attempt, uint32_t nsteps)
+__device__ __forceinline__ MoveDraw<Real> decode_move(const u32x4 &A, const u32x4 &B, uint32_t nsteps)
 {
-    MoveDraw d;
+    MoveDraw<Real> d;
     if constexpr (sizeof(Real) == 8) {
-        u32x4 A = draw(seed, run, RNG_MOVE, 2u * attempt, temp, sweep, m);
-        u32x4 B = draw(seed, run, RNG_MOVE, 2u * attempt + 1u, temp, sweep, m);
         d.ux = u53(A.x, A.y);
         d.uy = u53(A.z, A.w);
         d.uz = u53(B.x, B.y);
         d.umet = u53(B.z, B.w);
-        uint32_t w = (A.y & 0x7ffu) | ((A.w & 0x7ffu) << 11) | ((B.y & 0x3ffu) << 22);
+        const uint32_t w = (A.y & 0x7ffu) | ((A.w & 0x7ffu) << 11) | ((B.y & 0x3ffu) << 22);
         d.index = below(w, nsteps);
     } else {
-        u32x4 C = draw(seed, run, RNG_MOVE, attempt, temp, sweep, m);
-        d.index = below(C.x, nsteps);
-        d.ux = (double)u24(C.y);
-        d.uy = (double)u24(C.z);
-        d.uz = (double)u24(C.w);
-        uint32_t w = ((C.y & 0xffu) << 16) | ((C.z & 0xffu) << 8) | (C.w & 0xffu);
-        d.umet = (double)((float)w * 0x1.0p-24f);
+        (void)B;
+        d.index = below(A.x, nsteps);
+        d.ux = u24(A.y);
+        d.uy = u24(A.z);
+        d.uz = u24(A.w);
+        const uint32_t w = ((A.y & 0xffu) << 16) | ((A.z & 0xffu) << 8) | (A.w & 0xffu);
+        d.umet = (float)w * 0x1.0p-24f;
     }
     return d;
 }
 
+template <typename Real>
+__device__ __forceinline__ MoveDraw<Real> move_draw(uint64_t seed, uint32_t run, uint32_t temp, uint32_t sweep,
+                                                    uint32_t m, uint32_t attempt, uint32_t nsteps)
+{
+    if constexpr (sizeof(Real) == 8) {
+        const u32x4 A = draw(seed, run, RNG_MOVE, 2u * attempt, temp, sweep, m);
+        const u32x4 B = draw(seed, run, RNG_MOVE, 2u * attempt + 1u, temp, sweep, m);
+        return decode_move<Real>(A, B, nsteps);
+    } else {
+        const u32x4 A = draw(seed, run, RNG_MOVE, attempt, temp, sweep, m);
+        return decode_move<Real>(A, A, nsteps);
+    }
+}
+
 template <typename Real, int BC, bool DIV>
-__device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const MoveDraw &d, Real delta,
+__device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const MoveDraw<Real> &d, Real delta,
                                            Real L, Real invL, Real &tx, Real &ty, Real &tz)
 {
     if constexpr (sizeof(Real) == 8) {
@@ -209,9 +239,9 @@ __device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const Move
             tz = tz - L * __builtin_rint(DIV ? tz / L : tz * invL);
         }
     } else {
-        tx = xi + ((float)d.ux - 0.5f) * delta;
-        ty = yi + ((float)d.uy - 0.5f) * delta;
-        tz = zi + ((float)d.uz - 0.5f) * delta;
+        tx = xi + (d.ux - 0.5f) * delta;
+        ty = yi + (d.uy - 0.5f) * delta;
+        tz = zi + (d.uz - 0.5f) * delta;
         if constexpr (BC == BC_CUBIC) {
             float q;
             q = __builtin_rintf(tx * invL);
@@ -225,37 +255,55 @@ __device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const Move
 }
 
 template <typename Real>
-__device__ __forceinline__ bool outside_sphere(Real x, Real y, Real z, double r2)
+__device__ __forceinline__ bool outside_sphere(Real x, Real y, Real z, Real r2)
 {
     if constexpr (sizeof(Real) == 8) {
         return x * x + y * y + z * z > r2;
     } else {
         float s = x * x + y * y;
         s = s + z * z;
-        return s > (float)r2;
+        return s > r2;
     }
 }
 
+// LDS / global layout of a tile: element (slab s, component c, lane) at
+// (s * 3 + c) * LANES + lane.  The compile-time lane stride lets one base
+// register and immediate offsets address x, y, z of consecutive slabs.
+template <int LANES>
+__device__ __forceinline__ int xoff(int s, int c, int lane) { return (s * 3 + c) * LANES + lane; }
+
+// Parking place of padding atoms (j >= N) and of the moved atom during the
+// partner loop in a spherical container: every pair energy with it is
+// exactly zero in both precisions (1/r^6 underflows), so no per-pair mask
+// is needed.
+template <typename Real>
+__device__ __forceinline__ Real far_point();
+template <>
+__device__ __forceinline__ float far_point<float>() { return 1.0e18f; }
+template <>
+__device__ __forceinline__ double far_point<double>() { return 1.0e100; }
+
 // Full FP64 energy of a slot's configuration (src/EnergyEvaluation.jl:378-420):
 // rows j of this lane, partners l > j, summed by the cluster butterfly.
-template <typename Real, int PK, int KC, int BC, int ENS, int KL>
-__device__ double full_energy64(const Real *X, int A, int WGS, int gb, int q, int N, double box, double s2,
-                                double rc, const double (&c64)[KC])
+template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
+__device__ double full_energy64(const Real *X, int gb, int q, int N, double box, double s2, double rc,
+                                const double (&c64)[KC])
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
     const double invL = BC == BC_CUBIC ? 1.0 / box : 0.0;
+    const int A = (N + KL - 1) / KL;
     double part = 0.0;
     for (int s = 0; s < A; ++s) {
         const int j = s * KL + q;
         if (j >= N) break;
-        const double xj = (double)X[(0 * A + s) * WGS + gb + q];
-        const double yj = (double)X[(1 * A + s) * WGS + gb + q];
-        const double zj = (double)X[(2 * A + s) * WGS + gb + q];
+        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)];
+        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)];
+        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)];
         for (int l = j + 1; l < N; ++l) {
             const int sl = l / KL, ll = gb + (l % KL);
-            const double xl = (double)X[(0 * A + sl) * WGS + ll];
-            const double yl = (double)X[(1 * A + sl) * WGS + ll];
-            const double zl = (double)X[(2 * A + sl) * WGS + ll];
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
             double r2 = dist2_64(xj, yj, zj, xl, yl, zl, BC, box, invL, DIV);
             r2 = r2 * s2;
             if (ENS == ENS_NPT) {
@@ -270,28 +318,52 @@ __device__ double full_energy64(const Real *X, int A, int WGS, int gb, int q, in
     return part;
 }
 
-template <typename Real, int PK, int KC, int BC, int ENS, int KL>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+// Philox words of move mm for this lane (block A; block B only in FP64).
+template <typename Real>
+__device__ __forceinline__ void move_words(uint64_t seed, uint32_t run, uint32_t temp, uint32_t sweep, uint32_t mm,
+                                           u32x4 &A, u32x4 &B)
+{
+    A = draw(seed, run, RNG_MOVE, 0u, temp, sweep, mm);
+    if constexpr (sizeof(Real) == 8) B = draw(seed, run, RNG_MOVE, 1u, temp, sweep, mm);
+}
+
+template <typename Real>
+__device__ __forceinline__ void fetch_words(const u32x4 &mA, const u32x4 &mB, int src, u32x4 &A, u32x4 &B)
+{
+    A.x = lane_get(mA.x, src);
+    A.y = lane_get(mA.y, src);
+    A.z = lane_get(mA.z, src);
+    A.w = lane_get(mA.w, src);
+    if constexpr (sizeof(Real) == 8) {
+        B.x = lane_get(mB.x, src);
+        B.y = lane_get(mB.y, src);
+        B.z = lane_get(mB.z, src);
+        B.w = lane_get(mB.w, src);
+    }
+}
+
+template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
+__global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
+    constexpr bool F32 = sizeof(Real) == 4;
+    constexpr int NSLOT = LANES / KL;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const TileInfo tile = a.tiles[blockIdx.x];
-    const int WGS = blockDim.x;
     const int tid = threadIdx.x;
     const int lg = tid / KL;
     const int q = tid % KL;
     const int gb = lg * KL;
-    const int nslots_max = WGS / KL;
     const bool active = lg < tile.n_slots;
     const int N = tile.n_atoms;
     const int A = tile.n_slab;
     const uint32_t nsteps = (uint32_t)(ENS == ENS_NPT ? N + 1 : N);
 
     Real *X = reinterpret_cast<Real *>(lds_raw);
-    double *exE = reinterpret_cast<double *>(lds_raw + sizeof(Real) * 3 * A * WGS);
-    double *exV = exE + nslots_max;
-    int *exT = reinterpret_cast<int *>(exV + nslots_max);
-    int *exS = exT + nslots_max;
+    double *exE = reinterpret_cast<double *>(lds_raw + sizeof(Real) * 3 * A * LANES);
+    double *exV = exE + NSLOT;
+    int *exT = reinterpret_cast<int *>(exV + NSLOT);
+    int *exS = exT + NSLOT;
 
     // coefficients: Real copy for the move path, FP64 copy for totals
     Real c[KC];
@@ -308,38 +380,32 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     }
 
     Real *gX = reinterpret_cast<Real *>(a.coords) + tile.coord_off;
-    for (int e = tid; e < 3 * A * WGS; e += WGS) X[e] = gX[e];
+    for (int e = tid; e < 3 * A * LANES; e += LANES) X[e] = gX[e];
 
     const int slot = tile.slot0 + lg;
-    int run = 0, t = 0, R = 1, tb = 0, lb = 0;
+    int t = 0, R = 1, tb = 0, lb = 0;
     uint32_t run_key = 0;
-    double E = 0.0, box = 1.0, radius2 = 0.0, pressure = 0.0;
-    double en_min = 0.0, den = 1.0, v_min = 0.0, dvh = 1.0;
+    double E = 0.0, box = 1.0;
+    Real radius2 = 0;  // container radius^2 in the move precision
     if (active) {
-        run = a.slot_run[slot];
-        const RunInfo ri = a.runs[run];
+        const RunInfo &ri = a.runs[a.slot_run[slot]];
         R = ri.n_temps;
         tb = ri.temp_base;
         lb = ri.local_base;
         run_key = ri.run_key;
-        radius2 = ri.radius2;
-        pressure = ri.pressure;
-        en_min = ri.en_min;
-        den = ri.delta_en;
-        v_min = ri.v_min;
-        dvh = ri.delta_v;
+        radius2 = (Real)ri.radius2;
         t = a.slot_temp[slot];
         E = a.slot_en[slot];
-        box = a.slot_box[slot];
+        if constexpr (BC == BC_CUBIC) box = a.slot_box[slot];
         if (q == 0) {
             exT[lg] = t;
             exS[lb + t] = lg;
         }
     }
     __syncthreads();
-    if (a.init_energy && active && (a.init_run < 0 || run == a.init_run)) {
+    if (a.init_energy && active && (a.init_run < 0 || a.slot_run[slot] == a.init_run)) {
         // initialise_energy, src/EnergyEvaluation.jl:1662-1692
-        E = full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, 1.0, box * box / 4.0, c64);
+        E = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, 1.0, box * box / 4.0, c64);
     }
 
     for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
@@ -348,124 +414,184 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
         if (active) {
             const double beta = a.t_beta[tb + t];
             const double delta = a.t_displ[2 * (tb + t)];
-            const double delta_v = a.t_displ[2 * (tb + t) + 1];
             const Real delta_r = (Real)delta;
+            const float beta_l2e = (float)beta * 1.4426950408889634f;
             Real Lr = (Real)box;
             Real invLr = (Real)(BC == BC_CUBIC ? 1.0 / box : 0.0);
             Real rc = (Real)(box * box / 4.0);
             int nacc = 0, nvacc = 0;
+            // Shared Philox words: lane q of the cluster computes the block(s)
+            // of move m0 + q; the words of the next move are fetched while
+            // the current move runs.
+            u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
+            u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
+            if constexpr (KL > 1) {
+                if ((uint32_t)q < nsteps) move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                fetch_words<Real>(mA, mB, gb, nA, nB);
+            }
             for (uint32_t m = 0; m < nsteps; ++m) {
-                MoveDraw d0 = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, 0u, nsteps);
+                // ---- draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
+                MoveDraw<Real> d0;
+                if constexpr (KL == 1) {
+                    d0 = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, 0u, nsteps);
+                } else {
+                    d0 = decode_move<Real>(nA, nB, nsteps);
+                    const uint32_t m1 = m + 1;
+                    if (m1 < nsteps) {
+                        const uint32_t k1 = m1 & (KL - 1);
+                        if (k1 == 0 && m1 + q < nsteps)
+                            move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m1 + q, mA, mB);
+                        fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
+                    }
+                }
                 const int i = (int)d0.index;
                 bool accept = false;
                 double dE_used = 0.0;
                 if (i < N) {
                     const int si = i / KL, li = gb + (i % KL);
-                    Real xi = X[(0 * A + si) * WGS + li];
-                    Real yi = X[(1 * A + si) * WGS + li];
-                    Real zi = X[(2 * A + si) * WGS + li];
+                    Real xi = X[xoff<LANES>(si, 0, li)];
+                    Real yi = X[xoff<LANES>(si, 1, li)];
+                    Real zi = X[xoff<LANES>(si, 2, li)];
                     Real tx, ty, tz;
                     make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Lr, invLr, tx, ty, tz);
                     if constexpr (BC == BC_SPHERE) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
-                        int count = 0;
-                        uint32_t attempt = 0;
-                        while (outside_sphere<Real>(tx, ty, tz, radius2)) {
-                            count += 1;
-                            if (count == 50) {
-                                double sx = 0.0, sy = 0.0, sz = 0.0;
-                                for (int s = 0; s < A; ++s) {
-                                    if (s * KL + q >= N) break;
-                                    sx += (double)X[(0 * A + s) * WGS + gb + q];
-                                    sy += (double)X[(1 * A + s) * WGS + gb + q];
-                                    sz += (double)X[(2 * A + s) * WGS + gb + q];
-                                }
-                                sx = group_sum<KL>(sx) / (double)N;
-                                sy = group_sum<KL>(sy) / (double)N;
-                                sz = group_sum<KL>(sz) / (double)N;
-                                for (int s = 0; s < A; ++s) {
-                                    if (s * KL + q >= N) break;
-                                    Real *px = &X[(0 * A + s) * WGS + gb + q];
-                                    Real *py = &X[(1 * A + s) * WGS + gb + q];
-                                    Real *pz = &X[(2 * A + s) * WGS + gb + q];
-                                    *px = (Real)((double)*px - sx);
-                                    *py = (Real)((double)*py - sy);
-                                    *pz = (Real)((double)*pz - sz);
-                                }
-                                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                            } else {
-                                attempt += 1;
-                                MoveDraw dd = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, attempt, nsteps);
-                                xi = X[(0 * A + si) * WGS + li];
-                                yi = X[(1 * A + si) * WGS + li];
-                                zi = X[(2 * A + si) * WGS + li];
-                                make_trial<Real, BC, DIV>(xi, yi, zi, dd, delta_r, Lr, invLr, tx, ty, tz);
-                                if (count == 100) {
-                                    if (q == 0) a.run_error[run] = 1;
-                                    break;
+                        if (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                            int count = 0;
+                            uint32_t attempt = 0;
+                            while (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                                count += 1;
+                                if (count == 50) {
+                                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                                    for (int s = 0; s < A; ++s) {
+                                        if (s * KL + q >= N) break;
+                                        sx += (double)X[xoff<LANES>(s, 0, tid)];
+                                        sy += (double)X[xoff<LANES>(s, 1, tid)];
+                                        sz += (double)X[xoff<LANES>(s, 2, tid)];
+                                    }
+                                    sx = group_sum<KL>(sx) / (double)N;
+                                    sy = group_sum<KL>(sy) / (double)N;
+                                    sz = group_sum<KL>(sz) / (double)N;
+                                    for (int s = 0; s < A; ++s) {
+                                        if (s * KL + q >= N) break;
+                                        X[xoff<LANES>(s, 0, tid)] = (Real)((double)X[xoff<LANES>(s, 0, tid)] - sx);
+                                        X[xoff<LANES>(s, 1, tid)] = (Real)((double)X[xoff<LANES>(s, 1, tid)] - sy);
+                                        X[xoff<LANES>(s, 2, tid)] = (Real)((double)X[xoff<LANES>(s, 2, tid)] - sz);
+                                    }
+                                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                                } else {
+                                    attempt += 1;
+                                    MoveDraw<Real> dd =
+                                        move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, attempt, nsteps);
+                                    xi = X[xoff<LANES>(si, 0, li)];
+                                    yi = X[xoff<LANES>(si, 1, li)];
+                                    zi = X[xoff<LANES>(si, 2, li)];
+                                    make_trial<Real, BC, DIV>(xi, yi, zi, dd, delta_r, Lr, invLr, tx, ty, tz);
+                                    if (count == 100) {
+                                        if (q == 0) a.run_error[a.slot_run[slot]] = 1;
+                                        break;
+                                    }
                                 }
                             }
+                            xi = X[xoff<LANES>(si, 0, li)];
+                            yi = X[xoff<LANES>(si, 1, li)];
+                            zi = X[xoff<LANES>(si, 2, li)];
                         }
-                        xi = X[(0 * A + si) * WGS + li];
-                        yi = X[(1 * A + si) * WGS + li];
-                        zi = X[(2 * A + si) * WGS + li];
-                    }
-                    // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
-                    Real sn = 0, so = 0;
-#pragma unroll 2
-                    for (int s = 0; s < A; ++s) {
-                        const int j = s * KL + q;
-                        const Real xj = X[(0 * A + s) * WGS + tid];
-                        const Real yj = X[(1 * A + s) * WGS + tid];
-                        const Real zj = X[(2 * A + s) * WGS + tid];
-                        const Real r2n = dist2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, Lr, invLr);
-                        const Real r2o = dist2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, Lr, invLr);
-                        Real en = pair_energy<Real, PK, KC>(r2n, c);
-                        Real eo = pair_energy<Real, PK, KC>(r2o, c);
-                        bool ok = (j < N) & (j != i);
-                        bool okn = ok, oko = ok;
-                        if constexpr (ENS == ENS_NPT) {
-                            okn = ok & (r2n <= rc);
-                            oko = ok & (r2o <= rc);
-                        }
-                        sn += okn ? en : Real(0);
-                        so += oko ? eo : Real(0);
-                    }
-                    sn = group_sum<KL>(sn);
-                    so = group_sum<KL>(so);
-                    const Real delta_en = sn - so;
-                    const double new_en = (double)delta_en + E;
-                    const double dd = new_en - E;
-                    dE_used = dd;
-                    // Metropolis, src/Exchange.jl:53-57 and src/MCRun.jl:122-132
-                    if constexpr (sizeof(Real) == 8) {
-                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
-                    } else {
-                        float pr = __builtin_amdgcn_exp2f((float)(-dd * beta) * 1.4426950408889634f);
-                        accept = (dd <= 0.0) || (pr >= (float)d0.umet);
-                    }
-                    if (accept) {
+                        // park the moved atom so its self pair contributes zero
                         if (tid == li) {
-                            X[(0 * A + si) * WGS + li] = tx;
-                            X[(1 * A + si) * WGS + li] = ty;
-                            X[(2 * A + si) * WGS + li] = tz;
+                            X[xoff<LANES>(si, 0, li)] = far_point<Real>();
+                            X[xoff<LANES>(si, 1, li)] = far_point<Real>();
+                            X[xoff<LANES>(si, 2, li)] = far_point<Real>();
                         }
                         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                        E = new_en;
-                        nacc += 1;
                     }
+                    // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
+                    Real sn = 0, so = 0;  // FP64: the two sums; binary32: sum of differences in sn
+                    const Real *Xl = X + tid;
+                    constexpr int UNROLL = LANES > 256 ? 2 : 4;
+#pragma unroll UNROLL
+                    for (int s = 0; s < A; ++s) {
+                        const Real xj = Xl[xoff<LANES>(s, 0, 0)];
+                        const Real yj = Xl[xoff<LANES>(s, 1, 0)];
+                        const Real zj = Xl[xoff<LANES>(s, 2, 0)];
+                        const Real r2n = dist2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, Lr, invLr);
+                        const Real r2o = dist2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, Lr, invLr);
+                        Real en, eo;
+                        if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
+                            // one reciprocal serves the new and the old pair
+                            const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                            en = pair_energy_u<Real, PK, KC>(ip * r2o, c);
+                            eo = pair_energy_u<Real, PK, KC>(ip * r2n, c);
+                        } else {
+                            en = pair_energy<Real, PK, KC>(r2n, c);
+                            eo = pair_energy<Real, PK, KC>(r2o, c);
+                        }
+                        if constexpr (BC == BC_SPHERE) {
+                            if constexpr (F32) sn += en - eo;
+                            else {
+                                sn += en;
+                                so += eo;
+                            }
+                        } else {
+                            const int j = s * KL + q;
+                            bool ok = (j < N) & (j != i);
+                            bool okn = ok, oko = ok;
+                            if constexpr (ENS == ENS_NPT) {
+                                okn = ok & (r2n <= rc);
+                                oko = ok & (r2o <= rc);
+                            }
+                            if constexpr (F32) {
+                                sn += (okn ? en : Real(0)) - (oko ? eo : Real(0));
+                            } else {
+                                sn += okn ? en : Real(0);
+                                so += oko ? eo : Real(0);
+                            }
+                        }
+                    }
+                    if constexpr (F32) {
+                        // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
+                        const float dEf = group_sum<KL>((float)sn);
+                        dE_used = (double)dEf;
+                        const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
+                        accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                        if (accept) {
+                            E = E + (double)dEf;
+                            nacc += 1;
+                        }
+                    } else {
+                        sn = group_sum<KL>(sn);
+                        so = group_sum<KL>(so);
+                        const double delta_en = sn - so;
+                        const double new_en = delta_en + E;
+                        const double dd = new_en - E;
+                        dE_used = dd;
+                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                        if (accept) {
+                            E = new_en;
+                            nacc += 1;
+                        }
+                    }
+                    // commit (src/swap_config.jl:40-51) or restore the parked atom
+                    if (tid == li) {
+                        X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
+                        X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
+                        X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
+                    }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 } else {
                     // uniform volume move, src/MCMoves.jl:169-177, :277-287; energy
                     // src/EnergyEvaluation.jl:397-420; acceptance src/Exchange.jl:58-71
                     if constexpr (ENS == ENS_NPT) {
-                        const double y = (d0.ux - 0.5) * delta_v;
+                        const double delta_v = a.t_displ[2 * (tb + t) + 1];
+                        const double pressure = a.runs[a.slot_run[slot]].pressure;
+                        const double y = ((double)d0.ux - 0.5) * delta_v;
                         double scale = det_exp(y / 3.0);
                         if (box >= a.t_maxbox[tb + t] && scale > 1.0) scale = 1.0;
                         const double nbox = box * scale;
                         const double nrc = nbox * nbox / 4.0;
                         const double s2 = scale * scale;
                         const double new_en =
-                            full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, s2, nrc, c64);
+                            full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, s2, nrc, c64);
                         const double dd = new_en - E;
                         dE_used = dd;
                         const double vn = nbox * nbox * nbox, vo = box * box * box;
@@ -473,12 +599,12 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
                         const double arg = -dh * beta + (double)N * det_log(vn / vo);
                         double prob = det_exp(arg);
                         if (prob > 1.0) prob = 1.0;
-                        accept = prob >= d0.umet;
+                        accept = prob >= (double)d0.umet;
                         if (accept) {
                             for (int s = 0; s < A; ++s) {
-                                Real *px = &X[(0 * A + s) * WGS + tid];
-                                Real *py = &X[(1 * A + s) * WGS + tid];
-                                Real *pz = &X[(2 * A + s) * WGS + tid];
+                                Real *px = &X[xoff<LANES>(s, 0, tid)];
+                                Real *py = &X[xoff<LANES>(s, 1, tid)];
+                                Real *pz = &X[xoff<LANES>(s, 2, tid)];
                                 *px = (Real)((double)*px * scale);
                                 *py = (Real)((double)*py * scale);
                                 *pz = (Real)((double)*pz * scale);
@@ -511,7 +637,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
                     a.t_count_vol[2 * (tb + t) + 1] += nvacc;
                 }
                 exE[lg] = E;
-                exV[lg] = box * box * box;
+                if constexpr (ENS == ENS_NPT) exV[lg] = box * box * box;
             }
         }
         __syncthreads();
@@ -532,6 +658,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
                 const int sa = exS[lb + n], sb = exS[lb + n + 1];
                 double e1 = exE[sa], e2 = exE[sb];
                 if (a.ensemble == ENS_NPT) {
+                    const double pressure = a.runs[a.slot_run[slot]].pressure;
                     e1 = e1 + pressure * exV[sa];
                     e2 = e2 + pressure * exV[sb];
                 }
@@ -572,20 +699,21 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
                 }
                 // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
                 if (a.sampling && pidx % a.mc_sample == 0) {
+                    const RunInfo &ri = a.runs[a.slot_run[slot]];
                     if (ENS == ENS_NPT) {
                         const double v = box * box * box;
-                        const double h = E + pressure * v;
+                        const double h = E + ri.pressure * v;
                         a.t_ham[2 * T] += h;
                         a.t_ham[2 * T + 1] += h * h;
-                        const double ie = __builtin_floor((E - en_min) / den) + 1.0;
-                        const double iv = __builtin_floor((v - v_min) / dvh) + 1.0;
+                        const double ie = __builtin_floor((E - ri.en_min) / ri.delta_en) + 1.0;
+                        const double iv = __builtin_floor((v - ri.v_min) / ri.delta_v) + 1.0;
                         const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
                         const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
                         a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)] += 1u;
                     } else {
                         a.t_ham[2 * T] += E;
                         a.t_ham[2 * T + 1] += E * E;
-                        const double hi = (E - en_min) / den + 1.0;
+                        const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
                         const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
                         a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
                     }
@@ -593,8 +721,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
             }
             // periodic FP64 recompute bounding binary32 drift
             if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
-                const double ef = full_energy64<Real, PK, KC, BC, ENS, KL>(X, A, WGS, gb, q, N, box, 1.0,
-                                                                          box * box / 4.0, c64);
+                const double ef =
+                    full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, 1.0, box * box / 4.0, c64);
                 const double dr = __builtin_fabs(E - ef);
                 if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
                 E = ef;
@@ -603,11 +731,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     }
 
     __syncthreads();
-    for (int e = tid; e < 3 * A * WGS; e += WGS) gX[e] = X[e];
+    for (int e = tid; e < 3 * A * LANES; e += LANES) gX[e] = X[e];
     if (active && q == 0) {
         a.slot_temp[slot] = t;
         a.slot_en[slot] = E;
-        a.slot_box[slot] = box;
+        if constexpr (BC == BC_CUBIC) a.slot_box[slot] = box;
     }
 }
 

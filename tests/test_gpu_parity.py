@@ -110,6 +110,7 @@ def test_fp64_reference_form_matches_reference_arithmetic(oracle, ne13):
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
 def test_fp64_horner_bit_exact(oracle, ne13, lanes):
     eng, run, orc, _ = _ne13_pair(oracle, ne13, form="horner", lanes=lanes, exchange="even_odd")
+    assert eng.lanes_of(run) == lanes
     eng.set_trace(30)
     eng.start_phase(False)
     eng.cycles(30)
@@ -143,9 +144,11 @@ def test_fp64_cubic_bit_exact(oracle, ne13, ensemble):
                       exchange="even_odd", seed=99)
         run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=P_ATM if ensemble == "npt" else 0.0, run_key=3)
         eng.prepare()
+        used = eng.lanes_of(run)  # raised from 1 to 2 when a 256-lane FP64 slab does not fit LDS
+        assert used >= lanes
         orc = oracle.OracleRun(N, temp.beta_grid, x, pot=oracle.make_pot("elj_even", ne13["coeff"]), bc="cubic",
                                box=L, ensemble=ensemble, pressure=P_ATM if ensemble == "npt" else 0.0,
-                               mode=oracle.engine_mode(lanes), exchange_mode=2, n_adjust=25, seed=99, run_key=3)
+                               mode=oracle.engine_mode(used), exchange_mode=2, n_adjust=25, seed=99, run_key=3)
         assert np.array_equal(eng.energies(run), orc.en_tot)
         eng.start_phase(False)
         orc.start_phase(False)
