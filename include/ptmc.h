@@ -135,6 +135,20 @@ int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, in
 int ptmc_run_lanes(ptmc_handle h, int32_t run);
 int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset);
 
+/* Multihistogram heat capacity, multihistogram_NVT (src/multihist_NVT.jl:95-221),
+ * FP64 on the device for n_runs independent runs at once.  hist holds the
+ * sampled energy histograms (n_temps x (n_bin + 2): underflow, n_bin bins,
+ * overflow) of each run; beta_result / t_result the n_result-point output
+ * grid (the reference uses TempGrid{10 n_temps}); k_boltzmann the constant of
+ * the Cv formula (the reference uses 3.166811429e-6, src/multihist_NVT.jl:18).
+ * Outputs: cv (n_runs x n_result), free energies (n_runs x n_temps, may be
+ * NULL) and fixed-point iterations used (n_runs, may be NULL). */
+int ptmc_multihistogram_nvt(int32_t n_runs, int32_t n_temps, int32_t n_bin, const double *beta,
+                            const double *hist, const double *en_min, const double *en_max,
+                            int32_t n_result, const double *beta_result, const double *t_result,
+                            double k_boltzmann, double conv_threshold, int32_t max_iter, double *cv,
+                            double *free_energy, int32_t *iterations, int32_t device);
+
 #ifdef __cplusplus
 }
 #endif

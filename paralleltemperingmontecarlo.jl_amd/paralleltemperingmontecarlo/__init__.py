@@ -24,7 +24,9 @@ from .api import (  # noqa: F401
     check_e_bounds,
     kB,
     ptmc_run,
+    ptmc_run_batch,
 )
 from ._lib import PTMCError, load as load_library  # noqa: F401
+from .multihist import multihistogram_NVT, multihistogram_nvt_batch  # noqa: F401
 
 AtoBohr = 1.8897259886

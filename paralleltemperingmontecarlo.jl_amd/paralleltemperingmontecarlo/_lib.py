@@ -110,6 +110,12 @@ SIGNATURES = {
     "ptmc_kernel_time": [_h, _dp, _i64p, C.c_int32],
 }
 
+# stateless entry points (no handle)
+FREE_SIGNATURES = {
+    "ptmc_multihistogram_nvt": [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, C.c_int32, _dp, _dp,
+                                C.c_double, C.c_double, C.c_int32, _dp, _dp, _i32p, C.c_int32],
+}
+
 _LIB = None
 
 
@@ -125,13 +131,18 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "(python paralleltemperingmontecarlo.jl_amd/build.py)"
         )
     lib = C.CDLL(str(p))
-    for name, argtypes in SIGNATURES.items():
+    for name, argtypes in {**SIGNATURES, **FREE_SIGNATURES}.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = C.c_int
     if path is None:
         _LIB = lib
     return lib
+
+
+def check(rc: int, what: str):
+    if rc != PTMC_OK:
+        raise PTMCError(rc, what)
 
 
 def ptr(a: np.ndarray | None, ctype=C.c_double):

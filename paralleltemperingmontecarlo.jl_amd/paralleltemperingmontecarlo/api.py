@@ -544,3 +544,33 @@ def ptmc_run(mc_params: MCParams, temp: TempGrid, start_config, potential, ensem
         return finalise(eng, run, mc_params, temp, bounds, ensemble)
     finally:
         eng.close()
+
+
+def ptmc_run_batch(mc_params: MCParams, temps, start_configs, potential, ensemble, *,
+                   engine: EngineOptions | None = None, run_keys=None, lanes=None):
+    """Many independent ptmc_run! calls advanced together in one engine.
+
+    temps[r] and start_configs[r] (a Config or a list of Configs) describe run
+    r; all runs share mc_params, the potential and the ensemble kind.  Each run
+    gets its own Philox run word (run_keys[r], default r) and goes through the
+    same equilibration / histogram set-up / production / finalisation as
+    ptmc_run.  Returns a list of (mc_states, results)."""
+    opts = engine or EngineOptions()
+    n = len(temps)
+    bc0, _ = _start_positions(start_configs[0])
+    eng = PTEngine(potential, ensemble, bc0, mc_params, opts)
+    try:
+        for r in range(n):
+            bc, x0 = _start_positions(start_configs[r])
+            pressure = ensemble.pressure if isinstance(ensemble, NPT) else 0.0
+            eng.add_run(x0.shape[1], temps[r].beta_grid, x0, bc, pressure,
+                        lanes=0 if lanes is None else int(lanes[r] if hasattr(lanes, "__len__") else lanes),
+                        run_key=r if run_keys is None else int(run_keys[r]))
+        eng.prepare()
+        bounds = equilibrate(eng, list(range(n)), mc_params)
+        eng.start_phase(True)
+        eng.cycles(mc_params.mc_cycles)
+        eng.sync()
+        return [finalise(eng, r, mc_params, temps[r], bounds[r], ensemble) for r in range(n)]
+    finally:
+        eng.close()
