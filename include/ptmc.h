@@ -85,6 +85,10 @@ int ptmc_last_error(ptmc_handle h, char *buf, size_t len);
  * ELJPotentialEven / ELJPotential (:138-153, :891-908) and
  * EmbeddedAtomPotential (:1059-1073, coefficients as stored there). */
 int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, const double *coeff);
+/* EAM (NVT, spherical or cubic): E = sum_i ean phi_i - ecam sqrt(rho_i),
+ * phi_i = sum_j r_ij^-n, rho_i = sum_j r_ij^-m; the per-atom phi/rho stay
+ * resident next to the positions and are updated per move as
+ * energy_update! does (src/EnergyEvaluation.jl:1567-1591). */
 int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam);
 
 /* Runs.  Replaces initialisation (src/Initialization.jl:70-101) and

@@ -167,12 +167,14 @@ double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *
     return tot;
 }
 
-/* EAM components and energy, src/EnergyEvaluation.jl:1105-1162, 1800-1811. */
-static double eam_energy_from_comp(const orc_pot *pot, const double *comp, int N)
+/* EAM components and energy, src/EnergyEvaluation.jl:1105-1162, 1800-1811.
+ * k = 1: the sequential sum of the reference; k > 1: atoms dealt to lanes
+ * round-robin and combined by the butterfly (engine initialisation). */
+static double eam_energy_from_comp(const orc_pot *pot, const double *comp, int N, int k)
 {
-    double en = 0.0;
-    for (int i = 0; i < N; ++i) en += pot->eam_ean * comp[2 * i] - pot->eam_ecam * sqrt(comp[2 * i + 1]);
-    return en;
+    double part[64] = {0};
+    for (int i = 0; i < N; ++i) part[i % k] += pot->eam_ean * comp[2 * i] - pot->eam_ecam * sqrt(comp[2 * i + 1]);
+    return tree_sum(part, k);
 }
 
 static void eam_build(const orc_params *p, const orc_pot *pot, const double *x, double box,
@@ -210,7 +212,8 @@ int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s)
         }
         if (pot->kind == ORC_POT_EAM) {
             eam_build(p, pot, x, box, &s->comp[(size_t)r * N * 2]);
-            s->en_tot[r] = eam_energy_from_comp(pot, &s->comp[(size_t)r * N * 2], N);
+            s->en_tot[r] = eam_energy_from_comp(pot, &s->comp[(size_t)r * N * 2], N,
+                                                p->math_lib ? 1 : (p->lanes > 0 ? p->lanes : 1));
         } else {
             s->en_tot[r] = orc_energy_config(p, pot, x, box,
                                              p->ensemble == ORC_NPT ? s->r_cut[r] : 0.0, NULL);
@@ -349,7 +352,7 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
                 o->new_comp[2 * j + 1] += t2;
                 o->new_comp[2 * i + 1] += t2;
             }
-            o->new_en = eam_energy_from_comp(pot, o->new_comp, N);
+            o->new_en = eam_energy_from_comp(pot, o->new_comp, N, 1);
         } else {
             /* engine: per-lane partial sums of the j != i terms, butterfly,
              * then the moved atom's own term last */

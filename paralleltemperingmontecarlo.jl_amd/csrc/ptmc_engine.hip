@@ -73,6 +73,7 @@ struct ptmc_engine {
     int pk = PK_EVEN_H, kc = 8;
     int n_slots = 0, n_temps = 0;
     int64_t n_coord = 0;
+    int64_t n_comp = 0;  // EAM phi/rho elements
     int hist_stride = 0;
 
     // device state
@@ -80,7 +81,7 @@ struct ptmc_engine {
     DevBuf<RunInfo> d_runs;
     DevBuf<int32_t> d_slot_run, d_slot_temp, d_run_error;
     DevBuf<double> d_slot_en, d_slot_box, d_slot_drift;
-    DevBuf<unsigned char> d_coords;
+    DevBuf<unsigned char> d_coords, d_comps;
     DevBuf<double> d_beta, d_displ, d_maxbox, d_ham;
     DevBuf<long long> d_count_atom, d_count_vol, d_count_exc;
     DevBuf<uint32_t> d_hist;
@@ -146,8 +147,9 @@ int default_lanes(int n_atoms, int precision)
     return kl;
 }
 
-SweepFn lookup(int precision, int bc, int ens, int pk, int kc, int kl, int lanes)
+SweepFn lookup(int precision, int bc, int ens, int pot, int pk, int kc, int kl, int lanes)
 {
+    if (pot == POT_EAM) return ens == ENS_NVT ? lookup_eam(precision, bc, kl, lanes) : nullptr;
     if (precision == 0) {
         if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl, lanes);
@@ -160,10 +162,12 @@ SweepFn lookup(int precision, int bc, int ens, int pk, int kc, int kl, int lanes
     return nullptr;
 }
 
-size_t tile_lds(size_t rsize, int n_atoms, int kl, int lanes)
+// LDS of one tile: positions [3][A][lanes] (EAM adds two phi/rho buffers
+// [2][A][2][lanes]) and the per-slot exchange scratch.
+size_t tile_lds(size_t rsize, int n_atoms, int kl, int lanes, bool eam)
 {
     const int A = (n_atoms + kl - 1) / kl;
-    return rsize * 3 * A * lanes + (size_t)(lanes / kl) * (8 + 8 + 4 + 4);
+    return rsize * (eam ? 7 : 3) * A * lanes + (size_t)(lanes / kl) * (8 + 8 + 4 + 4);
 }
 
 int check_run(ptmc_engine *h, int32_t run)
@@ -180,6 +184,11 @@ int choose_pair(ptmc_engine *h)
 {
     const PotParams &p = h->pot;
     const int K = p.ncoeff;
+    if (p.kind == POT_EAM) {
+        if (h->cfg.ensemble != ENS_NVT) return fail(h, PTMC_E_UNSUPPORTED, "EAM runs in the NVT ensemble only");
+        if (h->cfg.pair_form == 1) return fail(h, PTMC_E_UNSUPPORTED, "EAM has no reference pair form");
+        return PTMC_OK;
+    }
     if (h->cfg.pair_form == 1) {
         if (h->cfg.precision != 0) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form requires FP64");
         if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form supports up to 8 coefficients");
@@ -208,6 +217,7 @@ int plan(ptmc_engine *h)
 {
     const int prec = h->cfg.precision;
     const size_t rsize = prec == 0 ? sizeof(double) : sizeof(float);
+    const bool eam = h->pot.kind == POT_EAM;
     int dev = 0;
     HIPCHK(h, hipGetDevice(&dev));
     int max_lds = 0;
@@ -225,10 +235,11 @@ int plan(ptmc_engine *h)
         // configuration when the slab does not fit LDS
         int kl = -1, lanes = 0;
         for (int w : {LANES_SMALL, LANES_LARGE}) {
+            if (eam && w != LANES_SMALL) continue;
             int k = req;
             if (w == LANES_LARGE && k < 2) k = 2;
-            while (k < 16 && tile_lds(rsize, N, k, w) > (size_t)max_lds) k *= 2;
-            if (R * k <= w && tile_lds(rsize, N, k, w) <= (size_t)max_lds) {
+            while (k < 16 && tile_lds(rsize, N, k, w, eam) > (size_t)max_lds) k *= 2;
+            if (R * k <= w && tile_lds(rsize, N, k, w, eam) <= (size_t)max_lds) {
                 kl = k;
                 lanes = w;
                 break;
@@ -245,7 +256,7 @@ int plan(ptmc_engine *h)
     h->tiles.clear();
     h->launches.clear();
     int slot = 0;
-    int64_t coord = 0;
+    int64_t coord = 0, comp = 0;
     // one launch per (lanes per configuration, workgroup width)
     std::map<std::pair<int, int>, std::vector<int>> tiles_by_kl;
     for (auto &g : groups) {
@@ -301,7 +312,7 @@ int plan(ptmc_engine *h)
         L.tile0 = (int)ordered.size();
         L.ntiles = (int)kv.second.size();
         L.wgs = kv.first.second;
-        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl, L.wgs);
+        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pot.kind, h->pk, h->kc, kl, L.wgs);
         if (!L.fn)
             return fail(h, PTMC_E_UNSUPPORTED, "no kernel for precision %d boundary %d ensemble %d pair %d/%d lanes %d",
                         prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
@@ -310,7 +321,9 @@ int plan(ptmc_engine *h)
             TileInfo ti = h->tiles[t];
             ti.coord_off = coord;
             coord += (int64_t)3 * ti.n_slab * L.wgs;
-            size_t lds = tile_lds(rsize, ti.n_atoms, kl, L.wgs);
+            ti.comp_off = comp;
+            if (eam) comp += (int64_t)2 * ti.n_slab * L.wgs;
+            size_t lds = tile_lds(rsize, ti.n_atoms, kl, L.wgs, eam);
             maxlds = std::max(maxlds, lds);
             old_to_new[t] = (int)ordered.size();
             ordered.push_back(ti);
@@ -335,6 +348,7 @@ int plan(ptmc_engine *h)
     for (auto &rh : h->runs) rh.tile = old_to_new[rh.tile];
     h->tiles = ordered;
     h->n_coord = coord;
+    h->n_comp = comp;
     // slot0 of reordered tiles keeps the values assigned above (slots stay with
     // their runs); coordinates are addressed by tile.coord_off.
     return PTMC_OK;
@@ -378,6 +392,7 @@ SweepArgs base_args(ptmc_engine *h)
     a.slot_box = h->d_slot_box.p;
     a.slot_drift = h->d_slot_drift.p;
     a.coords = h->d_coords.p;
+    a.comps = h->d_comps.p;
     a.t_beta = h->d_beta.p;
     a.t_displ = h->d_displ.p;
     a.t_maxbox = h->d_maxbox.p;
@@ -472,6 +487,7 @@ int ptmc_destroy(ptmc_handle h)
     h->d_slot_box.free_();
     h->d_slot_drift.free_();
     h->d_coords.free_();
+    h->d_comps.free_();
     h->d_beta.free_();
     h->d_displ.free_();
     h->d_maxbox.free_();
@@ -517,8 +533,18 @@ int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, 
 int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam)
 {
     if (!h) return PTMC_E_ARG;
-    (void)n; (void)m; (void)ean; (void)ecam;
-    return fail(h, PTMC_E_UNSUPPORTED, "EAM sweeps are provided by ptmc_eam_* entry points");
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    if (!(n > 0) || !(m > 0) || !std::isfinite(ean) || !std::isfinite(ecam))
+        return fail(h, PTMC_E_ARG, "bad EAM parameters");
+    PotParams p{};
+    p.kind = POT_EAM;
+    p.eam_n = n;
+    p.eam_m = m;
+    p.eam_ean = ean;
+    p.eam_ecam = ecam;
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
 }
 
 int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *d, int32_t *run_index)
@@ -635,6 +661,7 @@ int ptmc_prepare(ptmc_handle h)
         memcpy(raw.data(), buf.data(), raw.size());
         if ((rc = upload(h, h->d_coords, raw))) return rc;
     }
+    if ((rc = dalloc(h, h->d_comps, (size_t)h->n_comp * (h->cfg.precision == 0 ? 8 : 4)))) return rc;
     h->prepared = true;
 
     // initial energies (initialise_energy) on the device in FP64

@@ -20,6 +20,8 @@ SweepFn lookup_f64_cubic_npt(int pk, int kc, int kl, int lanes);
 SweepFn lookup_f32_sphere_nvt(int pk, int kc, int kl, int lanes);
 SweepFn lookup_f32_cubic_nvt(int pk, int kc, int kl, int lanes);
 SweepFn lookup_f32_cubic_npt(int pk, int kc, int kl, int lanes);
+// Sutton-Chen EAM, NVT, 256-lane workgroups
+SweepFn lookup_eam(int precision, int bc, int kl, int lanes);
 
 }  // namespace ptmc
 

@@ -283,6 +283,58 @@ __device__ __forceinline__ float far_point<float>() { return 1.0e18f; }
 template <>
 __device__ __forceinline__ double far_point<double>() { return 1.0e100; }
 
+// Trial moves leaving the spherical container are redrawn (src/MCMoves.jl:115-135):
+// at the 50th failure the configuration is recentred (sequential mean,
+// src/Configurations.jl:79-85) without a new draw, the 100th failure is an
+// error.  Returns false on that error.  On return xi/yi/zi hold the (possibly
+// recentred) old position of the moved atom and tx/ty/tz the trial.
+template <typename Real, int KL, int LANES>
+__device__ bool sphere_redraw(Real *X, int tid, int q, int N, int A, int si, int li, Real radius2, Real delta_r,
+                              uint64_t seed, uint32_t run_key, uint32_t temp, uint32_t sweep, uint32_t m,
+                              uint32_t nsteps, Real &xi, Real &yi, Real &zi, Real &tx, Real &ty, Real &tz)
+{
+    int count = 0;
+    uint32_t attempt = 0;
+    bool ok = true;
+    while (outside_sphere<Real>(tx, ty, tz, radius2)) {
+        count += 1;
+        if (count == 50) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int s = 0; s < A; ++s) {
+                if (s * KL + q >= N) break;
+                sx += (double)X[xoff<LANES>(s, 0, tid)];
+                sy += (double)X[xoff<LANES>(s, 1, tid)];
+                sz += (double)X[xoff<LANES>(s, 2, tid)];
+            }
+            sx = group_sum<KL>(sx) / (double)N;
+            sy = group_sum<KL>(sy) / (double)N;
+            sz = group_sum<KL>(sz) / (double)N;
+            for (int s = 0; s < A; ++s) {
+                if (s * KL + q >= N) break;
+                X[xoff<LANES>(s, 0, tid)] = (Real)((double)X[xoff<LANES>(s, 0, tid)] - sx);
+                X[xoff<LANES>(s, 1, tid)] = (Real)((double)X[xoff<LANES>(s, 1, tid)] - sy);
+                X[xoff<LANES>(s, 2, tid)] = (Real)((double)X[xoff<LANES>(s, 2, tid)] - sz);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        } else {
+            attempt += 1;
+            MoveDraw<Real> dd = move_draw<Real>(seed, run_key, temp, sweep, m, attempt, nsteps);
+            xi = X[xoff<LANES>(si, 0, li)];
+            yi = X[xoff<LANES>(si, 1, li)];
+            zi = X[xoff<LANES>(si, 2, li)];
+            make_trial<Real, BC_SPHERE, false>(xi, yi, zi, dd, delta_r, Real(0), Real(0), tx, ty, tz);
+            if (count == 100) {
+                ok = false;
+                break;
+            }
+        }
+    }
+    xi = X[xoff<LANES>(si, 0, li)];
+    yi = X[xoff<LANES>(si, 1, li)];
+    zi = X[xoff<LANES>(si, 2, li)];
+    return ok;
+}
+
 // Full FP64 energy of a slot's configuration (src/EnergyEvaluation.jl:378-420):
 // rows j of this lane, partners l > j, summed by the cluster butterfly.
 template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
@@ -457,45 +509,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     if constexpr (BC == BC_SPHERE) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
                         if (outside_sphere<Real>(tx, ty, tz, radius2)) {
-                            int count = 0;
-                            uint32_t attempt = 0;
-                            while (outside_sphere<Real>(tx, ty, tz, radius2)) {
-                                count += 1;
-                                if (count == 50) {
-                                    double sx = 0.0, sy = 0.0, sz = 0.0;
-                                    for (int s = 0; s < A; ++s) {
-                                        if (s * KL + q >= N) break;
-                                        sx += (double)X[xoff<LANES>(s, 0, tid)];
-                                        sy += (double)X[xoff<LANES>(s, 1, tid)];
-                                        sz += (double)X[xoff<LANES>(s, 2, tid)];
-                                    }
-                                    sx = group_sum<KL>(sx) / (double)N;
-                                    sy = group_sum<KL>(sy) / (double)N;
-                                    sz = group_sum<KL>(sz) / (double)N;
-                                    for (int s = 0; s < A; ++s) {
-                                        if (s * KL + q >= N) break;
-                                        X[xoff<LANES>(s, 0, tid)] = (Real)((double)X[xoff<LANES>(s, 0, tid)] - sx);
-                                        X[xoff<LANES>(s, 1, tid)] = (Real)((double)X[xoff<LANES>(s, 1, tid)] - sy);
-                                        X[xoff<LANES>(s, 2, tid)] = (Real)((double)X[xoff<LANES>(s, 2, tid)] - sz);
-                                    }
-                                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                                } else {
-                                    attempt += 1;
-                                    MoveDraw<Real> dd =
-                                        move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, attempt, nsteps);
-                                    xi = X[xoff<LANES>(si, 0, li)];
-                                    yi = X[xoff<LANES>(si, 1, li)];
-                                    zi = X[xoff<LANES>(si, 2, li)];
-                                    make_trial<Real, BC, DIV>(xi, yi, zi, dd, delta_r, Lr, invLr, tx, ty, tz);
-                                    if (count == 100) {
-                                        if (q == 0) a.run_error[a.slot_run[slot]] = 1;
-                                        break;
-                                    }
-                                }
-                            }
-                            xi = X[xoff<LANES>(si, 0, li)];
-                            yi = X[xoff<LANES>(si, 1, li)];
-                            zi = X[xoff<LANES>(si, 2, li)];
+                            if (!sphere_redraw<Real, KL, LANES>(X, tid, q, N, A, si, li, radius2, delta_r, a.seed, run_key,
+                                                                (uint32_t)t, sweep, m, nsteps, xi, yi, zi, tx, ty, tz) &&
+                                q == 0)
+                                a.run_error[a.slot_run[slot]] = 1;
                         }
                         // park the moved atom so its self pair contributes zero
                         if (tid == li) {

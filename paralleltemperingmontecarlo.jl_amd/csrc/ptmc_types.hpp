@@ -49,6 +49,7 @@ struct TileInfo {
     int32_t n_atoms;
     int32_t n_slab;     // atoms per lane = ceil(N / KL)
     int64_t coord_off;  // element offset of the tile's [3][n_slab][WGS] block
+    int64_t comp_off;   // EAM: element offset of the tile's [n_slab][2][WGS] phi/rho block
 };
 
 struct SweepArgs {
@@ -60,6 +61,7 @@ struct SweepArgs {
     double *slot_box;          // cubic box length per slot
     double *slot_drift;        // max |E_running - E_recomputed| per slot
     void *coords;              // per-tile [3][n_slab][WGS] (Real)
+    void *comps;               // EAM: per-tile [n_slab][2][WGS] (Real)
     const double *t_beta;      // per temperature
     double *t_displ;           // per temperature x2 (atom, volume)
     const double *t_maxbox;    // per temperature
