@@ -71,6 +71,38 @@ def moves_per_cycle(runs):
     return sum(r["n_temps"] * r["n_atoms"] for r in runs)
 
 
+def kernel_symbol(kl, wgs):
+    """Demangled name of the sweep kernel instantiation a launch group runs
+    (binary32, Lennard-Jones pair form, spherical container, NVT)."""
+    return f"void ptmc::sweep_kernel<float, 4, 4, 0, 0, {kl}, {wgs}>(ptmc::SweepArgs)"
+
+
+def copy_probe(n_bytes, reps=5):
+    """A plain device copy of n_bytes, repeated: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (each dispatch reads and writes n_bytes)."""
+    import torch
+
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n_bytes, "dispatches": reps}))
+
+
+def traffic_per_launch(kernel, cycles_per_launch):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), if one was recorded for this kernel and launch length."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    for rec in json.loads(f.read_text()).get("kernels", []):
+        if rec.get("kernel") == kernel and rec.get("cycles_per_launch") == cycles_per_launch:
+            return rec.get("hbm_bytes_per_launch")
+    return None
+
+
 def cpu_baseline(runs, target_s=15.0):
     """The oracle's restatement of the reference arithmetic (FP64, division
     pair loop, cached r^2 rows, sequential sums), one thread per run like
@@ -114,6 +146,7 @@ def main():
     ap.add_argument("--cycles-per-launch", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs shard "
@@ -125,6 +158,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import paralleltemperingmontecarlo as pt
 
     runs = batched_lj_workload(args.runs)
@@ -159,7 +195,15 @@ def main():
     moves = moves_per_cycle(runs) * args.steps
     value = moves / dt
     flops = flops_per_cycle(runs) * args.steps
-    achieved = flops / (kms * 1e-3) / 1e12
+    aggregate = flops / (kms * 1e-3) / 1e12
+    # dominant kernel: the launch group with the largest summed launch time;
+    # its algorithmic flops over the timed region / its own launch time
+    groups = [eng.group_info(g) for g in range(plan["launch_groups"])]
+    g_dom = max(range(len(groups)), key=lambda g: groups[g]["ms"])
+    gi = groups[g_dom]
+    dom_runs = [r for i, r in enumerate(runs) if eng.group_of(i) == g_dom]
+    dom_flops = flops_per_cycle(dom_runs) * args.steps
+    achieved = dom_flops / (gi["ms"] * 1e-3) / 1e12
     drift = max(float(np.max(eng.drift(i))) for i in range(len(runs)))
     acc = []
     for i in range(0, len(runs), 37):
@@ -198,8 +242,15 @@ def main():
             "peak": PEAK_FP32,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "traffic": None,
-            "kernel_ms_per_step": kms / args.steps,
+            "traffic": traffic_per_launch(kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
+                                          args.cycles_per_launch),
+            "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
+            "kernel_flops_per_launch": dom_flops / max(gi["launches"], 1),
+            "kernel_avg_launch_ms": gi["ms"] / max(gi["launches"], 1),
+            "kernel_launches": gi["launches"],
+            "kernel_workgroups": gi["tiles"],
+            "aggregate_achieved": aggregate,
+            "step_kernel_ms": kms / args.steps,
             "launches": nlaunch,
         },
     }

@@ -138,6 +138,13 @@ int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t 
 int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, int64_t *n_lanes);
 int ptmc_run_lanes(ptmc_handle h, int32_t run);
 int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset);
+/* Launch groups (one kernel instantiation each; groups run concurrently):
+ * lanes per configuration, workgroup width, workgroups, and the summed
+ * duration of the group's own launches (events on the group's stream).
+ * ptmc_run_group returns the group a run was planned into (-1 on error). */
+int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, int32_t *workgroup_lanes,
+                    int32_t *n_tiles, double *total_ms, int64_t *n_launches);
+int ptmc_run_group(ptmc_handle h, int32_t run);
 
 /* Multihistogram heat capacity, multihistogram_NVT (src/multihist_NVT.jl:95-221),
  * FP64 on the device for n_runs independent runs at once.  hist holds the

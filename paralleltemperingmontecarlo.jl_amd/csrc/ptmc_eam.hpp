@@ -122,8 +122,6 @@ __global__ void __launch_bounds__(LANES) eam_kernel(SweepArgs a)
         }
     }
     __syncthreads();
-    if (a.init_energy && active && (a.init_run < 0 || a.slot_run[slot] == a.init_run))
-        E = eam_build<Real, BC, KL, LANES>(X, Cp, A, 0, gb, q, N, box, n64, m64, ean64, ecam64);
 
     for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
         const uint32_t sweep = (uint32_t)(a.sweep0 + cyc);
@@ -322,6 +320,38 @@ __global__ void __launch_bounds__(LANES) eam_kernel(SweepArgs a)
         a.slot_en[slot] = E;
         if constexpr (BC == BC_CUBIC) a.slot_box[slot] = box;
     }
+}
+
+
+// set_variables + initialise_energy for EAM (src/EnergyEvaluation.jl:1800-1811,
+// :1662-1692): phi/rho of every atom and the FP64 total of each slot.
+template <typename Real, int BC, int KL, int LANES>
+__global__ void __launch_bounds__(LANES) eam_init_kernel(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const TileInfo tile = a.tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int lg = tid / KL;
+    const int q = tid % KL;
+    const int N = tile.n_atoms;
+    const int A = tile.n_slab;
+    Real *X = reinterpret_cast<Real *>(lds_raw);
+    Real *Cp = X + 3 * A * LANES;
+    const Real *gX = reinterpret_cast<const Real *>(a.coords) + tile.coord_off;
+    Real *gC = reinterpret_cast<Real *>(a.comps) + tile.comp_off;
+    for (int e = tid; e < 3 * A * LANES; e += LANES) X[e] = gX[e];
+    __syncthreads();
+    if (lg >= tile.n_slots) return;  // whole clusters leave together
+    const int slot = tile.slot0 + lg;
+    if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;
+    const double box = BC == BC_CUBIC ? a.slot_box[slot] : 1.0;
+    const double E = eam_build<Real, BC, KL, LANES>(X, Cp, A, 0, lg * KL, q, N, box, a.pot.eam_n, a.pot.eam_m,
+                                                    a.pot.eam_ean, a.pot.eam_ecam);
+    for (int s = 0; s < A; ++s) {
+        gC[(s * 2 + 0) * LANES + tid] = Cp[coff<LANES>(A, 0, s, 0, tid)];
+        gC[(s * 2 + 1) * LANES + tid] = Cp[coff<LANES>(A, 0, s, 1, tid)];
+    }
+    if (q == 0) a.slot_en[slot] = E;
 }
 
 }  // namespace ptmc

@@ -31,10 +31,20 @@ struct RunHost {
 };
 
 struct Launch {
-    SweepFn fn = nullptr;
+    SweepFn fn = nullptr;    // cycle kernel
+    SweepFn init = nullptr;  // initial-energy kernel
     int tile0 = 0, ntiles = 0;
     int wgs = 256;
+    int kl = 1;
     size_t lds = 0;
+    double ms = 0.0;       // summed duration of this group's launches
+    long long n_done = 0;  // launches timed
+};
+
+// start/stop events around one launch, read after the stream is synchronised
+struct LaunchTiming {
+    int group;
+    hipEvent_t a, b;
 };
 
 template <typename T>
@@ -66,6 +76,9 @@ struct ptmc_engine {
     std::vector<hipEvent_t> join;
     double kernel_ms = 0.0;
     long long n_launch = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<LaunchTiming> pending;
 
     // plan
     std::vector<TileInfo> tiles;
@@ -147,9 +160,9 @@ int default_lanes(int n_atoms, int precision)
     return kl;
 }
 
-SweepFn lookup(int precision, int bc, int ens, int pot, int pk, int kc, int kl, int lanes)
+KernelPair lookup(int precision, int bc, int ens, int pot, int pk, int kc, int kl, int lanes)
 {
-    if (pot == POT_EAM) return ens == ENS_NVT ? lookup_eam(precision, bc, kl, lanes) : nullptr;
+    if (pot == POT_EAM) return ens == ENS_NVT ? lookup_eam(precision, bc, kl, lanes) : KernelPair{};
     if (precision == 0) {
         if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl, lanes);
@@ -159,7 +172,7 @@ SweepFn lookup(int precision, int bc, int ens, int pot, int pk, int kc, int kl, 
         if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f32_cubic_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f32_cubic_npt(pk, kc, kl, lanes);
     }
-    return nullptr;
+    return {};
 }
 
 // LDS of one tile: positions [3][A][lanes] (EAM adds two phi/rho buffers
@@ -312,8 +325,11 @@ int plan(ptmc_engine *h)
         L.tile0 = (int)ordered.size();
         L.ntiles = (int)kv.second.size();
         L.wgs = kv.first.second;
-        L.fn = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pot.kind, h->pk, h->kc, kl, L.wgs);
-        if (!L.fn)
+        L.kl = kl;
+        const KernelPair kp = lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pot.kind, h->pk, h->kc, kl, L.wgs);
+        L.fn = kp.sweep;
+        L.init = kp.init;
+        if (!L.fn || !L.init)
             return fail(h, PTMC_E_UNSUPPORTED, "no kernel for precision %d boundary %d ensemble %d pair %d/%d lanes %d",
                         prec, h->cfg.boundary, h->cfg.ensemble, h->pk, h->kc, kl);
         size_t maxlds = 0;
@@ -333,6 +349,7 @@ int plan(ptmc_engine *h)
                         maxlds, max_lds);
         L.lds = maxlds;
         HIPCHK(h, hipFuncSetAttribute((const void *)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+        HIPCHK(h, hipFuncSetAttribute((const void *)L.init, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
         h->launches.push_back(L);
     }
     // heaviest launch group first (per-cycle work of its largest tile ~ N * slab)
@@ -416,30 +433,67 @@ SweepArgs base_args(ptmc_engine *h)
     return a;
 }
 
-int launch_all(ptmc_engine *h, SweepArgs a)
+int timing_event(ptmc_engine *h, hipEvent_t *ev)
 {
-    const size_t nl = h->launches.size();
-    if (nl == 1) {
-        const Launch &L = h->launches[0];
-        SweepArgs b = a;
-        b.tiles = h->d_tiles.p + L.tile0;
-        void *args[] = {&b};
-        HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, h->stream));
-        h->n_launch += 1;
+    if (h->ev_used == h->ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(h, hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+    }
+    *ev = h->ev_pool[h->ev_used++];
+    return PTMC_OK;
+}
+
+// one launch of group k on stream st; cycle launches are bracketed by
+// timing events
+int launch_one(ptmc_engine *h, size_t k, const SweepArgs &a, hipStream_t st, bool init)
+{
+    const Launch &L = h->launches[k];
+    SweepArgs b = a;
+    b.tiles = h->d_tiles.p + L.tile0;
+    void *args[] = {&b};
+    if (init) {
+        HIPCHK(h, hipLaunchKernel((const void *)L.init, dim3(L.ntiles), dim3(L.wgs), args, L.lds, st));
         return PTMC_OK;
     }
+    LaunchTiming lt{(int)k, nullptr, nullptr};
+    int rc;
+    if ((rc = timing_event(h, &lt.a)) || (rc = timing_event(h, &lt.b))) return rc;
+    HIPCHK(h, hipEventRecord(lt.a, st));
+    HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, st));
+    HIPCHK(h, hipEventRecord(lt.b, st));
+    h->pending.push_back(lt);
+    h->n_launch += 1;
+    return PTMC_OK;
+}
+
+// after the engine stream has been synchronised: fold launch durations into
+// the per-group sums and recycle the events
+int collect_timings(ptmc_engine *h)
+{
+    for (const LaunchTiming &lt : h->pending) {
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, lt.a, lt.b));
+        h->launches[lt.group].ms += ms;
+        h->launches[lt.group].n_done += 1;
+    }
+    h->pending.clear();
+    h->ev_used = 0;
+    return PTMC_OK;
+}
+
+int launch_all(ptmc_engine *h, SweepArgs a, bool init = false)
+{
+    const size_t nl = h->launches.size();
+    if (nl == 1) return launch_one(h, 0, a, h->stream, init);
     // launch groups are independent: fork them onto side streams, join back
     HIPCHK(h, hipEventRecord(h->fork, h->stream));
     for (size_t k = 0; k < nl; ++k) {
-        const Launch &L = h->launches[k];
         hipStream_t st = h->side[k % h->side.size()];
         HIPCHK(h, hipStreamWaitEvent(st, h->fork, 0));
-        SweepArgs b = a;
-        b.tiles = h->d_tiles.p + L.tile0;
-        void *args[] = {&b};
-        HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, st));
+        int rc = launch_one(h, k, a, st, init);
+        if (rc) return rc;
         HIPCHK(h, hipEventRecord(h->join[k], st));
-        h->n_launch += 1;
     }
     for (size_t k = 0; k < nl; ++k) HIPCHK(h, hipStreamWaitEvent(h->stream, h->join[k], 0));
     return PTMC_OK;
@@ -501,6 +555,7 @@ int ptmc_destroy(ptmc_handle h)
     h->d_trace_idx.free_();
     for (auto st : h->side) (void)hipStreamSynchronize(st);
     for (auto ev : h->join) (void)hipEventDestroy(ev);
+    for (auto ev : h->ev_pool) (void)hipEventDestroy(ev);
     if (h->fork) (void)hipEventDestroy(h->fork);
     for (auto st : h->side) (void)hipStreamDestroy(st);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -666,10 +721,8 @@ int ptmc_prepare(ptmc_handle h)
 
     // initial energies (initialise_energy) on the device in FP64
     SweepArgs a = base_args(h);
-    a.n_cycles = 0;
-    a.init_energy = 1;
     a.init_run = -1;
-    if ((rc = launch_all(h, a))) return rc;
+    if ((rc = launch_all(h, a, true))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return PTMC_OK;
 }
@@ -710,6 +763,11 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
             h->trace_pending = 0;
         }
         if ((rc = launch_all(h, a))) return rc;
+        if (h->pending.size() >= 512) {
+            // bound the event pool on long calls
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            if ((rc = collect_timings(h))) return rc;
+        }
         h->sweep_counter += chunk;
         h->phase_counter += chunk;
         done += chunk;
@@ -719,7 +777,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->kernel_ms += ms;
-    return PTMC_OK;
+    return collect_timings(h);
 }
 
 int ptmc_sync(ptmc_handle h)
@@ -925,10 +983,8 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)ti.coord_off * rsize, blk.data(), blk.size(), hipMemcpyHostToDevice));
         }
         SweepArgs a = base_args(h);
-        a.n_cycles = 0;
-        a.init_energy = 1;
         a.init_run = run;
-        if ((rc = launch_all(h, a))) return rc;
+        if ((rc = launch_all(h, a, true))) return rc;
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     return PTMC_OK;
@@ -999,8 +1055,36 @@ int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32
     if (reset) {
         h->kernel_ms = 0.0;
         h->n_launch = 0;
+        for (auto &L : h->launches) {
+            L.ms = 0.0;
+            L.n_done = 0;
+        }
     }
     return PTMC_OK;
+}
+
+int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, int32_t *workgroup_lanes,
+                    int32_t *n_tiles, double *total_ms, int64_t *n_launches)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
+    if (group < 0 || group >= (int)h->launches.size()) return fail(h, PTMC_E_ARG, "launch group %d out of range", group);
+    const Launch &L = h->launches[group];
+    if (lanes_per_replica) *lanes_per_replica = L.kl;
+    if (workgroup_lanes) *workgroup_lanes = L.wgs;
+    if (n_tiles) *n_tiles = L.ntiles;
+    if (total_ms) *total_ms = L.ms;
+    if (n_launches) *n_launches = L.n_done;
+    return PTMC_OK;
+}
+
+int ptmc_run_group(ptmc_handle h, int32_t run)
+{
+    if (!h || run < 0 || run >= (int)h->runs.size() || !h->prepared) return -1;
+    const int t = h->runs[run].tile;
+    for (size_t k = 0; k < h->launches.size(); ++k)
+        if (t >= h->launches[k].tile0 && t < h->launches[k].tile0 + h->launches[k].ntiles) return (int)k;
+    return -1;
 }
 
 }  // extern "C"

@@ -4,22 +4,25 @@
 
 namespace ptmc {
 
+#define PTMC_EAM_PAIR(KL_) \
+    KernelPair { &eam_kernel<Real, BC, KL_, LANES_SMALL>, &eam_init_kernel<Real, BC, KL_, LANES_SMALL> }
+
 template <typename Real, int BC>
-static SweepFn eam_kl(int kl)
+static KernelPair eam_kl(int kl)
 {
     switch (kl) {
-    case 1: return &eam_kernel<Real, BC, 1, LANES_SMALL>;
-    case 2: return &eam_kernel<Real, BC, 2, LANES_SMALL>;
-    case 4: return &eam_kernel<Real, BC, 4, LANES_SMALL>;
-    case 8: return &eam_kernel<Real, BC, 8, LANES_SMALL>;
-    case 16: return &eam_kernel<Real, BC, 16, LANES_SMALL>;
-    default: return nullptr;
+    case 1: return PTMC_EAM_PAIR(1);
+    case 2: return PTMC_EAM_PAIR(2);
+    case 4: return PTMC_EAM_PAIR(4);
+    case 8: return PTMC_EAM_PAIR(8);
+    case 16: return PTMC_EAM_PAIR(16);
+    default: return {};
     }
 }
 
-SweepFn lookup_eam(int precision, int bc, int kl, int lanes)
+KernelPair lookup_eam(int precision, int bc, int kl, int lanes)
 {
-    if (lanes != LANES_SMALL) return nullptr;
+    if (lanes != LANES_SMALL) return {};
     if (precision == 0) return bc == BC_SPHERE ? eam_kl<double, BC_SPHERE>(kl) : eam_kl<double, BC_CUBIC>(kl);
     return bc == BC_SPHERE ? eam_kl<float, BC_SPHERE>(kl) : eam_kl<float, BC_CUBIC>(kl);
 }

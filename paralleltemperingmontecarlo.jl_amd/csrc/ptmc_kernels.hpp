@@ -9,61 +9,70 @@ namespace ptmc {
 
 using SweepFn = void (*)(SweepArgs);
 
+// The cycle kernel of a configuration and its initial-energy kernel.
+struct KernelPair {
+    SweepFn sweep = nullptr;
+    SweepFn init = nullptr;
+};
+
 // Workgroup widths the kernels are built for (lanes = threads per workgroup).
 constexpr int LANES_SMALL = 256;
 constexpr int LANES_LARGE = 1024;
 
-// Returns nullptr if the combination is not built.
-SweepFn lookup_f64_sphere_nvt(int pk, int kc, int kl, int lanes);
-SweepFn lookup_f64_cubic_nvt(int pk, int kc, int kl, int lanes);
-SweepFn lookup_f64_cubic_npt(int pk, int kc, int kl, int lanes);
-SweepFn lookup_f32_sphere_nvt(int pk, int kc, int kl, int lanes);
-SweepFn lookup_f32_cubic_nvt(int pk, int kc, int kl, int lanes);
-SweepFn lookup_f32_cubic_npt(int pk, int kc, int kl, int lanes);
+// Returns null kernels if the combination is not built.
+KernelPair lookup_f64_sphere_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f64_cubic_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f64_cubic_npt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f32_sphere_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f32_cubic_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f32_cubic_npt(int pk, int kc, int kl, int lanes);
 // Sutton-Chen EAM, NVT, 256-lane workgroups
-SweepFn lookup_eam(int precision, int bc, int kl, int lanes);
+KernelPair lookup_eam(int precision, int bc, int kl, int lanes);
 
 }  // namespace ptmc
 
 namespace ptmc {
 // 1024-lane workgroups only where the kernel fits 128 VGPRs without spilling
 // (binary32, spherical container, NVT).
+#define PTMC_PAIR(KL_, W_) \
+    KernelPair { &sweep_kernel<Real, PK, KC, BC, ENS, KL_, W_>, &init_kernel<Real, PK, KC, BC, ENS, KL_, W_> }
+
 template <typename Real, int PK, int KC, int BC, int ENS>
-SweepFn kl_switch(int kl, int lanes)
+KernelPair kl_switch(int kl, int lanes)
 {
     if (lanes == 256) {
         switch (kl) {
-        case 1: return &sweep_kernel<Real, PK, KC, BC, ENS, 1, 256>;
-        case 2: return &sweep_kernel<Real, PK, KC, BC, ENS, 2, 256>;
-        case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 256>;
-        case 8: return &sweep_kernel<Real, PK, KC, BC, ENS, 8, 256>;
-        case 16: return &sweep_kernel<Real, PK, KC, BC, ENS, 16, 256>;
-        default: return nullptr;
+        case 1: return PTMC_PAIR(1, 256);
+        case 2: return PTMC_PAIR(2, 256);
+        case 4: return PTMC_PAIR(4, 256);
+        case 8: return PTMC_PAIR(8, 256);
+        case 16: return PTMC_PAIR(16, 256);
+        default: return {};
         }
     }
     if constexpr (sizeof(Real) == 4 && BC == BC_SPHERE && ENS == ENS_NVT) {
         if (lanes == 1024) {
             switch (kl) {
-            case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 1024>;
-            case 8: return &sweep_kernel<Real, PK, KC, BC, ENS, 8, 1024>;
-            case 16: return &sweep_kernel<Real, PK, KC, BC, ENS, 16, 1024>;
-            default: return nullptr;
+            case 4: return PTMC_PAIR(4, 1024);
+            case 8: return PTMC_PAIR(8, 1024);
+            case 16: return PTMC_PAIR(16, 1024);
+            default: return {};
             }
         }
     }
-    return nullptr;
+    return {};
 }
 
 // reference pair forms (parity runs): one to four lanes, 256-lane workgroups
 template <typename Real, int PK, int KC, int BC, int ENS>
-SweepFn kl_switch_small(int kl, int lanes)
+KernelPair kl_switch_small(int kl, int lanes)
 {
-    if (lanes != 256) return nullptr;
+    if (lanes != 256) return {};
     switch (kl) {
-    case 1: return &sweep_kernel<Real, PK, KC, BC, ENS, 1, 256>;
-    case 2: return &sweep_kernel<Real, PK, KC, BC, ENS, 2, 256>;
-    case 4: return &sweep_kernel<Real, PK, KC, BC, ENS, 4, 256>;
-    default: return nullptr;
+    case 1: return PTMC_PAIR(1, 256);
+    case 2: return PTMC_PAIR(2, 256);
+    case 4: return PTMC_PAIR(4, 256);
+    default: return {};
     }
 }
 }  // namespace ptmc
@@ -73,25 +82,25 @@ SweepFn kl_switch_small(int kl, int lanes)
 
 #define PTMC_DEFINE_LOOKUP_F64(NAME, BC, ENS)                                                        \
     namespace ptmc {                                                                                 \
-    SweepFn NAME(int pk, int kc, int kl, int lanes)                                                  \
+    KernelPair NAME(int pk, int kc, int kl, int lanes)                                                  \
     {                                                                                                \
         if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(double, PK_EVEN_H, 6, BC, ENS, kl, lanes) }   \
         if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(double, PK_EVEN_H, 8, BC, ENS, kl, lanes) }   \
         if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(double, PK_ODD_H, 8, BC, ENS, kl, lanes) }     \
         if (pk == PK_EVEN_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_EVEN_R, 8, BC, ENS, kl, lanes) } \
         if (pk == PK_ODD_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_ODD_R, 8, BC, ENS, kl, lanes) }   \
-        return nullptr;                                                                              \
+        return {};                                                                                   \
     }                                                                                                \
     }
 
 #define PTMC_DEFINE_LOOKUP_F32(NAME, BC, ENS)                                                        \
     namespace ptmc {                                                                                 \
-    SweepFn NAME(int pk, int kc, int kl, int lanes)                                                  \
+    KernelPair NAME(int pk, int kc, int kl, int lanes)                                                  \
     {                                                                                                \
         if (pk == PK_LJ && kc == 4) { PTMC_KL_SWITCH(float, PK_LJ, 4, BC, ENS, kl, lanes) }            \
         if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(float, PK_EVEN_H, 6, BC, ENS, kl, lanes) }    \
         if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(float, PK_EVEN_H, 8, BC, ENS, kl, lanes) }    \
         if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(float, PK_ODD_H, 8, BC, ENS, kl, lanes) }      \
-        return nullptr;                                                                              \
+        return {};                                                                                   \
     }                                                                                                \
     }

@@ -455,10 +455,6 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         }
     }
     __syncthreads();
-    if (a.init_energy && active && (a.init_run < 0 || a.slot_run[slot] == a.init_run)) {
-        // initialise_energy, src/EnergyEvaluation.jl:1662-1692
-        E = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, 1.0, box * box / 4.0, c64);
-    }
 
     for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
         const uint32_t sweep = (uint32_t)(a.sweep0 + cyc);
@@ -754,6 +750,34 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         a.slot_en[slot] = E;
         if constexpr (BC == BC_CUBIC) a.slot_box[slot] = box;
     }
+}
+
+
+// initialise_energy (src/EnergyEvaluation.jl:1662-1692): FP64 total energy of
+// each slot of the tile from its positions (all runs, or the one init_run).
+template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
+__global__ void __launch_bounds__(LANES) init_kernel(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const TileInfo tile = a.tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int lg = tid / KL;
+    const int q = tid % KL;
+    const int N = tile.n_atoms;
+    const int A = tile.n_slab;
+    Real *X = reinterpret_cast<Real *>(lds_raw);
+    const Real *gX = reinterpret_cast<const Real *>(a.coords) + tile.coord_off;
+    for (int e = tid; e < 3 * A * LANES; e += LANES) X[e] = gX[e];
+    __syncthreads();
+    if (lg >= tile.n_slots) return;  // whole clusters leave together
+    const int slot = tile.slot0 + lg;
+    if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;
+    double c64[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+    const double box = BC == BC_CUBIC ? a.slot_box[slot] : 1.0;
+    const double E = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, lg * KL, q, N, box, 1.0, box * box / 4.0, c64);
+    if (q == 0) a.slot_en[slot] = E;
 }
 
 }  // namespace ptmc

@@ -88,8 +88,7 @@ struct SweepArgs {
     int32_t sampling;
     int32_t recompute_every;
     int32_t trace_cycles;
-    int32_t init_energy;       // compute slot energies from positions before cycling
-    int32_t init_run;          // run whose energies are initialised (-1: all)
+    int32_t init_run;          // init kernels: run whose energies are initialised (-1: all)
     double min_acc, max_acc;
 };
 

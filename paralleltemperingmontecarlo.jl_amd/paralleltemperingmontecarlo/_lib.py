@@ -108,6 +108,8 @@ SIGNATURES = {
     "ptmc_plan_info": [_h, _i32p, _i32p, _i64p],
     "ptmc_run_lanes": [_h, C.c_int32],
     "ptmc_kernel_time": [_h, _dp, _i64p, C.c_int32],
+    "ptmc_group_info": [_h, C.c_int32, _i32p, _i32p, _i32p, _dp, _i64p],
+    "ptmc_run_group": [_h, C.c_int32],
 }
 
 # stateless entry points (no handle)

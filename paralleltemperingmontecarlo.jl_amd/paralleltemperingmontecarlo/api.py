@@ -419,6 +419,16 @@ class PTEngine:
     def lanes_of(self, run: int) -> int:
         return self.eng.lib.ptmc_run_lanes(self.eng.h, run)
 
+    def group_of(self, run: int) -> int:
+        return self.eng.lib.ptmc_run_group(self.eng.h, run)
+
+    def group_info(self, group: int):
+        kl, wgs, nt, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        ms = C.c_double()
+        self.eng.call("ptmc_group_info", int(group), C.byref(kl), C.byref(wgs), C.byref(nt), C.byref(ms), C.byref(n))
+        return dict(lanes_per_replica=kl.value, workgroup_lanes=wgs.value, tiles=nt.value, ms=ms.value,
+                    launches=n.value)
+
     def close(self):
         self.eng.close()
 
