@@ -129,6 +129,15 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
                    const double *max_displ, const int64_t *count_atom, const int64_t *count_vol,
                    const int64_t *count_exc, const double *ham);
 
+/* Histogram counts of a run (restart: setresults, src/ReadSave.jl:385-417);
+ * n_temps * hist_stride non-negative integral values. */
+int ptmc_set_histograms(ptmc_handle h, int32_t run, const double *hist);
+/* Position in the random stream: the next cycle draws Philox counter word
+ * `cycle` and has phase-local index phase_cycle + 1 (n_adjust / mc_sample
+ * cadence).  A run resumed from a checkpoint continues the stream it would
+ * have drawn without the interruption. */
+int ptmc_seek(ptmc_handle h, int64_t cycle, int64_t phase_cycle);
+
 /* Per-move trace of the next n_cycles cycles (parity tests). */
 int ptmc_set_trace(ptmc_handle h, int32_t n_cycles);
 int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t *idx);

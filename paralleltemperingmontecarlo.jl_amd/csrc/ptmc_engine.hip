@@ -922,6 +922,35 @@ int ptmc_get_histograms(ptmc_handle h, int32_t run, double *hist)
     return PTMC_OK;
 }
 
+int ptmc_set_histograms(ptmc_handle h, int32_t run, const double *hist)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    if (!hist) return fail(h, PTMC_E_ARG, "null histogram buffer");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t n = (size_t)rh.d.n_temps * h->hist_stride;
+    std::vector<uint32_t> tmp(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!(hist[i] >= 0.0) || hist[i] > 4294967295.0 || hist[i] != std::floor(hist[i]))
+            return fail(h, PTMC_E_ARG, "histogram entry %zu is not a count", i);
+        tmp[i] = (uint32_t)hist[i];
+    }
+    HIPCHK(h, hipMemcpy(h->d_hist.p + (size_t)rh.info.temp_base * h->hist_stride, tmp.data(), n * 4,
+                        hipMemcpyHostToDevice));
+    return PTMC_OK;
+}
+
+int ptmc_seek(ptmc_handle h, int64_t cycle, int64_t phase_cycle)
+{
+    if (!h) return PTMC_E_ARG;
+    if (cycle < 0 || phase_cycle < 0) return fail(h, PTMC_E_ARG, "negative cycle index");
+    h->sweep_counter = cycle;
+    h->phase_counter = phase_cycle;
+    return PTMC_OK;
+}
+
 int ptmc_get_drift(ptmc_handle h, int32_t run, double *drift)
 {
     int rc = check_run(h, run);

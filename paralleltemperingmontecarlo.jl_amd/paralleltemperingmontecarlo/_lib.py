@@ -103,6 +103,8 @@ SIGNATURES = {
     "ptmc_hist_stride": [_h],
     "ptmc_get_drift": [_h, C.c_int32, _dp],
     "ptmc_set_state": [_h, C.c_int32, _dp, _dp, _dp, _i64p, _i64p, _i64p, _dp],
+    "ptmc_set_histograms": [_h, C.c_int32, _dp],
+    "ptmc_seek": [_h, C.c_int64, C.c_int64],
     "ptmc_set_trace": [_h, C.c_int32],
     "ptmc_get_trace": [_h, C.c_int32, _dp, _i8p, _i32p],
     "ptmc_plan_info": [_h, _i32p, _i32p, _i64p],

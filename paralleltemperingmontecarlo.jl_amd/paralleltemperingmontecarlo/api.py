@@ -384,6 +384,16 @@ class PTEngine:
     def reset_counters(self):
         self.eng.call("ptmc_reset_counters")
 
+    def set_histograms(self, run: int, hist):
+        h = np.ascontiguousarray(hist, dtype=np.float64)
+        stride = self.eng.lib.ptmc_hist_stride(self.eng.h)
+        if h.size != self.runs[run]["n_temps"] * stride:
+            raise ValueError(f"histograms need {self.runs[run]['n_temps']} x {stride} counts")
+        self.eng.call("ptmc_set_histograms", run, _lib.ptr(h))
+
+    def seek(self, cycle: int, phase_cycle: int):
+        self.eng.call("ptmc_seek", int(cycle), int(phase_cycle))
+
     def set_state(self, run: int, x=None, box=None, max_displ=None, count_atom=None, count_vol=None,
                   count_exc=None, ham=None):
         f = lambda a, t=np.float64: None if a is None else np.ascontiguousarray(a, dtype=t)
