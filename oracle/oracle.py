@@ -38,9 +38,20 @@ class orc_params(C.Structure):
     ]
 
 
+_i32p = C.POINTER(C.c_int32)
+
+
+class orc_nnp(C.Structure):
+    _fields_ = [("n_rad", C.c_int32), ("n_ang", C.c_int32), ("rad", C.POINTER(C.c_double)),
+                ("ang", C.POINTER(C.c_double)), ("r_cut", C.c_double), ("hardcore2", C.c_double),
+                ("q10_compat", C.c_int32), ("n_layers", C.c_int32), ("nodes", _i32p), ("act", _i32p),
+                ("n_params", C.c_int32), ("params", C.POINTER(C.c_double))]
+
+
 class orc_pot(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ncoeff", C.c_int32), ("coeff", C.c_double * 16),
-                ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double)]
+                ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double),
+                ("nnp", C.POINTER(orc_nnp))]
 
 
 _dp = C.POINTER(C.c_double)
@@ -51,7 +62,8 @@ class orc_state(C.Structure):
     _fields_ = [
         ("x", _dp), ("box", _dp), ("max_box", _dp), ("r_cut", _dp), ("en_tot", _dp), ("beta", _dp),
         ("max_displ", _dp), ("count_atom", _i64p), ("count_vol", _i64p), ("count_exc", _i64p),
-        ("ham", _dp), ("hist", _dp), ("d2", _dp), ("comp", _dp), ("error", C.POINTER(C.c_int32)),
+        ("ham", _dp), ("hist", _dp), ("d2", _dp), ("comp", _dp), ("g", _dp), ("eat", _dp),
+        ("error", C.POINTER(C.c_int32)),
         ("trace_de", _dp), ("trace_acc", C.POINTER(C.c_int8)), ("trace_idx", C.POINTER(C.c_int32)),
         ("replay_acc", C.POINTER(C.c_int8)),
     ]
@@ -81,6 +93,12 @@ def lib():
         _L.orc_log_det.argtypes = [C.c_double]
         _L.orc_log_det.restype = C.c_double
         _L.orc_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        _L.orc_forward.argtypes = [_dp, _i32p, _i32p, _i32p, _i32p, _i32p, _dp, _i32p, _dp]
+        _L.orc_forward.restype = C.c_double
+        _L.orc_nnp_full_g.argtypes = [C.POINTER(orc_nnp), _dp, C.c_int, _dp]
+        _L.orc_nnp_move_g.argtypes = [C.POINTER(orc_nnp), _dp, C.c_int, C.c_int, _dp, _dp, _dp]
+        _L.orc_nnp_energy.argtypes = [C.POINTER(orc_nnp), _dp, C.c_int, _dp]
+        _L.orc_nnp_energy.restype = C.c_double
     return _L
 
 
@@ -96,15 +114,69 @@ def _p(a, t=C.c_double):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
 
-def make_pot(kind: str, coeff=(), eam=None) -> orc_pot:
+def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None) -> orc_pot:
     p = orc_pot()
-    p.kind = {"elj_even": 0, "elj": 1, "eam": 2}[kind]
+    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3}[kind]
+    if nnp is not None:
+        p.nnp = C.pointer(nnp.s)
+        p._nnp_keep = nnp
     p.ncoeff = len(coeff)
     for i, c in enumerate(coeff):
         p.coeff[i] = c
     if eam is not None:
         p.eam_n, p.eam_m, p.eam_ean, p.eam_ecam = eam
     return p
+
+
+class NNP:
+    """An orc_nnp with the arrays it points to kept alive.  rad rows: (eta,
+    g_norm, g_offset, active); ang rows: (eta, lambda, zeta, tpz, g_offset,
+    active)."""
+
+    def __init__(self, rad, ang, r_cut, nodes, act, params, hardcore2=0.0, q10_compat=False):
+        self.rad = np.ascontiguousarray(rad, dtype=np.float64).reshape(-1, 4)
+        self.ang = np.ascontiguousarray(ang, dtype=np.float64).reshape(-1, 6)
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        self.act = np.ascontiguousarray(act, dtype=np.int32)
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        self.s = orc_nnp(len(self.rad), len(self.ang), _p(self.rad), _p(self.ang), float(r_cut), float(hardcore2),
+                         1 if q10_compat else 0, len(self.nodes), _p(self.nodes, C.c_int32), _p(self.act, C.c_int32),
+                         len(self.params), _p(self.params))
+        self.n_features = len(self.rad) + len(self.ang)
+
+    def full_g(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros((len(x), self.n_features))
+        lib().orc_nnp_full_g(C.byref(self.s), _p(x), len(x), _p(g))
+        return g
+
+    def move_g(self, x, i, trial, g):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        t = np.ascontiguousarray(trial, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        gn = np.zeros_like(g)
+        lib().orc_nnp_move_g(C.byref(self.s), _p(x), len(x), int(i), _p(t), _p(g), _p(gn))
+        return gn
+
+    def energy(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        e = np.zeros(len(g))
+        tot = lib().orc_nnp_energy(C.byref(self.s), _p(g), len(g), _p(e))
+        return tot, e
+
+
+def forward(inp, nodes, act, params):
+    """orc_forward with the reference's argument list; inp is (batch, n_in)."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    act = np.ascontiguousarray(act, dtype=np.int32)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    b = len(inp)
+    eatom = np.zeros(b)
+    i32 = lambda v: C.byref(C.c_int32(int(v)))
+    ret = lib().orc_forward(_p(inp), i32(nodes[0]), i32(len(nodes)), _p(nodes, C.c_int32), i32(b), i32(len(params)),
+                            _p(params), _p(act, C.c_int32), _p(eatom))
+    return ret, eatom
 
 
 REFERENCE = dict(pair_form=0, dist_cache=1, lanes=1, math_lib=1)
@@ -159,6 +231,9 @@ class OracleRun:
         self.hist = np.zeros((R, nb2 * nb2 if ensemble == "npt" else nb2))
         self.d2 = np.zeros((R, N, N)) if mode["dist_cache"] else None
         self.comp = np.zeros((R, N, 2)) if pot.kind == 2 else None
+        nf = pot._nnp_keep.n_features if pot.kind == 3 else 0
+        self.g = np.zeros((R, N, nf)) if pot.kind == 3 else None
+        self.eat = np.zeros((R, N)) if pot.kind == 3 else None
         self.error = np.zeros(1, dtype=np.int32)
         self.trace_cycles = trace_cycles
         self.sweep = 0
@@ -176,6 +251,7 @@ class OracleRun:
         s.count_atom, s.count_vol, s.count_exc = (_p(self.count_atom, C.c_int64), _p(self.count_vol, C.c_int64),
                                                   _p(self.count_exc, C.c_int64))
         s.ham, s.hist, s.d2, s.comp = _p(self.ham), _p(self.hist), _p(self.d2), _p(self.comp)
+        s.g, s.eat = _p(self.g), _p(self.eat)
         s.error = _p(self.error, C.c_int32)
         self.s = s
 

@@ -1,0 +1,219 @@
+/*
+ * ORACLE — test infrastructure only (see orc_math.h).  FP64 restatement of
+ * the reference's single-species RuNNer neural-network potential:
+ *   cutoff            MachineLearningPotential/Cutoff.jl:79-90
+ *   angles            MachineLearningPotential/Cutoff.jl:31-68
+ *   G2 / G4 values    MachineLearningPotential/SymmFunc.jl:184-312
+ *   full G matrix     MachineLearningPotential/SymmFunc.jl:494-510
+ *   moved-atom update MachineLearningPotential/DeltaMatrix.jl:639-975
+ *   forward pass      the RuNNer Fortran `forward` behind
+ *                     MachineLearningPotential/ForwardPass.jl:60-84, restated
+ *                     from the network's definition (input-major weight
+ *                     blocks, each followed by its layer's biases), pinned by
+ *                     test/test_runner_forward.jl:73 and the reference's
+ *                     Cu55 energies in test/testing_data/save.data.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ptmc_oracle.h"
+
+double orc_forward(const double *input, const int32_t *n_in, const int32_t *n_layers, const int32_t *nodes,
+                   const int32_t *batch, const int32_t *n_params, const double *params, const int32_t *act,
+                   double *eatom)
+{
+    const int L = *n_layers, B = *batch;
+    if (L < 2 || *n_in != nodes[0] || B < 1) return NAN;
+    int need = 0, wmax = 0;
+    for (int l = 1; l < L; ++l) {
+        need += nodes[l - 1] * nodes[l] + nodes[l];
+        if (nodes[l] > wmax) wmax = nodes[l];
+    }
+    if (nodes[0] > wmax) wmax = nodes[0];
+    if (need != *n_params || nodes[L - 1] != 1) return NAN;
+    double *a = (double *)malloc(sizeof(double) * wmax), *z = (double *)malloc(sizeof(double) * wmax);
+    for (int b = 0; b < B; ++b) {
+        memcpy(a, input + (size_t)b * nodes[0], sizeof(double) * nodes[0]);
+        const double *p = params;
+        for (int l = 1; l < L; ++l) {
+            const int ni = nodes[l - 1], no = nodes[l];
+            const double *w = p, *bias = p + (size_t)ni * no;
+            for (int j = 0; j < no; ++j) {
+                double s = 0.0;
+                for (int i = 0; i < ni; ++i) s += a[i] * w[(size_t)i * no + j];
+                s += bias[j];
+                z[j] = act[l] == 2 ? tanh(s) : s;
+            }
+            memcpy(a, z, sizeof(double) * no);
+            p = bias + no;
+        }
+        eatom[b] = a[0];
+    }
+    free(a);
+    free(z);
+    return eatom[B - 1];
+}
+
+double orc_cutoff(double r, double r_cut)
+{
+    return r < r_cut ? 0.5 * (cos(M_PI * (r / r_cut)) + 1.0) : 0.0;
+}
+
+static double d2(const double *a, const double *b)
+{
+    const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+/* thetacalc(xy, xz, |xy|, |xz|) = xy . xz / (|xy| |xz|) */
+static double thetacalc(const double *xy, const double *xz, double dxy, double dxz)
+{
+    return (xy[0] * xz[0] + xy[1] * xz[1] + xy[2] * xz[2]) / (dxy * dxz);
+}
+
+/* all_angular_measure(a, b, c, r2ab, r2ac, r2bc): cosines at a, b and c */
+static void angles(const double *a, const double *b, const double *c, double r2ab, double r2ac, double r2bc,
+                   double th[3])
+{
+    double ab[3], ac[3], bc[3], mab[3], mac[3], mbc[3];
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = a[k] - b[k];
+        ac[k] = a[k] - c[k];
+        bc[k] = b[k] - c[k];
+        mab[k] = -ab[k];
+        mac[k] = -ac[k];
+        mbc[k] = -bc[k];
+    }
+    const double dab = sqrt(r2ab), dac = sqrt(r2ac), dbc = sqrt(r2bc);
+    th[0] = thetacalc(ab, ac, dab, dac);
+    th[1] = thetacalc(mab, bc, dab, dbc);
+    th[2] = thetacalc(mac, mbc, dac, dbc);
+}
+
+#define RAD(g, k) nn->rad[(g) * 4 + (k)]
+#define ANG(g, k) nn->ang[(g) * 6 + (k)]
+
+int orc_nnp_full_g(const orc_nnp *nn, const double *x, int N, double *g)
+{
+    const int nf = nn->n_rad + nn->n_ang;
+    double *r2 = (double *)malloc(sizeof(double) * N * N), *f = (double *)malloc(sizeof(double) * N * N);
+    if (!r2 || !f) return -1;
+    for (int a = 0; a < N; ++a)
+        for (int b = 0; b < N; ++b) {
+            r2[a * N + b] = d2(&x[3 * a], &x[3 * b]);
+            f[a * N + b] = orc_cutoff(sqrt(r2[a * N + b]), nn->r_cut);
+        }
+    memset(g, 0, sizeof(double) * nf * N);
+    double *row = (double *)malloc(sizeof(double) * N);
+    /* radial rows, calc_symm_vals!(RadialType2) SymmFunc.jl:259-278 */
+    for (int q = 0; q < nn->n_rad; ++q) {
+        if (RAD(q, 3) == 0.0) continue;  /* inactive type: zeros, no offset */
+        const double eta = RAD(q, 0), norm = RAD(q, 1), off = RAD(q, 2);
+        for (int a = 0; a < N; ++a) row[a] = 0.0;
+        for (int a = 0; a < N; ++a)
+            for (int b = a + 1; b < N; ++b) {
+                const double fc = f[a * N + b];
+                const double v = (fc != 0.0 && fc != 1.0) ? fc * exp(-eta * r2[a * N + b]) * norm : 0.0;
+                row[a] += v;
+                row[b] += v;
+            }
+        for (int a = 0; a < N; ++a) g[a * nf + q] = row[a] + off;
+    }
+    /* angular rows, SymmFunc.jl:280-312; total_symm_calc runs g_index over
+     * (Nrad+1):Nang, so with q10_compat the last n_rad rows stay zero */
+    const int n_ang_built = nn->q10_compat ? (nn->n_ang - nn->n_rad > 0 ? nn->n_ang - nn->n_rad : 0) : nn->n_ang;
+    for (int q = 0; q < n_ang_built; ++q) {
+        if (ANG(q, 5) == 0.0) continue;
+        const double eta = ANG(q, 0), lam = ANG(q, 1), zeta = ANG(q, 2), tpz = ANG(q, 3), off = ANG(q, 4);
+        for (int a = 0; a < N; ++a) row[a] = 0.0;
+        for (int a = 0; a < N; ++a)
+            for (int b = a + 1; b < N; ++b)
+                for (int c = b + 1; c < N; ++c) {
+                    double th[3];
+                    const double rab = r2[a * N + b], rac = r2[a * N + c], rbc = r2[b * N + c];
+                    angles(&x[3 * a], &x[3 * b], &x[3 * c], rab, rac, rbc, th);
+                    const double e = exp(-eta * (rab + rac + rbc)) * f[a * N + b] * f[a * N + c] * f[b * N + c];
+                    row[a] += e * pow(1.0 + lam * th[0], zeta) * tpz;
+                    row[b] += e * pow(1.0 + lam * th[1], zeta) * tpz;
+                    row[c] += e * pow(1.0 + lam * th[2], zeta) * tpz;
+                }
+        for (int a = 0; a < N; ++a) g[a * nf + nn->n_rad + q] = row[a] + off;
+    }
+    free(row);
+    free(r2);
+    free(f);
+    return 0;
+}
+
+int orc_nnp_move_g(const orc_nnp *nn, const double *x, int N, int i, const double *trial, const double *g,
+                   double *g_new)
+{
+    const int nf = nn->n_rad + nn->n_ang;
+    memcpy(g_new, g, sizeof(double) * nf * N);
+    double *rn = (double *)malloc(sizeof(double) * N), *fn = (double *)malloc(sizeof(double) * N);
+    double *ro = (double *)malloc(sizeof(double) * N), *fo = (double *)malloc(sizeof(double) * N);
+    for (int j = 0; j < N; ++j) {
+        rn[j] = j == i ? 0.0 : d2(trial, &x[3 * j]);
+        ro[j] = j == i ? 0.0 : d2(&x[3 * i], &x[3 * j]);
+        fn[j] = orc_cutoff(sqrt(rn[j]), nn->r_cut);
+        fo[j] = orc_cutoff(sqrt(ro[j]), nn->r_cut);
+    }
+    /* radial_symmetry_calculation!, DeltaMatrix.jl:794-822 */
+    for (int q = 0; q < nn->n_rad; ++q) {
+        if (RAD(q, 3) == 0.0) continue;
+        const double eta = RAD(q, 0), norm = RAD(q, 1);
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            double gi = g_new[i * nf + q], gj = g_new[j * nf + q];
+            const double vn = exp(-eta * rn[j]) * (fn[j] * norm);
+            gi += vn;
+            gj += vn;
+            const double vo = exp(-eta * ro[j]) * (-fo[j] * norm);
+            gi += vo;
+            gj += vo;
+            g_new[i * nf + q] = gi;
+            g_new[j * nf + q] = gj;
+        }
+    }
+    /* angular_symmetry_calculation!, DeltaMatrix.jl:827-872 */
+    for (int q = 0; q < nn->n_ang; ++q) {
+        if (ANG(q, 5) == 0.0) continue;
+        const double eta = ANG(q, 0), lam = ANG(q, 1), zeta = ANG(q, 2), tpz = ANG(q, 3);
+        const int col = nn->n_rad + q;
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            for (int k = j + 1; k < N; ++k) {
+                if (k == i) continue;
+                const double rjk = d2(&x[3 * j], &x[3 * k]);
+                const double fjk = orc_cutoff(sqrt(rjk), nn->r_cut);
+                double tn[3], to[3];
+                angles(trial, &x[3 * j], &x[3 * k], rn[j], rn[k], rjk, tn);
+                angles(&x[3 * i], &x[3 * j], &x[3 * k], ro[j], ro[k], rjk, to);
+                const double en = exp(-eta * (rn[j] + rn[k] + rjk)) * fn[j] * fn[k] * fjk;
+                const double eo = exp(-eta * (ro[j] + ro[k] + rjk)) * fo[j] * fo[k] * fjk;
+                const int idx[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    double v = g_new[idx[a] * nf + col];
+                    v += en * pow(1.0 + lam * tn[a], zeta) * tpz;
+                    v -= eo * pow(1.0 + lam * to[a], zeta) * tpz;
+                    g_new[idx[a] * nf + col] = v;
+                }
+            }
+        }
+    }
+    free(rn);
+    free(fn);
+    free(ro);
+    free(fo);
+    return 0;
+}
+
+double orc_nnp_energy(const orc_nnp *nn, const double *g, int N, double *eatom)
+{
+    const int32_t nin = nn->n_rad + nn->n_ang, batch = N;
+    orc_forward(g, &nin, &nn->n_layers, nn->nodes, &batch, &nn->n_params, nn->params, nn->act, eatom);
+    double e = 0.0;
+    for (int a = 0; a < N; ++a) e += eatom[a];
+    return e;
+}

@@ -210,7 +210,12 @@ int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s)
                         orc_distance2(&x[3 * i], &x[3 * j], p->bc, box, dist_form(p));
             }
         }
-        if (pot->kind == ORC_POT_EAM) {
+        if (pot->kind == ORC_POT_NNP) {
+            /* set_variables + initialise_energy, src/EnergyEvaluation.jl:1812-1822, :1735-1747 */
+            const int nf = pot->nnp->n_rad + pot->nnp->n_ang;
+            if (orc_nnp_full_g(pot->nnp, x, N, &s->g[(size_t)r * N * nf])) return -1;
+            s->en_tot[r] = orc_nnp_energy(pot->nnp, &s->g[(size_t)r * N * nf], N, &s->eat[(size_t)r * N]);
+        } else if (pot->kind == ORC_POT_EAM) {
             eam_build(p, pot, x, box, &s->comp[(size_t)r * N * 2]);
             s->en_tot[r] = eam_energy_from_comp(pot, &s->comp[(size_t)r * N * 2], N,
                                                 p->math_lib ? 1 : (p->lanes > 0 ? p->lanes : 1));
@@ -317,6 +322,8 @@ typedef struct {
     double new_en;
     double new_row[MAXN];
     double new_comp[2 * MAXN];
+    double *g_new;    /* NNP: N * n_features */
+    double *eat_new;  /* NNP: N */
 } trial_out;
 
 /* Energy of an atom move: src/EnergyEvaluation.jl:804-821 (dimer) and
@@ -334,6 +341,20 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
 
     for (int j = 0; j < N; ++j)
         o->new_row[j] = j == i ? 0.0 : orc_distance2(trial, &x[3 * j], p->bc, box, dist_form(p));
+
+    if (pot->kind == ORC_POT_NNP) {
+        /* energy_update!(RuNNerPotential), src/EnergyEvaluation.jl:1593-1621: the
+         * hard-core test gives the sentinel energy 100 */
+        for (int j = 0; j < N; ++j)
+            if (j != i && o->new_row[j] < pot->nnp->hardcore2) {
+                o->new_en = 100.0;
+                return;
+            }
+        const int nf = pot->nnp->n_rad + pot->nnp->n_ang;
+        orc_nnp_move_g(pot->nnp, x, N, i, trial, &s->g[(size_t)r * N * nf], o->g_new);
+        o->new_en = orc_nnp_energy(pot->nnp, o->g_new, N, o->eat_new);
+        return;
+    }
 
     if (pot->kind == ORC_POT_EAM) {
         const double *comp = &s->comp[(size_t)r * N * 2];
@@ -415,6 +436,10 @@ static void swap_states(const orc_params *p, const orc_pot *pot, orc_state *s, i
     if (p->bc == ORC_CUBIC) SWAPN(s->box, 1);
     if (p->ensemble == ORC_NPT) SWAPN(s->r_cut, 1);
     if (pot->kind == ORC_POT_EAM) SWAPN(s->comp, (size_t)N * 2);
+    if (pot->kind == ORC_POT_NNP) {
+        SWAPN(s->g, (size_t)N * (pot->nnp->n_rad + pot->nnp->n_ang));
+        SWAPN(s->eat, (size_t)N);
+    }
 #undef SWAPN
 }
 
@@ -456,9 +481,14 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
 {
     const int N = p->n_atoms, R = p->n_temps;
     const uint32_t nsteps = (uint32_t)(p->ensemble == ORC_NPT ? N + 1 : N);
-    trial_out *o = (trial_out *)malloc(sizeof(trial_out));
+    trial_out *o = (trial_out *)calloc(1, sizeof(trial_out));
     double *vol_x = (double *)malloc(sizeof(double) * 3 * N);
     if (!o || !vol_x) return -2;
+    if (pot->kind == ORC_POT_NNP) {
+        if (p->ensemble != ORC_NVT) return -3;
+        o->g_new = (double *)malloc(sizeof(double) * N * (pot->nnp->n_rad + pot->nnp->n_ang));
+        o->eat_new = (double *)malloc(sizeof(double) * N);
+    }
     if (s->error) *s->error = 0;
 
     for (int64_t c = 0; c < n_sweeps; ++c) {
@@ -491,7 +521,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                                 make_trial(p, &x[3 * i], s->max_displ[2 * r], box, &dd, t);
                                 if (count == 100) {
                                     if (s->error) *s->error = 1;
-                                    free(o); free(vol_x);
+                                    free(o->g_new); free(o->eat_new); free(o); free(vol_x);
                                     return 1;
                                 }
                             }
@@ -519,6 +549,12 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         }
                         if (pot->kind == ORC_POT_EAM)
                             memcpy(&s->comp[(size_t)r * N * 2], o->new_comp, sizeof(double) * 2 * N);
+                        if (pot->kind == ORC_POT_NNP && new_en != 100.0) {
+                            /* swap_vars!(NNPVariables), src/swap_config.jl:213-221 */
+                            const size_t nf = (size_t)(pot->nnp->n_rad + pot->nnp->n_ang);
+                            memcpy(&s->g[(size_t)r * N * nf], o->g_new, sizeof(double) * N * nf);
+                            memcpy(&s->eat[(size_t)r * N], o->eat_new, sizeof(double) * N);
+                        }
                         s->en_tot[r] = new_en;
                         s->count_atom[2 * r] += 1;
                         s->count_atom[2 * r + 1] += 1;
@@ -634,6 +670,8 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
             }
         }
     }
+    free(o->g_new);
+    free(o->eat_new);
     free(o);
     free(vol_x);
     return 0;

@@ -18,7 +18,7 @@ extern "C" {
 
 enum { ORC_NVT = 0, ORC_NPT = 1 };
 enum { ORC_SPHERE = 0, ORC_CUBIC = 1 };
-enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2 };
+enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3 };
 /* exchange_mode: 0 = reference (p = 0.1 per cycle, one random adjacent pair),
  *                1 = one random adjacent pair every cycle,
  *                2 = all even (cycle even) / odd (cycle odd) pairs every cycle */
@@ -45,11 +45,33 @@ typedef struct {
     double  en_min, delta_en_hist, v_min, delta_v_hist;
 } orc_params;
 
+/* RuNNer-style neural-network potential (src/EnergyEvaluation.jl:1182-1275,
+ * MachineLearningPotential/).  Symmetry functions:
+ *   rad: n_rad rows of (eta, g_norm, g_offset, active)      RadialType2, type 11
+ *   ang: n_ang rows of (eta, lambda, zeta, tpz, g_offset, active)  AngularType3, type 111
+ * with tpz = 2^(1-zeta) g_norm.  The feature matrix G is stored atom-major,
+ * G[a * (n_rad + n_ang) + g], i.e. the column-major (n_rad+n_ang) x N matrix
+ * of the reference. */
+typedef struct {
+    int32_t n_rad, n_ang;
+    const double *rad;
+    const double *ang;
+    double  r_cut;
+    double  hardcore2;    /* RuNNerPotential.boundary (squared); 0 = none */
+    int32_t q10_compat;   /* 1: initial G as total_symm_calc builds it (SymmFunc.jl:504) */
+    int32_t n_layers;
+    const int32_t *nodes;
+    const int32_t *act;   /* per layer: 1 linear, 2 tanh (layer 1 = input) */
+    int32_t n_params;
+    const double *params;
+} orc_nnp;
+
 typedef struct {
     int32_t kind;
     int32_t ncoeff;
     double  coeff[16];
     double  eam_n, eam_m, eam_ean, eam_ecam;
+    const orc_nnp *nnp;
 } orc_pot;
 
 typedef struct {
@@ -67,6 +89,8 @@ typedef struct {
     double  *hist;         /* R*(n_bin+2) NVT energy histograms, or R*(n_bin+2)^2 NPT (E,V) */
     double  *d2;           /* R*N*N cached squared distances (dist_cache = 1) */
     double  *comp;         /* R*N*2 EAM phi/rho components */
+    double  *g;            /* R*N*(n_rad+n_ang) NNP feature matrices */
+    double  *eat;          /* R*N NNP per-atom energies */
     int32_t *error;        /* out: 0 ok, 1 too many moves out of binding sphere */
     /* optional per-move trace, indexed [r][cycle][move] */
     double  *trace_de;     /* new_en - en_tot as used by the Metropolis test */
@@ -91,6 +115,21 @@ double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot);
 double orc_exp_det(double x);
 double orc_log_det(double x);
 void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* NNP pieces (ptmc_nnp.c). */
+/* the RuNNer forward pass with the argument list of the reference's ccall
+ * (MachineLearningPotential/ForwardPass.jl:60-84); returns eatom[batch-1] */
+double orc_forward(const double *input, const int32_t *n_in, const int32_t *n_layers, const int32_t *nodes,
+                   const int32_t *batch, const int32_t *n_params, const double *params, const int32_t *act,
+                   double *eatom);
+double orc_cutoff(double r, double r_cut);
+/* total_symm_calc (SymmFunc.jl:494-510): full feature matrix from positions */
+int    orc_nnp_full_g(const orc_nnp *nn, const double *x, int n_atoms, double *g);
+/* total_thr_symm! (DeltaMatrix.jl:933-975): features after moving atom i to trial */
+int    orc_nnp_move_g(const orc_nnp *nn, const double *x, int n_atoms, int i, const double *trial,
+                      const double *g, double *g_new);
+/* sum of the per-atom network energies of a feature matrix */
+double orc_nnp_energy(const orc_nnp *nn, const double *g, int n_atoms, double *eatom);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@ lib/libptmc.so; this package is the host-side mirror of the reference's
 Julia types and driver.
 """
 from .api import (  # noqa: F401
+    AngularType3,
     CubicBC,
     Config,
     ELJPotential,
@@ -15,16 +16,21 @@ from .api import (  # noqa: F401
     EngineOptions,
     MCParams,
     MCState,
+    NeuralNetworkPotential,
     NPT,
     NVT,
     Output,
     PTEngine,
+    RadialType2,
+    RuNNerPotential,
     SphericalBC,
     TempGrid,
     check_e_bounds,
     kB,
     ptmc_run,
     ptmc_run_batch,
+    read_runner_scaling,
+    read_runner_weights,
 )
 from ._lib import PTMCError, load as load_library  # noqa: F401
 from .multihist import multihistogram_NVT, multihistogram_nvt_batch  # noqa: F401

@@ -178,6 +178,110 @@ class EmbeddedAtomPotential:
         return cls(float(n), float(m), eps * a**n / 2, eps * C_ * a ** (m / 2))
 
 
+class NeuralNetworkPotential:
+    """MachineLearningPotential/ForwardPass.jl:23-39: layer sizes, activation
+    codes (1 linear, 2 tanh; the first entry belongs to the input layer) and
+    the flat parameter vector in RuNNer order (for each layer: the weights
+    input-major, w[i * n_out + j], then the layer's biases)."""
+
+    def __init__(self, num_nodes, activation_functions, parameters):
+        self.num_nodes = np.asarray(num_nodes, dtype=np.int32)
+        self.activation_functions = np.asarray(activation_functions, dtype=np.int32)
+        self.parameters = np.asarray(parameters, dtype=np.float64)
+        self.n_layers = len(self.num_nodes)
+        self.n_params = len(self.parameters)
+        need = sum(int(self.num_nodes[l - 1]) * int(self.num_nodes[l]) + int(self.num_nodes[l])
+                   for l in range(1, self.n_layers))
+        if need != self.n_params:
+            raise ValueError(f"{self.n_params} parameters for layers {list(self.num_nodes)} (need {need})")
+        if len(self.activation_functions) != self.n_layers or self.num_nodes[-1] != 1:
+            raise ValueError("one activation code per layer and one output node are required")
+        if not set(int(a) for a in self.activation_functions[1:]) <= {1, 2}:
+            raise ValueError("activation codes: 1 (linear) or 2 (tanh)")
+
+
+class RadialType2:
+    """G2 radial symmetry function, MachineLearningPotential/SymmFunc.jl:24-44:
+    G_i = norm sum_j f_c(r_ij) exp(-eta r_ij^2) + offset; G_vals = [G_max,
+    G_min] sets norm = 1/(G_max - G_min), offset = -G_min norm.  Only type 11
+    (single species) contributes."""
+
+    def __init__(self, eta, r_cut, type_vec=11, G_vals=None):
+        self.eta, self.r_cut, self.type_vec = float(eta), float(r_cut), int(type_vec)
+        if G_vals is None:
+            self.G_norm, self.G_offset = 1.0, 0.0
+        else:
+            self.G_norm = 1.0 / (float(G_vals[0]) - float(G_vals[1]))
+            self.G_offset = -float(G_vals[1]) * self.G_norm
+
+
+class AngularType3:
+    """G4 angular symmetry function, MachineLearningPotential/SymmFunc.jl:86-112:
+    G_i = tpz sum_{j<k} (1 + lambda cos theta_ijk)^zeta exp(-eta (r_ij^2 +
+    r_ik^2 + r_jk^2)) f_ij f_ik f_jk + offset with tpz = 2^(1-zeta) norm.  Only
+    type 111 contributes."""
+
+    def __init__(self, eta, lambda_, zeta, r_cut, type_vec=111, G_vals=None):
+        self.eta, self.lambda_, self.zeta = float(eta), float(lambda_), float(zeta)
+        self.r_cut, self.type_vec = float(r_cut), int(type_vec)
+        norm = 1.0 if G_vals is None else 1.0 / (float(G_vals[0]) - float(G_vals[1]))
+        self.G_offset = 0.0 if G_vals is None else -float(G_vals[1]) * norm
+        self.tpz = 2.0 ** (1.0 - self.zeta) * norm
+
+
+class RuNNerPotential:
+    """src/EnergyEvaluation.jl:1182-1213: per-atom network on G2 + G4 features;
+    E = sum_i NN(G_i).  `boundary` is a hard-core distance: a move closer than
+    it to another atom gets the energy 100 (:1604-1606).  q10_compat = True
+    builds the initial feature matrix as total_symm_calc does
+    (MachineLearningPotential/SymmFunc.jl:504 runs its angular loop over
+    (Nrad+1):Nang, leaving the last Nrad angular rows zero); the default
+    builds every row."""
+
+    def __init__(self, nnp: NeuralNetworkPotential, radsymvec, angsymvec, boundary: float = 0.0,
+                 q10_compat: bool = False):
+        self.nnp = nnp
+        self.radsymfunctions = list(radsymvec)
+        self.angsymfunctions = list(angsymvec)
+        self.r_cut = self.radsymfunctions[0].r_cut
+        self.boundary = float(boundary) ** 2
+        self.q10_compat = bool(q10_compat)
+        if self.nnp.num_nodes[0] != len(self.radsymfunctions) + len(self.angsymfunctions):
+            raise ValueError("network input size differs from the number of symmetry functions")
+        if any(f.r_cut != self.r_cut for f in self.radsymfunctions + self.angsymfunctions):
+            raise ValueError("all symmetry functions must share one cutoff radius")
+
+    def tables(self):
+        """(n_rad x 4: eta, norm, offset, active) and (n_ang x 6: eta, lambda,
+        zeta, tpz, offset, active)."""
+        rad = np.array([[f.eta, f.G_norm, f.G_offset, 1.0 if f.type_vec == 11 else 0.0]
+                        for f in self.radsymfunctions], dtype=np.float64)
+        ang = np.array([[f.eta, f.lambda_, f.zeta, f.tpz, f.G_offset, 1.0 if f.type_vec == 111 else 0.0]
+                        for f in self.angsymfunctions], dtype=np.float64)
+        return rad, ang
+
+
+def read_runner_weights(path) -> np.ndarray:
+    """A RuNNer weights.XXX.data file (first column) as the flat parameter vector."""
+    vals = []
+    for line in open(path):
+        t = line.split()
+        if t:
+            vals.append(float(t[0]))
+    return np.array(vals)
+
+
+def read_runner_scaling(path) -> np.ndarray:
+    """RuNNer scaling.data rows (element, function, G_min, G_max, G_mean) as an
+    array; the trailing energy line is left out."""
+    rows = []
+    for line in open(path):
+        t = line.split()
+        if len(t) == 5:
+            rows.append([float(v) for v in t])
+    return np.array(rows)
+
+
 @dataclasses.dataclass
 class EngineOptions:
     """Settings with no counterpart in the reference.
