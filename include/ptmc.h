@@ -90,6 +90,21 @@ int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, 
  * resident next to the positions and are updated per move as
  * energy_update! does (src/EnergyEvaluation.jl:1567-1591). */
 int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam);
+/* RuNNer-style network potential (RuNNerPotential, src/EnergyEvaluation.jl:1182-1213;
+ * symmetry functions MachineLearningPotential/SymmFunc.jl:24-112): rad holds
+ * n_rad rows (eta, G_norm, G_offset, active) of RadialType2, ang n_ang rows
+ * (eta, lambda, zeta, tpz, G_offset, active) of AngularType3 (active = type
+ * 11 / 111); feature rows are the radial functions then the angular ones.
+ * hardcore2 is RuNNerPotential.boundary (squared; 0 = none), q10_compat builds
+ * the initial features as total_symm_calc does (SymmFunc.jl:504).  The
+ * network is [n_rad + n_ang, h1, h2, 1] with activations [*, 2 (tanh),
+ * 2 (tanh), 1 (linear)] and RuNNer parameter order (NeuralNetworkPotential,
+ * ForwardPass.jl:23-39).  Binary32 features and network, FP64 totals;
+ * NVT in a spherical container, at most 64 atoms. */
+int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const double *rad, const double *ang,
+                           double r_cut, double hardcore2, int32_t q10_compat, int32_t n_layers,
+                           const int32_t *nodes, const int32_t *activations, int32_t n_params,
+                           const double *params);
 
 /* Runs.  Replaces initialisation (src/Initialization.jl:70-101) and
  * MCState construction (src/MCStates.jl:137-162): energies of the start

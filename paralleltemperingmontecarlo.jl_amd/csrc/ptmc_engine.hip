@@ -14,9 +14,16 @@
 
 #include "../../include/ptmc.h"
 #include "ptmc_kernels.hpp"
+#include "ptmc_nnp_layout.hpp"
 #include "ptmc_sweep.hpp"
 
 using namespace ptmc;
+
+namespace ptmc {
+SweepFn nnp_sweep_fn();
+SweepFn nnp_init_fn();
+void (*nnp_cycle_fn())(SweepArgs, int);
+}  // namespace ptmc
 
 namespace {
 
@@ -104,6 +111,19 @@ struct ptmc_engine {
     int trace_cycles = 0;
     int trace_pending = 0;
     int trace_nsteps = 0;
+
+    // neural-network potential (one workgroup per slot)
+    struct {
+        std::vector<float> rad, geta, afn, rowoff, w;
+        std::vector<int32_t> grp, abuild;
+        int n_rad = 0, n_ang = 0, nf = 0, h1 = 0, h2 = 0, n_rad_active = 0, n_groups = 0, q10 = 0;
+        double r_cut = 0.0, hardcore2 = 0.0;
+        float b3 = 0.0f;
+    } nnp;
+    DevBuf<float> d_nnp_rad, d_nnp_geta, d_nnp_afn, d_nnp_rowoff, d_nnp_w, d_feat, d_eatom;
+    DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_temp_slot;
+    NnpTables nnp_dev{};
+    size_t nnp_lds = 0;
 
     int64_t sweep_counter = 0;  // global cycle counter (Philox word 1)
     int64_t phase_counter = 0;  // phase-local cycle index
@@ -430,7 +450,99 @@ SweepArgs base_args(ptmc_engine *h)
     a.recompute_every = h->cfg.recompute_every;
     a.min_acc = h->cfg.min_acc;
     a.max_acc = h->cfg.max_acc;
+    a.nnp = h->nnp_dev;
+    a.feat = h->d_feat.p;
+    a.eatom = h->d_eatom.p;
+    a.temp_slot = h->d_temp_slot.p;
     return a;
+}
+
+bool is_nnp(const ptmc_engine *h) { return h->pot.kind == POT_NNP; }
+
+// Slots of a neural-network engine: one workgroup per slot, slots of a run
+// consecutive, temperature t of run r starting in slot slot_base + t.
+int prepare_nnp(ptmc_engine *h)
+{
+    if (h->cfg.precision != 1) return fail(h, PTMC_E_UNSUPPORTED, "the network potential runs in binary32 (precision 1)");
+    if (h->cfg.ensemble != ENS_NVT || h->cfg.boundary != BC_SPHERE)
+        return fail(h, PTMC_E_UNSUPPORTED, "the network potential runs NVT in a spherical container");
+    int dev = 0, max_lds = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    h->nnp_lds = nnp_lds_floats(h->nnp.nf, h->nnp.h1) * sizeof(float);
+    if ((long long)h->nnp_lds > max_lds) return fail(h, PTMC_E_UNSUPPORTED, "network tables need %zu B of LDS", h->nnp_lds);
+    const void *fs = (const void *)nnp_sweep_fn(), *fi = (const void *)nnp_init_fn();
+    HIPCHK(h, hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp_lds));
+    HIPCHK(h, hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp_lds));
+    int slot = 0;
+    for (auto &rh : h->runs) {
+        if (rh.d.n_atoms > NNP_NP) return fail(h, PTMC_E_UNSUPPORTED, "at most %d atoms with the network potential", NNP_NP);
+        rh.info.slot_base = slot;
+        rh.info.local_base = 0;
+        rh.kl = NNP_NT;
+        rh.lanes = NNP_NT;
+        rh.tile = -1;
+        slot += rh.d.n_temps;
+    }
+    h->n_slots = slot;
+    h->tiles.clear();
+    h->launches.clear();
+    Launch L;
+    L.fn = (SweepFn)fs;
+    L.init = (SweepFn)fi;
+    L.ntiles = slot;
+    L.wgs = NNP_NT;
+    L.kl = NNP_NT;
+    L.lds = h->nnp_lds;
+    h->launches.push_back(L);
+    h->n_coord = (int64_t)slot * 3 * NNP_NP;
+    h->n_comp = 0;
+    int rc;
+    auto &n = h->nnp;
+    if ((rc = upload(h, h->d_nnp_rad, n.rad)) || (rc = upload(h, h->d_nnp_geta, n.geta)) ||
+        (rc = upload(h, h->d_nnp_afn, n.afn)) || (rc = upload(h, h->d_nnp_rowoff, n.rowoff)) ||
+        (rc = upload(h, h->d_nnp_w, n.w)) || (rc = upload(h, h->d_nnp_grp, n.grp)) ||
+        (rc = upload(h, h->d_nnp_abuild, n.abuild)))
+        return rc;
+    if ((rc = dalloc(h, h->d_feat, (size_t)slot * n.nf * NNP_NP))) return rc;
+    if ((rc = dalloc(h, h->d_eatom, (size_t)slot * NNP_NP))) return rc;
+    NnpTables &t = h->nnp_dev;
+    t.n_rad = n.n_rad;
+    t.n_ang = n.n_ang;
+    t.nf = n.nf;
+    t.h1 = n.h1;
+    t.h2 = n.h2;
+    t.n_rad_active = n.n_rad_active;
+    t.n_groups = n.n_groups;
+    t.r_cut = (float)n.r_cut;
+    t.hardcore2 = n.hardcore2;
+    t.rad = h->d_nnp_rad.p;
+    t.grp = h->d_nnp_grp.p;
+    t.grp_eta = h->d_nnp_geta.p;
+    t.afn = h->d_nnp_afn.p;
+    t.abuild = h->d_nnp_abuild.p;
+    t.row_off = h->d_nnp_rowoff.p;
+    t.w1 = h->d_nnp_w.p;
+    t.b1 = t.w1 + (size_t)n.nf * n.h1;
+    t.w2 = t.b1 + n.h1;
+    t.b2 = t.w2 + (size_t)n.h1 * n.h2;
+    t.w3 = t.b2 + n.h2;
+    t.b3 = n.b3;
+    return PTMC_OK;
+}
+
+void scatter_coords_nnp(ptmc_engine *h, std::vector<float> &buf)
+{
+    buf.assign((size_t)h->n_coord, 0.0f);
+    for (auto &rh : h->runs) {
+        const int N = rh.d.n_atoms;
+        for (int t = 0; t < rh.d.n_temps; ++t) {
+            const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
+            float *dst = &buf[(size_t)(rh.info.slot_base + t) * 3 * NNP_NP];
+            for (int j = 0; j < N; ++j)
+                for (int c = 0; c < 3; ++c) dst[c * NNP_NP + j] = (float)x[3 * j + c];
+        }
+    }
 }
 
 int timing_event(ptmc_engine *h, hipEvent_t *ev)
@@ -542,6 +654,16 @@ int ptmc_destroy(ptmc_handle h)
     h->d_slot_drift.free_();
     h->d_coords.free_();
     h->d_comps.free_();
+    h->d_nnp_rad.free_();
+    h->d_nnp_geta.free_();
+    h->d_nnp_afn.free_();
+    h->d_nnp_rowoff.free_();
+    h->d_nnp_w.free_();
+    h->d_feat.free_();
+    h->d_eatom.free_();
+    h->d_nnp_grp.free_();
+    h->d_nnp_abuild.free_();
+    h->d_temp_slot.free_();
     h->d_beta.free_();
     h->d_displ.free_();
     h->d_maxbox.free_();
@@ -602,6 +724,71 @@ int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double
     return PTMC_OK;
 }
 
+int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const double *rad, const double *ang,
+                           double r_cut, double hardcore2, int32_t q10_compat, int32_t n_layers, const int32_t *nodes,
+                           const int32_t *activations, int32_t n_params, const double *params)
+{
+    if (!h) return PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    if (n_rad < 0 || n_ang < 0 || n_rad + n_ang < 1 || (n_rad && !rad) || (n_ang && !ang) || !(r_cut > 0) ||
+        !nodes || !activations || !params)
+        return fail(h, PTMC_E_ARG, "bad network potential description");
+    if (n_layers != 4 || nodes[0] != n_rad + n_ang || nodes[3] != 1 || nodes[1] < 1 || nodes[1] > NNP_MAXH ||
+        nodes[2] < 1 || nodes[2] > NNP_MAXH || activations[1] != 2 || activations[2] != 2 || activations[3] != 1)
+        return fail(h, PTMC_E_UNSUPPORTED, "network shape must be [n_features, h1 <= %d (tanh), h2 <= %d (tanh), 1 (linear)]",
+                    NNP_MAXH, NNP_MAXH);
+    const int nf = n_rad + n_ang, h1 = nodes[1], h2 = nodes[2];
+    if (n_params != nf * h1 + h1 + h1 * h2 + h2 + h2 + 1) return fail(h, PTMC_E_ARG, "parameter count does not match the layers");
+    auto &n = h->nnp;
+    n = {};
+    n.n_rad = n_rad;
+    n.n_ang = n_ang;
+    n.nf = nf;
+    n.h1 = h1;
+    n.h2 = h2;
+    n.r_cut = r_cut;
+    n.hardcore2 = hardcore2;
+    n.q10 = q10_compat ? 1 : 0;
+    n.rowoff.assign(nf, 0.0f);
+    for (int q = 0; q < n_rad; ++q) {
+        const double *r = rad + 4 * q;
+        if (r[3] == 0.0) continue;
+        n.rad.insert(n.rad.end(), {(float)r[0], (float)r[1], (float)q, 0.0f});
+        n.rowoff[q] = (float)r[2];
+        n.n_rad_active += 1;
+    }
+    // angular functions grouped by eta (stable order), at most NNP_GMAX per group
+    const int n_built = q10_compat ? std::max(0, n_ang - n_rad) : n_ang;
+    std::vector<int> idx;
+    for (int q = 0; q < n_ang; ++q)
+        if (ang[6 * q + 5] != 0.0) idx.push_back(q);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return ang[6 * x] < ang[6 * y]; });
+    for (size_t k = 0; k < idx.size();) {
+        const double eta = ang[6 * idx[k]];
+        const int first = (int)(n.afn.size() / 4);
+        int cnt = 0;
+        while (k < idx.size() && ang[6 * idx[k]] == eta && cnt < NNP_GMAX) {
+            const double *a = ang + 6 * idx[k];
+            n.afn.insert(n.afn.end(), {(float)a[1], (float)a[2], (float)a[3], (float)(n_rad + idx[k])});
+            n.abuild.push_back(idx[k] < n_built ? 1 : 0);
+            if (idx[k] < n_built) n.rowoff[n_rad + idx[k]] = (float)a[4];
+            ++cnt;
+            ++k;
+        }
+        n.grp.insert(n.grp.end(), {first, cnt});
+        n.geta.push_back((float)eta);
+        n.n_groups += 1;
+    }
+    n.w.resize((size_t)n_params - 1);
+    for (int k = 0; k + 1 < n_params; ++k) n.w[k] = (float)params[k];
+    n.b3 = (float)params[n_params - 1];
+    PotParams p{};
+    p.kind = POT_NNP;
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
+}
+
 int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *d, int32_t *run_index)
 {
     if (!h || !d) return PTMC_E_ARG;
@@ -632,9 +819,9 @@ int ptmc_prepare(ptmc_handle h)
     if (!h->pot_set) return fail(h, PTMC_E_STATE, "no potential set");
     if (h->runs.empty()) return fail(h, PTMC_E_STATE, "no runs added");
     HIPCHK(h, hipSetDevice(h->cfg.device));
-    int rc = choose_pair(h);
+    int rc = is_nnp(h) ? prepare_nnp(h) : choose_pair(h);
     if (rc) return rc;
-    rc = plan(h);
+    if (!is_nnp(h)) rc = plan(h);
     if (rc) return rc;
     if (h->launches.size() > 1) {
         const size_t ns = std::min<size_t>(h->launches.size(), 4);
@@ -686,6 +873,12 @@ int ptmc_prepare(ptmc_handle h)
         tb += rh.d.n_temps;
     }
     h->n_temps = tb;
+    {
+        std::vector<int32_t> ts(tb);
+        for (auto &rh : h->runs)
+            for (int t = 0; t < rh.d.n_temps; ++t) ts[rh.info.temp_base + t] = rh.info.slot_base + t;
+        if ((rc = upload(h, h->d_temp_slot, ts))) return rc;
+    }
 
     if ((rc = upload(h, h->d_tiles, h->tiles))) return rc;
     if ((rc = upload(h, h->d_runs, infos))) return rc;
@@ -703,7 +896,13 @@ int ptmc_prepare(ptmc_handle h)
     if ((rc = dalloc(h, h->d_count_vol, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_count_exc, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_hist, (size_t)h->hist_stride * tb))) return rc;
-    if (h->cfg.precision == 0) {
+    if (is_nnp(h)) {
+        std::vector<float> buf;
+        scatter_coords_nnp(h, buf);
+        std::vector<unsigned char> raw(buf.size() * sizeof(float));
+        memcpy(raw.data(), buf.data(), raw.size());
+        if ((rc = upload(h, h->d_coords, raw))) return rc;
+    } else if (h->cfg.precision == 0) {
         std::vector<double> buf;
         scatter_coords<double>(h, buf);
         std::vector<unsigned char> raw(buf.size() * sizeof(double));
@@ -744,6 +943,41 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
     HIPCHK(h, hipSetDevice(h->cfg.device));
     int64_t done = 0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    if (is_nnp(h)) {
+        // one launch per cycle, then exchange / adaptation / sampling
+        const bool tracing = h->trace_pending > 0;
+        if (tracing && n_cycles > h->trace_cycles)
+            return fail(h, PTMC_E_STATE, "traced cycles must be run in one ptmc_cycles call of at most %d cycles",
+                        h->trace_cycles);
+        h->trace_pending = 0;
+        int n_runs = (int)h->runs.size();
+        for (int64_t c = 0; c < n_cycles; ++c) {
+            SweepArgs a = base_args(h);
+            a.sweep0 = h->sweep_counter;
+            a.cycle0 = h->phase_counter;
+            a.n_cycles = 1;
+            a.sampling = h->production;
+            a.init_run = -1;
+            if (tracing) {
+                a.trace_de = h->d_trace_de.p;
+                a.trace_acc = h->d_trace_acc.p;
+                a.trace_idx = h->d_trace_idx.p;
+                a.trace_cycles = h->trace_cycles;
+                a.trace_cycle = (int32_t)c;
+            }
+            if ((rc = launch_one(h, 0, a, h->stream, false))) return rc;
+            void *args[] = {&a, &n_runs};
+            HIPCHK(h, hipLaunchKernel((const void *)nnp_cycle_fn(), dim3((n_runs + 63) / 64), dim3(64), args, 0,
+                                      h->stream));
+            if (h->pending.size() >= 512) {
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                if ((rc = collect_timings(h))) return rc;
+            }
+            h->sweep_counter += 1;
+            h->phase_counter += 1;
+        }
+        done = n_cycles;
+    }
     while (done < n_cycles) {
         const int64_t chunk = std::min<int64_t>(n_cycles - done, h->cfg.cycles_per_launch);
         SweepArgs a = base_args(h);
@@ -859,7 +1093,15 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
         if (en) en[t] = e[s];
         if (box) box[t] = b[s];
     }
-    if (x) {
+    if (x && is_nnp(h)) {
+        std::vector<float> blk((size_t)R * 3 * NNP_NP);
+        HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float),
+                            blk.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int s = 0; s < R; ++s)
+            for (int j = 0; j < N; ++j)
+                for (int c = 0; c < 3; ++c)
+                    x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * NNP_NP + j];
+    } else if (x) {
         const TileInfo &ti = h->tiles[rh.tile];
         const int A = ti.n_slab, kl = rh.kl;
         const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
@@ -987,7 +1229,20 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
         for (int s = 0; s < R; ++s) temp[s] = s;
         HIPCHK(h, hipMemcpy(h->d_slot_temp.p + rh.info.slot_base, temp.data(), R * 4, hipMemcpyHostToDevice));
         if (box) HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, box, R * 8, hipMemcpyHostToDevice));
-        if (x) {
+        if (is_nnp(h)) {
+            std::vector<int32_t> ts(R);
+            for (int s = 0; s < R; ++s) ts[s] = rh.info.slot_base + s;
+            HIPCHK(h, hipMemcpy(h->d_temp_slot.p + rh.info.temp_base, ts.data(), R * 4, hipMemcpyHostToDevice));
+        }
+        if (x && is_nnp(h)) {
+            std::vector<float> blk((size_t)R * 3 * NNP_NP, 0.0f);
+            for (int s = 0; s < R; ++s)
+                for (int j = 0; j < N; ++j)
+                    for (int c = 0; c < 3; ++c)
+                        blk[((size_t)s * 3 + c) * NNP_NP + j] = (float)x[((size_t)s * N + j) * 3 + c];
+            HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float), blk.data(),
+                                blk.size() * sizeof(float), hipMemcpyHostToDevice));
+        } else if (x) {
             const TileInfo &ti = h->tiles[rh.tile];
             const int A = ti.n_slab, kl = rh.kl;
             const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
@@ -1060,7 +1315,7 @@ int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, in
 {
     if (!h) return PTMC_E_ARG;
     if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
-    if (n_tiles) *n_tiles = (int32_t)h->tiles.size();
+    if (n_tiles) *n_tiles = is_nnp(h) ? h->n_slots : (int32_t)h->tiles.size();
     if (n_launch_groups) *n_launch_groups = (int32_t)h->launches.size();
     if (n_lanes) {
         int64_t s = 0;
@@ -1110,6 +1365,7 @@ int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, in
 int ptmc_run_group(ptmc_handle h, int32_t run)
 {
     if (!h || run < 0 || run >= (int)h->runs.size() || !h->prepared) return -1;
+    if (is_nnp(h)) return 0;
     const int t = h->runs[run].tile;
     for (size_t k = 0; k < h->launches.size(); ++k)
         if (t >= h->launches[k].tile0 && t < h->launches[k].tile0 + h->launches[k].ntiles) return (int)k;

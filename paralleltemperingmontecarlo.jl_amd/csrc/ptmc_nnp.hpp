@@ -1,0 +1,534 @@
+// Sweep kernels for the RuNNer-style neural-network potential
+// (src/EnergyEvaluation.jl:1182-1275, :1593-1621, :1735-1747, :1812-1822;
+// MachineLearningPotential/{Cutoff,SymmFunc,DeltaMatrix}.jl).
+//
+// One workgroup owns one configuration for a whole cycle.  Its positions,
+// feature matrix G (nf x atoms) and per-atom energies live in LDS.  A move
+// builds the change D of G caused by moving atom i (radial terms by pair,
+// angular terms by partner pair (j, k) with the exponentials shared by all
+// functions of one eta), evaluates the network on G + D for every atom, and
+// on acceptance folds D into G.  Exchange, step adaptation and sampling run
+// between cycles in nnp_cycle_kernel, so one launch of the sweep kernel is
+// one cycle of every configuration.
+#pragma once
+#include "ptmc_sweep.hpp"
+
+#include "ptmc_nnp_layout.hpp"
+
+namespace ptmc {
+
+struct NnpLds {
+    float *X, *G, *D, *W1, *H1, *H2, *rn2, *fn, *ro2, *fo, *eat, *eatn;
+};
+
+__device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
+{
+    NnpLds l;
+    l.X = base;
+    l.G = l.X + 3 * NNP_NP;
+    l.D = l.G + nf * NNP_NP;
+    l.W1 = l.D + nf * NNP_NP;
+    l.H1 = l.W1 + nf * h1;
+    l.H2 = l.H1 + NNP_NP * NNP_HS;
+    l.rn2 = l.H2 + NNP_NP * NNP_HS;
+    l.fn = l.rn2 + NNP_NP;
+    l.ro2 = l.fn + NNP_NP;
+    l.fo = l.ro2 + NNP_NP;
+    l.eat = l.fo + NNP_NP;
+    l.eatn = l.eat + NNP_NP;
+    return l;
+}
+
+// type-2 cutoff, MachineLearningPotential/Cutoff.jl:79-90
+__device__ __forceinline__ float nnp_cutoff(float r, float rc)
+{
+    return r < rc ? 0.5f * (cosf(3.14159265358979323846f * (r / rc)) + 1.0f) : 0.0f;
+}
+
+// (1 + lambda cos)^zeta for integral zeta by squaring, else powf
+__device__ __forceinline__ float nnp_theta(float lam, float zeta, float c)
+{
+    float b = 1.0f + lam * c;
+    const int z = (int)zeta;
+    if ((float)z == zeta && z >= 0 && z <= 64) {
+        float r = 1.0f;
+        int e = z;
+        while (e) {
+            if (e & 1) r *= b;
+            b *= b;
+            e >>= 1;
+        }
+        return r;
+    }
+    return powf(b, zeta);
+}
+
+// cosines of the triangle (a, b, c) at a, b and c (Cutoff.jl:48-58)
+__device__ __forceinline__ void nnp_angles(const float *a, const float *b, const float *c, float dab, float dac,
+                                           float dbc, float &ta, float &tb, float &tc)
+{
+    const float abx = a[0] - b[0], aby = a[1] - b[1], abz = a[2] - b[2];
+    const float acx = a[0] - c[0], acy = a[1] - c[1], acz = a[2] - c[2];
+    const float bcx = b[0] - c[0], bcy = b[1] - c[1], bcz = b[2] - c[2];
+    ta = (abx * acx + aby * acy + abz * acz) / (dab * dac);
+    tb = (-abx * bcx - aby * bcy - abz * bcz) / (dab * dbc);
+    tc = (acx * bcx + acy * bcy + acz * bcz) / (dac * dbc);
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// partner pair p of the N(N-1)/2 pairs j < k
+__device__ __forceinline__ void nnp_pair(int p, int N, int &j, int &k)
+{
+    // rows of length N-1, N-2, ...: find j with start(j) <= p < start(j+1)
+    int jj = 0, start = 0;
+    while (start + (N - 1 - jj) <= p) {
+        start += N - 1 - jj;
+        ++jj;
+    }
+    j = jj;
+    k = jj + 1 + (p - start);
+}
+
+// Full feature matrix from positions into L.G (total_symm_calc,
+// SymmFunc.jl:494-510; radial :259-278, angular :280-312).  All threads call.
+__device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, int N)
+{
+    const int tid = threadIdx.x;
+    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] = 0.0f;
+    __syncthreads();
+    // radial: one (function, atom) per item
+    for (int e = tid; e < T.n_rad_active * N; e += NNP_NT) {
+        const int q = e / N, a = e % N;
+        const float eta = T.rad[4 * q], norm = T.rad[4 * q + 1];
+        const int row = (int)T.rad[4 * q + 2];
+        float s = 0.0f;
+        for (int b = 0; b < N; ++b) {
+            if (b == a) continue;
+            const float dx = L.X[a] - L.X[b], dy = L.X[NNP_NP + a] - L.X[NNP_NP + b],
+                        dz = L.X[2 * NNP_NP + a] - L.X[2 * NNP_NP + b];
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float fc = nnp_cutoff(sqrtf(r2), T.r_cut);
+            if (fc != 0.0f && fc != 1.0f) s += fc * expf(-eta * r2) * norm;
+        }
+        L.G[row * NNP_NP + a] = s;
+    }
+    // angular: one triangle a < b < c per item, all functions
+    const int ntri = N * (N - 1) * (N - 2) / 6;
+    for (int e = tid; e < ntri; e += NNP_NT) {
+        // decode e -> (a, b, c)
+        int a = 0, rem = e;
+        while (rem >= (N - 1 - a) * (N - 2 - a) / 2) {
+            rem -= (N - 1 - a) * (N - 2 - a) / 2;
+            ++a;
+        }
+        int b = a + 1;
+        while (rem >= N - 1 - b) {
+            rem -= N - 1 - b;
+            ++b;
+        }
+        const int c = b + 1 + rem;
+        const float pa[3] = {L.X[a], L.X[NNP_NP + a], L.X[2 * NNP_NP + a]};
+        const float pb[3] = {L.X[b], L.X[NNP_NP + b], L.X[2 * NNP_NP + b]};
+        const float pc[3] = {L.X[c], L.X[NNP_NP + c], L.X[2 * NNP_NP + c]};
+        auto d2 = [](const float *u, const float *v) {
+            const float x = u[0] - v[0], y = u[1] - v[1], z = u[2] - v[2];
+            return x * x + y * y + z * z;
+        };
+        const float rab = d2(pa, pb), rac = d2(pa, pc), rbc = d2(pb, pc);
+        const float dab = sqrtf(rab), dac = sqrtf(rac), dbc = sqrtf(rbc);
+        const float fp = nnp_cutoff(dab, T.r_cut) * nnp_cutoff(dac, T.r_cut) * nnp_cutoff(dbc, T.r_cut);
+        if (fp == 0.0f) continue;
+        float ta, tb, tc;
+        nnp_angles(pa, pb, pc, dab, dac, dbc, ta, tb, tc);
+        const float rs = rab + rac + rbc;
+        for (int g = 0; g < T.n_groups; ++g) {
+            const float ex = expf(-T.grp_eta[g] * rs) * fp;
+            const int f0 = T.grp[2 * g], cnt = T.grp[2 * g + 1];
+            for (int s = 0; s < cnt; ++s) {
+                const int f = f0 + s;
+                if (!T.abuild[f]) continue;
+                const float lam = T.afn[4 * f], zeta = T.afn[4 * f + 1], tpz = T.afn[4 * f + 2];
+                const int row = (int)T.afn[4 * f + 3];
+                atomicAdd(&L.G[row * NNP_NP + a], ex * nnp_theta(lam, zeta, ta) * tpz);
+                atomicAdd(&L.G[row * NNP_NP + b], ex * nnp_theta(lam, zeta, tb) * tpz);
+                atomicAdd(&L.G[row * NNP_NP + c], ex * nnp_theta(lam, zeta, tc) * tpz);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < T.nf * N; e += NNP_NT) {
+        const int q = e / N, a = e % N;
+        L.G[q * NNP_NP + a] += T.row_off[q];
+    }
+    __syncthreads();
+}
+
+// Per-atom network energies of G (+ D if given) into out[N]
+// (88 -> tanh 20 -> tanh 20 -> linear 1, ForwardPass.jl:23-39).  All threads call.
+__device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *D, int N, float *out)
+{
+    const int tid = threadIdx.x;
+    const int nf = T.nf, h1 = T.h1, h2 = T.h2;
+    for (int e = tid; e < N * h1; e += NNP_NT) {
+        const int a = e / h1, h = e % h1;
+        float z = 0.0f;
+        if (D) {
+            for (int q = 0; q < nf; ++q) z += (L.G[q * NNP_NP + a] + D[q * NNP_NP + a]) * L.W1[q * h1 + h];
+        } else {
+            for (int q = 0; q < nf; ++q) z += L.G[q * NNP_NP + a] * L.W1[q * h1 + h];
+        }
+        L.H1[a * NNP_HS + h] = tanhf(z + T.b1[h]);
+    }
+    __syncthreads();
+    for (int e = tid; e < N * h2; e += NNP_NT) {
+        const int a = e / h2, h = e % h2;
+        float z = 0.0f;
+        for (int k = 0; k < h1; ++k) z += L.H1[a * NNP_HS + k] * T.w2[k * h2 + h];
+        L.H2[a * NNP_HS + h] = tanhf(z + T.b2[h]);
+    }
+    __syncthreads();
+    for (int a = tid; a < N; a += NNP_NT) {
+        float z = 0.0f;
+        for (int k = 0; k < h2; ++k) z += L.H2[a * NNP_HS + k] * T.w3[k];
+        out[a] = z + T.b3;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, int slot)
+{
+    const NnpTables &T = a.nnp;
+    const float *gX = reinterpret_cast<const float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
+    for (int e = threadIdx.x; e < 3 * NNP_NP; e += NNP_NT) L.X[e] = gX[e];
+    for (int e = threadIdx.x; e < T.nf * T.h1; e += NNP_NT) L.W1[e] = T.w1[e];
+}
+
+// set_variables + initialise_energy (src/EnergyEvaluation.jl:1812-1822,
+// :1735-1747): features, per-atom energies and the FP64 total of each slot.
+__global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const NnpTables &T = a.nnp;
+    const int slot = blockIdx.x;
+    if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;  // whole workgroup
+    const NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    const int N = a.runs[a.slot_run[slot]].n_atoms;
+    nnp_load(a, L, slot);
+    __syncthreads();
+    nnp_full_build(T, L, N);
+    nnp_forward(T, L, nullptr, N, L.eat);
+    float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
+    for (int e = threadIdx.x; e < T.nf * NNP_NP; e += NNP_NT) gG[e] = L.G[e];
+    for (int e = threadIdx.x; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = e < N ? L.eat[e] : 0.0f;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int e = 0; e < N; ++e) s += (double)L.eat[e];
+        a.slot_en[slot] = s;
+    }
+}
+
+// One Monte-Carlo cycle (N moves) of one configuration per workgroup
+// (mc_step!, src/MCRun.jl:169-182, with energy_update!(RuNNerPotential)).
+__global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int s_flag, s_err;
+    __shared__ uint16_t s_pairs[NNP_NP * (NNP_NP - 1) / 2];
+    __shared__ double s_en;
+    __shared__ float s_trial[3], s_old[3];
+    const NnpTables &T = a.nnp;
+    const int slot = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    const RunInfo &ri = a.runs[a.slot_run[slot]];
+    const int N = ri.n_atoms;
+    const uint32_t nsteps = (uint32_t)N;
+    const int t = a.slot_temp[slot];
+    const int Tg = ri.temp_base + t;
+    const uint32_t sweep = (uint32_t)a.sweep0;
+    const double beta = a.t_beta[Tg];
+    const float delta_r = (float)a.t_displ[2 * Tg];
+    const float radius2 = (float)ri.radius2;
+
+    nnp_load(a, L, slot);
+    const float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
+    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] = gG[e];
+    for (int e = tid; e < NNP_NP; e += NNP_NT) L.eat[e] = a.eatom[(size_t)slot * NNP_NP + e];
+    double E = a.slot_en[slot];
+    if (tid == 0) s_err = 0;
+    const int npairs = N * (N - 1) / 2;
+    for (int p = tid; p < npairs; p += NNP_NT) {
+        int j, k;
+        nnp_pair(p, N, j, k);
+        s_pairs[p] = (uint16_t)((j << 8) | k);
+    }
+    __syncthreads();
+    int nacc = 0;
+
+    for (uint32_t m = 0; m < nsteps; ++m) {
+        // draws and trial position (every thread the same), src/MCMoves.jl:115-135
+        MoveDraw<float> d0 = move_draw<float>(a.seed, ri.run_key, (uint32_t)t, sweep, m, 0u, nsteps);
+        const int i = (int)d0.index;
+        float xi = L.X[i], yi = L.X[NNP_NP + i], zi = L.X[2 * NNP_NP + i];
+        float tx, ty, tz;
+        make_trial<float, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, 0.0f, 0.0f, tx, ty, tz);
+        if (outside_sphere<float>(tx, ty, tz, radius2)) {
+            int count = 0;
+            uint32_t attempt = 0;
+            while (outside_sphere<float>(tx, ty, tz, radius2)) {
+                count += 1;
+                if (count == 50) {
+                    __syncthreads();
+                    if (tid == 0) {
+                        double c[3] = {0.0, 0.0, 0.0};
+                        for (int b = 0; b < N; ++b)
+                            for (int k = 0; k < 3; ++k) c[k] += (double)L.X[k * NNP_NP + b];
+                        for (int k = 0; k < 3; ++k) c[k] /= (double)N;
+                        for (int b = 0; b < N; ++b)
+                            for (int k = 0; k < 3; ++k)
+                                L.X[k * NNP_NP + b] = (float)((double)L.X[k * NNP_NP + b] - c[k]);
+                    }
+                    __syncthreads();
+                } else {
+                    attempt += 1;
+                    MoveDraw<float> dd = move_draw<float>(a.seed, ri.run_key, (uint32_t)t, sweep, m, attempt, nsteps);
+                    xi = L.X[i];
+                    yi = L.X[NNP_NP + i];
+                    zi = L.X[2 * NNP_NP + i];
+                    make_trial<float, BC_SPHERE, false>(xi, yi, zi, dd, delta_r, 0.0f, 0.0f, tx, ty, tz);
+                    if (count == 100) {
+                        if (tid == 0) s_err = 1;
+                        break;
+                    }
+                }
+            }
+            xi = L.X[i];
+            yi = L.X[NNP_NP + i];
+            zi = L.X[2 * NNP_NP + i];
+        }
+        const float pt[3] = {tx, ty, tz}, po[3] = {xi, yi, zi};
+        // distances and cutoffs to the new and old position; hard-core test
+        if (tid == 0) s_flag = 0;
+        __syncthreads();
+        for (int j = tid; j < N; j += NNP_NT) {
+            const float px = L.X[j], py = L.X[NNP_NP + j], pz = L.X[2 * NNP_NP + j];
+            float dx = tx - px, dy = ty - py, dz = tz - pz;
+            const float rn = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
+            dx = xi - px;
+            dy = yi - py;
+            dz = zi - pz;
+            const float ro = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
+            L.rn2[j] = rn;
+            L.ro2[j] = ro;
+            L.fn[j] = nnp_cutoff(sqrtf(rn), T.r_cut);
+            L.fo[j] = nnp_cutoff(sqrtf(ro), T.r_cut);
+            if (j != i && (double)rn < T.hardcore2) s_flag = 1;
+        }
+        for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.D[e] = 0.0f;
+        __syncthreads();
+        double new_en = 100.0;  // the hard-core sentinel, src/EnergyEvaluation.jl:1604-1606
+        if (!s_flag) {
+            // radial changes, DeltaMatrix.jl:794-822: one wave per function
+            for (int q = tid >> 6; q < T.n_rad_active; q += NNP_NT / 64) {
+                const float eta = T.rad[4 * q], norm = T.rad[4 * q + 1];
+                const int row = (int)T.rad[4 * q + 2];
+                float d = 0.0f;
+                if (lane < N && lane != i)
+                    d = expf(-eta * L.rn2[lane]) * (L.fn[lane] * norm) + expf(-eta * L.ro2[lane]) * (-L.fo[lane] * norm);
+                if (lane < N && lane != i) L.D[row * NNP_NP + lane] = d;
+                const float s = wave_sum(d);
+                if (lane == 0) L.D[row * NNP_NP + i] = s;
+            }
+            // angular changes, DeltaMatrix.jl:827-872: partner pairs (j, k), j, k != i
+            float rsn[NNP_PP], rso[NNP_PP], fpn[NNP_PP], fpo[NNP_PP];
+            float cn[NNP_PP][3], co[NNP_PP][3];
+            int pj[NNP_PP], pk[NNP_PP];
+#pragma unroll
+            for (int r = 0; r < NNP_PP; ++r) {
+                const int p = tid + r * NNP_NT;
+                fpn[r] = 0.0f;
+                fpo[r] = 0.0f;
+                rsn[r] = rso[r] = 0.0f;
+                pj[r] = pk[r] = 0;
+                cn[r][0] = cn[r][1] = cn[r][2] = co[r][0] = co[r][1] = co[r][2] = 0.0f;
+                if (p >= npairs) continue;
+                const int j = s_pairs[p] >> 8, k = s_pairs[p] & 0xff;
+                if (j == i || k == i) continue;
+                const float pj3[3] = {L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j]};
+                const float pk3[3] = {L.X[k], L.X[NNP_NP + k], L.X[2 * NNP_NP + k]};
+                const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
+                const float rjk = dx * dx + dy * dy + dz * dz;
+                const float djk = sqrtf(rjk);
+                const float fjk = nnp_cutoff(djk, T.r_cut);
+                if (fjk == 0.0f) continue;
+                pj[r] = j;
+                pk[r] = k;
+                fpn[r] = L.fn[j] * L.fn[k] * fjk;
+                fpo[r] = L.fo[j] * L.fo[k] * fjk;
+                rsn[r] = L.rn2[j] + L.rn2[k] + rjk;
+                rso[r] = L.ro2[j] + L.ro2[k] + rjk;
+                if (fpn[r] != 0.0f)
+                    nnp_angles(pt, pj3, pk3, sqrtf(L.rn2[j]), sqrtf(L.rn2[k]), djk, cn[r][0], cn[r][1], cn[r][2]);
+                if (fpo[r] != 0.0f)
+                    nnp_angles(po, pj3, pk3, sqrtf(L.ro2[j]), sqrtf(L.ro2[k]), djk, co[r][0], co[r][1], co[r][2]);
+            }
+            for (int g = 0; g < T.n_groups; ++g) {
+                const float eta = T.grp_eta[g];
+                const int f0 = T.grp[2 * g], cnt = T.grp[2 * g + 1];
+                float acc[NNP_GMAX];
+#pragma unroll
+                for (int s = 0; s < NNP_GMAX; ++s) acc[s] = 0.0f;
+#pragma unroll
+                for (int r = 0; r < NNP_PP; ++r) {
+                    if (fpn[r] == 0.0f && fpo[r] == 0.0f) continue;
+                    const float en = expf(-eta * rsn[r]) * fpn[r];
+                    const float eo = expf(-eta * rso[r]) * fpo[r];
+#pragma unroll
+                    for (int s = 0; s < NNP_GMAX; ++s) {
+                        if (s >= cnt) break;
+                        const int f = f0 + s;
+                        const float lam = T.afn[4 * f], zeta = T.afn[4 * f + 1], tpz = T.afn[4 * f + 2];
+                        const int row = (int)T.afn[4 * f + 3];
+                        const float di = en * nnp_theta(lam, zeta, cn[r][0]) * tpz - eo * nnp_theta(lam, zeta, co[r][0]) * tpz;
+                        const float dj = en * nnp_theta(lam, zeta, cn[r][1]) * tpz - eo * nnp_theta(lam, zeta, co[r][1]) * tpz;
+                        const float dk = en * nnp_theta(lam, zeta, cn[r][2]) * tpz - eo * nnp_theta(lam, zeta, co[r][2]) * tpz;
+                        acc[s] += di;
+                        atomicAdd(&L.D[row * NNP_NP + pj[r]], dj);
+                        atomicAdd(&L.D[row * NNP_NP + pk[r]], dk);
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NNP_GMAX; ++s) {
+                    if (s >= cnt) break;
+                    const float v = wave_sum(acc[s]);
+                    if (lane == 0 && v != 0.0f) atomicAdd(&L.D[(int)T.afn[4 * (f0 + s) + 3] * NNP_NP + i], v);
+                }
+            }
+            __syncthreads();
+            nnp_forward(T, L, L.D, N, L.eatn);
+            if (tid == 0) {
+                double s = 0.0;
+                for (int e = 0; e < N; ++e) s += (double)L.eatn[e];
+                s_en = s;
+            }
+            __syncthreads();
+            new_en = s_en;
+        }
+        // Metropolis, src/MCRun.jl:122-132 and src/Exchange.jl:53-57
+        const double dd = new_en - E;
+        const bool accept = (dd <= 0.0) || (det_exp(-dd * beta) >= (double)d0.umet);
+        if (tid == 0 && a.trace_de != nullptr && a.trace_cycle < a.trace_cycles) {
+            const size_t ti = ((size_t)Tg * a.trace_cycles + a.trace_cycle) * nsteps + m;
+            a.trace_de[ti] = dd;
+            a.trace_acc[ti] = (int8_t)accept;
+            a.trace_idx[ti] = i;
+        }
+        if (accept) {
+            // swap_atom_config! + swap_vars!(NNPVariables), src/swap_config.jl:40-51, :213-221
+            for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] += L.D[e];
+            for (int e = tid; e < N; e += NNP_NT) L.eat[e] = L.eatn[e];
+            if (tid == 0) {
+                L.X[i] = tx;
+                L.X[NNP_NP + i] = ty;
+                L.X[2 * NNP_NP + i] = tz;
+            }
+            E = new_en;
+            nacc += 1;
+        }
+        __syncthreads();
+    }
+
+    // periodic full rebuild bounds the drift of the incremental features
+    if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
+        nnp_full_build(T, L, N);
+        nnp_forward(T, L, nullptr, N, L.eat);
+        if (tid == 0) {
+            double s = 0.0;
+            for (int e = 0; e < N; ++e) s += (double)L.eat[e];
+            s_en = s;
+        }
+        __syncthreads();
+        const double dr = __builtin_fabs(E - s_en);
+        if (tid == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
+        E = s_en;
+    }
+
+    float *gX = reinterpret_cast<float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
+    float *gGw = a.feat + (size_t)slot * T.nf * NNP_NP;
+    for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) gX[e] = L.X[e];
+    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) gGw[e] = L.G[e];
+    for (int e = tid; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = L.eat[e];
+    if (tid == 0) {
+        a.slot_en[slot] = E;
+        a.t_count_atom[2 * Tg] += nacc;
+        a.t_count_atom[2 * Tg + 1] += nacc;
+        if (s_err) a.run_error[a.slot_run[slot]] = 1;
+    }
+}
+
+// Between cycles: replica exchange (src/Exchange.jl:103-155, src/MCRun.jl:203-205),
+// step adaptation (src/Exchange.jl:198-263) and sampling (src/Sampling.jl:330-371),
+// one thread per run.
+__global__ void nnp_cycle_kernel(SweepArgs a, int n_runs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_runs) return;
+    const RunInfo &ri = a.runs[r];
+    const int R = ri.n_temps, tb = ri.temp_base, N = ri.n_atoms;
+    const uint32_t sweep = (uint32_t)a.sweep0;
+    const long long pidx = a.cycle0 + 1;
+    auto try_pair = [&](int n) {
+        u32x4 X1 = draw(a.seed, ri.run_key, RNG_EXCH, 0u, 0u, sweep, 1u + (uint32_t)n);
+        const double u = u53(X1.x, X1.y);
+        const int sa = a.temp_slot[tb + n], sb = a.temp_slot[tb + n + 1];
+        const double de = a.slot_en[sa] - a.slot_en[sb];
+        const double db = a.t_beta[tb + n] - a.t_beta[tb + n + 1];
+        double acc = det_exp(db * de);
+        if (acc > 1.0) acc = 1.0;
+        a.t_count_exc[2 * (tb + n)] += 1;
+        a.t_count_exc[2 * (tb + n + 1)] += 1;
+        if (acc > u) {
+            a.t_count_exc[2 * (tb + n) + 1] += 1;
+            a.t_count_exc[2 * (tb + n + 1) + 1] += 1;
+            a.slot_temp[sa] = n + 1;
+            a.slot_temp[sb] = n;
+            a.temp_slot[tb + n] = sb;
+            a.temp_slot[tb + n + 1] = sa;
+        }
+    };
+    if (R > 1) {
+        if (a.exchange_mode == EXC_EVEN_ODD) {
+            for (int n = (int)(sweep & 1u); n + 1 < R; n += 2) try_pair(n);
+        } else {
+            u32x4 X0 = draw(a.seed, ri.run_key, RNG_EXCH, 0u, 0u, sweep, 0u);
+            const double u = u53(X0.x, X0.y);
+            if (a.exchange_mode == EXC_EVERY || u < 0.1) try_pair((int)below(X0.z, (uint32_t)(R - 1)));
+        }
+    }
+    for (int t = 0; t < R; ++t) {
+        const int T = tb + t;
+        if (pidx % a.n_adjust == 0) {
+            const double ar = (double)a.t_count_atom[2 * T + 1] / (double)((long long)a.n_adjust * N);
+            if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
+            else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
+            a.t_count_atom[2 * T + 1] = 0;
+        }
+        if (a.sampling && pidx % a.mc_sample == 0) {
+            const double E = a.slot_en[a.temp_slot[T]];
+            a.t_ham[2 * T] += E;
+            a.t_ham[2 * T + 1] += E * E;
+            const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
+            const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
+            a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+        }
+    }
+}
+
+}  // namespace ptmc

@@ -10,6 +10,7 @@ enum PotKind : int32_t {
     POT_ELJ_EVEN = 0,  // sum_k c_k r^-(4+2k), src/EnergyEvaluation.jl:138-153, :934-942
     POT_ELJ = 1,       // sum_k c_k r^-(5+k),  src/EnergyEvaluation.jl:891-933
     POT_EAM = 2,       // Sutton-Chen,          src/EnergyEvaluation.jl:1059-1162
+    POT_NNP = 3,       // RuNNer network,       src/EnergyEvaluation.jl:1182-1275
 };
 // How a pair energy is evaluated on the device.
 enum PairForm : int32_t {
@@ -52,6 +53,26 @@ struct TileInfo {
     int64_t comp_off;   // EAM: element offset of the tile's [n_slab][2][WGS] phi/rho block
 };
 
+// Neural-network potential tables (device memory), prepared on the host:
+// active angular functions sorted into groups of equal eta (at most 8 per
+// group), network weights in RuNNer order.
+struct NnpTables {
+    int32_t n_rad, n_ang, nf;      // radial, angular, total feature rows
+    int32_t h1, h2;                // hidden widths (input nf, output 1)
+    int32_t n_rad_active;          // radial rows that contribute
+    int32_t n_groups;              // angular eta groups
+    float r_cut;
+    double hardcore2;              // squared hard-core distance (0: none)
+    const float *rad;              // n_rad_active x 4: eta, norm, row, pad
+    const int32_t *grp;            // n_groups x 2: first function, count
+    const float *grp_eta;          // n_groups
+    const float *afn;              // functions x 4: lambda, zeta, tpz, row
+    const int32_t *abuild;         // functions: 1 if built by the full evaluation
+    const float *row_off;          // nf: offset added by the full evaluation
+    const float *w1, *b1, *w2, *b2, *w3;
+    float b3;
+};
+
 struct SweepArgs {
     const TileInfo *tiles;
     const RunInfo *runs;
@@ -88,7 +109,13 @@ struct SweepArgs {
     int32_t sampling;
     int32_t recompute_every;
     int32_t trace_cycles;
+    int32_t trace_cycle;       // one-cycle launches: index of this cycle in the trace
     int32_t init_run;          // init kernels: run whose energies are initialised (-1: all)
+    // neural-network potential (one workgroup per slot)
+    NnpTables nnp;
+    float *feat;               // per slot [nf][NNP_NP] features
+    float *eatom;              // per slot [NNP_NP] per-atom energies
+    int32_t *temp_slot;        // per temperature: slot holding it
     double min_acc, max_acc;
 };
 

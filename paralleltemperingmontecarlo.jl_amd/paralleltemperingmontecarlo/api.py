@@ -389,6 +389,16 @@ def _set_potential(eng: _lib.Engine, pot):
         eng.call("ptmc_set_potential_elj", 1 if isinstance(pot, ELJPotentialEven) else 0, len(c), _lib.ptr(c))
     elif isinstance(pot, EmbeddedAtomPotential):
         eng.call("ptmc_set_potential_eam", pot.n, pot.m, pot.ean, pot.eCam)
+    elif isinstance(pot, RuNNerPotential):
+        rad, ang = pot.tables()
+        nodes = np.ascontiguousarray(pot.nnp.num_nodes, dtype=np.int32)
+        act = np.ascontiguousarray(pot.nnp.activation_functions, dtype=np.int32)
+        par = np.ascontiguousarray(pot.nnp.parameters, dtype=np.float64)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        ang = np.ascontiguousarray(ang, dtype=np.float64)
+        eng.call("ptmc_set_potential_nnp", len(rad), len(ang), _lib.ptr(rad), _lib.ptr(ang), pot.r_cut,
+                 pot.boundary, 1 if pot.q10_compat else 0, len(nodes), _lib.ptr(nodes, C.c_int32),
+                 _lib.ptr(act, C.c_int32), len(par), _lib.ptr(par))
     else:
         raise TypeError(f"potential {type(pot).__name__} is not handled by the pair engine")
 

@@ -1,0 +1,122 @@
+"""The network potential on the device (binary32 features and network, FP64
+totals) against the reference's Cu55 energies and the FP64 oracle, through
+the C ABI.  Tolerances: start energies within 2e-6 of |E| of the reference
+values; per-move dE within 2e-6 of |E| of the oracle's FP64 dE on the same
+positions (the replay takes the device's decisions, so the positions must
+then agree bit for bit)."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+import paralleltemperingmontecarlo as pt  # noqa: E402
+from paralleltemperingmontecarlo.api import PTEngine  # noqa: E402
+from paralleltemperingmontecarlo.nnp_setup import (read_save_data, reference_cu55_potential,  # noqa: E402
+                                                    synthetic_cu55_potential)
+
+NNP = GOLDEN / "nnp"
+A2B = 1.8897259886
+RADIUS = 14 * A2B  # test/test_Cu55.jl:181
+
+
+@pytest.fixture(scope="module")
+def saved():
+    return read_save_data(NNP / "save.data")
+
+
+def _orc(oracle, pot):
+    rad, ang = pot.tables()
+    return oracle.NNP(rad, ang, pot.r_cut, pot.nnp.num_nodes, pot.nnp.activation_functions, pot.nnp.parameters,
+                      hardcore2=pot.boundary, q10_compat=pot.q10_compat)
+
+
+def _engine(pot, R, n_atoms=55, **kw):
+    opts = dict(precision="fp32", exchange="even_odd", seed=77)
+    opts.update(kw)
+    return PTEngine(pot, pt.NVT(n_atoms), pt.SphericalBC(RADIUS), pt.MCParams(10, R, n_atoms, n_adjust=5),
+                    pt.EngineOptions(**opts))
+
+
+def test_start_energies_match_reference(saved):
+    """The 20 saved Cu55 configurations as 20 temperatures of one run: the
+    device start energies reproduce the reference's total energies."""
+    pot = reference_cu55_potential(NNP / "scaling.data", NNP / "weights.029.data")
+    eng = _engine(pot, 20)
+    x = np.stack([s[0] for s in saved])
+    run = eng.add_run(55, np.full(20, 1000.0), x, pt.SphericalBC(RADIUS))
+    eng.prepare()
+    e = eng.energies(run)
+    ref = np.array([s[1] for s in saved])
+    assert np.max(np.abs(e - ref) / np.abs(ref)) < 2e-6, np.max(np.abs(e - ref))
+    eng.close()
+
+
+@pytest.mark.parametrize("q10", [False, True])
+def test_start_energy_matches_oracle(oracle, saved, q10):
+    """Both feature-matrix builds (every row, or the reference's
+    total_symm_calc rows) and the synthetic 8 + 80 potential."""
+    for pot in (reference_cu55_potential(NNP / "scaling.data", NNP / "weights.029.data", q10_compat=q10),
+                synthetic_cu55_potential(NNP / "scaling.data", q10_compat=q10)):
+        eng = _engine(pot, 2)
+        x = np.stack([saved[0][0], saved[7][0]])
+        run = eng.add_run(55, np.full(2, 1000.0), x, pt.SphericalBC(RADIUS))
+        eng.prepare()
+        e = eng.energies(run)
+        nn = _orc(oracle, pot)
+        ref = np.array([nn.energy(nn.full_g(xx))[0] for xx in x])
+        assert np.max(np.abs(e - ref) / np.abs(ref)) < 2e-6, (q10, e, ref)
+        eng.close()
+
+
+def test_moves_replay_against_oracle(oracle, saved):
+    """4 temperatures (300-1200 K), 3 cycles: every device dE agrees with the
+    oracle's FP64 dE (moved-atom feature update + network) on the same
+    binary32 positions, and the replayed positions agree bit for bit."""
+    pot = reference_cu55_potential(NNP / "scaling.data", NNP / "weights.029.data")
+    temp = pt.TempGrid(300.0, 1200.0, 4)
+    eng = _engine(pot, 4, seed=2024)
+    x0 = saved[0][0]
+    run = eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(RADIUS), run_key=9)
+    eng.prepare()
+    ncyc = 3
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc = oracle.OracleRun(55, temp.beta_grid, x0, pot=oracle.make_pot("nnp", nnp=_orc(oracle, pot)), radius=RADIUS,
+                           mode=oracle.engine_mode(1), fp32=True, exchange_mode=2, n_adjust=5, seed=2024, run_key=9)
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, replay_acc=acc)
+    assert np.array_equal(idx, tr["idx"])
+    scale = abs(orc.en_tot[0])
+    err = np.abs(de - tr["de"])
+    assert err.max() <= 2e-6 * scale, err.max() / scale
+    assert acc.mean() > 0.1
+    x_dev, en_dev, _ = eng.state(run)
+    assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
+    assert np.max(np.abs(en_dev - orc.en_tot)) <= 2e-6 * scale
+    ctr = eng.counters(run)
+    assert np.array_equal(ctr["count_exc"], orc.count_exc)
+    eng.close()
+
+
+def test_recompute_bounds_the_feature_drift(saved):
+    """With a full rebuild every cycle the running energy never departs from
+    the rebuilt one by more than the binary32 error."""
+    pot = synthetic_cu55_potential(NNP / "scaling.data")
+    temp = pt.TempGrid(200.0, 1400.0, 8)
+    eng = _engine(pot, 8, recompute_every=1)
+    run = eng.add_run(55, temp.beta_grid, saved[0][0][None], pt.SphericalBC(RADIUS))
+    eng.prepare()
+    eng.start_phase(False)
+    eng.cycles(4)
+    eng.sync()
+    e = eng.energies(run)
+    assert np.max(eng.drift(run)) < 2e-6 * np.max(np.abs(e))
+    eng.close()
