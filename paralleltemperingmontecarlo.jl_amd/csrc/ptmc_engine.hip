@@ -738,6 +738,7 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
         return fail(h, PTMC_E_UNSUPPORTED, "network shape must be [n_features, h1 <= %d (tanh), h2 <= %d (tanh), 1 (linear)]",
                     NNP_MAXH, NNP_MAXH);
     const int nf = n_rad + n_ang, h1 = nodes[1], h2 = nodes[2];
+    if (nf > NNP_MAXF) return fail(h, PTMC_E_UNSUPPORTED, "at most %d symmetry functions", NNP_MAXF);
     if (n_params != nf * h1 + h1 + h1 * h2 + h2 + h2 + 1) return fail(h, PTMC_E_ARG, "parameter count does not match the layers");
     auto &n = h->nnp;
     n = {};
@@ -769,7 +770,14 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
         int cnt = 0;
         while (k < idx.size() && ang[6 * idx[k]] == eta && cnt < NNP_GMAX) {
             const double *a = ang + 6 * idx[k];
-            n.afn.insert(n.afn.end(), {(float)a[1], (float)a[2], (float)a[3], (float)(n_rad + idx[k])});
+            // zeta = 2^k is stored as k (k squarings)
+            float zcode = -1.0f;
+            for (int e = 0; e <= 6; ++e)
+                if (a[2] == (double)(1 << e)) zcode = (float)e;
+            if (zcode < 0.0f)
+                return fail(h, PTMC_E_UNSUPPORTED, "angular function %d: zeta = %g is not a power of two up to 64",
+                            idx[k], a[2]);
+            n.afn.insert(n.afn.end(), {(float)a[1], zcode, (float)a[3], (float)(n_rad + idx[k])});
             n.abuild.push_back(idx[k] < n_built ? 1 : 0);
             if (idx[k] < n_built) n.rowoff[n_rad + idx[k]] = (float)a[4];
             ++cnt;

@@ -19,7 +19,15 @@ namespace ptmc {
 
 struct NnpLds {
     float *X, *G, *D, *W1, *H1, *H2, *rn2, *fn, *ro2, *fo, *eat, *eatn;
+    // function tables copied from the host-prepared arrays
+    float *rad, *afn, *geta;
+    int32_t *grp, *abuild;
 };
+
+// static LDS holding the function tables (declared in each kernel)
+#define NNP_TABLES_LDS                                                         \
+    __shared__ float s_rad[4 * NNP_MAXF], s_afn[4 * NNP_MAXF], s_geta[NNP_MAXF]; \
+    __shared__ int32_t s_grp[2 * NNP_MAXF], s_abuild[NNP_MAXF]
 
 __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
 {
@@ -36,6 +44,8 @@ __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
     l.fo = l.ro2 + NNP_NP;
     l.eat = l.fo + NNP_NP;
     l.eatn = l.eat + NNP_NP;
+    l.rad = l.afn = l.geta = nullptr;
+    l.grp = l.abuild = nullptr;
     return l;
 }
 
@@ -45,22 +55,20 @@ __device__ __forceinline__ float nnp_cutoff(float r, float rc)
     return r < rc ? 0.5f * (cosf(3.14159265358979323846f * (r / rc)) + 1.0f) : 0.0f;
 }
 
-// (1 + lambda cos)^zeta for integral zeta by squaring, else powf
-__device__ __forceinline__ float nnp_theta(float lam, float zeta, float c)
+// (1 + lambda cos)^zeta of three cosines for zeta = 2^k: k squarings (k is
+// uniform across the wave, so the loop is a scalar branch).
+__device__ __forceinline__ void nnp_theta3(float lam, float kpow, float c0, float c1, float c2, float &p0,
+                                           float &p1, float &p2)
 {
-    float b = 1.0f + lam * c;
-    const int z = (int)zeta;
-    if ((float)z == zeta && z >= 0 && z <= 64) {
-        float r = 1.0f;
-        int e = z;
-        while (e) {
-            if (e & 1) r *= b;
-            b *= b;
-            e >>= 1;
-        }
-        return r;
+    p0 = 1.0f + lam * c0;
+    p1 = 1.0f + lam * c1;
+    p2 = 1.0f + lam * c2;
+    const int k = __builtin_amdgcn_readfirstlane((int)kpow);
+    for (int s = 0; s < k; ++s) {
+        p0 *= p0;
+        p1 *= p1;
+        p2 *= p2;
     }
-    return powf(b, zeta);
 }
 
 // cosines of the triangle (a, b, c) at a, b and c (Cutoff.jl:48-58)
@@ -82,31 +90,32 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-// partner pair p of the N(N-1)/2 pairs j < k
+// Partner pair p of the N(N-1)/2 pairs j < k, ordered by diagonal
+// (k - j = 1, 2, ...): consecutive pairs have distinct j and distinct k, so
+// the lanes of a wave seldom add into the same feature entry.
 __device__ __forceinline__ void nnp_pair(int p, int N, int &j, int &k)
 {
-    // rows of length N-1, N-2, ...: find j with start(j) <= p < start(j+1)
-    int jj = 0, start = 0;
-    while (start + (N - 1 - jj) <= p) {
-        start += N - 1 - jj;
-        ++jj;
+    int d = 1, start = 0;
+    while (start + (N - d) <= p) {
+        start += N - d;
+        ++d;
     }
-    j = jj;
-    k = jj + 1 + (p - start);
+    j = p - start;
+    k = j + d;
 }
 
-// Full feature matrix from positions into L.G (total_symm_calc,
+// Full feature matrix from positions into Gout (total_symm_calc,
 // SymmFunc.jl:494-510; radial :259-278, angular :280-312).  All threads call.
-__device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, int N)
+__device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout, int N)
 {
     const int tid = threadIdx.x;
-    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] = 0.0f;
+    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Gout[e] = 0.0f;
     __syncthreads();
     // radial: one (function, atom) per item
     for (int e = tid; e < T.n_rad_active * N; e += NNP_NT) {
         const int q = e / N, a = e % N;
-        const float eta = T.rad[4 * q], norm = T.rad[4 * q + 1];
-        const int row = (int)T.rad[4 * q + 2];
+        const float eta = L.rad[4 * q], norm = L.rad[4 * q + 1];
+        const int row = (int)L.rad[4 * q + 2];
         float s = 0.0f;
         for (int b = 0; b < N; ++b) {
             if (b == a) continue;
@@ -116,7 +125,7 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, int N)
             const float fc = nnp_cutoff(sqrtf(r2), T.r_cut);
             if (fc != 0.0f && fc != 1.0f) s += fc * expf(-eta * r2) * norm;
         }
-        L.G[row * NNP_NP + a] = s;
+        Gout[row * NNP_NP + a] = s;
     }
     // angular: one triangle a < b < c per item, all functions
     const int ntri = N * (N - 1) * (N - 2) / 6;
@@ -148,55 +157,77 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, int N)
         nnp_angles(pa, pb, pc, dab, dac, dbc, ta, tb, tc);
         const float rs = rab + rac + rbc;
         for (int g = 0; g < T.n_groups; ++g) {
-            const float ex = expf(-T.grp_eta[g] * rs) * fp;
-            const int f0 = T.grp[2 * g], cnt = T.grp[2 * g + 1];
+            const float ex = expf(-L.geta[g] * rs) * fp;
+            const int f0 = L.grp[2 * g], cnt = L.grp[2 * g + 1];
             for (int s = 0; s < cnt; ++s) {
                 const int f = f0 + s;
-                if (!T.abuild[f]) continue;
-                const float lam = T.afn[4 * f], zeta = T.afn[4 * f + 1], tpz = T.afn[4 * f + 2];
-                const int row = (int)T.afn[4 * f + 3];
-                atomicAdd(&L.G[row * NNP_NP + a], ex * nnp_theta(lam, zeta, ta) * tpz);
-                atomicAdd(&L.G[row * NNP_NP + b], ex * nnp_theta(lam, zeta, tb) * tpz);
-                atomicAdd(&L.G[row * NNP_NP + c], ex * nnp_theta(lam, zeta, tc) * tpz);
+                if (!L.abuild[f]) continue;
+                const float lam = L.afn[4 * f], zc = L.afn[4 * f + 1], tpz = L.afn[4 * f + 2];
+                const int row = (int)L.afn[4 * f + 3];
+                float pa_, pb_, pc_;
+                nnp_theta3(lam, zc, ta, tb, tc, pa_, pb_, pc_);
+                atomicAdd(&Gout[row * NNP_NP + a], ex * pa_ * tpz);
+                atomicAdd(&Gout[row * NNP_NP + b], ex * pb_ * tpz);
+                atomicAdd(&Gout[row * NNP_NP + c], ex * pc_ * tpz);
             }
         }
     }
     __syncthreads();
     for (int e = tid; e < T.nf * N; e += NNP_NT) {
         const int q = e / N, a = e % N;
-        L.G[q * NNP_NP + a] += T.row_off[q];
+        Gout[q * NNP_NP + a] += T.row_off[q];
     }
     __syncthreads();
 }
 
-// Per-atom network energies of G (+ D if given) into out[N]
-// (88 -> tanh 20 -> tanh 20 -> linear 1, ForwardPass.jl:23-39).  All threads call.
-__device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *D, int N, float *out)
+// Per-atom network energies of the features Gin into out[N]
+// (n_features -> tanh h1 -> tanh h2 -> linear 1, ForwardPass.jl:23-39).
+// Lane = atom; each wave takes every fourth hidden unit, so the weights it
+// reads are the same for all its lanes.  All threads call.
+__device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gin, int N, float *out)
 {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, a = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int NW = NNP_NT / 64;
+    constexpr int HPW = (NNP_MAXH + NW - 1) / NW;
     const int nf = T.nf, h1 = T.h1, h2 = T.h2;
-    for (int e = tid; e < N * h1; e += NNP_NT) {
-        const int a = e / h1, h = e % h1;
-        float z = 0.0f;
-        if (D) {
-            for (int q = 0; q < nf; ++q) z += (L.G[q * NNP_NP + a] + D[q * NNP_NP + a]) * L.W1[q * h1 + h];
-        } else {
-            for (int q = 0; q < nf; ++q) z += L.G[q * NNP_NP + a] * L.W1[q * h1 + h];
+    float z[HPW];
+#pragma unroll
+    for (int s = 0; s < HPW; ++s) z[s] = 0.0f;
+    for (int q = 0; q < nf; ++q) {
+        const float x = Gin[q * NNP_NP + a];
+#pragma unroll
+        for (int s = 0; s < HPW; ++s) {
+            const int h = w + NW * s;
+            if (h < h1) z[s] += x * L.W1[q * h1 + h];
         }
-        L.H1[a * NNP_HS + h] = tanhf(z + T.b1[h]);
+    }
+#pragma unroll
+    for (int s = 0; s < HPW; ++s) {
+        const int h = w + NW * s;
+        if (h < h1 && a < N) L.H1[a * NNP_HS + h] = tanhf(z[s] + T.b1[h]);
     }
     __syncthreads();
-    for (int e = tid; e < N * h2; e += NNP_NT) {
-        const int a = e / h2, h = e % h2;
-        float z = 0.0f;
-        for (int k = 0; k < h1; ++k) z += L.H1[a * NNP_HS + k] * T.w2[k * h2 + h];
-        L.H2[a * NNP_HS + h] = tanhf(z + T.b2[h]);
+#pragma unroll
+    for (int s = 0; s < HPW; ++s) z[s] = 0.0f;
+    for (int k = 0; k < h1; ++k) {
+        const float x = L.H1[a * NNP_HS + k];
+#pragma unroll
+        for (int s = 0; s < HPW; ++s) {
+            const int h = w + NW * s;
+            if (h < h2) z[s] += x * T.w2[k * h2 + h];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < HPW; ++s) {
+        const int h = w + NW * s;
+        if (h < h2 && a < N) L.H2[a * NNP_HS + h] = tanhf(z[s] + T.b2[h]);
     }
     __syncthreads();
-    for (int a = tid; a < N; a += NNP_NT) {
-        float z = 0.0f;
-        for (int k = 0; k < h2; ++k) z += L.H2[a * NNP_HS + k] * T.w3[k];
-        out[a] = z + T.b3;
+    if (w == 0 && a < N) {
+        float y = 0.0f;
+        for (int k = 0; k < h2; ++k) y += L.H2[a * NNP_HS + k] * T.w3[k];
+        out[a] = y + T.b3;
     }
     __syncthreads();
 }
@@ -205,9 +236,26 @@ __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, in
 {
     const NnpTables &T = a.nnp;
     const float *gX = reinterpret_cast<const float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
-    for (int e = threadIdx.x; e < 3 * NNP_NP; e += NNP_NT) L.X[e] = gX[e];
-    for (int e = threadIdx.x; e < T.nf * T.h1; e += NNP_NT) L.W1[e] = T.w1[e];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) L.X[e] = gX[e];
+    for (int e = tid; e < T.nf * T.h1; e += NNP_NT) L.W1[e] = T.w1[e];
+    const int nfun = T.n_groups > 0 ? T.grp[2 * (T.n_groups - 1)] + T.grp[2 * (T.n_groups - 1) + 1] : 0;
+    for (int e = tid; e < 4 * T.n_rad_active; e += NNP_NT) L.rad[e] = T.rad[e];
+    for (int e = tid; e < 4 * nfun; e += NNP_NT) L.afn[e] = T.afn[e];
+    for (int e = tid; e < nfun; e += NNP_NT) L.abuild[e] = T.abuild[e];
+    for (int e = tid; e < T.n_groups; e += NNP_NT) {
+        L.geta[e] = T.grp_eta[e];
+        L.grp[2 * e] = T.grp[2 * e];
+        L.grp[2 * e + 1] = T.grp[2 * e + 1];
+    }
 }
+
+#define NNP_BIND_TABLES(L) \
+    L.rad = s_rad;         \
+    L.afn = s_afn;         \
+    L.geta = s_geta;       \
+    L.grp = s_grp;         \
+    L.abuild = s_abuild
 
 // set_variables + initialise_energy (src/EnergyEvaluation.jl:1812-1822,
 // :1735-1747): features, per-atom energies and the FP64 total of each slot.
@@ -217,12 +265,14 @@ __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
     const NnpTables &T = a.nnp;
     const int slot = blockIdx.x;
     if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;  // whole workgroup
-    const NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NNP_TABLES_LDS;
+    NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NNP_BIND_TABLES(L);
     const int N = a.runs[a.slot_run[slot]].n_atoms;
     nnp_load(a, L, slot);
     __syncthreads();
-    nnp_full_build(T, L, N);
-    nnp_forward(T, L, nullptr, N, L.eat);
+    nnp_full_build(T, L, L.G, N);
+    nnp_forward(T, L, L.G, N, L.eat);
     float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
     for (int e = threadIdx.x; e < T.nf * NNP_NP; e += NNP_NT) gG[e] = L.G[e];
     for (int e = threadIdx.x; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = e < N ? L.eat[e] : 0.0f;
@@ -235,7 +285,7 @@ __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
 
 // One Monte-Carlo cycle (N moves) of one configuration per workgroup
 // (mc_step!, src/MCRun.jl:169-182, with energy_update!(RuNNerPotential)).
-__global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
+__global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))) nnp_sweep_kernel(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ int s_flag, s_err;
@@ -246,7 +296,9 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
     const int slot = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NNP_TABLES_LDS;
+    NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NNP_BIND_TABLES(L);
     const RunInfo &ri = a.runs[a.slot_run[slot]];
     const int N = ri.n_atoms;
     const uint32_t nsteps = (uint32_t)N;
@@ -271,6 +323,9 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
     }
     __syncthreads();
     int nacc = 0;
+    // current features and the scratch that becomes the trial features; an
+    // accepted move swaps the two (swap_vars!(NNPVariables), src/swap_config.jl:213-221)
+    float *Gp = L.G, *Dp = L.D;
 
     for (uint32_t m = 0; m < nsteps; ++m) {
         // draws and trial position (every thread the same), src/MCMoves.jl:115-135
@@ -331,20 +386,20 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
             L.fo[j] = nnp_cutoff(sqrtf(ro), T.r_cut);
             if (j != i && (double)rn < T.hardcore2) s_flag = 1;
         }
-        for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.D[e] = 0.0f;
+        for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] = 0.0f;
         __syncthreads();
         double new_en = 100.0;  // the hard-core sentinel, src/EnergyEvaluation.jl:1604-1606
         if (!s_flag) {
             // radial changes, DeltaMatrix.jl:794-822: one wave per function
             for (int q = tid >> 6; q < T.n_rad_active; q += NNP_NT / 64) {
-                const float eta = T.rad[4 * q], norm = T.rad[4 * q + 1];
-                const int row = (int)T.rad[4 * q + 2];
+                const float eta = L.rad[4 * q], norm = L.rad[4 * q + 1];
+                const int row = (int)L.rad[4 * q + 2];
                 float d = 0.0f;
                 if (lane < N && lane != i)
                     d = expf(-eta * L.rn2[lane]) * (L.fn[lane] * norm) + expf(-eta * L.ro2[lane]) * (-L.fo[lane] * norm);
-                if (lane < N && lane != i) L.D[row * NNP_NP + lane] = d;
+                if (lane < N && lane != i) Dp[row * NNP_NP + lane] = d;
                 const float s = wave_sum(d);
-                if (lane == 0) L.D[row * NNP_NP + i] = s;
+                if (lane == 0) Dp[row * NNP_NP + i] = s;
             }
             // angular changes, DeltaMatrix.jl:827-872: partner pairs (j, k), j, k != i
             float rsn[NNP_PP], rso[NNP_PP], fpn[NNP_PP], fpo[NNP_PP];
@@ -380,11 +435,20 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
                     nnp_angles(po, pj3, pk3, sqrtf(L.ro2[j]), sqrtf(L.ro2[k]), djk, co[r][0], co[r][1], co[r][2]);
             }
             for (int g = 0; g < T.n_groups; ++g) {
-                const float eta = T.grp_eta[g];
-                const int f0 = T.grp[2 * g], cnt = T.grp[2 * g + 1];
-                float acc[NNP_GMAX];
+                const float eta = L.geta[g];
+                const int f0 = __builtin_amdgcn_readfirstlane(L.grp[2 * g]);
+                const int cnt = __builtin_amdgcn_readfirstlane(L.grp[2 * g + 1]);
+                float acc[NNP_GMAX], flam[NNP_GMAX], fzc[NNP_GMAX], ftpz[NNP_GMAX];
+                int frow[NNP_GMAX];
 #pragma unroll
-                for (int s = 0; s < NNP_GMAX; ++s) acc[s] = 0.0f;
+                for (int s = 0; s < NNP_GMAX; ++s) {
+                    acc[s] = 0.0f;
+                    const int f = f0 + (s < cnt ? s : 0);
+                    flam[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f])));
+                    fzc[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f + 1])));
+                    ftpz[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f + 2])));
+                    frow[s] = __builtin_amdgcn_readfirstlane((int)L.afn[4 * f + 3]);
+                }
 #pragma unroll
                 for (int r = 0; r < NNP_PP; ++r) {
                     if (fpn[r] == 0.0f && fpo[r] == 0.0f) continue;
@@ -393,26 +457,28 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
 #pragma unroll
                     for (int s = 0; s < NNP_GMAX; ++s) {
                         if (s >= cnt) break;
-                        const int f = f0 + s;
-                        const float lam = T.afn[4 * f], zeta = T.afn[4 * f + 1], tpz = T.afn[4 * f + 2];
-                        const int row = (int)T.afn[4 * f + 3];
-                        const float di = en * nnp_theta(lam, zeta, cn[r][0]) * tpz - eo * nnp_theta(lam, zeta, co[r][0]) * tpz;
-                        const float dj = en * nnp_theta(lam, zeta, cn[r][1]) * tpz - eo * nnp_theta(lam, zeta, co[r][1]) * tpz;
-                        const float dk = en * nnp_theta(lam, zeta, cn[r][2]) * tpz - eo * nnp_theta(lam, zeta, co[r][2]) * tpz;
-                        acc[s] += di;
-                        atomicAdd(&L.D[row * NNP_NP + pj[r]], dj);
-                        atomicAdd(&L.D[row * NNP_NP + pk[r]], dk);
+                        const float lam = flam[s], zc = fzc[s], tpz = ftpz[s];
+                        const int row = frow[s];
+                        float n0, n1, n2, o0, o1, o2;
+                        nnp_theta3(lam, zc, cn[r][0], cn[r][1], cn[r][2], n0, n1, n2);
+                        nnp_theta3(lam, zc, co[r][0], co[r][1], co[r][2], o0, o1, o2);
+                        const float ent = en * tpz, eot = eo * tpz;
+                        acc[s] += ent * n0 - eot * o0;
+                        atomicAdd(&Dp[row * NNP_NP + pj[r]], ent * n1 - eot * o1);
+                        atomicAdd(&Dp[row * NNP_NP + pk[r]], ent * n2 - eot * o2);
                     }
                 }
 #pragma unroll
                 for (int s = 0; s < NNP_GMAX; ++s) {
                     if (s >= cnt) break;
                     const float v = wave_sum(acc[s]);
-                    if (lane == 0 && v != 0.0f) atomicAdd(&L.D[(int)T.afn[4 * (f0 + s) + 3] * NNP_NP + i], v);
+                    if (lane == 0 && v != 0.0f) atomicAdd(&Dp[frow[s] * NNP_NP + i], v);
                 }
             }
             __syncthreads();
-            nnp_forward(T, L, L.D, N, L.eatn);
+            for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] += Gp[e];
+            __syncthreads();
+            nnp_forward(T, L, Dp, N, L.eatn);
             if (tid == 0) {
                 double s = 0.0;
                 for (int e = 0; e < N; ++e) s += (double)L.eatn[e];
@@ -432,7 +498,9 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
         }
         if (accept) {
             // swap_atom_config! + swap_vars!(NNPVariables), src/swap_config.jl:40-51, :213-221
-            for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] += L.D[e];
+            float *tmp = Gp;
+            Gp = Dp;
+            Dp = tmp;
             for (int e = tid; e < N; e += NNP_NT) L.eat[e] = L.eatn[e];
             if (tid == 0) {
                 L.X[i] = tx;
@@ -447,8 +515,8 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
 
     // periodic full rebuild bounds the drift of the incremental features
     if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
-        nnp_full_build(T, L, N);
-        nnp_forward(T, L, nullptr, N, L.eat);
+        nnp_full_build(T, L, Gp, N);
+        nnp_forward(T, L, Gp, N, L.eat);
         if (tid == 0) {
             double s = 0.0;
             for (int e = 0; e < N; ++e) s += (double)L.eat[e];
@@ -463,7 +531,7 @@ __global__ void __launch_bounds__(NNP_NT) nnp_sweep_kernel(SweepArgs a)
     float *gX = reinterpret_cast<float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
     float *gGw = a.feat + (size_t)slot * T.nf * NNP_NP;
     for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) gX[e] = L.X[e];
-    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) gGw[e] = L.G[e];
+    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) gGw[e] = Gp[e];
     for (int e = tid; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = L.eat[e];
     if (tid == 0) {
         a.slot_en[slot] = E;
