@@ -185,6 +185,18 @@ int ptmc_multihistogram_nvt(int32_t n_runs, int32_t n_temps, int32_t n_bin, cons
                             double k_boltzmann, double conv_threshold, int32_t max_iter, double *cv,
                             double *free_energy, int32_t *iterations, int32_t device);
 
+/* The RuNNer forward pass with the argument list of the reference's ccall
+ * (MachineLearningPotential/ForwardPass.jl:60-84, :121-145; the prebuilt
+ * librunnerjulia.so symbol `forward`), so ForwardPass.lib_path() can point at
+ * this library: input is column-major n_in x batch, params in RuNNer order
+ * (per layer: weights input-major, then biases), activations 1 (linear) or
+ * 2 (tanh) per layer (the input layer's code is not used).  Writes the
+ * per-atom energies to eatom[batch] and returns eatom[batch - 1]; returns NaN
+ * for an invalid description.  Computed on the GPU in FP64. */
+double forward(const double *input, const int32_t *n_in, const int32_t *n_layers, const int32_t *nodes,
+               const int32_t *batch, const int32_t *n_params, const double *params, const int32_t *activations,
+               double *eatom);
+
 #ifdef __cplusplus
 }
 #endif

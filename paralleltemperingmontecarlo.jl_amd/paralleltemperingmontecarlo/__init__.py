@@ -26,6 +26,7 @@ from .api import (  # noqa: F401
     SphericalBC,
     TempGrid,
     check_e_bounds,
+    forward_pass,
     kB,
     ptmc_run,
     ptmc_run_batch,

@@ -141,6 +141,8 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = C.c_int
+    lib.forward.argtypes = [_dp, _i32p, _i32p, _i32p, _i32p, _i32p, _dp, _i32p, _dp]
+    lib.forward.restype = C.c_double
     if path is None:
         _LIB = lib
     return lib

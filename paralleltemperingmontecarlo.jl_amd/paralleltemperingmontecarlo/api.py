@@ -261,6 +261,28 @@ class RuNNerPotential:
         return rad, ang
 
 
+def forward_pass(input, batchsize, nnparams: NeuralNetworkPotential) -> np.ndarray:
+    """forward_pass(input, batchsize, nnparams) (ForwardPass.jl:86-97): per-atom
+    energies of the columns of `input` (n_in x batchsize, or batchsize rows of
+    n_in features) through libptmc's `forward` (FP64, on the GPU)."""
+    x = np.asarray(input, dtype=np.float64)
+    n_in = int(nnparams.num_nodes[0])
+    if x.shape == (n_in, batchsize):
+        x = x.T
+    x = np.ascontiguousarray(x.reshape(batchsize, n_in))
+    lib = _lib.load()
+    eatom = np.zeros(batchsize)
+    i32 = lambda v: C.byref(C.c_int32(int(v)))
+    nodes = np.ascontiguousarray(nnparams.num_nodes, dtype=np.int32)
+    act = np.ascontiguousarray(nnparams.activation_functions, dtype=np.int32)
+    par = np.ascontiguousarray(nnparams.parameters, dtype=np.float64)
+    ret = lib.forward(_lib.ptr(x), i32(n_in), i32(len(nodes)), _lib.ptr(nodes, C.c_int32), i32(batchsize),
+                      i32(len(par)), _lib.ptr(par), _lib.ptr(act, C.c_int32), _lib.ptr(eatom))
+    if math.isnan(ret):
+        raise _lib.PTMCError(_lib.PTMC_E_ARG, "forward: invalid network description or device error")
+    return eatom
+
+
 def read_runner_weights(path) -> np.ndarray:
     """A RuNNer weights.XXX.data file (first column) as the flat parameter vector."""
     vals = []

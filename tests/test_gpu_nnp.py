@@ -120,3 +120,23 @@ def test_recompute_bounds_the_feature_drift(saved):
     e = eng.energies(run)
     assert np.max(eng.drift(run)) < 2e-6 * np.max(np.abs(e))
     eng.close()
+
+
+def test_forward_drop_in_all_ones():
+    """test/test_runner_forward.jl:47-73 through libptmc's `forward`."""
+    nnp = pt.NeuralNetworkPotential([88, 20, 20, 1], [1, 2, 2, 1], np.ones(88 * 20 + 20 + 20 * 20 + 20 + 20 + 1))
+    e = pt.forward_pass(np.ones((88, 2)), 2, nnp)
+    assert np.all(e == 21.0)
+
+
+def test_forward_drop_in_reference_energies(oracle, saved):
+    """FP64 features of saved Cu55 configurations (oracle) through `forward`:
+    the per-atom energies sum to the reference's total energies to 1e-13."""
+    pot = reference_cu55_potential(NNP / "scaling.data", NNP / "weights.029.data")
+    nn = _orc(oracle, pot)
+    for x, e in saved[:5]:
+        g = nn.full_g(x)  # atoms x features
+        eat = pt.forward_pass(g, len(g), pot.nnp)
+        _, ref_eat = nn.energy(g)
+        np.testing.assert_allclose(eat, ref_eat, rtol=1e-13, atol=1e-15)
+        assert eat.sum() == pytest.approx(e, rel=1e-13)
