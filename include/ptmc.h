@@ -185,6 +185,33 @@ int ptmc_multihistogram_nvt(int32_t n_runs, int32_t n_temps, int32_t n_bin, cons
                             double k_boltzmann, double conv_threshold, int32_t max_iter, double *cv,
                             double *free_energy, int32_t *iterations, int32_t device);
 
+/* Learned surrogate and delayed-acceptance Metropolis for the network
+ * potential.  No counterpart in the reference: these entry points serve the
+ * engine's own acceleration (Christen & Fox 2005).  The surrogate is a
+ * per-atom MLP on 64 Gaussian radial descriptors (64 -> 64 -> 64 SiLU -> 1),
+ * trained on the device on (configuration, exact energy) samples the run
+ * collects.  With delayed acceptance on, a move is first tested on the
+ * surrogate energy change and only a surviving move is evaluated with the
+ * network potential and accepted with min(1, exp(-beta (dE - dE_hat))), so
+ * the sampled distribution is the exact one.  Parameters are 8385 values:
+ * W1 [64][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3. */
+int ptmc_surrogate_init(ptmc_handle h, int32_t sample_capacity, uint64_t seed);
+int ptmc_surrogate_collect(ptmc_handle h, int32_t every_cycles);          /* 0 = stop */
+int ptmc_surrogate_samples(ptmc_handle h, int64_t *n_samples);
+int ptmc_surrogate_train(ptmc_handle h, int32_t epochs, int32_t batch, double learning_rate,
+                         double valid_fraction, double *rmse_train, double *rmse_valid);
+int ptmc_surrogate_enable(ptmc_handle h, int32_t delayed_acceptance);
+int ptmc_surrogate_get_params(ptmc_handle h, double *params);
+int ptmc_surrogate_set_params(ptmc_handle h, const double *params);
+/* Surrogate energies of n configurations (n x n_atoms x 3), and the training
+ * loss mean_c ((E_hat_c - E_c) / n_atoms)^2 with its gradient. */
+int ptmc_surrogate_predict(ptmc_handle h, int32_t n, int32_t n_atoms, const double *x, double *e_hat);
+int ptmc_surrogate_grad(ptmc_handle h, int32_t n, int32_t n_atoms, const double *x, const double *e,
+                        double *loss, double *grad);
+/* Delayed-acceptance counts of a run: proposals, surrogate-stage accepts
+ * (exact evaluations), final accepts. */
+int ptmc_surrogate_stats(ptmc_handle h, int32_t run, int64_t *proposals, int64_t *stage1, int64_t *accepted);
+
 /* The RuNNer forward pass with the argument list of the reference's ccall
  * (MachineLearningPotential/ForwardPass.jl:60-84, :121-145; the prebuilt
  * librunnerjulia.so symbol `forward`), so ForwardPass.lib_path() can point at

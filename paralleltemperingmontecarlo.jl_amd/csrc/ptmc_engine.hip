@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -21,8 +22,16 @@ using namespace ptmc;
 
 namespace ptmc {
 SweepFn nnp_sweep_fn();
+SweepFn nnp_sweep_da_fn();
 SweepFn nnp_init_fn();
 void (*nnp_cycle_fn())(SweepArgs, int);
+hipError_t sur_launch_desc(const float *x, const int32_t *n_atoms, float *bd, float r_cut, int n, hipStream_t st);
+hipError_t sur_launch_collect(const SweepArgs &a, int n_slots, float *bd, double *be, int32_t *bn, long long base,
+                              int cap, hipStream_t st);
+hipError_t sur_launch_train(const float *bd, const double *be, const int32_t *bn, const int32_t *idx, const float *w,
+                            float *partial, double *pred, int grad, int batch, hipStream_t st);
+hipError_t sur_launch_reduce(const float *partial, int batch, float *grad, hipStream_t st);
+hipError_t sur_launch_adam(float *w, const float *grad, float *m, float *v, int t, float lr, hipStream_t st);
 }  // namespace ptmc
 
 namespace {
@@ -124,6 +133,18 @@ struct ptmc_engine {
     DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_temp_slot;
     NnpTables nnp_dev{};
     size_t nnp_lds = 0;
+
+    // learned surrogate / delayed acceptance
+    struct {
+        bool init = false, da = false;
+        int cap = 0, collect_every = 0, t = 0;
+        long long n_samples = 0;
+        uint64_t seed = 0;
+    } sur;
+    DevBuf<float> d_sur_w, d_sur_m, d_sur_v, d_sur_grad, d_sur_partial, d_sur_D, d_sur_x;
+    DevBuf<double> d_sur_E, d_sur_pred;
+    DevBuf<int32_t> d_sur_N, d_sur_idx;
+    DevBuf<long long> d_da_stats;
 
     int64_t sweep_counter = 0;  // global cycle counter (Philox word 1)
     int64_t phase_counter = 0;  // phase-local cycle index
@@ -454,6 +475,10 @@ SweepArgs base_args(ptmc_engine *h)
     a.feat = h->d_feat.p;
     a.eatom = h->d_eatom.p;
     a.temp_slot = h->d_temp_slot.p;
+    a.sur_w = h->d_sur_w.p;
+    a.da = h->sur.da ? 1 : 0;
+    a.sur_rcut = (float)h->nnp.r_cut;
+    a.da_stats = h->d_da_stats.p;
     return a;
 }
 
@@ -664,6 +689,18 @@ int ptmc_destroy(ptmc_handle h)
     h->d_nnp_grp.free_();
     h->d_nnp_abuild.free_();
     h->d_temp_slot.free_();
+    h->d_sur_w.free_();
+    h->d_sur_m.free_();
+    h->d_sur_v.free_();
+    h->d_sur_grad.free_();
+    h->d_sur_partial.free_();
+    h->d_sur_D.free_();
+    h->d_sur_x.free_();
+    h->d_sur_E.free_();
+    h->d_sur_pred.free_();
+    h->d_sur_N.free_();
+    h->d_sur_idx.free_();
+    h->d_da_stats.free_();
     h->d_beta.free_();
     h->d_displ.free_();
     h->d_maxbox.free_();
@@ -977,6 +1014,11 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
             void *args[] = {&a, &n_runs};
             HIPCHK(h, hipLaunchKernel((const void *)nnp_cycle_fn(), dim3((n_runs + 63) / 64), dim3(64), args, 0,
                                       h->stream));
+            if (h->sur.init && h->sur.collect_every > 0 && (h->sweep_counter + 1) % h->sur.collect_every == 0) {
+                HIPCHK(h, sur_launch_collect(a, h->n_slots, h->d_sur_D.p, h->d_sur_E.p, h->d_sur_N.p, h->sur.n_samples,
+                                             h->sur.cap, h->stream));
+                h->sur.n_samples += h->n_slots;
+            }
             if (h->pending.size() >= 512) {
                 HIPCHK(h, hipStreamSynchronize(h->stream));
                 if ((rc = collect_timings(h))) return rc;
@@ -1378,6 +1420,286 @@ int ptmc_run_group(ptmc_handle h, int32_t run)
     for (size_t k = 0; k < h->launches.size(); ++k)
         if (t >= h->launches[k].tile0 && t < h->launches[k].tile0 + h->launches[k].ntiles) return (int)k;
     return -1;
+}
+
+
+// ------------------------------------------------------------------ surrogate
+
+static int sur_check(ptmc_engine *h)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!h->sur.init) return fail(h, PTMC_E_STATE, "surrogate not initialised (ptmc_surrogate_init)");
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_init(ptmc_handle h, int32_t capacity, uint64_t seed)
+{
+    if (!h) return PTMC_E_ARG;
+    int rc = ptmc_prepare(h);
+    if (rc) return rc;
+    if (!is_nnp(h)) return fail(h, PTMC_E_UNSUPPORTED, "the surrogate filters the network potential");
+    if (capacity < 1) return fail(h, PTMC_E_ARG, "sample capacity must be positive");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int dev = 0, max_lds = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t lds = h->nnp_lds + sur_lds_floats() * sizeof(float);
+    if ((long long)lds > max_lds) return fail(h, PTMC_E_UNSUPPORTED, "delayed acceptance needs %zu B of LDS", lds);
+    HIPCHK(h, hipFuncSetAttribute((const void *)nnp_sweep_da_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    h->sur.cap = capacity;
+    h->sur.seed = seed;
+    h->sur.n_samples = 0;
+    h->sur.t = 0;
+    if ((rc = dalloc(h, h->d_sur_D, (size_t)capacity * NNP_NP * SUR_K))) return rc;
+    if ((rc = dalloc(h, h->d_sur_E, (size_t)capacity))) return rc;
+    if ((rc = dalloc(h, h->d_sur_N, (size_t)capacity))) return rc;
+    if ((rc = dalloc(h, h->d_sur_pred, (size_t)capacity))) return rc;
+    if ((rc = dalloc(h, h->d_sur_idx, (size_t)capacity))) return rc;
+    if ((rc = dalloc(h, h->d_sur_m, SUR_P))) return rc;
+    if ((rc = dalloc(h, h->d_sur_v, SUR_P))) return rc;
+    if ((rc = dalloc(h, h->d_sur_grad, SUR_P))) return rc;
+    if ((rc = dalloc(h, h->d_da_stats, (size_t)3 * h->n_slots))) return rc;
+    // He-style start, b3 = mean atomic energy of the current states
+    std::vector<double> en(h->n_slots);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(en.data(), h->d_slot_en.p, en.size() * 8, hipMemcpyDeviceToHost));
+    double e_atom = 0.0;
+    for (auto &rh : h->runs)
+        for (int t = 0; t < rh.d.n_temps; ++t) e_atom += en[rh.info.slot_base + t] / rh.d.n_atoms;
+    e_atom /= h->n_slots;
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<float> w(SUR_P, 0.0f);
+    int o = 0;
+    for (int e = 0; e < SUR_K * SUR_H; ++e) w[o++] = (float)(nd(gen) * std::sqrt(2.0 / SUR_K));
+    o += SUR_H;
+    for (int e = 0; e < SUR_H * SUR_H; ++e) w[o++] = (float)(nd(gen) * std::sqrt(2.0 / SUR_H));
+    o += SUR_H;
+    for (int e = 0; e < SUR_H; ++e) w[o++] = (float)(nd(gen) * std::sqrt(1.0 / SUR_H));
+    w[o] = (float)e_atom;
+    if ((rc = upload(h, h->d_sur_w, w))) return rc;
+    h->sur.init = true;
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_collect(ptmc_handle h, int32_t every)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (every < 0) return fail(h, PTMC_E_ARG, "negative collection period");
+    h->sur.collect_every = every;
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_samples(ptmc_handle h, int64_t *n)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (n) *n = std::min<long long>(h->sur.n_samples, h->sur.cap);
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_enable(ptmc_handle h, int32_t on)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    h->sur.da = on != 0;
+    Launch &L = h->launches[0];
+    L.fn = h->sur.da ? nnp_sweep_da_fn() : nnp_sweep_fn();
+    L.lds = h->nnp_lds + (h->sur.da ? sur_lds_floats() * sizeof(float) : 0);
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_get_params(ptmc_handle h, double *p)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (!p) return fail(h, PTMC_E_ARG, "null buffer");
+    std::vector<float> w(SUR_P);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(w.data(), h->d_sur_w.p, SUR_P * 4, hipMemcpyDeviceToHost));
+    for (int e = 0; e < SUR_P; ++e) p[e] = w[e];
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_set_params(ptmc_handle h, const double *p)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (!p) return fail(h, PTMC_E_ARG, "null buffer");
+    std::vector<float> w(SUR_P);
+    for (int e = 0; e < SUR_P; ++e) w[e] = (float)p[e];
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->d_sur_w.p, w.data(), SUR_P * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(h->d_sur_m.p, 0, SUR_P * 4));
+    HIPCHK(h, hipMemset(h->d_sur_v.p, 0, SUR_P * 4));
+    h->sur.t = 0;
+    return PTMC_OK;
+}
+
+// descriptors of n host configurations (n x n_atoms x 3) into d_sur_x scratch
+static int sur_upload_configs(ptmc_engine *h, int32_t n, int32_t n_atoms, const double *x, DevBuf<float> &D,
+                              DevBuf<int32_t> &Nb)
+{
+    if (n < 1 || n_atoms < 2 || n_atoms > NNP_NP || !x) return fail(h, PTMC_E_ARG, "bad configurations");
+    std::vector<float> buf((size_t)n * 3 * NNP_NP, 0.0f);
+    for (int s = 0; s < n; ++s)
+        for (int j = 0; j < n_atoms; ++j)
+            for (int c = 0; c < 3; ++c) buf[((size_t)s * 3 + c) * NNP_NP + j] = (float)x[((size_t)s * n_atoms + j) * 3 + c];
+    int rc;
+    if ((rc = upload(h, h->d_sur_x, buf))) return rc;
+    std::vector<int32_t> nv(n, n_atoms);
+    if ((rc = upload(h, Nb, nv))) return rc;
+    if ((rc = dalloc(h, D, (size_t)n * NNP_NP * SUR_K))) return rc;
+    HIPCHK(h, sur_launch_desc(h->d_sur_x.p, Nb.p, D.p, (float)h->nnp.r_cut, n, h->stream));
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_predict(ptmc_handle h, int32_t n, int32_t n_atoms, const double *x, double *e_hat)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    DevBuf<float> D;
+    DevBuf<int32_t> Nb, idx;
+    DevBuf<double> E, pred;
+    if ((rc = sur_upload_configs(h, n, n_atoms, x, D, Nb))) return rc;
+    std::vector<int32_t> iv(n);
+    for (int s = 0; s < n; ++s) iv[s] = s;
+    if ((rc = upload(h, idx, iv)) || (rc = dalloc(h, E, n)) || (rc = dalloc(h, pred, n))) return rc;
+    HIPCHK(h, sur_launch_train(D.p, E.p, Nb.p, idx.p, h->d_sur_w.p, nullptr, pred.p, 0, n, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(e_hat, pred.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    D.free_(); Nb.free_(); idx.free_(); E.free_(); pred.free_();
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_grad(ptmc_handle h, int32_t n, int32_t n_atoms, const double *x, const double *e, double *loss,
+                        double *grad)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (!e) return fail(h, PTMC_E_ARG, "null energies");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    DevBuf<float> D, part;
+    DevBuf<int32_t> Nb, idx;
+    DevBuf<double> E, pred;
+    if ((rc = sur_upload_configs(h, n, n_atoms, x, D, Nb))) return rc;
+    std::vector<int32_t> iv(n);
+    for (int s = 0; s < n; ++s) iv[s] = s;
+    std::vector<double> ev(e, e + n);
+    if ((rc = upload(h, idx, iv)) || (rc = upload(h, E, ev)) || (rc = dalloc(h, pred, n)) ||
+        (rc = dalloc(h, part, (size_t)n * SUR_P)))
+        return rc;
+    HIPCHK(h, sur_launch_train(D.p, E.p, Nb.p, idx.p, h->d_sur_w.p, part.p, pred.p, 1, n, h->stream));
+    HIPCHK(h, sur_launch_reduce(part.p, n, h->d_sur_grad.p, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> pv(n);
+    HIPCHK(h, hipMemcpy(pv.data(), pred.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (loss) {
+        double l = 0.0;
+        for (int s = 0; s < n; ++s) {
+            const double r = (pv[s] - e[s]) / n_atoms;
+            l += r * r;
+        }
+        *loss = l / n;
+    }
+    if (grad) {
+        std::vector<float> g(SUR_P);
+        HIPCHK(h, hipMemcpy(g.data(), h->d_sur_grad.p, SUR_P * 4, hipMemcpyDeviceToHost));
+        for (int p = 0; p < SUR_P; ++p) grad[p] = g[p];
+    }
+    D.free_(); part.free_(); Nb.free_(); idx.free_(); E.free_(); pred.free_();
+    return PTMC_OK;
+}
+
+// RMSE (Hartree per configuration) of the samples listed in idx[0..n)
+static int sur_rmse(ptmc_engine *h, const std::vector<int32_t> &iv, const std::vector<double> &Eall, double *out)
+{
+    if (iv.empty()) {
+        *out = 0.0;
+        return PTMC_OK;
+    }
+    HIPCHK(h, hipMemcpy(h->d_sur_idx.p, iv.data(), iv.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, sur_launch_train(h->d_sur_D.p, h->d_sur_E.p, h->d_sur_N.p, h->d_sur_idx.p, h->d_sur_w.p, nullptr,
+                               h->d_sur_pred.p, 0, (int)iv.size(), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> pv(iv.size());
+    HIPCHK(h, hipMemcpy(pv.data(), h->d_sur_pred.p, pv.size() * 8, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (size_t k = 0; k < iv.size(); ++k) {
+        const double r = pv[k] - Eall[iv[k]];
+        s += r * r;
+    }
+    *out = std::sqrt(s / iv.size());
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_train(ptmc_handle h, int32_t epochs, int32_t batch, double lr, double valid_fraction,
+                         double *rmse_train, double *rmse_valid)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (epochs < 0 || batch < 1 || !(lr > 0) || !(valid_fraction >= 0 && valid_fraction < 1))
+        return fail(h, PTMC_E_ARG, "bad training settings");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int n = (int)std::min<long long>(h->sur.n_samples, h->sur.cap);
+    if (n < 2) return fail(h, PTMC_E_STATE, "fewer than 2 samples collected");
+    std::vector<double> Eall(h->sur.cap);
+    HIPCHK(h, hipMemcpy(Eall.data(), h->d_sur_E.p, Eall.size() * 8, hipMemcpyDeviceToHost));
+    // fixed split: a seeded permutation, the last valid_fraction held out
+    std::vector<int32_t> perm(n);
+    for (int s = 0; s < n; ++s) perm[s] = s;
+    std::mt19937_64 gen(h->sur.seed ^ 0x5eed5eedull);
+    std::shuffle(perm.begin(), perm.end(), gen);
+    const int n_valid = (int)(valid_fraction * n);
+    const int n_train = n - n_valid;
+    std::vector<int32_t> tr(perm.begin(), perm.begin() + n_train), va(perm.begin() + n_train, perm.end());
+    const int B = std::min(batch, n_train);
+    if ((size_t)B * SUR_P > h->d_sur_partial.n) {
+        if ((rc = dalloc(h, h->d_sur_partial, (size_t)B * SUR_P))) return rc;
+    }
+    std::mt19937_64 gen2(h->sur.seed + (uint64_t)h->sur.t);
+    for (int ep = 0; ep < epochs; ++ep) {
+        std::shuffle(tr.begin(), tr.end(), gen2);
+        HIPCHK(h, hipMemcpy(h->d_sur_idx.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
+        for (int b0 = 0; b0 + B <= n_train; b0 += B) {
+            HIPCHK(h, sur_launch_train(h->d_sur_D.p, h->d_sur_E.p, h->d_sur_N.p, h->d_sur_idx.p + b0, h->d_sur_w.p,
+                                       h->d_sur_partial.p, h->d_sur_pred.p, 1, B, h->stream));
+            HIPCHK(h, sur_launch_reduce(h->d_sur_partial.p, B, h->d_sur_grad.p, h->stream));
+            h->sur.t += 1;
+            HIPCHK(h, sur_launch_adam(h->d_sur_w.p, h->d_sur_grad.p, h->d_sur_m.p, h->d_sur_v.p, h->sur.t, (float)lr,
+                                      h->stream));
+        }
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    if (tr.size() > 4096) tr.resize(4096);
+    if (rmse_train && (rc = sur_rmse(h, tr, Eall, rmse_train))) return rc;
+    if (rmse_valid && (rc = sur_rmse(h, va, Eall, rmse_valid))) return rc;
+    return PTMC_OK;
+}
+
+int ptmc_surrogate_stats(ptmc_handle h, int32_t run, int64_t *proposals, int64_t *stage1, int64_t *accepted)
+{
+    int rc = sur_check(h);
+    if (rc) return rc;
+    if (run < 0 || run >= (int)h->runs.size()) return fail(h, PTMC_E_ARG, "run index out of range");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<long long> st((size_t)3 * h->n_slots);
+    HIPCHK(h, hipMemcpy(st.data(), h->d_da_stats.p, st.size() * 8, hipMemcpyDeviceToHost));
+    const RunHost &rh = h->runs[run];
+    long long p = 0, s1 = 0, acc = 0;
+    for (int t = 0; t < rh.d.n_temps; ++t) {
+        const int sl = rh.info.slot_base + t;
+        p += st[3 * sl];
+        s1 += st[3 * sl + 1];
+        acc += st[3 * sl + 2];
+    }
+    if (proposals) *proposals = p;
+    if (stage1) *stage1 = s1;
+    if (accepted) *accepted = acc;
+    return PTMC_OK;
 }
 
 }  // extern "C"

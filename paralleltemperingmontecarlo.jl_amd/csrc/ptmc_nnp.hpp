@@ -11,6 +11,7 @@
 // between cycles in nnp_cycle_kernel, so one launch of the sweep kernel is
 // one cycle of every configuration.
 #pragma once
+#include "ptmc_surrogate.hpp"
 #include "ptmc_sweep.hpp"
 
 #include "ptmc_nnp_layout.hpp"
@@ -285,7 +286,10 @@ __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
 
 // One Monte-Carlo cycle (N moves) of one configuration per workgroup
 // (mc_step!, src/MCRun.jl:169-182, with energy_update!(RuNNerPotential)).
-__global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))) nnp_sweep_kernel(SweepArgs a)
+// DA: delayed acceptance with the learned surrogate (one workgroup per CU by
+// LDS, so the register budget is relaxed for that variant).
+template <bool DA>
+__global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA ? 2 : 4))) nnp_sweep_kernel(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ int s_flag, s_err;
@@ -315,6 +319,41 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))
     for (int e = tid; e < NNP_NP; e += NNP_NT) L.eat[e] = a.eatom[(size_t)slot * NNP_NP + e];
     double E = a.slot_en[slot];
     if (tid == 0) s_err = 0;
+    // delayed acceptance: surrogate descriptors and atom energies in LDS
+    // (rebuilt from the positions at every launch)
+    __shared__ float s_acci[SUR_K];
+    __shared__ double s_ehat;
+    constexpr bool da = DA;
+    float *Sd = nullptr, *Sn = nullptr, *sat = nullptr, *satn = nullptr;
+    SurNet net{};
+    const SurGeom sg = sur_geom(a.sur_rcut);
+    double Ehat = 0.0;
+    long long n_prop = 0, n_acc1 = 0;
+    if (da) {
+        float *base = reinterpret_cast<float *>(lds_raw) + nnp_lds_floats(T.nf, T.h1);
+        Sd = base;
+        Sn = Sd + NNP_NP * SUR_SS;
+        net.W1 = reinterpret_cast<short *>(Sn + NNP_NP * SUR_SS);
+        net.W2 = net.W1 + SUR_K * SUR_H;
+        net.Hb = net.W2 + SUR_H * SUR_H;
+        sat = reinterpret_cast<float *>(net.Hb + NNP_NP * SUR_HS);
+        satn = sat + NNP_NP;
+        net.b1 = satn + NNP_NP;
+        net.b2 = net.b1 + SUR_H;
+        net.w3 = net.b2 + SUR_H;
+        sur_load_net(net, a.sur_w, NNP_NT);
+        sur_descriptors(sg, L.X, N, Sd, SUR_SS, NNP_NT);
+        for (int e = tid; e < NNP_NP * SUR_SS; e += NNP_NT) Sn[e] = 0.0f;
+        __syncthreads();
+        sur_atom_energies(net, Sd, N, L.H1, sat);
+        if (tid == 0) {
+            double h = 0.0;
+            for (int q = 0; q < N; ++q) h += (double)sat[q];
+            s_ehat = h;
+        }
+        __syncthreads();
+        Ehat = s_ehat;
+    }
     const int npairs = N * (N - 1) / 2;
     for (int p = tid; p < npairs; p += NNP_NT) {
         int j, k;
@@ -387,9 +426,44 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))
             if (j != i && (double)rn < T.hardcore2) s_flag = 1;
         }
         for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] = 0.0f;
+        if (da && tid < SUR_K) s_acci[tid] = 0.0f;
         __syncthreads();
+        // delayed acceptance, stage 1: Metropolis on the surrogate energy change
+        bool stage1 = true;
+        double dEh = 0.0, Ehn = 0.0;
+        if (da) {
+            float accq = 0.0f;
+            const int kq = tid & (SUR_K - 1);  // NNP_NT is a multiple of SUR_K: fixed k per thread
+            for (int e = tid; e < N * SUR_K; e += NNP_NT) {
+                const int j = e / SUR_K;
+                if (j == i) continue;
+                const float rn = sqrtf(L.rn2[j]), ro = sqrtf(L.ro2[j]);
+                const float fcn = sur_cutoff(rn, sg.r_cut), fco = sur_cutoff(ro, sg.r_cut);
+                const float pn = fcn != 0.0f ? sur_phi(sg, rn, fcn, kq) : 0.0f;
+                const float pq = fco != 0.0f ? sur_phi(sg, ro, fco, kq) : 0.0f;
+                Sn[j * SUR_SS + kq] = Sd[j * SUR_SS + kq] + SUR_SIN * (pn - pq);
+                accq += pn;
+            }
+            atomicAdd(&s_acci[kq], accq);
+            __syncthreads();
+            if (tid < SUR_K) Sn[i * SUR_SS + tid] = SUR_SIN * s_acci[tid];
+            __syncthreads();
+            sur_atom_energies(net, Sn, N, L.H1, satn);
+            if (tid == 0) {
+                double h = 0.0;
+                for (int q = 0; q < N; ++q) h += (double)satn[q];
+                s_ehat = h;
+            }
+            __syncthreads();
+            Ehn = s_ehat;
+            dEh = Ehn - Ehat;
+            stage1 = (dEh <= 0.0) || (det_exp(-dEh * beta) >= (double)d0.umet);
+            n_prop += 1;
+            n_acc1 += stage1 ? 1 : 0;
+        }
         double new_en = 100.0;  // the hard-core sentinel, src/EnergyEvaluation.jl:1604-1606
-        if (!s_flag) {
+        if (da && !stage1) new_en = __builtin_nan("");
+        if (stage1 && !s_flag) {
             // radial changes, DeltaMatrix.jl:794-822: one wave per function
             for (int q = tid >> 6; q < T.n_rad_active; q += NNP_NT / 64) {
                 const float eta = L.rad[4 * q], norm = L.rad[4 * q + 1];
@@ -487,9 +561,20 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))
             __syncthreads();
             new_en = s_en;
         }
-        // Metropolis, src/MCRun.jl:122-132 and src/Exchange.jl:53-57
+        // Metropolis, src/MCRun.jl:122-132 and src/Exchange.jl:53-57; with
+        // delayed acceptance the second stage corrects the surrogate's error:
+        // accept with min(1, exp(-beta ((E' - E) - (E_hat' - E_hat))))
         const double dd = new_en - E;
-        const bool accept = (dd <= 0.0) || (det_exp(-dd * beta) >= (double)d0.umet);
+        bool accept;
+        if (!da) {
+            accept = (dd <= 0.0) || (det_exp(-dd * beta) >= (double)d0.umet);
+        } else if (!stage1) {
+            accept = false;
+        } else {
+            const u32x4 X2 = draw(a.seed, ri.run_key, RNG_MOVE, 255u, (uint32_t)t, sweep, m);
+            const double x2 = dd - dEh;
+            accept = (x2 <= 0.0) || (det_exp(-x2 * beta) >= (double)u24(X2.x));
+        }
         if (tid == 0 && a.trace_de != nullptr && a.trace_cycle < a.trace_cycles) {
             const size_t ti = ((size_t)Tg * a.trace_cycles + a.trace_cycle) * nsteps + m;
             a.trace_de[ti] = dd;
@@ -502,6 +587,15 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))
             Gp = Dp;
             Dp = tmp;
             for (int e = tid; e < N; e += NNP_NT) L.eat[e] = L.eatn[e];
+            if (da) {
+                float *t1 = Sd;
+                Sd = Sn;
+                Sn = t1;
+                t1 = sat;
+                sat = satn;
+                satn = t1;
+                Ehat = Ehn;
+            }
             if (tid == 0) {
                 L.X[i] = tx;
                 L.X[NNP_NP + i] = ty;
@@ -538,6 +632,11 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(4))
         a.t_count_atom[2 * Tg] += nacc;
         a.t_count_atom[2 * Tg + 1] += nacc;
         if (s_err) a.run_error[a.slot_run[slot]] = 1;
+        if (da) {
+            a.da_stats[3 * slot] += n_prop;
+            a.da_stats[3 * slot + 1] += n_acc1;
+            a.da_stats[3 * slot + 2] += nacc;
+        }
     }
 }
 

@@ -1,5 +1,6 @@
 // Sizes shared by the host engine and the neural-network kernels.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stddef.h>
 
 namespace ptmc {
@@ -14,10 +15,36 @@ constexpr int NNP_PP = (NNP_NP * (NNP_NP - 1) / 2 + NNP_NT - 1) / NNP_NT;  // pa
 
 // LDS layout in floats: X[3][NP] | G[nf][NP] | D[nf][NP] | W1[nf][h1] |
 // H1[NP][HS] | H2[NP][HS] | rn2, fn, ro2, fo, eat, eatn [NP each]
-inline size_t nnp_lds_floats(int nf, int h1)
+__host__ __device__ inline size_t nnp_lds_floats(int nf, int h1)
 {
     return (size_t)3 * NNP_NP + 2 * (size_t)nf * NNP_NP + (size_t)nf * h1 + 2 * (size_t)NNP_NP * NNP_HS +
            6 * NNP_NP;
+}
+
+}  // namespace ptmc
+
+namespace ptmc {
+
+// Learned surrogate (delayed acceptance): 64 Gaussian radial descriptors per
+// atom -> 64 -> 64 (SiLU) -> 1.
+constexpr int SUR_K = 64;
+constexpr int SUR_H = 64;
+constexpr int SUR_P = SUR_K * SUR_H + SUR_H + SUR_H * SUR_H + SUR_H + SUR_H + 1;
+constexpr float SUR_RMIN = 2.0f;
+constexpr float SUR_SIN = 0.25f;
+
+// padded LDS row strides (bank-conflict-free column access)
+constexpr int SUR_SS = SUR_K + 1;  // descriptors, f32
+constexpr int SUR_HS = SUR_H + 2;  // hidden activations, bf16
+constexpr int SUR_YS = SUR_H + 1;  // output products, f32
+
+// extra LDS floats of a delayed-acceptance sweep: descriptors S, S' [NP][SS]
+// (f32), bf16 W1, W2 [K][H] and H [NP][HS], surrogate atom energies (current
+// and trial), b1, b2, w3
+__host__ __device__ inline size_t sur_lds_floats()
+{
+    return (size_t)2 * NNP_NP * SUR_SS + (size_t)SUR_K * SUR_H + (size_t)NNP_NP * SUR_HS / 2 + 2 * NNP_NP +
+           3 * SUR_H;
 }
 
 }  // namespace ptmc

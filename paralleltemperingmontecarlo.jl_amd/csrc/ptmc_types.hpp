@@ -116,6 +116,11 @@ struct SweepArgs {
     float *feat;               // per slot [nf][NNP_NP] features
     float *eatom;              // per slot [NNP_NP] per-atom energies
     int32_t *temp_slot;        // per temperature: slot holding it
+    // learned surrogate and delayed acceptance
+    const float *sur_w;        // surrogate parameters (f32, packed)
+    int32_t da;                // 1: delayed-acceptance sweeps
+    float sur_rcut;            // descriptor cutoff (bohr)
+    long long *da_stats;       // per slot x3: proposals, stage-1 accepted, accepted
     double min_acc, max_acc;
 };
 

@@ -113,6 +113,16 @@ SIGNATURES = {
     "ptmc_run_lanes": [_h, C.c_int32],
     "ptmc_kernel_time": [_h, _dp, _i64p, C.c_int32],
     "ptmc_group_info": [_h, C.c_int32, _i32p, _i32p, _i32p, _dp, _i64p],
+    "ptmc_surrogate_init": [_h, C.c_int32, C.c_uint64],
+    "ptmc_surrogate_collect": [_h, C.c_int32],
+    "ptmc_surrogate_samples": [_h, _i64p],
+    "ptmc_surrogate_train": [_h, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp, _dp],
+    "ptmc_surrogate_enable": [_h, C.c_int32],
+    "ptmc_surrogate_get_params": [_h, _dp],
+    "ptmc_surrogate_set_params": [_h, _dp],
+    "ptmc_surrogate_predict": [_h, C.c_int32, C.c_int32, _dp, _dp],
+    "ptmc_surrogate_grad": [_h, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp],
+    "ptmc_surrogate_stats": [_h, C.c_int32, _i64p, _i64p, _i64p],
     "ptmc_run_group": [_h, C.c_int32],
 }
 

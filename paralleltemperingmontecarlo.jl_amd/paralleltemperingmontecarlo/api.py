@@ -575,6 +575,63 @@ class PTEngine:
         return dict(lanes_per_replica=kl.value, workgroup_lanes=wgs.value, tiles=nt.value, ms=ms.value,
                     launches=n.value)
 
+    # ---- learned surrogate / delayed acceptance (network potential only)
+
+    SURROGATE_PARAMS = 8385
+
+    def surrogate_init(self, capacity: int = 32768, seed: int = 7):
+        self.eng.call("ptmc_surrogate_init", int(capacity), int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    def surrogate_collect(self, every_cycles: int):
+        self.eng.call("ptmc_surrogate_collect", int(every_cycles))
+
+    def surrogate_samples(self) -> int:
+        n = C.c_int64()
+        self.eng.call("ptmc_surrogate_samples", C.byref(n))
+        return n.value
+
+    def train_surrogate(self, epochs: int, batch: int = 64, lr: float = 1e-3, valid_fraction: float = 0.1):
+        """Returns (RMSE on training samples, RMSE on held-out samples) in
+        Hartree per configuration."""
+        a, b = C.c_double(), C.c_double()
+        self.eng.call("ptmc_surrogate_train", int(epochs), int(batch), float(lr), float(valid_fraction), C.byref(a),
+                      C.byref(b))
+        return a.value, b.value
+
+    def delayed_acceptance(self, on: bool):
+        self.eng.call("ptmc_surrogate_enable", 1 if on else 0)
+
+    def surrogate_params(self) -> np.ndarray:
+        p = np.zeros(self.SURROGATE_PARAMS)
+        self.eng.call("ptmc_surrogate_get_params", _lib.ptr(p))
+        return p
+
+    def set_surrogate_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.size == self.SURROGATE_PARAMS
+        self.eng.call("ptmc_surrogate_set_params", _lib.ptr(p))
+
+    def surrogate_predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n, na = x.shape[0], x.shape[1]
+        out = np.zeros(n)
+        self.eng.call("ptmc_surrogate_predict", n, na, _lib.ptr(x), _lib.ptr(out))
+        return out
+
+    def surrogate_grad(self, x, e):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        loss = C.c_double()
+        g = np.zeros(self.SURROGATE_PARAMS)
+        self.eng.call("ptmc_surrogate_grad", x.shape[0], x.shape[1], _lib.ptr(x), _lib.ptr(e), C.byref(loss),
+                      _lib.ptr(g))
+        return loss.value, g
+
+    def surrogate_stats(self, run: int):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.eng.call("ptmc_surrogate_stats", int(run), C.byref(a), C.byref(b), C.byref(c))
+        return dict(proposals=a.value, stage1=b.value, accepted=c.value)
+
     def close(self):
         self.eng.close()
 
