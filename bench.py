@@ -9,7 +9,13 @@ recompute every 1000 cycles, all in one engine.  A step is one Monte-Carlo
 cycle of every run (N moves per temperature); the metric counts attempted
 atom moves of all replicas per second.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+The line also carries "other_configs": the remaining configurations measured
+briefly on the same device (Ne13 FP64 single ladder and 512 ladders, periodic
+bulk neon NVT/NPT at 108/216/256 atoms, NNP Cu55 single ladder and 64
+ladders, delayed acceptance on 256 NNP Cu55 ladders against the plain path),
+with the oracle's CPU rate where one is quoted.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-configs]
 
 Prints one JSON line.
 """
@@ -134,6 +140,230 @@ def cpu_baseline(runs, target_s=15.0):
                        f"reference arithmetic in FP64 (oracle/ptmc_oracle.c), {threads} threads, {dt:.1f} s")
 
 
+NE13_SETUP = ROOT / "tests" / "golden" / "ne13_setup.json"
+NNP_DIR = ROOT / "tests" / "golden" / "nnp"
+A2B = 1.8897259886
+P_ATM = 101325 / 2.9421015697e13  # Pa -> Hartree/bohr^3
+
+
+def _cpu_threads():
+    return min(os.cpu_count() or 1, 16)
+
+
+def _production(eng, n_runs, warmup, steps):
+    """Warm up (step adaptation on), set histogram bounds from the energies
+    reached, then time `steps` production cycles.  Returns seconds."""
+    eng.start_phase(False)
+    eng.cycles(warmup)
+    eng.sync()
+    for i in range(n_runs):
+        e = eng.energies(i)
+        eng.set_histogram(i, float(e.min()), float(e.max()))
+    eng.reset_counters()
+    eng.start_phase(True)
+    t0 = time.perf_counter()
+    eng.cycles(steps)
+    eng.sync()
+    return time.perf_counter() - t0
+
+
+def _acceptance(eng, n_runs, n_atoms, steps):
+    c = [eng.counters(i)["count_atom"][:, 0] for i in range(n_runs)]
+    return float(np.mean(c)) / (n_atoms * steps)
+
+
+def _cpu_rate(make_run, moves_per_cycle, target_s):
+    """Independent copies of one ladder advanced by the oracle (FP64, the
+    reference arithmetic), one per host thread, like Threads.@threads over
+    replicas in the reference; aggregate attempted moves per second."""
+    threads = _cpu_threads()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        runs = list(ex.map(make_run, range(threads)))
+        t0 = time.perf_counter()
+        list(ex.map(lambda o: o.cycles(1), runs))
+        ncyc, dt = 1, time.perf_counter() - t0
+        if dt < 0.5 * target_s:  # the probe cycle is the sample when it is long enough
+            ncyc = max(1, int(target_s / max(dt, 1e-3)))
+            t0 = time.perf_counter()
+            list(ex.map(lambda o: o.cycles(ncyc), runs))
+            dt = time.perf_counter() - t0
+    return dict(cpu_moves_per_s=threads * moves_per_cycle * ncyc / dt, cpu_cores=threads, cpu_cycles=ncyc,
+                cpu_kind="port")
+
+
+def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
+    """Ne13 extended-LJ cluster in FP64: icosahedral start, 5.32 A container,
+    32 temperatures geometric 4-16 K, exchange every cycle."""
+    d = json.loads(NE13_SETUP.read_text())
+    pos = np.array(d["pos_angstrom"]) * d["AtoBohr"]
+    coeff, rad = np.array(d["coeff"]), d["radius_angstrom"] * d["AtoBohr"]
+    temp = pt.TempGrid(4.0, 16.0, 32)
+    mc = pt.MCParams(steps, 32, 13, n_adjust=100, mc_sample=1, n_bin=100)
+    eng = pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(13), pt.SphericalBC(rad), mc,
+                      pt.EngineOptions(precision="fp64", exchange="even_odd", seed=1234, recompute_every=1000))
+    for r in range(n_ladders):
+        eng.add_run(13, temp.beta_grid, pos[None], pt.SphericalBC(rad), run_key=r)
+    eng.prepare()
+    dt = _production(eng, n_ladders, warmup, steps)
+    out = dict(workload=f"Ne13 ELJ FP64, {n_ladders} ladder(s) x 32 temperatures 4-16 K, sphere 5.32 A",
+               moves_per_s=n_ladders * 32 * 13 * steps / dt, steps=steps, dtype="f64",
+               acceptance=_acceptance(eng, n_ladders, 13, steps),
+               max_drift=max(float(np.max(eng.drift(i))) for i in range(n_ladders)))
+    eng.close()
+    if cpu_s > 0:
+        from oracle import oracle as orc
+
+        orc.lib()
+        pot = orc.make_pot("elj_even", coeff)
+        out.update(_cpu_rate(lambda k: orc.OracleRun(13, temp.beta_grid, pos, pot=pot, radius=rad,
+                                                     mode=orc.REFERENCE, exchange_mode=2, seed=1234, run_key=k),
+                             32 * 13, cpu_s))
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_moves_per_s"]
+    return out
+
+
+def _bulk_start(n_atoms, vol_per_atom):
+    """fcc for 4 n^3 atoms, simple cubic for n^3 (216), at the given density."""
+    L = (n_atoms * vol_per_atom) ** (1.0 / 3.0)
+    nf = round((n_atoms / 4) ** (1.0 / 3.0))
+    if 4 * nf ** 3 == n_atoms:
+        base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+        n, a = nf, L / nf
+    else:
+        base = np.zeros((1, 3))
+        n = round(n_atoms ** (1.0 / 3.0))
+        a = L / n
+    pts = [(np.array([i, j, k]) + b) * a for i in range(n) for j in range(n) for k in range(n) for b in base]
+    return np.array(pts) - L / 2 + a / 4, L
+
+
+def bench_bulk(pt, n_atoms, ensemble, warmup, steps):
+    """Periodic neon (cubic box, minimum image) at the density of a 32-atom
+    fcc cell of edge 8.7674 A, 48 temperatures 10-40 K, binary32 pair sums
+    with FP64 totals; NPT adds one isotropic volume move per cycle at 1 atm."""
+    d = json.loads(NE13_SETUP.read_text())
+    coeff = np.array(d["coeff"])
+    x, L = _bulk_start(n_atoms, (8.7674 * A2B) ** 3 / 32)
+    temp = pt.TempGrid(10.0, 40.0, 48)
+    mc = pt.MCParams(steps, 48, n_atoms, n_adjust=100, mc_sample=1, n_bin=100)
+    ens = pt.NPT(n_atoms, P_ATM) if ensemble == "npt" else pt.NVT(n_atoms)
+    bc = pt.CubicBC(L)
+    eng = pt.PTEngine(pt.ELJPotentialEven(coeff), ens, bc, mc,
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000))
+    eng.add_run(n_atoms, temp.beta_grid, x[None], bc, pressure=P_ATM if ensemble == "npt" else 0.0, run_key=0)
+    eng.prepare()
+    dt = _production(eng, 1, warmup, steps)
+    out = dict(workload=f"bulk Ne{n_atoms} {ensemble.upper()} cubic box, 48 temperatures 10-40 K",
+               moves_per_s=48 * n_atoms * steps / dt, steps=steps, dtype="fp32",
+               acceptance=_acceptance(eng, 1, n_atoms, steps), max_fp32_vs_fp64_drift=float(np.max(eng.drift(0))))
+    eng.close()
+    return out
+
+
+def _cu55():
+    from paralleltemperingmontecarlo.nnp_setup import read_save_data, synthetic_cu55_potential
+
+    return synthetic_cu55_potential(NNP_DIR / "scaling.data"), read_save_data(NNP_DIR / "save.data")[0][0]
+
+
+def _nnp_engine(pt, pot, x0, n_ladders, steps):
+    temp = pt.TempGrid(200.0, 1400.0, 40)
+    rad = 14 * A2B
+    mc = pt.MCParams(steps, 40, 55, n_adjust=10, mc_sample=1, n_bin=100)
+    eng = pt.PTEngine(pot, pt.NVT(55), pt.SphericalBC(rad), mc,
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=100))
+    for r in range(n_ladders):
+        eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
+    eng.prepare()
+    return eng, temp, rad
+
+
+def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
+    """Cu55 with the Behler-Parrinello network (8 radial + 80 angular
+    functions, 88-20-20-1 tanh, N(0, 0.1) weights, key 4242), 40 temperatures
+    200-1400 K, 14 A container, binary32 features and layers, FP64 totals."""
+    pot, x0 = _cu55()
+    eng, temp, rad = _nnp_engine(pt, pot, x0, n_ladders, steps)
+    dt = _production(eng, n_ladders, warmup, steps)
+    out = dict(workload=f"NNP Cu55, {n_ladders} ladder(s) x 40 temperatures 200-1400 K",
+               moves_per_s=n_ladders * 40 * 55 * steps / dt, steps=steps, dtype="fp32",
+               acceptance=_acceptance(eng, n_ladders, 55, steps),
+               max_fp32_vs_fp64_drift=max(float(np.max(eng.drift(i))) for i in range(n_ladders)))
+    eng.close()
+    if cpu_s > 0:
+        from oracle import oracle as orc
+
+        orc.lib()
+        rt, at = pot.tables()
+        nn = orc.NNP(rt, at, pot.r_cut, pot.nnp.num_nodes, pot.nnp.activation_functions, pot.nnp.parameters)
+        opot = orc.make_pot("nnp", nnp=nn)
+        # a 4-temperature ladder over the same range: the cost of a move does
+        # not depend on the ladder length, and one cycle stays a few seconds
+        t4 = pt.TempGrid(200.0, 1400.0, 4)
+        out.update(_cpu_rate(lambda k: orc.OracleRun(55, t4.beta_grid, x0, pot=opot, radius=rad,
+                                                     mode=orc.REFERENCE, exchange_mode=2, seed=1234, run_key=k),
+                             4 * 55, cpu_s))
+        out["cpu_sample"] = "4-temperature ladders 200-1400 K, one per thread"
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_moves_per_s"]
+    return out
+
+
+def bench_delayed_acceptance(pt, n_ladders, warmup, steps, collect, epochs):
+    """Delayed acceptance on NNP Cu55: the same ladders first run plain,
+    then collect (configuration, energy) samples from their own cycles, train
+    the surrogate on the device and run with the surrogate as stage 1.  Every
+    proposal is a move of the exact chain, so proposals/s is the
+    exact-potential-equivalent rate."""
+    pot, x0 = _cu55()
+    eng, _, _ = _nnp_engine(pt, pot, x0, n_ladders, steps)
+    eng.surrogate_init(capacity=100000, seed=5)
+    eng.start_phase(False)
+    eng.cycles(warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.cycles(steps)
+    eng.sync()
+    plain = n_ladders * 40 * 55 * steps / (time.perf_counter() - t0)
+    eng.surrogate_collect(1)
+    eng.cycles(collect)
+    eng.surrogate_collect(0)
+    _, rmse0 = eng.train_surrogate(0)
+    t0 = time.perf_counter()
+    rmse_tr, rmse_va = eng.train_surrogate(epochs, batch=128, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    eng.delayed_acceptance(True)
+    eng.cycles(2)
+    eng.sync()
+    before = [eng.surrogate_stats(r) for r in range(n_ladders)]
+    t0 = time.perf_counter()
+    eng.cycles(steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    after = [eng.surrogate_stats(r) for r in range(n_ladders)]
+    d = {k: sum(a[k] - b[k] for a, b in zip(after, before)) for k in ("proposals", "stage1", "accepted")}
+    out = dict(workload=f"delayed acceptance, NNP Cu55, {n_ladders} ladders x 40 temperatures 200-1400 K",
+               moves_per_s=d["proposals"] / dt, plain_moves_per_s=plain, speedup_vs_plain=d["proposals"] / dt / plain,
+               exact_evaluations_per_proposal=d["stage1"] / d["proposals"],
+               acceptance=d["accepted"] / d["proposals"], stage2_acceptance=d["accepted"] / max(d["stage1"], 1),
+               samples=eng.surrogate_samples(), train_epochs=epochs, train_s=t_train,
+               surrogate_rmse_per_atom=dict(untrained=rmse0, train=rmse_tr, valid=rmse_va), steps=steps,
+               dtype="fp32 features/network, bf16 surrogate (matrix cores)")
+    eng.close()
+    return out
+
+
+def other_configs(pt, args):
+    cpu = 0.0 if args.no_cpu else args.cpu_seconds / 3
+    res = [bench_ne13(pt, 1, 500, 20000, cpu), bench_ne13(pt, 512, 500, 2000, 0.0)]
+    for n in (108, 216, 256):
+        for ens in ("nvt", "npt"):
+            res.append(bench_bulk(pt, n, ens, 200, 1000))
+    res.append(bench_nnp(pt, 1, 10, 40, cpu))
+    res.append(bench_nnp(pt, 64, 20, 10, 0.0))
+    res.append(bench_delayed_acceptance(pt, 256, 20, 4, 8, 30))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +377,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs shard "
@@ -256,6 +487,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(runs, args.cpu_seconds)
+    if not args.no_configs:
+        out["other_configs"] = other_configs(pt, args)
     print(json.dumps(out))
 
 

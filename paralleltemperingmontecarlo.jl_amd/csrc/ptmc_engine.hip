@@ -285,19 +285,20 @@ int plan(ptmc_engine *h)
         const int req = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
                         : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
                                                        : default_lanes(N, prec);
-        // smallest workgroup width that holds the whole ladder; more lanes per
-        // configuration when the slab does not fit LDS
+        // smallest built workgroup width that holds the whole ladder; more
+        // lanes per configuration when the slab does not fit LDS
         int kl = -1, lanes = 0;
-        for (int w : {LANES_SMALL, LANES_LARGE}) {
+        for (int w : {LANES_SMALL, LANES_MID, LANES_LARGE}) {
             if (eam && w != LANES_SMALL) continue;
-            int k = req;
-            if (w == LANES_LARGE && k < 2) k = 2;
-            while (k < 16 && tile_lds(rsize, N, k, w, eam) > (size_t)max_lds) k *= 2;
-            if (R * k <= w && tile_lds(rsize, N, k, w, eam) <= (size_t)max_lds) {
-                kl = k;
-                lanes = w;
-                break;
+            for (int k = (w != LANES_SMALL && req < 2) ? 2 : req; k <= 16 && R * k <= w; k *= 2) {
+                if (tile_lds(rsize, N, k, w, eam) <= (size_t)max_lds &&
+                    lookup(prec, h->cfg.boundary, h->cfg.ensemble, h->pot.kind, h->pk, h->kc, k, w).sweep) {
+                    kl = k;
+                    lanes = w;
+                    break;
+                }
             }
+            if (kl > 0) break;
         }
         if (kl < 0)
             return fail(h, PTMC_E_UNSUPPORTED, "run %d: %d temperatures x %d lanes per configuration do not fit one "

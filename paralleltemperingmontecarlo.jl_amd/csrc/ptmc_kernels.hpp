@@ -17,6 +17,7 @@ struct KernelPair {
 
 // Workgroup widths the kernels are built for (lanes = threads per workgroup).
 constexpr int LANES_SMALL = 256;
+constexpr int LANES_MID = 768;
 constexpr int LANES_LARGE = 1024;
 
 // Returns null kernels if the combination is not built.
@@ -33,7 +34,10 @@ KernelPair lookup_eam(int precision, int bc, int kl, int lanes);
 
 namespace ptmc {
 // 1024-lane workgroups only where the kernel fits 128 VGPRs without spilling
-// (binary32, spherical container, NVT).
+// (binary32, spherical container, NVT).  768-lane workgroups hold a
+// 48-temperature ladder of a periodic box at 8 or 16 lanes per configuration:
+// with 256-atom boxes the slab of 48 configurations (147 KB in binary32) fits
+// LDS only when the tile has exactly 48 slots.
 #define PTMC_PAIR(KL_, W_) \
     KernelPair { &sweep_kernel<Real, PK, KC, BC, ENS, KL_, W_>, &init_kernel<Real, PK, KC, BC, ENS, KL_, W_> }
 
@@ -48,6 +52,15 @@ KernelPair kl_switch(int kl, int lanes)
         case 8: return PTMC_PAIR(8, 256);
         case 16: return PTMC_PAIR(16, 256);
         default: return {};
+        }
+    }
+    if constexpr (sizeof(Real) == 4 && BC == BC_CUBIC) {
+        if (lanes == 768) {
+            switch (kl) {
+            case 8: return PTMC_PAIR(8, 768);
+            case 16: return PTMC_PAIR(16, 768);
+            default: return {};
+            }
         }
     }
     if constexpr (sizeof(Real) == 4 && BC == BC_SPHERE && ENS == ENS_NVT) {

@@ -165,6 +165,70 @@ def test_fp64_cubic_bit_exact(oracle, ne13, ensemble):
         eng.close()
 
 
+@pytest.mark.parametrize("n_cells,ensemble", [(3, "nvt"), (4, "npt")])
+def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble):
+    """A 48-temperature ladder of a periodic neon box (fcc 108 / 256 atoms at
+    the scripts/pbc32-short.jl density) in binary32 takes one 768-lane
+    workgroup.  Replaying the device decisions on the oracle reproduces every
+    move, every dE to 2e-5 of the atom energy scale, and the positions bit for
+    bit (NPT: with the volume moves).  The lattice is jittered by up to 0.05
+    bohr: in the perfect crystal many pairs sit exactly at the NPT cut-off
+    r^2 = L^2/4, where binary32 and FP64 distances fall on different sides
+    of the cut-off (a near-tie of the cut-off test, not of the arithmetic)."""
+    x, L = fcc(n_cells, 8.7674 * ne13["AtoBohr"] / 2)
+    x = x + np.random.default_rng(n_cells).uniform(-0.05, 0.05, x.shape)
+    N, R = len(x), 48
+    temp = pt.TempGrid(10.0, 40.0, R)
+    mc = pt.MCParams(100, R, N, n_adjust=3)
+    bc = pt.CubicBC(L)
+    p = P_ATM if ensemble == "npt" else 0.0
+    ens = pt.NPT(N, P_ATM) if ensemble == "npt" else pt.NVT(N)
+    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), ens, bc, mc, precision="fp32", exchange="even_odd", seed=31)
+    run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=p, run_key=4)
+    eng.prepare()
+    assert eng.group_info(eng.group_of(run))["workgroup_lanes"] == 768
+    ncyc = 6
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc = oracle.OracleRun(N, temp.beta_grid, x, pot=oracle.make_pot("elj_even", ne13["coeff"]), bc="cubic", box=L,
+                           ensemble=ensemble, pressure=p, mode=oracle.engine_mode(eng.lanes_of(run)), fp32=True,
+                           exchange_mode=2, n_adjust=3, seed=31, run_key=4)
+    scale = 4.0 * np.max(np.abs(orc.en_tot)) / N
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, replay_acc=acc)
+    assert np.array_equal(idx, tr["idx"])
+    assert 0 < acc.mean() < 1
+    err = np.abs(de - tr["de"])
+    atom = idx < N
+    x_dev, en_dev, box = eng.state(run)
+    e_scale = np.max(np.abs(orc.en_tot))
+    if ensemble == "nvt":
+        assert err.max() <= 2e-5 * scale, err.max() / scale
+        assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-5 * e_scale
+    else:
+        # NPT cut-off r^2 <= L^2/4: a pair within a few binary32 ulps of it
+        # can be counted on one side and not on the other (a near-tie of
+        # the cut-off test), which moves dE by one pair energy at r = L/2;
+        # that happens for about 1e-4 of the moves
+        r2c = np.min(np.concatenate([box, orc.box])) ** 2 / 4 * 0.99
+        e_rc = abs(sum(c * r2c ** (-(3 + k)) for k, c in enumerate(ne13["coeff"])))
+        big = err[atom] > 2e-5 * scale
+        assert big.mean() < 1e-3, big.mean()
+        assert err[atom].max() <= 2e-5 * scale + 3 * e_rc, (err[atom].max(), e_rc)
+        # a volume move's dE is an FP64 full energy minus the running total,
+        # so its error is the drift of the running total, flips included
+        assert err[~atom].max() <= 1e-4 * e_scale, err[~atom].max() / e_scale
+        assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-4 * e_scale
+    assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
+    assert np.array_equal(box, orc.box)
+    if ensemble == "npt":
+        assert eng.counters(run)["count_vol"][:, 0].sum() > 0
+    eng.close()
+
+
 def _lj_runs(n_atoms, n_temps, seed=7):
     """LJ cluster start: uniform in a sphere of radius 1.2 N^(1/3) sigma with
     minimum pair distance 0.8 sigma (sigma = epsilon = 1)."""
