@@ -124,13 +124,13 @@ struct ptmc_engine {
     // neural-network potential (one workgroup per slot)
     struct {
         std::vector<float> rad, geta, afn, rowoff, w;
-        std::vector<int32_t> grp, abuild;
+        std::vector<int32_t> grp, abuild, gdesc, wgroups;
         int n_rad = 0, n_ang = 0, nf = 0, h1 = 0, h2 = 0, n_rad_active = 0, n_groups = 0, q10 = 0;
         double r_cut = 0.0, hardcore2 = 0.0;
         float b3 = 0.0f;
     } nnp;
     DevBuf<float> d_nnp_rad, d_nnp_geta, d_nnp_afn, d_nnp_rowoff, d_nnp_w, d_feat, d_eatom;
-    DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_temp_slot;
+    DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_nnp_gdesc, d_nnp_wgroups, d_temp_slot;
     NnpTables nnp_dev{};
     size_t nnp_lds = 0;
 
@@ -528,7 +528,8 @@ int prepare_nnp(ptmc_engine *h)
     if ((rc = upload(h, h->d_nnp_rad, n.rad)) || (rc = upload(h, h->d_nnp_geta, n.geta)) ||
         (rc = upload(h, h->d_nnp_afn, n.afn)) || (rc = upload(h, h->d_nnp_rowoff, n.rowoff)) ||
         (rc = upload(h, h->d_nnp_w, n.w)) || (rc = upload(h, h->d_nnp_grp, n.grp)) ||
-        (rc = upload(h, h->d_nnp_abuild, n.abuild)))
+        (rc = upload(h, h->d_nnp_abuild, n.abuild)) || (rc = upload(h, h->d_nnp_gdesc, n.gdesc)) ||
+        (rc = upload(h, h->d_nnp_wgroups, n.wgroups)))
         return rc;
     if ((rc = dalloc(h, h->d_feat, (size_t)slot * n.nf * NNP_NP))) return rc;
     if ((rc = dalloc(h, h->d_eatom, (size_t)slot * NNP_NP))) return rc;
@@ -542,11 +543,28 @@ int prepare_nnp(ptmc_engine *h)
     t.n_groups = n.n_groups;
     t.r_cut = (float)n.r_cut;
     t.hardcore2 = n.hardcore2;
+    {
+        // The angular update of a non-moved atom accumulates in 32-bit fixed
+        // point (integer LDS adds run at the rate of plain stores, float adds
+        // far slower).  One visit changes a feature by at most tpz 2^zeta
+        // (both (1 + lambda cos)^zeta <= 2^zeta and the other factors <= 1),
+        // an atom has at most NNP_NP - 2 partners: the scale keeps the sum
+        // below 2^30.
+        double bound = 1e-30;
+        for (size_t f = 0; f < n.afn.size() / 4; ++f)
+            bound = std::max(bound, (double)std::fabs(n.afn[4 * f + 2]) * std::ldexp(1.0, 1 << (int)n.afn[4 * f + 1]));
+        bound *= (double)(NNP_NP - 2);
+        const int shift = std::min(60, (int)std::floor(std::log2(std::ldexp(1.0, 30) / bound)));
+        t.ang_scale = (float)std::ldexp(1.0, shift);
+        t.ang_unit = (float)std::ldexp(1.0, -shift);
+    }
     t.rad = h->d_nnp_rad.p;
     t.grp = h->d_nnp_grp.p;
     t.grp_eta = h->d_nnp_geta.p;
     t.afn = h->d_nnp_afn.p;
     t.abuild = h->d_nnp_abuild.p;
+    t.gdesc = h->d_nnp_gdesc.p;
+    t.wgroups = h->d_nnp_wgroups.p;
     t.row_off = h->d_nnp_rowoff.p;
     t.w1 = h->d_nnp_w.p;
     t.b1 = t.w1 + (size_t)n.nf * n.h1;
@@ -689,6 +707,8 @@ int ptmc_destroy(ptmc_handle h)
     h->d_eatom.free_();
     h->d_nnp_grp.free_();
     h->d_nnp_abuild.free_();
+    h->d_nnp_gdesc.free_();
+    h->d_nnp_wgroups.free_();
     h->d_temp_slot.free_();
     h->d_sur_w.free_();
     h->d_sur_m.free_();
@@ -796,37 +816,102 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
         n.rowoff[q] = (float)r[2];
         n.n_rad_active += 1;
     }
-    // angular functions grouped by eta (stable order), at most NNP_GMAX per group
+    // Angular functions grouped by eta (stable order), at most NNP_GMAX per
+    // group; inside a group lambda = +1 comes before -1 and zeta ascends, so
+    // the powers (1 + lambda cos)^zeta of a group follow from one chain of
+    // squarings per lambda.  On the device the feature rows of a group are
+    // consecutive (row = n_rad + table index; inactive functions follow), and
+    // the first-layer weights and the row offsets are permuted to match.
     const int n_built = q10_compat ? std::max(0, n_ang - n_rad) : n_ang;
+    auto zeta_code = [&](int q) {
+        for (int e = 0; e < NNP_KSLOT; ++e)
+            if (ang[6 * q + 2] == (double)(1 << e)) return e;
+        return -1;
+    };
     std::vector<int> idx;
-    for (int q = 0; q < n_ang; ++q)
-        if (ang[6 * q + 5] != 0.0) idx.push_back(q);
+    for (int q = 0; q < n_ang; ++q) {
+        if (ang[6 * q + 5] == 0.0) continue;
+        if (zeta_code(q) < 0)
+            return fail(h, PTMC_E_UNSUPPORTED, "angular function %d: zeta = %g is not a power of two up to %d", q,
+                        ang[6 * q + 2], 1 << (NNP_KSLOT - 1));
+        if (ang[6 * q + 1] != 1.0 && ang[6 * q + 1] != -1.0)
+            return fail(h, PTMC_E_UNSUPPORTED, "angular function %d: lambda = %g is not +-1", q, ang[6 * q + 1]);
+        idx.push_back(q);
+    }
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return ang[6 * x] < ang[6 * y]; });
+    std::vector<int> dev_row(nf, -1);  // original feature row -> device row
+    for (int q = 0; q < n_rad; ++q) dev_row[q] = q;
+    auto slot_of = [&](int q) { return (ang[6 * q + 1] > 0.0 ? 0 : NNP_KSLOT) + zeta_code(q); };
     for (size_t k = 0; k < idx.size();) {
         const double eta = ang[6 * idx[k]];
+        const size_t k0 = k;
+        uint32_t used = 0;  // one function per (lambda, k) slot in a group
+        while (k < idx.size() && ang[6 * idx[k]] == eta && !(used >> slot_of(idx[k]) & 1u)) used |= 1u << slot_of(idx[k++]);
+        std::stable_sort(idx.begin() + k0, idx.begin() + k, [&](int x, int y) { return slot_of(x) < slot_of(y); });
         const int first = (int)(n.afn.size() / 4);
-        int cnt = 0;
-        while (k < idx.size() && ang[6 * idx[k]] == eta && cnt < NNP_GMAX) {
-            const double *a = ang + 6 * idx[k];
+        std::vector<int32_t> d(NNP_GD, 0);
+        const float eta2 = (float)(eta * 1.4426950408889634);  // e^{-eta x} = 2^{-eta log2(e) x}
+        std::memcpy(&d[0], &eta2, 4);
+        d[1] = n_rad + first;
+        d[2] = d[3] = -1;
+        for (int sl = 0; sl < NNP_GMAX; ++sl) d[4 + NNP_GMAX + sl] = -1;
+        for (size_t e = k0; e < k; ++e) {
+            const int q = idx[e];
+            const double *a = ang + 6 * q;
+            const int row = n_rad + (int)(n.afn.size() / 4);
+            dev_row[n_rad + q] = row;
             // zeta = 2^k is stored as k (k squarings)
-            float zcode = -1.0f;
-            for (int e = 0; e <= 6; ++e)
-                if (a[2] == (double)(1 << e)) zcode = (float)e;
-            if (zcode < 0.0f)
-                return fail(h, PTMC_E_UNSUPPORTED, "angular function %d: zeta = %g is not a power of two up to 64",
-                            idx[k], a[2]);
-            n.afn.insert(n.afn.end(), {(float)a[1], zcode, (float)a[3], (float)(n_rad + idx[k])});
-            n.abuild.push_back(idx[k] < n_built ? 1 : 0);
-            if (idx[k] < n_built) n.rowoff[n_rad + idx[k]] = (float)a[4];
-            ++cnt;
-            ++k;
+            n.afn.insert(n.afn.end(), {(float)a[1], (float)zeta_code(q), (float)a[3], (float)row});
+            n.abuild.push_back(q < n_built ? 1 : 0);
+            if (q < n_built) n.rowoff[row] = (float)a[4];
+            const int sl = slot_of(q);
+            const float tpz = (float)a[3];
+            std::memcpy(&d[4 + sl], &tpz, 4);
+            d[4 + NNP_GMAX + sl] = row - d[1];
+            int &kmax = d[sl < NNP_KSLOT ? 2 : 3];
+            kmax = std::max(kmax, zeta_code(q));
         }
-        n.grp.insert(n.grp.end(), {first, cnt});
+        n.gdesc.insert(n.gdesc.end(), d.begin(), d.end());
+        n.grp.insert(n.grp.end(), {first, (int)(k - k0)});
         n.geta.push_back((float)eta);
         n.n_groups += 1;
     }
+    if (n.n_groups > NNP_MAXG)
+        return fail(h, PTMC_E_UNSUPPORTED, "%d angular groups (distinct eta, or repeated (lambda, zeta) within one eta); "
+                    "at most %d", n.n_groups, NNP_MAXG);
+    // groups dealt to the waves of the move update (each wave owns its groups'
+    // rows): largest work first onto the least loaded wave, at most NNP_WG each
+    {
+        constexpr int NW = NNP_NT / 64;
+        std::vector<int> order(n.n_groups);
+        std::vector<double> cost(n.n_groups), load(NW, 0.0);
+        std::vector<std::vector<int>> lists(NW);
+        for (int g = 0; g < n.n_groups; ++g) {
+            const int32_t *d = &n.gdesc[(size_t)g * NNP_GD];
+            order[g] = g;
+            cost[g] = 2.0 + 2.0 * ((d[2] + 1) + (d[3] + 1));  // exps + per-slot chain work
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        for (int g : order) {
+            int best = -1;
+            for (int w = 0; w < NW; ++w)
+                if ((int)lists[w].size() < NNP_WG && (best < 0 || load[w] < load[best])) best = w;
+            lists[best].push_back(g);
+            load[best] += cost[g];
+        }
+        n.wgroups.assign((size_t)NW * (NNP_WG + 1), 0);
+        for (int w = 0; w < NW; ++w) {
+            n.wgroups[(size_t)w * (NNP_WG + 1)] = (int32_t)lists[w].size();
+            for (size_t e = 0; e < lists[w].size(); ++e) n.wgroups[(size_t)w * (NNP_WG + 1) + 1 + e] = lists[w][e];
+        }
+    }
+    int next = n_rad + (int)(n.afn.size() / 4);
+    for (int q = 0; q < n_ang; ++q)
+        if (dev_row[n_rad + q] < 0) dev_row[n_rad + q] = next++;
     n.w.resize((size_t)n_params - 1);
     for (int k = 0; k + 1 < n_params; ++k) n.w[k] = (float)params[k];
+    for (int q = 0; q < nf; ++q)
+        for (int j = 0; j < h1; ++j) n.w[(size_t)dev_row[q] * h1 + j] = (float)params[(size_t)q * h1 + j];
     n.b3 = (float)params[n_params - 1];
     PotParams p{};
     p.kind = POT_NNP;

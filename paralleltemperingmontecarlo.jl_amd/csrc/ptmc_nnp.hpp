@@ -22,13 +22,15 @@ struct NnpLds {
     float *X, *G, *D, *W1, *H1, *H2, *rn2, *fn, *ro2, *fo, *eat, *eatn;
     // function tables copied from the host-prepared arrays
     float *rad, *afn, *geta;
-    int32_t *grp, *abuild;
+    int32_t *grp, *abuild, *gd, *wg;
 };
 
 // static LDS holding the function tables (declared in each kernel)
 #define NNP_TABLES_LDS                                                         \
     __shared__ float s_rad[4 * NNP_MAXF], s_afn[4 * NNP_MAXF], s_geta[NNP_MAXF]; \
-    __shared__ int32_t s_grp[2 * NNP_MAXF], s_abuild[NNP_MAXF]
+    __shared__ int32_t s_grp[2 * NNP_MAXF], s_abuild[NNP_MAXF];               \
+    __shared__ __align__(16) int32_t s_gdesc[NNP_MAXG * NNP_GD];              \
+    __shared__ int32_t s_wgrp[(NNP_NT / 64) * (NNP_WG + 1)]
 
 __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
 {
@@ -46,7 +48,7 @@ __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
     l.eat = l.fo + NNP_NP;
     l.eatn = l.eat + NNP_NP;
     l.rad = l.afn = l.geta = nullptr;
-    l.grp = l.abuild = nullptr;
+    l.grp = l.abuild = l.gd = l.wg = nullptr;
     return l;
 }
 
@@ -72,6 +74,20 @@ __device__ __forceinline__ void nnp_theta3(float lam, float kpow, float c0, floa
     }
 }
 
+// cosine of the angle at c between c->p and c->q (Cutoff.jl:48-58)
+__device__ __forceinline__ float nnp_cos(const float *c, const float *p, const float *q, float dcp, float dcq)
+{
+    const float ux = p[0] - c[0], uy = p[1] - c[1], uz = p[2] - c[2];
+    const float wx = q[0] - c[0], wy = q[1] - c[1], wz = q[2] - c[2];
+    return (ux * wx + uy * wy + uz * wz) / (dcp * dcq);
+}
+
+// number of set bits of a 64-lane mask below this lane
+__device__ __forceinline__ int mbcnt64(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // cosines of the triangle (a, b, c) at a, b and c (Cutoff.jl:48-58)
 __device__ __forceinline__ void nnp_angles(const float *a, const float *b, const float *c, float dab, float dac,
                                            float dbc, float &ta, float &tb, float &tc)
@@ -91,18 +107,131 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-// Partner pair p of the N(N-1)/2 pairs j < k, ordered by diagonal
-// (k - j = 1, 2, ...): consecutive pairs have distinct j and distinct k, so
-// the lanes of a wave seldom add into the same feature entry.
-__device__ __forceinline__ void nnp_pair(int p, int N, int &j, int &k)
+// DPP lane permutation of a float (all 64 lanes active)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
 {
-    int d = 1, start = 0;
-    while (start + (N - d) <= p) {
-        start += N - d;
-        ++d;
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+
+// Sum over each row of 16 lanes, left in every lane of the row: quad
+// exchanges (quad_perm [1,0,3,2] and [2,3,0,1]), then half-row and row mirrors.
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    return v;
+}
+
+// Sum over all 64 lanes, complete in lane 63: row sums, then the last lane
+// of rows 0 and 2 broadcast into rows 1 and 3, then lane 31 into rows 2 and 3.
+__device__ __forceinline__ float wave_sum_last(float v)
+{
+    v = row_sum16(v);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x142, 0xa, 0xf, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+
+// Accumulate e_new (1 + lam c_new)^(2^k) + e_old (1 + lam c_old)^(2^k) into
+// acc[S0 + k] for k = 0..kmax (kmax uniform; -1: no function of this lambda).
+// eo carries the sign of the old term.
+template <int S0>
+__device__ __forceinline__ void nnp_chain(int kmax, float lam, float cn, float co, float en, float eo, float *acc)
+{
+    if (kmax < 0) return;
+    float pn = 1.0f + lam * cn, po = 1.0f + lam * co;
+#pragma unroll
+    for (int k = 0; k < NNP_KSLOT; ++k) {
+        if (k <= kmax) {
+            acc[S0 + k] = __builtin_fmaf(en, pn, acc[S0 + k]);
+            acc[S0 + k] = __builtin_fmaf(eo, po, acc[S0 + k]);
+            pn *= pn;
+            po *= po;
+        }
     }
-    j = p - start;
-    k = j + d;
+}
+
+// One eta group of angular functions as the move update reads it: e^{-eta x}
+// = 2^{-eta2 x}; slots (lambda, k) = (+1, 0..4), (-1, 0..4); kp / km the
+// largest k per lambda (-1: none); per slot tpz and the row (-1: none).
+struct NnpGroup {
+    float eta2;
+    int row0, kp, km;
+    int rk[NNP_GMAX];
+    float tpz[NNP_GMAX];
+    __device__ void none()
+    {
+        eta2 = 0.0f;
+        row0 = 0;
+        kp = km = -1;
+#pragma unroll
+        for (int q = 0; q < NNP_GMAX; ++q) {
+            rk[q] = -1;
+            tpz[q] = 0.0f;
+        }
+    }
+};
+
+// Descriptor of group g from LDS in 16-byte reads; every word is uniform.
+__device__ __forceinline__ void nnp_group(const int32_t *gd, int g, NnpGroup &G)
+{
+    int dw[NNP_GD];
+#pragma unroll
+    for (int u = 0; u < NNP_GD / 4; ++u) {
+        const int4 w4 = reinterpret_cast<const int4 *>(gd + NNP_GD * g)[u];
+        dw[4 * u] = w4.x;
+        dw[4 * u + 1] = w4.y;
+        dw[4 * u + 2] = w4.z;
+        dw[4 * u + 3] = w4.w;
+    }
+    G.eta2 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(dw[0]));
+    G.row0 = __builtin_amdgcn_readfirstlane(dw[1]);
+    G.kp = __builtin_amdgcn_readfirstlane(dw[2]);
+    G.km = __builtin_amdgcn_readfirstlane(dw[3]);
+#pragma unroll
+    for (int q = 0; q < NNP_GMAX; ++q) {
+        G.tpz[q] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(dw[4 + q]));
+        G.rk[q] = __builtin_amdgcn_readfirstlane(dw[4 + NNP_GMAX + q]);
+    }
+}
+
+// One visit (j, k) for one group: the terms centred on j into aj and, for
+// own visits, those centred on i into ai (per slot, tpz applied at the store).
+__device__ __forceinline__ void nnp_visit(const NnpGroup &G, float rsn, float rso, float fpn, float fpo, float cjn,
+                                          float cjo, float cin, float cio, bool own, float *aj, float *ai)
+{
+    const float en = __builtin_amdgcn_exp2f(-G.eta2 * rsn) * fpn;
+    const float eo = -(__builtin_amdgcn_exp2f(-G.eta2 * rso) * fpo);
+    nnp_chain<0>(G.kp, 1.0f, cjn, cjo, en, eo, aj);
+    nnp_chain<NNP_KSLOT>(G.km, -1.0f, cjn, cjo, en, eo, aj);
+    if (own) {
+        nnp_chain<0>(G.kp, 1.0f, cin, cio, en, eo, ai);
+        nnp_chain<NNP_KSLOT>(G.km, -1.0f, cin, cio, en, eo, ai);
+    }
+}
+
+// Stores of one group: the P lanes of a target combine their parts (quad
+// exchanges), the first writes the target's entry; the wave's sum of the
+// terms centred on i goes to the moved atom's column.  All lanes call.
+__device__ __forceinline__ void nnp_store(const NnpGroup &G, int P, bool writer, int j, int i, int lane, float *Dp,
+                                          float *aj, float *ai)
+{
+#pragma unroll
+    for (int q = 0; q < NNP_GMAX; ++q) {
+        if (G.rk[q] >= 0) {
+            float v = aj[q];
+            if (P >= 2) v += dpp_f<0xB1>(v);
+            if (P >= 4) v += dpp_f<0x4E>(v);
+            float *Dr = Dp + (G.row0 + G.rk[q]) * NNP_NP;
+            if (writer) Dr[j] = v * G.tpz[q];
+            const float vi = wave_sum_last(ai[q]);
+            if (lane == 63) Dr[i] = vi * G.tpz[q];
+        }
+    }
 }
 
 // Full feature matrix from positions into Gout (total_symm_calc,
@@ -244,6 +373,8 @@ __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, in
     for (int e = tid; e < 4 * T.n_rad_active; e += NNP_NT) L.rad[e] = T.rad[e];
     for (int e = tid; e < 4 * nfun; e += NNP_NT) L.afn[e] = T.afn[e];
     for (int e = tid; e < nfun; e += NNP_NT) L.abuild[e] = T.abuild[e];
+    for (int e = tid; e < NNP_GD * T.n_groups; e += NNP_NT) L.gd[e] = T.gdesc[e];
+    for (int e = tid; e < (NNP_NT / 64) * (NNP_WG + 1); e += NNP_NT) L.wg[e] = T.wgroups[e];
     for (int e = tid; e < T.n_groups; e += NNP_NT) {
         L.geta[e] = T.grp_eta[e];
         L.grp[2 * e] = T.grp[2 * e];
@@ -256,7 +387,9 @@ __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, in
     L.afn = s_afn;         \
     L.geta = s_geta;       \
     L.grp = s_grp;         \
-    L.abuild = s_abuild
+    L.abuild = s_abuild;   \
+    L.gd = s_gdesc;        \
+    L.wg = s_wgrp
 
 // set_variables + initialise_energy (src/EnergyEvaluation.jl:1812-1822,
 // :1735-1747): features, per-atom energies and the FP64 total of each slot.
@@ -292,8 +425,8 @@ template <bool DA>
 __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA ? 2 : 4))) nnp_sweep_kernel(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ int s_flag, s_err;
-    __shared__ uint16_t s_pairs[NNP_NP * (NNP_NP - 1) / 2];
+    __shared__ int s_flag, s_err, s_nnear;
+    __shared__ uint8_t s_near[NNP_NP];
     __shared__ double s_en;
     __shared__ float s_trial[3], s_old[3];
     const NnpTables &T = a.nnp;
@@ -354,12 +487,6 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
         __syncthreads();
         Ehat = s_ehat;
     }
-    const int npairs = N * (N - 1) / 2;
-    for (int p = tid; p < npairs; p += NNP_NT) {
-        int j, k;
-        nnp_pair(p, N, j, k);
-        s_pairs[p] = (uint16_t)((j << 8) | k);
-    }
     __syncthreads();
     int nacc = 0;
     // current features and the scratch that becomes the trial features; an
@@ -409,21 +536,32 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
         }
         const float pt[3] = {tx, ty, tz}, po[3] = {xi, yi, zi};
         // distances and cutoffs to the new and old position; hard-core test
+        // (wave 0, one lane per atom) and the list of the atoms within the
+        // cut-off of either position, the only partners of angular terms
         if (tid == 0) s_flag = 0;
         __syncthreads();
-        for (int j = tid; j < N; j += NNP_NT) {
-            const float px = L.X[j], py = L.X[NNP_NP + j], pz = L.X[2 * NNP_NP + j];
-            float dx = tx - px, dy = ty - py, dz = tz - pz;
-            const float rn = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
-            dx = xi - px;
-            dy = yi - py;
-            dz = zi - pz;
-            const float ro = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
-            L.rn2[j] = rn;
-            L.ro2[j] = ro;
-            L.fn[j] = nnp_cutoff(sqrtf(rn), T.r_cut);
-            L.fo[j] = nnp_cutoff(sqrtf(ro), T.r_cut);
-            if (j != i && (double)rn < T.hardcore2) s_flag = 1;
+        if (tid < 64) {
+            const int j = lane;
+            bool near = false;
+            if (j < N) {
+                const float px = L.X[j], py = L.X[NNP_NP + j], pz = L.X[2 * NNP_NP + j];
+                float dx = tx - px, dy = ty - py, dz = tz - pz;
+                const float rn = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
+                dx = xi - px;
+                dy = yi - py;
+                dz = zi - pz;
+                const float ro = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
+                const float fnj = nnp_cutoff(sqrtf(rn), T.r_cut), foj = nnp_cutoff(sqrtf(ro), T.r_cut);
+                L.rn2[j] = rn;
+                L.ro2[j] = ro;
+                L.fn[j] = fnj;
+                L.fo[j] = foj;
+                if (j != i && (double)rn < T.hardcore2) s_flag = 1;
+                near = j != i && (fnj != 0.0f || foj != 0.0f);
+            }
+            const uint64_t b = __ballot(near);
+            if (near) s_near[mbcnt64(b)] = (uint8_t)j;
+            if (lane == 0) s_nnear = __popcll(b);
         }
         for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] = 0.0f;
         if (da && tid < SUR_K) s_acci[tid] = 0.0f;
@@ -475,79 +613,87 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 const float s = wave_sum(d);
                 if (lane == 0) Dp[row * NNP_NP + i] = s;
             }
-            // angular changes, DeltaMatrix.jl:827-872: partner pairs (j, k), j, k != i
-            float rsn[NNP_PP], rso[NNP_PP], fpn[NNP_PP], fpo[NNP_PP];
-            float cn[NNP_PP][3], co[NNP_PP][3];
-            int pj[NNP_PP], pk[NNP_PP];
-#pragma unroll
-            for (int r = 0; r < NNP_PP; ++r) {
-                const int p = tid + r * NNP_NT;
-                fpn[r] = 0.0f;
-                fpo[r] = 0.0f;
-                rsn[r] = rso[r] = 0.0f;
-                pj[r] = pk[r] = 0;
-                cn[r][0] = cn[r][1] = cn[r][2] = co[r][0] = co[r][1] = co[r][2] = 0.0f;
-                if (p >= npairs) continue;
-                const int j = s_pairs[p] >> 8, k = s_pairs[p] & 0xff;
-                if (j == i || k == i) continue;
-                const float pj3[3] = {L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j]};
-                const float pk3[3] = {L.X[k], L.X[NNP_NP + k], L.X[2 * NNP_NP + k]};
-                const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
-                const float rjk = dx * dx + dy * dy + dz * dz;
-                const float djk = sqrtf(rjk);
-                const float fjk = nnp_cutoff(djk, T.r_cut);
-                if (fjk == 0.0f) continue;
-                pj[r] = j;
-                pk[r] = k;
-                fpn[r] = L.fn[j] * L.fn[k] * fjk;
-                fpo[r] = L.fo[j] * L.fo[k] * fjk;
-                rsn[r] = L.rn2[j] + L.rn2[k] + rjk;
-                rso[r] = L.ro2[j] + L.ro2[k] + rjk;
-                if (fpn[r] != 0.0f)
-                    nnp_angles(pt, pj3, pk3, sqrtf(L.rn2[j]), sqrtf(L.rn2[k]), djk, cn[r][0], cn[r][1], cn[r][2]);
-                if (fpo[r] != 0.0f)
-                    nnp_angles(po, pj3, pk3, sqrtf(L.ro2[j]), sqrtf(L.ro2[k]), djk, co[r][0], co[r][1], co[r][2]);
-            }
-            for (int g = 0; g < T.n_groups; ++g) {
-                const float eta = L.geta[g];
-                const int f0 = __builtin_amdgcn_readfirstlane(L.grp[2 * g]);
-                const int cnt = __builtin_amdgcn_readfirstlane(L.grp[2 * g + 1]);
-                float acc[NNP_GMAX], flam[NNP_GMAX], fzc[NNP_GMAX], ftpz[NNP_GMAX];
-                int frow[NNP_GMAX];
-#pragma unroll
-                for (int s = 0; s < NNP_GMAX; ++s) {
-                    acc[s] = 0.0f;
-                    const int f = f0 + (s < cnt ? s : 0);
-                    flam[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f])));
-                    fzc[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f + 1])));
-                    ftpz[s] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, L.afn[4 * f + 2])));
-                    frow[s] = __builtin_amdgcn_readfirstlane((int)L.afn[4 * f + 3]);
-                }
-#pragma unroll
-                for (int r = 0; r < NNP_PP; ++r) {
-                    if (fpn[r] == 0.0f && fpo[r] == 0.0f) continue;
-                    const float en = expf(-eta * rsn[r]) * fpn[r];
-                    const float eo = expf(-eta * rso[r]) * fpo[r];
-#pragma unroll
-                    for (int s = 0; s < NNP_GMAX; ++s) {
-                        if (s >= cnt) break;
-                        const float lam = flam[s], zc = fzc[s], tpz = ftpz[s];
-                        const int row = frow[s];
-                        float n0, n1, n2, o0, o1, o2;
-                        nnp_theta3(lam, zc, cn[r][0], cn[r][1], cn[r][2], n0, n1, n2);
-                        nnp_theta3(lam, zc, co[r][0], co[r][1], co[r][2], o0, o1, o2);
-                        const float ent = en * tpz, eot = eo * tpz;
-                        acc[s] += ent * n0 - eot * o0;
-                        atomicAdd(&Dp[row * NNP_NP + pj[r]], ent * n1 - eot * o1);
-                        atomicAdd(&Dp[row * NNP_NP + pk[r]], ent * n2 - eot * o2);
+            // angular changes, DeltaMatrix.jl:827-872.  Every partner pair
+            // (j, k) of the moved atom i whose triangle lies inside the cut-off
+            // (old or new position) is visited from j and from k; a visit adds
+            // the terms centred on its own atom and, from the atom of smaller
+            // index, the terms centred on i.  Each wave owns a few eta groups
+            // (their feature rows), each lane one target atom j (or a half or
+            // quarter of its partners when few atoms are near), so every
+            // feature entry is written by one lane with a plain store: no LDS
+            // atomics (float LDS adds run far below the store rate).
+            const int n_near = s_nnear;
+            uint64_t *s_mask = reinterpret_cast<uint64_t *>(L.H1);  // scratch until the network pass
+            const float rc2 = T.r_cut * T.r_cut * 1.0001f;  // loose: the cut-off function decides below
+            for (int a = tid >> 6; a < n_near; a += NNP_NT / 64) {
+                const int j = s_near[a];
+                bool act = false;
+                if (lane < n_near && lane != a) {
+                    const int k = s_near[lane];
+                    if (L.fn[j] * L.fn[k] != 0.0f || L.fo[j] * L.fo[k] != 0.0f) {
+                        const float dx = L.X[j] - L.X[k], dy = L.X[NNP_NP + j] - L.X[NNP_NP + k],
+                                    dz = L.X[2 * NNP_NP + j] - L.X[2 * NNP_NP + k];
+                        act = dx * dx + dy * dy + dz * dz < rc2;
                     }
                 }
+                const uint64_t m = __ballot(act);
+                if (lane == 0) s_mask[a] = m;
+            }
+            __syncthreads();
+            // lane -> (target ta, part): P lanes share a target when it fits
+            const int lp = n_near <= 16 ? 2 : (n_near <= 32 ? 1 : 0);  // log2 P
+            const int P = 1 << lp;
+            const int ta = lane >> lp, part = lane & (P - 1);
+            const bool tact = ta < n_near;
+            const int j = tact ? (int)s_near[ta] : 0;
+            const uint64_t mask = tact ? s_mask[ta] : 0ull;
+            const uint64_t after = ta < 63 ? ~((2ull << ta) - 1ull) : 0ull;  // partners of larger index
+            const float pj3[3] = {L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j]};
+            const float dijn = sqrtf(L.rn2[j]), dijo = sqrtf(L.ro2[j]);
+            const int w = tid >> 6;
+            const int ngw = L.wg[w * (NNP_WG + 1)];
+            for (int gi = 0; gi < ngw; gi += 2) {
+                // up to two groups per pass over the visits
+                const int ga = L.wg[w * (NNP_WG + 1) + 1 + gi];
+                const int gb = gi + 1 < ngw ? L.wg[w * (NNP_WG + 1) + 2 + gi] : -1;
+                NnpGroup G0, G1;
+                nnp_group(L.gd, ga, G0);
+                if (gb >= 0) nnp_group(L.gd, gb, G1);
+                else G1.none();
+                float aj0[NNP_GMAX], ai0[NNP_GMAX], aj1[NNP_GMAX], ai1[NNP_GMAX];
 #pragma unroll
-                for (int s = 0; s < NNP_GMAX; ++s) {
-                    if (s >= cnt) break;
-                    const float v = wave_sum(acc[s]);
-                    if (lane == 0 && v != 0.0f) atomicAdd(&Dp[frow[s] * NNP_NP + i], v);
+                for (int q = 0; q < NNP_GMAX; ++q) aj0[q] = ai0[q] = aj1[q] = ai1[q] = 0.0f;
+                uint64_t m = mask;
+                for (int t = 0; t < part; ++t) m &= m - 1ull;  // this lane's first partner
+                while (m != 0ull) {
+                    const int bk = __builtin_ctzll(m);
+                    for (int t = 0; t < P; ++t) m &= m - 1ull;  // next partner of this lane
+                    const int k = s_near[bk];
+                    const bool own = (after >> bk) & 1ull;
+                    const float pk3[3] = {L.X[k], L.X[NNP_NP + k], L.X[2 * NNP_NP + k]};
+                    const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
+                    const float rjk = dx * dx + dy * dy + dz * dz;
+                    const float djk = sqrtf(rjk);
+                    const float fjk = nnp_cutoff(djk, T.r_cut);
+                    const float fpn = L.fn[j] * L.fn[k] * fjk, fpo = L.fo[j] * L.fo[k] * fjk;
+                    if (fpn == 0.0f && fpo == 0.0f) continue;
+                    const float rsn = L.rn2[j] + L.rn2[k] + rjk, rso = L.ro2[j] + L.ro2[k] + rjk;
+                    // cosines at j (and at i), new and old
+                    float cjn = 0.0f, cjo = 0.0f, cin = 0.0f, cio = 0.0f;
+                    const float dikn = sqrtf(L.rn2[k]), diko = sqrtf(L.ro2[k]);
+                    if (fpn != 0.0f) {
+                        cjn = nnp_cos(pj3, pt, pk3, dijn, djk);
+                        if (own) cin = nnp_cos(pt, pj3, pk3, dijn, dikn);
+                    }
+                    if (fpo != 0.0f) {
+                        cjo = nnp_cos(pj3, po, pk3, dijo, djk);
+                        if (own) cio = nnp_cos(po, pj3, pk3, dijo, diko);
+                    }
+                    nnp_visit(G0, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj0, ai0);
+                    if (gb >= 0) nnp_visit(G1, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj1, ai1);
                 }
+                nnp_store(G0, P, tact && part == 0, j, i, lane, Dp, aj0, ai0);
+                if (gb >= 0) nnp_store(G1, P, tact && part == 0, j, i, lane, Dp, aj1, ai1);
             }
             __syncthreads();
             for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] += Gp[e];

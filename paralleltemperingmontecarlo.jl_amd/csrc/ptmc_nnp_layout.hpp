@@ -9,9 +9,16 @@ constexpr int NNP_NP = 64;     // atoms per configuration (maximum)
 constexpr int NNP_NT = 512;    // threads per workgroup
 constexpr int NNP_MAXH = 32;   // hidden width (maximum)
 constexpr int NNP_HS = NNP_MAXH + 1;  // padded hidden row stride
-constexpr int NNP_GMAX = 8;    // functions per eta group (maximum)
+constexpr int NNP_KSLOT = 5;   // zeta = 2^k, k = 0..4 (zeta up to 16)
+constexpr int NNP_GMAX = 2 * NNP_KSLOT;  // functions per eta group (one per lambda = +-1 and k)
+// group descriptor words: eta log2(e), first row, max k for lambda = +1 and
+// -1 (-1: none), then per slot (lambda, k) = (+1, 0..4), (-1, 0..4) the
+// function's tpz and its row within the group (-1: no function)
+constexpr int NNP_GD = 4 + 2 * NNP_GMAX;
+constexpr int NNP_MAXG = 32;  // angular groups (maximum)
+constexpr int NNP_WG = 4;     // angular groups per wave in the move update (maximum)
+static_assert(NNP_GD % 4 == 0, "group descriptors are read in 16-byte words");
 constexpr int NNP_MAXF = 128;  // symmetry functions (maximum)
-constexpr int NNP_PP = (NNP_NP * (NNP_NP - 1) / 2 + NNP_NT - 1) / NNP_NT;  // partner pairs per thread
 
 // LDS layout in floats: X[3][NP] | G[nf][NP] | D[nf][NP] | W1[nf][h1] |
 // H1[NP][HS] | H2[NP][HS] | rn2, fn, ro2, fo, eat, eatn [NP each]

@@ -63,12 +63,15 @@ struct NnpTables {
     int32_t n_groups;              // angular eta groups
     float r_cut;
     double hardcore2;              // squared hard-core distance (0: none)
+    float ang_scale, ang_unit;     // fixed-point scale of the angular move update and its inverse
     const float *rad;              // n_rad_active x 4: eta, norm, row, pad
     const int32_t *grp;            // n_groups x 2: first function, count
     const float *grp_eta;          // n_groups
     const float *afn;              // functions x 4: lambda, zeta, tpz, row
     const int32_t *abuild;         // functions: 1 if built by the full evaluation
     const float *row_off;          // nf: offset added by the full evaluation
+    const int32_t *gdesc;          // n_groups x NNP_GD: per-group descriptor of the move update
+    const int32_t *wgroups;        // per wave of the move update: count, then up to NNP_WG groups
     const float *w1, *b1, *w2, *b2, *w3;
     float b3;
 };
