@@ -20,15 +20,15 @@ namespace ptmc {
 
 struct NnpLds {
     float *X, *G, *D, *W1, *H1, *H2, *rn2, *fn, *ro2, *fo, *eat, *eatn;
-    // function tables copied from the host-prepared arrays
-    float *rad, *afn, *geta;
-    int32_t *grp, *abuild, *gd, *wg;
+    // tables copied from the host-prepared arrays: radial functions, angular
+    // group descriptors and their waves, second-layer weights, b1 | b2 | w3
+    float *rad, *W2, *B;
+    int32_t *gd, *wg;
 };
 
 // static LDS holding the function tables (declared in each kernel)
 #define NNP_TABLES_LDS                                                         \
-    __shared__ float s_rad[4 * NNP_MAXF], s_afn[4 * NNP_MAXF], s_geta[NNP_MAXF]; \
-    __shared__ int32_t s_grp[2 * NNP_MAXF], s_abuild[NNP_MAXF];               \
+    __shared__ float s_rad[4 * NNP_MAXF], s_w2[NNP_MAXH * NNP_MAXH], s_bias[3 * NNP_MAXH]; \
     __shared__ __align__(16) int32_t s_gdesc[NNP_MAXG * NNP_GD];              \
     __shared__ int32_t s_wgrp[(NNP_NT / 64) * (NNP_WG + 1)]
 
@@ -47,8 +47,8 @@ __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
     l.fo = l.ro2 + NNP_NP;
     l.eat = l.fo + NNP_NP;
     l.eatn = l.eat + NNP_NP;
-    l.rad = l.afn = l.geta = nullptr;
-    l.grp = l.abuild = l.gd = l.wg = nullptr;
+    l.rad = l.W2 = l.B = nullptr;
+    l.gd = l.wg = nullptr;
     return l;
 }
 
@@ -214,18 +214,23 @@ __device__ __forceinline__ void nnp_visit(const NnpGroup &G, float rsn, float rs
     }
 }
 
-// Stores of one group: the P lanes of a target combine their parts (quad
-// exchanges), the first writes the target's entry; the wave's sum of the
-// terms centred on i goes to the moved atom's column.  All lanes call.
-__device__ __forceinline__ void nnp_store(const NnpGroup &G, int P, bool writer, int j, int i, int lane, float *Dp,
+// Stores of one group: the 2^lp lanes of a target (an aligned block)
+// combine their parts by quad exchanges and the half-row mirror, the first
+// writes the target's entry; the wave's sum of the terms centred on i goes
+// to the moved atom's column.  All lanes call.
+__device__ __forceinline__ void nnp_store(const NnpGroup &G, int lp, bool writer, int j, int i, int lane, float *Dp,
                                           float *aj, float *ai)
 {
 #pragma unroll
     for (int q = 0; q < NNP_GMAX; ++q) {
         if (G.rk[q] >= 0) {
             float v = aj[q];
-            if (P >= 2) v += dpp_f<0xB1>(v);
-            if (P >= 4) v += dpp_f<0x4E>(v);
+            const float t1 = dpp_f<0xB1>(v);
+            v = lp >= 1 ? v + t1 : v;
+            const float t2 = dpp_f<0x4E>(v);
+            v = lp >= 2 ? v + t2 : v;
+            const float t3 = dpp_f<0x141>(v);
+            v = lp >= 3 ? v + t3 : v;
             float *Dr = Dp + (G.row0 + G.rk[q]) * NNP_NP;
             if (writer) Dr[j] = v * G.tpz[q];
             const float vi = wave_sum_last(ai[q]);
@@ -287,13 +292,13 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout,
         nnp_angles(pa, pb, pc, dab, dac, dbc, ta, tb, tc);
         const float rs = rab + rac + rbc;
         for (int g = 0; g < T.n_groups; ++g) {
-            const float ex = expf(-L.geta[g] * rs) * fp;
-            const int f0 = L.grp[2 * g], cnt = L.grp[2 * g + 1];
+            const float ex = expf(-T.grp_eta[g] * rs) * fp;
+            const int f0 = T.grp[2 * g], cnt = T.grp[2 * g + 1];
             for (int s = 0; s < cnt; ++s) {
                 const int f = f0 + s;
-                if (!L.abuild[f]) continue;
-                const float lam = L.afn[4 * f], zc = L.afn[4 * f + 1], tpz = L.afn[4 * f + 2];
-                const int row = (int)L.afn[4 * f + 3];
+                if (!T.abuild[f]) continue;
+                const float lam = T.afn[4 * f], zc = T.afn[4 * f + 1], tpz = T.afn[4 * f + 2];
+                const int row = (int)T.afn[4 * f + 3];
                 float pa_, pb_, pc_;
                 nnp_theta3(lam, zc, ta, tb, tc, pa_, pb_, pc_);
                 atomicAdd(&Gout[row * NNP_NP + a], ex * pa_ * tpz);
@@ -310,54 +315,74 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout,
     __syncthreads();
 }
 
+typedef float nnp_f32x16 __attribute__((ext_vector_type(16)));
+
 // Per-atom network energies of the features Gin into out[N]
 // (n_features -> tanh h1 -> tanh h2 -> linear 1, ForwardPass.jl:23-39).
-// Lane = atom; each wave takes every fourth hidden unit, so the weights it
-// reads are the same for all its lanes.  All threads call.
+// The two hidden layers are 64 x 32 products on the matrix cores in binary32
+// (v_mfma_f32_32x32x2_f32; waves 0 and 1 take atoms 0-31 and 32-63, two
+// accumulators over alternating k pairs); lane layout: A[row l%32][k l/32],
+// B[k l/32][col l%32], output value v of lane l at row 8(v/4) + 4(l/32) + v%4,
+// column l%32.  The output layer is one dot product per atom.  All threads call.
 __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gin, int N, float *out)
 {
-    const int tid = threadIdx.x, a = tid & 63;
+    const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    constexpr int NW = NNP_NT / 64;
-    constexpr int HPW = (NNP_MAXH + NW - 1) / NW;
+    const int lr = l & 31, lh = l >> 5;
     const int nf = T.nf, h1 = T.h1, h2 = T.h2;
-    float z[HPW];
+    if (w < 2) {
+        const int r0 = 32 * w;
+        const int c1 = lr < h1 ? lr : 0;
+        const float m1 = lr < h1 ? 1.0f : 0.0f;
+        nnp_f32x16 za = {}, zb = {};
+        for (int kk = 0; kk < nf; kk += 4) {
+            const int ka = kk + lh, kb = kk + 2 + lh;
+            const int kac = ka < nf ? ka : nf - 1, kbc = kb < nf ? kb : nf - 1;
+            const float ma = ka < nf ? 1.0f : 0.0f, mb = kb < nf ? 1.0f : 0.0f;
+            const float xa = Gin[kac * NNP_NP + r0 + lr] * ma, xb = Gin[kbc * NNP_NP + r0 + lr] * mb;
+            const float wa = L.W1[kac * h1 + c1] * m1, wb = L.W1[kbc * h1 + c1] * m1;
+            za = __builtin_amdgcn_mfma_f32_32x32x2f32(xa, wa, za, 0, 0, 0);
+            zb = __builtin_amdgcn_mfma_f32_32x32x2f32(xb, wb, zb, 0, 0, 0);
+        }
+        if (lr < h1) {
+            const float bias = L.B[lr];
 #pragma unroll
-    for (int s = 0; s < HPW; ++s) z[s] = 0.0f;
-    for (int q = 0; q < nf; ++q) {
-        const float x = Gin[q * NNP_NP + a];
-#pragma unroll
-        for (int s = 0; s < HPW; ++s) {
-            const int h = w + NW * s;
-            if (h < h1) z[s] += x * L.W1[q * h1 + h];
+            for (int v = 0; v < 16; ++v) {
+                const int row = r0 + 8 * (v >> 2) + 4 * lh + (v & 3);
+                L.H1[row * NNP_HS + lr] = tanhf(za[v] + zb[v] + bias);
+            }
         }
     }
-#pragma unroll
-    for (int s = 0; s < HPW; ++s) {
-        const int h = w + NW * s;
-        if (h < h1 && a < N) L.H1[a * NNP_HS + h] = tanhf(z[s] + T.b1[h]);
-    }
     __syncthreads();
+    if (w < 2) {
+        const int r0 = 32 * w;
+        const int c2 = lr < h2 ? lr : 0;
+        const float m2 = lr < h2 ? 1.0f : 0.0f;
+        nnp_f32x16 za = {}, zb = {};
+        for (int kk = 0; kk < h1; kk += 4) {
+            const int ka = kk + lh, kb = kk + 2 + lh;
+            const int kac = ka < h1 ? ka : h1 - 1, kbc = kb < h1 ? kb : h1 - 1;
+            const float ma = ka < h1 ? 1.0f : 0.0f, mb = kb < h1 ? 1.0f : 0.0f;
+            const float xa = L.H1[(r0 + lr) * NNP_HS + kac] * ma, xb = L.H1[(r0 + lr) * NNP_HS + kbc] * mb;
+            const float wa = L.W2[kac * h2 + c2] * m2, wb = L.W2[kbc * h2 + c2] * m2;
+            za = __builtin_amdgcn_mfma_f32_32x32x2f32(xa, wa, za, 0, 0, 0);
+            zb = __builtin_amdgcn_mfma_f32_32x32x2f32(xb, wb, zb, 0, 0, 0);
+        }
+        if (lr < h2) {
+            // second hidden layer times the output weights
+            const float bias = L.B[NNP_MAXH + lr], w3 = L.B[2 * NNP_MAXH + lr];
 #pragma unroll
-    for (int s = 0; s < HPW; ++s) z[s] = 0.0f;
-    for (int k = 0; k < h1; ++k) {
-        const float x = L.H1[a * NNP_HS + k];
-#pragma unroll
-        for (int s = 0; s < HPW; ++s) {
-            const int h = w + NW * s;
-            if (h < h2) z[s] += x * T.w2[k * h2 + h];
+            for (int v = 0; v < 16; ++v) {
+                const int row = r0 + 8 * (v >> 2) + 4 * lh + (v & 3);
+                L.H2[row * NNP_HS + lr] = tanhf(za[v] + zb[v] + bias) * w3;
+            }
         }
     }
-#pragma unroll
-    for (int s = 0; s < HPW; ++s) {
-        const int h = w + NW * s;
-        if (h < h2 && a < N) L.H2[a * NNP_HS + h] = tanhf(z[s] + T.b2[h]);
-    }
     __syncthreads();
-    if (w == 0 && a < N) {
+    if (w == 0 && l < N) {
         float y = 0.0f;
-        for (int k = 0; k < h2; ++k) y += L.H2[a * NNP_HS + k] * T.w3[k];
-        out[a] = y + T.b3;
+        for (int k = 0; k < h2; ++k) y += L.H2[l * NNP_HS + k];
+        out[l] = y + T.b3;
     }
     __syncthreads();
 }
@@ -369,25 +394,21 @@ __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, in
     const int tid = threadIdx.x;
     for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) L.X[e] = gX[e];
     for (int e = tid; e < T.nf * T.h1; e += NNP_NT) L.W1[e] = T.w1[e];
-    const int nfun = T.n_groups > 0 ? T.grp[2 * (T.n_groups - 1)] + T.grp[2 * (T.n_groups - 1) + 1] : 0;
     for (int e = tid; e < 4 * T.n_rad_active; e += NNP_NT) L.rad[e] = T.rad[e];
-    for (int e = tid; e < 4 * nfun; e += NNP_NT) L.afn[e] = T.afn[e];
-    for (int e = tid; e < nfun; e += NNP_NT) L.abuild[e] = T.abuild[e];
     for (int e = tid; e < NNP_GD * T.n_groups; e += NNP_NT) L.gd[e] = T.gdesc[e];
     for (int e = tid; e < (NNP_NT / 64) * (NNP_WG + 1); e += NNP_NT) L.wg[e] = T.wgroups[e];
-    for (int e = tid; e < T.n_groups; e += NNP_NT) {
-        L.geta[e] = T.grp_eta[e];
-        L.grp[2 * e] = T.grp[2 * e];
-        L.grp[2 * e + 1] = T.grp[2 * e + 1];
+    for (int e = tid; e < T.h1 * T.h2; e += NNP_NT) L.W2[e] = T.w2[e];
+    for (int e = tid; e < NNP_MAXH; e += NNP_NT) {
+        L.B[e] = e < T.h1 ? T.b1[e] : 0.0f;
+        L.B[NNP_MAXH + e] = e < T.h2 ? T.b2[e] : 0.0f;
+        L.B[2 * NNP_MAXH + e] = e < T.h2 ? T.w3[e] : 0.0f;
     }
 }
 
 #define NNP_BIND_TABLES(L) \
     L.rad = s_rad;         \
-    L.afn = s_afn;         \
-    L.geta = s_geta;       \
-    L.grp = s_grp;         \
-    L.abuild = s_abuild;   \
+    L.W2 = s_w2;           \
+    L.B = s_bias;          \
     L.gd = s_gdesc;        \
     L.wg = s_wgrp
 
@@ -640,11 +661,46 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 if (lane == 0) s_mask[a] = m;
             }
             __syncthreads();
-            // lane -> (target ta, part): P lanes share a target when it fits
-            const int lp = n_near <= 16 ? 2 : (n_near <= 32 ? 1 : 0);  // log2 P
-            const int P = 1 << lp;
-            const int ta = lane >> lp, part = lane & (P - 1);
-            const bool tact = ta < n_near;
+            // lanes per target: target a gets P_a = 1, 2, 4 or 8 lanes, the
+            // fewest that bring its share of visits down to the load T, T the
+            // smallest load for which the shares fit 64 lanes (bisection, the
+            // lane count summed by ballots).  Blocks of P lanes are laid out
+            // by P (8, 4, 2, 1), so each is aligned for the quad and half-row
+            // exchanges that combine them.  Every wave builds the same table
+            // in its own LDS slice (no barrier needed within a wave).
+            const int La = lane < n_near ? __popcll(s_mask[lane]) : 0;
+            auto lanes_for = [&](int load) {
+                if (lane >= n_near) return 0;
+                const int q = (La + load - 1) / load;
+                return q <= 1 ? 1 : (q <= 2 ? 2 : (q <= 4 ? 4 : 8));
+            };
+            auto lanes_used = [&](int pa) {
+                int t = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) t += __popcll(__ballot(pa == (1 << c))) << c;
+                return t;
+            };
+            int lo = 1, hi = 64;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (lanes_used(lanes_for(mid)) <= 64) hi = mid;
+                else lo = mid + 1;
+            }
+            const int Pa = lanes_for(lo);
+            uint16_t *s_lt = reinterpret_cast<uint16_t *>(L.H1 + 2 * NNP_NP) + NNP_NP * (tid >> 6);
+            int used = 0;
+#pragma unroll
+            for (int c = 3; c >= 0; --c) {
+                const bool in = Pa == (1 << c);
+                const uint64_t b = __ballot(in);
+                if (in)
+                    for (int q = 0; q < Pa; ++q) s_lt[used + (mbcnt64(b) << c) + q] = (uint16_t)(lane | (q << 6) | (c << 9));
+                used += __popcll(b) << c;
+            }
+            const int ent = lane < used ? (int)s_lt[lane] : 0;
+            const int lp = ent >> 9, P = 1 << lp;
+            const int ta = ent & 63, part = (ent >> 6) & 7;
+            const bool tact = lane < used;
             const int j = tact ? (int)s_near[ta] : 0;
             const uint64_t mask = tact ? s_mask[ta] : 0ull;
             const uint64_t after = ta < 63 ? ~((2ull << ta) - 1ull) : 0ull;  // partners of larger index
@@ -692,8 +748,8 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                     nnp_visit(G0, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj0, ai0);
                     if (gb >= 0) nnp_visit(G1, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj1, ai1);
                 }
-                nnp_store(G0, P, tact && part == 0, j, i, lane, Dp, aj0, ai0);
-                if (gb >= 0) nnp_store(G1, P, tact && part == 0, j, i, lane, Dp, aj1, ai1);
+                nnp_store(G0, lp, tact && part == 0, j, i, lane, Dp, aj0, ai0);
+                if (gb >= 0) nnp_store(G1, lp, tact && part == 0, j, i, lane, Dp, aj1, ai1);
             }
             __syncthreads();
             for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] += Gp[e];
