@@ -52,10 +52,31 @@ __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
     return l;
 }
 
-// type-2 cutoff, MachineLearningPotential/Cutoff.jl:79-90
+// Taylor coefficient (-1)^k (pi/2)^(2k) / (2k)! of cos(pi t / 2) in u = t^2
+constexpr double nnp_cos_coef(int k)
+{
+    double c = 1.0;
+    for (int q = 1; q <= 2 * k; ++q) c *= 1.5707963267948966 / q;
+    return (k & 1) ? -c : c;
+}
+
+// type-2 cutoff, MachineLearningPotential/Cutoff.jl:79-90:
+// 0.5 (cos(pi r / rc) + 1) = cos^2(pi t / 2), t = r / rc < 1, the cosine by its
+// Taylor series in u = t^2 through u^7 (remainder below 1e-8 on [0, 1])
 __device__ __forceinline__ float nnp_cutoff(float r, float rc)
 {
-    return r < rc ? 0.5f * (cosf(3.14159265358979323846f * (r / rc)) + 1.0f) : 0.0f;
+    if (!(r < rc)) return 0.0f;
+    const float t = r * (1.0f / rc);
+    const float u = t * t;
+    float c = (float)nnp_cos_coef(7);
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(6));
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(5));
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(4));
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(3));
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(2));
+    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(1));
+    c = __builtin_fmaf(c, u, 1.0f);
+    return c * c;
 }
 
 // (1 + lambda cos)^zeta of three cosines for zeta = 2^k: k squarings (k is
@@ -513,6 +534,9 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     // current features and the scratch that becomes the trial features; an
     // accepted move swaps the two (swap_vars!(NNPVariables), src/swap_config.jl:213-221)
     float *Gp = L.G, *Dp = L.D;
+    float4 *nb_a = reinterpret_cast<float4 *>((reinterpret_cast<uintptr_t>(L.H2) + 15) & ~(uintptr_t)15);
+    float4 *nb_b = nb_a + NNP_NP;
+    float *nb_fo = reinterpret_cast<float *>(nb_b + NNP_NP);
 
     for (uint32_t m = 0; m < nsteps; ++m) {
         // draws and trial position (every thread the same), src/MCMoves.jl:115-135
@@ -581,7 +605,15 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 near = j != i && (fnj != 0.0f || foj != 0.0f);
             }
             const uint64_t b = __ballot(near);
-            if (near) s_near[mbcnt64(b)] = (uint8_t)j;
+            if (near) {
+                // packed per near atom for the angular visits (in the H2 scratch):
+                // {x, y, z, f_new}, {r2_new, r2_old, r_new, r_old}, f_old
+                const int q = mbcnt64(b);
+                s_near[q] = (uint8_t)j;
+                nb_a[q] = make_float4(L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j], L.fn[j]);
+                nb_b[q] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
+                nb_fo[q] = L.fo[j];
+            }
             if (lane == 0) s_nnear = __popcll(b);
         }
         for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] = 0.0f;
@@ -623,6 +655,16 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
         double new_en = 100.0;  // the hard-core sentinel, src/EnergyEvaluation.jl:1604-1606
         if (da && !stage1) new_en = __builtin_nan("");
         if (stage1 && !s_flag) {
+            if constexpr (DA) {
+                // the surrogate's layer scratch covered the packed near-atom data
+                if (tid < s_nnear) {
+                    const int j = s_near[tid];
+                    nb_a[tid] = make_float4(L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j], L.fn[j]);
+                    nb_b[tid] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
+                    nb_fo[tid] = L.fo[j];
+                }
+                __syncthreads();
+            }
             // radial changes, DeltaMatrix.jl:794-822: one wave per function
             for (int q = tid >> 6; q < T.n_rad_active; q += NNP_NT / 64) {
                 const float eta = L.rad[4 * q], norm = L.rad[4 * q + 1];
@@ -647,13 +689,13 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             uint64_t *s_mask = reinterpret_cast<uint64_t *>(L.H1);  // scratch until the network pass
             const float rc2 = T.r_cut * T.r_cut * 1.0001f;  // loose: the cut-off function decides below
             for (int a = tid >> 6; a < n_near; a += NNP_NT / 64) {
-                const int j = s_near[a];
+                const float4 pa = nb_a[a];
+                const float foa = nb_fo[a];
                 bool act = false;
                 if (lane < n_near && lane != a) {
-                    const int k = s_near[lane];
-                    if (L.fn[j] * L.fn[k] != 0.0f || L.fo[j] * L.fo[k] != 0.0f) {
-                        const float dx = L.X[j] - L.X[k], dy = L.X[NNP_NP + j] - L.X[NNP_NP + k],
-                                    dz = L.X[2 * NNP_NP + j] - L.X[2 * NNP_NP + k];
+                    const float4 pb = nb_a[lane];
+                    if (pa.w * pb.w != 0.0f || foa * nb_fo[lane] != 0.0f) {
+                        const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
                         act = dx * dx + dy * dy + dz * dz < rc2;
                     }
                 }
@@ -704,8 +746,10 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             const int j = tact ? (int)s_near[ta] : 0;
             const uint64_t mask = tact ? s_mask[ta] : 0ull;
             const uint64_t after = ta < 63 ? ~((2ull << ta) - 1ull) : 0ull;  // partners of larger index
-            const float pj3[3] = {L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j]};
-            const float dijn = sqrtf(L.rn2[j]), dijo = sqrtf(L.ro2[j]);
+            const float4 ja = nb_a[ta], jb = nb_b[ta];
+            const float jfo = nb_fo[ta];
+            const float pj3[3] = {ja.x, ja.y, ja.z};
+            const float dijn = jb.z, dijo = jb.w;
             const int w = tid >> 6;
             const int ngw = L.wg[w * (NNP_WG + 1)];
             for (int gi = 0; gi < ngw; gi += 2) {
@@ -724,19 +768,20 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 while (m != 0ull) {
                     const int bk = __builtin_ctzll(m);
                     for (int t = 0; t < P; ++t) m &= m - 1ull;  // next partner of this lane
-                    const int k = s_near[bk];
                     const bool own = (after >> bk) & 1ull;
-                    const float pk3[3] = {L.X[k], L.X[NNP_NP + k], L.X[2 * NNP_NP + k]};
+                    const float4 ka = nb_a[bk], kb = nb_b[bk];
+                    const float kfo = nb_fo[bk];
+                    const float pk3[3] = {ka.x, ka.y, ka.z};
                     const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
                     const float rjk = dx * dx + dy * dy + dz * dz;
                     const float djk = sqrtf(rjk);
                     const float fjk = nnp_cutoff(djk, T.r_cut);
-                    const float fpn = L.fn[j] * L.fn[k] * fjk, fpo = L.fo[j] * L.fo[k] * fjk;
+                    const float fpn = ja.w * ka.w * fjk, fpo = jfo * kfo * fjk;
                     if (fpn == 0.0f && fpo == 0.0f) continue;
-                    const float rsn = L.rn2[j] + L.rn2[k] + rjk, rso = L.ro2[j] + L.ro2[k] + rjk;
+                    const float rsn = jb.x + kb.x + rjk, rso = jb.y + kb.y + rjk;
                     // cosines at j (and at i), new and old
                     float cjn = 0.0f, cjo = 0.0f, cin = 0.0f, cio = 0.0f;
-                    const float dikn = sqrtf(L.rn2[k]), diko = sqrtf(L.ro2[k]);
+                    const float dikn = kb.z, diko = kb.w;
                     if (fpn != 0.0f) {
                         cjn = nnp_cos(pj3, pt, pk3, dijn, djk);
                         if (own) cin = nnp_cos(pt, pj3, pk3, dijn, dikn);
