@@ -271,7 +271,7 @@ def _nnp_engine(pt, pot, x0, n_ladders, steps):
     rad = 14 * A2B
     mc = pt.MCParams(steps, 40, 55, n_adjust=10, mc_sample=1, n_bin=100)
     eng = pt.PTEngine(pot, pt.NVT(55), pt.SphericalBC(rad), mc,
-                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=100))
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=25))
     for r in range(n_ladders):
         eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
     eng.prepare()
@@ -281,7 +281,8 @@ def _nnp_engine(pt, pot, x0, n_ladders, steps):
 def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
     """Cu55 with the Behler-Parrinello network (8 radial + 80 angular
     functions, 88-20-20-1 tanh, N(0, 0.1) weights, key 4242), 40 temperatures
-    200-1400 K, 14 A container, binary32 features and layers, FP64 totals."""
+    200-1400 K, 14 A container, binary32 features and layers, FP64 totals,
+    full FP64 rebuild every 25 cycles (the drift it finds is reported)."""
     pot, x0 = _cu55()
     eng, temp, rad = _nnp_engine(pt, pot, x0, n_ladders, steps)
     dt = _production(eng, n_ladders, warmup, steps)
@@ -360,7 +361,9 @@ def other_configs(pt, args):
             res.append(bench_bulk(pt, n, ens, 200, 1000))
     res.append(bench_nnp(pt, 1, 10, 40, cpu))
     res.append(bench_nnp(pt, 64, 20, 10, 0.0))
-    res.append(bench_delayed_acceptance(pt, 256, 20, 4, 8, 30))
+    # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
+    # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
+    res.append(bench_delayed_acceptance(pt, 256, 100, 4, 8, 30))
     return res
 
 

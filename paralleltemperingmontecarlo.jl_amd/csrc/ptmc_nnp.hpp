@@ -52,32 +52,8 @@ __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
     return l;
 }
 
-// Taylor coefficient (-1)^k (pi/2)^(2k) / (2k)! of cos(pi t / 2) in u = t^2
-constexpr double nnp_cos_coef(int k)
-{
-    double c = 1.0;
-    for (int q = 1; q <= 2 * k; ++q) c *= 1.5707963267948966 / q;
-    return (k & 1) ? -c : c;
-}
-
-// type-2 cutoff, MachineLearningPotential/Cutoff.jl:79-90:
-// 0.5 (cos(pi r / rc) + 1) = cos^2(pi t / 2), t = r / rc < 1, the cosine by its
-// Taylor series in u = t^2 through u^7 (remainder below 1e-8 on [0, 1])
-__device__ __forceinline__ float nnp_cutoff(float r, float rc)
-{
-    if (!(r < rc)) return 0.0f;
-    const float t = r * (1.0f / rc);
-    const float u = t * t;
-    float c = (float)nnp_cos_coef(7);
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(6));
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(5));
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(4));
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(3));
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(2));
-    c = __builtin_fmaf(c, u, (float)nnp_cos_coef(1));
-    c = __builtin_fmaf(c, u, 1.0f);
-    return c * c;
-}
+// type-2 cutoff, MachineLearningPotential/Cutoff.jl:79-90
+__device__ __forceinline__ float nnp_cutoff(float r, float rc) { return cutoff_cos2(r, rc); }
 
 // (1 + lambda cos)^zeta of three cosines for zeta = 2^k: k squarings (k is
 // uniform across the wave, so the loop is a scalar branch).
@@ -353,7 +329,8 @@ __device__ __forceinline__ float nnp_tanh(float x)
 // accumulators over alternating k pairs); lane layout: A[row l%32][k l/32],
 // B[k l/32][col l%32], output value v of lane l at row 8(v/4) + 4(l/32) + v%4,
 // column l%32.  The output layer is one dot product per atom.  All threads call.
-__device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gin, int N, float *out)
+__device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gin, int N, float *out,
+                            double *total = nullptr)
 {
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -408,10 +385,20 @@ __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gi
         }
     }
     __syncthreads();
-    if (w == 0 && l < N) {
+    if (w == 0) {
         float y = 0.0f;
-        for (int k = 0; k < h2; ++k) y += L.H2[l * NNP_HS + k];
-        out[l] = y + T.b3;
+        if (l < N) {
+            for (int k = 0; k < h2; ++k) y += L.H2[l * NNP_HS + k];
+            y += T.b3;
+            out[l] = y;
+        }
+        // FP64 total of the atomic energies (a butterfly over the wave)
+        if (total != nullptr) {
+            double t = l < N ? (double)y : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (l == 0) *total = t;
+        }
     }
     __syncthreads();
 }
@@ -456,15 +443,12 @@ __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
     nnp_load(a, L, slot);
     __syncthreads();
     nnp_full_build(T, L, L.G, N);
-    nnp_forward(T, L, L.G, N, L.eat);
+    __shared__ double s_tot;
+    nnp_forward(T, L, L.G, N, L.eat, &s_tot);
     float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
     for (int e = threadIdx.x; e < T.nf * NNP_NP; e += NNP_NT) gG[e] = L.G[e];
     for (int e = threadIdx.x; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = e < N ? L.eat[e] : 0.0f;
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int e = 0; e < N; ++e) s += (double)L.eat[e];
-        a.slot_en[slot] = s;
-    }
+    if (threadIdx.x == 0) a.slot_en[slot] = s_tot;
 }
 
 // One Monte-Carlo cycle (N moves) of one configuration per workgroup
@@ -507,7 +491,7 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     __shared__ float s_acci[SUR_K];
     __shared__ double s_ehat;
     constexpr bool da = DA;
-    float *Sd = nullptr, *Sn = nullptr, *sat = nullptr, *satn = nullptr;
+    float *Sd = nullptr, *Sn = nullptr, *sat = nullptr, *satn = nullptr, *sgeo = nullptr;
     SurNet net{};
     const SurGeom sg = sur_geom(a.sur_rcut);
     double Ehat = 0.0;
@@ -524,17 +508,12 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
         net.b1 = satn + NNP_NP;
         net.b2 = net.b1 + SUR_H;
         net.w3 = net.b2 + SUR_H;
+        sgeo = net.w3 + SUR_H;  // per atom: r_new, r_old, f_c(r_new), f_c(r_old)
         sur_load_net(net, a.sur_w, NNP_NT);
         sur_descriptors(sg, L.X, N, Sd, SUR_SS, NNP_NT);
         for (int e = tid; e < NNP_NP * SUR_SS; e += NNP_NT) Sn[e] = 0.0f;
         __syncthreads();
-        sur_atom_energies(net, Sd, N, L.H1, sat);
-        if (tid == 0) {
-            double h = 0.0;
-            for (int q = 0; q < N; ++q) h += (double)sat[q];
-            s_ehat = h;
-        }
-        __syncthreads();
+        sur_atom_energies(net, Sd, N, L.H1, sat, &s_ehat);
         Ehat = s_ehat;
     }
     __syncthreads();
@@ -631,29 +610,31 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
         bool stage1 = true;
         double dEh = 0.0, Ehn = 0.0;
         if (da) {
+            // distances and cutoffs of every atom to the two positions, once
+            if (tid < N) {
+                const float rn = sqrtf(L.rn2[tid]), ro = sqrtf(L.ro2[tid]);
+                sgeo[tid] = rn;
+                sgeo[NNP_NP + tid] = ro;
+                sgeo[2 * NNP_NP + tid] = tid == i ? 0.0f : sur_cutoff(rn, sg.r_cut);
+                sgeo[3 * NNP_NP + tid] = tid == i ? 0.0f : sur_cutoff(ro, sg.r_cut);
+            }
+            __syncthreads();
             float accq = 0.0f;
             const int kq = tid & (SUR_K - 1);  // NNP_NT is a multiple of SUR_K: fixed k per thread
-            for (int e = tid; e < N * SUR_K; e += NNP_NT) {
-                const int j = e / SUR_K;
+            const float mu = sur_mu(sg, kq);
+            for (int j = tid / SUR_K; j < N; j += NNP_NT / SUR_K) {
                 if (j == i) continue;
-                const float rn = sqrtf(L.rn2[j]), ro = sqrtf(L.ro2[j]);
-                const float fcn = sur_cutoff(rn, sg.r_cut), fco = sur_cutoff(ro, sg.r_cut);
-                const float pn = fcn != 0.0f ? sur_phi(sg, rn, fcn, kq) : 0.0f;
-                const float pq = fco != 0.0f ? sur_phi(sg, ro, fco, kq) : 0.0f;
-                Sn[j * SUR_SS + kq] = Sd[j * SUR_SS + kq] + SUR_SIN * (pn - pq);
+                const float fcn = sgeo[2 * NNP_NP + j], fco = sgeo[3 * NNP_NP + j];
+                const float pn = sur_phi_mu(sg, sgeo[j], fcn, mu);
+                const float pq = sur_phi_mu(sg, sgeo[NNP_NP + j], fco, mu);
+                Sn[j * SUR_SS + kq] = __builtin_fmaf(SUR_SIN, pn - pq, Sd[j * SUR_SS + kq]);
                 accq += pn;
             }
             atomicAdd(&s_acci[kq], accq);
             __syncthreads();
             if (tid < SUR_K) Sn[i * SUR_SS + tid] = SUR_SIN * s_acci[tid];
             __syncthreads();
-            sur_atom_energies(net, Sn, N, L.H1, satn);
-            if (tid == 0) {
-                double h = 0.0;
-                for (int q = 0; q < N; ++q) h += (double)satn[q];
-                s_ehat = h;
-            }
-            __syncthreads();
+            sur_atom_energies(net, Sn, N, L.H1, satn, &s_ehat);
             Ehn = s_ehat;
             dEh = Ehn - Ehat;
             stage1 = (dEh <= 0.0) || (det_exp(-dEh * beta) >= (double)d0.umet);
@@ -807,13 +788,7 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             __syncthreads();
             for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] += Gp[e];
             __syncthreads();
-            nnp_forward(T, L, Dp, N, L.eatn);
-            if (tid == 0) {
-                double s = 0.0;
-                for (int e = 0; e < N; ++e) s += (double)L.eatn[e];
-                s_en = s;
-            }
-            __syncthreads();
+            nnp_forward(T, L, Dp, N, L.eatn, &s_en);
             new_en = s_en;
         }
         // Metropolis, src/MCRun.jl:122-132 and src/Exchange.jl:53-57; with
@@ -865,13 +840,7 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     // periodic full rebuild bounds the drift of the incremental features
     if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
         nnp_full_build(T, L, Gp, N);
-        nnp_forward(T, L, Gp, N, L.eat);
-        if (tid == 0) {
-            double s = 0.0;
-            for (int e = 0; e < N; ++e) s += (double)L.eat[e];
-            s_en = s;
-        }
-        __syncthreads();
+        nnp_forward(T, L, Gp, N, L.eat, &s_en);
         const double dr = __builtin_fabs(E - s_en);
         if (tid == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
         E = s_en;

@@ -32,6 +32,34 @@ __host__ __device__ inline size_t nnp_lds_floats(int nf, int h1)
 
 namespace ptmc {
 
+// Taylor coefficient (-1)^k (pi/2)^(2k) / (2k)! of cos(pi t / 2) in u = t^2
+constexpr double cutoff_cos_coef(int k)
+{
+    double c = 1.0;
+    for (int q = 1; q <= 2 * k; ++q) c *= 1.5707963267948966 / q;
+    return (k & 1) ? -c : c;
+}
+
+// Cosine cutoff 0.5 (cos(pi r / rc) + 1) for r < rc, else 0
+// (MachineLearningPotential/Cutoff.jl:79-90): cos^2(pi t / 2), t = r / rc,
+// the cosine by its Taylor series in u = t^2 through u^7 (remainder below
+// 1e-8 on [0, 1])
+__device__ __forceinline__ float cutoff_cos2(float r, float rc)
+{
+    if (!(r < rc)) return 0.0f;
+    const float t = r * (1.0f / rc);
+    const float u = t * t;
+    float c = (float)cutoff_cos_coef(7);
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(6));
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(5));
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(4));
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(3));
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(2));
+    c = __builtin_fmaf(c, u, (float)cutoff_cos_coef(1));
+    c = __builtin_fmaf(c, u, 1.0f);
+    return c * c;
+}
+
 // Learned surrogate (delayed acceptance): 64 Gaussian radial descriptors per
 // atom -> 64 -> 64 (SiLU) -> 1.
 constexpr int SUR_K = 64;
@@ -51,7 +79,7 @@ constexpr int SUR_YS = SUR_H + 1;  // output products, f32
 __host__ __device__ inline size_t sur_lds_floats()
 {
     return (size_t)2 * NNP_NP * SUR_SS + (size_t)SUR_K * SUR_H + (size_t)NNP_NP * SUR_HS / 2 + 2 * NNP_NP +
-           3 * SUR_H;
+           3 * SUR_H + 4 * NNP_NP;
 }
 
 }  // namespace ptmc

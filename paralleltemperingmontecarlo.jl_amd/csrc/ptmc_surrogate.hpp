@@ -43,7 +43,7 @@ __device__ __forceinline__ short sur_bf16(float f)
 }
 
 struct SurGeom {
-    float mu0, dmu, inv_sig, r_cut;
+    float mu0, dmu, c2, r_cut;  // c2 = log2(e) / sigma^2
 };
 
 __device__ __forceinline__ SurGeom sur_geom(float r_cut)
@@ -51,22 +51,27 @@ __device__ __forceinline__ SurGeom sur_geom(float r_cut)
     SurGeom g;
     g.mu0 = SUR_RMIN;
     g.dmu = (r_cut - SUR_RMIN) / (float)(SUR_K - 1);
-    g.inv_sig = 1.0f / (1.5f * g.dmu);
+    const float inv_sig = 1.0f / (1.5f * g.dmu);
+    g.c2 = 1.4426950408889634f * inv_sig * inv_sig;
     g.r_cut = r_cut;
     return g;
 }
 
-// one descriptor term exp(-((r - mu_k)/sigma)^2) f_c(r)
-__device__ __forceinline__ float sur_phi(const SurGeom &g, float r, float fc, int k)
+__device__ __forceinline__ float sur_mu(const SurGeom &g, int k) { return g.mu0 + g.dmu * (float)k; }
+
+// one descriptor term exp(-((r - mu_k)/sigma)^2) f_c(r) = 2^{-c2 (r - mu_k)^2} f_c(r)
+__device__ __forceinline__ float sur_phi_mu(const SurGeom &g, float r, float fc, float mu)
 {
-    const float t = (r - (g.mu0 + g.dmu * (float)k)) * g.inv_sig;
-    return expf(-t * t) * fc;
+    const float t = r - mu;
+    return __builtin_amdgcn_exp2f(-g.c2 * t * t) * fc;
 }
 
-__device__ __forceinline__ float sur_cutoff(float r, float rc)
+__device__ __forceinline__ float sur_phi(const SurGeom &g, float r, float fc, int k)
 {
-    return r < rc ? 0.5f * (cosf(3.14159265358979323846f * (r / rc)) + 1.0f) : 0.0f;
+    return sur_phi_mu(g, r, fc, sur_mu(g, k));
 }
+
+__device__ __forceinline__ float sur_cutoff(float r, float rc) { return cutoff_cos2(r, rc); }
 
 // Full descriptors of N atoms (positions X[3][NNP_NP]) into S[NNP_NP][stride];
 // pad rows are zero.  nthreads threads call.
@@ -113,9 +118,11 @@ __device__ void sur_load_net(SurNet &n, const float *w, int nthreads)
 }
 
 // Surrogate atom energies of descriptors S [NNP_NP][SUR_SS] (f32) into
-// out[a], a < N.  Waves 0-3 each own one 32x32 tile (atom half, hidden half);
-// scratch Y [NNP_NP][SUR_YS] f32.  All threads of the workgroup call.
-__device__ void sur_atom_energies(const SurNet &n, const float *S, int N, float *Y, float *out)
+// out[a], a < N, and optionally their FP64 total.  Waves 0-3 each own one
+// 32x32 tile (atom half, hidden half); scratch Y [NNP_NP][SUR_YS] f32.  All
+// threads of the workgroup call.
+__device__ void sur_atom_energies(const SurNet &n, const float *S, int N, float *Y, float *out,
+                                  double *total = nullptr)
 {
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
     const int r0 = (w & 1) * 32, c0 = ((w >> 1) & 1) * 32;
@@ -160,10 +167,19 @@ __device__ void sur_atom_energies(const SurNet &n, const float *S, int N, float 
         }
     }
     __syncthreads();
-    if (w == 0 && l < N) {
-        float e = 0.0f;
-        for (int h = 0; h < SUR_H; ++h) e += Y[l * SUR_YS + h];
-        out[l] = e + n.b3;
+    if (w == 0) {
+        // row sums with eight independent partial sums, then the FP64 total
+        float p[8] = {};
+#pragma unroll
+        for (int h = 0; h < SUR_H; ++h) p[h & 7] += Y[l * SUR_YS + h];
+        const float e = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])) + n.b3;
+        if (l < N) out[l] = e;
+        if (total != nullptr) {
+            double t = l < N ? (double)e : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (l == 0) *total = t;
+        }
     }
     __syncthreads();
 }
