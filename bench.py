@@ -375,7 +375,8 @@ def main():
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--lanes-table", type=str, default="",
-                    help='JSON {"N": lanes per configuration}; default: the tuned table, else the engine default')
+                    help='JSON {"N": lanes per configuration}; default: tuned on the workload before timing')
+    ap.add_argument("--no-tune", action="store_true", help="use the engine's default lanes per configuration")
     ap.add_argument("--cycles-per-launch", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -401,8 +402,23 @@ def main():
     mc = pt.MCParams(args.steps, 1, 1, n_adjust=100, mc_sample=1, n_bin=100)
     opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
                             lanes_per_replica=args.lanes, cycles_per_launch=args.cycles_per_launch)
-    eng = pt.PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
-    table = {int(k): int(v) for k, v in json.loads(args.lanes_table).items()} if args.lanes_table else {}
+    def make_engine():
+        return pt.PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
+
+    t_tune = 0.0
+    if args.lanes_table:
+        table = {int(k): int(v) for k, v in json.loads(args.lanes_table).items()}
+    elif args.no_tune or args.lanes:
+        table = {}
+    else:
+        # lanes per configuration measured on this workload (untimed)
+        from paralleltemperingmontecarlo.autotune import tune_lanes
+
+        t0 = time.perf_counter()
+        table = tune_lanes(make_engine, [dict(r, bc=pt.SphericalBC(r["radius"])) for r in runs],
+                           log=lambda m: print("tune:", m, file=sys.stderr, flush=True))
+        t_tune = time.perf_counter() - t0
+    eng = make_engine()
     for r in runs:
         eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], pt.SphericalBC(r["radius"]), run_key=r["key"],
                     lanes=table.get(r["n_atoms"], 0))
@@ -467,6 +483,8 @@ def main():
             "workgroups": plan["tiles"],
             "launch_groups": plan["launch_groups"],
             "max_fp32_vs_fp64_energy_drift": drift,
+            "lanes_per_configuration": {str(k): v for k, v in sorted(table.items())} or "engine default",
+            "tuning_s": t_tune,
             "mean_atom_acceptance": float(np.mean(acc)),
         },
         "roofline": {
