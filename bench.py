@@ -193,20 +193,30 @@ def _cpu_rate(make_run, moves_per_cycle, target_s):
 
 def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
     """Ne13 extended-LJ cluster in FP64: icosahedral start, 5.32 A container,
-    32 temperatures geometric 4-16 K, exchange every cycle."""
+    32 temperatures geometric 4-16 K, exchange every cycle; lanes per
+    configuration tuned on the workload."""
+    from paralleltemperingmontecarlo.autotune import tune_lanes
+
     d = json.loads(NE13_SETUP.read_text())
     pos = np.array(d["pos_angstrom"]) * d["AtoBohr"]
     coeff, rad = np.array(d["coeff"]), d["radius_angstrom"] * d["AtoBohr"]
     temp = pt.TempGrid(4.0, 16.0, 32)
     mc = pt.MCParams(steps, 32, 13, n_adjust=100, mc_sample=1, n_bin=100)
-    eng = pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(13), pt.SphericalBC(rad), mc,
-                      pt.EngineOptions(precision="fp64", exchange="even_odd", seed=1234, recompute_every=1000))
+
+    def make():
+        return pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(13), pt.SphericalBC(rad), mc,
+                           pt.EngineOptions(precision="fp64", exchange="even_odd", seed=1234, recompute_every=1000))
+
+    runs = [dict(n_atoms=13, beta=temp.beta_grid, x0=pos, bc=pt.SphericalBC(rad), key=r) for r in range(n_ladders)]
+    table = tune_lanes(make, runs, cycles=max(500, 20000 // n_ladders), adapt_cycles=500)
+    eng = make()
     for r in range(n_ladders):
-        eng.add_run(13, temp.beta_grid, pos[None], pt.SphericalBC(rad), run_key=r)
+        eng.add_run(13, temp.beta_grid, pos[None], pt.SphericalBC(rad), run_key=r, lanes=table.get(13, 0))
     eng.prepare()
     dt = _production(eng, n_ladders, warmup, steps)
     out = dict(workload=f"Ne13 ELJ FP64, {n_ladders} ladder(s) x 32 temperatures 4-16 K, sphere 5.32 A",
                moves_per_s=n_ladders * 32 * 13 * steps / dt, steps=steps, dtype="f64",
+               lanes_per_configuration=eng.lanes_of(0),
                acceptance=_acceptance(eng, n_ladders, 13, steps),
                max_drift=max(float(np.max(eng.drift(i))) for i in range(n_ladders)))
     eng.close()
@@ -271,7 +281,7 @@ def _nnp_engine(pt, pot, x0, n_ladders, steps):
     rad = 14 * A2B
     mc = pt.MCParams(steps, 40, 55, n_adjust=10, mc_sample=1, n_bin=100)
     eng = pt.PTEngine(pot, pt.NVT(55), pt.SphericalBC(rad), mc,
-                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=25))
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=50))
     for r in range(n_ladders):
         eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
     eng.prepare()
@@ -282,7 +292,8 @@ def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
     """Cu55 with the Behler-Parrinello network (8 radial + 80 angular
     functions, 88-20-20-1 tanh, N(0, 0.1) weights, key 4242), 40 temperatures
     200-1400 K, 14 A container, binary32 features and layers, FP64 totals,
-    full FP64 rebuild every 25 cycles (the drift it finds is reported)."""
+    full FP64 rebuild every 50 cycles (one falls in the timed 50 cycles; the
+    drift it finds is reported)."""
     pot, x0 = _cu55()
     eng, temp, rad = _nnp_engine(pt, pot, x0, n_ladders, steps)
     dt = _production(eng, n_ladders, warmup, steps)
@@ -359,8 +370,8 @@ def other_configs(pt, args):
     for n in (108, 216, 256):
         for ens in ("nvt", "npt"):
             res.append(bench_bulk(pt, n, ens, 200, 1000))
-    res.append(bench_nnp(pt, 1, 10, 40, cpu))
-    res.append(bench_nnp(pt, 64, 20, 10, 0.0))
+    res.append(bench_nnp(pt, 1, 50, 50, cpu))
+    res.append(bench_nnp(pt, 64, 50, 50, 0.0))
     # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
     # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
     res.append(bench_delayed_acceptance(pt, 256, 100, 4, 8, 30))

@@ -1,0 +1,62 @@
+"""Lanes-per-configuration autotuner: the starting table on the host, and a
+tuned table on the device that the engine accepts and that leaves the
+physics untouched (a run's results depend on its lanes only through the
+summation tree, so tuned and default engines agree to binary32 rounding)."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+
+def test_start_table():
+    from paralleltemperingmontecarlo.autotune import start_table
+
+    assert start_table([13, 38, 55, 75, 98]) == {13: 2, 38: 4, 55: 8, 75: 16, 98: 16}
+    assert start_table([2, 5]) == {2: 1, 5: 1}
+    assert start_table([64, 63]) == {64: 16, 63: 8}
+
+
+def _runs(seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for key, (n, R) in enumerate([(13, 16), (38, 24), (13, 8)]):
+        rad = 1.2 * n ** (1 / 3)
+        pts = []
+        while len(pts) < n:
+            p = rng.uniform(-rad, rad, 3)
+            if p @ p <= (0.95 * rad) ** 2 and all(np.sum((p - q) ** 2) >= 0.64 for q in pts):
+                pts.append(p)
+        beta = 1.0 / (0.05 * (0.5 / 0.05) ** (np.arange(R) / (R - 1)))
+        out.append(dict(n_atoms=n, beta=beta, x0=np.array(pts), radius=rad, key=key))
+    return out
+
+
+@pytest.mark.gpu
+def test_tuned_table_runs():
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    import paralleltemperingmontecarlo as pt
+    from paralleltemperingmontecarlo.autotune import LANE_CHOICES, tune_lanes
+
+    runs = _runs()
+    mc = pt.MCParams(100, 1, 1, n_adjust=20)
+    opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=5)
+
+    def make():
+        return pt.PTEngine(pt.ELJPotentialEven([-4.0, 0.0, 0.0, 4.0]), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
+
+    rr = [dict(r, bc=pt.SphericalBC(r["radius"])) for r in runs]
+    table = tune_lanes(make, rr, warmup=5, cycles=50, rounds=1, adapt_cycles=50)
+    assert set(table) <= {13, 38} and all(v in LANE_CHOICES for v in table.values())
+    energies = []
+    for tab in (table, {}):
+        eng = make()
+        for r in rr:
+            eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], r["bc"], run_key=r["key"],
+                        lanes=tab.get(r["n_atoms"], 0))
+        eng.prepare()
+        energies.append([eng.energies(i) for i in range(len(rr))])
+        eng.close()
+    # FP64 start energies do not depend on the lanes beyond the summation tree
+    for a, b in zip(*energies):
+        np.testing.assert_allclose(a, b, rtol=1e-12)
