@@ -364,17 +364,22 @@ def bench_delayed_acceptance(pt, n_ladders, warmup, steps, collect, epochs):
     return out
 
 
-def other_configs(pt, args):
+def other_configs(pt, args, only=None):
     cpu = 0.0 if args.no_cpu else args.cpu_seconds / 3
-    res = [bench_ne13(pt, 1, 500, 20000, cpu), bench_ne13(pt, 512, 500, 2000, 0.0)]
-    for n in (108, 216, 256):
-        for ens in ("nvt", "npt"):
-            res.append(bench_bulk(pt, n, ens, 200, 1000))
-    res.append(bench_nnp(pt, 1, 50, 50, cpu))
-    res.append(bench_nnp(pt, 64, 50, 50, 0.0))
-    # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
-    # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
-    res.append(bench_delayed_acceptance(pt, 256, 100, 4, 8, 30))
+    todo = {
+        "ne13": lambda: [bench_ne13(pt, 1, 500, 20000, cpu)],
+        "ne13_batched": lambda: [bench_ne13(pt, 512, 500, 2000, 0.0)],
+        "bulk": lambda: [bench_bulk(pt, n, ens, 200, 1000) for n in (108, 216, 256) for ens in ("nvt", "npt")],
+        "nnp": lambda: [bench_nnp(pt, 1, 50, 50, cpu)],
+        "nnp_batched": lambda: [bench_nnp(pt, 64, 50, 50, 0.0)],
+        # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
+        # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
+        "delayed_acceptance": lambda: [bench_delayed_acceptance(pt, 256, 100, 4, 8, 30)],
+    }
+    res = []
+    for name, fn in todo.items():
+        if only is None or name == only:
+            res += fn()
     return res
 
 
@@ -393,6 +398,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
+    ap.add_argument("--config", type=str, default="",
+                    help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, "
+                         "delayed_acceptance) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs shard "
@@ -409,6 +417,9 @@ def main():
         return
     import paralleltemperingmontecarlo as pt
 
+    if args.config:
+        print(json.dumps(other_configs(pt, args, args.config)))
+        return
     runs = batched_lj_workload(args.runs)
     mc = pt.MCParams(args.steps, 1, 1, n_adjust=100, mc_sample=1, n_bin=100)
     opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
@@ -426,7 +437,7 @@ def main():
         from paralleltemperingmontecarlo.autotune import tune_lanes
 
         t0 = time.perf_counter()
-        table = tune_lanes(make_engine, [dict(r, bc=pt.SphericalBC(r["radius"])) for r in runs],
+        table = tune_lanes(make_engine, [dict(r, bc=pt.SphericalBC(r["radius"])) for r in runs], cycles=3000,
                            log=lambda m: print("tune:", m, file=sys.stderr, flush=True))
         t_tune = time.perf_counter() - t0
     eng = make_engine()
