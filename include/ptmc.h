@@ -101,7 +101,8 @@ int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double
  * 2 (tanh), 1 (linear)] and RuNNer parameter order (NeuralNetworkPotential,
  * ForwardPass.jl:23-39).  Binary32 features and network, FP64 totals;
  * NVT in a spherical container, at most 64 atoms and 128 functions, zeta a
- * power of two up to 64. */
+ * power of two up to 16, lambda = +-1, at most 32 groups of functions with
+ * equal eta (one function per (lambda, zeta) in a group). */
 int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const double *rad, const double *ang,
                            double r_cut, double hardcore2, int32_t q10_compat, int32_t n_layers,
                            const int32_t *nodes, const int32_t *activations, int32_t n_params,
