@@ -543,21 +543,6 @@ int prepare_nnp(ptmc_engine *h)
     t.n_groups = n.n_groups;
     t.r_cut = (float)n.r_cut;
     t.hardcore2 = n.hardcore2;
-    {
-        // The angular update of a non-moved atom accumulates in 32-bit fixed
-        // point (integer LDS adds run at the rate of plain stores, float adds
-        // far slower).  One visit changes a feature by at most tpz 2^zeta
-        // (both (1 + lambda cos)^zeta <= 2^zeta and the other factors <= 1),
-        // an atom has at most NNP_NP - 2 partners: the scale keeps the sum
-        // below 2^30.
-        double bound = 1e-30;
-        for (size_t f = 0; f < n.afn.size() / 4; ++f)
-            bound = std::max(bound, (double)std::fabs(n.afn[4 * f + 2]) * std::ldexp(1.0, 1 << (int)n.afn[4 * f + 1]));
-        bound *= (double)(NNP_NP - 2);
-        const int shift = std::min(60, (int)std::floor(std::log2(std::ldexp(1.0, 30) / bound)));
-        t.ang_scale = (float)std::ldexp(1.0, shift);
-        t.ang_unit = (float)std::ldexp(1.0, -shift);
-    }
     t.rad = h->d_nnp_rad.p;
     t.grp = h->d_nnp_grp.p;
     t.grp_eta = h->d_nnp_geta.p;

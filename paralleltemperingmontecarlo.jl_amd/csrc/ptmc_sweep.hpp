@@ -370,6 +370,64 @@ __device__ double full_energy64(const Real *X, int gb, int q, int N, double box,
     return part;
 }
 
+__device__ __forceinline__ double readlane_d(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// FP64 energy of the cubic-box configuration held by the cluster whose first
+// lane is wave lane `cl`, at squared scale s2 and cutoff rc (the volume-move
+// energy of src/EnergyEvaluation.jl:397-420), computed by every executing
+// lane of the wave together.  The N(N-1)/2 pairs are enumerated as
+// (j, j + d mod N) for d = 1..N/2 (at d = N/2 of an even N only j < N/2), and
+// dealt round-robin over the `live` lanes; cluster butterflies, then the
+// cluster totals of the `leaders` (first lanes) in lane order.  Every lane
+// returns the same value.  Pairs and cutoff test are those of full_energy64;
+// only the summation order differs.
+template <int PK, int KC, int KL, int LANES, typename Real>
+__device__ double wave_cubic_energy64(const Real *X, int cl, int N, double box, double s2, double rc,
+                                      const double (&c64)[KC], uint64_t live, uint64_t leaders)
+{
+    constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
+    const double invL = 1.0 / box;
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
+    const int nl = __popcll(live);
+    const int h = N / 2;
+    const bool even = (N & 1) == 0;
+    const int cb = (threadIdx.x & ~63) + cl;  // block lane of the cluster's first atom column
+    int j = rank % N, d = rank / N + 1;
+    double part = 0.0;
+    while (d <= h) {
+        int l = j + d;
+        if (l >= N) l -= N;
+        if (!(even && d == h && j >= h)) {
+            const int sj = j / KL, lj = cb + (j % KL);
+            const int sl = l / KL, ll = cb + (l % KL);
+            const double xj = (double)X[xoff<LANES>(sj, 0, lj)];
+            const double yj = (double)X[xoff<LANES>(sj, 1, lj)];
+            const double zj = (double)X[xoff<LANES>(sj, 2, lj)];
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
+            const double r2 = dist2_64(xj, yj, zj, xl, yl, zl, BC_CUBIC, box, invL, DIV) * s2;
+            if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+        }
+        j += nl;
+        while (j >= N) {
+            j -= N;
+            ++d;
+        }
+    }
+    part = group_sum<KL>(part);
+    double tot = 0.0;
+    for (uint64_t b = leaders; b != 0; b &= b - 1) tot += readlane_d(part, __builtin_ctzll(b));
+    return tot + lrc64<KC>(N, rc, c64);
+}
+
 // Philox words of move mm for this lane (block A; block B only in FP64).
 template <typename Real>
 __device__ __forceinline__ void move_words(uint64_t seed, uint32_t run, uint32_t temp, uint32_t sweep, uint32_t mm,
@@ -591,20 +649,42 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                } else {
+                }
+                if constexpr (ENS == ENS_NPT) {
                     // uniform volume move, src/MCMoves.jl:169-177, :277-287; energy
                     // src/EnergyEvaluation.jl:397-420; acceptance src/Exchange.jl:58-71
-                    if constexpr (ENS == ENS_NPT) {
-                        const double delta_v = a.t_displ[2 * (tb + t) + 1];
-                        const double pressure = a.runs[a.slot_run[slot]].pressure;
-                        const double y = ((double)d0.ux - 0.5) * delta_v;
-                        double scale = det_exp(y / 3.0);
+                    const bool vm = i >= N;
+                    double scale = 1.0, nbox = box, nrc = 0.0, s2 = 1.0;
+                    if (vm) {
+                        const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * (tb + t) + 1];
+                        scale = det_exp(y / 3.0);
                         if (box >= a.t_maxbox[tb + t] && scale > 1.0) scale = 1.0;
-                        const double nbox = box * scale;
-                        const double nrc = nbox * nbox / 4.0;
-                        const double s2 = scale * scale;
-                        const double new_en =
-                            full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, s2, nrc, c64);
+                        nbox = box * scale;
+                        nrc = nbox * nbox / 4.0;
+                        s2 = scale * scale;
+                    }
+                    double new_en = 0.0;
+                    if constexpr (F32 && BC == BC_CUBIC) {
+                        // the clusters of a wave evaluate each pending volume move
+                        // together instead of leaving the wave to one cluster's
+                        // O(N^2) sum, once per cluster
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        const uint64_t live = __ballot(1);
+                        const uint64_t leaders = __ballot(q == 0);
+                        uint64_t todo = __ballot(vm && q == 0);
+                        while (todo != 0) {
+                            const int cl = __builtin_ctzll(todo);
+                            todo &= todo - 1;
+                            const double e = wave_cubic_energy64<PK, KC, KL, LANES>(
+                                X, cl, N, readlane_d(box, cl), readlane_d(s2, cl), readlane_d(nrc, cl), c64, live,
+                                leaders);
+                            if ((gb & 63) == cl) new_en = e;
+                        }
+                    } else {
+                        if (vm) new_en = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, s2, nrc, c64);
+                    }
+                    if (vm) {
+                        const double pressure = a.runs[a.slot_run[slot]].pressure;
                         const double dd = new_en - E;
                         dE_used = dd;
                         const double vn = nbox * nbox * nbox, vo = box * box * box;

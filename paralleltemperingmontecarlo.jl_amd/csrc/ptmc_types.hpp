@@ -63,7 +63,6 @@ struct NnpTables {
     int32_t n_groups;              // angular eta groups
     float r_cut;
     double hardcore2;              // squared hard-core distance (0: none)
-    float ang_scale, ang_unit;     // fixed-point scale of the angular move update and its inverse
     const float *rad;              // n_rad_active x 4: eta, norm, row, pad
     const int32_t *grp;            // n_groups x 2: first function, count
     const float *grp_eta;          // n_groups
