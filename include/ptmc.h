@@ -186,6 +186,23 @@ int ptmc_multihistogram_nvt(int32_t n_runs, int32_t n_temps, int32_t n_bin, cons
                             double k_boltzmann, double conv_threshold, int32_t max_iter, double *cv,
                             double *free_energy, int32_t *iterations, int32_t device);
 
+/* Multihistogram heat capacity at constant pressure, multihistogram_NPT
+ * (src/multihist_NPT.jl:126-340), FP64 on the device for n_runs runs at once.
+ * hist holds each run's n_temps (n_bin + 2) x (n_bin + 2) histograms, rows
+ * energy and columns volume (underflow, n_bin bins, overflow), row-major as
+ * ptmc_get_histograms returns them; en_min/en_max/v_min/v_max the histogram
+ * ranges and pressure the run's pressure (n_runs each).  The bins weigh
+ * exp(-beta (E + pV)); Cp = (<h^2> - <h>^2) / (k T^2) with h = E + pV.
+ * Outputs: cp (n_runs x n_result); moments (n_runs x n_result x 3: <E>, <V>,
+ * <h>, may be NULL); free energies (n_runs x n_temps, may be NULL);
+ * iterations (n_runs, may be NULL). */
+int ptmc_multihistogram_npt(int32_t n_runs, int32_t n_temps, int32_t n_bin, const double *beta,
+                            const double *hist, const double *en_min, const double *en_max,
+                            const double *v_min, const double *v_max, const double *pressure,
+                            int32_t n_result, const double *beta_result, const double *t_result,
+                            double k_boltzmann, double conv_threshold, int32_t max_iter, double *cp,
+                            double *moments, double *free_energy, int32_t *iterations, int32_t device);
+
 /* Learned surrogate and delayed-acceptance Metropolis for the network
  * potential.  No counterpart in the reference: these entry points serve the
  * engine's own acceleration (Christen & Fox 2005).  The surrogate is a

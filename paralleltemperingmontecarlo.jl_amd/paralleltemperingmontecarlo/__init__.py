@@ -34,6 +34,7 @@ from .api import (  # noqa: F401
     read_runner_weights,
 )
 from ._lib import PTMCError, load as load_library  # noqa: F401
-from .multihist import multihistogram_NVT, multihistogram_nvt_batch  # noqa: F401
+from .multihist import (  # noqa: F401
+    multihistogram_NPT, multihistogram_npt_batch, multihistogram_NVT, multihistogram_nvt_batch)
 
 AtoBohr = 1.8897259886

@@ -88,6 +88,55 @@ def multihist_ne13(ref: pathlib.Path):
     (HERE / "multihist_ne13.json").write_text(json.dumps(out))
 
 
+def multihist_npt32(ref: pathlib.Path):
+    """The NPT testset of test/multihist_test.jl:19-31: a 3-temperature run
+    of 32 atoms (20-40 K) with its (energy, volume) histograms and four
+    expected heat capacities.  The testset sits inside a comment block in the
+    reference and its expected values index the heat-capacity array; the
+    reference function returns (temperatures, heat capacities).  The
+    histograms are stored sparse: [temperature, energy row, volume column,
+    count] with rows and columns 0..n_bin+1 (underflow, bins, overflow)."""
+    s = (ref / "test" / "multihist_test.jl").read_text()
+    s = s[s.index('@testset "multihist_NPT"'):]
+    ens = re.search(r"NPT\((\d+),\s*([-0-9.eE]+)", s)
+    temp = re.search(r"TempGrid\{(\d+)\}\((\d+),\s*(\d+)\)", s)
+    i = s.index("Output{Float64}(")
+    line = s[i: s.index("\n", i)]
+    args = _split_top(line[len("Output{Float64}("):])
+    mats = re.findall(r"\[([^\[\]]*)\]", args[9])
+    n_temps, n_bin = int(temp.group(1)), int(args[0])
+    sparse = []
+    for t, m in enumerate(mats):
+        rows = [r.split() for r in m.split(";")]
+        assert len(rows) == n_bin + 2 and all(len(r) == n_bin + 2 for r in rows)
+        for e, r in enumerate(rows):
+            for v, x in enumerate(r):
+                if float(x) != 0.0:
+                    sparse.append([t, e, v, float(x)])
+    cp = {int(k): float(v) for k, v in re.findall(r"cp_3\[(\d+)\]==([-0-9.eE]+)", s)}
+    out = dict(
+        source="test/multihist_test.jl:19-31 (reference NPT Output of 32 atoms and expected multihistogram_NPT values)",
+        n_atoms=int(ens.group(1)),
+        pressure=float(ens.group(2)),
+        n_temps=n_temps,
+        t_min=float(temp.group(2)),
+        t_max=float(temp.group(3)),
+        n_bin=n_bin,
+        en_min=float(args[1]),
+        en_max=float(args[2]),
+        v_min=float(args[3]),
+        v_max=float(args[4]),
+        en_avg=_floats(args[6]),
+        heat_cap=_floats(args[7]),
+        ev_histogram_sparse=sparse,
+        conv_threshold=1e-3,
+        expected_cp=cp,
+        expected_len=int(re.search(r"length\(cp_3\)==(\d+)", s).group(1)),
+    )
+    assert len(mats) == n_temps and len(cp) == 4
+    (HERE / "multihist_npt32.json").write_text(json.dumps(out))
+
+
 def known_answers(ref: pathlib.Path):
     """Deterministic known answers of the reference's tests, with the lines
     they come from.  Each value is read from the file it cites."""
@@ -119,6 +168,7 @@ def main():
     ref = pathlib.Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     ne13_setup(ref)
     multihist_ne13(ref)
+    multihist_npt32(ref)
     known_answers(ref)
     print("fixtures written to", HERE)
 
