@@ -38,8 +38,12 @@ typedef struct ptmc_engine *ptmc_handle;
  * (src/BoundaryConditions.jl:44-116). */
 typedef struct {
     int32_t precision;         /* 0: FP64 everywhere; 1: binary32 positions and dE, FP64 totals */
-    int32_t ensemble;          /* 0: NVT, 1: NPT (uniform volume moves) */
-    int32_t boundary;          /* 0: SphericalBC, 1: CubicBC */
+    int32_t ensemble;          /* 0: NVT, 1: NPT (uniform volume moves), 2: NPT with separated
+                                  volume moves, z 1/6, xy 1/3, xyz 1/2 of them (NPT(n, p, true),
+                                  src/MCMoves.jl:301-343; rectangular and rhombic boxes) */
+    int32_t boundary;          /* 0: SphericalBC, 1: CubicBC, 2: RectangularBC (L, L, H),
+                                  3: RhombicBC (60 degree prism, side L, height H),
+                                  src/BoundaryConditions.jl:80-116 */
     int32_t exchange_mode;     /* 0: reference (p = 0.1 per cycle, one random adjacent pair,
                                       src/MCRun.jl:203, src/Exchange.jl:135);
                                   1: one random adjacent pair every cycle;
@@ -65,7 +69,7 @@ typedef struct {
     int32_t n_atoms;
     int32_t n_temps;
     double radius;             /* SphericalBC radius (bohr) */
-    double box_length;         /* CubicBC side length (bohr) */
+    double box_length;         /* CubicBC side / RectangularBC, RhombicBC length (bohr) */
     double pressure;           /* NPT pressure (Hartree / bohr^3) */
     const double *beta;        /* n_temps inverse temperatures (1/Hartree) */
     const double *x0;          /* n_configs * n_atoms * 3 start positions (bohr) */
@@ -74,6 +78,7 @@ typedef struct {
     double max_displ_vol;      /* initial volume step (reference default 0.1) */
     int32_t lanes_per_replica; /* 0: engine default */
     uint32_t run_key;          /* Philox counter word for this run; 0xffffffff = run index */
+    double box_height;         /* RectangularBC, RhombicBC height (bohr) */
 } ptmc_run_desc;
 
 /* Lifecycle. */
@@ -132,7 +137,8 @@ int ptmc_set_histogram(ptmc_handle h, int32_t run, double en_min, double en_max,
 /* reset_counters (src/MCRun.jl:270-276) for every run. */
 int ptmc_reset_counters(ptmc_handle h);
 
-/* Read-back, temperature order. */
+/* Read-back, temperature order.  box: n_temps lengths, or n_temps (length,
+ * height) pairs for rectangular and rhombic boxes. */
 int ptmc_get_energies(ptmc_handle h, int32_t run, double *en);
 int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *box);
 int ptmc_get_counters(ptmc_handle h, int32_t run, double *max_displ, int64_t *count_atom,
@@ -141,10 +147,21 @@ int ptmc_get_histograms(ptmc_handle h, int32_t run, double *hist); /* n_temps * 
 int ptmc_hist_stride(ptmc_handle h);                                /* n_bin+2 or (n_bin+2)^2 */
 int ptmc_get_drift(ptmc_handle h, int32_t run, double *drift);     /* max |E_run - E_FP64| */
 
-/* Write-back of a run's state (checkpoint restart, src/ReadSave.jl:436-481). */
+/* Write-back of a run's state (checkpoint restart, src/ReadSave.jl:436-481);
+ * box as ptmc_get_state returns it. */
 int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *box,
                    const double *max_displ, const int64_t *count_atom, const int64_t *count_vol,
                    const int64_t *count_exc, const double *ham);
+
+/* Separated volume moves (NPT(n, p, true)): per temperature the xy and z
+ * steps (max_displ[3], [4] of MCState, src/MCStates.jl:104) and the accepted
+ * xy / z move counters (count_vol_xy, count_vol_z: total, window;
+ * src/swap_config.jl:142-148), n_temps x 2 each, temperature order; the
+ * uniform volume moves stay in ptmc_get_counters' count_vol. */
+int ptmc_get_counters_sep(ptmc_handle h, int32_t run, double *max_displ_sep, int64_t *count_vol_xy,
+                          int64_t *count_vol_z);
+int ptmc_set_counters_sep(ptmc_handle h, int32_t run, const double *max_displ_sep, const int64_t *count_vol_xy,
+                          const int64_t *count_vol_z);
 
 /* Histogram counts of a run (restart: setresults, src/ReadSave.jl:385-417);
  * n_temps * hist_stride non-negative integral values. */

@@ -35,6 +35,7 @@ class orc_params(C.Structure):
         ("seed", C.c_uint64), ("run", C.c_uint32),
         ("sweep0", C.c_int64), ("cycle0", C.c_int64),
         ("en_min", C.c_double), ("delta_en_hist", C.c_double), ("v_min", C.c_double), ("delta_v_hist", C.c_double),
+        ("separated", C.c_int32),
     ]
 
 
@@ -66,6 +67,8 @@ class orc_state(C.Structure):
         ("error", C.POINTER(C.c_int32)),
         ("trace_de", _dp), ("trace_acc", C.POINTER(C.c_int8)), ("trace_idx", C.POINTER(C.c_int32)),
         ("replay_acc", C.POINTER(C.c_int8)),
+        ("box_h", _dp), ("max_box_h", _dp), ("lh_ratio", _dp), ("max_displ_sep", _dp),
+        ("count_vol_xy", _i64p), ("count_vol_z", _i64p),
     ]
 
 
@@ -88,6 +91,17 @@ def lib():
         _L.orc_energy_config.restype = C.c_double
         _L.orc_lrc.argtypes = [C.c_int, C.c_double, C.POINTER(orc_pot)]
         _L.orc_lrc.restype = C.c_double
+        _L.orc_distance2_box.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int]
+        _L.orc_distance2_box.restype = C.c_double
+        _L.orc_energy_config_box.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), _dp, C.c_double, C.c_double,
+                                             C.c_double, _dp]
+        _L.orc_energy_config_box.restype = C.c_double
+        for f in ("orc_r_cut", "orc_volume", "orc_hist_volume"):
+            getattr(_L, f).argtypes = [C.c_int, C.c_double, C.c_double]
+            getattr(_L, f).restype = C.c_double
+        _L.orc_lrc_box.argtypes = [C.c_int, C.c_double, C.POINTER(orc_pot), C.c_int, C.c_double, C.c_double]
+        _L.orc_lrc_box.restype = C.c_double
+        _L.orc_wrap.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp]
         _L.orc_exp_det.argtypes = [C.c_double]
         _L.orc_exp_det.restype = C.c_double
         _L.orc_log_det.argtypes = [C.c_double]
@@ -179,6 +193,8 @@ def forward(inp, nodes, act, params):
     return ret, eatom
 
 
+BC_CODES = {"sphere": 0, "cubic": 1, "rect": 2, "rhomb": 3}
+
 REFERENCE = dict(pair_form=0, dist_cache=1, lanes=1, math_lib=1)
 
 
@@ -191,7 +207,8 @@ class OracleRun:
 
     def __init__(self, n_atoms, beta, x0, *, pot: orc_pot, bc="sphere", radius=0.0, box=0.0, ensemble="nvt",
                  pressure=0.0, mode=None, fp32=False, exchange_mode=0, n_adjust=100, mc_sample=1, n_bin=100,
-                 min_acc=0.4, max_acc=0.6, seed=1234, run_key=0, max_displ=(0.1, 0.1), trace_cycles=0):
+                 min_acc=0.4, max_acc=0.6, seed=1234, run_key=0, max_displ=(0.1, 0.1), trace_cycles=0,
+                 box_h=None, separated=False, max_displ_sep=(0.1, 0.1)):
         mode = dict(REFERENCE if mode is None else mode)
         R = len(beta)
         N = n_atoms
@@ -199,7 +216,8 @@ class OracleRun:
         p = orc_params()
         p.n_atoms, p.n_temps = N, R
         p.ensemble = 1 if ensemble == "npt" else 0
-        p.bc = 1 if bc == "cubic" else 0
+        p.bc = BC_CODES[bc]
+        p.separated = 1 if separated else 0
         p.pair_form, p.dist_cache, p.lanes, p.math_lib = (mode["pair_form"], mode["dist_cache"], mode["lanes"],
                                                           mode["math_lib"])
         p.fp32 = 1 if fp32 else 0
@@ -218,7 +236,15 @@ class OracleRun:
         if fp32:
             self.x = self.x.astype(np.float32).astype(np.float64)
         self.box = np.full(R, float(box))
-        self.max_box = np.full(R, float(box) * 1.8 if bc == "cubic" else 30.0)
+        self.max_box = np.full(R, float(box) * 1.8 if bc != "sphere" else 30.0)
+        # heights and their bounds (src/MCStates.jl:69-86, :106): cubic boxes keep H = L
+        h = float(box if box_h is None else box_h)
+        self.box_h = np.full(R, h) if bc in ("rect", "rhomb") else None
+        self.max_box_h = np.full(R, h * 1.8) if bc in ("rect", "rhomb") else None
+        self.lh_ratio = self.max_box / self.max_box_h if bc in ("rect", "rhomb") else None
+        self.max_displ_sep = np.tile(np.array(max_displ_sep, dtype=np.float64), (R, 1))
+        self.count_vol_xy = np.zeros((R, 2), dtype=np.int64)
+        self.count_vol_z = np.zeros((R, 2), dtype=np.int64)
         self.r_cut = np.zeros(R)
         self.en_tot = np.zeros(R)
         self.beta = np.ascontiguousarray(beta, dtype=np.float64)
@@ -253,6 +279,9 @@ class OracleRun:
         s.ham, s.hist, s.d2, s.comp = _p(self.ham), _p(self.hist), _p(self.d2), _p(self.comp)
         s.g, s.eat = _p(self.g), _p(self.eat)
         s.error = _p(self.error, C.c_int32)
+        s.box_h, s.max_box_h, s.lh_ratio = _p(self.box_h), _p(self.max_box_h), _p(self.lh_ratio)
+        s.max_displ_sep = _p(self.max_displ_sep)
+        s.count_vol_xy, s.count_vol_z = _p(self.count_vol_xy, C.c_int64), _p(self.count_vol_z, C.c_int64)
         self.s = s
 
     def start_phase(self, production: bool):
@@ -299,6 +328,8 @@ class OracleRun:
     def reset_counters(self):
         self.count_atom[:] = 0
         self.count_vol[:] = 0
+        self.count_vol_xy[:] = 0
+        self.count_vol_z[:] = 0
         self.count_exc[:] = 0
 
 
@@ -321,9 +352,12 @@ def run_ptmc(run: OracleRun, mc_cycles: int, eq_cycles: int, n_atoms: int):
         for e in run.en_tot:
             eb = check_e_bounds(float(e), eb)
     v_min = v_max = 0.0
-    if run.p.bc == 1:
-        v_min = run.box[0] ** 3 * 0.8
-        v_max = run.box[0] ** 3 * 2.0
+    if run.p.bc != 0:
+        # initialise_histograms!, src/Sampling.jl:160-250 (volume of temperature 1's box)
+        v0 = lib().orc_volume(run.p.bc, float(run.box[0]),
+                              float(run.box_h[0]) if run.box_h is not None else float(run.box[0]))
+        v_min = v0 * 0.8
+        v_max = v0 * 2.0
     run.reset_counters()
     run.set_histogram(eb[0], eb[1], v_min, v_max)
     run.start_phase(True)

@@ -97,21 +97,114 @@ double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot)
     return e;
 }
 
-/* Squared distance, src/Configurations.jl:101-109.  bc cubic applies the
- * minimum image to b: b' = b + L round((a-b)/L).  form 0 divides by L as the
- * reference does; form 1 multiplies by 1/L as the engine does. */
-double orc_distance2(const double *a, const double *b, int bc, double box, int form)
+/* 3^0.5 and 1 / 3^0.5 as the reference's rhombic formulas evaluate them */
+#define ORC_SQRT3 1.7320508075688772
+#define ORC_INV_SQRT3 (1.0 / 1.7320508075688772)
+
+/* Squared distance, src/Configurations.jl:101-126.  Periodic boxes move b to
+ * its image nearest a.  Cubic: b' = b + L round((a-b)/L) per axis;
+ * rectangular: the same with the height H on z; rhombic (60 degree prism):
+ * y by the row spacing sqrt(3)/2 L, x along the skewed axis, z by H.  form 0
+ * divides as the reference does; form 1 multiplies by reciprocals computed
+ * once per box, as the engine does. */
+double orc_distance2_box(const double *a, const double *b, int bc, double l, double h, int form)
 {
     double bb[3] = {b[0], b[1], b[2]};
-    if (bc == ORC_CUBIC) {
-        double inv = 1.0 / box;
+    if (bc == ORC_CUBIC || bc == ORC_RECT) {
+        const double box[3] = {l, l, bc == ORC_RECT ? h : l};
         for (int c = 0; c < 3; ++c) {
-            double q = form == 0 ? (a[c] - b[c]) / box : (a[c] - b[c]) * inv;
-            bb[c] = b[c] + box * rint(q);
+            double inv = 1.0 / box[c];
+            double q = form == 0 ? (a[c] - b[c]) / box[c] : (a[c] - b[c]) * inv;
+            bb[c] = b[c] + box[c] * rint(q);
+        }
+    } else if (bc == ORC_RHOMB) {
+        const double ly = ORC_SQRT3 / 2.0 * l;
+        if (form == 0) {
+            bb[1] = b[1] + ly * rint((a[1] - b[1]) / ly);
+            bb[0] = b[0] - b[1] / ORC_SQRT3 + l * rint(((a[0] - b[0]) - ORC_INV_SQRT3 * (a[1] - b[1])) / l) +
+                    ORC_INV_SQRT3 * bb[1];
+            bb[2] = b[2] + h * rint((a[2] - b[2]) / h);
+        } else {
+            const double inv_ly = 1.0 / ly, inv_l = 1.0 / l, inv_h = 1.0 / h;
+            bb[1] = b[1] + ly * rint((a[1] - b[1]) * inv_ly);
+            bb[0] = b[0] - b[1] * ORC_INV_SQRT3 + l * rint(((a[0] - b[0]) - ORC_INV_SQRT3 * (a[1] - b[1])) * inv_l) +
+                    ORC_INV_SQRT3 * bb[1];
+            bb[2] = b[2] + h * rint((a[2] - b[2]) * inv_h);
         }
     }
     double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
     return dx * dx + dy * dy + dz * dz;
+}
+
+double orc_distance2(const double *a, const double *b, int bc, double box, int form)
+{
+    return orc_distance2_box(a, b, bc, box, box, form);
+}
+
+/* get_r_cut, src/Ensembles.jl:118-155 (squared cut-off) */
+double orc_r_cut(int bc, double l, double h)
+{
+    if (bc == ORC_RECT) return fmin(l * l / 4.0, h * h / 4.0);
+    if (bc == ORC_RHOMB) return fmin(l * l * 3.0 / 16.0, h * h / 4.0);
+    return l * l / 4.0;
+}
+
+/* get_volume, src/Configurations.jl:246-256 */
+double orc_volume(int bc, double l, double h)
+{
+    if (bc == ORC_RECT) return l * l * h;
+    if (bc == ORC_RHOMB) return l * l * h * ORC_SQRT3 / 2.0;
+    return l * l * l;
+}
+
+/* the tail correction of a box, src/EnergyEvaluation.jl:420, :447-448, :469-473 */
+double orc_lrc_box(int n_atoms, double r_cut, const orc_pot *pot, int bc, double l, double h)
+{
+    const double e = orc_lrc(n_atoms, r_cut, pot);
+    if (bc == ORC_RHOMB) return e * 3.0 / 4.0 * l / h;
+    if (bc == ORC_RECT) return l < h ? e * l / h : e * (h * h) / (l * l);
+    return e;
+}
+
+/* the volume find_hist_index bins, src/Sampling.jl:72-125: the rhombic form
+ * there reads L^2 * 3 H^0.5 / 2 (Julia's 3x^0.5 is 3 * x^0.5), which this
+ * restates as written */
+double orc_hist_volume(int bc, double l, double h)
+{
+    if (bc == ORC_RECT) return l * l * h;
+    if (bc == ORC_RHOMB) return l * l * (3.0 * sqrt(h)) / 2.0;
+    return l * l * l;
+}
+
+/* wrap of a trial position into the box, src/MCMoves.jl:51-98 (form as in
+ * orc_distance2_box) */
+void orc_wrap(int bc, double l, double h, int form, double *t)
+{
+    if (bc == ORC_CUBIC || bc == ORC_RECT) {
+        const double box[3] = {l, l, bc == ORC_RECT ? h : l};
+        for (int c = 0; c < 3; ++c) {
+            double inv = 1.0 / box[c];
+            double q = form == 0 ? t[c] / box[c] : t[c] * inv;
+            t[c] = t[c] - box[c] * rint(q);
+        }
+    } else if (bc == ORC_RHOMB) {
+        const double w = l * ORC_SQRT3 / 2.0, qo = l * ORC_SQRT3 / 4.0;
+        double k1, k2, k3;
+        if (form == 0) {
+            k2 = rint((t[1] - qo) / w);
+            k1 = rint((t[0] - t[1] / ORC_SQRT3 - l / 2.0) / l);
+            k3 = rint((t[2] - h / 2.0) / h);
+        } else {
+            const double inv_w = 1.0 / w, inv_l = 1.0 / l, inv_h = 1.0 / h;
+            k2 = rint((t[1] - qo) * inv_w);
+            k1 = rint((t[0] - t[1] * ORC_INV_SQRT3 - l / 2.0) * inv_l);
+            k3 = rint((t[2] - h / 2.0) * inv_h);
+        }
+        const double d0 = l * k1 + l / 2.0 * k2, d1 = w * k2, d2 = h * k3;
+        t[0] = t[0] - d0;
+        t[1] = t[1] - d1;
+        t[2] = t[2] - d2;
+    }
 }
 
 /* EAM invrexp, src/EnergyEvaluation.jl:1088-1095. */
@@ -142,11 +235,12 @@ static double tree_sum(double *v, int k)
 
 static inline int dist_form(const orc_params *p) { return p->pair_form == 0 ? 0 : 1; }
 
-/* Full configuration energy, src/EnergyEvaluation.jl:378-420: sum over i<j
- * (NPT: pairs inside r_cut, plus lrc).  en_atom optional.  With lanes k > 1
- * rows are dealt to lanes round-robin and combined by the butterfly. */
-double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *x,
-                         double box, double r_cut, double *en_atom)
+/* Full configuration energy, src/EnergyEvaluation.jl:378-474: sum over i<j
+ * (NPT: pairs inside r_cut, plus the box's lrc).  en_atom optional.  With
+ * lanes k > 1 rows are dealt to lanes round-robin and combined by the
+ * butterfly. */
+double orc_energy_config_box(const orc_params *p, const orc_pot *pot, const double *x, double l, double h,
+                             double r_cut, double *en_atom)
 {
     const int N = p->n_atoms;
     const int k = p->lanes > 0 ? p->lanes : 1;
@@ -155,7 +249,7 @@ double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *
     for (int i = 0; i < N; ++i) {
         int q = i % k;
         for (int j = i + 1; j < N; ++j) {
-            double r2 = orc_distance2(&x[3 * i], &x[3 * j], p->bc, box, dist_form(p));
+            double r2 = orc_distance2_box(&x[3 * i], &x[3 * j], p->bc, l, h, dist_form(p));
             if (p->ensemble == ORC_NPT && !(r2 <= r_cut)) continue;
             double e = orc_dimer_energy(pot, r2, p->pair_form);
             if (en_atom) { en_atom[i] += e; en_atom[j] += e; }
@@ -163,8 +257,14 @@ double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *
         }
     }
     double tot = tree_sum(part, k);
-    if (p->ensemble == ORC_NPT) tot += orc_lrc(N, r_cut, pot);
+    if (p->ensemble == ORC_NPT) tot += orc_lrc_box(N, r_cut, pot, p->bc, l, h);
     return tot;
+}
+
+double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *x,
+                         double box, double r_cut, double *en_atom)
+{
+    return orc_energy_config_box(p, pot, x, box, box, r_cut, en_atom);
 }
 
 /* EAM components and energy, src/EnergyEvaluation.jl:1105-1162, 1800-1811.
@@ -200,14 +300,15 @@ int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s)
     for (int r = 0; r < R; ++r) {
         double *x = &s->x[(size_t)r * N * 3];
         double box = s->box ? s->box[r] : 0.0;
-        if (p->ensemble == ORC_NPT) s->r_cut[r] = box * box / 4.0;   /* src/Ensembles.jl:118-120 */
+        double h = s->box_h ? s->box_h[r] : box;
+        if (p->ensemble == ORC_NPT) s->r_cut[r] = orc_r_cut(p->bc, box, h);   /* src/Ensembles.jl:118-155 */
         if (s->d2) {
             double *d2 = &s->d2[(size_t)r * N * N];
             for (int i = 0; i < N; ++i) {
                 d2[i * N + i] = 0.0;
                 for (int j = i + 1; j < N; ++j)
                     d2[i * N + j] = d2[j * N + i] =
-                        orc_distance2(&x[3 * i], &x[3 * j], p->bc, box, dist_form(p));
+                        orc_distance2_box(&x[3 * i], &x[3 * j], p->bc, box, h, dist_form(p));
             }
         }
         if (pot->kind == ORC_POT_NNP) {
@@ -220,8 +321,8 @@ int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s)
             s->en_tot[r] = eam_energy_from_comp(pot, &s->comp[(size_t)r * N * 2], N,
                                                 p->math_lib ? 1 : (p->lanes > 0 ? p->lanes : 1));
         } else {
-            s->en_tot[r] = orc_energy_config(p, pot, x, box,
-                                             p->ensemble == ORC_NPT ? s->r_cut[r] : 0.0, NULL);
+            s->en_tot[r] = orc_energy_config_box(p, pot, x, box, h,
+                                                 p->ensemble == ORC_NPT ? s->r_cut[r] : 0.0, NULL);
         }
     }
     return 0;
@@ -259,36 +360,41 @@ static draws draw_move(const orc_params *p, uint32_t nsteps, int r, uint32_t swe
     return d;
 }
 
-/* trial position, src/MCMoves.jl:33-64: pos + (rand()-0.5)*delta per axis,
- * cubic: wrapped back with round(trial/L). */
-static void make_trial(const orc_params *p, const double *pos, double delta, double box,
+/* trial position, src/MCMoves.jl:33-98: pos + (rand()-0.5)*delta per axis,
+ * periodic boxes wrap it back (orc_wrap; binary32: the same steps in
+ * binary32 with the box constants rounded from FP64). */
+static void make_trial(const orc_params *p, const double *pos, double delta, double box, double h,
                        const draws *d, double *t)
 {
     double u[3] = {d->ux, d->uy, d->uz};
     if (!p->fp32) {
         for (int c = 0; c < 3; ++c) t[c] = pos[c] + (u[c] - 0.5) * delta;
-        if (p->bc == ORC_CUBIC) {
-            double inv = 1.0 / box;
-            for (int c = 0; c < 3; ++c) {
-                double q = p->pair_form == 0 ? t[c] / box : t[c] * inv;
-                t[c] = t[c] - box * rint(q);
-            }
-        }
+        if (p->bc != ORC_SPHERE) orc_wrap(p->bc, box, h, p->pair_form == 0 ? 0 : 1, t);
     } else {
         float fd = (float)delta;
-        for (int c = 0; c < 3; ++c) {
-            float v = (float)pos[c] + ((float)u[c] - 0.5f) * fd;
-            t[c] = (double)v;
-        }
-        if (p->bc == ORC_CUBIC) {
-            float L = (float)box, inv = (float)(1.0 / box);
+        float v[3];
+        for (int c = 0; c < 3; ++c) v[c] = (float)pos[c] + ((float)u[c] - 0.5f) * fd;
+        if (p->bc == ORC_CUBIC || p->bc == ORC_RECT) {
+            const double bx[3] = {box, box, p->bc == ORC_RECT ? h : box};
             for (int c = 0; c < 3; ++c) {
-                float v = (float)t[c];
-                float q = rintf(v * inv);
-                v = v - L * q;
-                t[c] = (double)v;
+                float L = (float)bx[c], inv = (float)(1.0 / bx[c]);
+                float q = rintf(v[c] * inv);
+                v[c] = v[c] - L * q;
             }
+        } else if (p->bc == ORC_RHOMB) {
+            const double w = box * ORC_SQRT3 / 2.0;
+            const float L = (float)box, H = (float)h, W = (float)w, qo = (float)(box * ORC_SQRT3 / 4.0);
+            const float inv_w = (float)(1.0 / w), inv_l = (float)(1.0 / box), inv_h = (float)(1.0 / h);
+            const float hl = (float)(box / 2.0), hh = (float)(h / 2.0), s3 = (float)ORC_INV_SQRT3;
+            float k2 = rintf((v[1] - qo) * inv_w);
+            float k1 = rintf((v[0] - v[1] * s3 - hl) * inv_l);
+            float k3 = rintf((v[2] - hh) * inv_h);
+            float d0 = L * k1 + hl * k2;
+            v[0] = v[0] - d0;
+            v[1] = v[1] - W * k2;
+            v[2] = v[2] - H * k3;
         }
+        for (int c = 0; c < 3; ++c) t[c] = (double)v[c];
     }
 }
 
@@ -335,12 +441,13 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
     const int k = p->lanes > 0 ? p->lanes : 1;
     const double *x = &s->x[(size_t)r * N * 3];
     const double box = s->box ? s->box[r] : 0.0;
+    const double h = s->box_h ? s->box_h[r] : box;
     const double *d2 = s->d2 ? &s->d2[(size_t)r * N * N] : NULL;
     const int cutoff = p->ensemble == ORC_NPT;
     const double rc = cutoff ? s->r_cut[r] : 0.0;
 
     for (int j = 0; j < N; ++j)
-        o->new_row[j] = j == i ? 0.0 : orc_distance2(trial, &x[3 * j], p->bc, box, dist_form(p));
+        o->new_row[j] = j == i ? 0.0 : orc_distance2_box(trial, &x[3 * j], p->bc, box, h, dist_form(p));
 
     if (pot->kind == ORC_POT_NNP) {
         /* energy_update!(RuNNerPotential), src/EnergyEvaluation.jl:1593-1621: the
@@ -410,7 +517,7 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
         int q = j % k;
         double rn = o->new_row[j];
         double ro = d2 ? d2[i * N + j]
-                       : orc_distance2(&x[3 * i], &x[3 * j], p->bc, box, dist_form(p));
+                       : orc_distance2_box(&x[3 * i], &x[3 * j], p->bc, box, h, dist_form(p));
         if (!cutoff || rn <= rc) pn[q] += orc_dimer_energy(pot, rn, p->pair_form);
         if (!cutoff || ro <= rc) po[q] += orc_dimer_energy(pot, ro, p->pair_form);
     }
@@ -433,7 +540,8 @@ static void swap_states(const orc_params *p, const orc_pot *pot, orc_state *s, i
     SWAPN(s->x, (size_t)N * 3);
     SWAPN(s->d2, (size_t)N * N);
     SWAPN(s->en_tot, 1);
-    if (p->bc == ORC_CUBIC) SWAPN(s->box, 1);
+    if (p->bc != ORC_SPHERE) SWAPN(s->box, 1);
+    if (p->bc == ORC_RECT || p->bc == ORC_RHOMB) SWAPN(s->box_h, 1);
     if (p->ensemble == ORC_NPT) SWAPN(s->r_cut, 1);
     if (pot->kind == ORC_POT_EAM) SWAPN(s->comp, (size_t)N * 2);
     if (pot->kind == ORC_POT_NNP) {
@@ -453,8 +561,8 @@ static void try_exchange(const orc_params *p, const orc_pot *pot, orc_state *s, 
     s->count_exc[2 * (n + 1)] += 1;
     double e1 = s->en_tot[n], e2 = s->en_tot[n + 1];
     if (p->ensemble == ORC_NPT) {
-        double v1 = s->box[n] * s->box[n] * s->box[n];
-        double v2 = s->box[n + 1] * s->box[n + 1] * s->box[n + 1];
+        double v1 = orc_volume(p->bc, s->box[n], s->box_h ? s->box_h[n] : s->box[n]);
+        double v2 = orc_volume(p->bc, s->box[n + 1], s->box_h ? s->box_h[n + 1] : s->box[n + 1]);
         e1 = e1 + p->pressure * v1;
         e2 = e2 + p->pressure * v2;
     }
@@ -483,7 +591,12 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
     const uint32_t nsteps = (uint32_t)(p->ensemble == ORC_NPT ? N + 1 : N);
     trial_out *o = (trial_out *)calloc(1, sizeof(trial_out));
     double *vol_x = (double *)malloc(sizeof(double) * 3 * N);
-    if (!o || !vol_x) return -2;
+    double *vol_d2 = (p->separated && s->d2) ? (double *)malloc(sizeof(double) * N * N) : NULL;
+    if (!o || !vol_x || (p->separated && s->d2 && !vol_d2)) return -2;
+    if (p->separated && (p->ensemble != ORC_NPT || (p->bc != ORC_RECT && p->bc != ORC_RHOMB) || !s->box_h ||
+                         !s->max_box_h || !s->lh_ratio || !s->max_displ_sep || !s->count_vol_xy ||
+                         !s->count_vol_z))
+        return -4;
     if (pot->kind == ORC_POT_NNP) {
         if (p->ensemble != ORC_NVT) return -3;
         o->g_new = (double *)malloc(sizeof(double) * N * (pot->nnp->n_rad + pot->nnp->n_ang));
@@ -499,6 +612,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
         for (int r = 0; r < R; ++r) {
             double *x = &s->x[(size_t)r * N * 3];
             double box = s->box ? s->box[r] : 0.0;
+            double h = s->box_h ? s->box_h[r] : box;
             for (uint32_t m = 0; m < nsteps; ++m) {
                 draws d0 = draw_move(p, nsteps, r, sweep, m, 0);
                 const int i = (int)d0.index;
@@ -508,7 +622,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                 if (i < N) {
                     /* atom move, src/MCMoves.jl:115-145 / :100-113 */
                     double t[3];
-                    make_trial(p, &x[3 * i], s->max_displ[2 * r], box, &d0, t);
+                    make_trial(p, &x[3 * i], s->max_displ[2 * r], box, h, &d0, t);
                     if (p->bc == ORC_SPHERE) {
                         int count = 0, attempt = 0;
                         while (outside(p, t)) {
@@ -518,10 +632,10 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                             } else {
                                 attempt += 1;
                                 draws dd = draw_move(p, nsteps, r, sweep, m, attempt);
-                                make_trial(p, &x[3 * i], s->max_displ[2 * r], box, &dd, t);
+                                make_trial(p, &x[3 * i], s->max_displ[2 * r], box, h, &dd, t);
                                 if (count == 100) {
                                     if (s->error) *s->error = 1;
-                                    free(o->g_new); free(o->eat_new); free(o); free(vol_x);
+                                    free(o->g_new); free(o->eat_new); free(o); free(vol_x); free(vol_d2);
                                     return 1;
                                 }
                             }
@@ -560,30 +674,82 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         s->count_atom[2 * r + 1] += 1;
                     }
                 } else {
-                    /* uniform volume move, src/MCMoves.jl:169-177, 277-287 */
-                    double scale;
-                    double y = (d0.ux - 0.5) * s->max_displ[2 * r + 1];
-                    if (p->math_lib) scale = pow(exp(y), 1.0 / 3.0);
-                    else scale = orc_exp(y / 3.0);
-                    if (box >= s->max_box[r] && scale > 1.0) scale = 1.0;
-                    double nbox = box * scale;
-                    double nrc = nbox * nbox / 4.0;
-                    double s2 = scale * scale;
-                    /* energy of the scaled configuration: scaled r^2 matrix,
-                     * src/EnergyEvaluation.jl:397-420 */
-                    double part[64] = {0};
-                    const int k = p->lanes > 0 ? p->lanes : 1;
-                    for (int a = 0; a < N; ++a) {
-                        for (int b = a + 1; b < N; ++b) {
-                            double r2 = s->d2 ? s->d2[(size_t)r * N * N + a * N + b]
-                                              : orc_distance2(&x[3 * a], &x[3 * b], p->bc, box, dist_form(p));
-                            r2 = r2 * s2;
-                            if (r2 <= nrc) part[a % k] += orc_dimer_energy(pot, r2, p->pair_form);
+                    /* volume move: uniform, src/MCMoves.jl:169-177, :277-287, or
+                     * separated (z 1/6, xy 1/3, xyz 1/2 of the moves),
+                     * src/MCMoves.jl:235-343 */
+                    int kind = 0; /* 0: xyz, 1: xy, 2: z (xy_or_z) */
+                    double sx, sz;
+                    if (!p->separated) {
+                        double y = (d0.ux - 0.5) * s->max_displ[2 * r + 1];
+                        double scale = p->math_lib ? pow(exp(y), 1.0 / 3.0) : orc_exp(y / 3.0);
+                        if (box >= s->max_box[r] && scale > 1.0) scale = 1.0;
+                        sx = sz = scale;
+                    } else {
+                        const int ra = 1 + (int)(d0.uy * 6.0);
+                        const double lh = box / h, lr = s->lh_ratio[r];
+                        if (ra == 1) {
+                            kind = 2;
+                            double y = (d0.ux - 0.5) * s->max_displ_sep[2 * r + 1];
+                            double scale = p->math_lib ? exp(y) : orc_exp(y);
+                            if (lh <= lr * 1.1 && scale > 1.0) scale = 1.0 / scale;
+                            else if (lh >= lr * 0.909 && scale < 1.0) scale = 1.0 / scale;
+                            if (h >= s->max_box_h[r] && scale > 1.0) scale = 1.0 / scale;
+                            sx = 1.0;
+                            sz = scale;
+                        } else if (ra <= 3) {
+                            kind = 1;
+                            double y = (d0.ux - 0.5) * s->max_displ_sep[2 * r];
+                            double scale = p->math_lib ? pow(exp(y), 0.5) : orc_exp(y / 2.0);
+                            if (lh >= lr * 1.1 && scale > 1.0) scale = 1.0 / scale;
+                            else if (lh <= lr * 0.909 && scale < 1.0) scale = 1.0 / scale;
+                            if (box >= s->max_box[r] && scale > 1.0) scale = 1.0 / scale;
+                            sx = scale;
+                            sz = 1.0;
+                        } else {
+                            double y = (d0.ux - 0.5) * s->max_displ[2 * r + 1];
+                            double scale = p->math_lib ? pow(exp(y), 1.0 / 3.0) : orc_exp(y / 3.0);
+                            if (box >= s->max_box[r] && scale > 1.0) scale = 1.0;
+                            sx = sz = scale;
                         }
                     }
-                    new_en = tree_sum(part, k) + orc_lrc(N, nrc, pot);
+                    double nbox = box * sx;
+                    double nh = h * sz;
+                    double nrc = orc_r_cut(p->bc, nbox, nh);
+                    /* energy of the trial configuration, src/EnergyEvaluation.jl:397-474:
+                     * uniform moves scale the r^2 matrix, separated moves
+                     * recompute it from the scaled positions in the new box */
+                    double part[64] = {0};
+                    const int k = p->lanes > 0 ? p->lanes : 1;
+                    if (!p->separated) {
+                        const double s2 = sx * sx;
+                        for (int a = 0; a < N; ++a) {
+                            for (int b = a + 1; b < N; ++b) {
+                                double r2 = s->d2 ? s->d2[(size_t)r * N * N + a * N + b]
+                                                  : orc_distance2_box(&x[3 * a], &x[3 * b], p->bc, box, h,
+                                                                      dist_form(p));
+                                r2 = r2 * s2;
+                                if (r2 <= nrc) part[a % k] += orc_dimer_energy(pot, r2, p->pair_form);
+                            }
+                        }
+                    } else {
+                        for (int a = 0; a < N; ++a) {
+                            vol_x[3 * a] = x[3 * a] * sx;
+                            vol_x[3 * a + 1] = x[3 * a + 1] * sx;
+                            vol_x[3 * a + 2] = x[3 * a + 2] * sz;
+                        }
+                        for (int a = 0; a < N; ++a) {
+                            if (vol_d2) vol_d2[a * N + a] = 0.0;
+                            for (int b = a + 1; b < N; ++b) {
+                                double r2 = orc_distance2_box(&vol_x[3 * a], &vol_x[3 * b], p->bc, nbox, nh,
+                                                              dist_form(p));
+                                if (vol_d2) vol_d2[a * N + b] = vol_d2[b * N + a] = r2;
+                                if (r2 <= nrc) part[a % k] += orc_dimer_energy(pot, r2, p->pair_form);
+                            }
+                        }
+                    }
+                    new_en = tree_sum(part, k) + orc_lrc_box(N, nrc, pot, p->bc, nbox, nh);
                     double dE = new_en - s->en_tot[r];
-                    double v_new = nbox * nbox * nbox, v_old = box * box * box;
+                    double v_new = orc_volume(p->bc, nbox, nh), v_old = orc_volume(p->bc, box, h);
                     double dh = dE + p->pressure * (v_new - v_old);
                     double arg = -dh * s->beta[r] + (double)N * xlog(p, v_new / v_old);
                     double prob = xexp(p, arg);
@@ -595,18 +761,30 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                     if (s->trace_idx) s->trace_idx[tix] = i;
                     if (accept) {
                         /* swap_config_v!, src/swap_config.jl:113-140 */
-                        for (int a = 0; a < 3 * N; ++a) {
-                            double v = x[a] * scale;
-                            x[a] = p->fp32 ? (double)(float)v : v;
+                        for (int a = 0; a < N; ++a) {
+                            const double f[3] = {sx, sx, sz};
+                            for (int c = 0; c < 3; ++c) {
+                                double v = x[3 * a + c] * f[c];
+                                x[3 * a + c] = p->fp32 ? (double)(float)v : v;
+                            }
                         }
-                        if (s->d2)
-                            for (int a = 0; a < N * N; ++a) s->d2[(size_t)r * N * N + a] *= s2;
+                        if (s->d2) {
+                            if (!p->separated) {
+                                const double s2 = sx * sx;
+                                for (int a = 0; a < N * N; ++a) s->d2[(size_t)r * N * N + a] *= s2;
+                            } else {
+                                memcpy(&s->d2[(size_t)r * N * N], vol_d2, sizeof(double) * N * N);
+                            }
+                        }
                         s->box[r] = nbox;
                         box = nbox;
+                        if (s->box_h) s->box_h[r] = nh;
+                        h = nh;
                         s->r_cut[r] = nrc;
                         s->en_tot[r] = new_en;
-                        s->count_vol[2 * r] += 1;
-                        s->count_vol[2 * r + 1] += 1;
+                        int64_t *cnt = kind == 0 ? s->count_vol : (kind == 1 ? s->count_vol_xy : s->count_vol_z);
+                        cnt[2 * r] += 1;
+                        cnt[2 * r + 1] += 1;
                     }
                 }
             }
@@ -633,11 +811,25 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                 if (acc < p->min_acc) s->max_displ[2 * r] *= 0.9;
                 else if (acc > p->max_acc) s->max_displ[2 * r] *= 1.1;
                 s->count_atom[2 * r + 1] = 0;
-                if (p->ensemble == ORC_NPT) {
+                if (p->ensemble == ORC_NPT && !p->separated) {
                     double av = (double)s->count_vol[2 * r + 1] / (double)((int64_t)p->n_adjust * 1);
                     if (av < p->min_acc) s->max_displ[2 * r + 1] *= 0.9;
                     else if (av > p->max_acc) s->max_displ[2 * r + 1] *= 1.1;
                     s->count_vol[2 * r + 1] = 0;
+                } else if (p->ensemble == ORC_NPT) {
+                    /* separated: expected shares 1/2, 1/3, 1/6 of the volume moves */
+                    double av = (double)s->count_vol[2 * r + 1] / ((double)p->n_adjust / 2.0);
+                    if (av < p->min_acc) s->max_displ[2 * r + 1] *= 0.9;
+                    else if (av > p->max_acc) s->max_displ[2 * r + 1] *= 1.1;
+                    s->count_vol[2 * r + 1] = 0;
+                    av = (double)s->count_vol_xy[2 * r + 1] / ((double)p->n_adjust / 3.0);
+                    if (av < p->min_acc) s->max_displ_sep[2 * r] *= 0.9;
+                    else if (av > p->max_acc) s->max_displ_sep[2 * r] *= 1.1;
+                    s->count_vol_xy[2 * r + 1] = 0;
+                    av = (double)s->count_vol_z[2 * r + 1] / ((double)p->n_adjust / 6.0);
+                    if (av < p->min_acc) s->max_displ_sep[2 * r + 1] *= 0.9;
+                    else if (av > p->max_acc) s->max_displ_sep[2 * r + 1] *= 1.1;
+                    s->count_vol_z[2 * r + 1] = 0;
                 }
             }
         }
@@ -648,13 +840,15 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
             for (int r = 0; r < R; ++r) {
                 double e = s->en_tot[r];
                 if (p->ensemble == ORC_NPT) {
-                    double v = s->box[r] * s->box[r] * s->box[r];
+                    const double bh = s->box_h ? s->box_h[r] : s->box[r];
+                    double v = orc_volume(p->bc, s->box[r], bh);
                     double h = e + p->pressure * v;
                     s->ham[2 * r] += h;
                     s->ham[2 * r + 1] += h * h;
                     if (s->hist) {
+                        double vh = orc_hist_volume(p->bc, s->box[r], bh);
                         double ie = floor((e - p->en_min) / p->delta_en_hist) + 1.0;
-                        double iv = floor((v - p->v_min) / p->delta_v_hist) + 1.0;
+                        double iv = floor((vh - p->v_min) / p->delta_v_hist) + 1.0;
                         int he = hist_index(ie, p->n_bin), hv = hist_index(iv, p->n_bin);
                         s->hist[(size_t)r * nb2 * nb2 + (size_t)(he - 1) * nb2 + (hv - 1)] += 1.0;
                     }
@@ -674,5 +868,6 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
     free(o->eat_new);
     free(o);
     free(vol_x);
+    free(vol_d2);
     return 0;
 }

@@ -17,7 +17,7 @@ extern "C" {
 #endif
 
 enum { ORC_NVT = 0, ORC_NPT = 1 };
-enum { ORC_SPHERE = 0, ORC_CUBIC = 1 };
+enum { ORC_SPHERE = 0, ORC_CUBIC = 1, ORC_RECT = 2, ORC_RHOMB = 3 };
 enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3 };
 /* exchange_mode: 0 = reference (p = 0.1 per cycle, one random adjacent pair),
  *                1 = one random adjacent pair every cycle,
@@ -43,6 +43,7 @@ typedef struct {
     int64_t sweep0;       /* global cycle counter of the first cycle (RNG counter) */
     int64_t cycle0;       /* phase-local index of the first cycle minus one */
     double  en_min, delta_en_hist, v_min, delta_v_hist;
+    int32_t separated;    /* NPT: separated xy / z / xyz volume moves (src/MCMoves.jl:301-343) */
 } orc_params;
 
 /* RuNNer-style neural-network potential (src/EnergyEvaluation.jl:1182-1275,
@@ -97,6 +98,13 @@ typedef struct {
     int8_t  *trace_acc;    /* 1 accepted */
     int32_t *trace_idx;    /* moved atom (N = volume move) */
     const int8_t *replay_acc; /* if non-null: take these accept decisions (fp32 replay) */
+    /* rectangular / rhombic boxes (NULL otherwise) */
+    double  *box_h;        /* R box heights */
+    double  *max_box_h;    /* R max box heights (src/MCStates.jl:75-86) */
+    double  *lh_ratio;     /* R length / height ratio bounds (src/MCStates.jl:106) */
+    double  *max_displ_sep;/* R*2 : xy, z volume steps (separated moves) */
+    int64_t *count_vol_xy; /* R*2 */
+    int64_t *count_vol_z;  /* R*2 */
 } orc_state;
 
 /* Build the r^2 matrices / EAM components and total energies from positions
@@ -112,6 +120,15 @@ double orc_distance2(const double *a, const double *b, int bc, double box, int c
 double orc_energy_config(const orc_params *p, const orc_pot *pot, const double *x,
                          double box, double r_cut, double *en_atom);
 double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot);
+/* boxes with a height (rectangular, rhombic; cubic and sphere ignore h) */
+double orc_distance2_box(const double *a, const double *b, int bc, double l, double h, int form);
+double orc_energy_config_box(const orc_params *p, const orc_pot *pot, const double *x, double l, double h,
+                             double r_cut, double *en_atom);
+double orc_r_cut(int bc, double l, double h);
+double orc_volume(int bc, double l, double h);
+double orc_lrc_box(int n_atoms, double r_cut, const orc_pot *pot, int bc, double l, double h);
+double orc_hist_volume(int bc, double l, double h);
+void   orc_wrap(int bc, double l, double h, int form, double *t);
 double orc_exp_det(double x);
 double orc_log_det(double x);
 void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -109,10 +109,15 @@ struct ptmc_engine {
     DevBuf<TileInfo> d_tiles;
     DevBuf<RunInfo> d_runs;
     DevBuf<int32_t> d_slot_run, d_slot_temp, d_run_error;
-    DevBuf<double> d_slot_en, d_slot_box, d_slot_drift;
+    DevBuf<double> d_slot_en, d_slot_box, d_slot_boxh, d_slot_drift;
     DevBuf<unsigned char> d_coords, d_comps;
     DevBuf<double> d_beta, d_displ, d_maxbox, d_ham;
     DevBuf<long long> d_count_atom, d_count_vol, d_count_exc;
+    // boxes with a height: per temperature max height, length/height bound,
+    // separated-move steps (xy, z) and acceptance counters
+    DevBuf<double> d_maxboxh, d_lh, d_displ_sep;
+    DevBuf<long long> d_count_volxy, d_count_volz;
+    bool separated = false;
     DevBuf<uint32_t> d_hist;
     DevBuf<double> d_trace_de;
     DevBuf<int8_t> d_trace_acc;
@@ -201,17 +206,26 @@ int default_lanes(int n_atoms, int precision)
     return kl;
 }
 
+// bc is the configured boundary: rectangular boxes run the cubic kernels
 KernelPair lookup(int precision, int bc, int ens, int pot, int pk, int kc, int kl, int lanes)
 {
-    if (pot == POT_EAM) return ens == ENS_NVT ? lookup_eam(precision, bc, kl, lanes) : KernelPair{};
+    if (pot == POT_EAM) {
+        if (bc != BC_SPHERE && bc != BC_CUBIC) return {};
+        return ens == ENS_NVT ? lookup_eam(precision, bc, kl, lanes) : KernelPair{};
+    }
+    if (bc == BC_RECT) bc = BC_CUBIC;
     if (precision == 0) {
         if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f64_sphere_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f64_cubic_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f64_cubic_npt(pk, kc, kl, lanes);
+        if (bc == BC_RHOMB && ens == ENS_NVT) return lookup_f64_rhomb_nvt(pk, kc, kl, lanes);
+        if (bc == BC_RHOMB && ens == ENS_NPT) return lookup_f64_rhomb_npt(pk, kc, kl, lanes);
     } else {
         if (bc == BC_SPHERE && ens == ENS_NVT) return lookup_f32_sphere_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NVT) return lookup_f32_cubic_nvt(pk, kc, kl, lanes);
         if (bc == BC_CUBIC && ens == ENS_NPT) return lookup_f32_cubic_npt(pk, kc, kl, lanes);
+        if (bc == BC_RHOMB && ens == ENS_NVT) return lookup_f32_rhomb_nvt(pk, kc, kl, lanes);
+        if (bc == BC_RHOMB && ens == ENS_NPT) return lookup_f32_rhomb_npt(pk, kc, kl, lanes);
     }
     return {};
 }
@@ -240,6 +254,8 @@ int choose_pair(ptmc_engine *h)
     const int K = p.ncoeff;
     if (p.kind == POT_EAM) {
         if (h->cfg.ensemble != ENS_NVT) return fail(h, PTMC_E_UNSUPPORTED, "EAM runs in the NVT ensemble only");
+        if (h->cfg.boundary != BC_SPHERE && h->cfg.boundary != BC_CUBIC)
+            return fail(h, PTMC_E_UNSUPPORTED, "EAM runs in a spherical container or a cubic box");
         if (h->cfg.pair_form == 1) return fail(h, PTMC_E_UNSUPPORTED, "EAM has no reference pair form");
         return PTMC_OK;
     }
@@ -449,12 +465,20 @@ SweepArgs base_args(ptmc_engine *h)
     a.slot_temp = h->d_slot_temp.p;
     a.slot_en = h->d_slot_en.p;
     a.slot_box = h->d_slot_box.p;
+    a.slot_boxh = h->d_slot_boxh.p;
     a.slot_drift = h->d_slot_drift.p;
     a.coords = h->d_coords.p;
     a.comps = h->d_comps.p;
     a.t_beta = h->d_beta.p;
     a.t_displ = h->d_displ.p;
     a.t_maxbox = h->d_maxbox.p;
+    a.t_maxboxh = h->d_maxboxh.p;
+    a.t_lh = h->d_lh.p;
+    a.t_displ_sep = h->d_displ_sep.p;
+    a.t_count_volxy = h->d_count_volxy.p;
+    a.t_count_volz = h->d_count_volz.p;
+    a.box_kind = h->cfg.boundary == BC_RECT ? BC_RECT : BC_CUBIC;
+    a.separated = h->separated ? 1 : 0;
     a.t_count_atom = h->d_count_atom.p;
     a.t_count_vol = h->d_count_vol.p;
     a.t_count_exc = h->d_count_exc.p;
@@ -651,13 +675,18 @@ int ptmc_create(const ptmc_config *cfg, ptmc_handle *out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PTMC_E_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return PTMC_E_ARG;
-    if (cfg->precision < 0 || cfg->precision > 1 || cfg->ensemble < 0 || cfg->ensemble > 1 ||
-        cfg->boundary < 0 || cfg->boundary > 1 || cfg->exchange_mode < 0 || cfg->exchange_mode > 2 ||
+    if (cfg->precision < 0 || cfg->precision > 1 || cfg->ensemble < 0 || cfg->ensemble > 2 ||
+        cfg->boundary < 0 || cfg->boundary > 3 || cfg->exchange_mode < 0 || cfg->exchange_mode > 2 ||
         cfg->n_adjust <= 0 || cfg->mc_sample <= 0 || cfg->n_bin < 2)
         return PTMC_E_ARG;
-    if (cfg->ensemble == ENS_NPT && cfg->boundary == BC_SPHERE) return PTMC_E_ARG;  // src/Ensembles.jl:191-193
+    if (cfg->ensemble != ENS_NVT && cfg->boundary == BC_SPHERE) return PTMC_E_ARG;  // src/Ensembles.jl:191-193
+    // separated volume moves scale x, y and z apart: boxes with a height only
+    // (scale_xy / scale_z have no CubicBC method, src/MCMoves.jl:189-225)
+    if (cfg->ensemble == 2 && cfg->boundary != BC_RECT && cfg->boundary != BC_RHOMB) return PTMC_E_ARG;
     ptmc_engine *h = new ptmc_engine();
     h->cfg = *cfg;
+    h->separated = cfg->ensemble == 2;
+    if (h->separated) h->cfg.ensemble = ENS_NPT;
     if (h->cfg.cycles_per_launch <= 0) h->cfg.cycles_per_launch = 256;
     if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
@@ -680,6 +709,12 @@ int ptmc_destroy(ptmc_handle h)
     h->d_run_error.free_();
     h->d_slot_en.free_();
     h->d_slot_box.free_();
+    h->d_slot_boxh.free_();
+    h->d_maxboxh.free_();
+    h->d_lh.free_();
+    h->d_displ_sep.free_();
+    h->d_count_volxy.free_();
+    h->d_count_volz.free_();
     h->d_slot_drift.free_();
     h->d_coords.free_();
     h->d_comps.free_();
@@ -912,7 +947,9 @@ int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *d, int32_t *run_index)
     if (d->n_atoms < 2 || d->n_temps < 1 || !d->beta || !d->x0 || d->n_configs < 1)
         return fail(h, PTMC_E_ARG, "bad run description");
     if (h->cfg.boundary == BC_SPHERE && !(d->radius > 0)) return fail(h, PTMC_E_ARG, "radius must be positive");
-    if (h->cfg.boundary == BC_CUBIC && !(d->box_length > 0)) return fail(h, PTMC_E_ARG, "box length must be positive");
+    if (h->cfg.boundary != BC_SPHERE && !(d->box_length > 0)) return fail(h, PTMC_E_ARG, "box length must be positive");
+    if ((h->cfg.boundary == BC_RECT || h->cfg.boundary == BC_RHOMB) && !(d->box_height > 0))
+        return fail(h, PTMC_E_ARG, "box height must be positive");
     if (d->n_temps > (1 << 20)) return fail(h, PTMC_E_ARG, "too many temperatures");
     if (d->lanes_per_replica != 0 && d->lanes_per_replica != 1 && d->lanes_per_replica != 2 &&
         d->lanes_per_replica != 4 && d->lanes_per_replica != 8 && d->lanes_per_replica != 16)
@@ -959,8 +996,10 @@ int ptmc_prepare(ptmc_handle h)
 
     std::vector<RunInfo> infos;
     std::vector<int32_t> slot_run(h->n_slots), slot_temp(h->n_slots);
-    std::vector<double> slot_en(h->n_slots, 0.0), slot_box(h->n_slots, 1.0);
-    std::vector<double> beta, displ, maxbox;
+    std::vector<double> slot_en(h->n_slots, 0.0), slot_box(h->n_slots, 1.0), slot_boxh(h->n_slots, 1.0);
+    std::vector<double> beta, displ, maxbox, maxboxh, lh, displ_sep;
+    const bool pbc = h->cfg.boundary != BC_SPHERE;
+    const bool has_h = h->cfg.boundary == BC_RECT || h->cfg.boundary == BC_RHOMB;
     int tb = 0;
     for (int r = 0; r < (int)h->runs.size(); ++r) {
         RunHost &rh = h->runs[r];
@@ -979,12 +1018,19 @@ int ptmc_prepare(ptmc_handle h)
         for (int t = 0; t < rh.d.n_temps; ++t) {
             slot_run[ri.slot_base + t] = r;
             slot_temp[ri.slot_base + t] = t;
-            slot_box[ri.slot_base + t] = h->cfg.boundary == BC_CUBIC ? rh.d.box_length : 1.0;
+            const double bl = pbc ? rh.d.box_length : 1.0, bh = has_h ? rh.d.box_height : bl;
+            slot_box[ri.slot_base + t] = bl;
+            slot_boxh[ri.slot_base + t] = bh;
             beta.push_back(rh.beta[t]);
             displ.push_back(rh.d.max_displ_atom);
             displ.push_back(rh.d.max_displ_vol);
-            // max_length(CubicBC) = 1.8 L, src/MCStates.jl:71-73
-            maxbox.push_back(h->cfg.boundary == BC_CUBIC ? rh.d.box_length * 1.8 : 30.0);
+            // max_length = 1.8 L, max_height = 1.8 H, lh_ratio = max_length / max_height
+            // (src/MCStates.jl:63-86, :106; the cubic box's height is its length)
+            maxbox.push_back(pbc ? bl * 1.8 : 30.0);
+            maxboxh.push_back(pbc ? bh * 1.8 : 30.0);
+            lh.push_back(maxbox.back() / maxboxh.back());
+            displ_sep.push_back(rh.d.max_displ_vol);  // max_displ[3] (xy) and [4] (z): the reference default 0.1 each
+            displ_sep.push_back(rh.d.max_displ_vol);
         }
         tb += rh.d.n_temps;
     }
@@ -1002,11 +1048,17 @@ int ptmc_prepare(ptmc_handle h)
     if ((rc = upload(h, h->d_slot_temp, slot_temp))) return rc;
     if ((rc = upload(h, h->d_slot_en, slot_en))) return rc;
     if ((rc = upload(h, h->d_slot_box, slot_box))) return rc;
+    if ((rc = upload(h, h->d_slot_boxh, slot_boxh))) return rc;
     if ((rc = dalloc(h, h->d_slot_drift, h->n_slots))) return rc;
     if ((rc = dalloc(h, h->d_run_error, h->runs.size()))) return rc;
     if ((rc = upload(h, h->d_beta, beta))) return rc;
     if ((rc = upload(h, h->d_displ, displ))) return rc;
     if ((rc = upload(h, h->d_maxbox, maxbox))) return rc;
+    if ((rc = upload(h, h->d_maxboxh, maxboxh))) return rc;
+    if ((rc = upload(h, h->d_lh, lh))) return rc;
+    if ((rc = upload(h, h->d_displ_sep, displ_sep))) return rc;
+    if ((rc = dalloc(h, h->d_count_volxy, (size_t)2 * tb))) return rc;
+    if ((rc = dalloc(h, h->d_count_volz, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_ham, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_count_atom, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_count_vol, (size_t)2 * tb))) return rc;
@@ -1173,6 +1225,10 @@ int ptmc_reset_counters(ptmc_handle h)
     HIPCHK(h, hipSetDevice(h->cfg.device));
     HIPCHK(h, hipMemsetAsync(h->d_count_atom.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_count_vol.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+    if (h->d_count_volxy.p) {
+        HIPCHK(h, hipMemsetAsync(h->d_count_volxy.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+        HIPCHK(h, hipMemsetAsync(h->d_count_volz.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
+    }
     HIPCHK(h, hipMemsetAsync(h->d_count_exc.p, 0, sizeof(long long) * 2 * h->n_temps, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return PTMC_OK;
@@ -1208,11 +1264,20 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
     if ((rc = fetch_slots(h, run, temp, e, b))) return rc;
     const RunHost &rh = h->runs[run];
     const int R = rh.d.n_temps, N = rh.d.n_atoms;
+    const bool has_h = h->cfg.boundary == BC_RECT || h->cfg.boundary == BC_RHOMB;
+    std::vector<double> bh(R, 0.0);
+    if (box && has_h)
+        HIPCHK(h, hipMemcpy(bh.data(), h->d_slot_boxh.p + rh.info.slot_base, R * sizeof(double), hipMemcpyDeviceToHost));
     for (int s = 0; s < R; ++s) {
         const int t = temp[s];
         if (t < 0 || t >= R) return fail(h, PTMC_E_DEVICE, "corrupt temperature label");
         if (en) en[t] = e[s];
-        if (box) box[t] = b[s];
+        if (box && has_h) {
+            box[2 * t] = b[s];
+            box[2 * t + 1] = bh[s];
+        } else if (box) {
+            box[t] = b[s];
+        }
     }
     if (x && is_nnp(h)) {
         std::vector<float> blk((size_t)R * 3 * NNP_NP);
@@ -1264,6 +1329,36 @@ int ptmc_get_counters(ptmc_handle h, int32_t run, double *max_displ, int64_t *co
     if (count_vol) HIPCHK(h, hipMemcpy(count_vol, h->d_count_vol.p + off, n * 8, hipMemcpyDeviceToHost));
     if (count_exc) HIPCHK(h, hipMemcpy(count_exc, h->d_count_exc.p + off, n * 8, hipMemcpyDeviceToHost));
     if (ham) HIPCHK(h, hipMemcpy(ham, h->d_ham.p + off, n * 8, hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_get_counters_sep(ptmc_handle h, int32_t run, double *max_displ_sep, int64_t *count_vol_xy,
+                          int64_t *count_vol_z)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t off = (size_t)2 * rh.info.temp_base, n = (size_t)2 * rh.d.n_temps;
+    if (max_displ_sep) HIPCHK(h, hipMemcpy(max_displ_sep, h->d_displ_sep.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (count_vol_xy) HIPCHK(h, hipMemcpy(count_vol_xy, h->d_count_volxy.p + off, n * 8, hipMemcpyDeviceToHost));
+    if (count_vol_z) HIPCHK(h, hipMemcpy(count_vol_z, h->d_count_volz.p + off, n * 8, hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_set_counters_sep(ptmc_handle h, int32_t run, const double *max_displ_sep, const int64_t *count_vol_xy,
+                          const int64_t *count_vol_z)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t off = (size_t)2 * rh.info.temp_base, n = (size_t)2 * rh.d.n_temps;
+    if (max_displ_sep) HIPCHK(h, hipMemcpy(h->d_displ_sep.p + off, max_displ_sep, n * 8, hipMemcpyHostToDevice));
+    if (count_vol_xy) HIPCHK(h, hipMemcpy(h->d_count_volxy.p + off, count_vol_xy, n * 8, hipMemcpyHostToDevice));
+    if (count_vol_z) HIPCHK(h, hipMemcpy(h->d_count_volz.p + off, count_vol_z, n * 8, hipMemcpyHostToDevice));
     return PTMC_OK;
 }
 
@@ -1349,7 +1444,20 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
         std::vector<int32_t> temp(R);
         for (int s = 0; s < R; ++s) temp[s] = s;
         HIPCHK(h, hipMemcpy(h->d_slot_temp.p + rh.info.slot_base, temp.data(), R * 4, hipMemcpyHostToDevice));
-        if (box) HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, box, R * 8, hipMemcpyHostToDevice));
+        if (box) {
+            // lengths, or (length, height) pairs for rectangular and rhombic boxes;
+            // a cubic box's height is its length
+            const bool has_h = h->cfg.boundary == BC_RECT || h->cfg.boundary == BC_RHOMB;
+            std::vector<double> bl(R), bh(R);
+            for (int t = 0; t < R; ++t) {
+                bl[t] = has_h ? box[2 * t] : box[t];
+                bh[t] = has_h ? box[2 * t + 1] : box[t];
+                if (h->cfg.boundary != BC_SPHERE && !(bl[t] > 0 && bh[t] > 0))
+                    return fail(h, PTMC_E_ARG, "box dimensions must be positive");
+            }
+            HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, bl.data(), R * 8, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->d_slot_boxh.p + rh.info.slot_base, bh.data(), R * 8, hipMemcpyHostToDevice));
+        }
         if (is_nnp(h)) {
             std::vector<int32_t> ts(R);
             for (int s = 0; s < R; ++s) ts[s] = rh.info.slot_base + s;

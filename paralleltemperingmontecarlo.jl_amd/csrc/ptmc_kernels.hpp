@@ -27,6 +27,11 @@ KernelPair lookup_f64_cubic_npt(int pk, int kc, int kl, int lanes);
 KernelPair lookup_f32_sphere_nvt(int pk, int kc, int kl, int lanes);
 KernelPair lookup_f32_cubic_nvt(int pk, int kc, int kl, int lanes);
 KernelPair lookup_f32_cubic_npt(int pk, int kc, int kl, int lanes);
+// rhombic prism (rectangular boxes use the cubic kernels)
+KernelPair lookup_f64_rhomb_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f64_rhomb_npt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f32_rhomb_nvt(int pk, int kc, int kl, int lanes);
+KernelPair lookup_f32_rhomb_npt(int pk, int kc, int kl, int lanes);
 // Sutton-Chen EAM, NVT, 256-lane workgroups
 KernelPair lookup_eam(int precision, int bc, int kl, int lanes);
 
@@ -54,7 +59,7 @@ KernelPair kl_switch(int kl, int lanes)
         default: return {};
         }
     }
-    if constexpr (sizeof(Real) == 4 && BC == BC_CUBIC) {
+    if constexpr (sizeof(Real) == 4 && (BC == BC_CUBIC || BC == BC_RHOMB)) {
         if (lanes == 768) {
             switch (kl) {
             case 8: return PTMC_PAIR(8, 768);

@@ -177,6 +177,118 @@ __device__ __forceinline__ double lrc64(int n_atoms, double r_cut, const double 
     return e;
 }
 
+// ---- periodic boxes: cubic (L), rectangular (L, L, H) and the rhombic prism
+// (60 degree rhombus of side L, height H).  Minimum image
+// src/Configurations.jl:101-126, trial wrap src/MCMoves.jl:51-98, cut-off
+// src/Ensembles.jl:118-155, volume src/Configurations.jl:246-256, tail
+// correction src/EnergyEvaluation.jl:420-474.
+constexpr double SQRT3 = 1.7320508075688772;  // 3^0.5
+constexpr double INV_SQRT3 = 1.0 / 1.7320508075688772;
+
+// Box constants in one precision, each rounded from its FP64 value.
+template <typename T>
+struct BoxC {
+    T L, iL, H, iH;  // length, height and reciprocals
+    T Ly, iLy;       // rhombic row spacing of the distance rule, (3^0.5 / 2) L
+    T W, iW, qo;     // rhombic wrap: L 3^0.5 / 2 and L 3^0.5 / 4
+    T hL, hH;        // L / 2, H / 2
+};
+
+template <typename T>
+__device__ __forceinline__ BoxC<T> make_box(double l, double h)
+{
+    BoxC<T> b;
+    const double ly = SQRT3 / 2.0 * l, w = l * SQRT3 / 2.0;
+    b.L = (T)l;
+    b.iL = (T)(1.0 / l);
+    b.H = (T)h;
+    b.iH = (T)(1.0 / h);
+    b.Ly = (T)ly;
+    b.iLy = (T)(1.0 / ly);
+    b.W = (T)w;
+    b.iW = (T)(1.0 / w);
+    b.qo = (T)(l * SQRT3 / 4.0);
+    b.hL = (T)(l / 2.0);
+    b.hH = (T)(h / 2.0);
+    return b;
+}
+
+// Squared distance from a to the image of b nearest it.  FP64 moves b as the
+// reference does (DIV: its divisions; else multiplications by the box
+// reciprocals); binary32 folds the image into the difference.  The rhombic
+// rule is the reference's: y by the row spacing, then x along the row after
+// removing the skew of the y difference, z by the height.
+template <typename T, int BC, bool DIV>
+__device__ __forceinline__ T box_d2(T ax, T ay, T az, T bx, T by, T bz, const BoxC<T> &b)
+{
+    if constexpr (sizeof(T) == 8) {
+        if constexpr (BC == BC_CUBIC) {
+            bx = bx + b.L * __builtin_rint(DIV ? (ax - bx) / b.L : (ax - bx) * b.iL);
+            by = by + b.L * __builtin_rint(DIV ? (ay - by) / b.L : (ay - by) * b.iL);
+            bz = bz + b.H * __builtin_rint(DIV ? (az - bz) / b.H : (az - bz) * b.iH);
+        } else if constexpr (BC == BC_RHOMB) {
+            const T by2 = by + b.Ly * __builtin_rint(DIV ? (ay - by) / b.Ly : (ay - by) * b.iLy);
+            const T sk = (ax - bx) - INV_SQRT3 * (ay - by);
+            const T kx = __builtin_rint(DIV ? sk / b.L : sk * b.iL);
+            bx = (DIV ? bx - by / SQRT3 : bx - by * INV_SQRT3) + b.L * kx + INV_SQRT3 * by2;
+            by = by2;
+            bz = bz + b.H * __builtin_rint(DIV ? (az - bz) / b.H : (az - bz) * b.iH);
+        }
+        const T dx = ax - bx, dy = ay - by, dz = az - bz;
+        return dx * dx + dy * dy + dz * dz;
+    } else {
+        T dx = ax - bx, dy = ay - by, dz = az - bz;
+        if constexpr (BC == BC_CUBIC) {
+            dx = __builtin_fmaf(-b.L, __builtin_rintf(dx * b.iL), dx);
+            dy = __builtin_fmaf(-b.L, __builtin_rintf(dy * b.iL), dy);
+            dz = __builtin_fmaf(-b.H, __builtin_rintf(dz * b.iH), dz);
+        } else if constexpr (BC == BC_RHOMB) {
+            const float k2 = __builtin_rintf(dy * b.iLy);
+            const float k1 = __builtin_rintf(__builtin_fmaf(-(float)INV_SQRT3, dy, dx) * b.iL);
+            dx = __builtin_fmaf(-b.hL, k2, __builtin_fmaf(-b.L, k1, dx));
+            dy = __builtin_fmaf(-b.Ly, k2, dy);
+            dz = __builtin_fmaf(-b.H, __builtin_rintf(dz * b.iH), dz);
+        }
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    }
+}
+
+// r_cut^2 of a box (src/Ensembles.jl:118-155); kind BC_CUBIC, BC_RECT or BC_RHOMB
+__device__ __forceinline__ double box_rcut(int kind, double l, double h)
+{
+    if (kind == BC_RECT) return __builtin_fmin(l * l / 4.0, h * h / 4.0);
+    if (kind == BC_RHOMB) return __builtin_fmin(l * l * 3.0 / 16.0, h * h / 4.0);
+    return l * l / 4.0;
+}
+
+// get_volume (src/Configurations.jl:246-256)
+__device__ __forceinline__ double box_volume(int kind, double l, double h)
+{
+    if (kind == BC_RECT) return l * l * h;
+    if (kind == BC_RHOMB) return l * l * h * SQRT3 / 2.0;
+    return l * l * l;
+}
+
+// the volume the NPT histograms bin (find_hist_index, src/Sampling.jl:72-125;
+// the rhombic form there evaluates L^2 * 3 H^0.5 / 2)
+__device__ __forceinline__ double box_hist_volume(int kind, double l, double h)
+{
+    if (kind == BC_RECT) return l * l * h;
+    if (kind == BC_RHOMB) return l * l * (3.0 * __builtin_sqrt(h)) / 2.0;
+    return l * l * l;
+}
+
+// tail correction of a box (src/EnergyEvaluation.jl:420, :447-448, :469-473)
+template <int KC>
+__device__ __forceinline__ double box_lrc(int kind, int n_atoms, double r_cut, double l, double h,
+                                          const double (&c)[KC])
+{
+    const double e = lrc64<KC>(n_atoms, r_cut, c);
+    if (kind == BC_RHOMB) return e * 3.0 / 4.0 * l / h;
+    if (kind == BC_RECT) return l < h ? e * l / h : e * (h * h) / (l * l);
+    return e;
+}
+
 template <typename Real>
 struct MoveDraw {
     uint32_t index;
@@ -250,6 +362,52 @@ __device__ __forceinline__ void make_trial(Real xi, Real yi, Real zi, const Move
             ty = ty - L * q;
             q = __builtin_rintf(tz * invL);
             tz = tz - L * q;
+        }
+    }
+}
+
+// Trial position in a periodic box (src/MCMoves.jl:51-98): displacement,
+// then the wrap back into the box; binary32 repeats the FP64 steps with the
+// box constants rounded to binary32.
+template <typename Real, int BC, bool DIV>
+__device__ __forceinline__ void box_trial(Real xi, Real yi, Real zi, const MoveDraw<Real> &d, Real delta,
+                                          const BoxC<Real> &b, Real &tx, Real &ty, Real &tz)
+{
+    const Real half = (Real)0.5;
+    tx = xi + (d.ux - half) * delta;
+    ty = yi + (d.uy - half) * delta;
+    tz = zi + (d.uz - half) * delta;
+    if constexpr (sizeof(Real) == 8) {
+        if constexpr (BC == BC_CUBIC) {
+            tx = tx - b.L * __builtin_rint(DIV ? tx / b.L : tx * b.iL);
+            ty = ty - b.L * __builtin_rint(DIV ? ty / b.L : ty * b.iL);
+            tz = tz - b.H * __builtin_rint(DIV ? tz / b.H : tz * b.iH);
+        } else if constexpr (BC == BC_RHOMB) {
+            const Real k2 = __builtin_rint(DIV ? (ty - b.qo) / b.W : (ty - b.qo) * b.iW);
+            const Real k1 = __builtin_rint(DIV ? (tx - ty / SQRT3 - b.hL) / b.L : (tx - ty * INV_SQRT3 - b.hL) * b.iL);
+            const Real k3 = __builtin_rint(DIV ? (tz - b.hH) / b.H : (tz - b.hH) * b.iH);
+            const Real d0 = b.L * k1 + b.hL * k2;
+            tx = tx - d0;
+            ty = ty - b.W * k2;
+            tz = tz - b.H * k3;
+        }
+    } else {
+        if constexpr (BC == BC_CUBIC) {
+            float q;
+            q = __builtin_rintf(tx * b.iL);
+            tx = tx - b.L * q;
+            q = __builtin_rintf(ty * b.iL);
+            ty = ty - b.L * q;
+            q = __builtin_rintf(tz * b.iH);
+            tz = tz - b.H * q;
+        } else if constexpr (BC == BC_RHOMB) {
+            const float k2 = __builtin_rintf((ty - b.qo) * b.iW);
+            const float k1 = __builtin_rintf((tx - ty * (float)INV_SQRT3 - b.hL) * b.iL);
+            const float k3 = __builtin_rintf((tz - b.hH) * b.iH);
+            const float d0 = b.L * k1 + b.hL * k2;
+            tx = tx - d0;
+            ty = ty - b.W * k2;
+            tz = tz - b.H * k3;
         }
     }
 }
@@ -335,29 +493,34 @@ __device__ bool sphere_redraw(Real *X, int tid, int q, int N, int A, int si, int
     return ok;
 }
 
-// Full FP64 energy of a slot's configuration (src/EnergyEvaluation.jl:378-420):
+// FP64 pair-energy sum of a slot's configuration (src/EnergyEvaluation.jl:378-474):
 // rows j of this lane, partners l > j, summed by the cluster butterfly.
+// Positions enter scaled by (px, px, pz) and the squared distances, taken in
+// box bx, are multiplied by m2: (1, 1, 1) for the configuration as it is,
+// (1, 1, s^2) for a uniformly scaled trial (the reference scales its r^2
+// matrix), (sx, sz, 1) with the trial box for a separated volume move (the
+// reference recomputes r^2 from the scaled positions).  NPT keeps pairs with
+// r^2 <= rc; the caller adds the tail correction.
 template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
-__device__ double full_energy64(const Real *X, int gb, int q, int N, double box, double s2, double rc,
-                                const double (&c64)[KC])
+__device__ double pair_sum64(const Real *X, int gb, int q, int N, const BoxC<double> &bx, double px, double pz,
+                             double m2, double rc, const double (&c64)[KC])
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
-    const double invL = BC == BC_CUBIC ? 1.0 / box : 0.0;
     const int A = (N + KL - 1) / KL;
     double part = 0.0;
     for (int s = 0; s < A; ++s) {
         const int j = s * KL + q;
         if (j >= N) break;
-        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)];
-        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)];
-        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)];
+        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)] * px;
+        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)] * px;
+        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)] * pz;
         for (int l = j + 1; l < N; ++l) {
             const int sl = l / KL, ll = gb + (l % KL);
-            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
-            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
-            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
-            double r2 = dist2_64(xj, yj, zj, xl, yl, zl, BC, box, invL, DIV);
-            r2 = r2 * s2;
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
+            double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx);
+            r2 = r2 * m2;
             if (ENS == ENS_NPT) {
                 if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
             } else {
@@ -365,9 +528,7 @@ __device__ double full_energy64(const Real *X, int gb, int q, int N, double box,
             }
         }
     }
-    part = group_sum<KL>(part);
-    if (ENS == ENS_NPT) part += lrc64<KC>(N, rc, c64);
-    return part;
+    return group_sum<KL>(part);
 }
 
 __device__ __forceinline__ double readlane_d(double v, int src)
@@ -378,21 +539,20 @@ __device__ __forceinline__ double readlane_d(double v, int src)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// FP64 energy of the cubic-box configuration held by the cluster whose first
-// lane is wave lane `cl`, at squared scale s2 and cutoff rc (the volume-move
-// energy of src/EnergyEvaluation.jl:397-420), computed by every executing
-// lane of the wave together.  The N(N-1)/2 pairs are enumerated as
-// (j, j + d mod N) for d = 1..N/2 (at d = N/2 of an even N only j < N/2), and
-// dealt round-robin over the `live` lanes; cluster butterflies, then the
-// cluster totals of the `leaders` (first lanes) in lane order.  Every lane
-// returns the same value.  Pairs and cutoff test are those of full_energy64;
-// only the summation order differs.
-template <int PK, int KC, int KL, int LANES, typename Real>
-__device__ double wave_cubic_energy64(const Real *X, int cl, int N, double box, double s2, double rc,
-                                      const double (&c64)[KC], uint64_t live, uint64_t leaders)
+// The pair sum of pair_sum64 (NPT form, cut-off rc) for the configuration
+// held by the cluster whose first lane is wave lane `cl`, computed by every
+// executing lane of the wave together (volume-move energies of the binary32
+// path).  The N(N-1)/2 pairs are enumerated as (j, j + d mod N) for
+// d = 1..N/2 (at d = N/2 of an even N only j < N/2) and dealt round-robin
+// over the `live` lanes; cluster butterflies, then the cluster totals of the
+// `leaders` (first lanes) in lane order.  Every lane returns the same value.
+// Pairs and cut-off test are those of pair_sum64; only the summation order
+// differs.
+template <int PK, int KC, int BC, int KL, int LANES, typename Real>
+__device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<double> &bx, double px, double pz,
+                                  double m2, double rc, const double (&c64)[KC], uint64_t live, uint64_t leaders)
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
-    const double invL = 1.0 / box;
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
     const int nl = __popcll(live);
@@ -407,13 +567,13 @@ __device__ double wave_cubic_energy64(const Real *X, int cl, int N, double box, 
         if (!(even && d == h && j >= h)) {
             const int sj = j / KL, lj = cb + (j % KL);
             const int sl = l / KL, ll = cb + (l % KL);
-            const double xj = (double)X[xoff<LANES>(sj, 0, lj)];
-            const double yj = (double)X[xoff<LANES>(sj, 1, lj)];
-            const double zj = (double)X[xoff<LANES>(sj, 2, lj)];
-            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
-            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
-            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
-            const double r2 = dist2_64(xj, yj, zj, xl, yl, zl, BC_CUBIC, box, invL, DIV) * s2;
+            const double xj = (double)X[xoff<LANES>(sj, 0, lj)] * px;
+            const double yj = (double)X[xoff<LANES>(sj, 1, lj)] * px;
+            const double zj = (double)X[xoff<LANES>(sj, 2, lj)] * pz;
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
+            const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
             if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
         }
         j += nl;
@@ -425,7 +585,7 @@ __device__ double wave_cubic_energy64(const Real *X, int cl, int N, double box, 
     part = group_sum<KL>(part);
     double tot = 0.0;
     for (uint64_t b = leaders; b != 0; b &= b - 1) tot += readlane_d(part, __builtin_ctzll(b));
-    return tot + lrc64<KC>(N, rc, c64);
+    return tot;
 }
 
 // Philox words of move mm for this lane (block A; block B only in FP64).
@@ -495,7 +655,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
     const int slot = tile.slot0 + lg;
     int t = 0, R = 1, tb = 0, lb = 0;
     uint32_t run_key = 0;
-    double E = 0.0, box = 1.0;
+    constexpr bool PBC = BC != BC_SPHERE;
+    // box kind of the geometry formulas: the cubic kernels serve cubic and rectangular boxes
+    const int kind = BC == BC_RHOMB ? (int)BC_RHOMB : (BC == BC_CUBIC ? a.box_kind : (int)BC_SPHERE);
+    double E = 0.0, box = 1.0, boxh = 1.0;
     Real radius2 = 0;  // container radius^2 in the move precision
     if (active) {
         const RunInfo &ri = a.runs[a.slot_run[slot]];
@@ -506,7 +669,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         radius2 = (Real)ri.radius2;
         t = a.slot_temp[slot];
         E = a.slot_en[slot];
-        if constexpr (BC == BC_CUBIC) box = a.slot_box[slot];
+        if constexpr (PBC) {
+            box = a.slot_box[slot];
+            boxh = a.slot_boxh[slot];
+        }
         if (q == 0) {
             exT[lg] = t;
             exS[lb + t] = lg;
@@ -522,10 +688,13 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             const double delta = a.t_displ[2 * (tb + t)];
             const Real delta_r = (Real)delta;
             const float beta_l2e = (float)beta * 1.4426950408889634f;
-            Real Lr = (Real)box;
-            Real invLr = (Real)(BC == BC_CUBIC ? 1.0 / box : 0.0);
-            Real rc = (Real)(box * box / 4.0);
-            int nacc = 0, nvacc = 0;
+            BoxC<Real> bxr{};
+            Real rc = 0;
+            if constexpr (PBC) {
+                bxr = make_box<Real>(box, boxh);
+                rc = (Real)box_rcut(kind, box, boxh);
+            }
+            int nacc = 0, nvacc = 0, nvxy = 0, nvz = 0;
             // Shared Philox words: lane q of the cluster computes the block(s)
             // of move m0 + q; the words of the next move are fetched while
             // the current move runs.
@@ -559,7 +728,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     Real yi = X[xoff<LANES>(si, 1, li)];
                     Real zi = X[xoff<LANES>(si, 2, li)];
                     Real tx, ty, tz;
-                    make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Lr, invLr, tx, ty, tz);
+                    if constexpr (PBC) box_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, bxr, tx, ty, tz);
+                    else make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
                     if constexpr (BC == BC_SPHERE) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
                         if (outside_sphere<Real>(tx, ty, tz, radius2)) {
@@ -585,8 +755,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         const Real xj = Xl[xoff<LANES>(s, 0, 0)];
                         const Real yj = Xl[xoff<LANES>(s, 1, 0)];
                         const Real zj = Xl[xoff<LANES>(s, 2, 0)];
-                        const Real r2n = dist2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, Lr, invLr);
-                        const Real r2o = dist2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, Lr, invLr);
+                        const Real r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
+                        const Real r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
                         Real en, eo;
                         if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
                             // one reciprocal serves the new and the old pair
@@ -651,20 +821,55 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 }
                 if constexpr (ENS == ENS_NPT) {
-                    // uniform volume move, src/MCMoves.jl:169-177, :277-287; energy
-                    // src/EnergyEvaluation.jl:397-420; acceptance src/Exchange.jl:58-71
+                    // volume move: uniform (src/MCMoves.jl:169-177, :277-287) or separated,
+                    // z 1/6, xy 1/3, xyz 1/2 of the moves (src/MCMoves.jl:235-343); energy
+                    // src/EnergyEvaluation.jl:397-474; acceptance src/Exchange.jl:58-71
                     const bool vm = i >= N;
-                    double scale = 1.0, nbox = box, nrc = 0.0, s2 = 1.0;
+                    int vkind = 0;  // 0: xyz, 1: xy, 2: z
+                    double sx = 1.0, sz = 1.0;
                     if (vm) {
-                        const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * (tb + t) + 1];
-                        scale = det_exp(y / 3.0);
-                        if (box >= a.t_maxbox[tb + t] && scale > 1.0) scale = 1.0;
-                        nbox = box * scale;
-                        nrc = nbox * nbox / 4.0;
-                        s2 = scale * scale;
+                        const int T = tb + t;
+                        if (!a.separated) {
+                            const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * T + 1];
+                            double scale = det_exp(y / 3.0);
+                            if (box >= a.t_maxbox[T] && scale > 1.0) scale = 1.0;
+                            sx = sz = scale;
+                        } else {
+                            const int ra = 1 + (int)((double)d0.uy * 6.0);
+                            const double lh = box / boxh, lr = a.t_lh[T];
+                            if (ra == 1) {
+                                vkind = 2;
+                                const double y = ((double)d0.ux - 0.5) * a.t_displ_sep[2 * T + 1];
+                                double scale = det_exp(y);
+                                if (lh <= lr * 1.1 && scale > 1.0) scale = 1.0 / scale;
+                                else if (lh >= lr * 0.909 && scale < 1.0) scale = 1.0 / scale;
+                                if (boxh >= a.t_maxboxh[T] && scale > 1.0) scale = 1.0 / scale;
+                                sz = scale;
+                            } else if (ra <= 3) {
+                                vkind = 1;
+                                const double y = ((double)d0.ux - 0.5) * a.t_displ_sep[2 * T];
+                                double scale = det_exp(y / 2.0);
+                                if (lh >= lr * 1.1 && scale > 1.0) scale = 1.0 / scale;
+                                else if (lh <= lr * 0.909 && scale < 1.0) scale = 1.0 / scale;
+                                if (box >= a.t_maxbox[T] && scale > 1.0) scale = 1.0 / scale;
+                                sx = scale;
+                            } else {
+                                const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * T + 1];
+                                double scale = det_exp(y / 3.0);
+                                if (box >= a.t_maxbox[T] && scale > 1.0) scale = 1.0;
+                                sx = sz = scale;
+                            }
+                        }
                     }
+                    const double nbox = box * sx, nh = boxh * sz;
+                    const double nrc = box_rcut(kind, nbox, nh);
+                    // uniform: the current positions in the current box, r^2 scaled by s^2;
+                    // separated: the scaled positions in the trial box
+                    const bool sep = a.separated != 0;
+                    const double px = sep ? sx : 1.0, pz = sep ? sz : 1.0, m2 = sep ? 1.0 : sx * sx;
+                    const double eb = sep ? nbox : box, ebh = sep ? nh : boxh;
                     double new_en = 0.0;
-                    if constexpr (F32 && BC == BC_CUBIC) {
+                    if constexpr (F32) {
                         // the clusters of a wave evaluate each pending volume move
                         // together instead of leaving the wave to one cluster's
                         // O(N^2) sum, once per cluster
@@ -675,19 +880,22 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         while (todo != 0) {
                             const int cl = __builtin_ctzll(todo);
                             todo &= todo - 1;
-                            const double e = wave_cubic_energy64<PK, KC, KL, LANES>(
-                                X, cl, N, readlane_d(box, cl), readlane_d(s2, cl), readlane_d(nrc, cl), c64, live,
-                                leaders);
+                            const double e = wave_pair_sum64<PK, KC, BC, KL, LANES>(
+                                X, cl, N, make_box<double>(readlane_d(eb, cl), readlane_d(ebh, cl)), readlane_d(px, cl),
+                                readlane_d(pz, cl), readlane_d(m2, cl), readlane_d(nrc, cl), c64, live, leaders);
                             if ((gb & 63) == cl) new_en = e;
                         }
                     } else {
-                        if (vm) new_en = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, s2, nrc, c64);
+                        if (vm)
+                            new_en = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(eb, ebh),
+                                                                                  px, pz, m2, nrc, c64);
                     }
                     if (vm) {
+                        new_en = new_en + box_lrc<KC>(kind, N, nrc, nbox, nh, c64);
                         const double pressure = a.runs[a.slot_run[slot]].pressure;
                         const double dd = new_en - E;
                         dE_used = dd;
-                        const double vn = nbox * nbox * nbox, vo = box * box * box;
+                        const double vn = box_volume(kind, nbox, nh), vo = box_volume(kind, box, boxh);
                         const double dh = dd + pressure * (vn - vo);
                         const double arg = -dh * beta + (double)N * det_log(vn / vo);
                         double prob = det_exp(arg);
@@ -695,17 +903,20 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         accept = prob >= (double)d0.umet;
                         if (accept) {
                             for (int s = 0; s < A; ++s) {
-                                Real *px = &X[xoff<LANES>(s, 0, tid)];
-                                Real *py = &X[xoff<LANES>(s, 1, tid)];
-                                Real *pz = &X[xoff<LANES>(s, 2, tid)];
-                                *px = (Real)((double)*px * scale);
-                                *py = (Real)((double)*py * scale);
-                                *pz = (Real)((double)*pz * scale);
+                                Real *pxp = &X[xoff<LANES>(s, 0, tid)];
+                                Real *pyp = &X[xoff<LANES>(s, 1, tid)];
+                                Real *pzp = &X[xoff<LANES>(s, 2, tid)];
+                                *pxp = (Real)((double)*pxp * sx);
+                                *pyp = (Real)((double)*pyp * sx);
+                                *pzp = (Real)((double)*pzp * sz);
                             }
                             __atomic_signal_fence(__ATOMIC_SEQ_CST);
                             box = nbox;
+                            boxh = nh;
                             E = new_en;
-                            nvacc += 1;
+                            if (vkind == 0) nvacc += 1;
+                            else if (vkind == 1) nvxy += 1;
+                            else nvz += 1;
                         }
                     }
                 }
@@ -717,9 +928,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                 }
                 if constexpr (ENS == ENS_NPT) {
                     // an accepted volume move changes the box for the remaining moves
-                    rc = (Real)(box * box / 4.0);
-                    Lr = (Real)box;
-                    invLr = (Real)(1.0 / box);
+                    if (i >= N) {
+                        bxr = make_box<Real>(box, boxh);
+                        rc = (Real)box_rcut(kind, box, boxh);
+                    }
                 }
             }
             if (q == 0) {
@@ -728,9 +940,15 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                 if (ENS == ENS_NPT) {
                     a.t_count_vol[2 * (tb + t)] += nvacc;
                     a.t_count_vol[2 * (tb + t) + 1] += nvacc;
+                    if (a.separated) {
+                        a.t_count_volxy[2 * (tb + t)] += nvxy;
+                        a.t_count_volxy[2 * (tb + t) + 1] += nvxy;
+                        a.t_count_volz[2 * (tb + t)] += nvz;
+                        a.t_count_volz[2 * (tb + t) + 1] += nvz;
+                    }
                 }
                 exE[lg] = E;
-                if constexpr (ENS == ENS_NPT) exV[lg] = box * box * box;
+                if constexpr (ENS == ENS_NPT) exV[lg] = box_volume(kind, box, boxh);
             }
         }
         __syncthreads();
@@ -783,23 +1001,37 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
                     else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
                     a.t_count_atom[2 * T + 1] = 0;
-                    if (ENS == ENS_NPT) {
+                    if (ENS == ENS_NPT && !a.separated) {
                         const double av = (double)a.t_count_vol[2 * T + 1] / (double)a.n_adjust;
                         if (av < a.min_acc) a.t_displ[2 * T + 1] *= 0.9;
                         else if (av > a.max_acc) a.t_displ[2 * T + 1] *= 1.1;
                         a.t_count_vol[2 * T + 1] = 0;
+                    } else if (ENS == ENS_NPT) {
+                        // separated moves: expected shares 1/2 (xyz), 1/3 (xy), 1/6 (z)
+                        double av = (double)a.t_count_vol[2 * T + 1] / ((double)a.n_adjust / 2.0);
+                        if (av < a.min_acc) a.t_displ[2 * T + 1] *= 0.9;
+                        else if (av > a.max_acc) a.t_displ[2 * T + 1] *= 1.1;
+                        a.t_count_vol[2 * T + 1] = 0;
+                        av = (double)a.t_count_volxy[2 * T + 1] / ((double)a.n_adjust / 3.0);
+                        if (av < a.min_acc) a.t_displ_sep[2 * T] *= 0.9;
+                        else if (av > a.max_acc) a.t_displ_sep[2 * T] *= 1.1;
+                        a.t_count_volxy[2 * T + 1] = 0;
+                        av = (double)a.t_count_volz[2 * T + 1] / ((double)a.n_adjust / 6.0);
+                        if (av < a.min_acc) a.t_displ_sep[2 * T + 1] *= 0.9;
+                        else if (av > a.max_acc) a.t_displ_sep[2 * T + 1] *= 1.1;
+                        a.t_count_volz[2 * T + 1] = 0;
                     }
                 }
                 // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
                 if (a.sampling && pidx % a.mc_sample == 0) {
                     const RunInfo &ri = a.runs[a.slot_run[slot]];
                     if (ENS == ENS_NPT) {
-                        const double v = box * box * box;
+                        const double v = box_volume(kind, box, boxh);
                         const double h = E + ri.pressure * v;
                         a.t_ham[2 * T] += h;
                         a.t_ham[2 * T + 1] += h * h;
                         const double ie = __builtin_floor((E - ri.en_min) / ri.delta_en) + 1.0;
-                        const double iv = __builtin_floor((v - ri.v_min) / ri.delta_v) + 1.0;
+                        const double iv = __builtin_floor((box_hist_volume(kind, box, boxh) - ri.v_min) / ri.delta_v) + 1.0;
                         const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
                         const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
                         a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)] += 1u;
@@ -814,8 +1046,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             }
             // periodic FP64 recompute bounding binary32 drift
             if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
-                const double ef =
-                    full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, box, 1.0, box * box / 4.0, c64);
+                const double rc64 = box_rcut(kind, box, boxh);
+                double ef = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(box, boxh), 1.0,
+                                                                         1.0, 1.0, rc64, c64);
+                if constexpr (ENS == ENS_NPT) ef = ef + box_lrc<KC>(kind, N, rc64, box, boxh, c64);
                 const double dr = __builtin_fabs(E - ef);
                 if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
                 E = ef;
@@ -828,7 +1062,10 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
     if (active && q == 0) {
         a.slot_temp[slot] = t;
         a.slot_en[slot] = E;
-        if constexpr (BC == BC_CUBIC) a.slot_box[slot] = box;
+        if constexpr (PBC) {
+            a.slot_box[slot] = box;
+            a.slot_boxh[slot] = boxh;
+        }
     }
 }
 
@@ -855,8 +1092,13 @@ __global__ void __launch_bounds__(LANES) init_kernel(SweepArgs a)
     double c64[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
-    const double box = BC == BC_CUBIC ? a.slot_box[slot] : 1.0;
-    const double E = full_energy64<Real, PK, KC, BC, ENS, KL, LANES>(X, lg * KL, q, N, box, 1.0, box * box / 4.0, c64);
+    const bool pbc = BC != BC_SPHERE;
+    const int kind = BC == BC_RHOMB ? (int)BC_RHOMB : (BC == BC_CUBIC ? a.box_kind : (int)BC_SPHERE);
+    const double box = pbc ? a.slot_box[slot] : 1.0, boxh = pbc ? a.slot_boxh[slot] : 1.0;
+    const double rc64 = pbc ? box_rcut(kind, box, boxh) : 0.0;
+    double E = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, lg * KL, q, N, make_box<double>(box, boxh), 1.0, 1.0,
+                                                            1.0, rc64, c64);
+    if constexpr (ENS == ENS_NPT) E = E + box_lrc<KC>(kind, N, rc64, box, boxh, c64);
     if (q == 0) a.slot_en[slot] = E;
 }
 

@@ -5,7 +5,10 @@
 namespace ptmc {
 
 enum Ensemble : int32_t { ENS_NVT = 0, ENS_NPT = 1 };
-enum Boundary : int32_t { BC_SPHERE = 0, BC_CUBIC = 1 };
+// Boundary conditions (src/BoundaryConditions.jl:44-116).  Rectangular boxes
+// (L, L, H) run the cubic kernels with the box kind set at run time; the
+// rhombic prism has kernels of its own.
+enum Boundary : int32_t { BC_SPHERE = 0, BC_CUBIC = 1, BC_RECT = 2, BC_RHOMB = 3 };
 enum PotKind : int32_t {
     POT_ELJ_EVEN = 0,  // sum_k c_k r^-(4+2k), src/EnergyEvaluation.jl:138-153, :934-942
     POT_ELJ = 1,       // sum_k c_k r^-(5+k),  src/EnergyEvaluation.jl:891-933
@@ -81,13 +84,19 @@ struct SweepArgs {
     const int32_t *slot_run;   // run of each slot
     int32_t *slot_temp;        // temperature index held by each slot
     double *slot_en;           // total energy (FP64) per slot
-    double *slot_box;          // cubic box length per slot
+    double *slot_box;          // box length per slot
+    double *slot_boxh;         // box height per slot (cubic: equal to the length)
     double *slot_drift;        // max |E_running - E_recomputed| per slot
     void *coords;              // per-tile [3][n_slab][WGS] (Real)
     void *comps;               // EAM: per-tile [n_slab][2][WGS] (Real)
     const double *t_beta;      // per temperature
     double *t_displ;           // per temperature x2 (atom, volume)
     const double *t_maxbox;    // per temperature
+    const double *t_maxboxh;   // per temperature: max box height
+    const double *t_lh;        // per temperature: length / height ratio bound
+    double *t_displ_sep;       // per temperature x2: xy and z volume steps (separated moves)
+    long long *t_count_volxy;  // per temperature x2
+    long long *t_count_volz;   // per temperature x2
     long long *t_count_atom;   // per temperature x2 (total, window)
     long long *t_count_vol;    // per temperature x2
     long long *t_count_exc;    // per temperature x2 (attempted, accepted)
@@ -113,6 +122,8 @@ struct SweepArgs {
     int32_t trace_cycles;
     int32_t trace_cycle;       // one-cycle launches: index of this cycle in the trace
     int32_t init_run;          // init kernels: run whose energies are initialised (-1: all)
+    int32_t box_kind;          // BC_CUBIC or BC_RECT for the cubic kernels
+    int32_t separated;         // NPT: separated xy / z / xyz volume moves
     // neural-network potential (one workgroup per slot)
     NnpTables nnp;
     float *feat;               // per slot [nf][NNP_NP] features

@@ -22,11 +22,14 @@ from .api import (  # noqa: F401
     Output,
     PTEngine,
     RadialType2,
+    RectangularBC,
+    RhombicBC,
     RuNNerPotential,
     SphericalBC,
     TempGrid,
     check_e_bounds,
     forward_pass,
+    get_volume,
     kB,
     ptmc_run,
     ptmc_run_batch,

@@ -73,6 +73,7 @@ class ptmc_run_desc(C.Structure):
         ("max_displ_vol", C.c_double),
         ("lanes_per_replica", C.c_int32),
         ("run_key", C.c_uint32),
+        ("box_height", C.c_double),
     ]
 
 
@@ -101,6 +102,8 @@ SIGNATURES = {
     "ptmc_get_energies": [_h, C.c_int32, _dp],
     "ptmc_get_state": [_h, C.c_int32, _dp, _dp, _dp],
     "ptmc_get_counters": [_h, C.c_int32, _dp, _i64p, _i64p, _i64p, _dp],
+    "ptmc_get_counters_sep": [_h, C.c_int32, _dp, _i64p, _i64p],
+    "ptmc_set_counters_sep": [_h, C.c_int32, _dp, _i64p, _i64p],
     "ptmc_get_histograms": [_h, C.c_int32, _dp],
     "ptmc_hist_stride": [_h],
     "ptmc_get_drift": [_h, C.c_int32, _dp],

@@ -101,6 +101,49 @@ class CubicBC:
     box_length: float
 
 
+@dataclasses.dataclass
+class RectangularBC:
+    """src/BoundaryConditions.jl:80-93: a square prism, side box_length (x, y)
+    and height box_height (z)."""
+
+    box_length: float
+    box_height: float
+
+
+@dataclasses.dataclass
+class RhombicBC:
+    """src/BoundaryConditions.jl:95-116: a prism over a 60 degree rhombus of
+    side box_length (rows sqrt(3)/2 box_length apart), height box_height."""
+
+    box_length: float
+    box_height: float
+
+
+_BC_CODE = {SphericalBC: 0, CubicBC: 1, RectangularBC: 2, RhombicBC: 3}
+
+
+def _bc_code(bc) -> int:
+    for cls, code in _BC_CODE.items():
+        if isinstance(bc, cls):
+            return code
+    raise TypeError(f"unknown boundary condition {type(bc).__name__}")
+
+
+def _has_height(bc) -> bool:
+    return isinstance(bc, (RectangularBC, RhombicBC))
+
+
+def get_volume(bc) -> float:
+    """src/Configurations.jl:246-256."""
+    if isinstance(bc, CubicBC):
+        return bc.box_length**3
+    if isinstance(bc, RectangularBC):
+        return bc.box_length * bc.box_length * bc.box_height
+    if isinstance(bc, RhombicBC):
+        return bc.box_length * bc.box_length * bc.box_height * 3**0.5 / 2
+    raise TypeError("a spherical container has no volume variable")
+
+
 class Config:
     """src/Configurations.jl:42-51: positions (N x 3, bohr) and a boundary condition."""
 
@@ -128,7 +171,9 @@ class NVT:
 
 @dataclasses.dataclass(frozen=True)
 class NPT:
-    """src/Ensembles.jl:75-86 (uniform volume moves; separated moves are not built)."""
+    """src/Ensembles.jl:75-86.  separated_volume: volume moves in z (1/6), in x
+    and y (1/3) or in all three directions (1/2), src/MCMoves.jl:301-343
+    (rectangular and rhombic boxes)."""
 
     n_atoms: int
     pressure: float
@@ -386,10 +431,10 @@ def check_e_bounds(energy: float, ebounds: list) -> list:
 def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _lib.ptmc_config:
     cfg = _lib.ptmc_config()
     cfg.precision = {"fp64": 0, "fp32": 1}[opts.precision]
-    cfg.ensemble = 1 if isinstance(ensemble, NPT) else 0
-    if isinstance(ensemble, NPT) and ensemble.separated_volume:
-        raise NotImplementedError("separated volume moves are not part of the engine")
-    cfg.boundary = 1 if isinstance(bc, CubicBC) else 0
+    cfg.ensemble = (2 if ensemble.separated_volume else 1) if isinstance(ensemble, NPT) else 0
+    if isinstance(ensemble, NPT) and ensemble.separated_volume and not _has_height(bc):
+        raise ValueError("separated volume moves need a RectangularBC or RhombicBC box")
+    cfg.boundary = _bc_code(bc)
     cfg.exchange_mode = _EXCHANGE[opts.exchange]
     cfg.pair_form = {"horner": 0, "reference": 1}[opts.pair_form]
     cfg.n_adjust = mc_params.n_adjust
@@ -430,7 +475,8 @@ def _run_desc(n_atoms, beta, x0, bc, pressure, opts: EngineOptions, lanes=0, run
     d.n_atoms = n_atoms
     d.n_temps = len(beta)
     d.radius = bc.radius if isinstance(bc, SphericalBC) else 0.0
-    d.box_length = bc.box_length if isinstance(bc, CubicBC) else 0.0
+    d.box_length = bc.box_length if not isinstance(bc, SphericalBC) else 0.0
+    d.box_height = bc.box_height if _has_height(bc) else 0.0
     d.pressure = pressure
     d._beta = np.ascontiguousarray(beta, dtype=np.float64)
     d._x0 = np.ascontiguousarray(x0, dtype=np.float64)
@@ -453,6 +499,7 @@ class PTEngine:
         self.ensemble = ensemble
         self.pot = pot
         self.eng = _lib.Engine(_engine_config(mc_params, ensemble, bc_kind, self.opts))
+        self.has_height = _has_height(bc_kind)
         _set_potential(self.eng, pot)
         self.runs = []
 
@@ -487,9 +534,9 @@ class PTEngine:
         r = self.runs[run]
         x = np.zeros((r["n_temps"], r["n_atoms"], 3))
         en = np.zeros(r["n_temps"])
-        box = np.zeros(r["n_temps"])
+        box = np.zeros((r["n_temps"], 2) if self.has_height else r["n_temps"])
         self.eng.call("ptmc_get_state", run, _lib.ptr(x), _lib.ptr(en), _lib.ptr(box))
-        return x, en, box
+        return x, en, box  # box: lengths, or (length, height) rows
 
     def counters(self, run: int):
         R = self.runs[run]["n_temps"]
@@ -500,7 +547,14 @@ class PTEngine:
         ham = np.zeros((R, 2))
         self.eng.call("ptmc_get_counters", run, _lib.ptr(md), _lib.ptr(ca, C.c_int64), _lib.ptr(cv, C.c_int64),
                       _lib.ptr(ce, C.c_int64), _lib.ptr(ham))
-        return dict(max_displ=md, count_atom=ca, count_vol=cv, count_exc=ce, ham=ham)
+        out = dict(max_displ=md, count_atom=ca, count_vol=cv, count_exc=ce, ham=ham)
+        if self.has_height:
+            ms = np.zeros((R, 2))
+            cxy = np.zeros((R, 2), dtype=np.int64)
+            cz = np.zeros((R, 2), dtype=np.int64)
+            self.eng.call("ptmc_get_counters_sep", run, _lib.ptr(ms), _lib.ptr(cxy, C.c_int64), _lib.ptr(cz, C.c_int64))
+            out.update(max_displ_sep=ms, count_vol_xy=cxy, count_vol_z=cz)
+        return out
 
     def histograms(self, run: int) -> np.ndarray:
         R = self.runs[run]["n_temps"]
@@ -531,12 +585,19 @@ class PTEngine:
         self.eng.call("ptmc_seek", int(cycle), int(phase_cycle))
 
     def set_state(self, run: int, x=None, box=None, max_displ=None, count_atom=None, count_vol=None,
-                  count_exc=None, ham=None):
+                  count_exc=None, ham=None, max_displ_sep=None, count_vol_xy=None, count_vol_z=None):
+        """box as state() returns it: lengths, or (length, height) rows."""
         f = lambda a, t=np.float64: None if a is None else np.ascontiguousarray(a, dtype=t)
         x, box, max_displ, ham = f(x), f(box), f(max_displ), f(ham)
+        if box is not None and box.size != self.runs[run]["n_temps"] * (2 if self.has_height else 1):
+            raise ValueError("box: one length per temperature, or (length, height) per temperature")
         ca, cv, ce = f(count_atom, np.int64), f(count_vol, np.int64), f(count_exc, np.int64)
         self.eng.call("ptmc_set_state", run, _lib.ptr(x), _lib.ptr(box), _lib.ptr(max_displ),
                       _lib.ptr(ca, C.c_int64), _lib.ptr(cv, C.c_int64), _lib.ptr(ce, C.c_int64), _lib.ptr(ham))
+        if max_displ_sep is not None or count_vol_xy is not None or count_vol_z is not None:
+            ms, cxy, cz = f(max_displ_sep), f(count_vol_xy, np.int64), f(count_vol_z, np.int64)
+            self.eng.call("ptmc_set_counters_sep", run, _lib.ptr(ms), _lib.ptr(cxy, C.c_int64),
+                          _lib.ptr(cz, C.c_int64))
 
     def set_trace(self, n_cycles: int):
         self.eng.call("ptmc_set_trace", int(n_cycles))
@@ -645,6 +706,15 @@ def _start_positions(start_config):
     return cfgs[0].boundary_condition, x0
 
 
+def _box_at(bc, box, t):
+    """The boundary condition of temperature t from a state() box array."""
+    if isinstance(bc, CubicBC):
+        return CubicBC(float(box[t]))
+    if _has_height(bc):
+        return type(bc)(float(box[t][0]), float(box[t][1]))
+    return bc
+
+
 def equilibrate(engine: PTEngine, run_ids, mc_params: MCParams):
     """equilibration / equilibration_cycle!, src/MCRun.jl:282-344: eq_cycles/2
     moving cycles, then the energy bounds are scanned eq_cycles - eq_cycles/2
@@ -662,11 +732,12 @@ def equilibrate(engine: PTEngine, run_ids, mc_params: MCParams):
                 eb = check_e_bounds(float(e), eb)
         v_min = v_max = 0.0
         bc = engine.runs[r]["bc"]
-        if isinstance(bc, CubicBC):
-            # the first state's current box, src/Sampling.jl:170-171
+        if not isinstance(bc, SphericalBC):
+            # the first state's current box, src/Sampling.jl:170-171, :199-200, :227-228
             _, _, box = engine.state(r)
-            v_min = box[0] ** 3 * 0.8
-            v_max = box[0] ** 3 * 2.0
+            b0 = _box_at(bc, box, 0)
+            v_min = get_volume(b0) * 0.8
+            v_max = get_volume(b0) * 2.0
         engine.set_histogram(r, eb[0], eb[1], v_min, v_max)
         bounds.append((eb[0], eb[1], v_min, v_max))
     engine.reset_counters()
@@ -702,7 +773,7 @@ def finalise(engine: PTEngine, run: int, mc_params: MCParams, temp: TempGrid, bo
     states = []
     for t in range(r["n_temps"]):
         bc = r["bc"]
-        bct = CubicBC(box[t]) if isinstance(bc, CubicBC) else bc
+        bct = _box_at(bc, box, t)
         states.append(MCState(temp=float(temp.t_grid[t]), beta=float(beta[t]), config=Config(x[t], bct),
                               en_tot=float(en[t]), max_displ=ctr["max_displ"][t].copy(),
                               count_atom=ctr["count_atom"][t].copy(), count_vol=ctr["count_vol"][t].copy(),

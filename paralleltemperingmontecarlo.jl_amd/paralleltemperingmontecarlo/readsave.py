@@ -23,6 +23,7 @@ import pathlib
 
 import numpy as np
 
+from . import api
 from .api import (CubicBC, Config, EmbeddedAtomPotential, ELJPotential, ELJPotentialEven, MCParams, MCState, NPT,
                   NVT, Output, SphericalBC, TempGrid)
 
@@ -155,8 +156,12 @@ def save_histparams(results, mc_params: MCParams | None = None, directory=None):
 
 
 def _bc_info(bc) -> str:
+    """bc_info, src/ReadSave.jl:131-135."""
     if isinstance(bc, SphericalBC):
         return f"SphericalBC{{Float64}} r2: {julia_float(bc.radius2)}"
+    if isinstance(bc, (api.RectangularBC, api.RhombicBC)):
+        return (f"{type(bc).__name__}{{Float64}} box_dims: {julia_float(bc.box_length)}, "
+                f"{julia_float(bc.box_height)}")
     return f"CubicBC{{Float64}} box_length: {julia_float(bc.box_length)}"
 
 
@@ -178,7 +183,7 @@ def _run_states(engine, run):
     x, en, box = engine.state(run)
     ctr = engine.counters(run)
     bc = r["bc"]
-    bcs = [CubicBC(float(box[t])) if isinstance(bc, CubicBC) else bc for t in range(r["n_temps"])]
+    bcs = [api._box_at(bc, box, t) for t in range(r["n_temps"])]
     return x, bcs, ctr
 
 
@@ -261,6 +266,10 @@ def _bc_from_comment(row):
         return SphericalBC(radius=math.sqrt(float(row[2]))), 3
     if "CubicBC" in str(row[0]):
         return CubicBC(float(row[2])), 3
+    # "box_dims: L, H" (src/ReadSave.jl:133-135, read back at :364-368)
+    for cls in (api.RhombicBC, api.RectangularBC):
+        if cls.__name__ in str(row[0]):
+            return cls(float(str(row[2]).rstrip(",")), float(row[3])), 4
     raise NotImplementedError(f"boundary {row[0]} is not handled")
 
 
@@ -354,7 +363,6 @@ def ptmc_run_restart(restart: bool = True, *, save=1000, eq_cycles: float = 0.2,
     draw the same numbers; the reference instead re-runs cycle `index`.
     The reference recentres restored configurations (src/Initialization.jl:114-116);
     recentre_configs=False keeps them as saved."""
-    from . import api
 
     opts = engine or api.EngineOptions()
     mc, temp, ensemble, potential = read_init(restart, eq_cycles, workingdirectory)
@@ -375,10 +383,14 @@ def ptmc_run_restart(restart: bool = True, *, save=1000, eq_cycles: float = 0.2,
     try:
         run = eng.add_run(mc.n_atoms, temp.beta_grid, x0, bc, pressure, run_key=0)
         eng.prepare()
-        boxes = np.array([s.config.boundary_condition.box_length if isinstance(bc, CubicBC) else 1.0
-                          for s in states])
+        if isinstance(bc, (api.RectangularBC, api.RhombicBC)):
+            boxes = np.array([[s.config.boundary_condition.box_length, s.config.boundary_condition.box_height]
+                              for s in states])
+        else:
+            boxes = np.array([s.config.boundary_condition.box_length if isinstance(bc, CubicBC) else 1.0
+                              for s in states])
         md = np.array([[s.max_displ[0], s.max_displ[1]] for s in states])
-        eng.set_state(run, x=x0, box=boxes if isinstance(bc, CubicBC) else None, max_displ=md,
+        eng.set_state(run, x=x0, box=None if isinstance(bc, SphericalBC) else boxes, max_displ=md,
                       count_atom=np.array([[s.count_atom[0], 0] for s in states]),
                       count_vol=np.array([[s.count_vol[0], 0] for s in states]))
         eng.set_histogram(run, results.en_min, results.en_max, results.v_min, results.v_max)
