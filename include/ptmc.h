@@ -95,6 +95,14 @@ int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, 
  * resident next to the positions and are updated per move as
  * energy_update! does (src/EnergyEvaluation.jl:1567-1591). */
 int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double ecam);
+/* ELJPotentialB, the extended Lennard-Jones potential in a strong magnetic
+ * field (src/EnergyEvaluation.jl:161-182, :943-958, tail correction
+ * :983-998): e(r, t) = sum_k c_k r^-p_k (1 + a_k cos 2t + b_k cos 4t), p = 6,
+ * 8, 9, 10, ..., t the angle of the pair vector to z, 0.1 inside r^2 < 5.30;
+ * 1 to 8 coefficients of each.  Every boundary and ensemble of the pair
+ * engine. */
+int ptmc_set_potential_eljb(ptmc_handle h, int32_t n_coeff, const double *coeff_c, const double *coeff_a,
+                            const double *coeff_b);
 /* RuNNer-style network potential (RuNNerPotential, src/EnergyEvaluation.jl:1182-1213;
  * symmetry functions MachineLearningPotential/SymmFunc.jl:24-112): rad holds
  * n_rad rows (eta, G_norm, G_offset, active) of RadialType2, ang n_ang rows

@@ -52,7 +52,7 @@ class orc_nnp(C.Structure):
 class orc_pot(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ncoeff", C.c_int32), ("coeff", C.c_double * 16),
                 ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double),
-                ("nnp", C.POINTER(orc_nnp))]
+                ("nnp", C.POINTER(orc_nnp)), ("coeff_a", C.c_double * 8), ("coeff_b", C.c_double * 8)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -102,6 +102,10 @@ def lib():
         _L.orc_lrc_box.argtypes = [C.c_int, C.c_double, C.POINTER(orc_pot), C.c_int, C.c_double, C.c_double]
         _L.orc_lrc_box.restype = C.c_double
         _L.orc_wrap.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        _L.orc_dimer_energy_b.argtypes = [C.POINTER(orc_pot), C.c_double, C.c_double]
+        _L.orc_dimer_energy_b.restype = C.c_double
+        _L.orc_pair_geom.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        _L.orc_pair_geom.restype = C.c_double
         _L.orc_exp_det.argtypes = [C.c_double]
         _L.orc_exp_det.restype = C.c_double
         _L.orc_log_det.argtypes = [C.c_double]
@@ -128,9 +132,14 @@ def _p(a, t=C.c_double):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
 
-def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None) -> orc_pot:
+def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None, coeff_a=(), coeff_b=()) -> orc_pot:
+    """kind "eljb": ELJPotentialB with c = coeff, a = coeff_a, b = coeff_b."""
     p = orc_pot()
-    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3}[kind]
+    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3, "eljb": 4}[kind]
+    for i, v in enumerate(coeff_a):
+        p.coeff_a[i] = v
+    for i, v in enumerate(coeff_b):
+        p.coeff_b[i] = v
     if nnp is not None:
         p.nnp = C.pointer(nnp.s)
         p._nnp_keep = nnp

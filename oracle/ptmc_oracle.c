@@ -81,9 +81,55 @@ double orc_dimer_energy(const orc_pot *pot, double r2, int pair_form)
     return 0.0;
 }
 
+/* ELJPotentialB, src/EnergyEvaluation.jl:943-958: c_k r^-p_k (1 + a_k cos 2t +
+ * b_k cos 4t) with p = 6, 8, 9, 10, ... and t the angle to z (cos 2t from
+ * tan t); a constant 0.1 inside r^2 < 5.30. */
+double orc_dimer_energy_b(const orc_pot *pot, double r2, double tan_z)
+{
+    if (!(r2 >= 5.30)) return 0.1;
+    double r6inv = 1.0 / (r2 * r2 * r2);
+    const double t2 = 2.0 / (tan_z * tan_z + 1.0) - 1.0;
+    const double t4 = 2.0 * (t2 * t2) - 1.0;
+    double sum1 = pot->coeff[0] * r6inv * (1.0 + pot->coeff_a[0] * t2 + pot->coeff_b[0] * t4);
+    r6inv /= r2;
+    for (int i = 1; i < pot->ncoeff; ++i) {
+        sum1 += pot->coeff[i] * r6inv * (1.0 + pot->coeff_a[i] * t2 + pot->coeff_b[i] * t4);
+        r6inv /= sqrt(r2);
+    }
+    return sum1;
+}
+
+/* the energy of the pair (a, b): positions a, b in the box (l, h), squared
+ * distance times m2 (a uniformly scaled trial keeps the angles) */
+static double pair_energy_ab(const orc_params *p, const orc_pot *pot, const double *a, const double *b, double l,
+                             double h, double m2, int form, double rc, int cutoff)
+{
+    double tz = 0.0;
+    double r2 = orc_pair_geom(a, b, p->bc, l, h, form, pot->kind == ORC_POT_ELJB ? &tz : NULL);
+    r2 = r2 * m2;
+    if (cutoff && !(r2 <= rc)) return 0.0;
+    return pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz) : orc_dimer_energy(pot, r2, p->pair_form);
+}
+
 /* Long-range correction, src/EnergyEvaluation.jl:968-982 (even powers only). */
 double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot)
 {
+    if (pot->kind == ORC_POT_ELJB) {
+        /* lrc(ELJPotentialB), src/EnergyEvaluation.jl:983-998: fixed coefficients */
+        static const double cb[4] = {-0.1279111890228638, -1.328138539967966, 12.260941135261255,
+                                     41.12212408251662};
+        if (r_cut <= 16.0) return 0.0;
+        double rcs = sqrt(r_cut);
+        double rc3 = r_cut * rcs;
+        double e = 0.0;
+        for (int i = 1; i <= 4; ++i) {
+            e += cb[i - 1] / rc3 / (double)(2 * i + 1);
+            rc3 *= r_cut;
+        }
+        double n2 = (double)((int64_t)n_atoms * n_atoms);
+        e *= M_PI * n2 / 4.0 / (rcs * rcs * rcs);
+        return e;
+    }
     if (r_cut <= 50.0) return 0.0;
     double rcs = sqrt(r_cut);
     double rc3 = r_cut * rcs;
@@ -107,7 +153,7 @@ double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot)
  * y by the row spacing sqrt(3)/2 L, x along the skewed axis, z by H.  form 0
  * divides as the reference does; form 1 multiplies by reciprocals computed
  * once per box, as the engine does. */
-double orc_distance2_box(const double *a, const double *b, int bc, double l, double h, int form)
+double orc_pair_geom(const double *a, const double *b, int bc, double l, double h, int form, double *tan_z)
 {
     double bb[3] = {b[0], b[1], b[2]};
     if (bc == ORC_CUBIC || bc == ORC_RECT) {
@@ -133,7 +179,14 @@ double orc_distance2_box(const double *a, const double *b, int bc, double l, dou
         }
     }
     double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
-    return dx * dx + dy * dy + dz * dz;
+    const double dxy2 = dx * dx + dy * dy;
+    if (tan_z) *tan_z = fabs(sqrt(dxy2) / dz);
+    return dxy2 + dz * dz;
+}
+
+double orc_distance2_box(const double *a, const double *b, int bc, double l, double h, int form)
+{
+    return orc_pair_geom(a, b, bc, l, h, form, NULL);
 }
 
 double orc_distance2(const double *a, const double *b, int bc, double box, int form)
@@ -249,9 +302,12 @@ double orc_energy_config_box(const orc_params *p, const orc_pot *pot, const doub
     for (int i = 0; i < N; ++i) {
         int q = i % k;
         for (int j = i + 1; j < N; ++j) {
-            double r2 = orc_distance2_box(&x[3 * i], &x[3 * j], p->bc, l, h, dist_form(p));
+            double tz = 0.0;
+            double r2 = orc_pair_geom(&x[3 * i], &x[3 * j], p->bc, l, h, dist_form(p),
+                                      pot->kind == ORC_POT_ELJB ? &tz : NULL);
             if (p->ensemble == ORC_NPT && !(r2 <= r_cut)) continue;
-            double e = orc_dimer_energy(pot, r2, p->pair_form);
+            double e = pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz)
+                                                 : orc_dimer_energy(pot, r2, p->pair_form);
             if (en_atom) { en_atom[i] += e; en_atom[j] += e; }
             part[q] += e;
         }
@@ -512,6 +568,17 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
     }
 
     double pn[64] = {0}, po[64] = {0};
+    if (pot->kind == ORC_POT_ELJB) {
+        /* src/EnergyEvaluation.jl:1508-1566: new and old tan rows with the r^2 rows */
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            int q = j % k;
+            pn[q] += pair_energy_ab(p, pot, trial, &x[3 * j], box, h, 1.0, dist_form(p), rc, cutoff);
+            po[q] += pair_energy_ab(p, pot, &x[3 * i], &x[3 * j], box, h, 1.0, dist_form(p), rc, cutoff);
+        }
+        o->new_en = (tree_sum(pn, k) - tree_sum(po, k)) + en_tot;
+        return;
+    }
     for (int j = 0; j < N; ++j) {
         if (j == i) continue;
         int q = j % k;
@@ -724,6 +791,11 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         const double s2 = sx * sx;
                         for (int a = 0; a < N; ++a) {
                             for (int b = a + 1; b < N; ++b) {
+                                if (pot->kind == ORC_POT_ELJB) {
+                                    part[a % k] += pair_energy_ab(p, pot, &x[3 * a], &x[3 * b], box, h, s2,
+                                                                  dist_form(p), nrc, 1);
+                                    continue;
+                                }
                                 double r2 = s->d2 ? s->d2[(size_t)r * N * N + a * N + b]
                                                   : orc_distance2_box(&x[3 * a], &x[3 * b], p->bc, box, h,
                                                                       dist_form(p));
@@ -740,10 +812,13 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         for (int a = 0; a < N; ++a) {
                             if (vol_d2) vol_d2[a * N + a] = 0.0;
                             for (int b = a + 1; b < N; ++b) {
-                                double r2 = orc_distance2_box(&vol_x[3 * a], &vol_x[3 * b], p->bc, nbox, nh,
-                                                              dist_form(p));
+                                double tz = 0.0;
+                                double r2 = orc_pair_geom(&vol_x[3 * a], &vol_x[3 * b], p->bc, nbox, nh,
+                                                          dist_form(p), pot->kind == ORC_POT_ELJB ? &tz : NULL);
                                 if (vol_d2) vol_d2[a * N + b] = vol_d2[b * N + a] = r2;
-                                if (r2 <= nrc) part[a % k] += orc_dimer_energy(pot, r2, p->pair_form);
+                                if (r2 <= nrc)
+                                    part[a % k] += pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz)
+                                                                             : orc_dimer_energy(pot, r2, p->pair_form);
                             }
                         }
                     }

@@ -259,6 +259,13 @@ int choose_pair(ptmc_engine *h)
         if (h->cfg.pair_form == 1) return fail(h, PTMC_E_UNSUPPORTED, "EAM has no reference pair form");
         return PTMC_OK;
     }
+    if (p.kind == POT_ELJB) {
+        // one form: FP64 follows the reference's steps, binary32 its own
+        if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "at most 8 ELJPotentialB coefficients");
+        h->pk = PK_ELJB;
+        h->kc = 8;
+        return PTMC_OK;
+    }
     if (h->cfg.pair_form == 1) {
         if (h->cfg.precision != 0) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form requires FP64");
         if (K > 8) return fail(h, PTMC_E_UNSUPPORTED, "reference pair form supports up to 8 coefficients");
@@ -780,6 +787,25 @@ int ptmc_set_potential_elj(ptmc_handle h, int32_t even_powers, int32_t n_coeff, 
     p.kind = even_powers ? POT_ELJ_EVEN : POT_ELJ;
     p.ncoeff = n_coeff;
     for (int i = 0; i < n_coeff; ++i) p.coeff[i] = coeff[i];
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
+}
+
+int ptmc_set_potential_eljb(ptmc_handle h, int32_t n_coeff, const double *coeff_c, const double *coeff_a,
+                            const double *coeff_b)
+{
+    if (!h || !coeff_c || !coeff_a || !coeff_b || n_coeff <= 0 || n_coeff > 8)
+        return h ? fail(h, PTMC_E_ARG, "bad ELJPotentialB coefficients (1 to 8 of each)") : PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    PotParams p{};
+    p.kind = POT_ELJB;
+    p.ncoeff = n_coeff;
+    for (int i = 0; i < n_coeff; ++i) {
+        p.coeff[i] = coeff_c[i];
+        p.coeff_a[i] = coeff_a[i];
+        p.coeff_b[i] = coeff_b[i];
+    }
     h->pot = p;
     h->pot_set = true;
     return PTMC_OK;

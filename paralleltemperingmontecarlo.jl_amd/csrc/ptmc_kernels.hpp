@@ -107,6 +107,7 @@ KernelPair kl_switch_small(int kl, int lanes)
         if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(double, PK_ODD_H, 8, BC, ENS, kl, lanes) }     \
         if (pk == PK_EVEN_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_EVEN_R, 8, BC, ENS, kl, lanes) } \
         if (pk == PK_ODD_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_ODD_R, 8, BC, ENS, kl, lanes) }   \
+        if (pk == PK_ELJB && kc == 8) { PTMC_KL_SWITCH(double, PK_ELJB, 8, BC, ENS, kl, lanes) }       \
         return {};                                                                                   \
     }                                                                                                \
     }
@@ -119,6 +120,7 @@ KernelPair kl_switch_small(int kl, int lanes)
         if (pk == PK_EVEN_H && kc == 6) { PTMC_KL_SWITCH(float, PK_EVEN_H, 6, BC, ENS, kl, lanes) }    \
         if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(float, PK_EVEN_H, 8, BC, ENS, kl, lanes) }    \
         if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(float, PK_ODD_H, 8, BC, ENS, kl, lanes) }      \
+        if (pk == PK_ELJB && kc == 8) { PTMC_KL_SWITCH(float, PK_ELJB, 8, BC, ENS, kl, lanes) }        \
         return {};                                                                                   \
     }                                                                                                \
     }

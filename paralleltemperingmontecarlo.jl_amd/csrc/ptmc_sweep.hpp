@@ -22,7 +22,7 @@
 
 namespace ptmc {
 
-enum PairKind : int { PK_EVEN_H = 0, PK_ODD_H = 1, PK_EVEN_R = 2, PK_ODD_R = 3, PK_LJ = 4 };
+enum PairKind : int { PK_EVEN_H = 0, PK_ODD_H = 1, PK_EVEN_R = 2, PK_ODD_R = 3, PK_LJ = 4, PK_ELJB = 5 };
 
 template <typename Real>
 __device__ __forceinline__ Real rcp_(Real x);
@@ -112,6 +112,48 @@ __device__ __forceinline__ double pair_energy64(double r2, const double (&c)[KC]
         return pair_energy<double, PK, KC>(r2, c);
     else
         return pair_energy<double, PK_EVEN_H, KC>(r2, c);
+}
+
+// ELJPotentialB pair energy (src/EnergyEvaluation.jl:943-958) from r^2 and the
+// pair vector's squared xy extent dxy2 and z extent dz: c_k r^-p_k (1 + a_k
+// cos 2t + b_k cos 4t), p = 6, 8, 9, 10, ..., t the angle to z; 0.1 inside
+// r^2 < 5.30.  FP64 follows the reference's steps (tan t = |sqrt(dxy2) / dz|,
+// cos 2t = 2 / (tan^2 + 1) - 1, the powers by division); binary32 takes
+// cos 2t = (dz^2 - dxy2) / r^2 and the powers from one reciprocal and one
+// reciprocal square root.  Zero-padded coefficients add exact zeros.
+template <typename Real, int KC>
+__device__ __forceinline__ Real pair_energy_b(Real r2, Real dxy2, Real dz, const Real (&c)[KC],
+                                              const Real (&ca)[KC], const Real (&cb)[KC])
+{
+    if constexpr (sizeof(Real) == 8) {
+        if (!(r2 >= 5.30)) return 0.1;
+        const double tz = __builtin_fabs(__builtin_sqrt(dxy2) / dz);
+        double r6inv = 1.0 / (r2 * r2 * r2);
+        const double t2 = 2.0 / (tz * tz + 1.0) - 1.0;
+        const double t4 = 2.0 * (t2 * t2) - 1.0;
+        double sum1 = c[0] * r6inv * (1.0 + ca[0] * t2 + cb[0] * t4);
+        r6inv /= r2;
+#pragma unroll
+        for (int i = 1; i < KC; ++i) {
+            sum1 += c[i] * r6inv * (1.0 + ca[i] * t2 + cb[i] * t4);
+            r6inv /= __builtin_sqrt(r2);
+        }
+        return sum1;
+    } else {
+        const float u = __builtin_amdgcn_rcpf(r2);
+        const float rs = __builtin_amdgcn_rsqf(r2);
+        const float t2 = (dz * dz - dxy2) * u;
+        const float t4 = __builtin_fmaf(2.0f * t2, t2, -1.0f);
+        float p = u * u * u;
+        float sum1 = c[0] * p * (1.0f + ca[0] * t2 + cb[0] * t4);
+        p = p * u;
+#pragma unroll
+        for (int i = 1; i < KC; ++i) {
+            sum1 += c[i] * p * (1.0f + ca[i] * t2 + cb[i] * t4);
+            p = p * rs;
+        }
+        return r2 >= 5.30f ? sum1 : 0.1f;
+    }
 }
 
 // Squared distance between a and b (src/Configurations.jl:101-109).
@@ -253,6 +295,47 @@ __device__ __forceinline__ T box_d2(T ax, T ay, T az, T bx, T by, T bz, const Bo
     }
 }
 
+// box_d2 that also returns the squared xy extent and the z extent of the
+// pair vector (the angle to z of get_tan, src/Configurations.jl:172-218).
+template <typename T, int BC, bool DIV>
+__device__ __forceinline__ T box_d2c(T ax, T ay, T az, T bx, T by, T bz, const BoxC<T> &b, T &dxy2, T &dzo)
+{
+    if constexpr (sizeof(T) == 8) {
+        if constexpr (BC == BC_CUBIC) {
+            bx = bx + b.L * __builtin_rint(DIV ? (ax - bx) / b.L : (ax - bx) * b.iL);
+            by = by + b.L * __builtin_rint(DIV ? (ay - by) / b.L : (ay - by) * b.iL);
+            bz = bz + b.H * __builtin_rint(DIV ? (az - bz) / b.H : (az - bz) * b.iH);
+        } else if constexpr (BC == BC_RHOMB) {
+            const T by2 = by + b.Ly * __builtin_rint(DIV ? (ay - by) / b.Ly : (ay - by) * b.iLy);
+            const T sk = (ax - bx) - INV_SQRT3 * (ay - by);
+            const T kx = __builtin_rint(DIV ? sk / b.L : sk * b.iL);
+            bx = (DIV ? bx - by / SQRT3 : bx - by * INV_SQRT3) + b.L * kx + INV_SQRT3 * by2;
+            by = by2;
+            bz = bz + b.H * __builtin_rint(DIV ? (az - bz) / b.H : (az - bz) * b.iH);
+        }
+        const T dx = ax - bx, dy = ay - by, dz = az - bz;
+        dxy2 = dx * dx + dy * dy;
+        dzo = dz;
+        return dxy2 + dz * dz;
+    } else {
+        T dx = ax - bx, dy = ay - by, dz = az - bz;
+        if constexpr (BC == BC_CUBIC) {
+            dx = __builtin_fmaf(-b.L, __builtin_rintf(dx * b.iL), dx);
+            dy = __builtin_fmaf(-b.L, __builtin_rintf(dy * b.iL), dy);
+            dz = __builtin_fmaf(-b.H, __builtin_rintf(dz * b.iH), dz);
+        } else if constexpr (BC == BC_RHOMB) {
+            const float k2 = __builtin_rintf(dy * b.iLy);
+            const float k1 = __builtin_rintf(__builtin_fmaf(-(float)INV_SQRT3, dy, dx) * b.iL);
+            dx = __builtin_fmaf(-b.hL, k2, __builtin_fmaf(-b.L, k1, dx));
+            dy = __builtin_fmaf(-b.Ly, k2, dy);
+            dz = __builtin_fmaf(-b.H, __builtin_rintf(dz * b.iH), dz);
+        }
+        dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+        dzo = dz;
+        return __builtin_fmaf(dz, dz, dxy2);
+    }
+}
+
 // r_cut^2 of a box (src/Ensembles.jl:118-155); kind BC_CUBIC, BC_RECT or BC_RHOMB
 __device__ __forceinline__ double box_rcut(int kind, double l, double h)
 {
@@ -278,12 +361,31 @@ __device__ __forceinline__ double box_hist_volume(int kind, double l, double h)
     return l * l * l;
 }
 
+// lrc(ELJPotentialB), src/EnergyEvaluation.jl:983-998: four fixed even-power
+// coefficients, nothing inside r_cut <= 16
+__device__ __forceinline__ double lrc_b64(int n_atoms, double r_cut)
+{
+    const double cb[4] = {-0.1279111890228638, -1.328138539967966, 12.260941135261255, 41.12212408251662};
+    if (r_cut <= 16.0) return 0.0;
+    double rcs = __builtin_sqrt(r_cut);
+    double rc3 = r_cut * rcs;
+    double e = 0.0;
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) {
+        e += cb[i - 1] / rc3 / (double)(2 * i + 1);
+        rc3 *= r_cut;
+    }
+    double n2 = (double)((long long)n_atoms * n_atoms);
+    e *= 3.14159265358979323846 * n2 / 4.0 / (rcs * rcs * rcs);
+    return e;
+}
+
 // tail correction of a box (src/EnergyEvaluation.jl:420, :447-448, :469-473)
-template <int KC>
+template <int PK, int KC>
 __device__ __forceinline__ double box_lrc(int kind, int n_atoms, double r_cut, double l, double h,
                                           const double (&c)[KC])
 {
-    const double e = lrc64<KC>(n_atoms, r_cut, c);
+    const double e = PK == PK_ELJB ? lrc_b64(n_atoms, r_cut) : lrc64<KC>(n_atoms, r_cut, c);
     if (kind == BC_RHOMB) return e * 3.0 / 4.0 * l / h;
     if (kind == BC_RECT) return l < h ? e * l / h : e * (h * h) / (l * l);
     return e;
@@ -503,7 +605,8 @@ __device__ bool sphere_redraw(Real *X, int tid, int q, int N, int A, int si, int
 // r^2 <= rc; the caller adds the tail correction.
 template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
 __device__ double pair_sum64(const Real *X, int gb, int q, int N, const BoxC<double> &bx, double px, double pz,
-                             double m2, double rc, const double (&c64)[KC])
+                             double m2, double rc, const double (&c64)[KC], const double (&ca64)[KC],
+                             const double (&cb64)[KC])
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
     const int A = (N + KL - 1) / KL;
@@ -519,12 +622,20 @@ __device__ double pair_sum64(const Real *X, int gb, int q, int N, const BoxC<dou
             const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
             const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
             const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
-            double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx);
-            r2 = r2 * m2;
-            if (ENS == ENS_NPT) {
-                if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+            if constexpr (PK == PK_ELJB) {
+                // the angle to z of the unscaled pair vector (a uniform scaling keeps it)
+                double dxy2, dz;
+                double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz);
+                r2 = r2 * m2;
+                if (ENS != ENS_NPT || r2 <= rc) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
             } else {
-                part += pair_energy64<PK, KC>(r2, c64);
+                double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx);
+                r2 = r2 * m2;
+                if (ENS == ENS_NPT) {
+                    if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+                } else {
+                    part += pair_energy64<PK, KC>(r2, c64);
+                }
             }
         }
     }
@@ -550,7 +661,8 @@ __device__ __forceinline__ double readlane_d(double v, int src)
 // differs.
 template <int PK, int KC, int BC, int KL, int LANES, typename Real>
 __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<double> &bx, double px, double pz,
-                                  double m2, double rc, const double (&c64)[KC], uint64_t live, uint64_t leaders)
+                                  double m2, double rc, const double (&c64)[KC], const double (&ca64)[KC],
+                                  const double (&cb64)[KC], uint64_t live, uint64_t leaders)
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
@@ -573,8 +685,14 @@ __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<doubl
             const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
             const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
             const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
-            const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
-            if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+            if constexpr (PK == PK_ELJB) {
+                double dxy2, dz;
+                const double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz) * m2;
+                if (r2 <= rc) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+            } else {
+                const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
+                if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+            }
         }
         j += nl;
         while (j >= N) {
@@ -637,12 +755,19 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
 
     // coefficients: Real copy for the move path, FP64 copy for totals
     Real c[KC];
-    double c64[KC];
+    double c64[KC], ca64[KC], cb64[KC];
+    Real ca[KC], cb[KC];  // ELJPotentialB anisotropy coefficients
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
         double v = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
         c64[k] = v;
         c[k] = (Real)v;
+        const double va = (PK == PK_ELJB && k < a.pot.ncoeff && k < 8) ? a.pot.coeff_a[k] : 0.0;
+        const double vb = (PK == PK_ELJB && k < a.pot.ncoeff && k < 8) ? a.pot.coeff_b[k] : 0.0;
+        ca64[k] = va;
+        cb64[k] = vb;
+        ca[k] = (Real)va;
+        cb[k] = (Real)vb;
     }
     if constexpr (PK == PK_LJ) {
         c[0] = (Real)a.pot.coeff[0];
@@ -755,9 +880,16 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         const Real xj = Xl[xoff<LANES>(s, 0, 0)];
                         const Real yj = Xl[xoff<LANES>(s, 1, 0)];
                         const Real zj = Xl[xoff<LANES>(s, 2, 0)];
-                        const Real r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
-                        const Real r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
-                        Real en, eo;
+                        Real r2n, r2o, en, eo;
+                        if constexpr (PK == PK_ELJB) {
+                            Real xyn, zn, xyo, zo;
+                            r2n = box_d2c<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr, xyn, zn);
+                            r2o = box_d2c<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr, xyo, zo);
+                            en = pair_energy_b<Real, KC>(r2n, xyn, zn, c, ca, cb);
+                            eo = pair_energy_b<Real, KC>(r2o, xyo, zo, c, ca, cb);
+                        } else {
+                        r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
+                        r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
                         if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
                             // one reciprocal serves the new and the old pair
                             const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
@@ -766,6 +898,7 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         } else {
                             en = pair_energy<Real, PK, KC>(r2n, c);
                             eo = pair_energy<Real, PK, KC>(r2o, c);
+                        }
                         }
                         if constexpr (BC == BC_SPHERE) {
                             if constexpr (F32) sn += en - eo;
@@ -882,16 +1015,17 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                             todo &= todo - 1;
                             const double e = wave_pair_sum64<PK, KC, BC, KL, LANES>(
                                 X, cl, N, make_box<double>(readlane_d(eb, cl), readlane_d(ebh, cl)), readlane_d(px, cl),
-                                readlane_d(pz, cl), readlane_d(m2, cl), readlane_d(nrc, cl), c64, live, leaders);
+                                readlane_d(pz, cl), readlane_d(m2, cl), readlane_d(nrc, cl), c64, ca64, cb64, live,
+                                leaders);
                             if ((gb & 63) == cl) new_en = e;
                         }
                     } else {
                         if (vm)
                             new_en = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(eb, ebh),
-                                                                                  px, pz, m2, nrc, c64);
+                                                                                  px, pz, m2, nrc, c64, ca64, cb64);
                     }
                     if (vm) {
-                        new_en = new_en + box_lrc<KC>(kind, N, nrc, nbox, nh, c64);
+                        new_en = new_en + box_lrc<PK, KC>(kind, N, nrc, nbox, nh, c64);
                         const double pressure = a.runs[a.slot_run[slot]].pressure;
                         const double dd = new_en - E;
                         dE_used = dd;
@@ -1048,8 +1182,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
                 const double rc64 = box_rcut(kind, box, boxh);
                 double ef = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(box, boxh), 1.0,
-                                                                         1.0, 1.0, rc64, c64);
-                if constexpr (ENS == ENS_NPT) ef = ef + box_lrc<KC>(kind, N, rc64, box, boxh, c64);
+                                                                         1.0, 1.0, rc64, c64, ca64, cb64);
+                if constexpr (ENS == ENS_NPT) ef = ef + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
                 const double dr = __builtin_fabs(E - ef);
                 if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
                 E = ef;
@@ -1089,16 +1223,20 @@ __global__ void __launch_bounds__(LANES) init_kernel(SweepArgs a)
     if (lg >= tile.n_slots) return;  // whole clusters leave together
     const int slot = tile.slot0 + lg;
     if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;
-    double c64[KC];
+    double c64[KC], ca64[KC], cb64[KC];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+    for (int k = 0; k < KC; ++k) {
+        c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+        ca64[k] = (PK == PK_ELJB && k < a.pot.ncoeff && k < 8) ? a.pot.coeff_a[k] : 0.0;
+        cb64[k] = (PK == PK_ELJB && k < a.pot.ncoeff && k < 8) ? a.pot.coeff_b[k] : 0.0;
+    }
     const bool pbc = BC != BC_SPHERE;
     const int kind = BC == BC_RHOMB ? (int)BC_RHOMB : (BC == BC_CUBIC ? a.box_kind : (int)BC_SPHERE);
     const double box = pbc ? a.slot_box[slot] : 1.0, boxh = pbc ? a.slot_boxh[slot] : 1.0;
     const double rc64 = pbc ? box_rcut(kind, box, boxh) : 0.0;
     double E = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, lg * KL, q, N, make_box<double>(box, boxh), 1.0, 1.0,
-                                                            1.0, rc64, c64);
-    if constexpr (ENS == ENS_NPT) E = E + box_lrc<KC>(kind, N, rc64, box, boxh, c64);
+                                                            1.0, rc64, c64, ca64, cb64);
+    if constexpr (ENS == ENS_NPT) E = E + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
     if (q == 0) a.slot_en[slot] = E;
 }
 

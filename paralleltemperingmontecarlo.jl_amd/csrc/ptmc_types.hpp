@@ -14,6 +14,8 @@ enum PotKind : int32_t {
     POT_ELJ = 1,       // sum_k c_k r^-(5+k),  src/EnergyEvaluation.jl:891-933
     POT_EAM = 2,       // Sutton-Chen,          src/EnergyEvaluation.jl:1059-1162
     POT_NNP = 3,       // RuNNer network,       src/EnergyEvaluation.jl:1182-1275
+    POT_ELJB = 4,      // ELJ in a magnetic field (anisotropic in the angle to z),
+                       //                       src/EnergyEvaluation.jl:161-182, :943-958
 };
 // How a pair energy is evaluated on the device.
 enum PairForm : int32_t {
@@ -30,6 +32,7 @@ struct PotParams {
     int32_t ncoeff;
     double coeff[MAX_COEFF];
     double eam_n, eam_m, eam_ean, eam_ecam;
+    double coeff_a[8], coeff_b[8];  // ELJPotentialB: coeff holds c, these a and b
 };
 
 // One PT run (one temperature ladder).

@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     CubicBC,
     Config,
     ELJPotential,
+    ELJPotentialB,
     ELJPotentialEven,
     EmbeddedAtomPotential,
     EngineOptions,

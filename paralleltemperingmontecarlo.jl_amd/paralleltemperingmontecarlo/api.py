@@ -189,6 +189,25 @@ class NPT:
 
 
 @dataclasses.dataclass(frozen=True)
+class ELJPotentialB:
+    """src/EnergyEvaluation.jl:161-182: ELJ in a magnetic field,
+    e(r, t) = sum_k c_k r^-p_k (1 + a_k cos 2t + b_k cos 4t), p = 6, 8, 9, ...,
+    t the angle of the pair vector to z (:943-958).  Constructed as
+    ELJPotentialB(a, b, c) like the reference."""
+
+    coeff_a: tuple
+    coeff_b: tuple
+    coeff_c: tuple
+
+    def __init__(self, a, b, c):
+        if not (len(a) == len(b) == len(c)):
+            raise ValueError("number of ELJ coefficients does not match given length")
+        object.__setattr__(self, "coeff_a", tuple(float(v) for v in a))
+        object.__setattr__(self, "coeff_b", tuple(float(v) for v in b))
+        object.__setattr__(self, "coeff_c", tuple(float(v) for v in c))
+
+
+@dataclasses.dataclass(frozen=True)
 class ELJPotentialEven:
     """sum_k c_k r^-(4+2k), k = 1..K; src/EnergyEvaluation.jl:138-153."""
 
@@ -454,6 +473,9 @@ def _set_potential(eng: _lib.Engine, pot):
     if isinstance(pot, (ELJPotentialEven, ELJPotential)):
         c = np.array(pot.coeff, dtype=np.float64)
         eng.call("ptmc_set_potential_elj", 1 if isinstance(pot, ELJPotentialEven) else 0, len(c), _lib.ptr(c))
+    elif isinstance(pot, ELJPotentialB):
+        ca, cb, cc = (np.array(v, dtype=np.float64) for v in (pot.coeff_a, pot.coeff_b, pot.coeff_c))
+        eng.call("ptmc_set_potential_eljb", len(cc), _lib.ptr(cc), _lib.ptr(ca), _lib.ptr(cb))
     elif isinstance(pot, EmbeddedAtomPotential):
         eng.call("ptmc_set_potential_eam", pot.n, pot.m, pot.ean, pot.eCam)
     elif isinstance(pot, RuNNerPotential):

@@ -137,6 +137,16 @@ def multihist_npt32(ref: pathlib.Path):
     (HERE / "multihist_npt32.json").write_text(json.dumps(out))
 
 
+def eljb_pbc216(ref: pathlib.Path):
+    """ELJPotentialB coefficients of the reference's periodic neon scripts
+    (scripts/pbc216.jl:56-59: a, b and c1 of ELJPotentialB{6}(a, b, c1))."""
+    s = (ref / "scripts" / "pbc216.jl").read_text()
+    vec = lambda name: _floats(re.search(r"\n" + name + r" = \[([^\]]*)\]", s).group(1))
+    out = dict(source="scripts/pbc216.jl:56-59", a=vec("a"), b=vec("b"), c=vec("c1"))
+    assert len(out["a"]) == len(out["b"]) == len(out["c"]) == 6
+    (HERE / "eljb_pbc216.json").write_text(json.dumps(out, indent=1))
+
+
 def known_answers(ref: pathlib.Path):
     """Deterministic known answers of the reference's tests, with the lines
     they come from.  Each value is read from the file it cites."""
@@ -169,6 +179,7 @@ def main():
     ne13_setup(ref)
     multihist_ne13(ref)
     multihist_npt32(ref)
+    eljb_pbc216(ref)
     known_answers(ref)
     print("fixtures written to", HERE)
 
