@@ -103,6 +103,14 @@ int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double
  * engine. */
 int ptmc_set_potential_eljb(ptmc_handle h, int32_t n_coeff, const double *coeff_c, const double *coeff_a,
                             const double *coeff_b);
+/* LookupTablePotential (src/EnergyEvaluation.jl:211-295): pair energies
+ * tabulated over the angle to z (n_angle rows, d_angle degrees apart from 0;
+ * the lookup uses rows 1..91) and the distance (n_dist columns from
+ * start_dist, d_dist apart), table[angle][dist] row-major; nearest-entry
+ * lookup, c6coeff / r^6 beyond the table, tail correction from c6coeff
+ * (:999-1008).  read_lookuptable (:229-263) gives the arguments. */
+int ptmc_set_potential_lookup(ptmc_handle h, int32_t n_dist, int32_t n_angle, double start_dist, double d_dist,
+                              double d_angle, double c6coeff, const double *table);
 /* RuNNer-style network potential (RuNNerPotential, src/EnergyEvaluation.jl:1182-1213;
  * symmetry functions MachineLearningPotential/SymmFunc.jl:24-112): rad holds
  * n_rad rows (eta, G_norm, G_offset, active) of RadialType2, ang n_ang rows

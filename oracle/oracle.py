@@ -52,7 +52,9 @@ class orc_nnp(C.Structure):
 class orc_pot(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ncoeff", C.c_int32), ("coeff", C.c_double * 16),
                 ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double),
-                ("nnp", C.POINTER(orc_nnp)), ("coeff_a", C.c_double * 8), ("coeff_b", C.c_double * 8)]
+                ("nnp", C.POINTER(orc_nnp)), ("coeff_a", C.c_double * 8), ("coeff_b", C.c_double * 8),
+                ("lut_ndist", C.c_int32), ("lut_nangle", C.c_int32), ("lut_start", C.c_double), ("lut_d", C.c_double),
+                ("lut_dangle", C.c_double), ("lut_c6", C.c_double), ("lut", C.POINTER(C.c_double))]
 
 
 _dp = C.POINTER(C.c_double)
@@ -104,6 +106,8 @@ def lib():
         _L.orc_wrap.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp]
         _L.orc_dimer_energy_b.argtypes = [C.POINTER(orc_pot), C.c_double, C.c_double]
         _L.orc_dimer_energy_b.restype = C.c_double
+        _L.orc_dimer_energy_lut.argtypes = [C.POINTER(orc_pot), C.c_double, C.c_double]
+        _L.orc_dimer_energy_lut.restype = C.c_double
         _L.orc_pair_geom.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
         _L.orc_pair_geom.restype = C.c_double
         _L.orc_exp_det.argtypes = [C.c_double]
@@ -132,10 +136,18 @@ def _p(a, t=C.c_double):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
 
-def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None, coeff_a=(), coeff_b=()) -> orc_pot:
-    """kind "eljb": ELJPotentialB with c = coeff, a = coeff_a, b = coeff_b."""
+def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None, coeff_a=(), coeff_b=(), lut=None) -> orc_pot:
+    """kind "eljb": ELJPotentialB with c = coeff, a = coeff_a, b = coeff_b;
+    kind "lut": LookupTablePotential from lut = (table [n_angle][n_dist],
+    start_dist, d_dist, d_angle, c6coeff)."""
     p = orc_pot()
-    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3, "eljb": 4}[kind]
+    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3, "eljb": 4, "lut": 5}[kind]
+    if lut is not None:
+        table, start, d, dangle, c6 = lut
+        p._lut_keep = np.ascontiguousarray(table, dtype=np.float64)
+        p.lut_nangle, p.lut_ndist = p._lut_keep.shape
+        p.lut_start, p.lut_d, p.lut_dangle, p.lut_c6 = start, d, dangle, c6
+        p.lut = _p(p._lut_keep)
     for i, v in enumerate(coeff_a):
         p.coeff_a[i] = v
     for i, v in enumerate(coeff_b):

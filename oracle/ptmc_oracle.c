@@ -99,21 +99,54 @@ double orc_dimer_energy_b(const orc_pot *pot, double r2, double tan_z)
     return sum1;
 }
 
+/* LookupTablePotential, src/EnergyEvaluation.jl:273-295: the table entry at
+ * the nearest angle (1 degree rows from 0: index 1 below tan 0.5 degrees,
+ * 91 above tan 89.5 degrees) and the nearest distance, the first column
+ * inside the first half step, c6 / r^6 beyond the table. */
+double orc_dimer_energy_lut(const orc_pot *pot, double r2, double tan_z)
+{
+    const double at = fabs(tan_z);
+    int ai = 1;
+    if (at > 0.00872687153 && at <= 114.592845357)
+        ai = (int)rint(atan(at) / pot->lut_dangle * 180.0 / M_PI + 1.0);
+    else if (at > 114.592845357)
+        ai = 91;
+    const double lo = pot->lut_start + 0.5 * pot->lut_d;
+    const double hi = pot->lut_start + (double)pot->lut_ndist * pot->lut_d;
+    if (r2 <= lo * lo) return pot->lut[(size_t)(ai - 1) * pot->lut_ndist];
+    if (r2 <= hi * hi) {
+        const int di = (int)rint((sqrt(r2) - pot->lut_start) / pot->lut_d);
+        return pot->lut[(size_t)(ai - 1) * pot->lut_ndist + (di - 1)];
+    }
+    return pot->lut_c6 / (r2 * r2 * r2);
+}
+
 /* the energy of the pair (a, b): positions a, b in the box (l, h), squared
  * distance times m2 (a uniformly scaled trial keeps the angles) */
 static double pair_energy_ab(const orc_params *p, const orc_pot *pot, const double *a, const double *b, double l,
                              double h, double m2, int form, double rc, int cutoff)
 {
     double tz = 0.0;
-    double r2 = orc_pair_geom(a, b, p->bc, l, h, form, pot->kind == ORC_POT_ELJB ? &tz : NULL);
+    const int ang = pot->kind == ORC_POT_ELJB || pot->kind == ORC_POT_LUT;
+    double r2 = orc_pair_geom(a, b, p->bc, l, h, form, ang ? &tz : NULL);
     r2 = r2 * m2;
     if (cutoff && !(r2 <= rc)) return 0.0;
+    if (pot->kind == ORC_POT_LUT) return orc_dimer_energy_lut(pot, r2, tz);
     return pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz) : orc_dimer_energy(pot, r2, p->pair_form);
 }
 
 /* Long-range correction, src/EnergyEvaluation.jl:968-982 (even powers only). */
 double orc_lrc(int n_atoms, double r_cut, const orc_pot *pot)
 {
+    if (pot->kind == ORC_POT_LUT) {
+        /* lrc(LookupTablePotential), src/EnergyEvaluation.jl:999-1008 (1.0 for r_cut <= 10) */
+        if (r_cut <= 10.0) return 1.0;
+        double rcs = sqrt(r_cut);
+        double rc3 = r_cut * rcs;
+        double e = pot->lut_c6 / rc3 / 3.0;
+        e *= M_PI * (double)((int64_t)n_atoms * n_atoms) / 4.0 / (rcs * rcs * rcs);
+        return e;
+    }
     if (pot->kind == ORC_POT_ELJB) {
         /* lrc(ELJPotentialB), src/EnergyEvaluation.jl:983-998: fixed coefficients */
         static const double cb[4] = {-0.1279111890228638, -1.328138539967966, 12.260941135261255,
@@ -304,10 +337,11 @@ double orc_energy_config_box(const orc_params *p, const orc_pot *pot, const doub
         for (int j = i + 1; j < N; ++j) {
             double tz = 0.0;
             double r2 = orc_pair_geom(&x[3 * i], &x[3 * j], p->bc, l, h, dist_form(p),
-                                      pot->kind == ORC_POT_ELJB ? &tz : NULL);
+                                      (pot->kind == ORC_POT_ELJB || pot->kind == ORC_POT_LUT) ? &tz : NULL);
             if (p->ensemble == ORC_NPT && !(r2 <= r_cut)) continue;
-            double e = pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz)
-                                                 : orc_dimer_energy(pot, r2, p->pair_form);
+            double e = pot->kind == ORC_POT_ELJB  ? orc_dimer_energy_b(pot, r2, tz)
+                       : pot->kind == ORC_POT_LUT ? orc_dimer_energy_lut(pot, r2, tz)
+                                                  : orc_dimer_energy(pot, r2, p->pair_form);
             if (en_atom) { en_atom[i] += e; en_atom[j] += e; }
             part[q] += e;
         }
@@ -568,7 +602,7 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
     }
 
     double pn[64] = {0}, po[64] = {0};
-    if (pot->kind == ORC_POT_ELJB) {
+    if (pot->kind == ORC_POT_ELJB || pot->kind == ORC_POT_LUT) {
         /* src/EnergyEvaluation.jl:1508-1566: new and old tan rows with the r^2 rows */
         for (int j = 0; j < N; ++j) {
             if (j == i) continue;
@@ -791,7 +825,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         const double s2 = sx * sx;
                         for (int a = 0; a < N; ++a) {
                             for (int b = a + 1; b < N; ++b) {
-                                if (pot->kind == ORC_POT_ELJB) {
+                                if (pot->kind == ORC_POT_ELJB || pot->kind == ORC_POT_LUT) {
                                     part[a % k] += pair_energy_ab(p, pot, &x[3 * a], &x[3 * b], box, h, s2,
                                                                   dist_form(p), nrc, 1);
                                     continue;
@@ -813,12 +847,14 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                             if (vol_d2) vol_d2[a * N + a] = 0.0;
                             for (int b = a + 1; b < N; ++b) {
                                 double tz = 0.0;
+                                const int ang = pot->kind == ORC_POT_ELJB || pot->kind == ORC_POT_LUT;
                                 double r2 = orc_pair_geom(&vol_x[3 * a], &vol_x[3 * b], p->bc, nbox, nh,
-                                                          dist_form(p), pot->kind == ORC_POT_ELJB ? &tz : NULL);
+                                                          dist_form(p), ang ? &tz : NULL);
                                 if (vol_d2) vol_d2[a * N + b] = vol_d2[b * N + a] = r2;
                                 if (r2 <= nrc)
-                                    part[a % k] += pot->kind == ORC_POT_ELJB ? orc_dimer_energy_b(pot, r2, tz)
-                                                                             : orc_dimer_energy(pot, r2, p->pair_form);
+                                    part[a % k] += pot->kind == ORC_POT_ELJB  ? orc_dimer_energy_b(pot, r2, tz)
+                                                   : pot->kind == ORC_POT_LUT ? orc_dimer_energy_lut(pot, r2, tz)
+                                                                              : orc_dimer_energy(pot, r2, p->pair_form);
                             }
                         }
                     }

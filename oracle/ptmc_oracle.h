@@ -18,7 +18,7 @@ extern "C" {
 
 enum { ORC_NVT = 0, ORC_NPT = 1 };
 enum { ORC_SPHERE = 0, ORC_CUBIC = 1, ORC_RECT = 2, ORC_RHOMB = 3 };
-enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3, ORC_POT_ELJB = 4 };
+enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3, ORC_POT_ELJB = 4, ORC_POT_LUT = 5 };
 /* exchange_mode: 0 = reference (p = 0.1 per cycle, one random adjacent pair),
  *                1 = one random adjacent pair every cycle,
  *                2 = all even (cycle even) / odd (cycle odd) pairs every cycle */
@@ -75,6 +75,10 @@ typedef struct {
     const orc_nnp *nnp;
     /* ELJPotentialB (src/EnergyEvaluation.jl:161-182): coeff holds c, these a and b */
     double  coeff_a[8], coeff_b[8];
+    /* LookupTablePotential (src/EnergyEvaluation.jl:211-295): table[angle][dist] */
+    int32_t lut_ndist, lut_nangle;
+    double  lut_start, lut_d, lut_dangle, lut_c6;
+    const double *lut;
 } orc_pot;
 
 typedef struct {
@@ -120,6 +124,8 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
 double orc_dimer_energy(const orc_pot *pot, double r2, int pair_form);
 /* ELJPotentialB pair energy at r^2 and tan of the angle to z (src/EnergyEvaluation.jl:943-958) */
 double orc_dimer_energy_b(const orc_pot *pot, double r2, double tan_z);
+/* LookupTablePotential pair energy (src/EnergyEvaluation.jl:273-295) */
+double orc_dimer_energy_lut(const orc_pot *pot, double r2, double tan_z);
 /* squared distance and tan of its angle to z (get_tan, src/Configurations.jl:172-218) */
 double orc_pair_geom(const double *a, const double *b, int bc, double l, double h, int form, double *tan_z);
 double orc_distance2(const double *a, const double *b, int bc, double box, int cached_form);

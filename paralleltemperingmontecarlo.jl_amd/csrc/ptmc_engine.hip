@@ -116,6 +116,10 @@ struct ptmc_engine {
     // boxes with a height: per temperature max height, length/height bound,
     // separated-move steps (xy, z) and acceptance counters
     DevBuf<double> d_maxboxh, d_lh, d_displ_sep;
+    // LookupTablePotential table, FP64 and binary32 copies
+    std::vector<double> lut_host;
+    DevBuf<double> d_lut64;
+    DevBuf<float> d_lut32;
     DevBuf<long long> d_count_volxy, d_count_volz;
     bool separated = false;
     DevBuf<uint32_t> d_hist;
@@ -257,6 +261,17 @@ int choose_pair(ptmc_engine *h)
         if (h->cfg.boundary != BC_SPHERE && h->cfg.boundary != BC_CUBIC)
             return fail(h, PTMC_E_UNSUPPORTED, "EAM runs in a spherical container or a cubic box");
         if (h->cfg.pair_form == 1) return fail(h, PTMC_E_UNSUPPORTED, "EAM has no reference pair form");
+        return PTMC_OK;
+    }
+    if (p.kind == POT_LUT) {
+        h->pk = PK_LUT;
+        h->kc = 1;
+        std::vector<float> f(h->lut_host.begin(), h->lut_host.end());
+        int rc = upload(h, h->d_lut64, h->lut_host);
+        if (rc) return rc;
+        if ((rc = upload(h, h->d_lut32, f))) return rc;
+        h->pot.lut64 = h->d_lut64.p;
+        h->pot.lut32 = h->d_lut32.p;
         return PTMC_OK;
     }
     if (p.kind == POT_ELJB) {
@@ -717,6 +732,8 @@ int ptmc_destroy(ptmc_handle h)
     h->d_slot_en.free_();
     h->d_slot_box.free_();
     h->d_slot_boxh.free_();
+    h->d_lut64.free_();
+    h->d_lut32.free_();
     h->d_maxboxh.free_();
     h->d_lh.free_();
     h->d_displ_sep.free_();
@@ -806,6 +823,36 @@ int ptmc_set_potential_eljb(ptmc_handle h, int32_t n_coeff, const double *coeff_
         p.coeff_a[i] = coeff_a[i];
         p.coeff_b[i] = coeff_b[i];
     }
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
+}
+
+int ptmc_set_potential_lookup(ptmc_handle h, int32_t n_dist, int32_t n_angle, double start_dist, double d_dist,
+                              double d_angle, double c6coeff, const double *table)
+{
+    if (!h) return PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    if (!table || n_dist < 1 || n_angle < 91 || !(d_dist > 0) || !(d_angle > 0) || !(start_dist >= 0) ||
+        !std::isfinite(c6coeff))
+        return fail(h, PTMC_E_ARG, "bad lookup table (n_angle >= 91 rows, positive steps)");
+    // the largest angle index the lookup forms, at 89.5 degrees, must be a row
+    if (std::rint(89.5 / d_angle + 1.0) > (double)n_angle)
+        return fail(h, PTMC_E_ARG, "angle step %g needs %d rows", d_angle, (int)std::rint(89.5 / d_angle + 1.0));
+    const size_t n = (size_t)n_dist * n_angle;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(table[i])) return fail(h, PTMC_E_ARG, "lookup table entry %zu is not finite", i);
+    PotParams p{};
+    p.kind = POT_LUT;
+    p.ncoeff = 1;
+    p.coeff[0] = c6coeff;  // the tail correction reads c6 from the coefficient slot
+    p.lut_ndist = n_dist;
+    p.lut_nangle = n_angle;
+    p.lut_start = start_dist;
+    p.lut_d = d_dist;
+    p.lut_dangle = d_angle;
+    p.lut_c6 = c6coeff;
+    h->lut_host.assign(table, table + n);
     h->pot = p;
     h->pot_set = true;
     return PTMC_OK;

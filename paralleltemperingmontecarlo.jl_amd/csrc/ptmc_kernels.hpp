@@ -108,6 +108,7 @@ KernelPair kl_switch_small(int kl, int lanes)
         if (pk == PK_EVEN_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_EVEN_R, 8, BC, ENS, kl, lanes) } \
         if (pk == PK_ODD_R && kc == 8) { PTMC_KL_SWITCH_SMALL(double, PK_ODD_R, 8, BC, ENS, kl, lanes) }   \
         if (pk == PK_ELJB && kc == 8) { PTMC_KL_SWITCH(double, PK_ELJB, 8, BC, ENS, kl, lanes) }       \
+        if (pk == PK_LUT && kc == 1) { PTMC_KL_SWITCH(double, PK_LUT, 1, BC, ENS, kl, lanes) }         \
         return {};                                                                                   \
     }                                                                                                \
     }
@@ -121,6 +122,7 @@ KernelPair kl_switch_small(int kl, int lanes)
         if (pk == PK_EVEN_H && kc == 8) { PTMC_KL_SWITCH(float, PK_EVEN_H, 8, BC, ENS, kl, lanes) }    \
         if (pk == PK_ODD_H && kc == 8) { PTMC_KL_SWITCH(float, PK_ODD_H, 8, BC, ENS, kl, lanes) }      \
         if (pk == PK_ELJB && kc == 8) { PTMC_KL_SWITCH(float, PK_ELJB, 8, BC, ENS, kl, lanes) }        \
+        if (pk == PK_LUT && kc == 1) { PTMC_KL_SWITCH(float, PK_LUT, 1, BC, ENS, kl, lanes) }          \
         return {};                                                                                   \
     }                                                                                                \
     }

@@ -22,7 +22,7 @@
 
 namespace ptmc {
 
-enum PairKind : int { PK_EVEN_H = 0, PK_ODD_H = 1, PK_EVEN_R = 2, PK_ODD_R = 3, PK_LJ = 4, PK_ELJB = 5 };
+enum PairKind : int { PK_EVEN_H = 0, PK_ODD_H = 1, PK_EVEN_R = 2, PK_ODD_R = 3, PK_LJ = 4, PK_ELJB = 5, PK_LUT = 6 };
 
 template <typename Real>
 __device__ __forceinline__ Real rcp_(Real x);
@@ -153,6 +153,56 @@ __device__ __forceinline__ Real pair_energy_b(Real r2, Real dxy2, Real dz, const
             p = p * rs;
         }
         return r2 >= 5.30f ? sum1 : 0.1f;
+    }
+}
+
+// LookupTablePotential pair energy (src/EnergyEvaluation.jl:273-295): the
+// entry at the nearest tabulated angle to z (index 1 below tan 0.5 degrees,
+// 91 above tan 89.5 degrees) and the nearest tabulated distance, the first
+// column inside the first half step, c6 / r^6 beyond the table.  FP64
+// follows the reference's steps; binary32 the same with binary32 atan and
+// square root and the binary32 copy of the table.  The indices are clamped
+// to the table (the engine admits only tables where that changes nothing).
+template <typename Real>
+__device__ __forceinline__ Real pair_energy_lut(Real r2, Real dxy2, Real dz, const PotParams &P)
+{
+    if constexpr (sizeof(Real) == 8) {
+        const double at = __builtin_fabs(__builtin_sqrt(dxy2) / dz);
+        int ai = 1;
+        if (at > 0.00872687153 && at <= 114.592845357)
+            ai = (int)__builtin_rint(atan(at) / P.lut_dangle * 180.0 / 3.14159265358979323846 + 1.0);
+        else if (at > 114.592845357)
+            ai = 91;
+        ai = ai < 1 ? 1 : (ai > P.lut_nangle ? P.lut_nangle : ai);
+        const double lo = P.lut_start + 0.5 * P.lut_d;
+        const double hi = P.lut_start + (double)P.lut_ndist * P.lut_d;
+        const double *row = P.lut64 + (size_t)(ai - 1) * P.lut_ndist;
+        if (r2 <= lo * lo) return row[0];
+        if (r2 <= hi * hi) {
+            int di = (int)__builtin_rint((__builtin_sqrt(r2) - P.lut_start) / P.lut_d);
+            di = di < 1 ? 1 : (di > P.lut_ndist ? P.lut_ndist : di);
+            return row[di - 1];
+        }
+        return P.lut_c6 / (r2 * r2 * r2);
+    } else {
+        const float at = __builtin_fabsf(__builtin_sqrtf(dxy2) / dz);
+        int ai = 1;
+        if (at > 0.00872687153f && at <= 114.592845357f)
+            ai = (int)__builtin_rintf(atanf(at) * (float)(180.0 / 3.14159265358979323846 / P.lut_dangle) + 1.0f);
+        else if (at > 114.592845357f)
+            ai = 91;
+        ai = ai < 1 ? 1 : (ai > P.lut_nangle ? P.lut_nangle : ai);
+        const float lo = (float)(P.lut_start + 0.5 * P.lut_d);
+        const float hi = (float)(P.lut_start + (double)P.lut_ndist * P.lut_d);
+        const float *row = P.lut32 + (size_t)(ai - 1) * P.lut_ndist;
+        if (r2 <= lo * lo) return row[0];
+        if (r2 <= hi * hi) {
+            int di = (int)__builtin_rintf((__builtin_sqrtf(r2) - (float)P.lut_start) * (float)(1.0 / P.lut_d));
+            di = di < 1 ? 1 : (di > P.lut_ndist ? P.lut_ndist : di);
+            return row[di - 1];
+        }
+        const float u = __builtin_amdgcn_rcpf(r2);
+        return (float)P.lut_c6 * (u * u * u);
     }
 }
 
@@ -385,7 +435,22 @@ template <int PK, int KC>
 __device__ __forceinline__ double box_lrc(int kind, int n_atoms, double r_cut, double l, double h,
                                           const double (&c)[KC])
 {
-    const double e = PK == PK_ELJB ? lrc_b64(n_atoms, r_cut) : lrc64<KC>(n_atoms, r_cut, c);
+    double e;
+    if constexpr (PK == PK_ELJB) {
+        e = lrc_b64(n_atoms, r_cut);
+    } else if constexpr (PK == PK_LUT) {
+        // lrc(LookupTablePotential), src/EnergyEvaluation.jl:999-1008 (1.0 for r_cut <= 10); c[0] holds c6
+        if (r_cut <= 10.0) {
+            e = 1.0;
+        } else {
+            const double rcs = __builtin_sqrt(r_cut);
+            const double rc3 = r_cut * rcs;
+            e = c[0] / rc3 / 3.0;
+            e *= 3.14159265358979323846 * (double)((long long)n_atoms * n_atoms) / 4.0 / (rcs * rcs * rcs);
+        }
+    } else {
+        e = lrc64<KC>(n_atoms, r_cut, c);
+    }
     if (kind == BC_RHOMB) return e * 3.0 / 4.0 * l / h;
     if (kind == BC_RECT) return l < h ? e * l / h : e * (h * h) / (l * l);
     return e;
@@ -606,7 +671,7 @@ __device__ bool sphere_redraw(Real *X, int tid, int q, int N, int A, int si, int
 template <typename Real, int PK, int KC, int BC, int ENS, int KL, int LANES>
 __device__ double pair_sum64(const Real *X, int gb, int q, int N, const BoxC<double> &bx, double px, double pz,
                              double m2, double rc, const double (&c64)[KC], const double (&ca64)[KC],
-                             const double (&cb64)[KC])
+                             const double (&cb64)[KC], const PotParams &pp)
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
     const int A = (N + KL - 1) / KL;
@@ -622,12 +687,15 @@ __device__ double pair_sum64(const Real *X, int gb, int q, int N, const BoxC<dou
             const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
             const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
             const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
-            if constexpr (PK == PK_ELJB) {
+            if constexpr (PK == PK_ELJB || PK == PK_LUT) {
                 // the angle to z of the unscaled pair vector (a uniform scaling keeps it)
                 double dxy2, dz;
                 double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz);
                 r2 = r2 * m2;
-                if (ENS != ENS_NPT || r2 <= rc) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+                if (ENS != ENS_NPT || r2 <= rc) {
+                    if constexpr (PK == PK_ELJB) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+                    else part += pair_energy_lut<double>(r2, dxy2, dz, pp);
+                }
             } else {
                 double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx);
                 r2 = r2 * m2;
@@ -662,7 +730,7 @@ __device__ __forceinline__ double readlane_d(double v, int src)
 template <int PK, int KC, int BC, int KL, int LANES, typename Real>
 __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<double> &bx, double px, double pz,
                                   double m2, double rc, const double (&c64)[KC], const double (&ca64)[KC],
-                                  const double (&cb64)[KC], uint64_t live, uint64_t leaders)
+                                  const double (&cb64)[KC], const PotParams &pp, uint64_t live, uint64_t leaders)
 {
     constexpr bool DIV = (PK == PK_EVEN_R || PK == PK_ODD_R);
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
@@ -685,10 +753,13 @@ __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<doubl
             const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
             const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
             const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
-            if constexpr (PK == PK_ELJB) {
+            if constexpr (PK == PK_ELJB || PK == PK_LUT) {
                 double dxy2, dz;
                 const double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz) * m2;
-                if (r2 <= rc) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+                if (r2 <= rc) {
+                    if constexpr (PK == PK_ELJB) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+                    else part += pair_energy_lut<double>(r2, dxy2, dz, pp);
+                }
             } else {
                 const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
                 if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
@@ -881,12 +952,17 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         const Real yj = Xl[xoff<LANES>(s, 1, 0)];
                         const Real zj = Xl[xoff<LANES>(s, 2, 0)];
                         Real r2n, r2o, en, eo;
-                        if constexpr (PK == PK_ELJB) {
+                        if constexpr (PK == PK_ELJB || PK == PK_LUT) {
                             Real xyn, zn, xyo, zo;
                             r2n = box_d2c<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr, xyn, zn);
                             r2o = box_d2c<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr, xyo, zo);
-                            en = pair_energy_b<Real, KC>(r2n, xyn, zn, c, ca, cb);
-                            eo = pair_energy_b<Real, KC>(r2o, xyo, zo, c, ca, cb);
+                            if constexpr (PK == PK_ELJB) {
+                                en = pair_energy_b<Real, KC>(r2n, xyn, zn, c, ca, cb);
+                                eo = pair_energy_b<Real, KC>(r2o, xyo, zo, c, ca, cb);
+                            } else {
+                                en = pair_energy_lut<Real>(r2n, xyn, zn, a.pot);
+                                eo = pair_energy_lut<Real>(r2o, xyo, zo, a.pot);
+                            }
                         } else {
                         r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
                         r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
@@ -1015,14 +1091,15 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                             todo &= todo - 1;
                             const double e = wave_pair_sum64<PK, KC, BC, KL, LANES>(
                                 X, cl, N, make_box<double>(readlane_d(eb, cl), readlane_d(ebh, cl)), readlane_d(px, cl),
-                                readlane_d(pz, cl), readlane_d(m2, cl), readlane_d(nrc, cl), c64, ca64, cb64, live,
-                                leaders);
+                                readlane_d(pz, cl), readlane_d(m2, cl), readlane_d(nrc, cl), c64, ca64, cb64, a.pot,
+                                live, leaders);
                             if ((gb & 63) == cl) new_en = e;
                         }
                     } else {
                         if (vm)
                             new_en = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(eb, ebh),
-                                                                                  px, pz, m2, nrc, c64, ca64, cb64);
+                                                                                  px, pz, m2, nrc, c64, ca64, cb64,
+                                                                                  a.pot);
                     }
                     if (vm) {
                         new_en = new_en + box_lrc<PK, KC>(kind, N, nrc, nbox, nh, c64);
@@ -1182,7 +1259,7 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
                 const double rc64 = box_rcut(kind, box, boxh);
                 double ef = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, gb, q, N, make_box<double>(box, boxh), 1.0,
-                                                                         1.0, 1.0, rc64, c64, ca64, cb64);
+                                                                         1.0, 1.0, rc64, c64, ca64, cb64, a.pot);
                 if constexpr (ENS == ENS_NPT) ef = ef + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
                 const double dr = __builtin_fabs(E - ef);
                 if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
@@ -1235,7 +1312,7 @@ __global__ void __launch_bounds__(LANES) init_kernel(SweepArgs a)
     const double box = pbc ? a.slot_box[slot] : 1.0, boxh = pbc ? a.slot_boxh[slot] : 1.0;
     const double rc64 = pbc ? box_rcut(kind, box, boxh) : 0.0;
     double E = pair_sum64<Real, PK, KC, BC, ENS, KL, LANES>(X, lg * KL, q, N, make_box<double>(box, boxh), 1.0, 1.0,
-                                                            1.0, rc64, c64, ca64, cb64);
+                                                            1.0, rc64, c64, ca64, cb64, a.pot);
     if constexpr (ENS == ENS_NPT) E = E + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
     if (q == 0) a.slot_en[slot] = E;
 }

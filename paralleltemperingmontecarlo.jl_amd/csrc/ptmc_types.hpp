@@ -16,6 +16,7 @@ enum PotKind : int32_t {
     POT_NNP = 3,       // RuNNer network,       src/EnergyEvaluation.jl:1182-1275
     POT_ELJB = 4,      // ELJ in a magnetic field (anisotropic in the angle to z),
                        //                       src/EnergyEvaluation.jl:161-182, :943-958
+    POT_LUT = 5,       // tabulated over distance and angle to z, src/EnergyEvaluation.jl:211-295
 };
 // How a pair energy is evaluated on the device.
 enum PairForm : int32_t {
@@ -33,6 +34,11 @@ struct PotParams {
     double coeff[MAX_COEFF];
     double eam_n, eam_m, eam_ean, eam_ecam;
     double coeff_a[8], coeff_b[8];  // ELJPotentialB: coeff holds c, these a and b
+    // LookupTablePotential: table [n_angle][n_dist] in FP64 and binary32
+    int32_t lut_ndist, lut_nangle;
+    double lut_start, lut_d, lut_dangle, lut_c6;
+    const double *lut64;
+    const float *lut32;
 };
 
 // One PT run (one temperature ladder).

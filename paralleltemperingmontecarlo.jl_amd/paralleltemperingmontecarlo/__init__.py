@@ -20,6 +20,7 @@ from .api import (  # noqa: F401
     NeuralNetworkPotential,
     NPT,
     NVT,
+    LookupTablePotential,
     Output,
     PTEngine,
     RadialType2,
@@ -34,6 +35,7 @@ from .api import (  # noqa: F401
     kB,
     ptmc_run,
     ptmc_run_batch,
+    read_lookuptable,
     read_runner_scaling,
     read_runner_weights,
 )

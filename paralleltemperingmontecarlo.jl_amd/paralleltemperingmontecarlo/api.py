@@ -207,6 +207,53 @@ class ELJPotentialB:
         object.__setattr__(self, "coeff_c", tuple(float(v) for v in c))
 
 
+def read_lookuptable(path):
+    """read_lookuptable (src/EnergyEvaluation.jl:229-263): line 1 "n_dist,
+    start_dist, d_dist", line 2 "n_angle, start_angle, d_angle" (text after
+    '!' is a comment), then n_angle x n_dist values, angle-major.  c6coeff
+    is the angle average of the 1000th distance column times that distance
+    to the sixth.  Returns (table, start_dist, start_angle, n_dist, n_angle,
+    d_dist, d_angle, c6coeff)."""
+    with open(path) as f:
+        def fields():
+            return [v for v in f.readline().split("!")[0].split(", ")]
+
+        a = fields()
+        l_dist, start_dist, d_dist = int(a[0]), float(a[1]), float(a[2])
+        a = fields()
+        l_angle, start_angle, d_angle = int(a[0]), float(a[1]), float(a[2])
+        table = np.empty((l_angle, l_dist))
+        for i in range(l_angle):
+            for j in range(l_dist):
+                table[i, j] = float(f.readline().split("!")[0])
+    avg_lr = 0.0
+    for i in range(l_angle):
+        avg_lr += table[i, 999]
+    avg_lr /= l_angle
+    c6coeff = avg_lr * (start_dist + 1000 * d_dist) ** 6
+    return table, start_dist, start_angle, l_dist, l_angle, d_dist, d_angle, c6coeff
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class LookupTablePotential:
+    """src/EnergyEvaluation.jl:211-295: pair energies tabulated over the angle
+    to z (rows, 1 degree from 0: 91 rows) and the distance (columns,
+    n_dist >= 1000), nearest-entry lookup, c6 / r^6 beyond the table."""
+
+    table: np.ndarray
+    start_dist: float
+    start_angle: float
+    l_dist: int
+    l_angle: int
+    d_dist: float
+    d_angle: float
+    c6coeff: float
+
+    @classmethod
+    def from_file(cls, path):
+        return cls(*read_lookuptable(path))
+
+
 @dataclasses.dataclass(frozen=True)
 class ELJPotentialEven:
     """sum_k c_k r^-(4+2k), k = 1..K; src/EnergyEvaluation.jl:138-153."""
@@ -476,6 +523,10 @@ def _set_potential(eng: _lib.Engine, pot):
     elif isinstance(pot, ELJPotentialB):
         ca, cb, cc = (np.array(v, dtype=np.float64) for v in (pot.coeff_a, pot.coeff_b, pot.coeff_c))
         eng.call("ptmc_set_potential_eljb", len(cc), _lib.ptr(cc), _lib.ptr(ca), _lib.ptr(cb))
+    elif isinstance(pot, LookupTablePotential):
+        t = np.ascontiguousarray(pot.table, dtype=np.float64)
+        eng.call("ptmc_set_potential_lookup", int(pot.l_dist), int(pot.l_angle), float(pot.start_dist),
+                 float(pot.d_dist), float(pot.d_angle), float(pot.c6coeff), _lib.ptr(t))
     elif isinstance(pot, EmbeddedAtomPotential):
         eng.call("ptmc_set_potential_eam", pot.n, pot.m, pot.ean, pot.eCam)
     elif isinstance(pot, RuNNerPotential):
