@@ -121,6 +121,9 @@ def lib():
         _L.orc_nnp_move_g.argtypes = [C.POINTER(orc_nnp), _dp, C.c_int, C.c_int, _dp, _dp, _dp]
         _L.orc_nnp_energy.argtypes = [C.POINTER(orc_nnp), _dp, C.c_int, _dp]
         _L.orc_nnp_energy.restype = C.c_double
+        _L.orc_nnp2_full_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, _dp]
+        _L.orc_nnp2_energy.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, _dp]
+        _L.orc_nnp2_energy.restype = C.c_double
     return _L
 
 
@@ -197,6 +200,41 @@ class NNP:
         g = np.ascontiguousarray(g, dtype=np.float64)
         e = np.zeros(len(g))
         tot = lib().orc_nnp_energy(C.byref(self.s), _p(g), len(g), _p(e))
+        return tot, e
+
+
+class orc_nnp2(C.Structure):
+    _fields_ = [("n_rad", C.c_int32), ("n_ang", C.c_int32), ("rad", C.POINTER(C.c_double)),
+                ("ang", C.POINTER(C.c_double)), ("r_cut", C.c_double), ("n_layers", C.c_int32), ("nodes", _i32p),
+                ("act", _i32p), ("n_params", C.c_int32), ("params1", C.POINTER(C.c_double)),
+                ("params2", C.POINTER(C.c_double))]
+
+
+class NNP2:
+    """An orc_nnp2 built from a RuNNerPotential2Atom, arrays kept alive."""
+
+    def __init__(self, pot):
+        self.rad, self.ang = (np.ascontiguousarray(t, dtype=np.float64) for t in pot.tables())
+        self.nodes = np.ascontiguousarray(pot.nnp1.num_nodes, dtype=np.int32)
+        self.act = np.ascontiguousarray(pot.nnp1.activation_functions, dtype=np.int32)
+        self.p1 = np.ascontiguousarray(pot.nnp1.parameters, dtype=np.float64)
+        self.p2 = np.ascontiguousarray(pot.nnp2.parameters, dtype=np.float64)
+        self.s = orc_nnp2(len(self.rad), len(self.ang), _p(self.rad), _p(self.ang), float(pot.r_cut), len(self.nodes),
+                          _p(self.nodes, C.c_int32), _p(self.act, C.c_int32), len(self.p1), _p(self.p1), _p(self.p2))
+        self.n_features = 2 * len(self.rad) + 3 * len(self.ang)
+
+    def full_g(self, x, n1, n2):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros((n1 + n2, self.n_features))
+        rc = lib().orc_nnp2_full_g(C.byref(self.s), _p(x), n1, n2, _p(g))
+        if rc:
+            raise RuntimeError("orc_nnp2_full_g failed")
+        return g
+
+    def energy(self, g, n1, n2):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        e = np.zeros(n1 + n2)
+        tot = lib().orc_nnp2_energy(C.byref(self.s), _p(g), n1, n2, _p(e))
         return tot, e
 
 

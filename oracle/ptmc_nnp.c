@@ -217,3 +217,117 @@ double orc_nnp_energy(const orc_nnp *nn, const double *g, int N, double *eatom)
     for (int a = 0; a < N; ++a) e += eatom[a];
     return e;
 }
+
+/* ---- two species (RuNNerPotential2Atom, src/EnergyEvaluation.jl:1276-1305,
+ * :1748-1768, :1822-1856; MachineLearningPotential/SymmFunc.jl:54-175,
+ * :313-547).  Atoms 0..n1-1 are species 1, n1..N-1 species 2.  Each radial
+ * function fills two rows (neighbours of species 1, of species 2), each
+ * angular function three (neighbour pairs 11, 12, 22); rows are scaled per
+ * centre species: radial g * norm + offset (4 pairs), angular g * tpz +
+ * offset (6 pairs).  The loops and the order in which each row accumulates
+ * are the reference's: pair / triple blocks 11(1), 22(2), 12 for radial and
+ * 111, 222, 112, 122 for angular. */
+#define RAD2(q, k) nn->rad[(q) * 9 + (k)]
+#define ANG2(q, k) nn->ang[(q) * 15 + (k)]
+
+int orc_nnp2_full_g(const orc_nnp2 *nn, const double *x, int n1, int n2, double *g)
+{
+    const int N = n1 + n2, Ng = 2 * nn->n_rad + 3 * nn->n_ang;
+    double *r2 = (double *)malloc(sizeof(double) * N * N), *f = (double *)malloc(sizeof(double) * N * N);
+    if (!r2 || !f) return -1;
+    for (int a = 0; a < N; ++a)
+        for (int b = 0; b < N; ++b) {
+            r2[a * N + b] = d2(&x[3 * a], &x[3 * b]);
+            f[a * N + b] = orc_cutoff(sqrt(r2[a * N + b]), nn->r_cut);
+        }
+    memset(g, 0, sizeof(double) * Ng * N);
+#define G(row, atom) g[(size_t)(atom) * Ng + (row)]
+    for (int q = 0; q < nn->n_rad; ++q) {
+        const int r0 = 2 * q;
+        const double eta = RAD2(q, 0);
+        /* calc_symm_vals! radial (SymmFunc.jl:319-361) */
+#define RV(a, b) ((f[(a) * N + (b)] != 0.0 && f[(a) * N + (b)] != 1.0) ? f[(a) * N + (b)] * exp(-eta * r2[(a) * N + (b)]) : 0.0)
+        for (int a = 0; a < n1; ++a)
+            for (int b = a + 1; b < n1; ++b) {
+                const double v = RV(a, b);
+                G(r0, a) += v;
+                G(r0, b) += v;
+            }
+        for (int a = n1; a < N; ++a)
+            for (int b = a + 1; b < N; ++b) {
+                const double v = RV(a, b);
+                G(r0 + 1, a) += v;
+                G(r0 + 1, b) += v;
+            }
+        for (int a = 0; a < n1; ++a)
+            for (int b = n1; b < N; ++b) {
+                const double v = RV(a, b);
+                G(r0 + 1, a) += v;
+                G(r0, b) += v;
+            }
+#undef RV
+        for (int a = 0; a < N; ++a) {
+            const int s = a < n1 ? 0 : 2;
+            G(r0, a) = G(r0, a) * RAD2(q, 1 + s) + RAD2(q, 5 + s);
+            G(r0 + 1, a) = G(r0 + 1, a) * RAD2(q, 2 + s) + RAD2(q, 6 + s);
+        }
+    }
+    for (int q = 0; q < nn->n_ang; ++q) {
+        const int r0 = 2 * nn->n_rad + 3 * q;
+        const double eta = ANG2(q, 0), lam = ANG2(q, 1), zeta = ANG2(q, 2);
+        /* calc_symm_vals! angular (SymmFunc.jl:363-468): the block of the
+         * triple (a < b < c) decides the row of each centre */
+        for (int blk = 0; blk < 4; ++blk) {
+            int a0, a1, ra, rb, rc_;
+            /* 111: rows 0,0,0; 222: 2,2,2; 112 (a, b in 1, c in 2): 1,1,0; 122: 2,1,1 */
+            if (blk == 0) { a0 = 0; a1 = n1; ra = 0; rb = 0; rc_ = 0; }
+            else if (blk == 1) { a0 = n1; a1 = N; ra = 2; rb = 2; rc_ = 2; }
+            else if (blk == 2) { a0 = 0; a1 = n1; ra = 1; rb = 1; rc_ = 0; }
+            else { a0 = 0; a1 = n1; ra = 2; rb = 1; rc_ = 1; }
+            for (int a = a0; a < a1; ++a) {
+                int blo, bhi;
+                if (blk == 0) { blo = a + 1; bhi = n1; }
+                else if (blk == 1) { blo = a + 1; bhi = N; }
+                else if (blk == 2) { blo = a + 1; bhi = n1; }
+                else { blo = n1; bhi = N; }
+                for (int b = blo; b < bhi; ++b) {
+                    int clo, chi;
+                    if (blk == 0) { clo = b + 1; chi = n1; }
+                    else if (blk == 1) { clo = b + 1; chi = N; }
+                    else if (blk == 2) { clo = n1; chi = N; }
+                    else { clo = b + 1; chi = N; }
+                    for (int c = clo; c < chi; ++c) {
+                        double th[3];
+                        const double rab = r2[a * N + b], rac = r2[a * N + c], rbc = r2[b * N + c];
+                        angles(&x[3 * a], &x[3 * b], &x[3 * c], rab, rac, rbc, th);
+                        const double e = exp(-eta * (rab + rac + rbc)) * f[a * N + b] * f[a * N + c] * f[b * N + c];
+                        G(r0 + ra, a) += e * pow(1.0 + lam * th[0], zeta);
+                        G(r0 + rb, b) += e * pow(1.0 + lam * th[1], zeta);
+                        G(r0 + rc_, c) += e * pow(1.0 + lam * th[2], zeta);
+                    }
+                }
+            }
+        }
+        for (int a = 0; a < N; ++a) {
+            const int s = a < n1 ? 0 : 3;
+            for (int k = 0; k < 3; ++k) G(r0 + k, a) = G(r0 + k, a) * ANG2(q, 3 + s + k) + ANG2(q, 9 + s + k);
+        }
+    }
+#undef G
+    free(r2);
+    free(f);
+    return 0;
+}
+
+double orc_nnp2_energy(const orc_nnp2 *nn, const double *g, int n1, int n2, double *eatom)
+{
+    const int Ng = 2 * nn->n_rad + 3 * nn->n_ang;
+    if (nn->nodes[0] != Ng) return NAN;
+    double tot = 0.0;
+    if (n1 > 0) orc_forward(g, &nn->nodes[0], &nn->n_layers, nn->nodes, &n1, &nn->n_params, nn->params1, nn->act, eatom);
+    if (n2 > 0)
+        orc_forward(g + (size_t)n1 * Ng, &nn->nodes[0], &nn->n_layers, nn->nodes, &n2, &nn->n_params, nn->params2,
+                    nn->act, eatom + n1);
+    for (int a = 0; a < n1 + n2; ++a) tot += eatom[a];
+    return tot;
+}

@@ -160,6 +160,27 @@ int    orc_nnp_move_g(const orc_nnp *nn, const double *x, int n_atoms, int i, co
 /* sum of the per-atom network energies of a feature matrix */
 double orc_nnp_energy(const orc_nnp *nn, const double *g, int n_atoms, double *eatom);
 
+/* Two-species network potential (RuNNerPotential2Atom).  rad: n_rad rows
+ * (eta, norm[4], offset[4]) of RadialType2a (norm / offset order: species 1
+ * rows 1, 2, species 2 rows 1, 2); ang: n_ang rows (eta, lambda, zeta,
+ * tpz[6], offset[6]) of AngularType3a.  params1 / params2: the networks of
+ * species 1 / 2 (same layout). */
+typedef struct {
+    int32_t n_rad, n_ang;
+    const double *rad;
+    const double *ang;
+    double  r_cut;
+    int32_t n_layers;
+    const int32_t *nodes;
+    const int32_t *act;
+    int32_t n_params;
+    const double *params1, *params2;
+} orc_nnp2;
+/* features (2 n_rad + 3 n_ang per atom, atom-major) of atoms 0..n1-1
+ * (species 1) and n1..n1+n2-1 (species 2) in a spherical container */
+int    orc_nnp2_full_g(const orc_nnp2 *nn, const double *x, int n1, int n2, double *g);
+double orc_nnp2_energy(const orc_nnp2 *nn, const double *g, int n1, int n2, double *eatom);
+
 #ifdef __cplusplus
 }
 #endif

@@ -394,6 +394,79 @@ def forward_pass(input, batchsize, nnparams: NeuralNetworkPotential) -> np.ndarr
     return eatom
 
 
+@dataclasses.dataclass(frozen=True)
+class RadialType2a:
+    """MachineLearningPotential/SymmFunc.jl:54-83: a radial function of a
+    two-species potential; G_norm / G_offset for (species 1: neighbours of
+    species 1, of species 2; species 2: the same), from [G_max, G_min] pairs."""
+
+    eta: float
+    r_cut: float
+    type_vec: int
+    G_offset: tuple
+    G_norm: tuple
+
+    @classmethod
+    def from_scaling(cls, eta, r_cut, type_vec, g_vals_a, g_vals_b):
+        norms, offs = [], []
+        for gmax, gmin in (g_vals_a[0], g_vals_a[1], g_vals_b[0], g_vals_b[1]):
+            n = 1 / (gmax - gmin)
+            norms.append(n)
+            offs.append(-gmin * n)
+        return cls(float(eta), float(r_cut), int(type_vec), tuple(offs), tuple(norms))
+
+
+@dataclasses.dataclass(frozen=True)
+class AngularType3a:
+    """MachineLearningPotential/SymmFunc.jl:112-175: an angular function of a
+    two-species potential; tpz = 2^(1-zeta) G_norm and G_offset for centre
+    species 1 (neighbour pairs 11, 12, 22) then species 2."""
+
+    eta: float
+    lambda_: float
+    zeta: float
+    r_cut: float
+    type_vec: int
+    tpz: tuple
+    G_offset: tuple
+
+    @classmethod
+    def from_scaling(cls, eta, lam, zeta, r_cut, type_vec, g_vals_a, g_vals_b):
+        tpz, offs = [], []
+        for gmax, gmin in list(g_vals_a[:3]) + list(g_vals_b[:3]):
+            n = 1 / (gmax - gmin)
+            tpz.append(2.0 ** (1 - zeta) * n)
+            offs.append(-gmin * n)
+        return cls(float(eta), float(lam), float(zeta), float(r_cut), int(type_vec), tuple(tpz), tuple(offs))
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class RuNNerPotential2Atom:
+    """src/EnergyEvaluation.jl:1287-1317: networks of species 1 and 2, the
+    two-species symmetry functions, n1 atoms of species 1 first, then n2 of
+    species 2; r_cut from the first radial function."""
+
+    nnp1: NeuralNetworkPotential
+    nnp2: NeuralNetworkPotential
+    radsymfunctions: tuple
+    angsymfunctions: tuple
+    n1: int
+    n2: int
+    boundary: float = 0.0
+
+    @property
+    def r_cut(self) -> float:
+        return self.radsymfunctions[0].r_cut
+
+    def tables(self):
+        """(rad [n_rad][9]: eta, norm[4], offset[4]; ang [n_ang][15]: eta,
+        lambda, zeta, tpz[6], offset[6])"""
+        rad = np.array([[f.eta, *f.G_norm, *f.G_offset] for f in self.radsymfunctions], dtype=np.float64)
+        ang = np.array([[f.eta, f.lambda_, f.zeta, *f.tpz, *f.G_offset] for f in self.angsymfunctions],
+                       dtype=np.float64)
+        return rad, ang
+
+
 def read_runner_weights(path) -> np.ndarray:
     """A RuNNer weights.XXX.data file (first column) as the flat parameter vector."""
     vals = []

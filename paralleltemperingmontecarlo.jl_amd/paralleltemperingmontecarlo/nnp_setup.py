@@ -94,3 +94,29 @@ def read_save_data(path):
             pos.append([float(v) for v in l.split()])
         out.append((np.array(pos), e))
     return out
+
+
+def reference_brass_potential(setup, scaling_path, weights1_path, weights2_path):
+    """The two-species potential of test/potentialfile.jl: 5 radial and 26
+    angular functions from the setup tables (nnp2_brass.json), scaling rows
+    1-88 for species 1 and 89-176 for species 2 as [G_max, G_min] pairs,
+    88-20-20-1 networks from weights.029 (species 1) and weights.030
+    (species 2)."""
+    from .api import (AngularType3a, NeuralNetworkPotential, RadialType2a, RuNNerPotential2Atom, read_runner_scaling,
+                      read_runner_weights)
+
+    sc = read_runner_scaling(scaling_path)
+    gv = [[r[3], r[2]] for r in sc[:88]]
+    gvb = [[r[3], r[2]] for r in sc[88:]]
+    rad = []
+    for k, row in enumerate(setup["radial"]):
+        rad.append(RadialType2a.from_scaling(row[1], row[3], int(row[0]), [gv[2 * k], gv[2 * k + 1]],
+                                             [gvb[2 * k], gvb[2 * k + 1]]))
+    ang = []
+    for j, el in enumerate(setup["angular"]):
+        i0 = 10 + j * 3
+        ang.append(AngularType3a.from_scaling(el[0], el[1], el[2], 11.338, 2, gv[i0:i0 + 3], gvb[i0:i0 + 3]))
+    nodes, act = setup["num_nodes"], setup["activations"]
+    nn1 = NeuralNetworkPotential(nodes, act, read_runner_weights(weights1_path))
+    nn2 = NeuralNetworkPotential(nodes, act, read_runner_weights(weights2_path))
+    return RuNNerPotential2Atom(nn1, nn2, tuple(rad), tuple(ang), setup["n1"], setup["n2"])

@@ -147,6 +147,41 @@ def eljb_pbc216(ref: pathlib.Path):
     (HERE / "eljb_pbc216.json").write_text(json.dumps(out, indent=1))
 
 
+def nnp2_potentialfile(ref: pathlib.Path):
+    """The two-species RuNNer potential of the reference's tests
+    (test/potentialfile.jl: radial X and angular V tables, network shape,
+    species counts) and its known answer (test/test_potentials.jl:68-88:
+    six atoms, E = -0.261652899); the second species' weights
+    (test/testing_data/weights.030.data) are copied next to weights.029."""
+    pf = (ref / "test" / "potentialfile.jl").read_text()
+    xb = pf[pf.index("X = ["): pf.index("]", pf.index("X = ["))]
+    X = [[float(v) for v in line.split()] for line in xb.splitlines()[1:] if line.strip()]
+    vb = pf[pf.index("V = ["): pf.index("\n]", pf.index("V = ["))]
+    V = [[float(v) for v in m.split(",")] for m in re.findall(r"\[([^\[\]]*)\]", vb[len("V = ["):])]
+    tp = (ref / "test" / "test_potentials.jl").read_text()
+    tb = tp[tp.index('@testset "RuNNerPotentialTest"'):]
+    pos = [[float(v) for v in m.split(",")] for m in re.findall(r"v\d = SVector\(([^)]*)\)", tb)]
+    pos = [[p[0], p[1], p[2]] for p in pos]
+    # "5.0 * 3^0.5"-style entries do not occur here: every coordinate is a literal
+    assert len(pos) == 6 and all(len(p) == 3 for p in pos)
+    out = dict(
+        source="test/potentialfile.jl:6-129 (tables), test/test_potentials.jl:68-88 (configuration, energy)",
+        radial=X, angular=V,
+        num_nodes=[int(v) for v in re.search(r"num_nodes::Vector\{Int32\} = \[([^\]]*)\]", pf).group(1).split(",")],
+        activations=[int(v) for v in re.search(r"activation_functions::Vector\{Int32\} = \[([^\]]*)\]",
+                                               pf).group(1).split(",")],
+        n1=4, n2=2,
+        positions=pos,
+        radius=float(re.search(r"radius=([0-9.]+)", tb).group(1)),
+        energy=float(re.search(r"E ≈ (-?[0-9.]+)", tb).group(1)),
+    )
+    assert len(X) == 5 and len(V) == 26 and out["num_nodes"] == [88, 20, 20, 1]
+    assert all(len(r) == 4 for r in X) and all(len(r) == 4 for r in V)
+    (HERE / "nnp2_brass.json").write_text(json.dumps(out, indent=1))
+    src = ref / "test" / "testing_data" / "weights.030.data"
+    (HERE / "nnp" / "weights.030.data").write_bytes(src.read_bytes())
+
+
 def known_answers(ref: pathlib.Path):
     """Deterministic known answers of the reference's tests, with the lines
     they come from.  Each value is read from the file it cites."""
@@ -180,6 +215,7 @@ def main():
     multihist_ne13(ref)
     multihist_npt32(ref)
     eljb_pbc216(ref)
+    nnp2_potentialfile(ref)
     known_answers(ref)
     print("fixtures written to", HERE)
 
