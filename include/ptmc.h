@@ -236,6 +236,22 @@ int ptmc_multihistogram_npt(int32_t n_runs, int32_t n_temps, int32_t n_bin, cons
                             double k_boltzmann, double conv_threshold, int32_t max_iter, double *cp,
                             double *moments, double *free_energy, int32_t *iterations, int32_t device);
 
+/* Two-species RuNNer potential energies (RuNNerPotential2Atom,
+ * src/EnergyEvaluation.jl:1287-1317, initialise_energy :1748-1768, features
+ * MachineLearningPotential/SymmFunc.jl:313-547), FP64 on the device for
+ * n_conf configurations of n1 atoms of species 1 then n2 of species 2 (at
+ * most 64 atoms, spherical container: no periodic images).  rad: n_rad rows
+ * (eta, norm[4], offset[4]) of RadialType2a, ang: n_ang rows (eta, lambda,
+ * zeta, tpz[6], offset[6]) of AngularType3a (norm / tpz / offset ordered
+ * species 1 rows then species 2 rows); features per atom 2 n_rad + 3 n_ang
+ * (at most 128) = nodes[0]; params1 / params2 the species' networks in the
+ * RuNNer layout (hidden widths up to 64).  Outputs energy (n_conf) and, if
+ * not NULL, eatom (n_conf x (n1 + n2)). */
+int ptmc_nnp2_energy(int32_t n_conf, int32_t n1, int32_t n2, const double *x, int32_t n_rad, int32_t n_ang,
+                     const double *rad, const double *ang, double r_cut, int32_t n_layers, const int32_t *nodes,
+                     const int32_t *activations, int32_t n_params, const double *params1, const double *params2,
+                     double *energy, double *eatom, int32_t device);
+
 /* Learned surrogate and delayed-acceptance Metropolis for the network
  * potential.  No counterpart in the reference: these entry points serve the
  * engine's own acceleration (Christen & Fox 2005).  The surrogate is a

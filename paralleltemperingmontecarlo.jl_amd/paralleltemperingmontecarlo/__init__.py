@@ -18,6 +18,10 @@ from .api import (  # noqa: F401
     MCParams,
     MCState,
     NeuralNetworkPotential,
+    RuNNerPotential2Atom,
+    RadialType2a,
+    AngularType3a,
+    nnp2_energies,
     NPT,
     NVT,
     LookupTablePotential,

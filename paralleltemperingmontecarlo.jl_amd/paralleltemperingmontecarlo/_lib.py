@@ -135,6 +135,8 @@ SIGNATURES = {
 FREE_SIGNATURES = {
     "ptmc_multihistogram_nvt": [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, C.c_int32, _dp, _dp,
                                 C.c_double, C.c_double, C.c_int32, _dp, _dp, _i32p, C.c_int32],
+    "ptmc_nnp2_energy": [C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp, _dp, C.c_double, C.c_int32,
+                         _i32p, _i32p, C.c_int32, _dp, _dp, _dp, _dp, C.c_int32],
     "ptmc_multihistogram_npt": [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int32,
                                 _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp, _i32p, C.c_int32],
 }

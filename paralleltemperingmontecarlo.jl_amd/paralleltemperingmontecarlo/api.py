@@ -467,6 +467,31 @@ class RuNNerPotential2Atom:
         return rad, ang
 
 
+def nnp2_energies(pot: "RuNNerPotential2Atom", x, device: int = 0):
+    """initialise_energy for RuNNerPotential2Atom (src/EnergyEvaluation.jl:
+    1748-1768) of a batch of configurations x [n_conf, n1 + n2, 3] on the
+    device (FP64).  Returns (energies [n_conf], per-atom energies)."""
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.ndim == 2:
+        x = x[None]
+    n_conf, N, _ = x.shape
+    if N != pot.n1 + pot.n2:
+        raise ValueError(f"configurations hold {N} atoms, the potential {pot.n1} + {pot.n2}")
+    rad, ang = (np.ascontiguousarray(t, dtype=np.float64) for t in pot.tables())
+    nodes = np.ascontiguousarray(pot.nnp1.num_nodes, dtype=np.int32)
+    act = np.ascontiguousarray(pot.nnp1.activation_functions, dtype=np.int32)
+    p1 = np.ascontiguousarray(pot.nnp1.parameters, dtype=np.float64)
+    p2 = np.ascontiguousarray(pot.nnp2.parameters, dtype=np.float64)
+    e = np.zeros(n_conf)
+    eat = np.zeros((n_conf, N))
+    rc = lib.ptmc_nnp2_energy(n_conf, pot.n1, pot.n2, _lib.ptr(x), len(rad), len(ang), _lib.ptr(rad), _lib.ptr(ang),
+                              float(pot.r_cut), len(nodes), _lib.ptr(nodes, C.c_int32), _lib.ptr(act, C.c_int32),
+                              len(p1), _lib.ptr(p1), _lib.ptr(p2), _lib.ptr(e), _lib.ptr(eat), int(device))
+    _lib.check(rc, "ptmc_nnp2_energy")
+    return e, eat
+
+
 def read_runner_weights(path) -> np.ndarray:
     """A RuNNer weights.XXX.data file (first column) as the flat parameter vector."""
     vals = []
