@@ -49,12 +49,20 @@ class orc_nnp(C.Structure):
                 ("n_params", C.c_int32), ("params", C.POINTER(C.c_double))]
 
 
+class orc_nnp2(C.Structure):
+    _fields_ = [("n_rad", C.c_int32), ("n_ang", C.c_int32), ("rad", C.POINTER(C.c_double)),
+                ("ang", C.POINTER(C.c_double)), ("r_cut", C.c_double), ("n_layers", C.c_int32), ("nodes", _i32p),
+                ("act", _i32p), ("n_params", C.c_int32), ("params1", C.POINTER(C.c_double)),
+                ("params2", C.POINTER(C.c_double))]
+
+
 class orc_pot(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ncoeff", C.c_int32), ("coeff", C.c_double * 16),
                 ("eam_n", C.c_double), ("eam_m", C.c_double), ("eam_ean", C.c_double), ("eam_ecam", C.c_double),
                 ("nnp", C.POINTER(orc_nnp)), ("coeff_a", C.c_double * 8), ("coeff_b", C.c_double * 8),
                 ("lut_ndist", C.c_int32), ("lut_nangle", C.c_int32), ("lut_start", C.c_double), ("lut_d", C.c_double),
-                ("lut_dangle", C.c_double), ("lut_c6", C.c_double), ("lut", C.POINTER(C.c_double))]
+                ("lut_dangle", C.c_double), ("lut_c6", C.c_double), ("lut", C.POINTER(C.c_double)),
+                ("nnp2", C.POINTER(orc_nnp2)), ("nnvt_n1", C.c_int32), ("nnvt_swaps", C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -124,6 +132,8 @@ def lib():
         _L.orc_nnp2_full_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, _dp]
         _L.orc_nnp2_energy.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, _dp]
         _L.orc_nnp2_energy.restype = C.c_double
+        _L.orc_nnp2_move_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        _L.orc_nnp2_swap_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
     return _L
 
 
@@ -139,12 +149,19 @@ def _p(a, t=C.c_double):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
 
-def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None, coeff_a=(), coeff_b=(), lut=None) -> orc_pot:
+def make_pot(kind: str, coeff=(), eam=None, nnp: NNP | None = None, coeff_a=(), coeff_b=(), lut=None,
+             nnp2: "NNP2 | None" = None, n1: int = 0, n_swaps: int = 1) -> orc_pot:
     """kind "eljb": ELJPotentialB with c = coeff, a = coeff_a, b = coeff_b;
     kind "lut": LookupTablePotential from lut = (table [n_angle][n_dist],
-    start_dist, d_dist, d_angle, c6coeff)."""
+    start_dist, d_dist, d_angle, c6coeff); kind "nnp2": RuNNerPotential2Atom
+    (nnp2) in the NNVT ensemble, atoms 0..n1-1 species 1, n_swaps atom swaps
+    per cycle."""
     p = orc_pot()
-    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3, "eljb": 4, "lut": 5}[kind]
+    p.kind = {"elj_even": 0, "elj": 1, "eam": 2, "nnp": 3, "eljb": 4, "lut": 5, "nnp2": 6}[kind]
+    if nnp2 is not None:
+        p.nnp2 = C.pointer(nnp2.s)
+        p._nnp2_keep = nnp2
+        p.nnvt_n1, p.nnvt_swaps = int(n1), int(n_swaps)
     if lut is not None:
         table, start, d, dangle, c6 = lut
         p._lut_keep = np.ascontiguousarray(table, dtype=np.float64)
@@ -203,13 +220,6 @@ class NNP:
         return tot, e
 
 
-class orc_nnp2(C.Structure):
-    _fields_ = [("n_rad", C.c_int32), ("n_ang", C.c_int32), ("rad", C.POINTER(C.c_double)),
-                ("ang", C.POINTER(C.c_double)), ("r_cut", C.c_double), ("n_layers", C.c_int32), ("nodes", _i32p),
-                ("act", _i32p), ("n_params", C.c_int32), ("params1", C.POINTER(C.c_double)),
-                ("params2", C.POINTER(C.c_double))]
-
-
 class NNP2:
     """An orc_nnp2 built from a RuNNerPotential2Atom, arrays kept alive."""
 
@@ -236,6 +246,25 @@ class NNP2:
         e = np.zeros(n1 + n2)
         tot = lib().orc_nnp2_energy(C.byref(self.s), _p(g), n1, n2, _p(e))
         return tot, e
+
+    def move_g(self, x, n1, n2, i, trial, g):
+        """Features after moving atom i to trial (the reference's delta update)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        t = np.ascontiguousarray(trial, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        gn = np.zeros_like(g)
+        if lib().orc_nnp2_move_g(C.byref(self.s), _p(x), n1, n2, int(i), _p(t), _p(g), _p(gn)):
+            raise RuntimeError("orc_nnp2_move_g failed")
+        return gn
+
+    def swap_g(self, x, n1, n2, i1, i2, g):
+        """Features after atoms i1 (species 1) and i2 (species 2) trade places."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        gn = np.zeros_like(g)
+        if lib().orc_nnp2_swap_g(C.byref(self.s), _p(x), n1, n2, int(i1), int(i2), _p(g), _p(gn)):
+            raise RuntimeError("orc_nnp2_swap_g failed")
+        return gn
 
 
 def forward(inp, nodes, act, params):
@@ -316,9 +345,9 @@ class OracleRun:
         self.hist = np.zeros((R, nb2 * nb2 if ensemble == "npt" else nb2))
         self.d2 = np.zeros((R, N, N)) if mode["dist_cache"] else None
         self.comp = np.zeros((R, N, 2)) if pot.kind == 2 else None
-        nf = pot._nnp_keep.n_features if pot.kind == 3 else 0
-        self.g = np.zeros((R, N, nf)) if pot.kind == 3 else None
-        self.eat = np.zeros((R, N)) if pot.kind == 3 else None
+        nf = pot._nnp_keep.n_features if pot.kind == 3 else (pot._nnp2_keep.n_features if pot.kind == 6 else 0)
+        self.g = np.zeros((R, N, nf)) if pot.kind in (3, 6) else None
+        self.eat = np.zeros((R, N)) if pot.kind in (3, 6) else None
         self.error = np.zeros(1, dtype=np.int32)
         self.trace_cycles = trace_cycles
         self.sweep = 0
@@ -352,7 +381,7 @@ class OracleRun:
             return None
         self.p.sweep0 = self.sweep
         self.p.cycle0 = self.phase
-        ns = self.N + (1 if self.p.ensemble == 1 else 0)
+        ns = self.N + (1 if self.p.ensemble == 1 else (self.pot.nnvt_swaps if self.pot.kind == 6 else 0))
         tr = None
         if trace or replay_acc is not None:
             tr = dict(de=np.zeros((self.R, n, ns)), acc=np.zeros((self.R, n, ns), dtype=np.int8),

@@ -331,3 +331,263 @@ double orc_nnp2_energy(const orc_nnp2 *nn, const double *g, int n1, int n2, doub
     for (int a = 0; a < n1 + n2; ++a) tot += eatom[a];
     return tot;
 }
+
+/* ---------------------------------------------------------------------------
+ * Two-species incremental updates.  Both return g_new = g + delta with the
+ * delta built in the reference's loop order, scaled per centre species after
+ * the loops, as the reference does; r^2 and the cutoffs come from the
+ * positions (the reference's cached dist2_mat / f_matrix hold the same
+ * values: a squared difference does not depend on the operand order). */
+
+/* exponential_part(eta, r2, f) (SymmFunc.jl:187) */
+static double expo1(double eta, double r2, double f) { return exp(-eta * r2) * f; }
+
+/* symmfunc_calc of three given cosines (SymmFunc.jl:184-202) */
+static void sym3(const double th[3], double r2ij, double r2ik, double r2jk, double fij, double fik, double fjk,
+                 double eta, double lam, double zeta, double out[3])
+{
+    const double e = exp(-eta * (r2ij + r2ik + r2jk)) * fij * fik * fjk;
+    for (int c = 0; c < 3; ++c) out[c] = e * pow(1.0 + lam * th[c], zeta);
+}
+
+/* calc_one_symm_val(position1, position2, position3, ...) (SymmFunc.jl:233-241) */
+static void one3(const double *a, const double *b, const double *c, double r2ab, double r2ac, double r2bc,
+                 double fab, double fac, double fbc, double eta, double lam, double zeta, double out[3])
+{
+    double th[3];
+    angles(a, b, c, r2ab, r2ac, r2bc, th);
+    sym3(th, r2ab, r2ac, r2bc, fab, fac, fbc, eta, lam, zeta, out);
+}
+
+/* calc_one_symm_val(r2, f, eta) (SymmFunc.jl:229-231): zero at f = 0 or 1 */
+static double one1(double r2, double f, double eta) { return (f != 0.0 && f != 1.0) ? f * exp(-eta * r2) : 0.0; }
+
+static int pair_tables(const orc_nnp2 *nn, const double *x, int N, double **r2, double **f)
+{
+    *r2 = (double *)malloc(sizeof(double) * N * N);
+    *f = (double *)malloc(sizeof(double) * N * N);
+    if (!*r2 || !*f) return -1;
+    for (int a = 0; a < N; ++a)
+        for (int b = 0; b < N; ++b) {
+            (*r2)[a * N + b] = d2(&x[3 * a], &x[3 * b]);
+            (*f)[a * N + b] = orc_cutoff(sqrt((*r2)[a * N + b]), nn->r_cut);
+        }
+    return 0;
+}
+
+/* the per-species scaling of a delta block (DeltaMatrix.jl:110-116, :208-215) */
+static void scale_rows(double *dg, int Ng, int n1, int N, int r0, int nrow, const double *s1, const double *s2)
+{
+    for (int a = 0; a < N; ++a)
+        for (int k = 0; k < nrow; ++k) dg[(size_t)a * Ng + r0 + k] *= a < n1 ? s1[k] : s2[k];
+}
+
+/* get_new_state_vars!(trial_pos, atomindex, ...) (src/EnergyEvaluation.jl:1351-1386)
+ * with calc_delta_matrix (DeltaMatrix.jl:219-262): features after moving atom
+ * i to trial, through calc_delta_symm_val! (radial :69-117, angular :119-217). */
+int orc_nnp2_move_g(const orc_nnp2 *nn, const double *x, int n1, int n2, int i, const double *trial,
+                    const double *g, double *g_new)
+{
+    const int N = n1 + n2, Ng = 2 * nn->n_rad + 3 * nn->n_ang;
+    double *r2, *f;
+    if (pair_tables(nn, x, N, &r2, &f)) return -1;
+    double *rn = (double *)malloc(sizeof(double) * N), *fn = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)calloc((size_t)Ng * N, sizeof(double));
+    if (!rn || !fn || !dg) return -1;
+    for (int j = 0; j < N; ++j) {
+        rn[j] = j == i ? 0.0 : d2(trial, &x[3 * j]);
+        fn[j] = orc_cutoff(sqrt(rn[j]), nn->r_cut);
+    }
+#define DG(row, atom) dg[(size_t)(atom) * Ng + (row)]
+    for (int q = 0; q < nn->n_rad; ++q) {
+        const int r0 = 2 * q, ind = i < n1 ? 0 : 1;
+        const double eta = RAD2(q, 0);
+        for (int j = 0; j < n1; ++j) {
+            if (j == i) continue;
+            const double gn = expo1(eta, rn[j], fn[j]) - expo1(eta, r2[i * N + j], f[i * N + j]);
+            DG(r0, i) += gn;
+            DG(r0 + ind, j) = gn;
+        }
+        for (int j = n1; j < N; ++j) {
+            if (j == i) continue;
+            const double gn = expo1(eta, rn[j], fn[j]) - expo1(eta, r2[i * N + j], f[i * N + j]);
+            DG(r0 + 1, i) += gn;
+            DG(r0 + ind, j) += gn;
+        }
+        const double s1[2] = {RAD2(q, 1), RAD2(q, 2)}, s2[2] = {RAD2(q, 3), RAD2(q, 4)};
+        scale_rows(dg, Ng, n1, N, r0, 2, s1, s2);
+    }
+    for (int q = 0; q < nn->n_ang; ++q) {
+        const int r0 = 2 * nn->n_rad + 3 * q;
+        const double eta = ANG2(q, 0), lam = ANG2(q, 1), zeta = ANG2(q, 2);
+        int iv[4] = {0, 1, 1, 0};
+        if (i >= n1)
+            for (int k = 0; k < 4; ++k) iv[k] += 1;
+        /* the three blocks of partner pairs: (11), (22), (12) */
+        for (int blk = 0; blk < 3; ++blk) {
+            const int jlo = blk == 1 ? n1 : 0, jhi = blk == 1 ? N : n1;
+            const int ci = blk == 0 ? 0 : (blk == 1 ? 2 : 1);
+            const int rj = blk == 0 ? iv[0] : (blk == 1 ? iv[1] : iv[2]);
+            const int rk = blk == 0 ? iv[0] : (blk == 1 ? iv[1] : iv[3]);
+            for (int j = jlo; j < jhi; ++j) {
+                if (j == i) continue;
+                const int klo = blk == 2 ? n1 : j + 1, khi = blk == 0 ? n1 : N;
+                for (int k = klo; k < khi; ++k) {
+                    if (k == i) continue;
+                    /* new_angular_symm_vals (DeltaMatrix.jl:30-61) */
+                    double tn[3], to[3], vn[3], vo[3];
+                    angles(trial, &x[3 * j], &x[3 * k], rn[j], rn[k], r2[j * N + k], tn);
+                    angles(&x[3 * i], &x[3 * j], &x[3 * k], r2[i * N + j], r2[i * N + k], r2[j * N + k], to);
+                    for (int c = 0; c < 3; ++c) {
+                        vo[c] = pow(1.0 + lam * to[c], zeta);
+                        vn[c] = pow(1.0 + lam * tn[c], zeta);
+                    }
+                    const double en = exp(-eta * (rn[j] + rn[k] + r2[j * N + k])) * fn[j] * fn[k] * f[j * N + k];
+                    const double eo = exp(-eta * (r2[i * N + j] + r2[i * N + k] + r2[j * N + k])) * f[i * N + j] *
+                                      f[i * N + k] * f[j * N + k];
+                    DG(r0 + ci, i) += vn[0] * en - vo[0] * eo;
+                    DG(r0 + rj, j) += vn[1] * en - vo[1] * eo;
+                    DG(r0 + rk, k) += vn[2] * en - vo[2] * eo;
+                }
+            }
+        }
+        const double s1[3] = {ANG2(q, 3), ANG2(q, 4), ANG2(q, 5)}, s2[3] = {ANG2(q, 6), ANG2(q, 7), ANG2(q, 8)};
+        scale_rows(dg, Ng, n1, N, r0, 3, s1, s2);
+    }
+#undef DG
+    for (size_t e = 0; e < (size_t)Ng * N; ++e) g_new[e] = g[e] + dg[e];
+    free(r2);
+    free(f);
+    free(rn);
+    free(fn);
+    free(dg);
+    return 0;
+}
+
+/* get_new_state_vars!(indices, ...) (src/EnergyEvaluation.jl:1387-1411) with
+ * calc_swap_matrix (DeltaMatrix.jl:583-624): features after atoms i1
+ * (species 1) and i2 (species 2) trade places, through calc_swap_symm_val
+ * (radial :322-371, angular :373-578).  Column i1 then describes a species-1
+ * atom at i2's old position and column i2 a species-2 atom at i1's. */
+int orc_nnp2_swap_g(const orc_nnp2 *nn, const double *x, int n1, int n2, int i1, int i2, const double *g,
+                    double *g_new)
+{
+    const int N = n1 + n2, Ng = 2 * nn->n_rad + 3 * nn->n_ang;
+    if (i1 < 0 || i1 >= n1 || i2 < n1 || i2 >= N) return -2;
+    double *r2, *f;
+    if (pair_tables(nn, x, N, &r2, &f)) return -1;
+    double *dg = (double *)calloc((size_t)Ng * N, sizeof(double));
+    if (!dg) return -1;
+#define DG(row, atom) dg[(size_t)(atom) * Ng + (row)]
+#define R2(a, b) r2[(a) * N + (b)]
+#define FC(a, b) f[(a) * N + (b)]
+#define P(a) (&x[3 * (a)])
+    for (int q = 0; q < nn->n_rad; ++q) {
+        const int r0 = 2 * q;
+        const double eta = RAD2(q, 0);
+        for (int idx = 0; idx < n1; ++idx) {
+            if (idx == i1) continue;
+            const double v1 = one1(R2(i1, idx), FC(i1, idx), eta), v2 = one1(R2(i2, idx), FC(i2, idx), eta);
+            DG(r0, idx) += v2 - v1;
+            DG(r0 + 1, idx) += v1 - v2;
+            DG(r0, i1) += v2 - v1;
+            DG(r0, i2) += v1 - v2;
+        }
+        for (int idx = n1; idx < N; ++idx) {
+            if (idx == i2) continue;
+            const double v1 = one1(R2(i1, idx), FC(i1, idx), eta), v2 = one1(R2(i2, idx), FC(i2, idx), eta);
+            DG(r0, idx) += v2 - v1;
+            DG(r0 + 1, idx) += v1 - v2;
+            DG(r0 + 1, i1) += v2 - v1;
+            DG(r0 + 1, i2) += v1 - v2;
+        }
+        const double s1[2] = {RAD2(q, 1), RAD2(q, 2)}, s2[2] = {RAD2(q, 3), RAD2(q, 4)};
+        scale_rows(dg, Ng, n1, N, r0, 2, s1, s2);
+    }
+    for (int q = 0; q < nn->n_ang; ++q) {
+        const int r0 = 2 * nn->n_rad + 3 * q;
+        const double eta = ANG2(q, 0), lam = ANG2(q, 1), zeta = ANG2(q, 2);
+        double a[3], b[3];
+        /* both partners species 1 */
+        for (int j = 0; j < n1; ++j) {
+            if (j == i1) continue;
+            for (int k = j + 1; k < n1; ++k) {
+                if (k == i1) continue;
+                one3(P(i1), P(j), P(k), R2(i1, j), R2(i1, k), R2(j, k), FC(i1, j), FC(i1, k), FC(j, k), eta, lam,
+                     zeta, a);
+                one3(P(i2), P(j), P(k), R2(i2, j), R2(i2, k), R2(j, k), FC(i2, j), FC(i2, k), FC(j, k), eta, lam,
+                     zeta, b);
+                DG(r0, i1) += b[0] - a[0];
+                DG(r0, i2) += a[0] - b[0];
+                DG(r0, j) += b[1] - a[1];
+                DG(r0, k) += b[2] - a[2];
+                DG(r0 + 1, j) += a[1] - b[1];
+                DG(r0 + 1, k) += a[2] - b[2];
+            }
+        }
+        /* both partners species 2 */
+        for (int j = n1; j < N; ++j) {
+            if (j == i2) continue;
+            for (int k = j + 1; k < N; ++k) {
+                if (k == i2) continue;
+                one3(P(i1), P(j), P(k), R2(i1, j), R2(i1, k), R2(j, k), FC(i1, j), FC(i1, k), FC(j, k), eta, lam,
+                     zeta, a);
+                one3(P(i2), P(j), P(k), R2(i2, j), R2(i2, k), R2(j, k), FC(i2, j), FC(i2, k), FC(j, k), eta, lam,
+                     zeta, b);
+                DG(r0 + 2, i1) += b[0] - a[0];
+                DG(r0 + 2, i2) += a[0] - b[0];
+                DG(r0 + 1, j) += b[1] - a[1];
+                DG(r0 + 1, k) += b[2] - a[2];
+                DG(r0 + 2, j) += a[1] - b[1];
+                DG(r0 + 2, k) += a[2] - b[2];
+            }
+        }
+        /* one partner of each species */
+        for (int j = 0; j < n1; ++j) {
+            if (j == i1) continue;
+            for (int k = n1; k < N; ++k) {
+                if (k == i2) continue;
+                one3(P(i1), P(j), P(k), R2(i1, j), R2(i1, k), R2(j, k), FC(i1, j), FC(i1, k), FC(j, k), eta, lam,
+                     zeta, a);
+                one3(P(i2), P(j), P(k), R2(i2, j), R2(i2, k), R2(j, k), FC(i2, j), FC(i2, k), FC(j, k), eta, lam,
+                     zeta, b);
+                DG(r0 + 1, i1) += b[0] - a[0];
+                DG(r0 + 1, i2) += a[0] - b[0];
+                DG(r0 + 1, j) += b[1] - a[1];
+                DG(r0, k) += b[2] - a[2];
+                DG(r0 + 2, j) += a[1] - b[1];
+                DG(r0 + 1, k) += a[2] - b[2];
+            }
+        }
+        /* triangles holding both swapped atoms: the third of species 1, then 2 */
+        for (int k = 0; k < n1; ++k) {
+            if (k == i1) continue;
+            one3(P(i1), P(i2), P(k), R2(i1, i2), R2(i1, k), R2(i2, k), FC(i1, i2), FC(i1, k), FC(i2, k), eta, lam,
+                 zeta, a);
+            DG(r0 + 1, i1) -= a[0];
+            DG(r0, i2) += a[0];
+            DG(r0, i2) -= a[1];
+            DG(r0 + 1, i1) += a[1];
+        }
+        for (int k = n1; k < N; ++k) {
+            if (k == i2) continue;
+            one3(P(i1), P(i2), P(k), R2(i1, i2), R2(i1, k), R2(i2, k), FC(i1, i2), FC(i1, k), FC(i2, k), eta, lam,
+                 zeta, a);
+            DG(r0 + 2, i1) -= a[0];
+            DG(r0 + 1, i2) += a[0];
+            DG(r0 + 1, i2) -= a[1];
+            DG(r0 + 2, i1) += a[1];
+        }
+        const double s1[3] = {ANG2(q, 3), ANG2(q, 4), ANG2(q, 5)}, s2[3] = {ANG2(q, 6), ANG2(q, 7), ANG2(q, 8)};
+        scale_rows(dg, Ng, n1, N, r0, 3, s1, s2);
+    }
+#undef DG
+#undef R2
+#undef FC
+#undef P
+    for (size_t e = 0; e < (size_t)Ng * N; ++e) g_new[e] = g[e] + dg[e];
+    free(r2);
+    free(f);
+    free(dg);
+    return 0;
+}

@@ -406,6 +406,12 @@ int orc_init_states(const orc_params *p, const orc_pot *pot, orc_state *s)
             const int nf = pot->nnp->n_rad + pot->nnp->n_ang;
             if (orc_nnp_full_g(pot->nnp, x, N, &s->g[(size_t)r * N * nf])) return -1;
             s->en_tot[r] = orc_nnp_energy(pot->nnp, &s->g[(size_t)r * N * nf], N, &s->eat[(size_t)r * N]);
+        } else if (pot->kind == ORC_POT_NNP2) {
+            /* set_variables + initialise_energy (RuNNerPotential2Atom),
+             * src/EnergyEvaluation.jl:1824-1856, :1748-1768 */
+            const int n1 = pot->nnvt_n1, ng = 2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang;
+            if (orc_nnp2_full_g(pot->nnp2, x, n1, N - n1, &s->g[(size_t)r * N * ng])) return -1;
+            s->en_tot[r] = orc_nnp2_energy(pot->nnp2, &s->g[(size_t)r * N * ng], n1, N - n1, &s->eat[(size_t)r * N]);
         } else if (pot->kind == ORC_POT_EAM) {
             eam_build(p, pot, x, box, &s->comp[(size_t)r * N * 2]);
             s->en_tot[r] = eam_energy_from_comp(pot, &s->comp[(size_t)r * N * 2], N,
@@ -553,6 +559,14 @@ static void atom_move_energy(const orc_params *p, const orc_pot *pot, const orc_
         return;
     }
 
+    if (pot->kind == ORC_POT_NNP2) {
+        /* energy_update!(RuNNerPotential2Atom), src/EnergyEvaluation.jl:1622-1643 */
+        const int n1 = pot->nnvt_n1, ng = 2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang;
+        orc_nnp2_move_g(pot->nnp2, x, n1, N - n1, i, trial, &s->g[(size_t)r * N * ng], o->g_new);
+        o->new_en = orc_nnp2_energy(pot->nnp2, o->g_new, n1, N - n1, o->eat_new);
+        return;
+    }
+
     if (pot->kind == ORC_POT_EAM) {
         const double *comp = &s->comp[(size_t)r * N * 2];
         if (p->math_lib) {
@@ -649,6 +663,10 @@ static void swap_states(const orc_params *p, const orc_pot *pot, orc_state *s, i
         SWAPN(s->g, (size_t)N * (pot->nnp->n_rad + pot->nnp->n_ang));
         SWAPN(s->eat, (size_t)N);
     }
+    if (pot->kind == ORC_POT_NNP2) {
+        SWAPN(s->g, (size_t)N * (2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang));
+        SWAPN(s->eat, (size_t)N);
+    }
 #undef SWAPN
 }
 
@@ -689,7 +707,8 @@ static int hist_index(double hi, int n_bin)
 int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_t n_sweeps)
 {
     const int N = p->n_atoms, R = p->n_temps;
-    const uint32_t nsteps = (uint32_t)(p->ensemble == ORC_NPT ? N + 1 : N);
+    const int nnvt = pot->kind == ORC_POT_NNP2;
+    const uint32_t nsteps = (uint32_t)(p->ensemble == ORC_NPT ? N + 1 : (nnvt ? N + pot->nnvt_swaps : N));
     trial_out *o = (trial_out *)calloc(1, sizeof(trial_out));
     double *vol_x = (double *)malloc(sizeof(double) * 3 * N);
     double *vol_d2 = (p->separated && s->d2) ? (double *)malloc(sizeof(double) * N * N) : NULL;
@@ -701,6 +720,14 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
     if (pot->kind == ORC_POT_NNP) {
         if (p->ensemble != ORC_NVT) return -3;
         o->g_new = (double *)malloc(sizeof(double) * N * (pot->nnp->n_rad + pot->nnp->n_ang));
+        o->eat_new = (double *)malloc(sizeof(double) * N);
+    }
+    if (nnvt) {
+        const int n1 = pot->nnvt_n1;
+        if (p->ensemble != ORC_NVT || n1 < 0 || n1 > N || pot->nnvt_swaps < 0 ||
+            (pot->nnvt_swaps > 0 && (n1 < 1 || n1 == N)))
+            return -3;
+        o->g_new = (double *)malloc(sizeof(double) * N * (2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang));
         o->eat_new = (double *)malloc(sizeof(double) * N);
     }
     if (s->error) *s->error = 0;
@@ -764,6 +791,11 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         }
                         if (pot->kind == ORC_POT_EAM)
                             memcpy(&s->comp[(size_t)r * N * 2], o->new_comp, sizeof(double) * 2 * N);
+                        if (nnvt) {
+                            const size_t ng = (size_t)(2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang);
+                            memcpy(&s->g[(size_t)r * N * ng], o->g_new, sizeof(double) * N * ng);
+                            memcpy(&s->eat[(size_t)r * N], o->eat_new, sizeof(double) * N);
+                        }
                         if (pot->kind == ORC_POT_NNP && new_en != 100.0) {
                             /* swap_vars!(NNPVariables), src/swap_config.jl:213-221 */
                             const size_t nf = (size_t)(pot->nnp->n_rad + pot->nnp->n_ang);
@@ -773,6 +805,49 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         s->en_tot[r] = new_en;
                         s->count_atom[2 * r] += 1;
                         s->count_atom[2 * r + 1] += 1;
+                    }
+                } else if (nnvt) {
+                    /* atom swap: swap_atoms (src/MCMoves.jl:363-370) picks one
+                     * atom of each species, swap_energy_update
+                     * (src/EnergyEvaluation.jl:1438-1448), Metropolis on the
+                     * energy change (src/Exchange.jl:82-83), swap_move_config!
+                     * (src/swap_config.jl:238-277) on acceptance */
+                    const int n1 = pot->nnvt_n1, ng = 2 * pot->nnp2->n_rad + 3 * pot->nnp2->n_ang;
+                    orc_u32x4 S = orc_draw(p->seed, p->run, ORC_T_MOVE, 254u, r, sweep, m);
+                    const int i1 = (int)orc_below(S.v[0], (uint32_t)n1);
+                    const int i2 = n1 + (int)orc_below(S.v[1], (uint32_t)(N - n1));
+                    orc_nnp2_swap_g(pot->nnp2, x, n1, N - n1, i1, i2, &s->g[(size_t)r * N * ng], o->g_new);
+                    new_en = orc_nnp2_energy(pot->nnp2, o->g_new, n1, N - n1, o->eat_new);
+                    const double dE = new_en - s->en_tot[r];
+                    double prob = xexp(p, -dE * s->beta[r]);
+                    if (prob > 1.0) prob = 1.0;
+                    accept = prob >= d0.umet;
+                    if (s->trace_de) s->trace_de[tix] = dE;
+                    if (s->trace_acc) s->trace_acc[tix] = (int8_t)accept;
+                    if (s->trace_idx) s->trace_idx[tix] = N + i1 * N + i2;
+                    s->count_vol[2 * r + 1] += 1;
+                    if (accept) {
+                        for (int k = 0; k < 3; ++k) {
+                            const double t_ = x[3 * i1 + k];
+                            x[3 * i1 + k] = x[3 * i2 + k];
+                            x[3 * i2 + k] = t_;
+                        }
+                        if (s->d2) {
+                            double *d2 = &s->d2[(size_t)r * N * N];
+                            for (int pass = 0; pass < 2; ++pass)  /* rows, then columns */
+                                for (int j = 0; j < N; ++j) {
+                                    double *u = pass ? &d2[(size_t)j * N + i1] : &d2[(size_t)i1 * N + j];
+                                    double *v = pass ? &d2[(size_t)j * N + i2] : &d2[(size_t)i2 * N + j];
+                                    const double t_ = *u;
+                                    *u = *v;
+                                    *v = t_;
+                                }
+                            d2[(size_t)i1 * N + i1] = d2[(size_t)i2 * N + i2] = 0.0;
+                        }
+                        memcpy(&s->g[(size_t)r * N * ng], o->g_new, sizeof(double) * N * ng);
+                        memcpy(&s->eat[(size_t)r * N], o->eat_new, sizeof(double) * N);
+                        s->en_tot[r] = new_en;
+                        s->count_vol[2 * r] += 1;
                     }
                 } else {
                     /* volume move: uniform, src/MCMoves.jl:169-177, :277-287, or

@@ -18,7 +18,7 @@ extern "C" {
 
 enum { ORC_NVT = 0, ORC_NPT = 1 };
 enum { ORC_SPHERE = 0, ORC_CUBIC = 1, ORC_RECT = 2, ORC_RHOMB = 3 };
-enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3, ORC_POT_ELJB = 4, ORC_POT_LUT = 5 };
+enum { ORC_POT_ELJ_EVEN = 0, ORC_POT_ELJ = 1, ORC_POT_EAM = 2, ORC_POT_NNP = 3, ORC_POT_ELJB = 4, ORC_POT_LUT = 5, ORC_POT_NNP2 = 6 };
 /* exchange_mode: 0 = reference (p = 0.1 per cycle, one random adjacent pair),
  *                1 = one random adjacent pair every cycle,
  *                2 = all even (cycle even) / odd (cycle odd) pairs every cycle */
@@ -67,6 +67,23 @@ typedef struct {
     const double *params;
 } orc_nnp;
 
+/* Two-species network potential (RuNNerPotential2Atom).  rad: n_rad rows
+ * (eta, norm[4], offset[4]) of RadialType2a (norm / offset order: species 1
+ * rows 1, 2, species 2 rows 1, 2); ang: n_ang rows (eta, lambda, zeta,
+ * tpz[6], offset[6]) of AngularType3a.  params1 / params2: the networks of
+ * species 1 / 2 (same layout). */
+typedef struct {
+    int32_t n_rad, n_ang;
+    const double *rad;
+    const double *ang;
+    double  r_cut;
+    int32_t n_layers;
+    const int32_t *nodes;
+    const int32_t *act;
+    int32_t n_params;
+    const double *params1, *params2;
+} orc_nnp2;
+
 typedef struct {
     int32_t kind;
     int32_t ncoeff;
@@ -79,6 +96,11 @@ typedef struct {
     int32_t lut_ndist, lut_nangle;
     double  lut_start, lut_d, lut_dangle, lut_c6;
     const double *lut;
+    /* RuNNerPotential2Atom in the NNVT ensemble (src/Ensembles.jl:138-150):
+     * atoms 0..nnvt_n1-1 are species 1, the rest species 2; a cycle holds
+     * N atom moves and nnvt_swaps atom swaps (src/Ensembles.jl:270-282) */
+    const orc_nnp2 *nnp2;
+    int32_t nnvt_n1, nnvt_swaps;
 } orc_pot;
 
 typedef struct {
@@ -160,26 +182,17 @@ int    orc_nnp_move_g(const orc_nnp *nn, const double *x, int n_atoms, int i, co
 /* sum of the per-atom network energies of a feature matrix */
 double orc_nnp_energy(const orc_nnp *nn, const double *g, int n_atoms, double *eatom);
 
-/* Two-species network potential (RuNNerPotential2Atom).  rad: n_rad rows
- * (eta, norm[4], offset[4]) of RadialType2a (norm / offset order: species 1
- * rows 1, 2, species 2 rows 1, 2); ang: n_ang rows (eta, lambda, zeta,
- * tpz[6], offset[6]) of AngularType3a.  params1 / params2: the networks of
- * species 1 / 2 (same layout). */
-typedef struct {
-    int32_t n_rad, n_ang;
-    const double *rad;
-    const double *ang;
-    double  r_cut;
-    int32_t n_layers;
-    const int32_t *nodes;
-    const int32_t *act;
-    int32_t n_params;
-    const double *params1, *params2;
-} orc_nnp2;
 /* features (2 n_rad + 3 n_ang per atom, atom-major) of atoms 0..n1-1
  * (species 1) and n1..n1+n2-1 (species 2) in a spherical container */
 int    orc_nnp2_full_g(const orc_nnp2 *nn, const double *x, int n1, int n2, double *g);
 double orc_nnp2_energy(const orc_nnp2 *nn, const double *g, int n1, int n2, double *eatom);
+/* calc_delta_matrix (DeltaMatrix.jl:219-262): features after moving atom i to trial */
+int    orc_nnp2_move_g(const orc_nnp2 *nn, const double *x, int n1, int n2, int i, const double *trial,
+                       const double *g, double *g_new);
+/* calc_swap_matrix (DeltaMatrix.jl:583-624): features after atoms i1 (species 1)
+ * and i2 (species 2) trade places */
+int    orc_nnp2_swap_g(const orc_nnp2 *nn, const double *x, int n1, int n2, int i1, int i2, const double *g,
+                       double *g_new);
 
 #ifdef __cplusplus
 }

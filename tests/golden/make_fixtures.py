@@ -182,6 +182,31 @@ def nnp2_potentialfile(ref: pathlib.Path):
     (HERE / "nnp" / "weights.030.data").write_bytes(src.read_bytes())
 
 
+def nnvt_states(ref: pathlib.Path):
+    """The NNVT testset of test/test_states.jl:110-182: six atoms (four of
+    species 1, two of species 2) in a sphere of radius 8, the potential of
+    test/potentialfile.jl, and the energy change of swapping atoms 3 and 6
+    (1-based; the indices follow from the test's own assertions on the
+    swapped positions, :179-180), delen2 = -0.0017084901948."""
+    s = (ref / "test" / "test_states.jl").read_text()
+    s = s[s.index('@testset "States NNVT"'):]
+    s = s[: s.index("\nend")]
+    pos = [[float(v) for v in m.split(",")] for m in re.findall(r"v\d = SVector\(([^)]*)\)", s)]
+    swap = re.search(r"state\.config\[(\d+)\] == refstate\.config\[(\d+)\]", s)
+    out = dict(
+        source="test/test_states.jl:110-182 (NNVT configuration, swap indices, swap energy change)",
+        positions=pos,
+        n1=int(re.search(r"NNVT\(\[(\d+), (\d+)\]\)", s).group(1)),
+        n2=int(re.search(r"NNVT\(\[(\d+), (\d+)\]\)", s).group(2)),
+        radius=float(re.search(r"radius=([0-9.]+)", s).group(1)),
+        swap=[int(swap.group(1)), int(swap.group(2))],
+        delta_swap=float(re.search(r"delen2 ≈ (-?[0-9.eE-]+)", s).group(1)),
+        delta_move_rng=float(re.search(r"delen ≈ (-?[0-9.eE-]+)", s).group(1)),
+    )
+    assert len(pos) == 6 and out["n1"] == 4 and out["n2"] == 2 and out["swap"] == [3, 6]
+    (HERE / "nnvt_states.json").write_text(json.dumps(out, indent=1))
+
+
 def known_answers(ref: pathlib.Path):
     """Deterministic known answers of the reference's tests, with the lines
     they come from.  Each value is read from the file it cites."""
@@ -216,6 +241,7 @@ def main():
     multihist_npt32(ref)
     eljb_pbc216(ref)
     nnp2_potentialfile(ref)
+    nnvt_states(ref)
     known_answers(ref)
     print("fixtures written to", HERE)
 
