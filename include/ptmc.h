@@ -40,7 +40,10 @@ typedef struct {
     int32_t precision;         /* 0: FP64 everywhere; 1: binary32 positions and dE, FP64 totals */
     int32_t ensemble;          /* 0: NVT, 1: NPT (uniform volume moves), 2: NPT with separated
                                   volume moves, z 1/6, xy 1/3, xyz 1/2 of them (NPT(n, p, true),
-                                  src/MCMoves.jl:301-343; rectangular and rhombic boxes) */
+                                  src/MCMoves.jl:301-343; rectangular and rhombic boxes);
+                                  3: NNVT, N atom moves and n_swaps atom swaps per cycle
+                                  (NNVT, src/Ensembles.jl:138-150, :270-282; with
+                                  ptmc_set_potential_nnp2 only) */
     int32_t boundary;          /* 0: SphericalBC, 1: CubicBC, 2: RectangularBC (L, L, H),
                                   3: RhombicBC (60 degree prism, side L, height H),
                                   src/BoundaryConditions.jl:80-116 */
@@ -128,6 +131,26 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
                            double r_cut, double hardcore2, int32_t q10_compat, int32_t n_layers,
                            const int32_t *nodes, const int32_t *activations, int32_t n_params,
                            const double *params);
+
+/* Two-species RuNNer network potential in the NNVT ensemble
+ * (RuNNerPotential2Atom, src/EnergyEvaluation.jl:1287-1317; moves
+ * :1622-1643, swaps :1438-1448, src/MCMoves.jl:363-370; commit
+ * src/swap_config.jl:238-277).  Atoms 0..n1-1 of every run are species 1,
+ * n1..n1+n2-1 species 2 (every run holds n1 + n2 <= 64 atoms); a cycle
+ * draws N + n_swaps moves, each an atom displacement or, for indices
+ * beyond N, a swap of one atom of each species, accepted with
+ * min(1, exp(-beta dE)).  rad / ang: the tables of ptmc_nnp2_energy;
+ * params1 / params2: the networks of species 1 / 2, shape [2 n_rad +
+ * 3 n_ang, h1, h2, 1], activations 1 (linear) or 2 (tanh) per layer.  FP64
+ * (precision 0), spherical container.  The volume counters of
+ * ptmc_get_counters hold (swaps accepted, swaps attempted) per temperature.
+ * No counterpart of the reference for the split of a cycle's draws: the
+ * reference's move index, atom and species picks come from Julia's RNG,
+ * here from Philox (as for every other move). */
+int ptmc_set_potential_nnp2(ptmc_handle h, int32_t n1, int32_t n2, int32_t n_swaps, int32_t n_rad, int32_t n_ang,
+                            const double *rad, const double *ang, double r_cut, int32_t n_layers,
+                            const int32_t *nodes, const int32_t *activations, int32_t n_params,
+                            const double *params1, const double *params2);
 
 /* Runs.  Replaces initialisation (src/Initialization.jl:70-101) and
  * MCState construction (src/MCStates.jl:137-162): energies of the start

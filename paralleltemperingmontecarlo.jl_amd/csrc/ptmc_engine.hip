@@ -24,6 +24,8 @@ namespace ptmc {
 SweepFn nnp_sweep_fn();
 SweepFn nnp_sweep_da_fn();
 SweepFn nnp_init_fn();
+SweepFn nnp2_sweep_fn();
+SweepFn nnp2_init_fn();
 void (*nnp_cycle_fn())(SweepArgs, int);
 hipError_t sur_launch_desc(const float *x, const int32_t *n_atoms, float *bd, float r_cut, int n, hipStream_t st);
 hipError_t sur_launch_collect(const SweepArgs &a, int n_slots, float *bd, double *be, int32_t *bn, long long base,
@@ -142,6 +144,18 @@ struct ptmc_engine {
     DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_nnp_gdesc, d_nnp_wgroups, d_temp_slot;
     NnpTables nnp_dev{};
     size_t nnp_lds = 0;
+
+    // two-species network potential, NNVT (one workgroup per slot, FP64)
+    bool nnvt = false;
+    struct {
+        std::vector<double> rad, ang, p1, p2;
+        int n1 = 0, n2 = 0, n_swaps = 0, n_rad = 0, n_ang = 0, ng = 0, h1 = 0, h2 = 0;
+        int act[4] = {0, 0, 0, 0};
+        double r_cut = 0.0;
+    } nnp2;
+    DevBuf<double> d_n2_rad, d_n2_ang, d_n2_p1, d_n2_p2, d_n2_feat, d_n2_eatom;
+    Nnp2Tables nnp2_dev{};
+    size_t nnp2_lds = 0;
 
     // learned surrogate / delayed acceptance
     struct {
@@ -519,6 +533,7 @@ SweepArgs base_args(ptmc_engine *h)
     a.min_acc = h->cfg.min_acc;
     a.max_acc = h->cfg.max_acc;
     a.nnp = h->nnp_dev;
+    a.nnp2 = h->nnp2_dev;
     a.feat = h->d_feat.p;
     a.eatom = h->d_eatom.p;
     a.temp_slot = h->d_temp_slot.p;
@@ -530,6 +545,78 @@ SweepArgs base_args(ptmc_engine *h)
 }
 
 bool is_nnp(const ptmc_engine *h) { return h->pot.kind == POT_NNP; }
+bool is_nnp2(const ptmc_engine *h) { return h->pot.kind == POT_NNP2; }
+// potentials run one workgroup per slot (label exchange between cycles)
+bool is_slot(const ptmc_engine *h) { return is_nnp(h) || is_nnp2(h); }
+
+// Slots of the two-species engine: as prepare_nnp, FP64 positions
+// [slot][3][NNP_NP], features [slot][NNP_NP][ng] and atom energies in HBM.
+int prepare_nnp2(ptmc_engine *h)
+{
+    auto &n = h->nnp2;
+    if (h->cfg.precision != 0) return fail(h, PTMC_E_UNSUPPORTED, "the two-species network potential runs in FP64 (precision 0)");
+    if (h->cfg.boundary != BC_SPHERE) return fail(h, PTMC_E_UNSUPPORTED, "the two-species network potential runs in a spherical container");
+    int dev = 0, max_lds = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const int N = n.n1 + n.n2;
+    h->nnp2_lds = nnp2_lds_bytes(N, n.ng, n.n_rad + n.n_ang, n.h1, n.h2);
+    if ((long long)h->nnp2_lds > max_lds)
+        return fail(h, PTMC_E_UNSUPPORTED, "%d atoms with %d feature rows need %zu B of LDS (%d available)", N, n.ng,
+                    h->nnp2_lds, max_lds);
+    const void *fs = (const void *)nnp2_sweep_fn(), *fi = (const void *)nnp2_init_fn();
+    HIPCHK(h, hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp2_lds));
+    HIPCHK(h, hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp2_lds));
+    int slot = 0;
+    for (auto &rh : h->runs) {
+        if (rh.d.n_atoms != N)
+            return fail(h, PTMC_E_ARG, "every run holds the potential's %d + %d atoms (got %d)", n.n1, n.n2, rh.d.n_atoms);
+        rh.info.slot_base = slot;
+        rh.info.local_base = 0;
+        rh.kl = NNP2_NT;
+        rh.lanes = NNP2_NT;
+        rh.tile = -1;
+        slot += rh.d.n_temps;
+    }
+    h->n_slots = slot;
+    h->tiles.clear();
+    h->launches.clear();
+    Launch L;
+    L.fn = (SweepFn)fs;
+    L.init = (SweepFn)fi;
+    L.ntiles = slot;
+    L.wgs = NNP2_NT;
+    L.kl = NNP2_NT;
+    L.lds = h->nnp2_lds;
+    h->launches.push_back(L);
+    h->n_coord = (int64_t)slot * 3 * NNP_NP;
+    h->n_comp = 0;
+    int rc;
+    if ((rc = upload(h, h->d_n2_rad, n.rad)) || (rc = upload(h, h->d_n2_ang, n.ang)) ||
+        (rc = upload(h, h->d_n2_p1, n.p1)) || (rc = upload(h, h->d_n2_p2, n.p2)))
+        return rc;
+    if ((rc = dalloc(h, h->d_n2_feat, (size_t)slot * NNP_NP * n.ng))) return rc;
+    if ((rc = dalloc(h, h->d_n2_eatom, (size_t)slot * NNP_NP))) return rc;
+    Nnp2Tables &t = h->nnp2_dev;
+    t.n1 = n.n1;
+    t.n_swaps = n.n_swaps;
+    t.n_rad = n.n_rad;
+    t.n_ang = n.n_ang;
+    t.ng = n.ng;
+    t.h1 = n.h1;
+    t.h2 = n.h2;
+    t.act1 = n.act[1];
+    t.act2 = n.act[2];
+    t.act3 = n.act[3];
+    t.r_cut = n.r_cut;
+    t.rad = h->d_n2_rad.p;
+    t.ang = h->d_n2_ang.p;
+    t.p1 = h->d_n2_p1.p;
+    t.p2 = h->d_n2_p2.p;
+    t.feat = h->d_n2_feat.p;
+    t.eatom = h->d_n2_eatom.p;
+    return PTMC_OK;
+}
 
 // Slots of a neural-network engine: one workgroup per slot, slots of a run
 // consecutive, temperature t of run r starting in slot slot_base + t.
@@ -606,16 +693,17 @@ int prepare_nnp(ptmc_engine *h)
     return PTMC_OK;
 }
 
-void scatter_coords_nnp(ptmc_engine *h, std::vector<float> &buf)
+template <typename Real>
+void scatter_coords_nnp(ptmc_engine *h, std::vector<Real> &buf)
 {
-    buf.assign((size_t)h->n_coord, 0.0f);
+    buf.assign((size_t)h->n_coord, (Real)0);
     for (auto &rh : h->runs) {
         const int N = rh.d.n_atoms;
         for (int t = 0; t < rh.d.n_temps; ++t) {
             const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
-            float *dst = &buf[(size_t)(rh.info.slot_base + t) * 3 * NNP_NP];
+            Real *dst = &buf[(size_t)(rh.info.slot_base + t) * 3 * NNP_NP];
             for (int j = 0; j < N; ++j)
-                for (int c = 0; c < 3; ++c) dst[c * NNP_NP + j] = (float)x[3 * j + c];
+                for (int c = 0; c < 3; ++c) dst[c * NNP_NP + j] = (Real)x[3 * j + c];
         }
     }
 }
@@ -697,11 +785,11 @@ int ptmc_create(const ptmc_config *cfg, ptmc_handle *out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PTMC_E_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return PTMC_E_ARG;
-    if (cfg->precision < 0 || cfg->precision > 1 || cfg->ensemble < 0 || cfg->ensemble > 2 ||
+    if (cfg->precision < 0 || cfg->precision > 1 || cfg->ensemble < 0 || cfg->ensemble > 3 ||
         cfg->boundary < 0 || cfg->boundary > 3 || cfg->exchange_mode < 0 || cfg->exchange_mode > 2 ||
         cfg->n_adjust <= 0 || cfg->mc_sample <= 0 || cfg->n_bin < 2)
         return PTMC_E_ARG;
-    if (cfg->ensemble != ENS_NVT && cfg->boundary == BC_SPHERE) return PTMC_E_ARG;  // src/Ensembles.jl:191-193
+    if ((cfg->ensemble == 1 || cfg->ensemble == 2) && cfg->boundary == BC_SPHERE) return PTMC_E_ARG;  // src/Ensembles.jl:191-193
     // separated volume moves scale x, y and z apart: boxes with a height only
     // (scale_xy / scale_z have no CubicBC method, src/MCMoves.jl:189-225)
     if (cfg->ensemble == 2 && cfg->boundary != BC_RECT && cfg->boundary != BC_RHOMB) return PTMC_E_ARG;
@@ -709,6 +797,9 @@ int ptmc_create(const ptmc_config *cfg, ptmc_handle *out)
     h->cfg = *cfg;
     h->separated = cfg->ensemble == 2;
     if (h->separated) h->cfg.ensemble = ENS_NPT;
+    // NNVT: atom moves plus atom swaps at fixed volume (src/Ensembles.jl:138-150)
+    h->nnvt = cfg->ensemble == 3;
+    if (h->nnvt) h->cfg.ensemble = ENS_NVT;
     if (h->cfg.cycles_per_launch <= 0) h->cfg.cycles_per_launch = 256;
     if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
@@ -754,6 +845,12 @@ int ptmc_destroy(ptmc_handle h)
     h->d_nnp_gdesc.free_();
     h->d_nnp_wgroups.free_();
     h->d_temp_slot.free_();
+    h->d_n2_rad.free_();
+    h->d_n2_ang.free_();
+    h->d_n2_p1.free_();
+    h->d_n2_p2.free_();
+    h->d_n2_feat.free_();
+    h->d_n2_eatom.free_();
     h->d_sur_w.free_();
     h->d_sur_m.free_();
     h->d_sur_v.free_();
@@ -870,6 +967,58 @@ int ptmc_set_potential_eam(ptmc_handle h, double n, double m, double ean, double
     p.eam_m = m;
     p.eam_ean = ean;
     p.eam_ecam = ecam;
+    h->pot = p;
+    h->pot_set = true;
+    return PTMC_OK;
+}
+
+int ptmc_set_potential_nnp2(ptmc_handle h, int32_t n1, int32_t n2, int32_t n_swaps, int32_t n_rad, int32_t n_ang,
+                            const double *rad, const double *ang, double r_cut, int32_t n_layers, const int32_t *nodes,
+                            const int32_t *activations, int32_t n_params, const double *params1,
+                            const double *params2)
+{
+    if (!h) return PTMC_E_ARG;
+    if (h->prepared) return fail(h, PTMC_E_STATE, "potential must be set before ptmc_prepare");
+    if (!h->nnvt) return fail(h, PTMC_E_UNSUPPORTED, "the two-species network potential runs in the NNVT ensemble (3)");
+    if (n1 < 0 || n2 < 0 || n1 + n2 < 2 || n1 + n2 > NNP_NP || n_swaps < 0 || (n_swaps > 0 && (n1 < 1 || n2 < 1)))
+        return fail(h, PTMC_E_ARG, "bad species counts (%d, %d, at most %d atoms) or swaps per cycle (%d)", n1, n2,
+                    NNP_NP, n_swaps);
+    if (n_rad < 0 || n_ang < 0 || n_rad + n_ang < 1 || (n_rad && !rad) || (n_ang && !ang) || !(r_cut > 0) ||
+        !nodes || !activations || !params1 || !params2)
+        return fail(h, PTMC_E_ARG, "bad network potential description");
+    const int ng = 2 * n_rad + 3 * n_ang;
+    if (n_layers != 4 || nodes[0] != ng || nodes[3] != 1 || nodes[1] < 1 || nodes[2] < 1)
+        return fail(h, PTMC_E_UNSUPPORTED, "network shape must be [2 n_rad + 3 n_ang, h1, h2, 1]");
+    for (int l = 1; l < 4; ++l)
+        if (activations[l] != 1 && activations[l] != 2)
+            return fail(h, PTMC_E_UNSUPPORTED, "activations are 1 (linear) or 2 (tanh)");
+    const int h1 = nodes[1], h2 = nodes[2];
+    if (n_params != ng * h1 + h1 + h1 * h2 + h2 + h2 + 1)
+        return fail(h, PTMC_E_ARG, "parameter count does not match the layers");
+    for (int q = 0; q < n_ang; ++q) {
+        const double zeta = ang[(size_t)q * 15 + 2], base_lo = 1.0 - fabs(ang[(size_t)q * 15 + 1]);
+        if (!(zeta >= 0.0) || (base_lo < 0.0 && zeta != rint(zeta)))
+            return fail(h, PTMC_E_ARG, "angular function %d: zeta must be non-negative (integer when |lambda| > 1)", q);
+    }
+    auto &n = h->nnp2;
+    n.n1 = n1;
+    n.n2 = n2;
+    n.n_swaps = n_swaps;
+    n.n_rad = n_rad;
+    n.n_ang = n_ang;
+    n.ng = ng;
+    n.h1 = h1;
+    n.h2 = h2;
+    for (int l = 0; l < 4; ++l) n.act[l] = activations[l];
+    n.r_cut = r_cut;
+    n.rad.assign(rad, rad + (size_t)n_rad * 9);
+    n.ang.assign(ang, ang + (size_t)n_ang * 15);
+    if (n.rad.empty()) n.rad.assign(9, 0.0);
+    if (n.ang.empty()) n.ang.assign(15, 0.0);
+    n.p1.assign(params1, params1 + n_params);
+    n.p2.assign(params2, params2 + n_params);
+    PotParams p{};
+    p.kind = POT_NNP2;
     h->pot = p;
     h->pot_set = true;
     return PTMC_OK;
@@ -1045,9 +1194,11 @@ int ptmc_prepare(ptmc_handle h)
     if (!h->pot_set) return fail(h, PTMC_E_STATE, "no potential set");
     if (h->runs.empty()) return fail(h, PTMC_E_STATE, "no runs added");
     HIPCHK(h, hipSetDevice(h->cfg.device));
-    int rc = is_nnp(h) ? prepare_nnp(h) : choose_pair(h);
+    if (h->nnvt != is_nnp2(h))
+        return fail(h, PTMC_E_UNSUPPORTED, "the NNVT ensemble and the two-species network potential go together");
+    int rc = is_nnp(h) ? prepare_nnp(h) : (is_nnp2(h) ? prepare_nnp2(h) : choose_pair(h));
     if (rc) return rc;
-    if (!is_nnp(h)) rc = plan(h);
+    if (!is_slot(h)) rc = plan(h);
     if (rc) return rc;
     if (h->launches.size() > 1) {
         const size_t ns = std::min<size_t>(h->launches.size(), 4);
@@ -1143,6 +1294,12 @@ int ptmc_prepare(ptmc_handle h)
         std::vector<unsigned char> raw(buf.size() * sizeof(float));
         memcpy(raw.data(), buf.data(), raw.size());
         if ((rc = upload(h, h->d_coords, raw))) return rc;
+    } else if (is_nnp2(h)) {
+        std::vector<double> buf;
+        scatter_coords_nnp(h, buf);
+        std::vector<unsigned char> raw(buf.size() * sizeof(double));
+        memcpy(raw.data(), buf.data(), raw.size());
+        if ((rc = upload(h, h->d_coords, raw))) return rc;
     } else if (h->cfg.precision == 0) {
         std::vector<double> buf;
         scatter_coords<double>(h, buf);
@@ -1184,7 +1341,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
     HIPCHK(h, hipSetDevice(h->cfg.device));
     int64_t done = 0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    if (is_nnp(h)) {
+    if (is_slot(h)) {
         // one launch per cycle, then exchange / adaptation / sampling
         const bool tracing = h->trace_pending > 0;
         if (tracing && n_cycles > h->trace_cycles)
@@ -1360,6 +1517,14 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
             for (int j = 0; j < N; ++j)
                 for (int c = 0; c < 3; ++c)
                     x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * NNP_NP + j];
+    } else if (x && is_nnp2(h)) {
+        std::vector<double> blk((size_t)R * 3 * NNP_NP);
+        HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(double),
+                            blk.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int s = 0; s < R; ++s)
+            for (int j = 0; j < N; ++j)
+                for (int c = 0; c < 3; ++c)
+                    x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * NNP_NP + j];
     } else if (x) {
         const TileInfo &ti = h->tiles[rh.tile];
         const int A = ti.n_slab, kl = rh.kl;
@@ -1531,7 +1696,7 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, bl.data(), R * 8, hipMemcpyHostToDevice));
             HIPCHK(h, hipMemcpy(h->d_slot_boxh.p + rh.info.slot_base, bh.data(), R * 8, hipMemcpyHostToDevice));
         }
-        if (is_nnp(h)) {
+        if (is_slot(h)) {
             std::vector<int32_t> ts(R);
             for (int s = 0; s < R; ++s) ts[s] = rh.info.slot_base + s;
             HIPCHK(h, hipMemcpy(h->d_temp_slot.p + rh.info.temp_base, ts.data(), R * 4, hipMemcpyHostToDevice));
@@ -1544,6 +1709,13 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
                         blk[((size_t)s * 3 + c) * NNP_NP + j] = (float)x[((size_t)s * N + j) * 3 + c];
             HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float), blk.data(),
                                 blk.size() * sizeof(float), hipMemcpyHostToDevice));
+        } else if (x && is_nnp2(h)) {
+            std::vector<double> blk((size_t)R * 3 * NNP_NP, 0.0);
+            for (int s = 0; s < R; ++s)
+                for (int j = 0; j < N; ++j)
+                    for (int c = 0; c < 3; ++c) blk[((size_t)s * 3 + c) * NNP_NP + j] = x[((size_t)s * N + j) * 3 + c];
+            HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(double), blk.data(),
+                                blk.size() * sizeof(double), hipMemcpyHostToDevice));
         } else if (x) {
             const TileInfo &ti = h->tiles[rh.tile];
             const int A = ti.n_slab, kl = rh.kl;
@@ -1584,7 +1756,7 @@ int ptmc_set_trace(ptmc_handle h, int32_t n_cycles)
     if (n_cycles <= 0) return fail(h, PTMC_E_ARG, "trace length must be positive");
     int maxN = 0;
     for (auto &rh : h->runs) maxN = std::max(maxN, rh.d.n_atoms);
-    const int nsteps = h->cfg.ensemble == ENS_NPT ? maxN + 1 : maxN;
+    const int nsteps = h->cfg.ensemble == ENS_NPT ? maxN + 1 : (is_nnp2(h) ? maxN + h->nnp2.n_swaps : maxN);
     const size_t n = (size_t)h->n_temps * n_cycles * nsteps;
     for (auto &rh : h->runs)
         if (rh.d.n_atoms != maxN) return fail(h, PTMC_E_UNSUPPORTED, "trace needs equal atom counts");
@@ -1617,7 +1789,7 @@ int ptmc_plan_info(ptmc_handle h, int32_t *n_tiles, int32_t *n_launch_groups, in
 {
     if (!h) return PTMC_E_ARG;
     if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
-    if (n_tiles) *n_tiles = is_nnp(h) ? h->n_slots : (int32_t)h->tiles.size();
+    if (n_tiles) *n_tiles = is_slot(h) ? h->n_slots : (int32_t)h->tiles.size();
     if (n_launch_groups) *n_launch_groups = (int32_t)h->launches.size();
     if (n_lanes) {
         int64_t s = 0;
@@ -1667,7 +1839,7 @@ int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, in
 int ptmc_run_group(ptmc_handle h, int32_t run)
 {
     if (!h || run < 0 || run >= (int)h->runs.size() || !h->prepared) return -1;
-    if (is_nnp(h)) return 0;
+    if (is_slot(h)) return 0;
     const int t = h->runs[run].tile;
     for (size_t k = 0; k < h->launches.size(); ++k)
         if (t >= h->launches[k].tile0 && t < h->launches[k].tile0 + h->launches[k].ntiles) return (int)k;

@@ -17,6 +17,8 @@ enum PotKind : int32_t {
     POT_ELJB = 4,      // ELJ in a magnetic field (anisotropic in the angle to z),
                        //                       src/EnergyEvaluation.jl:161-182, :943-958
     POT_LUT = 5,       // tabulated over distance and angle to z, src/EnergyEvaluation.jl:211-295
+    POT_NNP2 = 6,      // two-species RuNNer network in the NNVT ensemble,
+                       // src/EnergyEvaluation.jl:1287-1433, src/Ensembles.jl:138-150
 };
 // How a pair energy is evaluated on the device.
 enum PairForm : int32_t {
@@ -87,6 +89,21 @@ struct NnpTables {
     float b3;
 };
 
+// Two-species network potential with atom swaps (one workgroup per slot,
+// FP64 throughout).  Atoms 0..n1-1 of every configuration are species 1.
+struct Nnp2Tables {
+    int32_t n1, n_swaps;       // species-1 atoms; atom swaps per cycle
+    int32_t n_rad, n_ang, ng;  // functions; feature rows 2 n_rad + 3 n_ang
+    int32_t h1, h2;            // hidden widths (input ng, output 1)
+    int32_t act1, act2, act3;  // activation of each layer: 1 linear, 2 tanh
+    double r_cut;
+    const double *rad;         // n_rad x 9: eta, norm[4], offset[4]
+    const double *ang;         // n_ang x 15: eta, lambda, zeta, tpz[6], offset[6]
+    const double *p1, *p2;     // networks of species 1 and 2 (RuNNer layout)
+    double *feat;              // per slot [NNP_NP][ng] features, atom-major
+    double *eatom;             // per slot [NNP_NP] per-atom energies
+};
+
 struct SweepArgs {
     const TileInfo *tiles;
     const RunInfo *runs;
@@ -138,6 +155,7 @@ struct SweepArgs {
     float *feat;               // per slot [nf][NNP_NP] features
     float *eatom;              // per slot [NNP_NP] per-atom energies
     int32_t *temp_slot;        // per temperature: slot holding it
+    Nnp2Tables nnp2;
     // learned surrogate and delayed acceptance
     const float *sur_w;        // surrogate parameters (f32, packed)
     int32_t da;                // 1: delayed-acceptance sweeps

@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     nnp2_energies,
     NPT,
     NVT,
+    NNVT,
     LookupTablePotential,
     Output,
     PTEngine,

@@ -92,6 +92,8 @@ SIGNATURES = {
     "ptmc_set_potential_eam": [_h, C.c_double, C.c_double, C.c_double, C.c_double],
     "ptmc_set_potential_eljb": [_h, C.c_int32, _dp, _dp, _dp],
     "ptmc_set_potential_lookup": [_h, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _dp],
+    "ptmc_set_potential_nnp2": [_h, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, C.c_double,
+                                C.c_int32, _i32p, _i32p, C.c_int32, _dp, _dp],
     "ptmc_set_potential_nnp": [_h, C.c_int32, C.c_int32, _dp, _dp, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                _i32p, _i32p, C.c_int32, _dp],
     "ptmc_add_run": [_h, C.POINTER(ptmc_run_desc), _i32p],

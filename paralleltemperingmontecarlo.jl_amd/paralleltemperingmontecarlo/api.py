@@ -189,6 +189,29 @@ class NPT:
 
 
 @dataclasses.dataclass(frozen=True)
+class NNVT:
+    """src/Ensembles.jl:138-150: two species, natoms = (n1, n2), atom moves
+    (default n1 + n2, the only count the engine runs) and atom swaps per
+    cycle (default 1).  Build it as the reference does with
+    NNVT.of([n1, n2], natomswaps=1)."""
+
+    natoms: tuple
+    n_atom_moves: int
+    n_atom_swaps: int = 1
+
+    @classmethod
+    def of(cls, natomsvec, natomswaps: int = 1, natommoves: int | None = None) -> "NNVT":
+        n = tuple(int(v) for v in natomsvec)
+        if len(n) != 2:
+            raise ValueError("NNVT holds two species")
+        return cls(n, sum(n) if natommoves is None else int(natommoves), int(natomswaps))
+
+    @property
+    def n_atoms(self) -> int:
+        return sum(self.natoms)
+
+
+@dataclasses.dataclass(frozen=True)
 class ELJPotentialB:
     """src/EnergyEvaluation.jl:161-182: ELJ in a magnetic field,
     e(r, t) = sum_k c_k r^-p_k (1 + a_k cos 2t + b_k cos 4t), p = 6, 8, 9, ...,
@@ -595,7 +618,8 @@ def check_e_bounds(energy: float, ebounds: list) -> list:
 def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _lib.ptmc_config:
     cfg = _lib.ptmc_config()
     cfg.precision = {"fp64": 0, "fp32": 1}[opts.precision]
-    cfg.ensemble = (2 if ensemble.separated_volume else 1) if isinstance(ensemble, NPT) else 0
+    cfg.ensemble = (2 if ensemble.separated_volume else 1) if isinstance(ensemble, NPT) else (
+        3 if isinstance(ensemble, NNVT) else 0)
     if isinstance(ensemble, NPT) and ensemble.separated_volume and not _has_height(bc):
         raise ValueError("separated volume moves need a RectangularBC or RhombicBC box")
     cfg.boundary = _bc_code(bc)
@@ -614,7 +638,23 @@ def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _l
     return cfg
 
 
-def _set_potential(eng: _lib.Engine, pot):
+def _set_potential(eng: _lib.Engine, pot, ensemble=None):
+    if isinstance(pot, RuNNerPotential2Atom):
+        if not isinstance(ensemble, NNVT):
+            raise TypeError("RuNNerPotential2Atom runs in the NNVT ensemble")
+        if tuple(ensemble.natoms) != (pot.n1, pot.n2):
+            raise ValueError(f"NNVT holds {ensemble.natoms}, the potential ({pot.n1}, {pot.n2}) atoms")
+        if ensemble.n_atom_moves != ensemble.n_atoms:
+            raise ValueError("the engine makes n1 + n2 atom moves per cycle (NNVT's default)")
+        rad, ang = (np.ascontiguousarray(t, dtype=np.float64) for t in pot.tables())
+        nodes = np.ascontiguousarray(pot.nnp1.num_nodes, dtype=np.int32)
+        act = np.ascontiguousarray(pot.nnp1.activation_functions, dtype=np.int32)
+        p1 = np.ascontiguousarray(pot.nnp1.parameters, dtype=np.float64)
+        p2 = np.ascontiguousarray(pot.nnp2.parameters, dtype=np.float64)
+        eng.call("ptmc_set_potential_nnp2", pot.n1, pot.n2, ensemble.n_atom_swaps, len(rad), len(ang), _lib.ptr(rad),
+                 _lib.ptr(ang), float(pot.r_cut), len(nodes), _lib.ptr(nodes, C.c_int32), _lib.ptr(act, C.c_int32),
+                 len(p1), _lib.ptr(p1), _lib.ptr(p2))
+        return
     if isinstance(pot, (ELJPotentialEven, ELJPotential)):
         c = np.array(pot.coeff, dtype=np.float64)
         eng.call("ptmc_set_potential_elj", 1 if isinstance(pot, ELJPotentialEven) else 0, len(c), _lib.ptr(c))
@@ -671,7 +711,7 @@ class PTEngine:
         self.pot = pot
         self.eng = _lib.Engine(_engine_config(mc_params, ensemble, bc_kind, self.opts))
         self.has_height = _has_height(bc_kind)
-        _set_potential(self.eng, pot)
+        _set_potential(self.eng, pot, ensemble)
         self.runs = []
 
     def add_run(self, n_atoms: int, beta, x0, bc, pressure: float = 0.0, lanes: int = 0,
@@ -775,7 +815,8 @@ class PTEngine:
 
     def trace(self, run: int, n_cycles: int):
         r = self.runs[run]
-        ns = r["n_atoms"] + (1 if isinstance(self.ensemble, NPT) else 0)
+        ns = r["n_atoms"] + (1 if isinstance(self.ensemble, NPT) else
+                             (self.ensemble.n_atom_swaps if isinstance(self.ensemble, NNVT) else 0))
         shape = (r["n_temps"], n_cycles, ns)
         de = np.zeros(shape)
         acc = np.zeros(shape, dtype=np.int8)

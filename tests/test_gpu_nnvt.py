@@ -1,0 +1,110 @@
+"""NNVT Monte Carlo with the two-species RuNNer potential on the device
+(PTEngine with NNVT and RuNNerPotential2Atom, through the C ABI), against the
+oracle's restatement of the reference (atom moves with calc_delta_matrix,
+atom swaps with calc_swap_matrix, MachineLearningPotential/DeltaMatrix.jl):
+the same draws give the same moves, accept decisions, positions (bit for
+bit), exchanges and counters; energy changes agree to
+1e-10 x (|E| + 1) per move and totals to 1e-11 relative; the running totals
+stay equal to a full rebuild of the final positions."""
+import json
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+import paralleltemperingmontecarlo as pt  # noqa: E402
+
+NNP = GOLDEN / "nnp"
+KB = 3.166811429e-6
+
+
+@pytest.fixture(scope="module")
+def base_pot():
+    from paralleltemperingmontecarlo.nnp_setup import reference_brass_potential
+
+    setup = json.loads((GOLDEN / "nnp2_brass.json").read_text())
+    return reference_brass_potential(setup, NNP / "scaling.data", NNP / "weights.029.data", NNP / "weights.030.data")
+
+
+def cluster(n, seed, spacing=4.6):
+    rng = np.random.default_rng(seed)
+    g = np.arange(-3, 4) * spacing
+    pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
+    pts = pts[np.argsort((pts**2).sum(1), kind="stable")][:n]
+    return pts + rng.uniform(-0.4, 0.4, pts.shape)
+
+
+def _engine(pot, ens, beta, x0, radius, seed, exchange="every", n_adjust=5):
+    mc = pt.MCParams(100, len(beta), ens.n_atoms, n_adjust=n_adjust)
+    opts = pt.EngineOptions(precision="fp64", exchange=exchange, seed=seed)
+    eng = pt.PTEngine(pot, ens, pt.SphericalBC(radius), mc, opts)
+    eng.add_run(ens.n_atoms, beta, x0[None], pt.SphericalBC(radius))
+    eng.prepare()
+    return eng
+
+
+@pytest.mark.parametrize("n1,n2,n_swaps", [(10, 4, 2), (5, 7, 3)])
+def test_nnvt_matches_oracle(base_pot, oracle, n1, n2, n_swaps):
+    pot = replace(base_pot, n1=n1, n2=n2)
+    ens = pt.NNVT.of([n1, n2], natomswaps=n_swaps)
+    N = n1 + n2
+    x0 = cluster(N, seed=N)
+    beta = 1.0 / (KB * np.array([300.0, 550.0, 900.0, 1400.0]))
+    radius, seed, cycles = 14.0, 4242 + n1, 24
+    print(f"NNVT parity n1={n1} n2={n2} swaps={n_swaps} cycles={cycles}", flush=True)
+    eng = _engine(pot, ens, beta, x0, radius, seed)
+    nn2 = oracle.NNP2(pot)
+    run = oracle.OracleRun(N, beta, x0[None], pot=oracle.make_pot("nnp2", nnp2=nn2, n1=n1, n_swaps=n_swaps),
+                           radius=radius, exchange_mode=1, mode=oracle.engine_mode(), n_adjust=5, seed=seed)
+    np.testing.assert_allclose(eng.energies(0), run.en_tot, rtol=1e-12)
+    eng.set_trace(cycles)
+    eng.cycles(cycles)
+    eng.sync()
+    de, acc, idx = eng.trace(0, cycles)
+    tr = run.cycles(cycles, trace=True)
+    np.testing.assert_array_equal(idx, tr["idx"])
+    np.testing.assert_array_equal(acc, tr["acc"])
+    scale = np.abs(run.en_tot).max() + 1.0
+    assert np.abs(de - tr["de"]).max() <= 1e-10 * scale
+    swaps = idx >= N
+    assert swaps.any() and acc[swaps].any() and acc[~swaps].any()
+    x, en, _ = eng.state(0)
+    np.testing.assert_array_equal(x, run.x)
+    np.testing.assert_allclose(en, run.en_tot, rtol=1e-11)
+    c = eng.counters(0)
+    np.testing.assert_array_equal(c["count_atom"], run.count_atom)
+    np.testing.assert_array_equal(c["count_vol"], run.count_vol)
+    np.testing.assert_array_equal(c["count_exc"], run.count_exc)
+    np.testing.assert_allclose(c["max_displ"][:, 0], run.max_displ[:, 0], rtol=0)
+    # running totals against a full rebuild of the final positions (device)
+    e_full, _ = pt.nnp2_energies(pot, x)
+    np.testing.assert_allclose(en, e_full, rtol=1e-11)
+    eng.close()
+
+
+def test_brass55_cycles(base_pot):
+    """The reference's brass cluster size (scripts/brass55.jl: NNVT([50, 5];
+    natomswaps=2), 250-900 K): cycles run, energies track a rebuild, every
+    configuration keeps 50 + 5 atoms inside the container and swaps happen."""
+    pot = replace(base_pot, n1=50, n2=5)
+    ens = pt.NNVT.of([50, 5], natomswaps=2)
+    x0 = cluster(55, seed=55)
+    beta = 1.0 / (KB * np.geomspace(250.0, 900.0, 10))
+    print("NNVT brass55 20 cycles", flush=True)
+    eng = _engine(pot, ens, beta, x0, radius=26.456, seed=1234, exchange="reference", n_adjust=100)
+    eng.cycles(20)
+    eng.sync()
+    x, en, _ = eng.state(0)
+    e_full, _ = pt.nnp2_energies(pot, x)
+    np.testing.assert_allclose(en, e_full, rtol=1e-11)
+    assert np.all((x**2).sum(-1) <= 26.456**2 + 1e-9)
+    c = eng.counters(0)
+    assert c["count_vol"][:, 1].sum() > 0 and c["count_atom"][:, 0].sum() > 0
+    eng.close()
