@@ -320,6 +320,53 @@ def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
     return out
 
 
+def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s):
+    """Brass cluster in the NNVT ensemble (scripts/brass55.jl: 50 Cu + 5 Zn,
+    NNVT([50, 5]; natomswaps=2), 10 temperatures 250-900 K, 14 A container,
+    the reference's test potential: 5 radial + 26 angular functions,
+    88-20-20-1 tanh networks weights.029 / weights.030, scaling.data).  Start:
+    the icosahedral Cu55 of test/testing_data/save.data, atoms 51-55 Zn.  FP64
+    features, networks and totals; a cycle is 55 atom moves + 2 swaps."""
+    from dataclasses import replace
+
+    from paralleltemperingmontecarlo.nnp_setup import reference_brass_potential, read_save_data
+
+    setup = json.loads((ROOT / "tests" / "golden" / "nnp2_brass.json").read_text())
+    pot = replace(reference_brass_potential(setup, NNP_DIR / "scaling.data", NNP_DIR / "weights.029.data",
+                                            NNP_DIR / "weights.030.data"), n1=50, n2=5)
+    x0 = read_save_data(NNP_DIR / "save.data")[0][0]
+    ens = pt.NNVT.of([50, 5], natomswaps=2)
+    temp = pt.TempGrid(250.0, 900.0, 10)
+    rad = 14 * A2B
+    mc = pt.MCParams(steps, 10, 55, n_adjust=100, mc_sample=1, n_bin=100)
+    eng = pt.PTEngine(pot, ens, pt.SphericalBC(rad), mc, pt.EngineOptions(precision="fp64", seed=1234))
+    for r in range(n_ladders):
+        eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
+    eng.prepare()
+    eng.kernel_time(reset=True)
+    dt = _production(eng, n_ladders, warmup, steps)
+    ms, nl = eng.kernel_time()
+    swaps = [eng.counters(i)["count_vol"] for i in range(n_ladders)]
+    out = dict(workload=f"NNVT brass Cu50Zn5, {n_ladders} ladder(s) x 10 temperatures 250-900 K",
+               moves_per_s=n_ladders * 10 * 57 * steps / dt, steps=steps, dtype="fp64",
+               acceptance=_acceptance(eng, n_ladders, 55, steps),
+               swap_acceptance=float(sum(c[:, 0].sum() for c in swaps)) / max(sum(c[:, 1].sum() for c in swaps), 1),
+               kernel_avg_launch_ms=ms / max(nl, 1), workgroups=n_ladders * 10)
+    eng.close()
+    if cpu_s > 0:
+        from oracle import oracle as orc
+
+        orc.lib()
+        nn = orc.NNP2(pot)
+        opot = orc.make_pot("nnp2", nnp2=nn, n1=50, n_swaps=2)
+        # one temperature per thread (a move costs the same at any temperature)
+        out.update(_cpu_rate(lambda k: orc.OracleRun(55, temp.beta_grid[:1], x0, pot=opot, radius=rad,
+                                                     mode=orc.REFERENCE, seed=1234, run_key=k), 57, cpu_s))
+        out["cpu_sample"] = "one 250 K replica per thread"
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_moves_per_s"]
+    return out
+
+
 def bench_delayed_acceptance(pt, n_ladders, warmup, steps, collect, epochs):
     """Delayed acceptance on NNP Cu55: the same ladders first run plain,
     then collect (configuration, energy) samples from their own cycles, train
@@ -375,6 +422,8 @@ def other_configs(pt, args, only=None):
         # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
         # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
         "delayed_acceptance": lambda: [bench_delayed_acceptance(pt, 256, 100, 4, 8, 30)],
+        "nnvt": lambda: [bench_nnvt(pt, 1, 5, 20, cpu)],
+        "nnvt_batched": lambda: [bench_nnvt(pt, 25, 5, 20, 0.0)],
     }
     res = []
     for name, fn in todo.items():
@@ -400,7 +449,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--config", type=str, default="",
                     help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, "
-                         "delayed_acceptance) and print its JSON")
+                         "delayed_acceptance, nnvt, nnvt_batched) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs shard "

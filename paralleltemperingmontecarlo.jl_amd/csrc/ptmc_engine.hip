@@ -149,11 +149,13 @@ struct ptmc_engine {
     bool nnvt = false;
     struct {
         std::vector<double> rad, ang, p1, p2;
-        int n1 = 0, n2 = 0, n_swaps = 0, n_rad = 0, n_ang = 0, ng = 0, h1 = 0, h2 = 0;
+        std::vector<int32_t> grp, gfn;
+        int n1 = 0, n2 = 0, n_swaps = 0, n_rad = 0, n_ang = 0, ng = 0, h1 = 0, h2 = 0, n_groups = 0;
         int act[4] = {0, 0, 0, 0};
         double r_cut = 0.0;
     } nnp2;
     DevBuf<double> d_n2_rad, d_n2_ang, d_n2_p1, d_n2_p2, d_n2_feat, d_n2_eatom;
+    DevBuf<int32_t> d_n2_grp, d_n2_gfn;
     Nnp2Tables nnp2_dev{};
     size_t nnp2_lds = 0;
 
@@ -593,7 +595,8 @@ int prepare_nnp2(ptmc_engine *h)
     h->n_comp = 0;
     int rc;
     if ((rc = upload(h, h->d_n2_rad, n.rad)) || (rc = upload(h, h->d_n2_ang, n.ang)) ||
-        (rc = upload(h, h->d_n2_p1, n.p1)) || (rc = upload(h, h->d_n2_p2, n.p2)))
+        (rc = upload(h, h->d_n2_p1, n.p1)) || (rc = upload(h, h->d_n2_p2, n.p2)) ||
+        (rc = upload(h, h->d_n2_grp, n.grp)) || (rc = upload(h, h->d_n2_gfn, n.gfn)))
         return rc;
     if ((rc = dalloc(h, h->d_n2_feat, (size_t)slot * NNP_NP * n.ng))) return rc;
     if ((rc = dalloc(h, h->d_n2_eatom, (size_t)slot * NNP_NP))) return rc;
@@ -611,6 +614,9 @@ int prepare_nnp2(ptmc_engine *h)
     t.r_cut = n.r_cut;
     t.rad = h->d_n2_rad.p;
     t.ang = h->d_n2_ang.p;
+    t.n_groups = n.n_groups;
+    t.grp = h->d_n2_grp.p;
+    t.gfn = h->d_n2_gfn.p;
     t.p1 = h->d_n2_p1.p;
     t.p2 = h->d_n2_p2.p;
     t.feat = h->d_n2_feat.p;
@@ -851,6 +857,8 @@ int ptmc_destroy(ptmc_handle h)
     h->d_n2_p2.free_();
     h->d_n2_feat.free_();
     h->d_n2_eatom.free_();
+    h->d_n2_grp.free_();
+    h->d_n2_gfn.free_();
     h->d_sur_w.free_();
     h->d_sur_m.free_();
     h->d_sur_v.free_();
@@ -1017,6 +1025,34 @@ int ptmc_set_potential_nnp2(ptmc_handle h, int32_t n1, int32_t n2, int32_t n_swa
     if (n.ang.empty()) n.ang.assign(15, 0.0);
     n.p1.assign(params1, params1 + n_params);
     n.p2.assign(params2, params2 + n_params);
+    // angular functions grouped by equal eta (first appearance order, stable
+    // inside a group), at most NNP2_GMAX per group
+    n.grp.clear();
+    n.gfn.clear();
+    {
+        std::vector<char> done(n_ang, 0);
+        for (int q = 0; q < n_ang; ++q) {
+            if (done[q]) continue;
+            const double eta = ang[(size_t)q * 15];
+            int cnt = 0;
+            for (int r = q; r < n_ang; ++r) {
+                if (done[r] || ang[(size_t)r * 15] != eta) continue;
+                if (cnt == NNP2_GMAX) {
+                    n.grp.push_back((int32_t)n.gfn.size() - cnt);
+                    n.grp.push_back(cnt);
+                    cnt = 0;
+                }
+                n.gfn.push_back(r);
+                done[r] = 1;
+                ++cnt;
+            }
+            n.grp.push_back((int32_t)n.gfn.size() - cnt);
+            n.grp.push_back(cnt);
+        }
+        n.n_groups = (int)n.grp.size() / 2;
+        if (n.gfn.empty()) n.gfn.assign(1, 0);
+        if (n.grp.empty()) n.grp.assign(2, 0);
+    }
     PotParams p{};
     p.kind = POT_NNP2;
     h->pot = p;

@@ -35,7 +35,7 @@ namespace ptmc {
 namespace {
 
 struct N2Lds {
-    double *X, *XT, *R2, *FC, *RN, *FN, *D, *eat, *eatn, *S;
+    double *X, *XT, *R2, *FC, *RN, *FN, *D, *eat, *eatn, *W, *S;
     int *mi, *chg;
 };
 
@@ -52,7 +52,8 @@ __device__ inline N2Lds n2_lds(unsigned char *raw, int N, int ng, int nfn, int h
     l.D = l.FN + 2 * N;
     l.eat = l.D + (size_t)N * ng;
     l.eatn = l.eat + N;
-    l.S = l.eatn + N;
+    l.W = l.eatn + N;
+    l.S = l.W + 2 * nnp2_params(ng, h1, h2);
     l.mi = reinterpret_cast<int *>(l.S + nnp2_scratch_doubles(N, nfn, h1, h2));
     l.chg = l.mi + N;
     return l;
@@ -92,18 +93,6 @@ __device__ __forceinline__ double n2_pow(double b, double zeta)
     return pow(b, zeta);
 }
 
-// angular value of triangle (c; j, k) at centre c (SymmFunc.jl:184-202,
-// Cutoff.jl:48-68): exp(-eta (r_cj^2 + r_ck^2 + r_jk^2)) f_cj f_ck f_jk (1 + lambda cos)^zeta
-__device__ __forceinline__ double n2_tri(const double *pc, const double *pj, const double *pk, double rcj, double rck,
-                                         double rjk, double fcj, double fck, double fjk, double eta, double lam,
-                                         double zeta)
-{
-    const double ux = pj[0] - pc[0], uy = pj[1] - pc[1], uz = pj[2] - pc[2];
-    const double wx = pk[0] - pc[0], wy = pk[1] - pc[1], wz = pk[2] - pc[2];
-    const double cth = (ux * wx + uy * wy + uz * wz) / (sqrt(rcj) * sqrt(rck));
-    return exp(-eta * (rcj + rck + rjk)) * fcj * fck * fjk * n2_pow(1.0 + lam * cth, zeta);
-}
-
 // squared distance / cutoff between atoms a and b in the trial configuration
 __device__ __forceinline__ double n2_rt(const N2Lds &L, int N, int a, int b)
 {
@@ -127,24 +116,84 @@ __device__ void n2_tables(const N2Lds &L, int N, double rc)
     }
 }
 
-// Sums of feature function fn at centre c over the trial configuration XT
-// (partner pairs (j, k), j = ch, ch + nch, ...; radial sums in chunk 0):
-// s[0..1] radial rows (partner species), s[0..2] angular rows (pair 11, 12, 22).
-__device__ void n2_column_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c, int fn, int ch, int nch, double s[3])
+// An angular group: the functions of one eta (at most NNP2_GMAX), which
+// share every triangle's geometry and exponential.
+struct N2Grp {
+    int cnt;
+    int q[NNP2_GMAX];
+    double lam[NNP2_GMAX], zeta[NNP2_GMAX];
+    double eta;
+};
+
+__device__ __forceinline__ void n2_load_group(const Nnp2Tables &T, int g, N2Grp &G)
+{
+    const int first = T.grp[2 * g], cnt = T.grp[2 * g + 1];
+    G.cnt = cnt;
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) {
+        const int q = T.gfn[first + (l < cnt ? l : 0)];
+        G.q[l] = q;
+        G.lam[l] = T.ang[(size_t)q * 15 + 1];
+        G.zeta[l] = T.ang[(size_t)q * 15 + 2];
+    }
+    G.eta = T.ang[(size_t)T.gfn[first] * 15];
+}
+
+// cosine at c and exp(-eta (r_cj^2 + r_ck^2 + r_jk^2)) f_cj f_ck f_jk of triangle (c; j, k)
+// (SymmFunc.jl:184-202, Cutoff.jl:48-68)
+__device__ __forceinline__ void n2_tri_geom(const double *pc, const double *pj, const double *pk, double rcj,
+                                            double rck, double rjk, double fcj, double fck, double fjk, double eta,
+                                            double &cth, double &e)
+{
+    const double ux = pj[0] - pc[0], uy = pj[1] - pc[1], uz = pj[2] - pc[2];
+    const double wx = pk[0] - pc[0], wy = pk[1] - pc[1], wz = pk[2] - pc[2];
+    cth = (ux * wx + uy * wy + uz * wz) / (sqrt(rcj) * sqrt(rck));
+    e = exp(-eta * (rcj + rck + rjk)) * fcj * fck * fjk;
+}
+
+// acc[l] += e (1 + lambda_l cos)^zeta_l for the functions of the group
+__device__ __forceinline__ void n2_group_add(const N2Grp &G, double cth, double e, double (&acc)[NNP2_GMAX])
+{
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l)
+        if (l < G.cnt) acc[l] += e * n2_pow(1.0 + G.lam[l] * cth, G.zeta[l]);
+}
+
+// row sums r[row][l] += acc[l] for a row known only at run time (selects,
+// not branches, keep the three arrays in registers)
+__device__ __forceinline__ void n2_fold(int row, const double (&acc)[NNP2_GMAX], double (&r0)[NNP2_GMAX],
+                                        double (&r1)[NNP2_GMAX], double (&r2)[NNP2_GMAX])
+{
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) {
+        r0[l] += row == 0 ? acc[l] : 0.0;
+        r1[l] += row == 1 ? acc[l] : 0.0;
+        r2[l] += row == 2 ? acc[l] : 0.0;
+    }
+}
+
+// Radial sums of function q at centre c over the trial configuration:
+// s[0] partners of species 1, s[1] of species 2.
+__device__ void n2_radial_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c, int q, double s[2])
 {
     const int n1 = T.n1;
-    s[0] = s[1] = s[2] = 0.0;
-    if (fn < T.n_rad) {
-        if (ch != 0) return;
-        const double eta = T.rad[(size_t)fn * 9];
-        for (int j = 0; j < N; ++j) {
-            if (j == c) continue;
-            s[j < n1 ? 0 : 1] += n2_rad(n2_rt(L, N, c, j), n2_ft(L, N, c, j), eta);
-        }
-        return;
+    const double eta = T.rad[(size_t)q * 9];
+    s[0] = s[1] = 0.0;
+    for (int j = 0; j < N; ++j) {
+        if (j == c) continue;
+        s[j < n1 ? 0 : 1] += n2_rad(n2_rt(L, N, c, j), n2_ft(L, N, c, j), eta);
     }
-    const double *A = T.ang + (size_t)(fn - T.n_rad) * 15;
-    const double eta = A[0], lam = A[1], zeta = A[2];
+}
+
+// Angular sums of group g at centre c over the trial configuration XT, over
+// the partner pairs (j, k) with j = ch, ch + nch, ...: r0 / r1 / r2 the rows
+// of partner pairs 11 / 12 / 22.
+__device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, int N, int c, int ch, int nch,
+                              double (&r0)[NNP2_GMAX], double (&r1)[NNP2_GMAX], double (&r2)[NNP2_GMAX])
+{
+    const int n1 = T.n1;
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) r0[l] = r1[l] = r2[l] = 0.0;
     const double *pc = &L.XT[3 * c];
     for (int j = ch; j < N; j += nch) {
         if (j == c) continue;
@@ -152,86 +201,125 @@ __device__ void n2_column_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c
         if (fcj == 0.0) continue;
         const double rcj = n2_rt(L, N, c, j);
         const int sj = j < n1 ? 0 : 1;
-        for (int k = j + 1; k < N; ++k) {
-            if (k == c) continue;
-            const double fck = n2_ft(L, N, c, k);
-            if (fck == 0.0) continue;
-            const double fjk = n2_ft(L, N, j, k);
-            if (fjk == 0.0) continue;
-            s[sj + (k < n1 ? 0 : 1)] += n2_tri(pc, &L.XT[3 * j], &L.XT[3 * k], rcj, n2_rt(L, N, c, k),
-                                               n2_rt(L, N, j, k), fcj, fck, fjk, eta, lam, zeta);
+        for (int blk = 0; blk < 2; ++blk) {
+            const int klo = blk ? (j + 1 > n1 ? j + 1 : n1) : j + 1, khi = blk ? N : n1;
+            if (klo >= khi) continue;
+            double acc[NNP2_GMAX];
+#pragma unroll
+            for (int l = 0; l < NNP2_GMAX; ++l) acc[l] = 0.0;
+            for (int k = klo; k < khi; ++k) {
+                if (k == c) continue;
+                const double fck = n2_ft(L, N, c, k);
+                if (fck == 0.0) continue;
+                const double fjk = n2_ft(L, N, j, k);
+                if (fjk == 0.0) continue;
+                double cth, e;
+                n2_tri_geom(pc, &L.XT[3 * j], &L.XT[3 * k], rcj, n2_rt(L, N, c, k), n2_rt(L, N, j, k), fcj, fck, fjk,
+                            G.eta, cth, e);
+                n2_group_add(G, cth, e, acc);
+            }
+            n2_fold(sj + blk, acc, r0, r1, r2);
         }
     }
 }
 
-// scale and offset of the rows of function fn at a centre of species sc
-__device__ __forceinline__ void n2_store_column(const Nnp2Tables &T, double *Dc, int fn, int sc, const double s[3])
+// scaled rows of radial function q / of the group at a centre of species sc
+__device__ __forceinline__ void n2_store_radial(const Nnp2Tables &T, double *Dc, int q, int sc, const double s[2])
 {
-    if (fn < T.n_rad) {
-        const double *R = T.rad + (size_t)fn * 9;
-        for (int r = 0; r < 2; ++r) Dc[2 * fn + r] = s[r] * R[1 + 2 * sc + r] + R[5 + 2 * sc + r];
-    } else {
-        const int q = fn - T.n_rad;
-        const double *A = T.ang + (size_t)q * 15;
-        for (int r = 0; r < 3; ++r) Dc[2 * T.n_rad + 3 * q + r] = s[r] * A[3 + 3 * sc + r] + A[9 + 3 * sc + r];
+    const double *R = T.rad + (size_t)q * 9;
+    for (int r = 0; r < 2; ++r) Dc[2 * q + r] = s[r] * R[1 + 2 * sc + r] + R[5 + 2 * sc + r];
+}
+__device__ __forceinline__ void n2_store_group(const Nnp2Tables &T, double *Dc, const N2Grp &G, int sc,
+                                               const double (&r0)[NNP2_GMAX], const double (&r1)[NNP2_GMAX],
+                                               const double (&r2)[NNP2_GMAX])
+{
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) {
+        if (l >= G.cnt) continue;
+        const double *A = T.ang + (size_t)G.q[l] * 15;
+        double *o = Dc + 2 * T.n_rad + 3 * G.q[l];
+        o[0] = r0[l] * A[3 + 3 * sc] + A[9 + 3 * sc];
+        o[1] = r1[l] * A[4 + 3 * sc] + A[10 + 3 * sc];
+        o[2] = r2[l] * A[5 + 3 * sc] + A[11 + 3 * sc];
     }
 }
 
-// rows of function fn at an unmoved centre c: G plus the change of every
-// term that holds a moved atom (ma[0..nm-1]), per-species scale
-__device__ void n2_delta(const Nnp2Tables &T, const N2Lds &L, const double *Gc, int N, int c, int fn, int nm,
-                         const int *ma)
+// radial rows of function q at an unmoved centre c: G plus the change of
+// every pair with a moved atom (ma[0..nm-1]), per-species scale
+__device__ void n2_delta_radial(const Nnp2Tables &T, const N2Lds &L, const double *Gc, int N, int c, int q, int nm,
+                                const int *ma)
 {
     const int n1 = T.n1, sc = c < n1 ? 0 : 1;
-    double *Dc = &L.D[(size_t)c * T.ng];
-    if (fn < T.n_rad) {
-        const double *R = T.rad + (size_t)fn * 9;
-        const double eta = R[0];
-        double acc[2] = {0.0, 0.0};
-        for (int m = 0; m < nm; ++m) {
-            const int a = ma[m];
-            acc[a < n1 ? 0 : 1] +=
-                n2_rad(L.RN[m * N + c], L.FN[m * N + c], eta) - n2_rad(L.R2[c * N + a], L.FC[c * N + a], eta);
-        }
-        for (int r = 0; r < 2; ++r) Dc[2 * fn + r] = Gc[2 * fn + r] + acc[r] * R[1 + 2 * sc + r];
-        return;
+    const double *R = T.rad + (size_t)q * 9;
+    const double eta = R[0];
+    double acc[2] = {0.0, 0.0};
+    for (int m = 0; m < nm; ++m) {
+        const int a = ma[m];
+        acc[a < n1 ? 0 : 1] +=
+            n2_rad(L.RN[m * N + c], L.FN[m * N + c], eta) - n2_rad(L.R2[c * N + a], L.FC[c * N + a], eta);
     }
-    const int q = fn - T.n_rad;
-    const double *A = T.ang + (size_t)q * 15;
-    const double eta = A[0], lam = A[1], zeta = A[2];
+    double *Dc = &L.D[(size_t)c * T.ng];
+    for (int r = 0; r < 2; ++r) Dc[2 * q + r] = Gc[2 * q + r] + acc[r] * R[1 + 2 * sc + r];
+}
+
+// angular rows of group g at an unmoved centre c: G plus the change of every
+// triangle with a moved atom (new minus old), per-species scale
+__device__ __forceinline__ void n2_delta_group(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, const double *Gc, int N, int c,
+                               int nm, const int *ma)
+{
+    const int n1 = T.n1, sc = c < n1 ? 0 : 1;
     const double *pc = &L.X[3 * c];
-    double acc[3] = {0.0, 0.0, 0.0};
+    double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) r0[l] = r1[l] = r2[l] = 0.0;
     for (int m = 0; m < nm; ++m) {
         const int a = ma[m], sa = a < n1 ? 0 : 1;
         const double foca = L.FC[c * N + a], fnca = L.FN[m * N + c];
         if (foca == 0.0 && fnca == 0.0) continue;
         const double roca = L.R2[c * N + a], rnca = L.RN[m * N + c];
-        for (int k = 0; k < N; ++k) {
-            if (k == c || k == a) continue;
-            const int mk = L.mi[k];
-            if (mk >= 0 && mk < m) continue;  // the pair of two moved atoms counts once
-            double v = 0.0;
-            const double fck = L.FC[c * N + k];
-            if (foca != 0.0 && fck != 0.0) {
-                const double fak = L.FC[a * N + k];
-                if (fak != 0.0)
-                    v -= n2_tri(pc, &L.X[3 * a], &L.X[3 * k], roca, L.R2[c * N + k], L.R2[a * N + k], foca, fck, fak,
-                                eta, lam, zeta);
-            }
-            const double fnck = mk >= 0 ? L.FN[mk * N + c] : fck;
-            if (fnca != 0.0 && fnck != 0.0) {
-                const double fnak = L.FN[m * N + k];
-                if (fnak != 0.0) {
-                    const double rnck = mk >= 0 ? L.RN[mk * N + c] : L.R2[c * N + k];
-                    v += n2_tri(pc, &L.XT[3 * a], &L.XT[3 * k], rnca, rnck, L.RN[m * N + k], fnca, fnck, fnak, eta,
-                                lam, zeta);
+        for (int blk = 0; blk < 2; ++blk) {
+            double acc[NNP2_GMAX];
+#pragma unroll
+            for (int l = 0; l < NNP2_GMAX; ++l) acc[l] = 0.0;
+            const int klo = blk ? n1 : 0, khi = blk ? N : n1;
+            for (int k = klo; k < khi; ++k) {
+                if (k == c || k == a) continue;
+                const int mk = L.mi[k];
+                if (mk >= 0 && mk < m) continue;  // the pair of two moved atoms counts once
+                double cth, e;
+                const double fck = L.FC[c * N + k];
+                if (foca != 0.0 && fck != 0.0) {
+                    const double fak = L.FC[a * N + k];
+                    if (fak != 0.0) {
+                        n2_tri_geom(pc, &L.X[3 * a], &L.X[3 * k], roca, L.R2[c * N + k], L.R2[a * N + k], foca, fck,
+                                    fak, G.eta, cth, e);
+                        n2_group_add(G, cth, -e, acc);
+                    }
+                }
+                const double fnck = mk >= 0 ? L.FN[mk * N + c] : fck;
+                if (fnca != 0.0 && fnck != 0.0) {
+                    const double fnak = L.FN[m * N + k];
+                    if (fnak != 0.0) {
+                        const double rnck = mk >= 0 ? L.RN[mk * N + c] : L.R2[c * N + k];
+                        n2_tri_geom(pc, &L.XT[3 * a], &L.XT[3 * k], rnca, rnck, L.RN[m * N + k], fnca, fnck, fnak,
+                                    G.eta, cth, e);
+                        n2_group_add(G, cth, e, acc);
+                    }
                 }
             }
-            acc[sa + (k < n1 ? 0 : 1)] += v;
+            n2_fold(sa + blk, acc, r0, r1, r2);
         }
     }
-    for (int r = 0; r < 3; ++r)
-        Dc[2 * T.n_rad + 3 * q + r] = Gc[2 * T.n_rad + 3 * q + r] + acc[r] * A[3 + 3 * sc + r];
+    double *Dc = &L.D[(size_t)c * T.ng];
+#pragma unroll
+    for (int l = 0; l < NNP2_GMAX; ++l) {
+        if (l >= G.cnt) continue;
+        const double *A = T.ang + (size_t)G.q[l] * 15;
+        const int r = 2 * T.n_rad + 3 * G.q[l];
+        Dc[r] = Gc[r] + r0[l] * A[3 + 3 * sc];
+        Dc[r + 1] = Gc[r + 1] + r1[l] * A[4 + 3 * sc];
+        Dc[r + 2] = Gc[r + 2] + r2[l] * A[5 + 3 * sc];
+    }
 }
 
 __device__ __forceinline__ double n2_act(int code, double s) { return code == 2 ? tanh(s) : s; }
@@ -241,10 +329,11 @@ __device__ __forceinline__ double n2_act(int code, double s) { return code == 2 
 __device__ void n2_forward(const Nnp2Tables &T, const N2Lds &L, int N, int nc, const int *list)
 {
     const int tid = threadIdx.x, ng = T.ng, h1 = T.h1, h2 = T.h2, n1 = T.n1;
+    const size_t np = nnp2_params(ng, h1, h2);
     double *H1 = L.S, *H2 = L.S + (size_t)N * h1;
     for (int t = tid; t < nc * h1; t += NNP2_NT) {
         const int ci = t / h1, j = t % h1, c = list[ci];
-        const double *p = c < n1 ? T.p1 : T.p2;
+        const double *p = c < n1 ? L.W : L.W + np;
         const double *Dc = &L.D[(size_t)c * ng];
         double s = 0.0;
         for (int k = 0; k < ng; ++k) s += Dc[k] * p[(size_t)k * h1 + j];
@@ -255,7 +344,7 @@ __device__ void n2_forward(const Nnp2Tables &T, const N2Lds &L, int N, int nc, c
     const size_t o2 = (size_t)ng * h1 + h1;
     for (int t = tid; t < nc * h2; t += NNP2_NT) {
         const int ci = t / h2, j = t % h2, c = list[ci];
-        const double *p = (c < n1 ? T.p1 : T.p2) + o2;
+        const double *p = (c < n1 ? L.W : L.W + np) + o2;
         double s = 0.0;
         for (int k = 0; k < h1; ++k) s += H1[ci * h1 + k] * p[(size_t)k * h2 + j];
         s += p[(size_t)h1 * h2 + j];
@@ -265,7 +354,7 @@ __device__ void n2_forward(const Nnp2Tables &T, const N2Lds &L, int N, int nc, c
     const size_t o3 = o2 + (size_t)h1 * h2 + h2;
     for (int ci = tid; ci < nc; ci += NNP2_NT) {
         const int c = list[ci];
-        const double *p = (c < n1 ? T.p1 : T.p2) + o3;
+        const double *p = (c < n1 ? L.W : L.W + np) + o3;
         double s = 0.0;
         for (int k = 0; k < h2; ++k) s += H2[ci * h2 + k] * p[k];
         s += p[h2];
@@ -277,14 +366,26 @@ __device__ void n2_forward(const Nnp2Tables &T, const N2Lds &L, int N, int nc, c
 // every column rebuilt from X (XT = X, nothing moved): D, eatn and the total
 __device__ double n2_rebuild(const Nnp2Tables &T, const N2Lds &L, int N, double *s_tot)
 {
-    const int nfn = T.n_rad + T.n_ang;
     for (int e = threadIdx.x; e < 3 * N; e += NNP2_NT) L.XT[e] = L.X[e];
     __syncthreads();
-    for (int t = threadIdx.x; t < N * nfn; t += NNP2_NT) {
-        const int c = t / nfn, fn = t % nfn;
-        double s[3];
-        n2_column_sums(T, L, N, c, fn, 0, 1, s);
-        n2_store_column(T, &L.D[(size_t)c * T.ng], fn, c < T.n1 ? 0 : 1, s);
+    const int nga = T.n_groups, nt = N * (nga + T.n_rad);
+    for (int t = threadIdx.x; t < nt; t += NNP2_NT) {
+        double *Dc;
+        if (t < N * nga) {
+            const int c = t / nga, g = t % nga;
+            N2Grp G;
+            n2_load_group(T, g, G);
+            double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
+            n2_group_sums(T, L, G, N, c, 0, 1, r0, r1, r2);
+            Dc = &L.D[(size_t)c * T.ng];
+            n2_store_group(T, Dc, G, c < T.n1 ? 0 : 1, r0, r1, r2);
+        } else {
+            const int u = t - N * nga, c = u / T.n_rad, q = u % T.n_rad;
+            double s[2];
+            n2_radial_sums(T, L, N, c, q, s);
+            Dc = &L.D[(size_t)c * T.ng];
+            n2_store_radial(T, Dc, q, c < T.n1 ? 0 : 1, s);
+        }
     }
     for (int c = threadIdx.x; c < N; c += NNP2_NT) L.chg[c] = c;
     __syncthreads();
@@ -307,6 +408,10 @@ __device__ void n2_load_x(const SweepArgs &a, const N2Lds &L, int slot, int N)
         L.XT[e] = L.X[e];
     }
     for (int e = threadIdx.x; e < N; e += NNP2_NT) L.mi[e] = -1;
+    // both networks, staged once per launch
+    const Nnp2Tables &T = a.nnp2;
+    const int np = (int)nnp2_params(T.ng, T.h1, T.h2);
+    for (int e = threadIdx.x; e < 2 * np; e += NNP2_NT) L.W[e] = e < np ? T.p1[e] : T.p2[e - np];
 }
 
 // set_variables + initialise_energy (src/EnergyEvaluation.jl:1824-1856, :1748-1768)
@@ -335,7 +440,7 @@ __global__ void __launch_bounds__(NNP2_NT) nnp2_init_kernel(SweepArgs a)
 __global__ void __launch_bounds__(NNP2_NT) nnp2_sweep_kernel(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ int s_err, s_nc;
+    __shared__ int s_err, s_nc, s_next;
     __shared__ double s_en;
     const Nnp2Tables &T = a.nnp2;
     const int slot = blockIdx.x, tid = threadIdx.x;
@@ -418,6 +523,7 @@ __global__ void __launch_bounds__(NNP2_NT) nnp2_sweep_kernel(SweepArgs a)
         if (tid == 0) {
             L.mi[ma[0]] = 0;
             if (swap) L.mi[ma[1]] = 1;
+            s_next = NNP2_NT;
         }
         __syncthreads();
         // distances and cutoffs of the moved atoms in the trial configuration
@@ -442,32 +548,64 @@ __global__ void __launch_bounds__(NNP2_NT) nnp2_sweep_kernel(SweepArgs a)
             if (f) L.chg[__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = tid;
             if (tid == 0) s_nc = __builtin_popcountll(b);
         }
-        // trial features: unmoved columns (copy or delta), partial sums of the
-        // rebuilt moved columns
-        const int nA = N * nfn, nB = nm * nfn * NNP2_NCH;
-        for (int u = tid; u < nA + nB; u += NNP2_NT) {
-            if (u < nA) {
-                const int c = u / nfn, fn = u % nfn;
+        // trial features from a queue of tasks, heaviest first: angular groups
+        // of unmoved changed columns (delta) and chunks of the moved columns
+        // (partial sums), then the radial functions of both
+        const int nga = T.n_groups, nAg = N * nga, nBg = nm * nga * NNP2_NCH, nAr = N * T.n_rad;
+        const int ntask = nAg + nBg + nAr + nm * T.n_rad;
+        for (int u = tid; u < ntask; u = atomicAdd(&s_next, 1)) {
+            if (u < nAg) {
+                // group-major: the lanes of a wave mostly share one group
+                const int g = u / N, c = u % N;
                 if (L.mi[c] >= 0 || !changed(c)) continue;  // moved: rebuilt below; unchanged: not read
-                n2_delta(T, L, gG + (size_t)c * ng, N, c, fn, nm, ma);
+                N2Grp G;
+                n2_load_group(T, g, G);
+                n2_delta_group(T, L, G, gG + (size_t)c * ng, N, c, nm, ma);
+            } else if (u < nAg + nBg) {
+                const int v = u - nAg, mm = v / (nga * NNP2_NCH), g = (v / NNP2_NCH) % nga, ch = v % NNP2_NCH;
+                N2Grp G;
+                n2_load_group(T, g, G);
+                double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
+                n2_group_sums(T, L, G, N, ma[mm], ch, NNP2_NCH, r0, r1, r2);
+#pragma unroll
+                for (int l = 0; l < NNP2_GMAX; ++l) {
+                    if (l >= G.cnt) continue;
+                    double *P = &L.S[(((size_t)mm * nfn + T.n_rad + G.q[l]) * NNP2_NCH + ch) * 3];
+                    P[0] = r0[l];
+                    P[1] = r1[l];
+                    P[2] = r2[l];
+                }
+            } else if (u < nAg + nBg + nAr) {
+                const int v = u - nAg - nBg, c = v / T.n_rad, q = v % T.n_rad;
+                if (L.mi[c] >= 0 || !changed(c)) continue;
+                n2_delta_radial(T, L, gG + (size_t)c * ng, N, c, q, nm, ma);
             } else {
-                const int v = u - nA, mm = v / (nfn * NNP2_NCH), fn = (v / NNP2_NCH) % nfn, ch = v % NNP2_NCH;
-                double s[3];
-                n2_column_sums(T, L, N, ma[mm], fn, ch, NNP2_NCH, s);
-                double *P = &L.S[(((size_t)mm * nfn + fn) * NNP2_NCH + ch) * 3];
-                P[0] = s[0];
-                P[1] = s[1];
-                P[2] = s[2];
+                const int v = u - nAg - nBg - nAr, mm = v / T.n_rad, q = v % T.n_rad;
+                double s2[2];
+                n2_radial_sums(T, L, N, ma[mm], q, s2);
+                double *P = &L.S[((size_t)mm * nfn + q) * NNP2_NCH * 3];
+                P[0] = s2[0];
+                P[1] = s2[1];
+                P[2] = 0.0;
+                for (int e = 3; e < 3 * NNP2_NCH; ++e) P[e] = 0.0;
             }
         }
         __syncthreads();
         for (int u = tid; u < nm * nfn; u += NNP2_NT) {
             const int mm = u / nfn, fn = u % nfn, c = ma[mm];
-            double s[3] = {0.0, 0.0, 0.0};
+            double sum[3] = {0.0, 0.0, 0.0};
             const double *P = &L.S[((size_t)mm * nfn + fn) * NNP2_NCH * 3];
             for (int ch = 0; ch < NNP2_NCH; ++ch)
-                for (int r = 0; r < 3; ++r) s[r] += P[3 * ch + r];
-            n2_store_column(T, &L.D[(size_t)c * ng], fn, c < n1 ? 0 : 1, s);
+                for (int r = 0; r < 3; ++r) sum[r] += P[3 * ch + r];
+            double *Dc = &L.D[(size_t)c * ng];
+            const int sc = c < n1 ? 0 : 1;
+            if (fn < T.n_rad) {
+                n2_store_radial(T, Dc, fn, sc, sum);
+            } else {
+                const double *A = T.ang + (size_t)(fn - T.n_rad) * 15;
+                for (int r = 0; r < 3; ++r)
+                    Dc[2 * T.n_rad + 3 * (fn - T.n_rad) + r] = sum[r] * A[3 + 3 * sc + r] + A[9 + 3 * sc + r];
+            }
         }
         __syncthreads();
         const int nc = s_nc;

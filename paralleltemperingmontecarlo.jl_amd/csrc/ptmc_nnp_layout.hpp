@@ -85,19 +85,25 @@ __host__ __device__ inline size_t sur_lds_floats()
 // Two-species NNVT kernels: threads per workgroup, chunks of the partner
 // pairs of a fully rebuilt feature column, and the LDS layout in doubles:
 // X[N][3] | XT[N][3] | R2[N][N] | FC[N][N] | RN[2][N] | FN[2][N] |
-// D[N][ng] | eat[N] | eatn[N] | scratch (layer activations, or the partial
-// sums of rebuilt columns) | ints: moved index[N], changed list[N]
+// D[N][ng] | eat[N] | eatn[N] | W[2][n_params] (both networks) | scratch
+// (layer activations, or the partial sums of rebuilt columns) | ints:
+// moved index[N], changed list[N]
 constexpr int NNP2_NT = 512;
 constexpr int NNP2_NCH = 16;
+constexpr int NNP2_GMAX = 6;  // angular functions per eta group (larger groups are split)
 __host__ __device__ inline size_t nnp2_scratch_doubles(int N, int nfn, int h1, int h2)
 {
     const size_t act = (size_t)N * (h1 + h2), part = (size_t)2 * nfn * NNP2_NCH * 3;
     return act > part ? act : part;
 }
+__host__ __device__ inline size_t nnp2_params(int ng, int h1, int h2)
+{
+    return (size_t)ng * h1 + h1 + (size_t)h1 * h2 + h2 + h2 + 1;
+}
 __host__ __device__ inline size_t nnp2_lds_bytes(int N, int ng, int nfn, int h1, int h2)
 {
     const size_t d = (size_t)6 * N + 2 * (size_t)N * N + 4 * (size_t)N + (size_t)N * ng + 2 * (size_t)N +
-                     nnp2_scratch_doubles(N, nfn, h1, h2);
+                     2 * nnp2_params(ng, h1, h2) + nnp2_scratch_doubles(N, nfn, h1, h2);
     return d * sizeof(double) + 2 * (size_t)N * sizeof(int);
 }
 

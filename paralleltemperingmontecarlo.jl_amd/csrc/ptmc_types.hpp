@@ -99,6 +99,9 @@ struct Nnp2Tables {
     double r_cut;
     const double *rad;         // n_rad x 9: eta, norm[4], offset[4]
     const double *ang;         // n_ang x 15: eta, lambda, zeta, tpz[6], offset[6]
+    int32_t n_groups;          // angular functions grouped by eta
+    const int32_t *grp;        // n_groups x 2: first entry in gfn, count
+    const int32_t *gfn;        // n_ang angular function indices, group by group
     const double *p1, *p2;     // networks of species 1 and 2 (RuNNer layout)
     double *feat;              // per slot [NNP_NP][ng] features, atom-major
     double *eatom;             // per slot [NNP_NP] per-atom energies
