@@ -159,19 +159,6 @@ __device__ __forceinline__ void n2_group_add(const N2Grp &G, double cth, double 
         if (l < G.cnt) acc[l] += e * n2_pow(1.0 + G.lam[l] * cth, G.zeta[l]);
 }
 
-// row sums r[row][l] += acc[l] for a row known only at run time (selects,
-// not branches, keep the three arrays in registers)
-__device__ __forceinline__ void n2_fold(int row, const double (&acc)[NNP2_GMAX], double (&r0)[NNP2_GMAX],
-                                        double (&r1)[NNP2_GMAX], double (&r2)[NNP2_GMAX])
-{
-#pragma unroll
-    for (int l = 0; l < NNP2_GMAX; ++l) {
-        r0[l] += row == 0 ? acc[l] : 0.0;
-        r1[l] += row == 1 ? acc[l] : 0.0;
-        r2[l] += row == 2 ? acc[l] : 0.0;
-    }
-}
-
 // Radial sums of function q at centre c over the trial configuration:
 // s[0] partners of species 1, s[1] of species 2.
 __device__ void n2_radial_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c, int q, double s[2])
@@ -186,14 +173,13 @@ __device__ void n2_radial_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c
 }
 
 // Angular sums of group g at centre c over the trial configuration XT, over
-// the partner pairs (j, k) with j = ch, ch + nch, ...: r0 / r1 / r2 the rows
-// of partner pairs 11 / 12 / 22.
-__device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, int N, int c, int ch, int nch,
-                              double (&r0)[NNP2_GMAX], double (&r1)[NNP2_GMAX], double (&r2)[NNP2_GMAX])
+// the partner pairs (j, k) with j = ch, ch + nch, ...: out(l, row, v) adds
+// v to row `row` (partner pair 11 / 12 / 22) of the group's function l.
+template <typename Out>
+__device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, int N, int c,
+                                              int ch, int nch, Out out)
 {
     const int n1 = T.n1;
-#pragma unroll
-    for (int l = 0; l < NNP2_GMAX; ++l) r0[l] = r1[l] = r2[l] = 0.0;
     const double *pc = &L.XT[3 * c];
     for (int j = ch; j < N; j += nch) {
         if (j == c) continue;
@@ -218,7 +204,9 @@ __device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &
                             G.eta, cth, e);
                 n2_group_add(G, cth, e, acc);
             }
-            n2_fold(sj + blk, acc, r0, r1, r2);
+#pragma unroll
+            for (int l = 0; l < NNP2_GMAX; ++l)
+                if (l < G.cnt) out(l, sj + blk, acc[l]);
         }
     }
 }
@@ -229,21 +217,6 @@ __device__ __forceinline__ void n2_store_radial(const Nnp2Tables &T, double *Dc,
     const double *R = T.rad + (size_t)q * 9;
     for (int r = 0; r < 2; ++r) Dc[2 * q + r] = s[r] * R[1 + 2 * sc + r] + R[5 + 2 * sc + r];
 }
-__device__ __forceinline__ void n2_store_group(const Nnp2Tables &T, double *Dc, const N2Grp &G, int sc,
-                                               const double (&r0)[NNP2_GMAX], const double (&r1)[NNP2_GMAX],
-                                               const double (&r2)[NNP2_GMAX])
-{
-#pragma unroll
-    for (int l = 0; l < NNP2_GMAX; ++l) {
-        if (l >= G.cnt) continue;
-        const double *A = T.ang + (size_t)G.q[l] * 15;
-        double *o = Dc + 2 * T.n_rad + 3 * G.q[l];
-        o[0] = r0[l] * A[3 + 3 * sc] + A[9 + 3 * sc];
-        o[1] = r1[l] * A[4 + 3 * sc] + A[10 + 3 * sc];
-        o[2] = r2[l] * A[5 + 3 * sc] + A[11 + 3 * sc];
-    }
-}
-
 // radial rows of function q at an unmoved centre c: G plus the change of
 // every pair with a moved atom (ma[0..nm-1]), per-species scale
 __device__ void n2_delta_radial(const Nnp2Tables &T, const N2Lds &L, const double *Gc, int N, int c, int q, int nm,
@@ -263,15 +236,22 @@ __device__ void n2_delta_radial(const Nnp2Tables &T, const N2Lds &L, const doubl
 }
 
 // angular rows of group g at an unmoved centre c: G plus the change of every
-// triangle with a moved atom (new minus old), per-species scale
-__device__ __forceinline__ void n2_delta_group(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, const double *Gc, int N, int c,
-                               int nm, const int *ma)
+// triangle with a moved atom (new minus old), per-species scale, folded into
+// D block by block
+__device__ __forceinline__ void n2_delta_group(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, const double *Gc,
+                                               int N, int c, int nm, const int *ma)
 {
     const int n1 = T.n1, sc = c < n1 ? 0 : 1;
     const double *pc = &L.X[3 * c];
-    double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
+    double *Dc = &L.D[(size_t)c * T.ng];
 #pragma unroll
-    for (int l = 0; l < NNP2_GMAX; ++l) r0[l] = r1[l] = r2[l] = 0.0;
+    for (int l = 0; l < NNP2_GMAX; ++l) {
+        if (l >= G.cnt) continue;
+        const int r = 2 * T.n_rad + 3 * G.q[l];
+        Dc[r] = Gc[r];
+        Dc[r + 1] = Gc[r + 1];
+        Dc[r + 2] = Gc[r + 2];
+    }
     for (int m = 0; m < nm; ++m) {
         const int a = ma[m], sa = a < n1 ? 0 : 1;
         const double foca = L.FC[c * N + a], fnca = L.FN[m * N + c];
@@ -307,18 +287,14 @@ __device__ __forceinline__ void n2_delta_group(const Nnp2Tables &T, const N2Lds 
                     }
                 }
             }
-            n2_fold(sa + blk, acc, r0, r1, r2);
-        }
-    }
-    double *Dc = &L.D[(size_t)c * T.ng];
+            const int row = sa + blk;
 #pragma unroll
-    for (int l = 0; l < NNP2_GMAX; ++l) {
-        if (l >= G.cnt) continue;
-        const double *A = T.ang + (size_t)G.q[l] * 15;
-        const int r = 2 * T.n_rad + 3 * G.q[l];
-        Dc[r] = Gc[r] + r0[l] * A[3 + 3 * sc];
-        Dc[r + 1] = Gc[r + 1] + r1[l] * A[4 + 3 * sc];
-        Dc[r + 2] = Gc[r + 2] + r2[l] * A[5 + 3 * sc];
+            for (int l = 0; l < NNP2_GMAX; ++l) {
+                if (l >= G.cnt) continue;
+                const double *A = T.ang + (size_t)G.q[l] * 15;
+                Dc[2 * T.n_rad + 3 * G.q[l] + row] += acc[l] * A[3 + 3 * sc + row];
+            }
+        }
     }
 }
 
@@ -375,10 +351,19 @@ __device__ double n2_rebuild(const Nnp2Tables &T, const N2Lds &L, int N, double 
             const int c = t / nga, g = t % nga;
             N2Grp G;
             n2_load_group(T, g, G);
-            double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
-            n2_group_sums(T, L, G, N, c, 0, 1, r0, r1, r2);
             Dc = &L.D[(size_t)c * T.ng];
-            n2_store_group(T, Dc, G, c < T.n1 ? 0 : 1, r0, r1, r2);
+            const int sc = c < T.n1 ? 0 : 1;
+            // rows start at the offsets; each block adds its scaled sum
+#pragma unroll
+            for (int l = 0; l < NNP2_GMAX; ++l) {
+                if (l >= G.cnt) continue;
+                const double *A = T.ang + (size_t)G.q[l] * 15;
+                for (int r = 0; r < 3; ++r) Dc[2 * T.n_rad + 3 * G.q[l] + r] = A[9 + 3 * sc + r];
+            }
+            n2_group_sums(T, L, G, N, c, 0, 1, [&](int l, int row, double v) {
+                const double *A = T.ang + (size_t)G.q[l] * 15;
+                Dc[2 * T.n_rad + 3 * G.q[l] + row] += v * A[3 + 3 * sc + row];
+            });
         } else {
             const int u = t - N * nga, c = u / T.n_rad, q = u % T.n_rad;
             double s[2];
@@ -565,16 +550,15 @@ __global__ void __launch_bounds__(NNP2_NT) nnp2_sweep_kernel(SweepArgs a)
                 const int v = u - nAg, mm = v / (nga * NNP2_NCH), g = (v / NNP2_NCH) % nga, ch = v % NNP2_NCH;
                 N2Grp G;
                 n2_load_group(T, g, G);
-                double r0[NNP2_GMAX], r1[NNP2_GMAX], r2[NNP2_GMAX];
-                n2_group_sums(T, L, G, N, ma[mm], ch, NNP2_NCH, r0, r1, r2);
 #pragma unroll
                 for (int l = 0; l < NNP2_GMAX; ++l) {
                     if (l >= G.cnt) continue;
                     double *P = &L.S[(((size_t)mm * nfn + T.n_rad + G.q[l]) * NNP2_NCH + ch) * 3];
-                    P[0] = r0[l];
-                    P[1] = r1[l];
-                    P[2] = r2[l];
+                    P[0] = P[1] = P[2] = 0.0;
                 }
+                n2_group_sums(T, L, G, N, ma[mm], ch, NNP2_NCH, [&](int l, int row, double v) {
+                    L.S[(((size_t)mm * nfn + T.n_rad + G.q[l]) * NNP2_NCH + ch) * 3 + row] += v;
+                });
             } else if (u < nAg + nBg + nAr) {
                 const int v = u - nAg - nBg, c = v / T.n_rad, q = v % T.n_rad;
                 if (L.mi[c] >= 0 || !changed(c)) continue;
