@@ -24,8 +24,8 @@ import pathlib
 import numpy as np
 
 from . import api
-from .api import (CubicBC, Config, EmbeddedAtomPotential, ELJPotential, ELJPotentialEven, MCParams, MCState, NPT,
-                  NVT, Output, SphericalBC, TempGrid)
+from .api import (NNVT, CubicBC, Config, ELJPotentialB, EmbeddedAtomPotential, ELJPotential, ELJPotentialEven,
+                  LookupTablePotential, MCParams, MCState, NPT, NVT, Output, SphericalBC, TempGrid)
 
 
 # ------------------------------------------------------------ formatting
@@ -111,6 +111,13 @@ def _potential_line(pot) -> str:
         return f"ELJPotential{{{len(pot.coeff)}, Float64}} " + _row(pot.coeff)
     if isinstance(pot, EmbeddedAtomPotential):
         return f"EAM: {_fmt(pot.n)} {_fmt(pot.m)} {_fmt(pot.ean)} {_fmt(pot.eCam)} \n"
+    if isinstance(pot, ELJPotentialB):
+        # writepotential(::AbstractDimerPotentialB), src/ReadSave.jl:70-76: the
+        # count, then the a, b and c rows tab-delimited (writedlm's default)
+        return (f"ELJB {len(pot.coeff_a)} \n" + _row(pot.coeff_a, "\t") + _row(pot.coeff_b, "\t") +
+                _row(pot.coeff_c, "\t"))
+    if isinstance(pot, LookupTablePotential):
+        return "LookupTablePotential \n"
     return "runnerpotential \n"
 
 
@@ -132,6 +139,10 @@ def save_init(potential, ensemble, mc_params: MCParams, temp: TempGrid, director
         text += _row(["ensemble", "n_atom_moves", "n_volume_moves", "n_atom_swaps", "pressure", "separated_volume"])
         text += _row(["NPT", ensemble.n_atoms, ensemble.n_atom_moves, ensemble.n_volume_moves, 0,
                       float(ensemble.pressure), bool(ensemble.separated_volume)])
+    elif isinstance(ensemble, NNVT):
+        # writeensemble(::NNVT), src/ReadSave.jl:51-56
+        text += _row(["ensemble", "n_1", "n_2", "n_atom_moves", "n_atom_swaps"])
+        text += _row(["NNVT", ensemble.natoms[0], ensemble.natoms[1], ensemble.n_atom_moves, ensemble.n_atom_swaps])
     else:
         text += _row(["ensemble", "n_atom_moves", "n_atom_swaps"])
         text += _row(["NVT", ensemble.n_atoms, ensemble.n_atom_moves, 0])
@@ -225,6 +236,8 @@ def readensemble(row):
         return NVT(int(row[1]))
     if "NPT" in str(row[0]):
         return NPT(int(row[1]), float(row[5]), bool(row[6]))
+    if "NNVT" in str(row[0]):
+        return NNVT((int(row[1]), int(row[2])), int(row[3]), int(row[4]))
     raise NotImplementedError(f"ensemble {row[0]} is not part of the engine")
 
 
@@ -239,6 +252,10 @@ def readpotential(rows):
         return ELJPotential([float(v) for v in rows[0][2:2 + n]])
     if head == "EAM:":
         return EmbeddedAtomPotential(*(float(v) for v in rows[0][1:5]))
+    if head == "ELJB":
+        n = int(rows[0][1])
+        a, b, c = ([float(v) for v in rows[r][:n]] for r in (1, 2, 3))
+        return ELJPotentialB(tuple(a), tuple(b), tuple(c))
     raise NotImplementedError(f"potential line {head!r} is not handled")
 
 

@@ -54,6 +54,28 @@ def test_params_round_trip(rs, tmp_path):
     assert mc_r.eq_cycles == 0
 
 
+def test_params_nnvt_eljb_lookup(rs, tmp_path):
+    """The ensemble and potential lines of writeensemble(::NNVT) and
+    writepotential(::AbstractDimerPotentialB / ::LookupTablePotential)
+    (src/ReadSave.jl:51-56, :70-76, :85-87), read back by readensemble /
+    readpotential (:231-251, :265-270)."""
+    import paralleltemperingmontecarlo as pt
+
+    mc, temp, _, _ = rs.read_init(restart=False, eq_cycles=0.2, directory=GOLDEN)
+    ens = pt.NNVT.of([50, 5], natomswaps=2)
+    pot = pt.ELJPotentialB((0.1, -0.25, 1e-5), (0.5, 0.0, 2.0), (-10.5, 989.7, -101383.865938807))
+    rs.save_init(pot, ens, mc, temp, tmp_path)
+    lines = (tmp_path / "checkpoint" / "params.data").read_text().splitlines()
+    assert lines[2:4] == ["ensemble n_1 n_2 n_atom_moves n_atom_swaps", "NNVT 50 5 55 2"]
+    assert lines[4:8] == ["ELJB 3 ", "0.1\t-0.25\t1.0e-5", "0.5\t0.0\t2.0", "-10.5\t989.7\t-101383.865938807"]
+    _, _, ens_r, pot_r = rs.read_init(restart=False, directory=tmp_path)
+    assert ens_r == ens and pot_r == pot
+    d2 = tmp_path / "lut"
+    lut = pt.LookupTablePotential(np.zeros((1, 1)), 0.1, 0.0, 1, 1, 0.01, 1.0, 0.0)
+    rs.save_init(lut, pt.NVT(55), mc, temp, d2)
+    assert (d2 / "checkpoint" / "params.data").read_text().splitlines()[4] == "LookupTablePotential "
+
+
 def test_hist_info_round_trip(rs, tmp_path):
     rows = rs._readdlm(CK / "hist_info.data")
     res = rs.setresults(rows[1], rs._readdlm(CK / "histograms.data"))
