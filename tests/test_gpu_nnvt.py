@@ -108,3 +108,25 @@ def test_brass55_cycles(base_pot):
     c = eng.counters(0)
     assert c["count_vol"][:, 1].sum() > 0 and c["count_atom"][:, 0].sum() > 0
     eng.close()
+
+
+def test_ptmc_run_nnvt(base_pot):
+    """ptmc_run (src/MCRun.jl:363-432) with RuNNerPotential2Atom and NNVT on
+    the six-atom configuration of test/test_states.jl:113-118 (radius 8,
+    500 and 650 K): equilibration, production, results and final states;
+    every state's energy equals a rebuild of its positions and every atom
+    stays inside the container."""
+    st = json.loads((GOLDEN / "nnvt_states.json").read_text())
+    x0 = np.array(st["positions"])
+    ens = pt.NNVT.of([st["n1"], st["n2"]])
+    temp = pt.TempGrid(500.0, 650.0, 2)
+    mc = pt.MCParams(400, 2, 6, n_adjust=50)
+    print("NNVT ptmc_run 6 atoms 400 cycles", flush=True)
+    states, res = pt.ptmc_run(mc, temp, pt.Config(x0, pt.SphericalBC(st["radius"])), base_pot, ens)
+    assert len(res.en_avg) == 2 and len(res.heat_cap) == 2 and np.all(np.isfinite(res.en_avg))
+    assert all(0.0 < a <= 1.0 for a in res.count_stat_atom)
+    assert sum(int(s.count_vol[1]) for s in states) > 0  # swaps attempted
+    e_full, _ = pt.nnp2_energies(base_pot, np.stack([s.config.positions for s in states]))
+    np.testing.assert_allclose([s.en_tot for s in states], e_full, rtol=1e-11)
+    for s in states:
+        assert np.all((s.config.positions**2).sum(-1) <= st["radius"] ** 2 + 1e-9)
