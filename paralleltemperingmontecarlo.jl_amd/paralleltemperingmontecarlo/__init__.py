@@ -46,6 +46,7 @@ from .api import (  # noqa: F401
 )
 from ._lib import PTMCError, load as load_library  # noqa: F401
 from .multihist import (  # noqa: F401
-    multihistogram_NPT, multihistogram_npt_batch, multihistogram_NVT, multihistogram_nvt_batch)
+    multihistogram, multihistogram_NPT, multihistogram_npt_batch, multihistogram_NVT, multihistogram_nvt_batch,
+    postprocess)
 
 AtoBohr = 1.8897259886
