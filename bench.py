@@ -452,8 +452,8 @@ def main():
                          "delayed_acceptance, nnvt, nnvt_batched) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs shard "
-              "across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs would "
+              "shard across GPUs without communication; no multi-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
