@@ -12,8 +12,9 @@ atom moves of all replicas per second.
 The line also carries "other_configs": the remaining configurations measured
 briefly on the same device (Ne13 FP64 single ladder and 512 ladders, periodic
 bulk neon NVT/NPT at 108/216/256 atoms, NNP Cu55 single ladder and 64
-ladders, delayed acceptance on 256 NNP Cu55 ladders against the plain path),
-with the oracle's CPU rate where one is quoted.
+ladders, delayed acceptance on 256 NNP Cu55 ladders against the plain path,
+NNVT brass Cu50Zn5 single ladder and 25 ladders), with the oracle's CPU rate
+where one is quoted.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-configs]
 
@@ -435,8 +436,8 @@ def other_configs(pt, args, only=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--lanes-table", type=str, default="",
