@@ -130,3 +130,29 @@ def test_ptmc_run_nnvt(base_pot):
     np.testing.assert_allclose([s.en_tot for s in states], e_full, rtol=1e-11)
     for s in states:
         assert np.all((s.config.positions**2).sum(-1) <= st["radius"] ** 2 + 1e-9)
+
+
+def test_nnvt_argument_errors(base_pot):
+    """Host-side checks of the C ABI: the two-species potential needs the
+    NNVT ensemble and the reverse; every run holds n1 + n2 atoms; n_atom_moves
+    other than n1 + n2 is refused."""
+    mc = pt.MCParams(10, 2, 6)
+    beta = 1.0 / (KB * np.array([500.0, 650.0]))
+    x0 = cluster(6, seed=1)
+    with pytest.raises(TypeError):
+        pt.PTEngine(base_pot, pt.NVT(6), pt.SphericalBC(8.0), mc)
+    with pytest.raises(ValueError):
+        pt.PTEngine(base_pot, pt.NNVT.of([4, 2], natommoves=3), pt.SphericalBC(8.0), mc)
+    with pytest.raises(ValueError):
+        pt.PTEngine(base_pot, pt.NNVT.of([3, 3]), pt.SphericalBC(8.0), mc)
+    eng = pt.PTEngine(base_pot, pt.NNVT.of([4, 2]), pt.SphericalBC(8.0), mc)
+    eng.add_run(7, beta, cluster(7, seed=2)[None], pt.SphericalBC(8.0))
+    with pytest.raises(pt.PTMCError):
+        eng.prepare()
+    eng.close()
+    # an NNVT engine without the two-species potential is refused at prepare
+    eng = pt.PTEngine(pt.ELJPotentialEven([-4.0, 0.0, 0.0, 4.0]), pt.NNVT.of([4, 2]), pt.SphericalBC(8.0), mc)
+    eng.add_run(6, beta, x0[None], pt.SphericalBC(8.0))
+    with pytest.raises(pt.PTMCError):
+        eng.prepare()
+    eng.close()
