@@ -156,3 +156,31 @@ def test_nnvt_argument_errors(base_pot):
     with pytest.raises(pt.PTMCError):
         eng.prepare()
     eng.close()
+
+
+def test_nnvt_recentre_matches_oracle(base_pot, oracle):
+    """The start cluster sits 12 bohr off the centre of an 8 bohr container:
+    every first trial lands outside, the 50th failure recentres the
+    configuration (src/MCMoves.jl:121-123, src/Configurations.jl:79-85) and
+    the redraws that follow land inside.  Device and oracle recentre with the
+    same arithmetic: identical moves, decisions and positions."""
+    st = json.loads((GOLDEN / "nnvt_states.json").read_text())
+    x0 = np.array(st["positions"]) + np.array([12.0, 0.0, 0.0])
+    ens = pt.NNVT.of([st["n1"], st["n2"]])
+    beta = 1.0 / (KB * np.array([500.0, 650.0]))
+    print("NNVT recentre 6 atoms", flush=True)
+    eng = _engine(base_pot, ens, beta, x0, st["radius"], seed=77)
+    run = oracle.OracleRun(6, beta, x0[None], pot=oracle.make_pot("nnp2", nnp2=oracle.NNP2(base_pot), n1=4, n_swaps=1),
+                           radius=st["radius"], exchange_mode=1, mode=oracle.engine_mode(), n_adjust=5, seed=77)
+    eng.set_trace(3)
+    eng.cycles(3)
+    eng.sync()
+    de, acc, idx = eng.trace(0, 3)
+    tr = run.cycles(3, trace=True)
+    np.testing.assert_array_equal(idx, tr["idx"])
+    np.testing.assert_array_equal(acc, tr["acc"])
+    x, en, _ = eng.state(0)
+    np.testing.assert_array_equal(x, run.x)
+    assert np.all((x**2).sum(-1) <= st["radius"] ** 2)
+    np.testing.assert_allclose(en, run.en_tot, rtol=1e-11)
+    eng.close()
