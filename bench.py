@@ -321,13 +321,14 @@ def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
     return out
 
 
-def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s):
+def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s, precision="fp64"):
     """Brass cluster in the NNVT ensemble (scripts/brass55.jl: 50 Cu + 5 Zn,
     NNVT([50, 5]; natomswaps=2), 10 temperatures 250-900 K, 14 A container,
     the reference's test potential: 5 radial + 26 angular functions,
     88-20-20-1 tanh networks weights.029 / weights.030, scaling.data).  Start:
     the icosahedral Cu55 of test/testing_data/save.data, atoms 51-55 Zn.  FP64
-    features, networks and totals; a cycle is 55 atom moves + 2 swaps."""
+    features, networks and totals (precision "fp64"), or binary32 ones with
+    FP64 totals ("fp32"); a cycle is 55 atom moves + 2 swaps."""
     from dataclasses import replace
 
     from paralleltemperingmontecarlo.nnp_setup import reference_brass_potential, read_save_data
@@ -340,7 +341,7 @@ def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s):
     temp = pt.TempGrid(250.0, 900.0, 10)
     rad = 14 * A2B
     mc = pt.MCParams(steps, 10, 55, n_adjust=100, mc_sample=1, n_bin=100)
-    eng = pt.PTEngine(pot, ens, pt.SphericalBC(rad), mc, pt.EngineOptions(precision="fp64", seed=1234))
+    eng = pt.PTEngine(pot, ens, pt.SphericalBC(rad), mc, pt.EngineOptions(precision=precision, seed=1234))
     for r in range(n_ladders):
         eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
     eng.prepare()
@@ -349,7 +350,7 @@ def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s):
     ms, nl = eng.kernel_time()
     swaps = [eng.counters(i)["count_vol"] for i in range(n_ladders)]
     out = dict(workload=f"NNVT brass Cu50Zn5, {n_ladders} ladder(s) x 10 temperatures 250-900 K",
-               moves_per_s=n_ladders * 10 * 57 * steps / dt, steps=steps, dtype="fp64",
+               moves_per_s=n_ladders * 10 * 57 * steps / dt, steps=steps, dtype=precision,
                acceptance=_acceptance(eng, n_ladders, 55, steps),
                swap_acceptance=float(sum(c[:, 0].sum() for c in swaps)) / max(sum(c[:, 1].sum() for c in swaps), 1),
                kernel_avg_launch_ms=ms / max(nl, 1), workgroups=n_ladders * 10)
@@ -425,6 +426,8 @@ def other_configs(pt, args, only=None):
         "delayed_acceptance": lambda: [bench_delayed_acceptance(pt, 256, 100, 4, 8, 30)],
         "nnvt": lambda: [bench_nnvt(pt, 1, 5, 20, cpu)],
         "nnvt_batched": lambda: [bench_nnvt(pt, 25, 5, 20, 0.0)],
+        "nnvt_f32": lambda: [bench_nnvt(pt, 1, 5, 20, 0.0, "fp32")],
+        "nnvt_f32_batched": lambda: [bench_nnvt(pt, 50, 5, 20, 0.0, "fp32")],
     }
     res = []
     for name, fn in todo.items():
@@ -450,7 +453,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--config", type=str, default="",
                     help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, "
-                         "delayed_acceptance, nnvt, nnvt_batched) and print its JSON")
+                         "delayed_acceptance, nnvt, nnvt_batched, nnvt_f32, nnvt_f32_batched) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs would "

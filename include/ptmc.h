@@ -141,8 +141,10 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
  * beyond N, a swap of one atom of each species, accepted with
  * min(1, exp(-beta dE)).  rad / ang: the tables of ptmc_nnp2_energy;
  * params1 / params2: the networks of species 1 / 2, shape [2 n_rad +
- * 3 n_ang, h1, h2, 1], activations 1 (linear) or 2 (tanh) per layer.  FP64
- * (precision 0), spherical container.  The volume counters of
+ * 3 n_ang, h1, h2, 1], activations 1 (linear) or 2 (tanh) per layer.
+ * Precision 0: FP64 throughout; 1: binary32 positions, distances, features
+ * and networks with FP64 energy sums and Metropolis test.  Spherical
+ * container.  The volume counters of
  * ptmc_get_counters hold (swaps accepted, swaps attempted) per temperature.
  * No counterpart of the reference for the split of a cycle's draws: the
  * reference's move index, atom and species picks come from Julia's RNG,

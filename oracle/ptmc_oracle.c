@@ -819,9 +819,13 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                     orc_nnp2_swap_g(pot->nnp2, x, n1, N - n1, i1, i2, &s->g[(size_t)r * N * ng], o->g_new);
                     new_en = orc_nnp2_energy(pot->nnp2, o->g_new, n1, N - n1, o->eat_new);
                     const double dE = new_en - s->en_tot[r];
-                    double prob = xexp(p, -dE * s->beta[r]);
-                    if (prob > 1.0) prob = 1.0;
-                    accept = prob >= d0.umet;
+                    if (p->fp32 && s->replay_acc) {
+                        accept = s->replay_acc[tix];
+                    } else {
+                        double prob = xexp(p, -dE * s->beta[r]);
+                        if (prob > 1.0) prob = 1.0;
+                        accept = prob >= d0.umet;
+                    }
                     if (s->trace_de) s->trace_de[tix] = dE;
                     if (s->trace_acc) s->trace_acc[tix] = (int8_t)accept;
                     if (s->trace_idx) s->trace_idx[tix] = N + i1 * N + i2;

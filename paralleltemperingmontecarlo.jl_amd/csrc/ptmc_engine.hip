@@ -26,6 +26,8 @@ SweepFn nnp_sweep_da_fn();
 SweepFn nnp_init_fn();
 SweepFn nnp2_sweep_fn();
 SweepFn nnp2_init_fn();
+SweepFn nnp2_sweep_fn_f32();
+SweepFn nnp2_init_fn_f32();
 void (*nnp_cycle_fn())(SweepArgs, int);
 hipError_t sur_launch_desc(const float *x, const int32_t *n_atoms, float *bd, float r_cut, int n, hipStream_t st);
 hipError_t sur_launch_collect(const SweepArgs &a, int n_slots, float *bd, double *be, int32_t *bn, long long base,
@@ -550,23 +552,26 @@ bool is_nnp(const ptmc_engine *h) { return h->pot.kind == POT_NNP; }
 bool is_nnp2(const ptmc_engine *h) { return h->pot.kind == POT_NNP2; }
 // potentials run one workgroup per slot (label exchange between cycles)
 bool is_slot(const ptmc_engine *h) { return is_nnp(h) || is_nnp2(h); }
+// slot engines keeping binary32 positions [slot][3][NNP_NP]
+bool slot_f32(const ptmc_engine *h) { return is_nnp(h) || (is_nnp2(h) && h->cfg.precision == 1); }
 
 // Slots of the two-species engine: as prepare_nnp, FP64 positions
 // [slot][3][NNP_NP], features [slot][NNP_NP][ng] and atom energies in HBM.
 int prepare_nnp2(ptmc_engine *h)
 {
     auto &n = h->nnp2;
-    if (h->cfg.precision != 0) return fail(h, PTMC_E_UNSUPPORTED, "the two-species network potential runs in FP64 (precision 0)");
     if (h->cfg.boundary != BC_SPHERE) return fail(h, PTMC_E_UNSUPPORTED, "the two-species network potential runs in a spherical container");
     int dev = 0, max_lds = 0;
     HIPCHK(h, hipGetDevice(&dev));
     HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     const int N = n.n1 + n.n2;
-    h->nnp2_lds = nnp2_lds_bytes(N, n.ng, n.n_rad + n.n_ang, n.h1, n.h2);
+    const bool f32 = h->cfg.precision == 1;
+    h->nnp2_lds = nnp2_lds_bytes(N, n.ng, n.n_rad + n.n_ang, n.h1, n.h2, f32 ? 4 : 8);
     if ((long long)h->nnp2_lds > max_lds)
         return fail(h, PTMC_E_UNSUPPORTED, "%d atoms with %d feature rows need %zu B of LDS (%d available)", N, n.ng,
                     h->nnp2_lds, max_lds);
-    const void *fs = (const void *)nnp2_sweep_fn(), *fi = (const void *)nnp2_init_fn();
+    const void *fs = (const void *)(f32 ? nnp2_sweep_fn_f32() : nnp2_sweep_fn());
+    const void *fi = (const void *)(f32 ? nnp2_init_fn_f32() : nnp2_init_fn());
     HIPCHK(h, hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp2_lds));
     HIPCHK(h, hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp2_lds));
     int slot = 0;
@@ -1324,7 +1329,7 @@ int ptmc_prepare(ptmc_handle h)
     if ((rc = dalloc(h, h->d_count_vol, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_count_exc, (size_t)2 * tb))) return rc;
     if ((rc = dalloc(h, h->d_hist, (size_t)h->hist_stride * tb))) return rc;
-    if (is_nnp(h)) {
+    if (slot_f32(h)) {
         std::vector<float> buf;
         scatter_coords_nnp(h, buf);
         std::vector<unsigned char> raw(buf.size() * sizeof(float));
@@ -1545,7 +1550,7 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
             box[t] = b[s];
         }
     }
-    if (x && is_nnp(h)) {
+    if (x && slot_f32(h)) {
         std::vector<float> blk((size_t)R * 3 * NNP_NP);
         HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float),
                             blk.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -1737,7 +1742,7 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             for (int s = 0; s < R; ++s) ts[s] = rh.info.slot_base + s;
             HIPCHK(h, hipMemcpy(h->d_temp_slot.p + rh.info.temp_base, ts.data(), R * 4, hipMemcpyHostToDevice));
         }
-        if (x && is_nnp(h)) {
+        if (x && slot_f32(h)) {
             std::vector<float> blk((size_t)R * 3 * NNP_NP, 0.0f);
             for (int s = 0; s < R; ++s)
                 for (int j = 0; j < N; ++j)

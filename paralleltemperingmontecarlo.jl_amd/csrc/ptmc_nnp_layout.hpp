@@ -100,11 +100,12 @@ __host__ __device__ inline size_t nnp2_params(int ng, int h1, int h2)
 {
     return (size_t)ng * h1 + h1 + (size_t)h1 * h2 + h2 + h2 + 1;
 }
-__host__ __device__ inline size_t nnp2_lds_bytes(int N, int ng, int nfn, int h1, int h2)
+// esz: bytes per element (8 for the FP64 kernels, 4 for binary32)
+__host__ __device__ inline size_t nnp2_lds_bytes(int N, int ng, int nfn, int h1, int h2, int esz)
 {
     const size_t d = (size_t)6 * N + 2 * (size_t)N * N + 4 * (size_t)N + (size_t)N * ng + 2 * (size_t)N +
                      2 * nnp2_params(ng, h1, h2) + nnp2_scratch_doubles(N, nfn, h1, h2);
-    return d * sizeof(double) + 2 * (size_t)N * sizeof(int);
+    return d * (size_t)esz + 2 * (size_t)N * sizeof(int);
 }
 
 }  // namespace ptmc

@@ -184,3 +184,41 @@ def test_nnvt_recentre_matches_oracle(base_pot, oracle):
     assert np.all((x**2).sum(-1) <= st["radius"] ** 2)
     np.testing.assert_allclose(en, run.en_tot, rtol=1e-11)
     eng.close()
+
+
+def test_nnvt_binary32_replay(base_pot, oracle):
+    """Binary32 positions, features and networks (FP64 energy sums): the
+    oracle replays the device's accept decisions in its FP64 arithmetic from
+    the same binary32 positions; moves and swaps match one for one,
+    positions bit for bit, energy changes to 2e-5 of |E| and the running
+    totals to 1e-5 relative (binary32 feature rounding)."""
+    n1, n2, n_swaps = 10, 4, 2
+    pot = replace(base_pot, n1=n1, n2=n2)
+    ens = pt.NNVT.of([n1, n2], natomswaps=n_swaps)
+    N = n1 + n2
+    x0 = cluster(N, seed=N)
+    beta = 1.0 / (KB * np.array([300.0, 550.0, 900.0, 1400.0]))
+    cycles = 16
+    print("NNVT binary32 replay", flush=True)
+    mc = pt.MCParams(100, len(beta), N, n_adjust=5)
+    eng = pt.PTEngine(pot, ens, pt.SphericalBC(14.0), mc, pt.EngineOptions(precision="fp32", exchange="every", seed=31))
+    eng.add_run(N, beta, x0[None], pt.SphericalBC(14.0))
+    eng.prepare()
+    run = oracle.OracleRun(N, beta, x0[None], pot=oracle.make_pot("nnp2", nnp2=oracle.NNP2(pot), n1=n1, n_swaps=n_swaps),
+                           radius=14.0, exchange_mode=1, mode=oracle.engine_mode(), n_adjust=5, seed=31, fp32=True)
+    np.testing.assert_allclose(eng.energies(0), run.en_tot, rtol=1e-5)
+    eng.set_trace(cycles)
+    eng.cycles(cycles)
+    eng.sync()
+    de, acc, idx = eng.trace(0, cycles)
+    tr = run.cycles(cycles, trace=True, replay_acc=acc)
+    np.testing.assert_array_equal(idx, tr["idx"])
+    scale = np.abs(run.en_tot).max()
+    assert np.abs(de - tr["de"]).max() <= 2e-5 * scale
+    assert (idx >= N).any() and acc[idx >= N].any()
+    x, en, _ = eng.state(0)
+    np.testing.assert_array_equal(x, run.x)
+    np.testing.assert_allclose(en, run.en_tot, rtol=1e-5)
+    c = eng.counters(0)
+    np.testing.assert_array_equal(c["count_vol"], run.count_vol)
+    eng.close()
