@@ -2,6 +2,10 @@
 // ensemble, compiled once per element type N2R (double: FP64 throughout;
 // float: binary32 positions, distances, features and networks, FP64 energy
 // sums and Metropolis test) by ptmc_nnp2_mc.hip and ptmc_nnp2_mc_f32.hip.
+// Reference: RuNNerPotential2Atom (src/EnergyEvaluation.jl:1287-1448,
+// :1622-1643, :1748-1768, :1824-1856), NNVT (src/Ensembles.jl:138-150,
+// :270-282), moves src/MCMoves.jl:115-145, :363-385, commit
+// src/swap_config.jl:223-277.
 //
 // One workgroup owns one configuration for a cycle of N + n_swaps moves, as
 // the single-species kernels do.  Its positions, squared distances and
@@ -25,7 +29,9 @@
 // G plus deltas accumulated per pair / triangle (DeltaMatrix.jl:23-624);
 // every sum here is gathered by one thread in a fixed order, so a run does
 // not depend on the schedule and agrees with the reference's arithmetic to
-// rounding.
+// rounding.  Angular functions of equal eta form groups that share each
+// triangle's cosine and exponential; (column, group) and (moved column,
+// group, chunk) tasks come from an LDS queue, heaviest first.
 // (no include guard: included once per element type)
 #include "ptmc_nnp_layout.hpp"
 #include "ptmc_sweep.hpp"
