@@ -179,19 +179,25 @@ __device__ void n2_radial_sums(const Nnp2Tables &T, const N2Lds &L, int N, int c
 }
 
 // Angular sums of group g at centre c over the trial configuration XT, over
-// the partner pairs (j, k) with j = ch, ch + nch, ...: out(l, row, v) adds
-// v to row `row` (partner pair 11 / 12 / 22) of the group's function l.
+// the partner pairs (j, k) with j = ch, ch + nch, ... (strided, so the
+// chunks mix near and far partners): out(l, row, v) adds v to row `row`
+// (partner pair 11 / 12 / 22) of the group's function l.  b is the other
+// moved atom of a swap (mb its row of RN / FN) or -1; the centre's own trial
+// row is read directly.
 template <typename Out>
-__device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, int N, int c,
-                                              int ch, int nch, Out out)
+__device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &L, const N2Grp &G, int N, int c, int b,
+                                              int mb, int ch, int nch, Out out)
 {
     const int n1 = T.n1;
+    const int mc = L.mi[c];
+    const N2R *rc_row = mc >= 0 ? &L.RN[mc * N] : &L.R2[c * N];
+    const N2R *fc_row = mc >= 0 ? &L.FN[mc * N] : &L.FC[c * N];
     const N2R *pc = &L.XT[3 * c];
     for (int j = ch; j < N; j += nch) {
         if (j == c) continue;
-        const N2R fcj = n2_ft(L, N, c, j);
+        const N2R fcj = fc_row[j];
         if (fcj == N2R(0.0)) continue;
-        const N2R rcj = n2_rt(L, N, c, j);
+        const N2R rcj = rc_row[j];
         const int sj = j < n1 ? 0 : 1;
         for (int blk = 0; blk < 2; ++blk) {
             const int klo = blk ? (j + 1 > n1 ? j + 1 : n1) : j + 1, khi = blk ? N : n1;
@@ -201,13 +207,13 @@ __device__ __forceinline__ void n2_group_sums(const Nnp2Tables &T, const N2Lds &
             for (int l = 0; l < NNP2_GMAX; ++l) acc[l] = N2R(0.0);
             for (int k = klo; k < khi; ++k) {
                 if (k == c) continue;
-                const N2R fck = n2_ft(L, N, c, k);
+                const N2R fck = fc_row[k];
                 if (fck == N2R(0.0)) continue;
-                const N2R fjk = n2_ft(L, N, j, k);
+                const N2R fjk = j == b ? L.FN[mb * N + k] : (k == b ? L.FN[mb * N + j] : L.FC[j * N + k]);
                 if (fjk == N2R(0.0)) continue;
+                const N2R rjk = j == b ? L.RN[mb * N + k] : (k == b ? L.RN[mb * N + j] : L.R2[j * N + k]);
                 N2R cth, e;
-                n2_tri_geom(pc, &L.XT[3 * j], &L.XT[3 * k], rcj, n2_rt(L, N, c, k), n2_rt(L, N, j, k), fcj, fck, fjk,
-                            G.eta, cth, e);
+                n2_tri_geom(pc, &L.XT[3 * j], &L.XT[3 * k], rcj, rc_row[k], rjk, fcj, fck, fjk, G.eta, cth, e);
                 n2_group_add(G, cth, e, acc);
             }
 #pragma unroll
@@ -366,7 +372,7 @@ __device__ double n2_rebuild(const Nnp2Tables &T, const N2Lds &L, int N, double 
                 const double *A = T.ang + (size_t)G.q[l] * 15;
                 for (int r = 0; r < 3; ++r) Dc[2 * T.n_rad + 3 * G.q[l] + r] = A[9 + 3 * sc + r];
             }
-            n2_group_sums(T, L, G, N, c, 0, 1, [&](int l, int row, N2R v) {
+            n2_group_sums(T, L, G, N, c, -1, 0, 0, 1, [&](int l, int row, N2R v) {
                 const double *A = T.ang + (size_t)G.q[l] * 15;
                 Dc[2 * T.n_rad + 3 * G.q[l] + row] += v * A[3 + 3 * sc + row];
             });
@@ -562,7 +568,8 @@ __global__ void __launch_bounds__(NNP2_NT) N2_SWEEP_ATTR nnp2_sweep_kernel(Sweep
                     N2R *P = &L.S[(((size_t)mm * nfn + T.n_rad + G.q[l]) * NNP2_NCH + ch) * 3];
                     P[0] = P[1] = P[2] = N2R(0.0);
                 }
-                n2_group_sums(T, L, G, N, ma[mm], ch, NNP2_NCH, [&](int l, int row, N2R v) {
+                const int other = nm == 2 ? ma[1 - mm] : -1;
+                n2_group_sums(T, L, G, N, ma[mm], other, 1 - mm, ch, NNP2_NCH, [&](int l, int row, N2R v) {
                     L.S[(((size_t)mm * nfn + T.n_rad + G.q[l]) * NNP2_NCH + ch) * 3 + row] += v;
                 });
             } else if (u < nAg + nBg + nAr) {
