@@ -89,7 +89,7 @@ __host__ __device__ inline size_t sur_lds_floats()
 // (layer activations, or the partial sums of rebuilt columns) | ints:
 // moved index[N], changed list[N]
 constexpr int NNP2_NT = 512;
-constexpr int NNP2_NCH = 16;
+constexpr int NNP2_NCH = 20;
 constexpr int NNP2_GMAX = 6;  // angular functions per eta group (larger groups are split)
 __host__ __device__ inline size_t nnp2_scratch_doubles(int N, int nfn, int h1, int h2)
 {
