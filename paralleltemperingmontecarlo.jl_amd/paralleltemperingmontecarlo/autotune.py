@@ -87,8 +87,10 @@ def tune_lanes(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500
     """Coordinate descent over {n_atoms: lanes} from ``start`` (default:
     ``start_table``); returns the fastest table found (moves per second of
     the whole batch is the objective), every candidate timed from the state
-    reached after ``adapt_cycles`` equilibration cycles.  Choices the engine
-    cannot plan (a ladder that does not fit one workgroup) are skipped."""
+    reached after ``adapt_cycles`` equilibration cycles; a candidate more than
+    3 % faster is timed again beside the current table and taken only if the
+    two timings together still say so.  Choices the engine cannot plan (a
+    ladder that does not fit one workgroup) are skipped."""
     sizes = sorted({r["n_atoms"] for r in runs}, reverse=True)
     table = dict(start if start is not None else start_table(sizes))
     state = _adapted_state(make_engine, runs, table, adapt_cycles) if adapt_cycles > 0 else None
@@ -111,7 +113,16 @@ def tune_lanes(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500
                 if log:
                     log(f"  N={n} lanes={kl}: {rate:.4g} moves/s")
                 if rate > best * 1.03:  # changes inside the timing noise are not taken
-                    best, table, improved = rate, trial, True
+                    # confirm against a fresh timing of the current table: a
+                    # single lucky timing does not move the table
+                    again = _rate(make_engine, runs, trial, warmup, cycles, state)
+                    cur = _rate(make_engine, runs, table, warmup, cycles, state)
+                    if log:
+                        log(f"  confirm N={n} lanes={kl}: {again:.4g} vs current {cur:.4g} moves/s")
+                    if rate + again > (best + cur) * 1.03:
+                        best, table, improved = 0.5 * (rate + again), trial, True
+                    else:
+                        best = 0.5 * (best + cur)
         if not improved:
             break
     return table
