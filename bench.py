@@ -110,15 +110,41 @@ def traffic_per_launch(kernel, cycles_per_launch):
     return None
 
 
-def cpu_baseline(runs, target_s=15.0):
+def host_info():
+    """The host the CPU legs run on: logical CPUs the OS reports, the CPUs
+    this process may run on, the thread count used (the CPU share of one GPU
+    on a shared node: OMP_NUM_THREADS when set, at most the usable CPUs) and
+    the CPU model name."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(usable, share) if share > 0 else usable)
+    model = "unknown"
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(host_nproc=nproc, host_usable_cpus=usable, threads=threads, host_model=model)
+
+
+def cpu_baseline(runs, target_s=15.0, one_core_s=5.0):
     """The oracle's restatement of the reference arithmetic (FP64, division
     pair loop, cached r^2 rows, sequential sums), one thread per run like
     Threads.@threads over replicas in the reference, timed on a bounded sample
-    of the same workload: all runs for as many cycles as fit about target_s."""
+    of the same workload: all runs for as many cycles as fit about target_s on
+    every usable thread, then the first runs on one thread for about
+    one_core_s."""
     from oracle import oracle as orc
 
     orc.lib()
-    threads = min(os.cpu_count() or 1, 16)
+    hi = host_info()
+    threads = hi["threads"]
     pot = orc.make_pot("elj_even", LJ_COEFF)
 
     def mk(r):
@@ -136,9 +162,19 @@ def cpu_baseline(runs, target_s=15.0):
         list(ex.map(lambda o: o.cycles(ncyc), oruns))
         dt = time.perf_counter() - t0
     moves = moves_per_cycle(runs) * ncyc
-    return dict(value=moves / dt, unit="moves/s", cores=threads, kind="port",
+    # one core: whole runs in workload order until about one_core_s
+    t0 = time.perf_counter()
+    m1 = 0
+    for o, r in zip(oruns, runs):
+        o.cycles(1)
+        m1 += r["n_atoms"] * r["n_temps"]
+        if time.perf_counter() - t0 > one_core_s:
+            break
+    dt1 = time.perf_counter() - t0
+    return dict(value=moves / dt, unit="moves/s", cores=threads, kind="port", one_core_value=m1 / dt1, **hi,
                 sample=f"all {len(runs)} runs of the workload, {ncyc} cycles after 1 probe cycle, "
-                       f"reference arithmetic in FP64 (oracle/ptmc_oracle.c), {threads} threads, {dt:.1f} s")
+                       f"reference arithmetic in FP64 (oracle/ptmc_oracle.c), {threads} threads, {dt:.1f} s; "
+                       f"one core: one cycle of the first runs, {m1} moves, {dt1:.1f} s")
 
 
 NE13_SETUP = ROOT / "tests" / "golden" / "ne13_setup.json"
@@ -148,7 +184,7 @@ P_ATM = 101325 / 2.9421015697e13  # Pa -> Hartree/bohr^3
 
 
 def _cpu_threads():
-    return min(os.cpu_count() or 1, 16)
+    return host_info()["threads"]
 
 
 def _production(eng, n_runs, warmup, steps):
@@ -188,8 +224,12 @@ def _cpu_rate(make_run, moves_per_cycle, target_s):
             t0 = time.perf_counter()
             list(ex.map(lambda o: o.cycles(ncyc), runs))
             dt = time.perf_counter() - t0
+    # one core: one copy of the ladder for one cycle (or the sample's cycles)
+    t0 = time.perf_counter()
+    runs[0].cycles(max(1, ncyc // 4))
+    dt1 = time.perf_counter() - t0
     return dict(cpu_moves_per_s=threads * moves_per_cycle * ncyc / dt, cpu_cores=threads, cpu_cycles=ncyc,
-                cpu_kind="port")
+                cpu_kind="port", cpu_one_core_moves_per_s=moves_per_cycle * max(1, ncyc // 4) / dt1)
 
 
 def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
@@ -248,7 +288,7 @@ def _bulk_start(n_atoms, vol_per_atom):
     return np.array(pts) - L / 2 + a / 4, L
 
 
-def bench_bulk(pt, n_atoms, ensemble, warmup, steps):
+def bench_bulk(pt, n_atoms, ensemble, warmup, steps, cpu_s=0.0):
     """Periodic neon (cubic box, minimum image) at the density of a 32-atom
     fcc cell of edge 8.7674 A, 48 temperatures 10-40 K, binary32 pair sums
     with FP64 totals; NPT adds one isotropic volume move per cycle at 1 atm."""
@@ -268,6 +308,18 @@ def bench_bulk(pt, n_atoms, ensemble, warmup, steps):
                moves_per_s=48 * n_atoms * steps / dt, steps=steps, dtype="fp32",
                acceptance=_acceptance(eng, 1, n_atoms, steps), max_fp32_vs_fp64_drift=float(np.max(eng.drift(0))))
     eng.close()
+    if cpu_s > 0:
+        from oracle import oracle as orc
+
+        orc.lib()
+        pot = orc.make_pot("elj_even", coeff)
+        p = P_ATM if ensemble == "npt" else 0.0
+        out.update(_cpu_rate(lambda k: orc.OracleRun(n_atoms, temp.beta_grid, x, pot=pot, bc="cubic", box=L,
+                                                     ensemble=ensemble, pressure=p, mode=orc.REFERENCE,
+                                                     exchange_mode=2, seed=1234, run_key=k),
+                             48 * n_atoms, cpu_s))
+        out["cpu_sample"] = "independent copies of the 48-temperature ladder, one per thread"
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_moves_per_s"]
     return out
 
 
@@ -418,7 +470,8 @@ def other_configs(pt, args, only=None):
     todo = {
         "ne13": lambda: [bench_ne13(pt, 1, 500, 20000, cpu)],
         "ne13_batched": lambda: [bench_ne13(pt, 512, 500, 2000, 0.0)],
-        "bulk": lambda: [bench_bulk(pt, n, ens, 200, 1000) for n in (108, 216, 256) for ens in ("nvt", "npt")],
+        "bulk": lambda: [bench_bulk(pt, n, ens, 200, 1000, cpu / 6 if n == 108 else 0.0)
+                         for n in (108, 216, 256) for ens in ("nvt", "npt")],
         "nnp": lambda: [bench_nnp(pt, 1, 50, 50, cpu)],
         "nnp_batched": lambda: [bench_nnp(pt, 64, 50, 50, 0.0)],
         # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
@@ -447,6 +500,8 @@ def main():
                     help='JSON {"N": lanes per configuration}; default: tuned on the workload before timing')
     ap.add_argument("--no-tune", action="store_true", help="use the engine's default lanes per configuration")
     ap.add_argument("--cycles-per-launch", type=int, default=500)
+    ap.add_argument("--adapt", type=int, default=3000,
+                    help="untimed step-size adaptation cycles before --warmup (n_adjust = 100)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")
@@ -500,8 +555,14 @@ def main():
     eng.prepare()
     plan = eng.plan_info()
 
+    # Untimed: step-size adaptation (every 100 cycles towards the 0.4-0.6
+    # acceptance window of the reference), then the warm-up, padded so that the
+    # timed region ends on a cycle with the periodic FP64 recompute: the drift
+    # it finds is the one reported.
+    recompute = 1000
+    pad = (-(args.adapt + args.warmup + args.steps)) % recompute
     eng.start_phase(False)
-    eng.cycles(args.warmup)
+    eng.cycles(args.adapt + args.warmup + pad)
     eng.sync()
     for i in range(len(runs)):
         e = eng.energies(i)
@@ -530,11 +591,17 @@ def main():
     dom_flops = flops_per_cycle(dom_runs) * args.steps
     achieved = dom_flops / (gi["ms"] * 1e-3) / 1e12
     drift = max(float(np.max(eng.drift(i))) for i in range(len(runs)))
-    acc = []
-    for i in range(0, len(runs), 37):
+    acc, acc_cold = [], []
+    for i in range(len(runs)):
         c = eng.counters(i)
-        acc.append(float(np.mean(c["count_atom"][:, 0])) / (runs[i]["n_atoms"] * args.steps))
+        a = c["count_atom"][:, 0] / (runs[i]["n_atoms"] * args.steps)
+        acc.append(float(np.mean(a)))
+        acc_cold.append(float(np.mean(a[: runs[i]["n_temps"] // 2])))
+    lanes_used = {}
+    for i, r in enumerate(runs):
+        lanes_used.setdefault(str(r["n_atoms"]), eng.lanes_of(i))
     eng.close()
+    cyc_launch = min(args.steps, args.cycles_per_launch)
 
     out = {
         "metric": METRIC,
@@ -558,19 +625,28 @@ def main():
             "workgroups": plan["tiles"],
             "launch_groups": plan["launch_groups"],
             "max_fp32_vs_fp64_energy_drift": drift,
-            "lanes_per_configuration": {str(k): v for k, v in sorted(table.items())} or "engine default",
+            "drift_note": "largest |E_running - E_FP64| (epsilon units) over all configurations and all FP64 "
+                          "recomputes (every 1000 cycles; the last one on the last timed cycle)",
+            "lanes_per_configuration": dict(sorted(lanes_used.items(), key=lambda kv: int(kv[0]))),
+            "lanes_requested": {str(k): v for k, v in sorted(table.items())} or "engine default",
             "tuning_s": t_tune,
+            "adapt_cycles": args.adapt,
+            "untimed_cycles_before": args.adapt + args.warmup + pad,
             "mean_atom_acceptance": float(np.mean(acc)),
+            "cold_half_atom_acceptance": [float(np.min(acc_cold)), float(np.median(acc_cold)),
+                                          float(np.max(acc_cold))],
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "VALU fp32 (pair energies are not GEMM-shaped; peak is the published fp32 rate)",
+            "bound": "valu",
+            "pipe": "VALU fp32 (pair energies are not GEMM-shaped; no HBM traffic in the move loop; peak is the "
+                    "published fp32 vector rate)",
             "achieved": achieved,
             "peak": PEAK_FP32,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": traffic_per_launch(kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
-                                          args.cycles_per_launch),
+                                          cyc_launch),
+            "cycles_per_launch": cyc_launch,
             "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
             "kernel_flops_per_launch": dom_flops / max(gi["launches"], 1),
             "kernel_avg_launch_ms": gi["ms"] / max(gi["launches"], 1),

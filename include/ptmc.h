@@ -216,6 +216,9 @@ int ptmc_seek(ptmc_handle h, int64_t cycle, int64_t phase_cycle);
 /* Per-move trace of the next n_cycles cycles (parity tests). */
 int ptmc_set_trace(ptmc_handle h, int32_t n_cycles);
 int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t *idx);
+/* Delayed acceptance: the surrogate energy change E_hat' - E_hat of each
+ * traced move ([temperature][cycle][move], NaN where none was evaluated). */
+int ptmc_get_trace_aux(ptmc_handle h, int32_t run, double *aux);
 
 /* Introspection for benchmarks: number of workgroups, kernel time (HIP
  * events on the engine's stream) accumulated since the last reset. */

@@ -35,7 +35,7 @@ class orc_params(C.Structure):
         ("seed", C.c_uint64), ("run", C.c_uint32),
         ("sweep0", C.c_int64), ("cycle0", C.c_int64),
         ("en_min", C.c_double), ("delta_en_hist", C.c_double), ("v_min", C.c_double), ("delta_v_hist", C.c_double),
-        ("separated", C.c_int32),
+        ("separated", C.c_int32), ("plant_decision", C.c_int32),
     ]
 
 
@@ -79,6 +79,7 @@ class orc_state(C.Structure):
         ("replay_acc", C.POINTER(C.c_int8)),
         ("box_h", _dp), ("max_box_h", _dp), ("lh_ratio", _dp), ("max_displ_sep", _dp),
         ("count_vol_xy", _i64p), ("count_vol_z", _i64p),
+        ("trace_prob", _dp), ("trace_u", _dp),
     ]
 
 
@@ -385,10 +386,13 @@ class OracleRun:
         tr = None
         if trace or replay_acc is not None:
             tr = dict(de=np.zeros((self.R, n, ns)), acc=np.zeros((self.R, n, ns), dtype=np.int8),
-                      idx=np.zeros((self.R, n, ns), dtype=np.int32))
+                      idx=np.zeros((self.R, n, ns), dtype=np.int32), prob=np.full((self.R, n, ns), np.nan),
+                      u=np.full((self.R, n, ns), np.nan))
             self.s.trace_de = _p(tr["de"])
             self.s.trace_acc = _p(tr["acc"], C.c_int8)
             self.s.trace_idx = _p(tr["idx"], C.c_int32)
+            self.s.trace_prob = _p(tr["prob"])
+            self.s.trace_u = _p(tr["u"])
         if replay_acc is not None:
             self._replay = np.ascontiguousarray(replay_acc, dtype=np.int8)
             self.s.replay_acc = _p(self._replay, C.c_int8)
@@ -396,6 +400,8 @@ class OracleRun:
         self.s.trace_de = None
         self.s.trace_acc = None
         self.s.trace_idx = None
+        self.s.trace_prob = None
+        self.s.trace_u = None
         self.s.replay_acc = None
         if rc == 1:
             raise RuntimeError("Error: too many moves out of binding sphere")

@@ -772,12 +772,23 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                     atom_move_energy(p, pot, s, r, i, t, s->en_tot[r], o);
                     new_en = o->new_en;
                     double dE = new_en - s->en_tot[r];
-                    if (p->fp32 && s->replay_acc) {
-                        accept = s->replay_acc[tix];
-                    } else {
+                    {
+                        /* src/Exchange.jl:53-57: the oracle's own FP64 decision,
+                         * traced also when the device's decision is replayed */
                         double prob = xexp(p, -dE * s->beta[r]);
                         if (prob > 1.0) prob = 1.0;
                         accept = prob >= d0.umet;
+                        if (s->trace_prob) s->trace_prob[tix] = prob;
+                        if (s->trace_u) s->trace_u[tix] = d0.umet;
+                        if (p->plant_decision) {
+                            double bp = s->beta[r], up = d0.umet;
+                            if (p->plant_decision == 1) bp = 2.0 * bp;
+                            if (p->plant_decision == 2) bp = -bp;
+                            if (p->plant_decision == 3) up = 1.0 - up;
+                            double pp = xexp(p, -dE * bp);
+                            accept = (pp > 1.0 ? 1.0 : pp) >= up;
+                        }
+                        if (p->fp32 && s->replay_acc) accept = s->replay_acc[tix];
                     }
                     if (s->trace_de) s->trace_de[tix] = dE;
                     if (s->trace_acc) s->trace_acc[tix] = (int8_t)accept;
@@ -819,12 +830,15 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                     orc_nnp2_swap_g(pot->nnp2, x, n1, N - n1, i1, i2, &s->g[(size_t)r * N * ng], o->g_new);
                     new_en = orc_nnp2_energy(pot->nnp2, o->g_new, n1, N - n1, o->eat_new);
                     const double dE = new_en - s->en_tot[r];
-                    if (p->fp32 && s->replay_acc) {
-                        accept = s->replay_acc[tix];
-                    } else {
+                    {
+                        /* src/Exchange.jl:53-57: the oracle's own FP64 decision,
+                         * traced also when the device's decision is replayed */
                         double prob = xexp(p, -dE * s->beta[r]);
                         if (prob > 1.0) prob = 1.0;
                         accept = prob >= d0.umet;
+                        if (s->trace_prob) s->trace_prob[tix] = prob;
+                        if (s->trace_u) s->trace_u[tix] = d0.umet;
+                        if (p->fp32 && s->replay_acc) accept = s->replay_acc[tix];
                     }
                     if (s->trace_de) s->trace_de[tix] = dE;
                     if (s->trace_acc) s->trace_acc[tix] = (int8_t)accept;
@@ -944,8 +958,10 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                     double arg = -dh * s->beta[r] + (double)N * xlog(p, v_new / v_old);
                     double prob = xexp(p, arg);
                     if (prob > 1.0) prob = 1.0;
+                    accept = prob >= d0.umet;
+                    if (s->trace_prob) s->trace_prob[tix] = prob;
+                    if (s->trace_u) s->trace_u[tix] = d0.umet;
                     if (p->fp32 && s->replay_acc) accept = s->replay_acc[tix];
-                    else accept = prob >= d0.umet;
                     if (s->trace_de) s->trace_de[tix] = dE;
                     if (s->trace_acc) s->trace_acc[tix] = (int8_t)accept;
                     if (s->trace_idx) s->trace_idx[tix] = i;

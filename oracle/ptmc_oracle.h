@@ -44,6 +44,10 @@ typedef struct {
     int64_t cycle0;       /* phase-local index of the first cycle minus one */
     double  en_min, delta_en_hist, v_min, delta_v_hist;
     int32_t separated;    /* NPT: separated xy / z / xyz volume moves (src/MCMoves.jl:301-343) */
+    /* planted mistake in the atom-move Metropolis decision, for tests that
+     * show a decision check catches it: 0 none, 1 beta doubled, 2 exponent
+     * sign flipped, 3 uniform taken as 1 - u */
+    int32_t plant_decision;
 } orc_params;
 
 /* RuNNer-style neural-network potential (src/EnergyEvaluation.jl:1182-1275,
@@ -133,6 +137,11 @@ typedef struct {
     double  *max_displ_sep;/* R*2 : xy, z volume steps (separated moves) */
     int64_t *count_vol_xy; /* R*2 */
     int64_t *count_vol_z;  /* R*2 */
+    /* optional per-move trace of the oracle's own Metropolis test in FP64,
+     * also in replay mode: prob = min(1, e^{-beta dE}) (NPT volume moves:
+     * with the P dV and N ln(V'/V) terms) and the uniform it is compared with */
+    double  *trace_prob;
+    double  *trace_u;
 } orc_state;
 
 /* Build the r^2 matrices / EAM components and total energies from positions

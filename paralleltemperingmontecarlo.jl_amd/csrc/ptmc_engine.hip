@@ -130,6 +130,7 @@ struct ptmc_engine {
     DevBuf<double> d_trace_de;
     DevBuf<int8_t> d_trace_acc;
     DevBuf<int32_t> d_trace_idx;
+    DevBuf<double> d_trace_aux;
     int trace_cycles = 0;
     int trace_pending = 0;
     int trace_nsteps = 0;
@@ -887,6 +888,7 @@ int ptmc_destroy(ptmc_handle h)
     h->d_trace_de.free_();
     h->d_trace_acc.free_();
     h->d_trace_idx.free_();
+    h->d_trace_aux.free_();
     for (auto st : h->side) (void)hipStreamSynchronize(st);
     for (auto ev : h->join) (void)hipEventDestroy(ev);
     for (auto ev : h->ev_pool) (void)hipEventDestroy(ev);
@@ -1401,6 +1403,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                 a.trace_de = h->d_trace_de.p;
                 a.trace_acc = h->d_trace_acc.p;
                 a.trace_idx = h->d_trace_idx.p;
+                a.trace_aux = h->d_trace_aux.p;
                 a.trace_cycles = h->trace_cycles;
                 a.trace_cycle = (int32_t)c;
             }
@@ -1804,6 +1807,9 @@ int ptmc_set_trace(ptmc_handle h, int32_t n_cycles)
     if ((rc = dalloc(h, h->d_trace_de, n))) return rc;
     if ((rc = dalloc(h, h->d_trace_acc, n))) return rc;
     if ((rc = dalloc(h, h->d_trace_idx, n))) return rc;
+    if ((rc = dalloc(h, h->d_trace_aux, n))) return rc;
+    // NaN (all bits set) where no surrogate change is recorded
+    HIPCHK(h, hipMemset(h->d_trace_aux.p, 0xff, n * sizeof(double)));
     h->trace_cycles = n_cycles;
     h->trace_pending = 1;
     h->trace_nsteps = nsteps;
@@ -1823,6 +1829,21 @@ int ptmc_get_trace(ptmc_handle h, int32_t run, double *de, int8_t *acc, int32_t 
     if (de) HIPCHK(h, hipMemcpy(de, h->d_trace_de.p + off, n * 8, hipMemcpyDeviceToHost));
     if (acc) HIPCHK(h, hipMemcpy(acc, h->d_trace_acc.p + off, n, hipMemcpyDeviceToHost));
     if (idx) HIPCHK(h, hipMemcpy(idx, h->d_trace_idx.p + off, n * 4, hipMemcpyDeviceToHost));
+    return PTMC_OK;
+}
+
+int ptmc_get_trace_aux(ptmc_handle h, int32_t run, double *aux)
+{
+    int rc = check_run(h, run);
+    if (rc) return rc;
+    if (!h->d_trace_aux.p) return fail(h, PTMC_E_STATE, "no trace recorded");
+    if (!aux) return fail(h, PTMC_E_ARG, "null buffer");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const RunHost &rh = h->runs[run];
+    const size_t per = (size_t)h->trace_cycles * h->trace_nsteps;
+    const size_t off = (size_t)rh.info.temp_base * per, n = (size_t)rh.d.n_temps * per;
+    HIPCHK(h, hipMemcpy(aux, h->d_trace_aux.p + off, n * 8, hipMemcpyDeviceToHost));
     return PTMC_OK;
 }
 

@@ -810,6 +810,7 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             a.trace_de[ti] = dd;
             a.trace_acc[ti] = (int8_t)accept;
             a.trace_idx[ti] = i;
+            if (da && a.trace_aux != nullptr) a.trace_aux[ti] = dEh;
         }
         if (accept) {
             // swap_atom_config! + swap_vars!(NNPVariables), src/swap_config.jl:40-51, :213-221

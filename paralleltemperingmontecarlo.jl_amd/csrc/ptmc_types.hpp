@@ -135,6 +135,7 @@ struct SweepArgs {
     double *trace_de;          // optional [temp][cycle][move]
     int8_t *trace_acc;
     int32_t *trace_idx;
+    double *trace_aux;         // optional: delayed acceptance's surrogate change per traced move
     PotParams pot;
     uint64_t seed;
     int64_t sweep0;            // global cycle counter of the first cycle

@@ -116,6 +116,7 @@ SIGNATURES = {
     "ptmc_seek": [_h, C.c_int64, C.c_int64],
     "ptmc_set_trace": [_h, C.c_int32],
     "ptmc_get_trace": [_h, C.c_int32, _dp, _i8p, _i32p],
+    "ptmc_get_trace_aux": [_h, C.c_int32, _dp],
     "ptmc_plan_info": [_h, _i32p, _i32p, _i64p],
     "ptmc_run_lanes": [_h, C.c_int32],
     "ptmc_kernel_time": [_h, _dp, _i64p, C.c_int32],

@@ -824,6 +824,14 @@ class PTEngine:
         self.eng.call("ptmc_get_trace", run, _lib.ptr(de), _lib.ptr(acc, C.c_int8), _lib.ptr(idx, C.c_int32))
         return de, acc, idx
 
+    def trace_aux(self, run: int, n_cycles: int):
+        """Surrogate energy change of every traced delayed-acceptance move
+        (NaN where none was evaluated), shaped like trace()."""
+        r = self.runs[run]
+        aux = np.zeros((r["n_temps"], n_cycles, r["n_atoms"]))
+        self.eng.call("ptmc_get_trace_aux", run, _lib.ptr(aux))
+        return aux
+
     def kernel_time(self, reset: bool = False):
         ms = C.c_double()
         n = C.c_int64()

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import gpu_available
+from decisions import assert_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -151,6 +152,8 @@ def test_fp32_separated_replay(oracle, ne13, kind):
     assert big.mean() < 1e-3, big.mean()
     assert err[atom].max() <= 2e-5 * scale + 3 * e_rc, (err[atom].max(), e_rc)
     assert err[~atom].max() <= 1e-4 * e_scale, err[~atom].max() / e_scale
+    assert_decisions(tr, acc, temp.beta_grid, np.where(atom, 2e-5 * scale + 3 * e_rc, 1e-4 * e_scale),
+                     min_compared=acc.size)
     assert np.max(np.abs(en - orc.en_tot)) <= 1e-4 * e_scale
     assert np.array_equal(xd, orc.x.reshape(xd.shape))
     assert np.array_equal(box[:, 0], orc.box) and np.array_equal(box[:, 1], orc.box_h)

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gpu_available
+from decisions import assert_decisions
 from eam_setup import brute_force_energy, cuboctahedron, eam_constants, fcc_box
 
 pytestmark = pytest.mark.gpu
@@ -161,6 +162,7 @@ def test_fp32_eam_replay(oracle, lanes):
     scale = abs(orc.en_tot[0])
     err = np.abs(de - tr["de"])
     assert err.max() <= 2e-6 * scale, err.max() / scale
+    assert_decisions(tr, acc, temp.beta_grid, 2e-6 * scale, min_compared=acc.size)
     x_dev, en_dev, _ = eng.state(run)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
     assert np.max(np.abs(en_dev - orc.en_tot)) <= 2e-5 * scale

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import gpu_available
+from decisions import assert_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +133,8 @@ def test_fp32_lookup_replay(oracle, lut):
     # a volume move's dE is the FP64 full energy minus the running total, which
     # carries the entry-off lookups of the accepted atom moves before it
     assert err[~atom].max() <= 1e-4 * e_scale + 10 * step, (err[~atom].max(), step)
+    assert_decisions(tr, acc, temp.beta_grid, np.where(atom, 3 * step, 1e-4 * e_scale + 10 * step),
+                     min_compared=acc.size)
     xd, en, box = eng.state(run)
     assert np.array_equal(xd, orc.x.reshape(xd.shape))
     assert np.array_equal(box, orc.box)

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gpu_available
+from decisions import assert_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -98,11 +99,46 @@ def test_moves_replay_against_oracle(oracle, saved):
     err = np.abs(de - tr["de"])
     assert err.max() <= 2e-6 * scale, err.max() / scale
     assert acc.mean() > 0.1
+    assert_decisions(tr, acc, temp.beta_grid, 2e-6 * scale, min_compared=acc.size)
     x_dev, en_dev, _ = eng.state(run)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
     assert np.max(np.abs(en_dev - orc.en_tot)) <= 2e-6 * scale
     ctr = eng.counters(run)
     assert np.array_equal(ctr["count_exc"], orc.count_exc)
+    eng.close()
+
+
+def test_synthetic_cu55_replay_40_temperatures(oracle, saved):
+    """The benchmarked network potential (8 radial + 80 angular functions,
+    88-20-20-1 tanh, N(0, 0.1) weights) on its own ladder, 40 temperatures
+    200-1400 K: every device dE within 2e-6 of |E| of the oracle's FP64 dE on
+    the same positions, every decision equal to the oracle's own FP64
+    Metropolis decision outside the near-tie band, positions bit for bit."""
+    pot = synthetic_cu55_potential(NNP / "scaling.data")
+    temp = pt.TempGrid(200.0, 1400.0, 40)
+    eng = _engine(pot, 40, seed=4242)
+    x0 = saved[0][0]
+    run = eng.add_run(55, temp.beta_grid, x0[None], pt.SphericalBC(RADIUS), run_key=3)
+    eng.prepare()
+    ncyc = 2
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc = oracle.OracleRun(55, temp.beta_grid, x0, pot=oracle.make_pot("nnp", nnp=_orc(oracle, pot)), radius=RADIUS,
+                           mode=oracle.engine_mode(1), fp32=True, exchange_mode=2, n_adjust=5, seed=4242, run_key=3)
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, replay_acc=acc)
+    assert np.array_equal(idx, tr["idx"])
+    scale = abs(orc.en_tot[0])
+    err = np.abs(de - tr["de"])
+    assert err.max() <= 2e-6 * scale, err.max() / scale
+    assert 0.05 < acc.mean() < 0.99
+    assert_decisions(tr, acc, temp.beta_grid, 2e-6 * scale, min_compared=acc.size)
+    x_dev, en_dev, _ = eng.state(run)
+    assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
+    assert np.max(np.abs(en_dev - orc.en_tot)) <= 2e-6 * scale
     eng.close()
 
 

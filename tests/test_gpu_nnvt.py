@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gpu_available
+from decisions import assert_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -216,6 +217,8 @@ def test_nnvt_binary32_replay(base_pot, oracle):
     scale = np.abs(run.en_tot).max()
     assert np.abs(de - tr["de"]).max() <= 2e-5 * scale
     assert (idx >= N).any() and acc[idx >= N].any()
+    # atom moves and swaps: the oracle's own FP64 decision outside the near-tie band
+    assert_decisions(tr, acc, beta, 2e-5 * scale, min_compared=acc.size)
     x, en, _ = eng.state(0)
     np.testing.assert_array_equal(x, run.x)
     np.testing.assert_allclose(en, run.en_tot, rtol=1e-5)

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import gpu_available
+from decisions import assert_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -205,6 +206,7 @@ def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble):
     atom = idx < N
     x_dev, en_dev, box = eng.state(run)
     e_scale = np.max(np.abs(orc.en_tot))
+    e_rc = 0.0
     if ensemble == "nvt":
         assert err.max() <= 2e-5 * scale, err.max() / scale
         assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-5 * e_scale
@@ -222,6 +224,10 @@ def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble):
         # so its error is the drift of the running total, flips included
         assert err[~atom].max() <= 1e-4 * e_scale, err[~atom].max() / e_scale
         assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-4 * e_scale
+    # decisions: atom moves within the atom dE bound (NPT: plus the cut-off
+    # near-tie pair), volume moves within the running-total bound
+    eps = np.where(atom, 2e-5 * scale + (0.0 if ensemble == "nvt" else 3 * e_rc), 1e-4 * e_scale)
+    assert_decisions(tr, acc, temp.beta_grid, eps, min_compared=acc.size)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
     assert np.array_equal(box, orc.box)
     if ensemble == "npt":
@@ -246,13 +252,22 @@ def _lj_runs(n_atoms, n_temps, seed=7):
 LJ_COEFF = [-4.0, 0.0, 0.0, 4.0]  # ELJPotentialEven{4}([-4 eps sigma^6, 0, 0, 4 eps sigma^12])
 
 
-@pytest.mark.parametrize("n_atoms,lanes", [(13, 1), (38, 2), (55, 4)])
-def test_fp32_lj_replay(oracle, n_atoms, lanes):
+# (N, lanes per configuration, temperatures, workgroup lanes the planner must
+# pick): the small cases and the instantiations the batched workload runs on
+# (sweep_kernel<float,4,4,0,0,KL,W>: KL 4 / W 256 for N = 75, KL 16 and 8 / W
+# 1024 for N = 98, KL 4 / W 256 for a 64-temperature N = 98 ladder)
+LJ_REPLAY_CASES = [(13, 1, 12, 256), (38, 2, 12, 256), (55, 4, 12, 256), (75, 4, 40, 256), (98, 16, 44, 1024),
+                   (98, 8, 40, 1024), (98, 4, 64, 256), (55, 8, 64, 1024)]
+
+
+@pytest.mark.parametrize("n_atoms,lanes,n_temps,wg", LJ_REPLAY_CASES)
+def test_fp32_lj_replay(oracle, n_atoms, lanes, n_temps, wg):
     """binary32 positions and pair sums: every device dE agrees with the FP64
     evaluation on the same positions to 2e-5 of the atom's interaction scale,
-    decisions differ only where |e^{-beta dE} - u| is within that error, and
-    replaying the device decisions reproduces the device positions bit for bit."""
-    x, rad, beta = _lj_runs(n_atoms, 12)
+    every device decision equals the oracle's own FP64 Metropolis decision
+    outside the near-tie band, and replaying the device decisions reproduces
+    the device positions bit for bit."""
+    x, rad, beta = _lj_runs(n_atoms, n_temps)
     R = len(beta)
     mc = pt.MCParams(100, R, n_atoms, n_adjust=20)
     bc = pt.SphericalBC(radius=rad)
@@ -260,6 +275,8 @@ def test_fp32_lj_replay(oracle, n_atoms, lanes):
                   lanes_per_replica=lanes, exchange="even_odd", seed=5)
     run = eng.add_run(n_atoms, beta, x[None], bc, run_key=11)
     eng.prepare()
+    assert eng.lanes_of(run) == lanes
+    assert eng.group_info(eng.group_of(run))["workgroup_lanes"] == wg
     ncyc = 25
     eng.set_trace(ncyc)
     eng.start_phase(False)
@@ -274,6 +291,8 @@ def test_fp32_lj_replay(oracle, n_atoms, lanes):
     scale = 6.0 * n_atoms ** 0  # pair well depth 1, ~12 neighbours at most: |sum e_ij| <= ~6
     err = np.abs(de - tr["de"])
     assert err.max() <= 2e-5 * scale, err.max()
+    assert 0.05 < acc.mean() < 0.95
+    assert_decisions(tr, acc, beta, 2e-5 * scale, min_compared=acc.size)
     x_dev, en_dev, _ = eng.state(run)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
     # FP64 running totals of both sides agree to the accumulated binary32 error

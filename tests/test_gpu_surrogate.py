@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gpu_available
+from decisions import decision_mismatches
 
 pytestmark = pytest.mark.gpu
 
@@ -68,32 +69,68 @@ def _cu13(saved):
     return saved[0][0][:13]
 
 
-def test_delayed_acceptance_replay(oracle, pot, saved):
+def _da_decisions(oracle, tr, de, acc, aux, beta, seed, run_key, eps):
+    """Both stages of every traced delayed-acceptance move against FP64
+    Metropolis tests: stage 1 accepts iff min(1, e^{-beta dEhat}) >= u with the
+    move's own uniform u (the oracle's trace gives u; dEhat is the device's
+    surrogate change); stage 2 accepts iff min(1, e^{-beta (dE - dEhat)}) >= u2,
+    dE the oracle's FP64 change on the replayed positions and u2 the 24-bit
+    uniform of the move's Philox block 255.  Stage-2 decisions may differ only
+    inside the near-tie band of the dE bound eps."""
+    have = np.isfinite(de)
+    assert np.isfinite(aux).all()
+    b = np.asarray(beta).reshape(-1, 1, 1)
+    p1 = np.minimum(1.0, np.exp(-b * aux))
+    s1 = p1 >= tr["u"]
+    tie1 = np.abs(p1 - tr["u"]) <= 1e-12 * p1
+    assert np.array_equal(s1[~tie1], have[~tie1])
+    R, nc, ns = de.shape
+    u2 = np.zeros(de.shape)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for t in range(R):
+        for c in range(nc):
+            for m in range(ns):
+                w = oracle.philox4x32([m, c, (255 << 20) | t, run_key], key)
+                u2[t, c, m] = (w[0] >> 8) * 2.0**-24
+    x2 = tr["de"] - aux
+    p2 = np.minimum(1.0, np.exp(-b * x2))
+    n, inside, outside = decision_mismatches(dict(prob=p2, u=u2, de=x2), acc, beta, eps, valid=have)
+    assert n == have.sum() and outside == 0, (n, inside, outside)
+    assert not acc[~have].any()
+
+
+@pytest.mark.parametrize("n_atoms,n_temps,ncyc", [(13, 4, 6), (55, 4, 3)])
+def test_delayed_acceptance_replay(oracle, pot, saved, n_atoms, n_temps, ncyc):
     """Stage-2 energies of delayed acceptance are the exact ones: replaying the
-    device decisions on the oracle reproduces positions and every evaluated dE."""
-    temp = pt.TempGrid(400.0, 900.0, 4)
-    radius = 8 * A2B
-    eng = _engine(pot, 13, radius, 21, 4)
-    run = eng.add_run(13, temp.beta_grid, _cu13(saved)[None], pt.SphericalBC(radius), run_key=2)
+    device decisions on the oracle reproduces positions and every evaluated
+    dE; both stages' decisions agree with FP64 Metropolis tests (Cu13, and the
+    Cu55 cluster of test/testing_data/save.data)."""
+    temp = pt.TempGrid(400.0, 900.0, n_temps)
+    radius = 8 * A2B if n_atoms == 13 else 14 * A2B
+    x0 = _cu13(saved) if n_atoms == 13 else saved[0][0]
+    eng = _engine(pot, n_atoms, radius, 21, n_temps)
+    run = eng.add_run(n_atoms, temp.beta_grid, x0[None], pt.SphericalBC(radius), run_key=2)
     eng.prepare()
     eng.surrogate_init(capacity=64, seed=3)
     eng.delayed_acceptance(True)
-    ncyc = 6
     eng.set_trace(ncyc)
     eng.start_phase(False)
     eng.cycles(ncyc)
     eng.sync()
     de, acc, idx = eng.trace(run, ncyc)
+    aux = eng.trace_aux(run, ncyc)
     rad, ang = pot.tables()
     nn = oracle.NNP(rad, ang, pot.r_cut, pot.nnp.num_nodes, pot.nnp.activation_functions, pot.nnp.parameters)
-    orc = oracle.OracleRun(13, temp.beta_grid, _cu13(saved), pot=oracle.make_pot("nnp", nnp=nn), radius=radius,
+    orc = oracle.OracleRun(n_atoms, temp.beta_grid, x0, pot=oracle.make_pot("nnp", nnp=nn), radius=radius,
                            mode=oracle.engine_mode(1), fp32=True, exchange_mode=2, n_adjust=10, seed=21, run_key=2)
     orc.start_phase(False)
     tr = orc.cycles(ncyc, replay_acc=acc)
     assert np.array_equal(idx, tr["idx"])
     have = np.isfinite(de)
     assert 0 < have.sum() < have.size  # some moves stopped at the surrogate stage
-    assert np.max(np.abs(de[have] - tr["de"][have])) <= 2e-6 * abs(orc.en_tot[0])
+    eps = 2e-6 * abs(orc.en_tot[0])
+    assert np.max(np.abs(de[have] - tr["de"][have])) <= eps
+    _da_decisions(oracle, tr, de, acc, aux, temp.beta_grid, 21, 2, eps)
     x_dev, _, _ = eng.state(run)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
     st = eng.surrogate_stats(run)
