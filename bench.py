@@ -80,7 +80,10 @@ def moves_per_cycle(runs):
 
 def kernel_symbol(kl, wgs):
     """Demangled name of the sweep kernel instantiation a launch group runs
-    (binary32, Lennard-Jones pair form, spherical container, NVT)."""
+    (binary32, Lennard-Jones pair form, spherical container, NVT); kl = 0:
+    the spread layout's kernel (lanes per configuration read per tile)."""
+    if kl == 0:
+        return f"void ptmc::spread_sweep_kernel<float, 4, 4, {wgs}>(ptmc::SweepArgs)"
     return f"void ptmc::sweep_kernel<float, 4, 4, 0, 0, {kl}, {wgs}>(ptmc::SweepArgs)"
 
 
@@ -500,6 +503,10 @@ def main():
                     help='JSON {"N": lanes per configuration}; default: tuned on the workload before timing')
     ap.add_argument("--no-tune", action="store_true", help="use the engine's default lanes per configuration")
     ap.add_argument("--cycles-per-launch", type=int, default=500)
+    ap.add_argument("--spread-streams", type=int, default=0,
+                    help="spread layout: independent run groups on their own streams")
+    ap.add_argument("--workgroup-lanes", type=int, default=0,
+                    help="0: each ladder in one workgroup; 64/128/256: ladders spread over workgroups of this width")
     ap.add_argument("--adapt", type=int, default=3000,
                     help="untimed step-size adaptation cycles before --warmup (n_adjust = 100)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -531,7 +538,8 @@ def main():
     runs = batched_lj_workload(args.runs)
     mc = pt.MCParams(args.steps, 1, 1, n_adjust=100, mc_sample=1, n_bin=100)
     opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
-                            lanes_per_replica=args.lanes, cycles_per_launch=args.cycles_per_launch)
+                            lanes_per_replica=args.lanes, cycles_per_launch=args.cycles_per_launch,
+                            workgroup_lanes=args.workgroup_lanes, spread_streams=args.spread_streams)
     def make_engine():
         return pt.PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
 
@@ -601,7 +609,7 @@ def main():
     for i, r in enumerate(runs):
         lanes_used.setdefault(str(r["n_atoms"]), eng.lanes_of(i))
     eng.close()
-    cyc_launch = min(args.steps, args.cycles_per_launch)
+    cyc_launch = 1 if args.workgroup_lanes else min(args.steps, args.cycles_per_launch)
 
     out = {
         "metric": METRIC,

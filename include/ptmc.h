@@ -62,6 +62,14 @@ typedef struct {
     int32_t cycles_per_launch; /* cycles per kernel launch (0 = 256) */
     int32_t device;            /* HIP device ordinal */
     uint64_t seed;             /* Philox key */
+    int32_t workgroup_lanes;   /* 0: every ladder in one workgroup, cycles_per_launch cycles per
+                                  launch with the exchange inside the kernel; 64, 128 or 256:
+                                  ladders spread over workgroups of this many lanes, one cycle per
+                                  launch and a per-ladder exchange kernel (pair potentials,
+                                  SphericalBC, NVT); same results either way */
+    int32_t spread_streams;    /* spread layout: the runs are cut into this many independent
+                                  groups launched on their own streams, so that one group's
+                                  cycle overlaps another's (0 or 1: one group); same results */
 } ptmc_config;
 
 /* One PT run.  Counterparts: TempGrid (src/InputParams.jl:77-95), the start

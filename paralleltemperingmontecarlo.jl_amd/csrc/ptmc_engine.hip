@@ -40,6 +40,12 @@ hipError_t sur_launch_adam(float *w, const float *grad, float *m, float *v, int 
 
 namespace {
 
+// A run's configurations t0 .. t0 + count - 1 held by lane clusters
+// local0 .. of one tile (pair potentials)
+struct Seg {
+    int tile, local0, t0, count;
+};
+
 struct RunHost {
     ptmc_run_desc d;
     std::vector<double> beta;
@@ -48,12 +54,14 @@ struct RunHost {
     int lanes = 256;  // workgroup width of the run's tile
     RunInfo info{};
     int tile = -1;
+    std::vector<Seg> segs;  // one segment, or several in the spread layout
 };
 
 struct Launch {
     SweepFn fn = nullptr;    // cycle kernel
     SweepFn init = nullptr;  // initial-energy kernel
     int tile0 = 0, ntiles = 0;
+    int run0 = 0, nruns = 0;  // spread layout: the group's runs in the run list
     int wgs = 256;
     int kl = 1;
     size_t lds = 0;
@@ -87,6 +95,7 @@ struct ptmc_engine {
     bool pot_set = false;
     std::vector<RunHost> runs;
     bool prepared = false;
+    bool spread = false;  // ladders spread over workgroups (cfg.workgroup_lanes > 0)
     std::string err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -112,6 +121,8 @@ struct ptmc_engine {
     // device state
     DevBuf<TileInfo> d_tiles;
     DevBuf<RunInfo> d_runs;
+    DevBuf<int32_t> d_run_list;  // spread layout: runs of each launch group, group after group
+    std::vector<int32_t> run_list;
     DevBuf<int32_t> d_slot_run, d_slot_temp, d_run_error;
     DevBuf<double> d_slot_en, d_slot_box, d_slot_boxh, d_slot_drift;
     DevBuf<unsigned char> d_coords, d_comps;
@@ -324,8 +335,199 @@ int choose_pair(ptmc_engine *h)
     return PTMC_OK;
 }
 
+// Spread layout (cfg.workgroup_lanes = W): runs of equal (lanes per
+// configuration, N) share tiles of W lanes; a ladder longer than a tile is cut
+// into nearly equal parts on consecutive tiles, so a run's slots stay
+// contiguous.  The runs are dealt into cfg.spread_streams groups of nearly
+// equal work (largest first, each to the lightest group); a group's tiles go
+// into one launch of the spread kernel, ordered by decreasing work so that
+// the heaviest workgroups are dispatched first, and the groups run on their
+// own streams.
+int plan_spread(ptmc_engine *h)
+{
+    const int W = h->cfg.workgroup_lanes;
+    const int prec = h->cfg.precision;
+    const size_t rsize = prec == 0 ? sizeof(double) : sizeof(float);
+    if (W != 64 && W != 128 && W != 256)
+        return fail(h, PTMC_E_ARG, "workgroup_lanes must be 0, 64, 128 or 256");
+    if (h->cfg.boundary != BC_SPHERE || h->cfg.ensemble != ENS_NVT || h->pot.kind == POT_EAM ||
+        h->pot.kind == POT_ELJB || h->pot.kind == POT_LUT || h->cfg.pair_form != 0)
+        return fail(h, PTMC_E_UNSUPPORTED,
+                    "the spread layout covers Horner-form pair potentials in a spherical container (NVT)");
+    if (h->cfg.spread_streams < 0 || h->cfg.spread_streams > 4)
+        return fail(h, PTMC_E_ARG, "spread_streams must be 0 to 4");
+    const KernelPair kp = lookup_spread(prec, h->pk, h->kc, W);
+    if (!kp.sweep) return fail(h, PTMC_E_UNSUPPORTED, "no spread kernel for pair form %d/%d", h->pk, h->kc);
+    int dev = 0;
+    HIPCHK(h, hipGetDevice(&dev));
+    int max_lds = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+
+    const int n_runs = (int)h->runs.size();
+    for (int r = 0; r < n_runs; ++r) {
+        RunHost &rh = h->runs[r];
+        const int N = rh.d.n_atoms;
+        const int kl = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
+                       : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
+                                                      : default_lanes(N, prec);
+        if (kl < 1 || kl > 16 || (kl & (kl - 1)) != 0 || kl > W)
+            return fail(h, PTMC_E_ARG, "run %d: lanes per configuration must be 1, 2, 4, 8 or 16 (at most %d)", r, W);
+        if (rh.d.n_temps > SPREAD_MAX_TEMPS)
+            return fail(h, PTMC_E_UNSUPPORTED, "run %d: at most %d temperatures in the spread layout", r,
+                        SPREAD_MAX_TEMPS);
+        const size_t lds = rsize * 3 * (size_t)((N + kl - 1) / kl) * W;
+        if (lds > (size_t)max_lds)
+            return fail(h, PTMC_E_UNSUPPORTED, "run %d: a tile needs %zu B of LDS (device limit %d)", r, lds, max_lds);
+        rh.kl = kl;
+        rh.lanes = W;
+        rh.segs.clear();
+    }
+    // runs into groups of nearly equal work (configurations x moves x slabs)
+    const int G = std::max(1, std::min(h->cfg.spread_streams, n_runs));
+    auto run_work = [&](int r) {
+        const RunHost &rh = h->runs[r];
+        return (double)rh.d.n_temps * rh.d.n_atoms * ((rh.d.n_atoms + rh.kl - 1) / rh.kl + 3) * rh.kl;
+    };
+    std::vector<int> by_work(n_runs);
+    for (int r = 0; r < n_runs; ++r) by_work[r] = r;
+    std::stable_sort(by_work.begin(), by_work.end(), [&](int x, int y) { return run_work(x) > run_work(y); });
+    std::vector<std::vector<int>> grp(G);
+    std::vector<double> load(G, 0.0);
+    for (int r : by_work) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        grp[g].push_back(r);
+        load[g] += run_work(r);
+    }
+    for (auto &v : grp) std::sort(v.begin(), v.end());
+
+    struct Part {
+        int run, t0, count;
+    };
+    h->tiles.clear();
+    h->launches.clear();
+    h->run_list.clear();
+    std::vector<TileInfo> ordered;
+    int slot = 0;
+    int64_t coord = 0;
+    size_t maxlds = 0;
+    for (int g = 0; g < G; ++g) {
+        std::map<std::pair<int, int>, std::vector<int>> groups;  // (kl, N) -> runs
+        for (int r : grp[g]) groups[{h->runs[r].kl, h->runs[r].d.n_atoms}].push_back(r);
+        std::vector<std::vector<Part>> packs;
+        std::vector<int> pack_kl, pack_n;
+        for (auto &gg : groups) {
+            const int kl = gg.first.first, N = gg.first.second, cap = W / kl;
+            std::vector<int> rs = gg.second;
+            std::stable_sort(rs.begin(), rs.end(),
+                             [&](int a, int b) { return h->runs[a].d.n_temps > h->runs[b].d.n_temps; });
+            const size_t first = packs.size();
+            std::vector<int> used;
+            for (int r : rs) {
+                const int R = h->runs[r].d.n_temps;
+                if (R > cap) {
+                    // consecutive tiles of its own, parts of nearly equal size
+                    const int k = (R + cap - 1) / cap;
+                    int t0 = 0;
+                    for (int p = 0; p < k; ++p) {
+                        const int c = R / k + (p < R % k ? 1 : 0);
+                        packs.push_back({Part{r, t0, c}});
+                        pack_kl.push_back(kl);
+                        pack_n.push_back(N);
+                        used.push_back(cap);
+                        t0 += c;
+                    }
+                    continue;
+                }
+                int t = -1;
+                for (size_t k = first; k < packs.size(); ++k)
+                    if (used[k - first] + R <= cap) {
+                        t = (int)k;
+                        break;
+                    }
+                if (t < 0) {
+                    packs.push_back({});
+                    pack_kl.push_back(kl);
+                    pack_n.push_back(N);
+                    used.push_back(0);
+                    t = (int)packs.size() - 1;
+                }
+                packs[t].push_back(Part{r, 0, R});
+                used[t - first] += R;
+            }
+        }
+        // slots in pack order: a run's parts sit on consecutive tiles
+        std::vector<TileInfo> tiles;
+        std::vector<std::vector<std::pair<int, Seg>>> segs_of(packs.size());
+        for (size_t k = 0; k < packs.size(); ++k) {
+            TileInfo ti{};
+            ti.slot0 = slot;
+            ti.n_atoms = pack_n[k];
+            ti.kl = pack_kl[k];
+            ti.n_slab = (ti.n_atoms + ti.kl - 1) / ti.kl;
+            int local = 0;
+            for (const Part &p : packs[k]) {
+                RunHost &rh = h->runs[p.run];
+                if (p.t0 == 0) {
+                    rh.info.slot_base = slot + local;
+                    rh.info.local_base = local;
+                }
+                segs_of[k].push_back({p.run, Seg{-1, local, p.t0, p.count}});
+                local += p.count;
+            }
+            ti.n_slots = local;
+            slot += local;
+            tiles.push_back(ti);
+        }
+        // heaviest tiles first: configurations x moves x (slabs + per-move overhead)
+        std::vector<int> order(tiles.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        auto work = [&](int k) { return (long long)tiles[k].n_slots * tiles[k].n_atoms * (tiles[k].n_slab + 3); };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+        Launch L;
+        L.tile0 = (int)ordered.size();
+        L.ntiles = (int)tiles.size();
+        L.wgs = W;
+        L.kl = 0;
+        L.fn = kp.sweep;
+        L.init = kp.init;
+        L.run0 = (int)h->run_list.size();
+        L.nruns = (int)grp[g].size();
+        for (int k : order) {
+            TileInfo ti = tiles[k];
+            ti.coord_off = coord;
+            coord += (int64_t)3 * ti.n_slab * W;
+            L.lds = std::max(L.lds, rsize * 3 * (size_t)ti.n_slab * W);
+            for (auto &rs : segs_of[k]) {
+                Seg sg = rs.second;
+                sg.tile = (int)ordered.size();
+                RunHost &rh = h->runs[rs.first];
+                if (sg.t0 == 0) rh.tile = sg.tile;
+                rh.segs.push_back(sg);
+            }
+            ordered.push_back(ti);
+        }
+        for (int r : grp[g]) h->run_list.push_back(r);
+        maxlds = std::max(maxlds, L.lds);
+        h->launches.push_back(L);
+    }
+    for (auto &rh : h->runs)
+        std::sort(rh.segs.begin(), rh.segs.end(), [](const Seg &x, const Seg &y) { return x.t0 < y.t0; });
+    h->tiles = ordered;
+    h->n_slots = slot;
+    h->n_coord = coord;
+    h->n_comp = 0;
+    HIPCHK(h, hipFuncSetAttribute((const void *)kp.sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+    HIPCHK(h, hipFuncSetAttribute((const void *)kp.init, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+    int rc = upload(h, h->d_run_list, h->run_list);
+    return rc;
+}
+
 int plan(ptmc_engine *h)
 {
+    if (h->cfg.workgroup_lanes > 0) {
+        h->spread = true;
+        return plan_spread(h);
+    }
     const int prec = h->cfg.precision;
     const size_t rsize = prec == 0 ? sizeof(double) : sizeof(float);
     const bool eam = h->pot.kind == POT_EAM;
@@ -398,6 +600,7 @@ int plan(ptmc_engine *h)
             ti.slot0 = slot;
             ti.n_atoms = N;
             ti.n_slab = (N + kl - 1) / kl;
+            ti.kl = kl;
             ti.coord_off = 0;  // filled below (needs the launch's wgs)
             int local = 0;
             for (int r : packs[k]) {
@@ -405,6 +608,7 @@ int plan(ptmc_engine *h)
                 rh.info.slot_base = slot + local;
                 rh.info.local_base = local;
                 rh.tile = (int)h->tiles.size();
+                rh.segs = {Seg{rh.tile, local, 0, rh.d.n_temps}};
                 local += rh.d.n_temps;
             }
             ti.n_slots = local;
@@ -461,7 +665,10 @@ int plan(ptmc_engine *h)
         };
         return work(x) > work(y);
     });
-    for (auto &rh : h->runs) rh.tile = old_to_new[rh.tile];
+    for (auto &rh : h->runs) {
+        rh.tile = old_to_new[rh.tile];
+        for (auto &g : rh.segs) g.tile = old_to_new[g.tile];
+    }
     h->tiles = ordered;
     h->n_coord = coord;
     h->n_comp = comp;
@@ -475,19 +682,21 @@ void scatter_coords(ptmc_engine *h, std::vector<Real> &buf)
 {
     buf.assign((size_t)h->n_coord, (Real)0);
     const Real far = h->cfg.boundary == BC_SPHERE ? (Real)(sizeof(Real) == 4 ? 1.0e18 : 1.0e100) : (Real)0;
-    for (auto &rh : h->runs) {
-        const TileInfo &ti = h->tiles[rh.tile];
-        const int N = rh.d.n_atoms, A = ti.n_slab, kl = rh.kl;
-        for (int t = 0; t < rh.d.n_temps; ++t) {
-            const int lg = rh.info.local_base + t;  // slot t holds temperature t at start
-            const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
-            for (int j = 0; j < A * kl; ++j) {
-                const int s = j / kl, lane = lg * kl + j % kl;
-                for (int c = 0; c < 3; ++c)
-                    buf[(size_t)ti.coord_off + ((size_t)s * 3 + c) * rh.lanes + lane] = j < N ? (Real)x[3 * j + c] : far;
+    for (auto &rh : h->runs)
+        for (const Seg &g : rh.segs) {
+            const TileInfo &ti = h->tiles[g.tile];
+            const int N = rh.d.n_atoms, A = ti.n_slab, kl = rh.kl;
+            for (int t = g.t0; t < g.t0 + g.count; ++t) {
+                const int lg = g.local0 + (t - g.t0);  // slot t holds temperature t at start
+                const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
+                for (int j = 0; j < A * kl; ++j) {
+                    const int s = j / kl, lane = lg * kl + j % kl;
+                    for (int c = 0; c < 3; ++c)
+                        buf[(size_t)ti.coord_off + ((size_t)s * 3 + c) * rh.lanes + lane] =
+                            j < N ? (Real)x[3 * j + c] : far;
+                }
             }
         }
-    }
 }
 
 int launch_wgs(ptmc_engine *h, int tile)
@@ -829,6 +1038,7 @@ int ptmc_destroy(ptmc_handle h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->d_tiles.free_();
     h->d_runs.free_();
+    h->d_run_list.free_();
     h->d_slot_run.free_();
     h->d_slot_temp.free_();
     h->d_run_error.free_();
@@ -1425,6 +1635,70 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
         }
         done = n_cycles;
     }
+    if (h->spread) {
+        // one launch of the tiles per cycle, then the per-ladder exchange,
+        // adaptation and sampling; a last launch without moves applies a
+        // recompute due after the final cycle
+        const bool tracing = h->trace_pending > 0;
+        if (tracing && n_cycles > h->trace_cycles)
+            return fail(h, PTMC_E_STATE, "traced cycles must be run in one ptmc_cycles call of at most %d cycles",
+                        h->trace_cycles);
+        h->trace_pending = 0;
+        const size_t ng = h->launches.size();
+        // groups on their own streams (one group: the engine stream), joined at the end
+        auto stream_of = [&](size_t g) { return ng == 1 ? h->stream : h->side[g % h->side.size()]; };
+        if (ng > 1) {
+            HIPCHK(h, hipEventRecord(h->fork, h->stream));
+            for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamWaitEvent(stream_of(g), h->fork, 0));
+        }
+        for (int64_t c = 0; c < n_cycles; ++c) {
+            for (size_t g = 0; g < ng; ++g) {
+                const Launch &L = h->launches[g];
+                SweepArgs a = base_args(h);
+                a.sweep0 = h->sweep_counter;
+                a.cycle0 = h->phase_counter;
+                a.n_cycles = 1;
+                a.sampling = h->production;
+                a.init_run = -1;
+                a.run_list = h->d_run_list.p + L.run0;
+                if (tracing) {
+                    a.trace_de = h->d_trace_de.p;
+                    a.trace_acc = h->d_trace_acc.p;
+                    a.trace_idx = h->d_trace_idx.p;
+                    a.trace_cycles = h->trace_cycles;
+                    a.trace_cycle = (int32_t)c;
+                }
+                if ((rc = launch_one(h, g, a, stream_of(g), false))) return rc;
+                void *args[] = {&a};
+                HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(L.nruns), dim3(64), args, 0, stream_of(g)));
+            }
+            if (h->pending.size() >= 512) {
+                for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamSynchronize(stream_of(g)));
+                if ((rc = collect_timings(h))) return rc;
+            }
+            h->sweep_counter += 1;
+            h->phase_counter += 1;
+        }
+        if (n_cycles > 0 && h->cfg.recompute_every > 0 && h->sweep_counter % h->cfg.recompute_every == 0) {
+            for (size_t g = 0; g < ng; ++g) {
+                const Launch &L = h->launches[g];
+                SweepArgs a = base_args(h);
+                a.tiles = h->d_tiles.p + L.tile0;
+                a.sweep0 = h->sweep_counter;
+                a.cycle0 = h->phase_counter;
+                a.n_cycles = 0;
+                a.init_run = -1;
+                void *args[] = {&a};
+                HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, stream_of(g)));
+            }
+        }
+        if (ng > 1)
+            for (size_t g = 0; g < ng; ++g) {
+                HIPCHK(h, hipEventRecord(h->join[g], stream_of(g)));
+                HIPCHK(h, hipStreamWaitEvent(h->stream, h->join[g], 0));
+            }
+        done = n_cycles;
+    }
     while (done < n_cycles) {
         const int64_t chunk = std::min<int64_t>(n_cycles - done, h->cfg.cycles_per_launch);
         SweepArgs a = base_args(h);
@@ -1570,14 +1844,15 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
                 for (int c = 0; c < 3; ++c)
                     x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * NNP_NP + j];
     } else if (x) {
-        const TileInfo &ti = h->tiles[rh.tile];
+      for (const Seg &g : rh.segs) {
+        const TileInfo &ti = h->tiles[g.tile];
         const int A = ti.n_slab, kl = rh.kl;
         const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
         const size_t nblk = (size_t)3 * A * rh.lanes;
         std::vector<unsigned char> blk(nblk * rsize);
         HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
-        for (int s = 0; s < R; ++s) {
-            const int t = temp[s], lg = rh.info.local_base + s;
+        for (int s = g.t0; s < g.t0 + g.count; ++s) {
+            const int t = temp[s], lg = g.local0 + (s - g.t0);
             for (int j = 0; j < N; ++j) {
                 const int sl = j / kl, lane = lg * kl + j % kl;
                 for (int c = 0; c < 3; ++c) {
@@ -1593,6 +1868,7 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
                 }
             }
         }
+      }
     }
     return PTMC_OK;
 }
@@ -1740,7 +2016,7 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             HIPCHK(h, hipMemcpy(h->d_slot_box.p + rh.info.slot_base, bl.data(), R * 8, hipMemcpyHostToDevice));
             HIPCHK(h, hipMemcpy(h->d_slot_boxh.p + rh.info.slot_base, bh.data(), R * 8, hipMemcpyHostToDevice));
         }
-        if (is_slot(h)) {
+        if (is_slot(h) || h->spread) {
             std::vector<int32_t> ts(R);
             for (int s = 0; s < R; ++s) ts[s] = rh.info.slot_base + s;
             HIPCHK(h, hipMemcpy(h->d_temp_slot.p + rh.info.temp_base, ts.data(), R * 4, hipMemcpyHostToDevice));
@@ -1761,14 +2037,15 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(double), blk.data(),
                                 blk.size() * sizeof(double), hipMemcpyHostToDevice));
         } else if (x) {
-            const TileInfo &ti = h->tiles[rh.tile];
+          for (const Seg &g : rh.segs) {
+            const TileInfo &ti = h->tiles[g.tile];
             const int A = ti.n_slab, kl = rh.kl;
             const size_t rsize = h->cfg.precision == 0 ? 8 : 4;
             const size_t nblk = (size_t)3 * A * rh.lanes;
             std::vector<unsigned char> blk(nblk * rsize);
             HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)ti.coord_off * rsize, blk.size(), hipMemcpyDeviceToHost));
-            for (int s = 0; s < R; ++s) {
-                const int lg = rh.info.local_base + s;
+            for (int s = g.t0; s < g.t0 + g.count; ++s) {
+                const int lg = g.local0 + (s - g.t0);
                 for (int j = 0; j < N; ++j) {
                     const int sl = j / kl, lane = lg * kl + j % kl;
                     for (int c = 0; c < 3; ++c) {
@@ -1783,6 +2060,7 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
                 }
             }
             HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)ti.coord_off * rsize, blk.data(), blk.size(), hipMemcpyHostToDevice));
+          }
         }
         SweepArgs a = base_args(h);
         a.init_run = run;

@@ -1,0 +1,423 @@
+// Spread layout of the pair-potential sweep (spherical container, NVT).
+//
+// The ladder-per-workgroup kernel (sweep_kernel) keeps every temperature of a
+// run in one workgroup so the exchange can run in LDS, many cycles per
+// launch.  A 64-temperature ladder of a 98-atom cluster then fills one CU
+// (16 waves, 4 per SIMD) while most of the chip idles, and the cycle takes as
+// long as that CU needs.  Here a ladder is cut into tiles of at most
+// LANES / KL configurations (LANES = 64, 128 or 256 lanes, KL lanes per
+// configuration chosen per run and read from the tile at run time), so its
+// configurations spread over as many CUs as there are tiles, and the tiles of
+// all runs go out in one launch, heaviest first.  One launch is one cycle of
+// moves (src/MCRun.jl:169-182); the exchange, step-size adaptation and
+// sampling of the cycle (src/MCRun.jl:203-212, :248-250) follow in
+// spread_cycle_kernel, one workgroup per ladder.  The arithmetic of a move,
+// the random stream and the order of the per-cluster sums are those of
+// sweep_kernel, so both layouts give the same results bit for bit.
+#pragma once
+#include "ptmc_sweep.hpp"
+
+namespace ptmc {
+
+// group_sum for a cluster width kl known at run time (uniform across the
+// workgroup, so the branches are scalar); the same steps as group_sum<KL>.
+template <typename T>
+__device__ __forceinline__ T group_sum_rt(T v, int kl)
+{
+    if (kl >= 2) v = v + dpp_perm<0xB1>(v);
+    if (kl >= 4) v = v + dpp_perm<0x4E>(v);
+    if (kl >= 8) v = v + dpp_perm<0x141>(v);
+    if (kl >= 16) v = v + dpp_perm<0x140>(v);
+    return v;
+}
+
+// pair_sum64 of a spherical-container configuration (no cut-off) for a run
+// time cluster width: rows j of this lane, partners l > j, cluster sum.
+template <typename Real, int PK, int KC, int LANES>
+__device__ double sphere_pair_sum64(const Real *X, int gb, int q, int N, int kl, int kls, const double (&c64)[KC])
+{
+    const int A = (N + kl - 1) >> kls;
+    double part = 0.0;
+    for (int s = 0; s < A; ++s) {
+        const int j = (s << kls) + q;
+        if (j >= N) break;
+        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)];
+        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)];
+        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)];
+        for (int l = j + 1; l < N; ++l) {
+            const int sl = l >> kls, ll = gb + (l & (kl - 1));
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
+            const double dx = xj - xl, dy = yj - yl, dz = zj - zl;
+            part += pair_energy64<PK, KC>(dx * dx + dy * dy + dz * dz, c64);
+        }
+    }
+    return group_sum_rt(part, kl);
+}
+
+// sphere_redraw for a run-time cluster width (src/MCMoves.jl:115-135).
+template <typename Real, int LANES>
+__device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls, int A, int si, int li,
+                                 Real radius2, Real delta_r, uint64_t seed, uint32_t run_key, uint32_t temp,
+                                 uint32_t sweep, uint32_t m, uint32_t nsteps, Real &xi, Real &yi, Real &zi, Real &tx,
+                                 Real &ty, Real &tz)
+{
+    int count = 0;
+    uint32_t attempt = 0;
+    bool ok = true;
+    while (outside_sphere<Real>(tx, ty, tz, radius2)) {
+        count += 1;
+        if (count == 50) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int s = 0; s < A; ++s) {
+                if ((s << kls) + q >= N) break;
+                sx += (double)X[xoff<LANES>(s, 0, tid)];
+                sy += (double)X[xoff<LANES>(s, 1, tid)];
+                sz += (double)X[xoff<LANES>(s, 2, tid)];
+            }
+            sx = group_sum_rt(sx, kl) / (double)N;
+            sy = group_sum_rt(sy, kl) / (double)N;
+            sz = group_sum_rt(sz, kl) / (double)N;
+            for (int s = 0; s < A; ++s) {
+                if ((s << kls) + q >= N) break;
+                X[xoff<LANES>(s, 0, tid)] = (Real)((double)X[xoff<LANES>(s, 0, tid)] - sx);
+                X[xoff<LANES>(s, 1, tid)] = (Real)((double)X[xoff<LANES>(s, 1, tid)] - sy);
+                X[xoff<LANES>(s, 2, tid)] = (Real)((double)X[xoff<LANES>(s, 2, tid)] - sz);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        } else {
+            attempt += 1;
+            MoveDraw<Real> dd = move_draw<Real>(seed, run_key, temp, sweep, m, attempt, nsteps);
+            xi = X[xoff<LANES>(si, 0, li)];
+            yi = X[xoff<LANES>(si, 1, li)];
+            zi = X[xoff<LANES>(si, 2, li)];
+            make_trial<Real, BC_SPHERE, false>(xi, yi, zi, dd, delta_r, Real(0), Real(0), tx, ty, tz);
+            if (count == 100) {
+                ok = false;
+                break;
+            }
+        }
+    }
+    xi = X[xoff<LANES>(si, 0, li)];
+    yi = X[xoff<LANES>(si, 1, li)];
+    zi = X[xoff<LANES>(si, 2, li)];
+    return ok;
+}
+
+// One cycle of moves of every configuration of the launch's tiles.  With
+// n_cycles = 0 the launch only applies a pending FP64 recompute.
+//
+// The FP64 recompute that sweep_kernel makes at the end of cycle c (after
+// its exchange and sampling) is made here at the start of the launch of cycle
+// c + 1, when sweep0 % recompute_every == 0; the engine adds a launch with
+// n_cycles = 0 after a call's last cycle so that it is never left pending.
+// Repeating it is harmless: the second recompute finds E equal to the sum.
+template <typename Real, int PK, int KC, int LANES>
+__global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
+{
+    constexpr bool F32 = sizeof(Real) == 4;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const TileInfo tile = a.tiles[blockIdx.x];
+    const int KL = tile.kl;
+    const int kls = __builtin_ctz((unsigned)KL);
+    const int tid = threadIdx.x;
+    const int lg = tid >> kls;
+    const int q = tid & (KL - 1);
+    const int gb = lg << kls;
+    const bool active = lg < tile.n_slots;
+    const int N = tile.n_atoms;
+    const int A = tile.n_slab;
+    const uint32_t nsteps = (uint32_t)N;
+    Real *X = reinterpret_cast<Real *>(lds_raw);
+
+    Real c[KC];
+    double c64[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const double v = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+        c64[k] = v;
+        c[k] = (Real)v;
+    }
+    if constexpr (PK == PK_LJ) {
+        c[0] = (Real)a.pot.coeff[0];
+        c[1] = (Real)a.pot.coeff[3];
+    }
+
+    Real *gX = reinterpret_cast<Real *>(a.coords) + tile.coord_off;
+    for (int e = tid; e < 3 * A * LANES; e += LANES) X[e] = gX[e];
+    __syncthreads();
+
+    const int slot = tile.slot0 + lg;
+    if (active) {
+        const RunInfo &ri = a.runs[a.slot_run[slot]];
+        const int tb = ri.temp_base;
+        const uint32_t run_key = ri.run_key;
+        const Real radius2 = (Real)ri.radius2;
+        const int t = a.slot_temp[slot];
+        double E = a.slot_en[slot];
+        if (a.recompute_every > 0 && a.sweep0 > 0 && a.sweep0 % a.recompute_every == 0) {
+            const double ef = sphere_pair_sum64<Real, PK, KC, LANES>(X, gb, q, N, KL, kls, c64);
+            const double dr = __builtin_fabs(E - ef);
+            if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
+            E = ef;
+        }
+        if (a.n_cycles > 0) {
+            const uint32_t sweep = (uint32_t)a.sweep0;
+            const double beta = a.t_beta[tb + t];
+            const double delta = a.t_displ[2 * (tb + t)];
+            const Real delta_r = (Real)delta;
+            const float beta_l2e = (float)beta * 1.4426950408889634f;
+            // binary32 displacement (u - 1/2) delta as one fma of the 24-bit
+            // integer of u: u - 1/2 and the products by powers of two are
+            // exact, so the result is the rounding of the same exact value
+            const float dsc = (float)delta_r * 0x1.0p-24f, dof = -0.5f * (float)delta_r;
+            int nacc = 0;
+            u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
+            u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
+            if (KL > 1) {
+                if ((uint32_t)q < nsteps) move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                fetch_words<Real>(mA, mB, gb, nA, nB);
+            }
+            for (uint32_t m = 0; m < nsteps; ++m) {
+                // draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
+                u32x4 cur = nA, curB = nB;
+                if (KL == 1) {
+                    move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m, cur, curB);
+                } else {
+                    const uint32_t m1 = m + 1;
+                    if (m1 < nsteps) {
+                        const uint32_t k1 = m1 & (uint32_t)(KL - 1);
+                        if (k1 == 0 && m1 + q < nsteps)
+                            move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m1 + q, mA, mB);
+                        fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
+                    }
+                }
+                const MoveDraw<Real> d0 = decode_move<Real>(cur, curB, nsteps);
+                const uint32_t i = d0.index;
+                const int si = (int)(i >> kls), li = gb + (int)(i & (uint32_t)(KL - 1));
+                Real xi = X[xoff<LANES>(si, 0, li)];
+                Real yi = X[xoff<LANES>(si, 1, li)];
+                Real zi = X[xoff<LANES>(si, 2, li)];
+                Real tx, ty, tz;
+                if constexpr (F32) {
+                    tx = xi + __builtin_fmaf((float)(cur.y >> 8), dsc, dof);
+                    ty = yi + __builtin_fmaf((float)(cur.z >> 8), dsc, dof);
+                    tz = zi + __builtin_fmaf((float)(cur.w >> 8), dsc, dof);
+                } else {
+                    make_trial<Real, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
+                }
+                // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
+                if (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                    if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, si, li, radius2, delta_r, a.seed,
+                                                       run_key, (uint32_t)t, sweep, m, nsteps, xi, yi, zi, tx, ty,
+                                                       tz) &&
+                        q == 0)
+                        a.run_error[a.slot_run[slot]] = 1;
+                }
+                // park the moved atom so its self pair contributes zero
+                if (tid == li) {
+                    X[xoff<LANES>(si, 0, li)] = far_point<Real>();
+                    X[xoff<LANES>(si, 1, li)] = far_point<Real>();
+                    X[xoff<LANES>(si, 2, li)] = far_point<Real>();
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
+                Real sn = 0, so = 0;  // FP64: the two sums; binary32: sum of differences in sn
+                const Real *Xl = X + tid;
+                // binary32: trial distances from the old ones (dist2_pair_f32)
+                float m2dx = 0.0f, m2dy = 0.0f, m2dz = 0.0f, ddd = 0.0f;
+                if constexpr (F32) {
+                    const float dx = (float)tx - (float)xi, dy = (float)ty - (float)yi, dz = (float)tz - (float)zi;
+                    m2dx = -2.0f * dx;
+                    m2dy = -2.0f * dy;
+                    m2dz = -2.0f * dz;
+                    ddd = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                }
+#pragma unroll 2
+                for (int s = 0; s < A; ++s) {
+                    const Real xj = Xl[xoff<LANES>(s, 0, 0)];
+                    const Real yj = Xl[xoff<LANES>(s, 1, 0)];
+                    const Real zj = Xl[xoff<LANES>(s, 2, 0)];
+                    Real r2n, r2o;
+                    if constexpr (F32) {
+                        float fn, fo;
+                        dist2_pair_f32((float)xj, (float)yj, (float)zj, (float)xi, (float)yi, (float)zi, m2dx, m2dy, m2dz,
+                                       ddd, fn, fo);
+                        r2n = (Real)fn;
+                        r2o = (Real)fo;
+                    } else {
+                        r2n = box_d2<Real, BC_SPHERE, false>(tx, ty, tz, xj, yj, zj, BoxC<Real>{});
+                        r2o = box_d2<Real, BC_SPHERE, false>(xi, yi, zi, xj, yj, zj, BoxC<Real>{});
+                    }
+                    if constexpr (F32 && PK == PK_LJ) {
+                        // e(u) = u^3 (c12 u^3 + c6): e_new - e_old = (a - b)(c12 (a + b) + c6),
+                        // a = u_new^3, b = u_old^3; one reciprocal serves both pairs
+                        const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                        const float un = ip * r2o, uo = ip * r2n;
+                        const float an = un * un * un, bo = uo * uo * uo;
+                        sn = __builtin_fmaf(an - bo, __builtin_fmaf(c[1], an + bo, c[0]), sn);
+                    } else if constexpr (F32 && PK == PK_EVEN_H) {
+                        const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                        sn += pair_energy_u<Real, PK, KC>(ip * r2o, c) - pair_energy_u<Real, PK, KC>(ip * r2n, c);
+                    } else if constexpr (F32) {
+                        sn += pair_energy<Real, PK, KC>(r2n, c) - pair_energy<Real, PK, KC>(r2o, c);
+                    } else {
+                        sn += pair_energy<Real, PK, KC>(r2n, c);
+                        so += pair_energy<Real, PK, KC>(r2o, c);
+                    }
+                }
+                bool accept;
+                double dE_used;
+                if constexpr (F32) {
+                    // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
+                    const float dEf = group_sum_rt((float)sn, KL);
+                    dE_used = (double)dEf;
+                    const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
+                    accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                    if (accept) {
+                        E = E + (double)dEf;
+                        nacc += 1;
+                    }
+                } else {
+                    sn = group_sum_rt(sn, KL);
+                    so = group_sum_rt(so, KL);
+                    const double new_en = (sn - so) + E;
+                    const double dd = new_en - E;
+                    dE_used = dd;
+                    accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                    if (accept) {
+                        E = new_en;
+                        nacc += 1;
+                    }
+                }
+                // commit (src/swap_config.jl:40-51) or restore the parked atom
+                if (tid == li) {
+                    X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
+                    X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
+                    X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (a.trace_de != nullptr && q == 0 && a.trace_cycle < a.trace_cycles) {
+                    const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle) * nsteps + m;
+                    a.trace_de[ti] = dE_used;
+                    a.trace_acc[ti] = (int8_t)accept;
+                    a.trace_idx[ti] = (int32_t)i;
+                }
+            }
+            if (q == 0) {
+                a.t_count_atom[2 * (tb + t)] += nacc;
+                a.t_count_atom[2 * (tb + t) + 1] += nacc;
+            }
+        }
+        if (q == 0) a.slot_en[slot] = E;
+    }
+    __syncthreads();
+    if (a.n_cycles > 0)
+        for (int e = tid; e < 3 * A * LANES; e += LANES) gX[e] = X[e];
+}
+
+// initialise_energy (src/EnergyEvaluation.jl:1662-1692) for the spread
+// layout: FP64 total energy of each configuration of the tile.
+template <typename Real, int PK, int KC, int LANES>
+__global__ void __launch_bounds__(LANES) spread_init_kernel(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const TileInfo tile = a.tiles[blockIdx.x];
+    const int KL = tile.kl;
+    const int kls = __builtin_ctz((unsigned)KL);
+    const int tid = threadIdx.x;
+    const int lg = tid >> kls;
+    const int q = tid & (KL - 1);
+    const int N = tile.n_atoms;
+    const int A = tile.n_slab;
+    Real *X = reinterpret_cast<Real *>(lds_raw);
+    const Real *gX = reinterpret_cast<const Real *>(a.coords) + tile.coord_off;
+    for (int e = tid; e < 3 * A * LANES; e += LANES) X[e] = gX[e];
+    __syncthreads();
+    if (lg >= tile.n_slots) return;  // whole clusters leave together
+    const int slot = tile.slot0 + lg;
+    if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;
+    double c64[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
+    const double E = sphere_pair_sum64<Real, PK, KC, LANES>(X, lg << kls, q, N, KL, kls, c64);
+    if (q == 0) a.slot_en[slot] = E;
+}
+
+// Exchange, step-size adaptation and sampling of one cycle of every ladder
+// (src/MCRun.jl:203-212, :248-250), one 64-lane workgroup per run, after the
+// cycle's moves.  The ladder's energies and slot map are staged in LDS; the
+// even/odd pairs of a cycle are disjoint, so lanes take them in parallel.
+// Draws, tests and counters are those of the exchange in sweep_kernel
+// (src/Exchange.jl:103-155, :198-263; src/Sampling.jl:29-66, :258-280).
+__global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
+{
+    __shared__ double s_e[SPREAD_MAX_TEMPS];
+    __shared__ int32_t s_slot[SPREAD_MAX_TEMPS];
+    const RunInfo ri = a.runs[a.run_list != nullptr ? a.run_list[blockIdx.x] : (int)blockIdx.x];
+    const int R = ri.n_temps, tb = ri.temp_base, N = ri.n_atoms;
+    const int lane = threadIdx.x;
+    const uint32_t sweep = (uint32_t)a.sweep0;
+    const long long pidx = a.cycle0 + 1;
+    for (int t = lane; t < R; t += 64) {
+        const int s = a.temp_slot[tb + t];
+        s_slot[t] = s;
+        s_e[t] = a.slot_en[s];
+    }
+    __syncthreads();
+    auto try_pair = [&](int n) {
+        const u32x4 X1 = draw(a.seed, ri.run_key, RNG_EXCH, 0u, 0u, sweep, 1u + (uint32_t)n);
+        const double u = u53(X1.x, X1.y);
+        const double de = s_e[n] - s_e[n + 1];
+        const double db = a.t_beta[tb + n] - a.t_beta[tb + n + 1];
+        double acc = det_exp(db * de);
+        if (acc > 1.0) acc = 1.0;
+        a.t_count_exc[2 * (tb + n)] += 1;
+        a.t_count_exc[2 * (tb + n + 1)] += 1;
+        if (acc > u) {
+            a.t_count_exc[2 * (tb + n) + 1] += 1;
+            a.t_count_exc[2 * (tb + n + 1) + 1] += 1;
+            const int s0 = s_slot[n];
+            s_slot[n] = s_slot[n + 1];
+            s_slot[n + 1] = s0;
+            const double e0 = s_e[n];
+            s_e[n] = s_e[n + 1];
+            s_e[n + 1] = e0;
+        }
+    };
+    if (R > 1) {
+        if (a.exchange_mode == EXC_EVEN_ODD) {
+            for (int n = (int)(sweep & 1u) + 2 * lane; n + 1 < R; n += 128) try_pair(n);
+        } else if (lane == 0) {
+            const u32x4 X0 = draw(a.seed, ri.run_key, RNG_EXCH, 0u, 0u, sweep, 0u);
+            const double u = u53(X0.x, X0.y);
+            if (a.exchange_mode == EXC_EVERY || u < 0.1) try_pair((int)below(X0.z, (uint32_t)(R - 1)));
+        }
+    }
+    __syncthreads();
+    for (int t = lane; t < R; t += 64) {
+        const int T = tb + t;
+        const int s = s_slot[t];
+        a.temp_slot[T] = s;
+        a.slot_temp[s] = t;
+        // update_max_stepsize!, src/Exchange.jl:198-263
+        if (pidx % a.n_adjust == 0) {
+            const double ar = (double)a.t_count_atom[2 * T + 1] / (double)((long long)a.n_adjust * N);
+            if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
+            else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
+            a.t_count_atom[2 * T + 1] = 0;
+        }
+        // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
+        if (a.sampling && pidx % a.mc_sample == 0) {
+            const double E = s_e[t];
+            a.t_ham[2 * T] += E;
+            a.t_ham[2 * T + 1] += E * E;
+            const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
+            const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
+            a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+        }
+    }
+}
+
+}  // namespace ptmc

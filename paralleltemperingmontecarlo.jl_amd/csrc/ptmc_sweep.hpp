@@ -456,6 +456,20 @@ __device__ __forceinline__ double box_lrc(int kind, int n_atoms, double r_cut, d
     return e;
 }
 
+// binary32 squared distances of a partner x_j to the old position o of the
+// moved atom and to its trial o + d: r2o = |v|^2 with v = x_j - o, and
+// r2n = |v - d|^2 = r2o + |d|^2 - 2 v.d (m2d = -2 d, dd = |d|^2 per move):
+// ten operations for the pair of distances instead of twelve.  |d| is at
+// most the step size, so r2n is never small against r2o and the rounding
+// stays a few binary32 ulps of r2n.
+__device__ __forceinline__ void dist2_pair_f32(float xj, float yj, float zj, float ox, float oy, float oz,
+                                               float m2dx, float m2dy, float m2dz, float dd, float &r2n, float &r2o)
+{
+    const float vx = xj - ox, vy = yj - oy, vz = zj - oz;
+    r2o = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
+    r2n = __builtin_fmaf(m2dz, vz, __builtin_fmaf(m2dy, vy, __builtin_fmaf(m2dx, vx, r2o + dd)));
+}
+
 template <typename Real>
 struct MoveDraw {
     uint32_t index;
@@ -945,6 +959,15 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
                     Real sn = 0, so = 0;  // FP64: the two sums; binary32: sum of differences in sn
                     const Real *Xl = X + tid;
+                    // binary32 sphere: trial distances from the old ones (dist2_pair_f32)
+                    float m2dx = 0.0f, m2dy = 0.0f, m2dz = 0.0f, ddd = 0.0f;
+                    if constexpr (F32 && BC == BC_SPHERE) {
+                        const float dx = (float)tx - (float)xi, dy = (float)ty - (float)yi, dz = (float)tz - (float)zi;
+                        m2dx = -2.0f * dx;
+                        m2dy = -2.0f * dy;
+                        m2dz = -2.0f * dz;
+                        ddd = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    }
                     constexpr int UNROLL = LANES > 256 ? 2 : 4;
 #pragma unroll UNROLL
                     for (int s = 0; s < A; ++s) {
@@ -964,9 +987,26 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                                 eo = pair_energy_lut<Real>(r2o, xyo, zo, a.pot);
                             }
                         } else {
-                        r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
-                        r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
-                        if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
+                        if constexpr (F32 && BC == BC_SPHERE) {
+                            float fn, fo;
+                            dist2_pair_f32((float)xj, (float)yj, (float)zj, (float)xi, (float)yi, (float)zi, m2dx, m2dy,
+                                           m2dz, ddd, fn, fo);
+                            r2n = (Real)fn;
+                            r2o = (Real)fo;
+                        } else {
+                            r2n = box_d2<Real, BC, DIV>(tx, ty, tz, xj, yj, zj, bxr);
+                            r2o = box_d2<Real, BC, DIV>(xi, yi, zi, xj, yj, zj, bxr);
+                        }
+                        if constexpr (F32 && PK == PK_LJ && BC == BC_SPHERE) {
+                            // e(u) = u^3 (c12 u^3 + c6): e_new - e_old = (a - b)(c12 (a + b) + c6)
+                            // with a = u_new^3, b = u_old^3 (accumulated below); one
+                            // reciprocal serves both pairs
+                            const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                            const float un = ip * r2o, uo = ip * r2n;
+                            const float an = un * un * un, bo = uo * uo * uo;
+                            en = an - bo;
+                            eo = __builtin_fmaf(c[1], an + bo, c[0]);
+                        } else if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
                             // one reciprocal serves the new and the old pair
                             const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
                             en = pair_energy_u<Real, PK, KC>(ip * r2o, c);
@@ -977,7 +1017,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         }
                         }
                         if constexpr (BC == BC_SPHERE) {
-                            if constexpr (F32) sn += en - eo;
+                            if constexpr (F32 && PK == PK_LJ) sn = __builtin_fmaf(en, eo, sn);
+                            else if constexpr (F32) sn += en - eo;
                             else {
                                 sn += en;
                                 so += eo;

@@ -29,6 +29,8 @@ enum PairForm : int32_t {
 enum ExchangeMode : int32_t { EXC_REFERENCE = 0, EXC_EVERY = 1, EXC_EVEN_ODD = 2 };
 
 constexpr int MAX_COEFF = 16;
+// longest ladder of the spread layout (its exchange kernel stages a ladder in LDS)
+constexpr int SPREAD_MAX_TEMPS = 1024;
 
 struct PotParams {
     int32_t kind;
@@ -65,6 +67,8 @@ struct TileInfo {
     int32_t n_slab;     // atoms per lane = ceil(N / KL)
     int64_t coord_off;  // element offset of the tile's [3][n_slab][WGS] block
     int64_t comp_off;   // EAM: element offset of the tile's [n_slab][2][WGS] phi/rho block
+    int32_t kl;         // lanes per configuration (spread layout: read at run time)
+    int32_t pad_;
 };
 
 // Neural-network potential tables (device memory), prepared on the host:
@@ -136,6 +140,7 @@ struct SweepArgs {
     int8_t *trace_acc;
     int32_t *trace_idx;
     double *trace_aux;         // optional: delayed acceptance's surrogate change per traced move
+    const int32_t *run_list;   // spread layout: runs of the launch group (cycle kernel blocks)
     PotParams pot;
     uint64_t seed;
     int64_t sweep0;            // global cycle counter of the first cycle

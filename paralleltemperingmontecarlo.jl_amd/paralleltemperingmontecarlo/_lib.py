@@ -56,6 +56,8 @@ class ptmc_config(C.Structure):
         ("cycles_per_launch", C.c_int32),
         ("device", C.c_int32),
         ("seed", C.c_uint64),
+        ("workgroup_lanes", C.c_int32),
+        ("spread_streams", C.c_int32),
     ]
 
 

@@ -544,7 +544,11 @@ class EngineOptions:
     pair sums, FP64 totals).  exchange: "reference" (p = 0.1 per cycle, one
     random adjacent pair, src/MCRun.jl:203), "every" (one random pair each
     cycle) or "even_odd".  pair_form: "horner" or "reference"
-    (the reference's division loop, FP64 only)."""
+    (the reference's division loop, FP64 only).  workgroup_lanes: 0 keeps
+    every ladder in one workgroup; 64, 128 or 256 spreads ladders over
+    workgroups of that width (one cycle per launch; pair potentials in a
+    spherical container, NVT), with the same results; spread_streams > 1
+    cuts the runs into that many groups launched on their own streams."""
 
     precision: str = "fp64"
     exchange: str = "reference"
@@ -554,6 +558,8 @@ class EngineOptions:
     lanes_per_replica: int = 0
     cycles_per_launch: int = 256
     device: int = 0
+    workgroup_lanes: int = 0
+    spread_streams: int = 0
     max_displ: tuple = (0.1, 0.1)
 
 
@@ -633,6 +639,8 @@ def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _l
     cfg.recompute_every = opts.recompute_every
     cfg.lanes_per_replica = opts.lanes_per_replica
     cfg.cycles_per_launch = opts.cycles_per_launch
+    cfg.workgroup_lanes = opts.workgroup_lanes
+    cfg.spread_streams = opts.spread_streams
     cfg.device = opts.device
     cfg.seed = opts.seed & 0xFFFFFFFFFFFFFFFF
     return cfg

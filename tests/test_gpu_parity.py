@@ -288,15 +288,19 @@ def test_fp32_lj_replay(oracle, n_atoms, lanes, n_temps, wg):
     orc.start_phase(False)
     tr = orc.cycles(ncyc, replay_acc=acc)
     assert np.array_equal(idx, tr["idx"])
-    scale = 6.0 * n_atoms ** 0  # pair well depth 1, ~12 neighbours at most: |sum e_ij| <= ~6
+    # binary32 rounding scales with the pair terms: at least the atom's
+    # interaction scale (well depth 1, ~12 neighbours: ~6), more for moves
+    # that open a close contact
+    eps = 2e-5 * np.maximum(6.0, np.abs(tr["de"]))
     err = np.abs(de - tr["de"])
-    assert err.max() <= 2e-5 * scale, err.max()
+    assert np.all(err <= eps), np.max(err / eps)
     assert 0.05 < acc.mean() < 0.95
-    assert_decisions(tr, acc, beta, 2e-5 * scale, min_compared=acc.size)
+    assert_decisions(tr, acc, beta, eps, min_compared=acc.size)
     x_dev, en_dev, _ = eng.state(run)
     assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
-    # FP64 running totals of both sides agree to the accumulated binary32 error
-    assert np.max(np.abs(en_dev - orc.en_tot)) <= 1e-4
+    # FP64 running totals of both sides agree to the accumulated binary32 error:
+    # every accepted move adds one binary32-rounded dE (terms of size ~|E|/N)
+    assert np.max(np.abs(en_dev - orc.en_tot)) <= max(1e-4, 2e-6 * np.max(np.abs(orc.en_tot)))
     eng.close()
 
 
