@@ -272,6 +272,26 @@ int ptmc_multihistogram_npt(int32_t n_runs, int32_t n_temps, int32_t n_bin, cons
                             double k_boltzmann, double conv_threshold, int32_t max_iter, double *cp,
                             double *moments, double *free_energy, int32_t *iterations, int32_t device);
 
+/* ptmc_multihistogram_npt for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  Same arguments and
+ * results, but every array is a device pointer the caller owns, free_energy
+ * and iterations may be NULL, the kernel is queued on `stream` (a
+ * hipStream_t, NULL for the default stream) and the call returns without
+ * waiting.  The per-bin scratch is a device-resident block of
+ * PTMC_NOALLOC_NPT_SCRATCH_DOUBLES doubles (16 MiB) per GPU, held until the
+ * process ends; calls needing more than 3 n_runs n_bin^2 of it return
+ * PTMC_E_UNSUPPORTED.  The call allocates nothing; calls that overlap on one
+ * GPU share the block and race. */
+#define PTMC_NOALLOC_NPT_SCRATCH_DOUBLES (1 << 21)
+int ptmc_multihistogram_npt_noalloc(int32_t n_runs, int32_t n_temps, int32_t n_bin, const double *beta,
+                                    const double *hist, const double *en_min, const double *en_max,
+                                    const double *v_min, const double *v_max, const double *pressure,
+                                    int32_t n_result, const double *beta_result, const double *t_result,
+                                    double k_boltzmann, double conv_threshold, int32_t max_iter, double *cp,
+                                    double *moments, double *free_energy, int32_t *iterations, int32_t device,
+                                    void *stream);
+
 /* Two-species RuNNer potential energies (RuNNerPotential2Atom,
  * src/EnergyEvaluation.jl:1287-1317, initialise_energy :1748-1768, features
  * MachineLearningPotential/SymmFunc.jl:313-547), FP64 on the device for

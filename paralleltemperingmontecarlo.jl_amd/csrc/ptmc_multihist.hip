@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ptmc.h"
@@ -282,9 +283,23 @@ __global__ void __launch_bounds__(MHN_THREADS) multihist_npt_kernel(MHNArgs a)
             }
         }
     }
-    for (int i = tid; i < R; i += MHN_THREADS) a.free_energy[(size_t)run * R + i] = fe[i];
-    if (tid == 0) a.iterations[run] = it;
+    if (a.free_energy)
+        for (int i = tid; i < R; i += MHN_THREADS) a.free_energy[(size_t)run * R + i] = fe[i];
+    if (tid == 0 && a.iterations) a.iterations[run] = it;
 }
+
+// Device-resident scratch of the allocation-free entry point: the per-bin
+// terms of every run of one call (3 n_bin^2 doubles per run), one copy per
+// GPU for the life of the process.  Every element a call reads it has
+// written first (histogram sums, then offsets and ratios per iteration), so
+// nothing needs clearing between calls; calls that overlap on one GPU share
+// it and race.
+static __device__ __align__(256) double npt_scratch[PTMC_NOALLOC_NPT_SCRATCH_DOUBLES];
+
+// device address of npt_scratch on each GPU, looked up once per device
+constexpr int MAX_DEVICES = 64;
+std::mutex npt_scratch_mutex;
+double *npt_scratch_ptr[MAX_DEVICES] = {};
 
 template <typename T>
 struct DBuf {
@@ -435,5 +450,63 @@ extern "C" int ptmc_multihistogram_npt(int32_t n_runs, int32_t n_temps, int32_t 
     if (free_energy) MHCHK(hipMemcpy(free_energy, d_fe.p, nR * 8, hipMemcpyDeviceToHost));
     if (iterations) MHCHK(hipMemcpy(iterations, d_it.p, n_runs * sizeof(int), hipMemcpyDeviceToHost));
 #undef MHCHK
+    return PTMC_OK;
+}
+
+extern "C" int ptmc_multihistogram_npt_noalloc(int32_t n_runs, int32_t n_temps, int32_t n_bin, const double *beta,
+                                               const double *hist, const double *en_min, const double *en_max,
+                                               const double *v_min, const double *v_max, const double *pressure,
+                                               int32_t n_result, const double *beta_result, const double *t_result,
+                                               double k_boltzmann, double conv_threshold, int32_t max_iter,
+                                               double *cp, double *moments, double *free_energy,
+                                               int32_t *iterations, int32_t device, void *stream)
+{
+    if (n_runs <= 0 || n_temps <= 0 || n_bin <= 0 || n_result <= 0 || max_iter <= 0 || !beta || !hist ||
+        !en_min || !en_max || !v_min || !v_max || !pressure || !beta_result || !t_result || !cp)
+        return PTMC_E_ARG;
+    if ((int64_t)n_runs * 3 * n_bin * n_bin > (int64_t)PTMC_NOALLOC_NPT_SCRATCH_DOUBLES) return PTMC_E_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PTMC_E_NO_DEVICE;
+    if (device < 0 || device >= ndev || device >= MAX_DEVICES || hipSetDevice(device) != hipSuccess)
+        return PTMC_E_ARG;
+    const size_t lds = sizeof(double) * (4 * (size_t)n_temps + MHN_THREADS / 64) + sizeof(int);
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return PTMC_E_DEVICE;
+    if (lds > (size_t)max_lds) return PTMC_E_UNSUPPORTED;
+    double *scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(npt_scratch_mutex);
+        if (!npt_scratch_ptr[device]) {
+            void *p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(npt_scratch)) != hipSuccess || !p) return PTMC_E_DEVICE;
+            npt_scratch_ptr[device] = static_cast<double *>(p);
+        }
+        scr = npt_scratch_ptr[device];
+    }
+    MHNArgs a{};
+    a.n_temps = n_temps;
+    a.n_bin = n_bin;
+    a.n_res = n_result;
+    a.max_iter = max_iter;
+    a.conv = conv_threshold;
+    a.beta = beta;
+    a.hist = hist;
+    a.en_min = en_min;
+    a.en_max = en_max;
+    a.v_min = v_min;
+    a.v_max = v_max;
+    a.pressure = pressure;
+    a.beta_res = beta_result;
+    a.t_res = t_result;
+    a.k_boltzmann = k_boltzmann;
+    a.cp = cp;
+    a.moments = moments;
+    a.free_energy = free_energy;
+    a.iterations = iterations;
+    a.scratch = scr;
+    hipLaunchKernelGGL(multihist_npt_kernel, dim3(n_runs), dim3(MHN_THREADS), lds, static_cast<hipStream_t>(stream),
+                       a);
+    if (hipGetLastError() != hipSuccess) return PTMC_E_DEVICE;
     return PTMC_OK;
 }

@@ -144,6 +144,10 @@ FREE_SIGNATURES = {
                          _i32p, _i32p, C.c_int32, _dp, _dp, _dp, _dp, C.c_int32],
     "ptmc_multihistogram_npt": [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int32,
                                 _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp, _i32p, C.c_int32],
+    # device pointers and a stream handle
+    "ptmc_multihistogram_npt_noalloc": [C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 7 + [C.c_int32] +
+                                       [C.c_void_p] * 2 + [C.c_double, C.c_double, C.c_int32] + [C.c_void_p] * 4 +
+                                       [C.c_int32, C.c_void_p],
 }
 
 _LIB = None
