@@ -568,27 +568,37 @@ def main():
     plan = eng.plan_info()
 
     # Untimed: step-size adaptation (every 100 cycles towards the 0.4-0.6
-    # acceptance window of the reference), then the warm-up, padded so that the
-    # timed region ends on a cycle with the periodic FP64 recompute: the drift
-    # it finds is the one reported.
+    # acceptance window of the reference), the histogram bounds from the
+    # energies reached, then the warm-up in the production phase right before
+    # the timed region (the device stays busy up to it), padded so that a
+    # periodic FP64 recompute (every 1000 cycles) falls just before the timed
+    # region: the timed cycles then hold the workload's share of recomputes
+    # (steps / 1000), and after them the run continues untimed to the next
+    # recompute, whose drift is the one reported.
     recompute = 1000
-    pad = (-(args.adapt + args.warmup + args.steps)) % recompute
+    pad = (-(args.adapt + args.warmup)) % recompute
     eng.start_phase(False)
-    eng.cycles(args.adapt + args.warmup + pad)
+    eng.cycles(args.adapt)
     eng.sync()
     for i in range(len(runs)):
         e = eng.energies(i)
         eng.set_histogram(i, float(e.min()), float(e.max()))
-    eng.reset_counters()
     eng.start_phase(True)
+    eng.cycles(args.warmup + pad)
+    eng.reset_counters()
+    eng.sync()  # folds the warm-up's launch timings in before they are cleared
     eng.kernel_time(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eng.cycles(args.steps)
-    eng.sync()
-    torch.cuda.synchronize()
+    torch.cuda.synchronize()  # every stream of the device, the engine's included
     dt = time.perf_counter() - t0
+    eng.sync()  # raises if a run reported an error
     kms, nlaunch = eng.kernel_time()
+    groups = [eng.group_info(g) for g in range(plan["launch_groups"])]  # the timed launches only
+    tail = (-args.steps) % recompute
+    eng.cycles(tail)  # untimed, up to the next recompute
+    eng.sync()
 
     moves = moves_per_cycle(runs) * args.steps
     value = moves / dt
@@ -596,7 +606,6 @@ def main():
     aggregate = flops / (kms * 1e-3) / 1e12
     # dominant kernel: the launch group with the largest summed launch time;
     # its algorithmic flops over the timed region / its own launch time
-    groups = [eng.group_info(g) for g in range(plan["launch_groups"])]
     spread = args.workgroup_lanes > 0
     if spread:
         # spread layout: every launch group runs the same kernel, one launch
@@ -617,7 +626,7 @@ def main():
     acc, acc_cold = [], []
     for i in range(len(runs)):
         c = eng.counters(i)
-        a = c["count_atom"][:, 0] / (runs[i]["n_atoms"] * args.steps)
+        a = c["count_atom"][:, 0] / (runs[i]["n_atoms"] * (args.steps + tail))  # counted since the timed region
         acc.append(float(np.mean(a)))
         acc_cold.append(float(np.mean(a[: runs[i]["n_temps"] // 2])))
     lanes_used = {}
@@ -652,7 +661,8 @@ def main():
                       else "one workgroup per ladder group, %d cycles per launch" % args.cycles_per_launch,
             "max_fp32_vs_fp64_energy_drift": drift,
             "drift_note": "largest |E_running - E_FP64| (epsilon units) over all configurations and all FP64 "
-                          "recomputes (every 1000 cycles; the last one on the last timed cycle)",
+                          "recomputes (every 1000 cycles; the last one after the timed cycles, 1000 cycles "
+                          "after the one before them)",
             "lanes_per_configuration": dict(sorted(lanes_used.items(), key=lambda kv: int(kv[0]))),
             "tuned_rate": tuned["rate"] if tuned else None,
             "lanes_requested": {str(k): v for k, v in sorted(table.items())} or "engine default",

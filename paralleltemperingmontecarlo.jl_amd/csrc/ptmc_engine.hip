@@ -96,6 +96,7 @@ struct ptmc_engine {
     std::vector<RunHost> runs;
     bool prepared = false;
     bool spread = false;  // ladders spread over workgroups (cfg.workgroup_lanes > 0)
+    int64_t recompute_applied = -1;  // spread layout: sweep whose pending recompute a launch applied
     std::string err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -105,8 +106,8 @@ struct ptmc_engine {
     std::vector<hipEvent_t> join;
     double kernel_ms = 0.0;
     long long n_launch = 0;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
+    std::vector<hipEvent_t> ev_pool;  // every timing event created (destroyed with the handle)
+    std::vector<hipEvent_t> ev_free;  // those not attached to a pending launch
     std::vector<LaunchTiming> pending;
 
     // plan
@@ -931,12 +932,15 @@ void scatter_coords_nnp(ptmc_engine *h, std::vector<Real> &buf)
 
 int timing_event(ptmc_engine *h, hipEvent_t *ev)
 {
-    if (h->ev_used == h->ev_pool.size()) {
+    if (h->ev_free.empty()) {
         hipEvent_t e;
         HIPCHK(h, hipEventCreate(&e));
         h->ev_pool.push_back(e);
+        *ev = e;
+        return PTMC_OK;
     }
-    *ev = h->ev_pool[h->ev_used++];
+    *ev = h->ev_free.back();
+    h->ev_free.pop_back();
     return PTMC_OK;
 }
 
@@ -965,16 +969,22 @@ int launch_one(ptmc_engine *h, size_t k, const SweepArgs &a, hipStream_t st, boo
 
 // after the engine stream has been synchronised: fold launch durations into
 // the per-group sums and recycle the events
-int collect_timings(ptmc_engine *h)
+// fold the durations of the oldest `n` pending launches (all by default) into
+// the per-group sums, waiting for each to finish, and recycle their events
+int collect_timings(ptmc_engine *h, size_t n = (size_t)-1)
 {
-    for (const LaunchTiming &lt : h->pending) {
+    n = std::min(n, h->pending.size());
+    for (size_t k = 0; k < n; ++k) {
+        const LaunchTiming &lt = h->pending[k];
         float ms = 0.f;
+        HIPCHK(h, hipEventSynchronize(lt.b));
         HIPCHK(h, hipEventElapsedTime(&ms, lt.a, lt.b));
         h->launches[lt.group].ms += ms;
         h->launches[lt.group].n_done += 1;
+        h->ev_free.push_back(lt.a);
+        h->ev_free.push_back(lt.b);
     }
-    h->pending.clear();
-    h->ev_used = 0;
+    h->pending.erase(h->pending.begin(), h->pending.begin() + (long)n);
     return PTMC_OK;
 }
 
@@ -1626,10 +1636,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                                              h->sur.cap, h->stream));
                 h->sur.n_samples += h->n_slots;
             }
-            if (h->pending.size() >= 512) {
-                HIPCHK(h, hipStreamSynchronize(h->stream));
-                if ((rc = collect_timings(h))) return rc;
-            }
+            if (h->pending.size() >= 512 && (rc = collect_timings(h, 256))) return rc;
             h->sweep_counter += 1;
             h->phase_counter += 1;
         }
@@ -1661,6 +1668,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                 a.sampling = h->production;
                 a.init_run = -1;
                 a.run_list = h->d_run_list.p + L.run0;
+                a.recompute_done = h->recompute_applied == h->sweep_counter ? 1 : 0;
                 if (tracing) {
                     a.trace_de = h->d_trace_de.p;
                     a.trace_acc = h->d_trace_acc.p;
@@ -1672,10 +1680,8 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                 void *args[] = {&a};
                 HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(L.nruns), dim3(64), args, 0, stream_of(g)));
             }
-            if (h->pending.size() >= 512) {
-                for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamSynchronize(stream_of(g)));
-                if ((rc = collect_timings(h))) return rc;
-            }
+            // bound the event pool on long calls: the oldest launches are long done
+            if (h->pending.size() >= 512 && (rc = collect_timings(h, 256))) return rc;
             h->sweep_counter += 1;
             h->phase_counter += 1;
         }
@@ -1691,6 +1697,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                 void *args[] = {&a};
                 HIPCHK(h, hipLaunchKernel((const void *)L.fn, dim3(L.ntiles), dim3(L.wgs), args, L.lds, stream_of(g)));
             }
+            h->recompute_applied = h->sweep_counter;
         }
         if (ng > 1)
             for (size_t g = 0; g < ng; ++g) {
@@ -1718,11 +1725,8 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
             h->trace_pending = 0;
         }
         if ((rc = launch_all(h, a))) return rc;
-        if (h->pending.size() >= 512) {
-            // bound the event pool on long calls
-            HIPCHK(h, hipStreamSynchronize(h->stream));
-            if ((rc = collect_timings(h))) return rc;
-        }
+        // bound the event pool on long calls
+        if (h->pending.size() >= 512 && (rc = collect_timings(h, 256))) return rc;
         h->sweep_counter += chunk;
         h->phase_counter += chunk;
         done += chunk;
@@ -1732,7 +1736,9 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->kernel_ms += ms;
-    return collect_timings(h);
+    // the per-launch timings are folded in when they are read
+    // (ptmc_kernel_time, ptmc_group_info) or when many are pending
+    return PTMC_OK;
 }
 
 int ptmc_sync(ptmc_handle h)
@@ -1963,6 +1969,7 @@ int ptmc_seek(ptmc_handle h, int64_t cycle, int64_t phase_cycle)
     if (!h) return PTMC_E_ARG;
     if (cycle < 0 || phase_cycle < 0) return fail(h, PTMC_E_ARG, "negative cycle index");
     h->sweep_counter = cycle;
+    h->recompute_applied = -1;
     h->phase_counter = phase_cycle;
     return PTMC_OK;
 }
@@ -2148,6 +2155,12 @@ int ptmc_run_lanes(ptmc_handle h, int32_t run)
 int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset)
 {
     if (!h) return PTMC_E_ARG;
+    if (!h->pending.empty()) {
+        HIPCHK(h, hipSetDevice(h->cfg.device));
+        HIPCHK(h, hipDeviceSynchronize());
+        int rc = collect_timings(h);
+        if (rc) return rc;
+    }
     if (total_ms) *total_ms = h->kernel_ms;
     if (n_launches) *n_launches = h->n_launch;
     if (reset) {
@@ -2167,6 +2180,12 @@ int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, in
     if (!h) return PTMC_E_ARG;
     if (!h->prepared) return fail(h, PTMC_E_STATE, "engine not prepared");
     if (group < 0 || group >= (int)h->launches.size()) return fail(h, PTMC_E_ARG, "launch group %d out of range", group);
+    if (!h->pending.empty()) {
+        HIPCHK(h, hipSetDevice(h->cfg.device));
+        HIPCHK(h, hipDeviceSynchronize());
+        int rc = collect_timings(h);
+        if (rc) return rc;
+    }
     const Launch &L = h->launches[group];
     if (lanes_per_replica) *lanes_per_replica = L.kl;
     if (workgroup_lanes) *workgroup_lanes = L.wgs;

@@ -111,8 +111,8 @@ __device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls
 // The FP64 recompute that sweep_kernel makes at the end of cycle c (after
 // its exchange and sampling) is made here at the start of the launch of cycle
 // c + 1, when sweep0 % recompute_every == 0; the engine adds a launch with
-// n_cycles = 0 after a call's last cycle so that it is never left pending.
-// Repeating it is harmless: the second recompute finds E equal to the sum.
+// n_cycles = 0 after a call's last cycle so that it is never left pending,
+// and marks the next call's first launch (recompute_done) not to repeat it.
 template <typename Real, int PK, int KC, int LANES>
 __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
 {
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
         const Real radius2 = (Real)ri.radius2;
         const int t = a.slot_temp[slot];
         double E = a.slot_en[slot];
-        if (a.recompute_every > 0 && a.sweep0 > 0 && a.sweep0 % a.recompute_every == 0) {
+        if (!a.recompute_done && a.recompute_every > 0 && a.sweep0 > 0 && a.sweep0 % a.recompute_every == 0) {
             const double ef = sphere_pair_sum64<Real, PK, KC, LANES>(X, gb, q, N, KL, kls, c64);
             const double dr = __builtin_fabs(E - ef);
             if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;

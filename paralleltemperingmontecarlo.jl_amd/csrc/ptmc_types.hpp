@@ -141,6 +141,7 @@ struct SweepArgs {
     int32_t *trace_idx;
     double *trace_aux;         // optional: delayed acceptance's surrogate change per traced move
     const int32_t *run_list;   // spread layout: runs of the launch group (cycle kernel blocks)
+    int32_t recompute_done;    // spread layout: the recompute due at sweep0 was applied already
     PotParams pot;
     uint64_t seed;
     int64_t sweep0;            // global cycle counter of the first cycle

@@ -60,3 +60,36 @@ def test_tuned_table_runs():
     # FP64 start energies do not depend on the lanes beyond the summation tree
     for a, b in zip(*energies):
         np.testing.assert_allclose(a, b, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_layout_tuner_returns_a_runnable_plan():
+    """tune() over layouts and lanes: the winner is one of the candidate
+    layouts, its table holds lanes the engine accepts, and an engine built
+    with it plans every run with the reported lanes."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    import paralleltemperingmontecarlo as pt
+    from paralleltemperingmontecarlo.autotune import LAYOUTS, tune
+
+    runs = [dict(r, bc=pt.SphericalBC(r["radius"])) for r in _runs()]
+    mc = pt.MCParams(100, 1, 1, n_adjust=20)
+
+    def make(wg=0, streams=0):
+        return pt.PTEngine(pt.ELJPotentialEven([-4.0, 0.0, 0.0, 4.0]), pt.NVT(13), pt.SphericalBC(1.0), mc,
+                           pt.EngineOptions(precision="fp32", exchange="even_odd", seed=5, workgroup_lanes=wg,
+                                            spread_streams=streams))
+
+    best = tune(make, runs, warmup=5, cycles=40, rounds=1, adapt_cycles=40,
+                layouts=((0, 0), (64, 2), (128, 1)))
+    assert (best["workgroup_lanes"], best["spread_streams"]) in ((0, 0), (64, 2), (128, 1))
+    assert best["rate"] > 0
+    eng = make(best["workgroup_lanes"], best["spread_streams"])
+    for r in runs:
+        eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], r["bc"], run_key=r["key"],
+                    lanes=best["table"].get(r["n_atoms"], 0))
+    eng.prepare()
+    for i, r in enumerate(runs):
+        assert eng.lanes_of(i) == best["lanes_used"][r["n_atoms"]]
+    eng.close()
+    assert set(LAYOUTS) >= {(0, 0), (64, 2), (128, 3)}
