@@ -308,6 +308,56 @@ int ptmc_nnp2_energy(int32_t n_conf, int32_t n1, int32_t n2, const double *x, in
                      const int32_t *activations, int32_t n_params, const double *params1, const double *params2,
                      double *energy, double *eatom, int32_t device);
 
+/* The potential plugin functions, batched over n_conf configurations of
+ * n_atoms atoms that the caller holds (the engine's runs are not touched),
+ * with the handle's potential, boundary (ptmc_config.boundary) and ensemble,
+ * in FP64 on the device.  Counterparts: set_variables
+ * (src/EnergyEvaluation.jl:1778-1822), initialise_energy (:1662-1747) and
+ * energy_update! (:1474-1621) for ELJPotentialEven / ELJPotential,
+ * ELJPotentialB, LookupTablePotential, EmbeddedAtomPotential and
+ * RuNNerPotential (spherical container).  The arithmetic is the
+ * reference's: pairs i < j in order with sequential sums, the reference pair
+ * form (r^-6 then repeated division) and minimum image with divisions when
+ * ptmc_config.pair_form = 1 (Horner in 1/r^2 and reciprocal products
+ * otherwise), EAM terms (1 / sqrt(r^2))^n by pow, the network in FP64.
+ *
+ *   x      n_conf x n_atoms x 3 positions (bohr); may be NULL for
+ *          initialise_energy of a distance-only potential (ELJ, EAM) when d2
+ *          is given
+ *   d2     NULL, or n_conf x n_atoms x n_atoms squared distances (the
+ *          reference's dist2_mat, get_distance2_mat src/Configurations.jl:
+ *          135-162); only for ELJ and EAM.  NULL: built from x as
+ *          get_distance2_mat does (pair i < j as distance2(x_i, x_j), mirrored)
+ *   box    n_conf x 2 (box length, box height) in a periodic box (the height
+ *          of a cubic box is not read); NULL in a spherical container.  NPT
+ *          (ensemble 1, 2) applies the box's cut-off r_cut (get_r_cut,
+ *          src/Ensembles.jl:118-155) and tail correction (lrc, :968-1008;
+ *          none for ELJPotential with odd powers, PTMC_E_UNSUPPORTED)
+ *   vars   the potential variables, ptmc_potential_vars values per
+ *          configuration: EAM component_vector (n_atoms x 2: phi, rho),
+ *          network g_matrix (atom-major n_atoms x (n_rad + n_ang), i.e. the
+ *          reference's column-major features x atoms); none for pair
+ *          potentials.
+ *
+ * ptmc_initialise_energy writes en_tot (n_conf) and, if not NULL, en_atom
+ * (n_conf x n_atoms: the dimer rows of dimer_energy_config, the network's
+ * per-atom energies, the EAM terms ean phi_i - ecam sqrt(rho_i)) and vars.
+ * ptmc_energy_update moves atom index[c] of configuration c to trial[c]
+ * (n_conf x 3) and writes new_en = the reference's new_en (dimer: the row
+ * difference + en_tot[c]; EAM: calc_energies_from_components of the new
+ * components; network: the new features' energy, or 100.0 when a partner is
+ * inside the hard-core distance, src/EnergyEvaluation.jl:1601-1606), and,
+ * for EAM and the network, new_vars and new_en_atom (a configuration that
+ * failed the hard-core test gets new_vars = vars and keeps its new_en_atom
+ * row as passed in).  Pair potentials keep no variables: en_tot is the
+ * current energy, vars / new_vars / new_en_atom are not used. */
+int ptmc_potential_vars(ptmc_handle h, int32_t n_atoms, int64_t *n_vars);
+int ptmc_initialise_energy(ptmc_handle h, int32_t n_conf, int32_t n_atoms, const double *x, const double *d2,
+                           const double *box, double *en_tot, double *en_atom, double *vars);
+int ptmc_energy_update(ptmc_handle h, int32_t n_conf, int32_t n_atoms, const double *x, const double *d2,
+                       const double *box, const int32_t *index, const double *trial, const double *en_tot,
+                       const double *vars, double *new_en, double *new_en_atom, double *new_vars);
+
 /* Learned surrogate and delayed-acceptance Metropolis for the network
  * potential.  No counterpart in the reference: these entry points serve the
  * engine's own acceleration (Christen & Fox 2005).  The surrogate is a

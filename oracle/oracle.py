@@ -135,6 +135,16 @@ def lib():
         _L.orc_nnp2_energy.restype = C.c_double
         _L.orc_nnp2_move_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         _L.orc_nnp2_swap_g.argtypes = [C.POINTER(orc_nnp2), _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        _L.orc_distance2_mat.argtypes = [C.POINTER(orc_params), _dp, C.c_double, C.c_double, _dp]
+        _L.orc_energy_config_d2.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), _dp, C.c_double, C.c_double,
+                                            C.c_double, _dp]
+        _L.orc_energy_config_d2.restype = C.c_double
+        _L.orc_eam_components.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), _dp, _dp]
+        _L.orc_eam_energy.argtypes = [C.POINTER(orc_pot), _dp, C.c_int]
+        _L.orc_eam_energy.restype = C.c_double
+        _L.orc_energy_update.argtypes = [C.POINTER(orc_params), C.POINTER(orc_pot), _dp, _dp, C.c_double, C.c_double,
+                                         C.c_int, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        _L.orc_energy_update.restype = C.c_double
     return _L
 
 
@@ -289,6 +299,91 @@ REFERENCE = dict(pair_form=0, dist_cache=1, lanes=1, math_lib=1)
 
 def engine_mode(lanes=1):
     return dict(pair_form=1, dist_cache=0, lanes=lanes, math_lib=0)
+
+
+# ---- the potential plugin functions (src/EnergyEvaluation.jl:1474-1856) for
+# one configuration at a time, in the reference's arithmetic: reference pair
+# form, sequential sums, libm pow for the EAM terms.
+
+def _plugin_params(n_atoms, bc, ensemble, pair_form):
+    p = orc_params()
+    p.n_atoms, p.n_temps = n_atoms, 1
+    p.bc = BC_CODES[bc]
+    p.ensemble = 1 if ensemble == "npt" else 0
+    p.pair_form = pair_form
+    p.dist_cache, p.lanes, p.math_lib = 1, 1, 1
+    return p
+
+
+def distance2_mat(x, bc="sphere", box=(0.0, 0.0), pair_form=0):
+    """get_distance2_mat (src/Configurations.jl:135-162)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    d2 = np.zeros((len(x), len(x)))
+    lib().orc_distance2_mat(C.byref(_plugin_params(len(x), bc, "nvt", pair_form)), _p(x), float(box[0]),
+                            float(box[1]), _p(d2))
+    return d2
+
+
+def plugin_initialise(pot: orc_pot, n_atoms, *, x=None, d2=None, bc="sphere", box=(0.0, 0.0), ensemble="nvt",
+                      pair_form=0):
+    """set_variables + initialise_energy of one configuration: (en_tot,
+    en_atom, vars) with vars the EAM components (N x 2) or the network
+    features (N x n_features), None for pair potentials.  d2 defaults to the
+    matrix of x."""
+    p = _plugin_params(n_atoms, bc, ensemble, pair_form)
+    kind = pot.kind
+    if kind == 3:  # network: features from positions (spherical container)
+        nn = pot._nnp_keep
+        g = nn.full_g(x)
+        tot, e = nn.energy(g)
+        return tot, e, g
+    if d2 is None:
+        d2 = distance2_mat(x, bc, box, pair_form)
+    d2 = np.ascontiguousarray(d2, dtype=np.float64)
+    if kind == 2:
+        comp = np.zeros((n_atoms, 2))
+        lib().orc_eam_components(C.byref(p), C.byref(pot), _p(d2), _p(comp))
+        ea = pot.eam_ean * comp[:, 0] - pot.eam_ecam * np.sqrt(comp[:, 1])
+        return lib().orc_eam_energy(C.byref(pot), _p(comp), n_atoms), ea, comp
+    en_atom = np.zeros(n_atoms)
+    r_cut = lib().orc_r_cut(p.bc, float(box[0]), float(box[1])) if ensemble == "npt" else 0.0
+    if kind in (4, 5):  # angle-dependent pairs: geometry from positions
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        tot = lib().orc_energy_config_box(C.byref(p), C.byref(pot), _p(x), float(box[0]), float(box[1]), r_cut,
+                                          _p(en_atom))
+        return tot, en_atom, None
+    tot = lib().orc_energy_config_d2(C.byref(p), C.byref(pot), _p(d2), r_cut, float(box[0]), float(box[1]),
+                                     _p(en_atom))
+    return tot, en_atom, None
+
+
+def plugin_update(pot: orc_pot, x, i, trial, en_tot, *, vars=None, d2=None, bc="sphere", box=(0.0, 0.0),
+                  ensemble="nvt", pair_form=0):
+    """energy_update! of one configuration whose atom i moves to trial:
+    (new_en, new_vars, new_en_atom); d2 defaults to the matrix of x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    p = _plugin_params(n, bc, ensemble, pair_form)
+    if d2 is None and pot.kind != 3:
+        d2 = distance2_mat(x, bc, box, pair_form)
+    d2 = None if d2 is None else np.ascontiguousarray(d2, dtype=np.float64)
+    t = np.ascontiguousarray(trial, dtype=np.float64)
+    comp = g = new_comp = g_new = eat = None
+    if pot.kind == 2:
+        comp = np.ascontiguousarray(vars, dtype=np.float64)
+        new_comp = np.zeros_like(comp)
+    elif pot.kind == 3:
+        g = np.ascontiguousarray(vars, dtype=np.float64)
+        g_new = np.zeros_like(g)
+        eat = np.full(n, np.nan)
+    e = lib().orc_energy_update(C.byref(p), C.byref(pot), _p(x), _p(d2), float(box[0]), float(box[1]), int(i), _p(t),
+                                float(en_tot), _p(comp), _p(g), _p(new_comp), _p(g_new), _p(eat))
+    if pot.kind == 2:
+        ea = pot.eam_ean * new_comp[:, 0] - pot.eam_ecam * np.sqrt(new_comp[:, 1])
+        return e, new_comp, ea
+    if pot.kind == 3:
+        return e, g_new, eat
+    return e, None, None
 
 
 class OracleRun:

@@ -1077,3 +1077,101 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
     free(vol_d2);
     return 0;
 }
+
+/* ------------------------------------------- the potential plugin functions */
+
+/* get_distance2_mat (src/Configurations.jl:135-162): each pair i < j once,
+ * distance2(config[i], config[j]), mirrored; zero diagonal. */
+void orc_distance2_mat(const orc_params *p, const double *x, double l, double h, double *d2)
+{
+    const int N = p->n_atoms;
+    for (int i = 0; i < N; ++i) {
+        d2[(size_t)i * N + i] = 0.0;
+        for (int j = i + 1; j < N; ++j)
+            d2[(size_t)i * N + j] = d2[(size_t)j * N + i] =
+                orc_distance2_box(&x[3 * i], &x[3 * j], p->bc, l, h, dist_form(p));
+    }
+}
+
+/* dimer_energy_config from a squared-distance matrix (src/EnergyEvaluation.jl:
+ * 378-474), ELJ potentials: pairs i < j in order, the pair energy added to
+ * both atoms' rows and to the total; NPT skips pairs beyond r_cut and adds
+ * the box's tail correction. */
+double orc_energy_config_d2(const orc_params *p, const orc_pot *pot, const double *d2, double r_cut, double l,
+                            double h, double *en_atom)
+{
+    const int N = p->n_atoms, cut = p->ensemble == ORC_NPT;
+    double tot = 0.0;
+    if (en_atom) for (int i = 0; i < N; ++i) en_atom[i] = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) {
+            const double r2 = d2[(size_t)i * N + j];
+            if (cut && !(r2 <= r_cut)) continue;
+            const double e = orc_dimer_energy(pot, r2, p->pair_form);
+            if (en_atom) { en_atom[i] += e; en_atom[j] += e; }
+            tot += e;
+        }
+    if (cut) tot += orc_lrc_box(N, r_cut, pot, p->bc, l, h);
+    return tot;
+}
+
+/* set_variables(EmbeddedAtomPotential) (src/EnergyEvaluation.jl:1800-1811):
+ * calc_components (:1105-1110) over each row of the squared-distance matrix,
+ * diagonal included (invrexp(0) = (0, 0)). */
+void orc_eam_components(const orc_params *p, const orc_pot *pot, const double *d2, double *comp)
+{
+    const int N = p->n_atoms;
+    for (int i = 0; i < N; ++i) {
+        double a = 0.0, b = 0.0;
+        for (int j = 0; j < N; ++j) {
+            double f, g;
+            eam_terms(p, pot, d2[(size_t)i * N + j], &f, &g);
+            a += f;
+            b += g;
+        }
+        comp[2 * i] = a;
+        comp[2 * i + 1] = b;
+    }
+}
+
+/* calc_energies_from_components (src/EnergyEvaluation.jl:1156-1162) */
+double orc_eam_energy(const orc_pot *pot, const double *comp, int N)
+{
+    return eam_energy_from_comp(pot, comp, N, 1);
+}
+
+/* energy_update! (src/EnergyEvaluation.jl:1474-1643) of one configuration x
+ * (box l, h) whose atom i moves to trial: the move energy of the sweep above
+ * for a one-replica state.  d2: the configuration's squared-distance matrix
+ * (NULL: distances from x); comp: EAM components (N x 2); g: network
+ * features (atom-major).  Writes new_comp (EAM), g_new and eat_new (network;
+ * g_new is a copy of g when the hard-core test gives 100).  Returns the new
+ * total energy. */
+double orc_energy_update(const orc_params *p, const orc_pot *pot, const double *x, const double *d2, double l,
+                         double h, int i, const double *trial, double en_tot, const double *comp, const double *g,
+                         double *new_comp, double *g_new, double *eat_new)
+{
+    const int N = p->n_atoms;
+    if (N < 2 || N > MAXN || i < 0 || i >= N) return NAN;
+    orc_state s;
+    memset(&s, 0, sizeof s);
+    double box = l, boxh = h, rc = p->ensemble == ORC_NPT ? orc_r_cut(p->bc, l, h) : 0.0;
+    s.x = (double *)x;
+    s.box = &box;
+    s.box_h = &boxh;
+    s.r_cut = &rc;
+    s.d2 = (double *)d2;
+    s.comp = (double *)comp;
+    s.g = (double *)g;
+    trial_out *o = (trial_out *)malloc(sizeof *o);
+    if (!o) return NAN;
+    o->g_new = g_new;
+    o->eat_new = eat_new;
+    if (pot->kind == ORC_POT_NNP && g && g_new)
+        memcpy(g_new, g, sizeof(double) * (size_t)N * (pot->nnp->n_rad + pot->nnp->n_ang));
+    atom_move_energy(p, pot, &s, 0, i, trial, en_tot, o);
+    if (pot->kind == ORC_POT_EAM && new_comp) memcpy(new_comp, o->new_comp, sizeof(double) * 2 * N);
+    const double e = o->new_en;
+    free(o);
+    return e;
+}

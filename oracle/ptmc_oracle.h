@@ -176,6 +176,17 @@ double orc_exp_det(double x);
 double orc_log_det(double x);
 void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* The potential plugin functions of src/EnergyEvaluation.jl for one
+ * configuration (r^2 matrices N x N row-major, EAM components N x 2). */
+void   orc_distance2_mat(const orc_params *p, const double *x, double l, double h, double *d2);
+double orc_energy_config_d2(const orc_params *p, const orc_pot *pot, const double *d2, double r_cut, double l,
+                            double h, double *en_atom);
+void   orc_eam_components(const orc_params *p, const orc_pot *pot, const double *d2, double *comp);
+double orc_eam_energy(const orc_pot *pot, const double *comp, int n_atoms);
+double orc_energy_update(const orc_params *p, const orc_pot *pot, const double *x, const double *d2, double l,
+                         double h, int i, const double *trial, double en_tot, const double *comp, const double *g,
+                         double *new_comp, double *g_new, double *eat_new);
+
 /* NNP pieces (ptmc_nnp.c). */
 /* the RuNNer forward pass with the argument list of the reference's ccall
  * (MachineLearningPotential/ForwardPass.jl:60-84); returns eatom[batch-1] */

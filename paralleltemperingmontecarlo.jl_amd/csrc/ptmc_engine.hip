@@ -16,6 +16,7 @@
 #include "../../include/ptmc.h"
 #include "ptmc_kernels.hpp"
 #include "ptmc_nnp_layout.hpp"
+#include "ptmc_plugin.hpp"
 #include "ptmc_sweep.hpp"
 
 using namespace ptmc;
@@ -154,9 +155,15 @@ struct ptmc_engine {
         int n_rad = 0, n_ang = 0, nf = 0, h1 = 0, h2 = 0, n_rad_active = 0, n_groups = 0, q10 = 0;
         double r_cut = 0.0, hardcore2 = 0.0;
         float b3 = 0.0f;
+        // FP64 tables as given, for the plugin functions
+        std::vector<double> rad64, ang64, w64;
     } nnp;
     DevBuf<float> d_nnp_rad, d_nnp_geta, d_nnp_afn, d_nnp_rowoff, d_nnp_w, d_feat, d_eatom;
     DevBuf<int32_t> d_nnp_grp, d_nnp_abuild, d_nnp_gdesc, d_nnp_wgroups, d_temp_slot;
+    // plugin functions: FP64 network tables (uploaded on first use) and a
+    // grow-only arena for each call's inputs, outputs and workspace
+    DevBuf<double> d_plug_rad, d_plug_ang, d_plug_w, d_plug;
+    bool plug_nnp_up = false;
     NnpTables nnp_dev{};
     size_t nnp_lds = 0;
 
@@ -1057,6 +1064,10 @@ int ptmc_destroy(ptmc_handle h)
     h->d_slot_boxh.free_();
     h->d_lut64.free_();
     h->d_lut32.free_();
+    h->d_plug_rad.free_();
+    h->d_plug_ang.free_();
+    h->d_plug_w.free_();
+    h->d_plug.free_();
     h->d_maxboxh.free_();
     h->d_lh.free_();
     h->d_displ_sep.free_();
@@ -1418,6 +1429,10 @@ int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const do
     for (int q = 0; q < nf; ++q)
         for (int j = 0; j < h1; ++j) n.w[(size_t)dev_row[q] * h1 + j] = (float)params[(size_t)q * h1 + j];
     n.b3 = (float)params[n_params - 1];
+    n.rad64.assign(rad, rad + (size_t)4 * n_rad);
+    n.ang64.assign(ang, ang + (size_t)6 * n_ang);
+    n.w64.assign(params, params + n_params);
+    h->plug_nnp_up = false;
     PotParams p{};
     p.kind = POT_NNP;
     h->pot = p;
@@ -2482,6 +2497,237 @@ int ptmc_surrogate_stats(ptmc_handle h, int32_t run, int64_t *proposals, int64_t
     if (proposals) *proposals = p;
     if (stage1) *stage1 = s1;
     if (accepted) *accepted = acc;
+    return PTMC_OK;
+}
+
+}  // extern "C"
+
+// ---- potential plugin functions: set_variables / initialise_energy /
+// energy_update! on caller-supplied configurations (ptmc_plugin.hpp)
+
+namespace {
+
+// values of the potential's per-configuration state (the reference's
+// PotentialVariables that these functions pass on): EAM components, network
+// features; pair potentials keep none
+int64_t plugin_nvars(const ptmc_engine *h, int n_atoms)
+{
+    if (h->pot.kind == POT_EAM) return 2 * (int64_t)n_atoms;
+    if (h->pot.kind == POT_NNP) return (int64_t)n_atoms * h->nnp.nf;
+    return 0;
+}
+
+int plugin_job(ptmc_engine *h, int32_t n_conf, int32_t n_atoms, const double *x, const double *d2, const double *box,
+               bool update, PluginJob &j)
+{
+    if (!h->pot_set) return fail(h, PTMC_E_STATE, "no potential set");
+    if (n_conf < 1 || n_atoms < 2 || n_atoms > PLUGIN_MAXN)
+        return fail(h, PTMC_E_ARG, "need n_conf >= 1 and 2 <= n_atoms <= %d", PLUGIN_MAXN);
+    const PotParams &p = h->pot;
+    j.n_conf = n_conf;
+    j.N = n_atoms;
+    j.kind = p.kind;
+    j.bc = h->cfg.boundary;
+    j.div = h->cfg.pair_form == 1 ? 1 : 0;
+    j.pot = p;
+    const bool npt = (h->cfg.ensemble == 1 || h->cfg.ensemble == 2) && j.bc != BC_SPHERE;
+    const bool dist_only = p.kind == POT_ELJ_EVEN || p.kind == POT_ELJ || p.kind == POT_EAM;
+    if (d2 && !dist_only)
+        return fail(h, PTMC_E_ARG, "distance matrices serve the distance-only potentials (ELJ, EAM); this one needs positions");
+    if (!x && (update || !d2)) return fail(h, PTMC_E_ARG, "positions required");
+    if (j.bc != BC_SPHERE && !box) return fail(h, PTMC_E_ARG, "box lengths required in a periodic box");
+    switch (p.kind) {
+    case POT_ELJ_EVEN:
+    case POT_ELJ:
+        if (p.ncoeff > 8) return fail(h, PTMC_E_UNSUPPORTED, "at most 8 ELJ coefficients");
+        if (npt && p.kind == POT_ELJ)
+            return fail(h, PTMC_E_UNSUPPORTED, "ELJPotential (odd powers) has no tail correction for NPT");
+        j.pk = p.kind == POT_ELJ_EVEN ? (j.div ? PK_EVEN_R : PK_EVEN_H) : (j.div ? PK_ODD_R : PK_ODD_H);
+        j.cutoff = npt ? 1 : 0;
+        break;
+    case POT_ELJB:
+        j.pk = PK_ELJB;
+        j.cutoff = npt ? 1 : 0;
+        break;
+    case POT_LUT: {
+        j.pk = PK_LUT;
+        j.cutoff = npt ? 1 : 0;
+        if (!h->d_lut64.p) {
+            const int rc = upload(h, h->d_lut64, h->lut_host);
+            if (rc) return rc;
+        }
+        j.pot.lut64 = h->d_lut64.p;
+        break;
+    }
+    case POT_EAM: break;
+    case POT_NNP: {
+        const auto &n = h->nnp;
+        if (j.bc != BC_SPHERE)
+            return fail(h, PTMC_E_UNSUPPORTED, "the network potential's plugin functions run in a spherical container");
+        if (n.h1 > PLUGIN_MAXH || n.h2 > PLUGIN_MAXH) return fail(h, PTMC_E_UNSUPPORTED, "hidden layers wider than %d", PLUGIN_MAXH);
+        if (!h->plug_nnp_up) {
+            int rc = upload(h, h->d_plug_rad, n.rad64.empty() ? std::vector<double>(4, 0.0) : n.rad64);
+            if (!rc) rc = upload(h, h->d_plug_ang, n.ang64.empty() ? std::vector<double>(6, 0.0) : n.ang64);
+            if (!rc) rc = upload(h, h->d_plug_w, n.w64);
+            if (rc) return rc;
+            h->plug_nnp_up = true;
+        }
+        j.n_rad = n.n_rad;
+        j.n_ang = n.n_ang;
+        j.n_ang_built = n.q10 ? std::max(0, n.n_ang - n.n_rad) : n.n_ang;
+        j.h1 = n.h1;
+        j.h2 = n.h2;
+        j.r_cut = n.r_cut;
+        j.hardcore2 = n.hardcore2;
+        j.rad = h->d_plug_rad.p;
+        j.ang = h->d_plug_ang.p;
+        j.w = h->d_plug_w.p;
+        break;
+    }
+    default:
+        return fail(h, PTMC_E_UNSUPPORTED, "no plugin functions for this potential (two species: ptmc_nnp2_energy)");
+    }
+    return PTMC_OK;
+}
+
+// device arena of one call: regions in doubles, 256-byte aligned
+struct PlugLayout {
+    size_t x = 0, d2 = 0, box = 0, index = 0, trial = 0, en_in = 0, vars_in = 0, en = 0, en_atom = 0, vars_out = 0,
+           ws = 0, total = 0;
+};
+
+PlugLayout plug_layout(const PluginJob &j, int64_t nv, bool has_d2, bool has_box, bool update)
+{
+    const size_t C = (size_t)j.n_conf, N = (size_t)j.N;
+    PlugLayout L;
+    size_t o = 0;
+    auto take = [&](size_t n) {
+        const size_t at = o;
+        o += (n + 31) / 32 * 32;
+        return at;
+    };
+    L.x = take(C * N * 3);
+    L.d2 = take(has_d2 ? C * N * N : 0);
+    L.box = take(has_box ? C * 2 : 0);
+    L.index = take(update ? (C + 1) / 2 : 0);
+    L.trial = take(update ? C * 3 : 0);
+    L.en_in = take(update ? C : 0);
+    L.vars_in = take(update ? C * (size_t)nv : 0);
+    L.en = take(C);
+    L.en_atom = take(C * N);
+    L.vars_out = take(C * (size_t)nv);
+    L.ws = take(plugin_workspace_doubles(j));
+    L.total = o;
+    return L;
+}
+
+int plug_arena(ptmc_engine *h, size_t n)
+{
+    if (h->d_plug.n >= n) return PTMC_OK;
+    h->d_plug.free_();
+    HIPCHK(h, hipMalloc((void **)&h->d_plug.p, n * sizeof(double)));
+    h->d_plug.n = n;
+    return PTMC_OK;
+}
+
+int plug_put(ptmc_engine *h, size_t at, const void *src, size_t bytes)
+{
+    if (src && bytes) HIPCHK(h, hipMemcpyAsync(h->d_plug.p + at, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return PTMC_OK;
+}
+
+int plug_get(ptmc_engine *h, void *dst, size_t at, size_t bytes)
+{
+    if (dst && bytes) HIPCHK(h, hipMemcpyAsync(dst, h->d_plug.p + at, bytes, hipMemcpyDeviceToHost, h->stream));
+    return PTMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptmc_potential_vars(ptmc_handle h, int32_t n_atoms, int64_t *n_vars)
+{
+    if (!h || !n_vars || n_atoms < 1) return h ? fail(h, PTMC_E_ARG, "bad arguments") : PTMC_E_ARG;
+    if (!h->pot_set) return fail(h, PTMC_E_STATE, "no potential set");
+    *n_vars = plugin_nvars(h, n_atoms);
+    return PTMC_OK;
+}
+
+int ptmc_initialise_energy(ptmc_handle h, int32_t n_conf, int32_t n_atoms, const double *x, const double *d2,
+                           const double *box, double *en_tot, double *en_atom, double *vars)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!en_tot) return fail(h, PTMC_E_ARG, "en_tot required");
+    PluginJob j;
+    int rc = plugin_job(h, n_conf, n_atoms, x, d2, box, false, j);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const int64_t nv = plugin_nvars(h, n_atoms);
+    const size_t C = (size_t)n_conf, N = (size_t)n_atoms;
+    const PlugLayout L = plug_layout(j, nv, d2 != nullptr, box != nullptr, false);
+    if ((rc = plug_arena(h, L.total))) return rc;
+    double *A = h->d_plug.p;
+    if ((rc = plug_put(h, L.x, x, C * N * 3 * 8)) || (rc = plug_put(h, L.d2, d2, C * N * N * 8)) ||
+        (rc = plug_put(h, L.box, box, C * 2 * 8)))
+        return rc;
+    j.x = x ? A + L.x : nullptr;
+    j.d2_in = d2 ? A + L.d2 : nullptr;
+    j.box = box ? A + L.box : nullptr;
+    j.en = A + L.en;
+    j.en_atom = A + L.en_atom;
+    j.vars_out = nv ? A + L.vars_out : nullptr;
+    j.ws = A + L.ws;
+    HIPCHK(h, plugin_initialise(j, h->stream));
+    if ((rc = plug_get(h, en_tot, L.en, C * 8)) || (rc = plug_get(h, en_atom, L.en_atom, C * N * 8)) ||
+        (rc = plug_get(h, nv ? vars : nullptr, L.vars_out, C * (size_t)nv * 8)))
+        return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return PTMC_OK;
+}
+
+int ptmc_energy_update(ptmc_handle h, int32_t n_conf, int32_t n_atoms, const double *x, const double *d2,
+                       const double *box, const int32_t *index, const double *trial, const double *en_tot,
+                       const double *vars, double *new_en, double *new_en_atom, double *new_vars)
+{
+    if (!h) return PTMC_E_ARG;
+    if (!index || !trial || !en_tot || !new_en) return fail(h, PTMC_E_ARG, "index, trial, en_tot and new_en required");
+    PluginJob j;
+    int rc = plugin_job(h, n_conf, n_atoms, x, d2, box, true, j);
+    if (rc) return rc;
+    const int64_t nv = plugin_nvars(h, n_atoms);
+    if (nv && !vars) return fail(h, PTMC_E_ARG, "this potential's update needs its variables (ptmc_initialise_energy)");
+    for (int c = 0; c < n_conf; ++c)
+        if (index[c] < 0 || index[c] >= n_atoms) return fail(h, PTMC_E_ARG, "atom index %d out of range", index[c]);
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const size_t C = (size_t)n_conf, N = (size_t)n_atoms;
+    const PlugLayout L = plug_layout(j, nv, d2 != nullptr, box != nullptr, true);
+    if ((rc = plug_arena(h, L.total))) return rc;
+    double *A = h->d_plug.p;
+    if ((rc = plug_put(h, L.x, x, C * N * 3 * 8)) || (rc = plug_put(h, L.d2, d2, C * N * N * 8)) ||
+        (rc = plug_put(h, L.box, box, C * 2 * 8)) || (rc = plug_put(h, L.index, index, C * 4)) ||
+        (rc = plug_put(h, L.trial, trial, C * 3 * 8)) || (rc = plug_put(h, L.en_in, en_tot, C * 8)) ||
+        (rc = plug_put(h, L.vars_in, nv ? vars : nullptr, C * (size_t)nv * 8)))
+        return rc;
+    // a network configuration that fails the hard-core test keeps its per-atom energies
+    if (j.kind == POT_NNP && (rc = plug_put(h, L.en_atom, new_en_atom, C * N * 8))) return rc;
+    j.x = A + L.x;
+    j.d2_in = d2 ? A + L.d2 : nullptr;
+    j.box = box ? A + L.box : nullptr;
+    j.index = reinterpret_cast<const int32_t *>(A + L.index);
+    j.trial = A + L.trial;
+    j.en_in = A + L.en_in;
+    j.vars_in = nv ? A + L.vars_in : nullptr;
+    j.en = A + L.en;
+    j.en_atom = A + L.en_atom;
+    j.vars_out = nv ? A + L.vars_out : nullptr;
+    j.ws = A + L.ws;
+    HIPCHK(h, plugin_update(j, h->stream));
+    if ((rc = plug_get(h, new_en, L.en, C * 8)) ||
+        (rc = plug_get(h, nv ? new_en_atom : nullptr, L.en_atom, C * N * 8)) ||
+        (rc = plug_get(h, nv ? new_vars : nullptr, L.vars_out, C * (size_t)nv * 8)))
+        return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return PTMC_OK;
 }
 

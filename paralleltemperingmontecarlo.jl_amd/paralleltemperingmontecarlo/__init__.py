@@ -28,6 +28,7 @@ from .api import (  # noqa: F401
     LookupTablePotential,
     Output,
     PTEngine,
+    PotentialPlugin,
     RadialType2,
     RectangularBC,
     RhombicBC,

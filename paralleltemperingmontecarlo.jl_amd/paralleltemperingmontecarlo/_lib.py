@@ -134,6 +134,9 @@ SIGNATURES = {
     "ptmc_surrogate_grad": [_h, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp],
     "ptmc_surrogate_stats": [_h, C.c_int32, _i64p, _i64p, _i64p],
     "ptmc_run_group": [_h, C.c_int32],
+    "ptmc_potential_vars": [_h, C.c_int32, _i64p],
+    "ptmc_initialise_energy": [_h, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ptmc_energy_update": [_h, C.c_int32, C.c_int32, _dp, _dp, _dp, _i32p, _dp, _dp, _dp, _dp, _dp, _dp],
 }
 
 # stateless entry points (no handle)

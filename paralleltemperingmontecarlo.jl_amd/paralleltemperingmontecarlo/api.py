@@ -709,6 +709,114 @@ def _run_desc(n_atoms, beta, x0, bc, pressure, opts: EngineOptions, lanes=0, run
     return d
 
 
+class PotentialPlugin:
+    """The potential plugin functions of the reference on the GPU, batched
+    over configurations the caller holds: set_variables + initialise_energy
+    (src/EnergyEvaluation.jl:1662-1747, :1778-1822) and energy_update! for
+    one displaced atom (:1474-1621), in FP64, for ELJPotentialEven /
+    ELJPotential, ELJPotentialB, LookupTablePotential, EmbeddedAtomPotential
+    and RuNNerPotential (spherical container).
+
+        plug = PotentialPlugin(pot, NVT(N), SphericalBC(radius=r))
+        en_tot, en_atom, vars = plug.initialise_energy(x)          # x: (n_conf, N, 3)
+        new_en, new_vars, new_en_atom = plug.energy_update(x, index, trial, en_tot, vars)
+
+    vars are the reference's potential variables per configuration: EAM
+    components (N, 2), network features (N, n_rad + n_ang) (atom-major: the
+    transpose of the reference's g_matrix); None for pair potentials.
+    ``d2`` (ELJ, EAM) passes the reference's squared-distance matrices
+    instead of deriving them from x.  pair_form "reference" (default)
+    evaluates pairs and minimum images with the reference's divisions,
+    "horner" with Horner in 1/r^2 and reciprocal products."""
+
+    def __init__(self, pot, ensemble=None, bc=None, *, pair_form: str = "reference", device: int = 0):
+        ensemble = ensemble if ensemble is not None else NVT(0)
+        bc = bc if bc is not None else SphericalBC(radius=1.0)
+        cfg = _lib.ptmc_config()
+        cfg.precision = 0
+        cfg.ensemble = (2 if ensemble.separated_volume else 1) if isinstance(ensemble, NPT) else 0
+        cfg.boundary = _bc_code(bc)
+        cfg.pair_form = {"horner": 0, "reference": 1}[pair_form]
+        cfg.device = device
+        # cycle settings a handle carries (the plugin functions run no cycles)
+        cfg.n_adjust, cfg.mc_sample, cfg.n_bin, cfg.min_acc, cfg.max_acc = 100, 1, 100, 0.4, 0.6
+        self.bc = bc
+        self.periodic = not isinstance(bc, SphericalBC)
+        self.eng = _lib.Engine(cfg)
+        _set_potential(self.eng, pot)
+        self.pot = pot
+
+    def n_vars(self, n_atoms: int) -> int:
+        n = C.c_int64()
+        self.eng.call("ptmc_potential_vars", int(n_atoms), C.byref(n))
+        return n.value
+
+    def _box(self, n_conf, box):
+        if not self.periodic:
+            return None
+        if box is None:
+            h = self.bc.box_height if _has_height(self.bc) else self.bc.box_length
+            box = [(self.bc.box_length, h)] * n_conf
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(box, np.float64).reshape(-1, 2), (n_conf, 2)))
+
+    def _vars_shape(self, n_conf, n_atoms):
+        nv = self.n_vars(n_atoms)
+        return None if nv == 0 else (n_conf, n_atoms, nv // n_atoms)
+
+    def initialise_energy(self, x=None, *, d2=None, box=None):
+        """(en_tot (n_conf,), en_atom (n_conf, N), vars or None)."""
+        if x is not None:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            x = x.reshape(-1, x.shape[-2], 3)
+            n_conf, n_atoms = x.shape[:2]
+        if d2 is not None:
+            d2 = np.ascontiguousarray(d2, dtype=np.float64)
+            d2 = d2.reshape(-1, d2.shape[-1], d2.shape[-1])
+            n_conf, n_atoms = d2.shape[:2]
+        if x is None and d2 is None:
+            raise ValueError("positions or distance matrices required")
+        b = self._box(n_conf, box)
+        en = np.zeros(n_conf)
+        ea = np.zeros((n_conf, n_atoms))
+        vs = self._vars_shape(n_conf, n_atoms)
+        v = None if vs is None else np.zeros(vs)
+        self.eng.call("ptmc_initialise_energy", n_conf, n_atoms, _lib.ptr(x), _lib.ptr(d2), _lib.ptr(b),
+                      _lib.ptr(en), _lib.ptr(ea), _lib.ptr(v))
+        return en, ea, v
+
+    def energy_update(self, x, index, trial, en_tot, vars=None, *, d2=None, box=None, en_atom=None):
+        """(new_en (n_conf,), new_vars or None, new_en_atom or None) after
+        moving atom index[c] of configuration c to trial[c].  en_atom: the
+        per-atom energies a configuration that fails the network's
+        hard-core test keeps (default NaN)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = x.reshape(-1, x.shape[-2], 3)
+        n_conf, n_atoms = x.shape[:2]
+        idx = np.ascontiguousarray(np.broadcast_to(np.asarray(index, np.int32), (n_conf,)))
+        t = np.ascontiguousarray(np.asarray(trial, np.float64).reshape(n_conf, 3))
+        e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(en_tot, np.float64), (n_conf,)))
+        if d2 is not None:
+            d2 = np.ascontiguousarray(d2, dtype=np.float64).reshape(n_conf, n_atoms, n_atoms)
+        b = self._box(n_conf, box)
+        vs = self._vars_shape(n_conf, n_atoms)
+        v_in = nv = nea = None
+        if vs is not None:
+            if vars is None:
+                raise ValueError("this potential's update needs the variables initialise_energy returned")
+            v_in = np.ascontiguousarray(vars, dtype=np.float64).reshape(vs)
+            nv = np.zeros(vs)
+            nea = (np.full((n_conf, n_atoms), np.nan) if en_atom is None
+                   else np.array(np.broadcast_to(en_atom, (n_conf, n_atoms)), dtype=np.float64))
+        ne = np.zeros(n_conf)
+        self.eng.call("ptmc_energy_update", n_conf, n_atoms, _lib.ptr(x), _lib.ptr(d2), _lib.ptr(b),
+                      _lib.ptr(idx, C.c_int32), _lib.ptr(t), _lib.ptr(e0), _lib.ptr(v_in), _lib.ptr(ne),
+                      _lib.ptr(nea), _lib.ptr(nv))
+        return ne, nv, nea
+
+    def close(self):
+        self.eng.close()
+
+
 class PTEngine:
     """A batch of PT runs that advance together on the GPU."""
 
