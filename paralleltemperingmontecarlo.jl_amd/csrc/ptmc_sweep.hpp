@@ -638,16 +638,19 @@ __device__ bool sphere_redraw(Real *X, int tid, int q, int N, int A, int si, int
     while (outside_sphere<Real>(tx, ty, tz, radius2)) {
         count += 1;
         if (count == 50) {
+            // every lane of the cluster forms the same sequential mean, atom by atom
+            const int gb = tid - q;
             double sx = 0.0, sy = 0.0, sz = 0.0;
-            for (int s = 0; s < A; ++s) {
-                if (s * KL + q >= N) break;
-                sx += (double)X[xoff<LANES>(s, 0, tid)];
-                sy += (double)X[xoff<LANES>(s, 1, tid)];
-                sz += (double)X[xoff<LANES>(s, 2, tid)];
+            for (int j = 0; j < N; ++j) {
+                const int sj = j / KL, lj = gb + (j % KL);
+                sx += (double)X[xoff<LANES>(sj, 0, lj)];
+                sy += (double)X[xoff<LANES>(sj, 1, lj)];
+                sz += (double)X[xoff<LANES>(sj, 2, lj)];
             }
-            sx = group_sum<KL>(sx) / (double)N;
-            sy = group_sum<KL>(sy) / (double)N;
-            sz = group_sum<KL>(sz) / (double)N;
+            sx /= (double)N;
+            sy /= (double)N;
+            sz /= (double)N;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             for (int s = 0; s < A; ++s) {
                 if (s * KL + q >= N) break;
                 X[xoff<LANES>(s, 0, tid)] = (Real)((double)X[xoff<LANES>(s, 0, tid)] - sx);

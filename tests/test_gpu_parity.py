@@ -342,3 +342,71 @@ def test_batch_packing_is_invisible(oracle):
     for a, b in zip(*results):
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("lanes,precision,wg", [(1, "fp64", 0), (4, "fp64", 0), (8, "fp64", 0), (4, "fp32", 64)])
+def test_sphere_recentre_matches_oracle(oracle, ne13, lanes, precision, wg):
+    """The Ne13 start sits 20 bohr off the centre of its 10 bohr container:
+    every first trial lands outside, the 50th failure recentres the
+    configuration on its sequential mean (src/MCMoves.jl:115-135,
+    src/Configurations.jl:79-85) and the redraws that follow land inside.
+    FP64 (one, four and eight lanes per configuration): the same moves,
+    decisions and positions as the oracle, bit for bit; binary32 in the
+    spread layout: the same moves and positions with the device's decisions
+    replayed."""
+    x0 = ne13["pos"] + np.array([20.0, 0.0, 0.0])
+    n_temps = 4
+    temp = pt.TempGrid(4.0, 16.0, n_temps)
+    mc = pt.MCParams(100, n_temps, 13, n_adjust=50)
+    bc = pt.SphericalBC(radius=ne13["radius"])
+    opts = dict(precision=precision, lanes_per_replica=lanes, exchange="every", seed=99)
+    if wg:
+        opts["workgroup_lanes"] = wg
+    print(f"recentre lanes={lanes} {precision} wg={wg}", flush=True)
+    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), pt.NVT(13), bc, mc, **opts)
+    run = eng.add_run(13, temp.beta_grid, x0[None], bc, run_key=0)
+    eng.prepare()
+    assert eng.lanes_of(run) == lanes
+    orc = oracle.OracleRun(13, temp.beta_grid, x0, pot=oracle.make_pot("elj_even", ne13["coeff"]),
+                           radius=ne13["radius"], mode=oracle.engine_mode(lanes), fp32=precision == "fp32",
+                           exchange_mode=1, n_adjust=50, seed=99, run_key=0)
+    eng.set_trace(3)
+    eng.cycles(3)
+    eng.sync()
+    de, acc, idx = eng.trace(run, 3)
+    tr = orc.cycles(3, trace=True, replay_acc=acc if precision == "fp32" else None)
+    np.testing.assert_array_equal(idx, tr["idx"])
+    x, en, _ = eng.state(run)
+    assert np.all(np.abs(x.mean(axis=1)) < 1.0), x.mean(axis=1)  # recentred
+    assert np.all((x**2).sum(-1) <= ne13["radius"] ** 2)
+    assert np.array_equal(x, orc.x.reshape(x.shape)), np.max(np.abs(x - orc.x.reshape(x.shape)))
+    if precision == "fp64":
+        np.testing.assert_array_equal(acc, tr["acc"])
+        _compare_exact(eng, run, orc)
+    eng.close()
+
+
+def test_sphere_error_after_100_failures(oracle, ne13):
+    """A 3 bohr container around Ne13 (outer shell at 5.9 bohr): a moved
+    shell atom's trials never come inside, the recentre at the 50th failure
+    does not help and the 100th is the reference's error "too many moves out
+    of binding sphere" (src/MCMoves.jl:132): PTMC_E_SPHERE from ptmc_sync, as
+    the oracle raises its error flag on the same run."""
+    temp = pt.TempGrid(4.0, 16.0, 4)
+    mc = pt.MCParams(100, 4, 13, n_adjust=50)
+    bc = pt.SphericalBC(radius=3.0)
+    print("sphere error at 100 failures", flush=True)
+    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), pt.NVT(13), bc, mc, precision="fp64", lanes_per_replica=4,
+                  exchange="every", seed=5)
+    eng.add_run(13, temp.beta_grid, ne13["pos"][None], bc, run_key=0)
+    eng.prepare()
+    eng.cycles(1)
+    with pytest.raises(pt.PTMCError) as e:
+        eng.sync()
+    assert e.value.code == 4  # PTMC_E_SPHERE
+    orc = oracle.OracleRun(13, temp.beta_grid, ne13["pos"], pot=oracle.make_pot("elj_even", ne13["coeff"]),
+                           radius=3.0, mode=oracle.engine_mode(4), exchange_mode=1, n_adjust=50, seed=5, run_key=0)
+    with pytest.raises(RuntimeError, match="binding sphere"):
+        orc.cycles(1)
+    assert orc.error[0] == 1
+    eng.close()
