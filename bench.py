@@ -100,16 +100,20 @@ def copy_probe(n_bytes, reps=5):
     print(json.dumps({"copy_probe_bytes": n_bytes, "dispatches": reps}))
 
 
-def traffic_per_launch(kernel, cycles_per_launch):
+def traffic_per_launch(kernel, cycles_per_launch, tiles_per_launch=None):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration), if one was recorded for this kernel and launch length."""
+    calibration), if one was recorded for this kernel and launch length; a
+    record that states its workgroups per launch is scaled to this launch's."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     for rec in json.loads(f.read_text()).get("kernels", []):
         if rec.get("kernel") == kernel and rec.get("cycles_per_launch") == cycles_per_launch:
-            return rec.get("hbm_bytes_per_launch")
+            b = rec.get("hbm_bytes_per_launch")
+            if tiles_per_launch and rec.get("workgroups_per_launch"):
+                b = b * tiles_per_launch / rec["workgroups_per_launch"]
+            return b
     return None
 
 
@@ -584,10 +588,16 @@ def main():
         e = eng.energies(i)
         eng.set_histogram(i, float(e.min()), float(e.max()))
     eng.start_phase(True)
-    eng.cycles(args.warmup + pad)
+    # the warm-up's bookkeeping (counters cleared, its launch timings folded in
+    # and cleared) happens before its last few cycles, so that the device runs
+    # up to the timed region instead of idling through it
+    prime = min(5, args.warmup + pad)
+    eng.cycles(args.warmup + pad - prime)
     eng.reset_counters()
-    eng.sync()  # folds the warm-up's launch timings in before they are cleared
+    eng.sync()
     eng.kernel_time(reset=True)
+    eng.cycles(prime)
+    eng.kernel_time(reset=True)  # only the last cycles' few launches to fold in
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eng.cycles(args.steps)
@@ -626,7 +636,7 @@ def main():
     acc, acc_cold = [], []
     for i in range(len(runs)):
         c = eng.counters(i)
-        a = c["count_atom"][:, 0] / (runs[i]["n_atoms"] * (args.steps + tail))  # counted since the timed region
+        a = c["count_atom"][:, 0] / (runs[i]["n_atoms"] * (prime + args.steps + tail))  # counted from the last warm-up cycles
         acc.append(float(np.mean(a)))
         acc_cold.append(float(np.mean(a[: runs[i]["n_temps"] // 2])))
     lanes_used = {}
@@ -684,7 +694,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": traffic_per_launch(kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
-                                          cyc_launch),
+                                          cyc_launch, plan["tiles"] / max(plan["launch_groups"], 1) if spread
+                                          else None),
             "cycles_per_launch": cyc_launch,
             "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
             "kernel_flops_per_launch": dom_flops / max(gi["launches"], 1),
