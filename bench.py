@@ -461,12 +461,20 @@ def bench_delayed_acceptance(pt, n_ladders, warmup, steps, collect, epochs):
     dt = time.perf_counter() - t0
     after = [eng.surrogate_stats(r) for r in range(n_ladders)]
     d = {k: sum(a[k] - b[k] for a, b in zip(after, before)) for k in ("proposals", "stage1", "accepted")}
+    # one in-run retraining (5 more epochs on the buffer, from the current
+    # weights) and its cost amortised over the 10^4 sweeps between retrainings
+    t0 = time.perf_counter()
+    _, rmse_re = eng.train_surrogate(5, batch=128, lr=1e-3)
+    t_retrain = time.perf_counter() - t0
+    per_sweep = dt / steps
+    amortised = d["proposals"] / dt * (1e4 * per_sweep) / (1e4 * per_sweep + t_retrain)
     out = dict(workload=f"delayed acceptance, NNP Cu55, {n_ladders} ladders x 40 temperatures 200-1400 K",
                moves_per_s=d["proposals"] / dt, plain_moves_per_s=plain, speedup_vs_plain=d["proposals"] / dt / plain,
                exact_evaluations_per_proposal=d["stage1"] / d["proposals"],
                acceptance=d["accepted"] / d["proposals"], stage2_acceptance=d["accepted"] / max(d["stage1"], 1),
                samples=eng.surrogate_samples(), train_epochs=epochs, train_s=t_train,
-               surrogate_rmse_per_atom=dict(untrained=rmse0, train=rmse_tr, valid=rmse_va), steps=steps,
+               surrogate_rmse_per_atom=dict(untrained=rmse0, train=rmse_tr, valid=rmse_va, after_retrain=rmse_re),
+               retrain_every_sweeps=10000, retrain_s=t_retrain, moves_per_s_with_retraining=amortised, steps=steps,
                dtype="fp32 features/network, bf16 surrogate (matrix cores)")
     eng.close()
     return out

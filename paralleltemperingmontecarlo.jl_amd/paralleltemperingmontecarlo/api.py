@@ -1029,6 +1029,42 @@ class PTEngine:
         self.eng.call("ptmc_surrogate_stats", int(run), C.byref(a), C.byref(b), C.byref(c))
         return dict(proposals=a.value, stage1=b.value, accepted=c.value)
 
+    def delayed_acceptance_cycles(self, n_cycles: int, retrain_every: int = 10_000, epochs: int = 5,
+                                  batch: int = 128, lr: float = 1e-3, collect_every: int = 1,
+                                  valid_fraction: float = 0.1, log=None):
+        """n_cycles cycles with delayed acceptance while the run keeps adding
+        (configuration, exact energy) samples to the surrogate's ring buffer
+        (one per configuration every collect_every cycles), and every
+        retrain_every cycles the surrogate is trained further on the buffer
+        (epochs more epochs from its current weights and Adam state) before
+        the run continues with it.  Requires surrogate_init.  Returns one
+        record per retraining: cycle, samples held, seconds, RMSE per atom on
+        training and held-out samples."""
+        if retrain_every < 1 or n_cycles < 0:
+            raise ValueError("retrain_every must be positive and n_cycles non-negative")
+        self.delayed_acceptance(True)
+        self.surrogate_collect(collect_every)
+        history, done = [], 0
+        try:
+            while done < n_cycles:
+                k = min(retrain_every, n_cycles - done)
+                self.cycles(k)
+                done += k
+                if k == retrain_every and done < n_cycles:
+                    self.sync()
+                    import time
+
+                    t0 = time.perf_counter()
+                    rt, rv = self.train_surrogate(epochs, batch=batch, lr=lr, valid_fraction=valid_fraction)
+                    rec = dict(cycle=done, samples=self.surrogate_samples(), train_s=time.perf_counter() - t0,
+                               rmse_train=rt, rmse_valid=rv)
+                    history.append(rec)
+                    if log:
+                        log(rec)
+        finally:
+            self.surrogate_collect(0)
+        return history
+
     def close(self):
         self.eng.close()
 

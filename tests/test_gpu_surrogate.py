@@ -214,3 +214,40 @@ def test_training_gains_over_the_untrained_network(pot, saved):
     assert eff1 > eff0, (eff0, eff1)
     assert exact_per_proposal < 1.0
     eng.close()
+
+
+def test_retraining_inside_the_run(pot, saved):
+    """Delayed acceptance with the surrogate retrained on a schedule inside
+    the run (the configuration asks for every 10^4 sweeps; here every 40
+    cycles): the run keeps adding its own configurations to the sample
+    buffer, each retraining continues from the current weights, its held-out
+    error falls below the untrained network's, and every cycle is a
+    delayed-acceptance cycle (one proposal per atom and configuration)."""
+    temp = pt.TempGrid(500.0, 800.0, 4)
+    radius = 8 * A2B
+    eng = _engine(pot, 13, radius, 13, 4)
+    for r in range(16):
+        eng.add_run(13, temp.beta_grid, _cu13(saved)[None], pt.SphericalBC(radius), run_key=r)
+    eng.prepare()
+    eng.surrogate_init(capacity=32768, seed=6)
+    eng.start_phase(False)
+    eng.cycles(20)
+    eng.surrogate_collect(1)
+    eng.cycles(40)
+    eng.surrogate_collect(0)
+    _, rmse0 = eng.train_surrogate(0)
+    p0 = eng.surrogate_params()
+    before = [eng.surrogate_stats(r) for r in range(16)]
+    print("delayed acceptance with retraining every 40 cycles", flush=True)
+    hist = eng.delayed_acceptance_cycles(120, retrain_every=40, epochs=4, batch=64, lr=2e-3)
+    eng.sync()
+    after = [eng.surrogate_stats(r) for r in range(16)]
+    assert [h["cycle"] for h in hist] == [40, 80]
+    assert hist[0]["samples"] == 40 * 64 + 40 * 64 and hist[1]["samples"] == hist[0]["samples"] + 40 * 64
+    assert all(np.isfinite(h["rmse_valid"]) and h["rmse_valid"] < rmse0 for h in hist), (rmse0, hist)
+    p1 = eng.surrogate_params()
+    assert not np.array_equal(p0, p1)
+    proposals = sum(a["proposals"] - b["proposals"] for a, b in zip(after, before))
+    assert proposals == 120 * 16 * 4 * 13
+    assert eng.surrogate_samples() == 40 * 64 + 120 * 64
+    eng.close()
