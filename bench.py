@@ -83,7 +83,7 @@ def kernel_symbol(kl, wgs):
     (binary32, Lennard-Jones pair form, spherical container, NVT); kl = 0:
     the spread layout's kernel (lanes per configuration read per tile)."""
     if kl == 0:
-        return f"void ptmc::spread_sweep_kernel<float, 4, 4, {wgs}>(ptmc::SweepArgs)"
+        return f"void ptmc::spread_sweep_kernel<float, 4, 4, 0, 0, {wgs}>(ptmc::SweepArgs)"
     return f"void ptmc::sweep_kernel<float, 4, 4, 0, 0, {kl}, {wgs}>(ptmc::SweepArgs)"
 
 

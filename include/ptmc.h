@@ -65,8 +65,9 @@ typedef struct {
     int32_t workgroup_lanes;   /* 0: every ladder in one workgroup, cycles_per_launch cycles per
                                   launch with the exchange inside the kernel; 64, 128 or 256:
                                   ladders spread over workgroups of this many lanes, one cycle per
-                                  launch and a per-ladder exchange kernel (pair potentials,
-                                  SphericalBC, NVT); same results either way */
+                                  launch and a per-ladder exchange kernel (pair potentials in
+                                  the Horner form: SphericalBC in NVT, CubicBC or RectangularBC
+                                  in NVT or NPT with uniform volume moves); same results */
     int32_t spread_streams;    /* spread layout: the runs are cut into this many independent
                                   groups launched on their own streams, so that one group's
                                   cycle overlaps another's (0 or 1: one group); same results */

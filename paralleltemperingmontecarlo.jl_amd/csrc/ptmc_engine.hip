@@ -358,13 +358,16 @@ int plan_spread(ptmc_engine *h)
     const size_t rsize = prec == 0 ? sizeof(double) : sizeof(float);
     if (W != 64 && W != 128 && W != 256)
         return fail(h, PTMC_E_ARG, "workgroup_lanes must be 0, 64, 128 or 256");
-    if (h->cfg.boundary != BC_SPHERE || h->cfg.ensemble != ENS_NVT || h->pot.kind == POT_EAM ||
-        h->pot.kind == POT_ELJB || h->pot.kind == POT_LUT || h->cfg.pair_form != 0)
+    const bool sphere = h->cfg.boundary == BC_SPHERE;
+    const bool box_ok = h->cfg.boundary == BC_CUBIC || h->cfg.boundary == BC_RECT;
+    if (!((sphere && h->cfg.ensemble == ENS_NVT) || (box_ok && !h->separated)) || h->nnvt ||
+        h->pot.kind == POT_EAM || h->pot.kind == POT_ELJB || h->pot.kind == POT_LUT || h->cfg.pair_form != 0)
         return fail(h, PTMC_E_UNSUPPORTED,
-                    "the spread layout covers Horner-form pair potentials in a spherical container (NVT)");
+                    "the spread layout covers Horner-form pair potentials in a spherical container (NVT) or a "
+                    "cubic / rectangular box (NVT, NPT with uniform volume moves)");
     if (h->cfg.spread_streams < 0 || h->cfg.spread_streams > 4)
         return fail(h, PTMC_E_ARG, "spread_streams must be 0 to 4");
-    const KernelPair kp = lookup_spread(prec, h->pk, h->kc, W);
+    const KernelPair kp = lookup_spread(prec, h->pk, h->kc, W, sphere ? BC_SPHERE : BC_CUBIC, h->cfg.ensemble);
     if (!kp.sweep) return fail(h, PTMC_E_UNSUPPORTED, "no spread kernel for pair form %d/%d", h->pk, h->kc);
     int dev = 0;
     HIPCHK(h, hipGetDevice(&dev));

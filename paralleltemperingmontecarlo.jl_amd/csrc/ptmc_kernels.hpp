@@ -34,9 +34,10 @@ KernelPair lookup_f32_rhomb_nvt(int pk, int kc, int kl, int lanes);
 KernelPair lookup_f32_rhomb_npt(int pk, int kc, int kl, int lanes);
 // Sutton-Chen EAM, NVT, 256-lane workgroups
 KernelPair lookup_eam(int precision, int bc, int kl, int lanes);
-// spread layout (spherical container, NVT): 64-, 128- or 256-lane workgroups,
-// lanes per configuration read from each tile; and its per-ladder cycle kernel
-KernelPair lookup_spread(int precision, int pk, int kc, int lanes);
+// spread layout (spherical container in NVT; cubic / rectangular boxes, bc
+// BC_CUBIC, in NVT and NPT): 64-, 128- or 256-lane workgroups, lanes per
+// configuration read from each tile; and its per-ladder cycle kernel
+KernelPair lookup_spread(int precision, int pk, int kc, int lanes, int bc, int ens);
 const void *spread_cycle_fn();
 
 }  // namespace ptmc

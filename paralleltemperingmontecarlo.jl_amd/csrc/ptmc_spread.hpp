@@ -1,4 +1,5 @@
-// Spread layout of the pair-potential sweep (spherical container, NVT).
+// Spread layout of the pair-potential sweep (spherical container in NVT; cubic
+// and rectangular boxes in NVT and NPT with uniform volume moves).
 //
 // The ladder-per-workgroup kernel (sweep_kernel) keeps every temperature of a
 // run in one workgroup so the exchange can run in LDS, many cycles per
@@ -31,29 +32,81 @@ __device__ __forceinline__ T group_sum_rt(T v, int kl)
     return v;
 }
 
-// pair_sum64 of a spherical-container configuration (no cut-off) for a run
-// time cluster width: rows j of this lane, partners l > j, cluster sum.
-template <typename Real, int PK, int KC, int LANES>
-__device__ double sphere_pair_sum64(const Real *X, int gb, int q, int N, int kl, int kls, const double (&c64)[KC])
+// pair_sum64 for a run-time cluster width: rows j of this lane, partners
+// l > j, cluster sum.  Positions scaled by (px, px, pz), squared distances in
+// box bx (minimum image in a periodic box) times m2; NPT keeps pairs with
+// r^2 <= rc.  The same operations and order as pair_sum64.
+template <typename Real, int PK, int KC, int BC, int ENS, int LANES>
+__device__ double spread_pair_sum64(const Real *X, int gb, int q, int N, int kl, int kls, const BoxC<double> &bx,
+                                    double px, double pz, double m2, double rc, const double (&c64)[KC])
 {
     const int A = (N + kl - 1) >> kls;
     double part = 0.0;
     for (int s = 0; s < A; ++s) {
         const int j = (s << kls) + q;
         if (j >= N) break;
-        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)];
-        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)];
-        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)];
+        const double xj = (double)X[xoff<LANES>(s, 0, gb + q)] * px;
+        const double yj = (double)X[xoff<LANES>(s, 1, gb + q)] * px;
+        const double zj = (double)X[xoff<LANES>(s, 2, gb + q)] * pz;
         for (int l = j + 1; l < N; ++l) {
             const int sl = l >> kls, ll = gb + (l & (kl - 1));
-            const double xl = (double)X[xoff<LANES>(sl, 0, ll)];
-            const double yl = (double)X[xoff<LANES>(sl, 1, ll)];
-            const double zl = (double)X[xoff<LANES>(sl, 2, ll)];
-            const double dx = xj - xl, dy = yj - yl, dz = zj - zl;
-            part += pair_energy64<PK, KC>(dx * dx + dy * dy + dz * dz, c64);
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
+            double r2 = box_d2<double, BC, false>(xj, yj, zj, xl, yl, zl, bx);
+            r2 = r2 * m2;
+            if (ENS == ENS_NPT) {
+                if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+            } else {
+                part += pair_energy64<PK, KC>(r2, c64);
+            }
         }
     }
     return group_sum_rt(part, kl);
+}
+
+// wave_pair_sum64 for a run-time cluster width: the NPT pair sum of the
+// configuration of the cluster whose first lane is wave lane `cl`, computed
+// by every executing lane of the wave together; pairs (j, j + d mod N) dealt
+// round-robin over the `live` lanes, cluster sums, then the totals of the
+// `leaders` in lane order (the same operations and order as wave_pair_sum64).
+template <int PK, int KC, int BC, int LANES, typename Real>
+__device__ double wave_pair_sum64_rt(const Real *X, int cl, int N, int kl, int kls, const BoxC<double> &bx,
+                                     double m2, double rc, const double (&c64)[KC], uint64_t live, uint64_t leaders)
+{
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
+    const int nl = __popcll(live);
+    const int h = N / 2;
+    const bool even = (N & 1) == 0;
+    const int cb = (threadIdx.x & ~63) + cl;
+    int j = rank % N, d = rank / N + 1;
+    double part = 0.0;
+    while (d <= h) {
+        int l = j + d;
+        if (l >= N) l -= N;
+        if (!(even && d == h && j >= h)) {
+            const int sj = j >> kls, lj = cb + (j & (kl - 1));
+            const int sl = l >> kls, ll = cb + (l & (kl - 1));
+            const double xj = (double)X[xoff<LANES>(sj, 0, lj)] * 1.0;
+            const double yj = (double)X[xoff<LANES>(sj, 1, lj)] * 1.0;
+            const double zj = (double)X[xoff<LANES>(sj, 2, lj)] * 1.0;
+            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * 1.0;
+            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * 1.0;
+            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * 1.0;
+            const double r2 = box_d2<double, BC, false>(xj, yj, zj, xl, yl, zl, bx) * m2;
+            if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
+        }
+        j += nl;
+        while (j >= N) {
+            j -= N;
+            ++d;
+        }
+    }
+    part = group_sum_rt(part, kl);
+    double tot = 0.0;
+    for (uint64_t b = leaders; b != 0; b &= b - 1) tot += readlane_d(part, __builtin_ctzll(b));
+    return tot;
 }
 
 // sphere_redraw for a run-time cluster width (src/MCMoves.jl:115-135).
@@ -116,10 +169,17 @@ __device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls
 // c + 1, when sweep0 % recompute_every == 0; the engine adds a launch with
 // n_cycles = 0 after a call's last cycle so that it is never left pending,
 // and marks the next call's first launch (recompute_done) not to repeat it.
-template <typename Real, int PK, int KC, int LANES>
+// BC_CUBIC serves cubic and rectangular boxes (box kind at run time); NPT
+// adds the uniform volume move as the cycle's (N + 1)-th move
+// (src/MCMoves.jl:169-177, :277-287; energy src/EnergyEvaluation.jl:397-474;
+// acceptance src/Exchange.jl:58-71).
+template <typename Real, int PK, int KC, int BC, int ENS, int LANES>
 __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
 {
+    static_assert((BC == BC_SPHERE && ENS == ENS_NVT) || BC == BC_CUBIC,
+                  "spread layout: spherical container (NVT) or cubic / rectangular boxes");
     constexpr bool F32 = sizeof(Real) == 4;
+    constexpr bool PBC = BC != BC_SPHERE;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const TileInfo tile = a.tiles[blockIdx.x];
     const int KL = tile.kl;
@@ -131,7 +191,8 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
     const bool active = lg < tile.n_slots;
     const int N = tile.n_atoms;
     const int A = tile.n_slab;
-    const uint32_t nsteps = (uint32_t)N;
+    const uint32_t nsteps = (uint32_t)(ENS == ENS_NPT ? N + 1 : N);
+    const int kind = PBC ? a.box_kind : (int)BC_SPHERE;
     Real *X = reinterpret_cast<Real *>(lds_raw);
 
     Real c[KC];
@@ -159,8 +220,17 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
         const Real radius2 = (Real)ri.radius2;
         const int t = a.slot_temp[slot];
         double E = a.slot_en[slot];
+        double box = 1.0, boxh = 1.0;
+        if constexpr (PBC) {
+            box = a.slot_box[slot];
+            boxh = a.slot_boxh[slot];
+        }
         if (!a.recompute_done && a.recompute_every > 0 && a.sweep0 > 0 && a.sweep0 % a.recompute_every == 0) {
-            const double ef = sphere_pair_sum64<Real, PK, KC, LANES>(X, gb, q, N, KL, kls, c64);
+            const double rc64 = PBC ? box_rcut(kind, box, boxh) : 0.0;
+            double ef = spread_pair_sum64<Real, PK, KC, BC, ENS, LANES>(X, gb, q, N, KL, kls,
+                                                                       make_box<double>(box, boxh), 1.0, 1.0, 1.0,
+                                                                       rc64, c64);
+            if constexpr (ENS == ENS_NPT) ef = ef + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
             const double dr = __builtin_fabs(E - ef);
             if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
             E = ef;
@@ -175,7 +245,13 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             // integer of u: u - 1/2 and the products by powers of two are
             // exact, so the result is the rounding of the same exact value
             const float dsc = (float)delta_r * 0x1.0p-24f, dof = -0.5f * (float)delta_r;
-            int nacc = 0;
+            BoxC<Real> bxr{};
+            Real rc = 0;
+            if constexpr (PBC) {
+                bxr = make_box<Real>(box, boxh);
+                rc = (Real)box_rcut(kind, box, boxh);
+            }
+            int nacc = 0, nvacc = 0;
             u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
             u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
             if (KL > 1) {
@@ -198,109 +274,208 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 }
                 const MoveDraw<Real> d0 = decode_move<Real>(cur, curB, nsteps);
                 const uint32_t i = d0.index;
-                const int si = (int)(i >> kls), li = gb + (int)(i & (uint32_t)(KL - 1));
-                Real xi = X[xoff<LANES>(si, 0, li)];
-                Real yi = X[xoff<LANES>(si, 1, li)];
-                Real zi = X[xoff<LANES>(si, 2, li)];
-                Real tx, ty, tz;
-                if constexpr (F32) {
-                    tx = xi + __builtin_fmaf((float)(cur.y >> 8), dsc, dof);
-                    ty = yi + __builtin_fmaf((float)(cur.z >> 8), dsc, dof);
-                    tz = zi + __builtin_fmaf((float)(cur.w >> 8), dsc, dof);
-                } else {
-                    make_trial<Real, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
-                }
-                // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
-                if (outside_sphere<Real>(tx, ty, tz, radius2)) {
-                    if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, si, li, radius2, delta_r, a.seed,
-                                                       run_key, (uint32_t)t, sweep, m, nsteps, xi, yi, zi, tx, ty,
-                                                       tz) &&
-                        q == 0)
-                        a.run_error[a.slot_run[slot]] = 1;
-                }
-                // park the moved atom so its self pair contributes zero
-                if (tid == li) {
-                    X[xoff<LANES>(si, 0, li)] = far_point<Real>();
-                    X[xoff<LANES>(si, 1, li)] = far_point<Real>();
-                    X[xoff<LANES>(si, 2, li)] = far_point<Real>();
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
-                Real sn = 0, so = 0;  // FP64: the two sums; binary32: sum of differences in sn
-                const Real *Xl = X + tid;
-                // binary32: trial distances from the old ones (dist2_pair_f32)
-                float m2dx = 0.0f, m2dy = 0.0f, m2dz = 0.0f, ddd = 0.0f;
-                if constexpr (F32) {
-                    const float dx = (float)tx - (float)xi, dy = (float)ty - (float)yi, dz = (float)tz - (float)zi;
-                    m2dx = -2.0f * dx;
-                    m2dy = -2.0f * dy;
-                    m2dz = -2.0f * dz;
-                    ddd = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                }
-#pragma unroll 2
-                for (int s = 0; s < A; ++s) {
-                    const Real xj = Xl[xoff<LANES>(s, 0, 0)];
-                    const Real yj = Xl[xoff<LANES>(s, 1, 0)];
-                    const Real zj = Xl[xoff<LANES>(s, 2, 0)];
-                    Real r2n, r2o;
-                    if constexpr (F32) {
-                        float fn, fo;
-                        dist2_pair_f32((float)xj, (float)yj, (float)zj, (float)xi, (float)yi, (float)zi, m2dx, m2dy, m2dz,
-                                       ddd, fn, fo);
-                        r2n = (Real)fn;
-                        r2o = (Real)fo;
-                    } else {
-                        r2n = box_d2<Real, BC_SPHERE, false>(tx, ty, tz, xj, yj, zj, BoxC<Real>{});
-                        r2o = box_d2<Real, BC_SPHERE, false>(xi, yi, zi, xj, yj, zj, BoxC<Real>{});
-                    }
-                    if constexpr (F32 && PK == PK_LJ) {
-                        // e(u) = u^3 (c12 u^3 + c6): e_new - e_old = (a - b)(c12 (a + b) + c6),
-                        // a = u_new^3, b = u_old^3; one reciprocal serves both pairs
-                        const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
-                        const float un = ip * r2o, uo = ip * r2n;
-                        const float an = un * un * un, bo = uo * uo * uo;
-                        sn = __builtin_fmaf(an - bo, __builtin_fmaf(c[1], an + bo, c[0]), sn);
-                    } else if constexpr (F32 && PK == PK_EVEN_H) {
-                        const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
-                        sn += pair_energy_u<Real, PK, KC>(ip * r2o, c) - pair_energy_u<Real, PK, KC>(ip * r2n, c);
+                bool accept = false;
+                double dE_used = 0.0;
+                if (i < (uint32_t)N) {
+                    const int si = (int)(i >> kls), li = gb + (int)(i & (uint32_t)(KL - 1));
+                    Real xi = X[xoff<LANES>(si, 0, li)];
+                    Real yi = X[xoff<LANES>(si, 1, li)];
+                    Real zi = X[xoff<LANES>(si, 2, li)];
+                    Real tx, ty, tz;
+                    if constexpr (PBC) {
+                        box_trial<Real, BC, false>(xi, yi, zi, d0, delta_r, bxr, tx, ty, tz);
                     } else if constexpr (F32) {
-                        sn += pair_energy<Real, PK, KC>(r2n, c) - pair_energy<Real, PK, KC>(r2o, c);
+                        tx = xi + __builtin_fmaf((float)(cur.y >> 8), dsc, dof);
+                        ty = yi + __builtin_fmaf((float)(cur.z >> 8), dsc, dof);
+                        tz = zi + __builtin_fmaf((float)(cur.w >> 8), dsc, dof);
                     } else {
-                        sn += pair_energy<Real, PK, KC>(r2n, c);
-                        so += pair_energy<Real, PK, KC>(r2o, c);
+                        make_trial<Real, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
+                    }
+                    if constexpr (!PBC) {
+                        // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
+                        if (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                            if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, si, li, radius2, delta_r,
+                                                               a.seed, run_key, (uint32_t)t, sweep, m, nsteps, xi, yi,
+                                                               zi, tx, ty, tz) &&
+                                q == 0)
+                                a.run_error[a.slot_run[slot]] = 1;
+                        }
+                        // park the moved atom so its self pair contributes zero
+                        if (tid == li) {
+                            X[xoff<LANES>(si, 0, li)] = far_point<Real>();
+                            X[xoff<LANES>(si, 1, li)] = far_point<Real>();
+                            X[xoff<LANES>(si, 2, li)] = far_point<Real>();
+                        }
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    }
+                    // dE = sum_j e(r'_ij) - sum_j e(r_ij), src/EnergyEvaluation.jl:804-821
+                    Real sn = 0, so = 0;  // FP64: the two sums; binary32: sum of differences in sn
+                    const Real *Xl = X + tid;
+                    // binary32 sphere: trial distances from the old ones (dist2_pair_f32)
+                    float m2dx = 0.0f, m2dy = 0.0f, m2dz = 0.0f, ddd = 0.0f;
+                    if constexpr (F32 && !PBC) {
+                        const float dx = (float)tx - (float)xi, dy = (float)ty - (float)yi, dz = (float)tz - (float)zi;
+                        m2dx = -2.0f * dx;
+                        m2dy = -2.0f * dy;
+                        m2dz = -2.0f * dz;
+                        ddd = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    }
+#pragma unroll 2
+                    for (int s = 0; s < A; ++s) {
+                        const Real xj = Xl[xoff<LANES>(s, 0, 0)];
+                        const Real yj = Xl[xoff<LANES>(s, 1, 0)];
+                        const Real zj = Xl[xoff<LANES>(s, 2, 0)];
+                        if constexpr (!PBC) {
+                            Real r2n, r2o;
+                            if constexpr (F32) {
+                                float fn, fo;
+                                dist2_pair_f32((float)xj, (float)yj, (float)zj, (float)xi, (float)yi, (float)zi, m2dx,
+                                               m2dy, m2dz, ddd, fn, fo);
+                                r2n = (Real)fn;
+                                r2o = (Real)fo;
+                            } else {
+                                r2n = box_d2<Real, BC_SPHERE, false>(tx, ty, tz, xj, yj, zj, BoxC<Real>{});
+                                r2o = box_d2<Real, BC_SPHERE, false>(xi, yi, zi, xj, yj, zj, BoxC<Real>{});
+                            }
+                            if constexpr (F32 && PK == PK_LJ) {
+                                // e(u) = u^3 (c12 u^3 + c6): e_new - e_old = (a - b)(c12 (a + b) + c6),
+                                // a = u_new^3, b = u_old^3; one reciprocal serves both pairs
+                                const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                                const float un = ip * r2o, uo = ip * r2n;
+                                const float an = un * un * un, bo = uo * uo * uo;
+                                sn = __builtin_fmaf(an - bo, __builtin_fmaf(c[1], an + bo, c[0]), sn);
+                            } else if constexpr (F32 && PK == PK_EVEN_H) {
+                                const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                                sn += pair_energy_u<Real, PK, KC>(ip * r2o, c) - pair_energy_u<Real, PK, KC>(ip * r2n, c);
+                            } else if constexpr (F32) {
+                                sn += pair_energy<Real, PK, KC>(r2n, c) - pair_energy<Real, PK, KC>(r2o, c);
+                            } else {
+                                sn += pair_energy<Real, PK, KC>(r2n, c);
+                                so += pair_energy<Real, PK, KC>(r2o, c);
+                            }
+                        } else {
+                            // minimum image; the moved atom and padding atoms are masked, NPT
+                            // keeps pairs inside the cut-off (src/EnergyEvaluation.jl:323-336)
+                            const Real r2n = box_d2<Real, BC, false>(tx, ty, tz, xj, yj, zj, bxr);
+                            const Real r2o = box_d2<Real, BC, false>(xi, yi, zi, xj, yj, zj, bxr);
+                            Real en, eo;
+                            if constexpr (F32 && (PK == PK_LJ || PK == PK_EVEN_H)) {
+                                const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
+                                en = pair_energy_u<Real, PK, KC>(ip * r2o, c);
+                                eo = pair_energy_u<Real, PK, KC>(ip * r2n, c);
+                            } else {
+                                en = pair_energy<Real, PK, KC>(r2n, c);
+                                eo = pair_energy<Real, PK, KC>(r2o, c);
+                            }
+                            const int j = (s << kls) + q;
+                            const bool ok = (j < N) & (j != (int)i);
+                            bool okn = ok, oko = ok;
+                            if constexpr (ENS == ENS_NPT) {
+                                okn = ok & (r2n <= rc);
+                                oko = ok & (r2o <= rc);
+                            }
+                            if constexpr (F32) {
+                                sn += (okn ? en : Real(0)) - (oko ? eo : Real(0));
+                            } else {
+                                sn += okn ? en : Real(0);
+                                so += oko ? eo : Real(0);
+                            }
+                        }
+                    }
+                    if constexpr (F32) {
+                        // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
+                        const float dEf = group_sum_rt((float)sn, KL);
+                        dE_used = (double)dEf;
+                        const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
+                        accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                        if (accept) {
+                            E = E + (double)dEf;
+                            nacc += 1;
+                        }
+                    } else {
+                        sn = group_sum_rt(sn, KL);
+                        so = group_sum_rt(so, KL);
+                        const double new_en = (sn - so) + E;
+                        const double dd = new_en - E;
+                        dE_used = dd;
+                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                        if (accept) {
+                            E = new_en;
+                            nacc += 1;
+                        }
+                    }
+                    // commit (src/swap_config.jl:40-51) or restore the parked atom
+                    if (tid == li) {
+                        X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
+                        X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
+                        X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
+                    }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                }
+                if constexpr (ENS == ENS_NPT) {
+                    // uniform volume move: r^2 of the current positions scaled by s^2
+                    const bool vm = i >= (uint32_t)N;
+                    const int T = tb + t;
+                    double sx = 1.0;
+                    if (vm) {
+                        const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * T + 1];
+                        double scale = det_exp(y / 3.0);
+                        if (box >= a.t_maxbox[T] && scale > 1.0) scale = 1.0;
+                        sx = scale;
+                    }
+                    const double nbox = box * sx, nh = boxh * sx;
+                    const double nrc = box_rcut(kind, nbox, nh);
+                    const double m2 = sx * sx;
+                    double new_en = 0.0;
+                    if constexpr (F32) {
+                        // the clusters of a wave evaluate each pending volume move together
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        const uint64_t live = __ballot(1);
+                        const uint64_t leaders = __ballot(q == 0);
+                        uint64_t todo = __ballot(vm && q == 0);
+                        while (todo != 0) {
+                            const int cl = __builtin_ctzll(todo);
+                            todo &= todo - 1;
+                            const double e = wave_pair_sum64_rt<PK, KC, BC, LANES>(
+                                X, cl, N, KL, kls, make_box<double>(readlane_d(box, cl), readlane_d(boxh, cl)),
+                                readlane_d(m2, cl), readlane_d(nrc, cl), c64, live, leaders);
+                            if ((gb & 63) == cl) new_en = e;
+                        }
+                    } else {
+                        if (vm)
+                            new_en = spread_pair_sum64<Real, PK, KC, BC, ENS, LANES>(
+                                X, gb, q, N, KL, kls, make_box<double>(box, boxh), 1.0, 1.0, m2, nrc, c64);
+                    }
+                    if (vm) {
+                        new_en = new_en + box_lrc<PK, KC>(kind, N, nrc, nbox, nh, c64);
+                        const double pressure = ri.pressure;
+                        const double dd = new_en - E;
+                        dE_used = dd;
+                        const double vn = box_volume(kind, nbox, nh), vo = box_volume(kind, box, boxh);
+                        const double dh = dd + pressure * (vn - vo);
+                        const double arg = -dh * beta + (double)N * det_log(vn / vo);
+                        double prob = det_exp(arg);
+                        if (prob > 1.0) prob = 1.0;
+                        accept = prob >= (double)d0.umet;
+                        if (accept) {
+                            for (int s = 0; s < A; ++s) {
+                                Real *pxp = &X[xoff<LANES>(s, 0, tid)];
+                                Real *pyp = &X[xoff<LANES>(s, 1, tid)];
+                                Real *pzp = &X[xoff<LANES>(s, 2, tid)];
+                                *pxp = (Real)((double)*pxp * sx);
+                                *pyp = (Real)((double)*pyp * sx);
+                                *pzp = (Real)((double)*pzp * sx);
+                            }
+                            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                            box = nbox;
+                            boxh = nh;
+                            E = new_en;
+                            nvacc += 1;
+                            bxr = make_box<Real>(box, boxh);
+                            rc = (Real)box_rcut(kind, box, boxh);
+                        }
                     }
                 }
-                bool accept;
-                double dE_used;
-                if constexpr (F32) {
-                    // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
-                    const float dEf = group_sum_rt((float)sn, KL);
-                    dE_used = (double)dEf;
-                    const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
-                    accept = (dEf <= 0.0f) || (pr >= d0.umet);
-                    if (accept) {
-                        E = E + (double)dEf;
-                        nacc += 1;
-                    }
-                } else {
-                    sn = group_sum_rt(sn, KL);
-                    so = group_sum_rt(so, KL);
-                    const double new_en = (sn - so) + E;
-                    const double dd = new_en - E;
-                    dE_used = dd;
-                    accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
-                    if (accept) {
-                        E = new_en;
-                        nacc += 1;
-                    }
-                }
-                // commit (src/swap_config.jl:40-51) or restore the parked atom
-                if (tid == li) {
-                    X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
-                    X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
-                    X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 if (a.trace_de != nullptr && q == 0 && a.trace_cycle < a.trace_cycles) {
                     const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle) * nsteps + m;
                     a.trace_de[ti] = dE_used;
@@ -311,9 +486,19 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             if (q == 0) {
                 a.t_count_atom[2 * (tb + t)] += nacc;
                 a.t_count_atom[2 * (tb + t) + 1] += nacc;
+                if (ENS == ENS_NPT) {
+                    a.t_count_vol[2 * (tb + t)] += nvacc;
+                    a.t_count_vol[2 * (tb + t) + 1] += nvacc;
+                }
             }
         }
-        if (q == 0) a.slot_en[slot] = E;
+        if (q == 0) {
+            a.slot_en[slot] = E;
+            if constexpr (PBC) {
+                a.slot_box[slot] = box;
+                a.slot_boxh[slot] = boxh;
+            }
+        }
     }
     __syncthreads();
     if (a.n_cycles > 0)
@@ -321,10 +506,12 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
 }
 
 // initialise_energy (src/EnergyEvaluation.jl:1662-1692) for the spread
-// layout: FP64 total energy of each configuration of the tile.
-template <typename Real, int PK, int KC, int LANES>
+// layout: FP64 total energy of each configuration of the tile (with the
+// box's cut-off and tail correction in NPT).
+template <typename Real, int PK, int KC, int BC, int ENS, int LANES>
 __global__ void __launch_bounds__(LANES) spread_init_kernel(SweepArgs a)
 {
+    constexpr bool PBC = BC != BC_SPHERE;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const TileInfo tile = a.tiles[blockIdx.x];
     const int KL = tile.kl;
@@ -344,22 +531,31 @@ __global__ void __launch_bounds__(LANES) spread_init_kernel(SweepArgs a)
     double c64[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) c64[k] = k < a.pot.ncoeff ? a.pot.coeff[k] : 0.0;
-    const double E = sphere_pair_sum64<Real, PK, KC, LANES>(X, lg << kls, q, N, KL, kls, c64);
+    const int kind = PBC ? a.box_kind : (int)BC_SPHERE;
+    const double box = PBC ? a.slot_box[slot] : 1.0, boxh = PBC ? a.slot_boxh[slot] : 1.0;
+    const double rc64 = PBC ? box_rcut(kind, box, boxh) : 0.0;
+    double E = spread_pair_sum64<Real, PK, KC, BC, ENS, LANES>(X, lg << kls, q, N, KL, kls,
+                                                              make_box<double>(box, boxh), 1.0, 1.0, 1.0, rc64, c64);
+    if constexpr (ENS == ENS_NPT) E = E + box_lrc<PK, KC>(kind, N, rc64, box, boxh, c64);
     if (q == 0) a.slot_en[slot] = E;
 }
 
 // Exchange, step-size adaptation and sampling of one cycle of every ladder
 // (src/MCRun.jl:203-212, :248-250), one 64-lane workgroup per run, after the
-// cycle's moves.  The ladder's energies and slot map are staged in LDS; the
-// even/odd pairs of a cycle are disjoint, so lanes take them in parallel.
-// Draws, tests and counters are those of the exchange in sweep_kernel
-// (src/Exchange.jl:103-155, :198-263; src/Sampling.jl:29-66, :258-280).
+// cycle's moves.  The ladder's energies (NPT: and volumes) and slot map are
+// staged in LDS; the even/odd pairs of a cycle are disjoint, so lanes take
+// them in parallel.  Draws, tests and counters are those of the exchange in
+// sweep_kernel (src/Exchange.jl:103-155, :198-263; src/Sampling.jl:29-125,
+// :258-280).
 __global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
 {
     __shared__ double s_e[SPREAD_MAX_TEMPS];
+    __shared__ double s_v[SPREAD_MAX_TEMPS];
     __shared__ int32_t s_slot[SPREAD_MAX_TEMPS];
     const RunInfo ri = a.runs[a.run_list != nullptr ? a.run_list[blockIdx.x] : (int)blockIdx.x];
     const int R = ri.n_temps, tb = ri.temp_base, N = ri.n_atoms;
+    const bool npt = a.ensemble == ENS_NPT;
+    const int kind = a.box_kind;
     const int lane = threadIdx.x;
     const uint32_t sweep = (uint32_t)a.sweep0;
     const long long pidx = a.cycle0 + 1;
@@ -367,12 +563,19 @@ __global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
         const int s = a.temp_slot[tb + t];
         s_slot[t] = s;
         s_e[t] = a.slot_en[s];
+        s_v[t] = npt ? box_volume(kind, a.slot_box[s], a.slot_boxh[s]) : 0.0;
     }
     __syncthreads();
     auto try_pair = [&](int n) {
         const u32x4 X1 = draw(a.seed, ri.run_key, RNG_EXCH, 0u, 0u, sweep, 1u + (uint32_t)n);
         const double u = u53(X1.x, X1.y);
-        const double de = s_e[n] - s_e[n + 1];
+        double e1 = s_e[n], e2 = s_e[n + 1];
+        if (npt) {
+            // enthalpies E + PV (src/Exchange.jl:115-124)
+            e1 = e1 + ri.pressure * s_v[n];
+            e2 = e2 + ri.pressure * s_v[n + 1];
+        }
+        const double de = e1 - e2;
         const double db = a.t_beta[tb + n] - a.t_beta[tb + n + 1];
         double acc = det_exp(db * de);
         if (acc > 1.0) acc = 1.0;
@@ -387,6 +590,9 @@ __global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
             const double e0 = s_e[n];
             s_e[n] = s_e[n + 1];
             s_e[n + 1] = e0;
+            const double v0 = s_v[n];
+            s_v[n] = s_v[n + 1];
+            s_v[n + 1] = v0;
         }
     };
     if (R > 1) {
@@ -410,15 +616,33 @@ __global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
             if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
             else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
             a.t_count_atom[2 * T + 1] = 0;
+            if (npt) {
+                const double av = (double)a.t_count_vol[2 * T + 1] / (double)a.n_adjust;
+                if (av < a.min_acc) a.t_displ[2 * T + 1] *= 0.9;
+                else if (av > a.max_acc) a.t_displ[2 * T + 1] *= 1.1;
+                a.t_count_vol[2 * T + 1] = 0;
+            }
         }
-        // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
+        // sampling_step!, src/Sampling.jl:29-125, :258-280, :330-371
         if (a.sampling && pidx % a.mc_sample == 0) {
             const double E = s_e[t];
-            a.t_ham[2 * T] += E;
-            a.t_ham[2 * T + 1] += E * E;
-            const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
-            const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
-            a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+            if (npt) {
+                const double box = a.slot_box[s], boxh = a.slot_boxh[s];
+                const double h = E + ri.pressure * box_volume(kind, box, boxh);
+                a.t_ham[2 * T] += h;
+                a.t_ham[2 * T + 1] += h * h;
+                const double ie = __builtin_floor((E - ri.en_min) / ri.delta_en) + 1.0;
+                const double iv = __builtin_floor((box_hist_volume(kind, box, boxh) - ri.v_min) / ri.delta_v) + 1.0;
+                const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
+                const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
+                a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)] += 1u;
+            } else {
+                a.t_ham[2 * T] += E;
+                a.t_ham[2 * T + 1] += E * E;
+                const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
+                const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
+                a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+            }
         }
     }
 }

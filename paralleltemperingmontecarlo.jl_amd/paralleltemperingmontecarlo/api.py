@@ -546,8 +546,10 @@ class EngineOptions:
     cycle) or "even_odd".  pair_form: "horner" or "reference"
     (the reference's division loop, FP64 only).  workgroup_lanes: 0 keeps
     every ladder in one workgroup; 64, 128 or 256 spreads ladders over
-    workgroups of that width (one cycle per launch; pair potentials in a
-    spherical container, NVT), with the same results; spread_streams > 1
+    workgroups of that width (one cycle per launch; pair potentials in the
+    Horner form in a spherical container in NVT, or in a cubic or
+    rectangular box in NVT or NPT with uniform volume moves), with the same
+    results; spread_streams > 1
     cuts the runs into that many groups launched on their own streams."""
 
     precision: str = "fp64"

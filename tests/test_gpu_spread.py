@@ -189,15 +189,146 @@ def test_binary32_spread_replay(oracle, n_atoms, lanes, n_temps):
     eng.close()
 
 
+P_ATM = 101325 / 2.9421015697e13
+
+
+def _neon_box(kind, n_cells, seed, ne13):
+    """Jittered fcc neon at the density of scripts/pbc32-short.jl: a cubic box
+    of n_cells^3 cells, or a rectangular one whose height is 1.25 times its
+    side (the lattice stretched along z)."""
+    a = 8.7674 * ne13["AtoBohr"] / 2
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.array([[i, j, k] for i in range(n_cells) for j in range(n_cells) for k in range(n_cells)])
+    x = ((cells[:, None, :] + base[None, :, :]).reshape(-1, 3) + 0.25) * a
+    l = n_cells * a
+    h = l if kind == "cubic" else 1.25 * l
+    x[:, 2] *= h / l
+    x = x + np.random.default_rng(seed).uniform(-0.05, 0.05, x.shape)
+    bc = pt.CubicBC(l) if kind == "cubic" else pt.RectangularBC(l, h)
+    return x, l, h, bc
+
+
+def _run_box_layout(kind, ens, wg, precision, lanes, ne13, n_temps=48, n_cells=2, cycles=40, recompute=0):
+    x, l, h, bc = _neon_box(kind, n_cells, 3, ne13)
+    N = len(x)
+    p = P_ATM if ens == "npt" else 0.0
+    ensemble = pt.NPT(N, P_ATM) if ens == "npt" else pt.NVT(N)
+    mc = pt.MCParams(cycles, n_temps, N, n_adjust=10, mc_sample=1, n_bin=40)
+    beta = pt.TempGrid(10.0, 40.0, n_temps).beta_grid
+    eng = PTEngine(pt.ELJPotentialEven(ne13["coeff"]), ensemble, bc, mc,
+                   pt.EngineOptions(precision=precision, exchange="even_odd", seed=17, workgroup_lanes=wg,
+                                    recompute_every=recompute))
+    runs = [eng.add_run(N, beta, x[None], bc, pressure=p, run_key=k, lanes=lanes) for k in range(2)]
+    eng.prepare()
+    assert all(eng.lanes_of(r) == lanes for r in runs)
+    eng.start_phase(False)
+    eng.cycles(cycles // 2)
+    eng.sync()
+    for r in runs:
+        e = eng.energies(r)
+        _, _, box = eng.state(r)
+        vol = box[:, 0] ** 2 * box[:, 1] if kind == "rect" else box ** 3
+        eng.set_histogram(r, float(e.min()), float(e.max()), float(vol.min()) * 0.98, float(vol.max()) * 1.02)
+    eng.reset_counters()
+    eng.start_phase(True)
+    eng.cycles(cycles - cycles // 2)
+    eng.sync()
+    out = [(eng.state(r), eng.counters(r), eng.histograms(r), eng.drift(r)) for r in runs]
+    plan = eng.plan_info()
+    eng.close()
+    return out, plan
+
+
+@pytest.mark.parametrize("kind", ["cubic", "rect"])
+@pytest.mark.parametrize("ens", ["nvt", "npt"])
+@pytest.mark.parametrize("precision,n_temps,lanes,wg", [("fp64", 32, 8, 64), ("fp32", 48, 16, 64),
+                                                         ("fp32", 48, 8, 128)])
+def test_box_spread_equals_one_workgroup_layout(ne13, kind, ens, precision, n_temps, lanes, wg):
+    """Two ladders of periodic neon (32 atoms, cubic or rectangular box; 48
+    temperatures in binary32, 32 in FP64 so that the ladder fits one 256-lane
+    FP64 workgroup) with minimum image; NPT adds the uniform volume move, the
+    cut-off and tail correction, volume-step adaptation and (E, V) histograms.
+    Every tile is full, so the clusters that evaluate a binary32 volume move
+    together are the same in both layouts: positions, energies, boxes,
+    counters, histograms and recompute drifts agree bit for bit."""
+    print(f"box spread {kind} {ens} {precision} R={n_temps} lanes={lanes} wg={wg}", flush=True)
+    rec = 10 if precision == "fp32" else 0
+    a, _ = _run_box_layout(kind, ens, 0, precision, lanes, ne13, n_temps=n_temps, recompute=rec)
+    b, plan = _run_box_layout(kind, ens, wg, precision, lanes, ne13, n_temps=n_temps, recompute=rec)
+    assert plan["tiles"] == 2 * n_temps * lanes // wg
+    _same(a, b)
+    for (_, ctr, hist, _) in b:
+        assert ctr["count_atom"][:, 0].sum() > 0
+        assert hist.sum() > 0
+        if ens == "npt":
+            assert ctr["count_vol"][:, 0].sum() > 0
+
+
+@pytest.mark.parametrize("kind", ["cubic", "rect"])
+@pytest.mark.parametrize("ens", ["nvt", "npt"])
+def test_fp64_box_spread_bit_exact_against_oracle(oracle, ne13, kind, ens):
+    """One 24-temperature FP64 ladder of periodic neon at 4 lanes per
+    configuration on 64-lane tiles (the ladder cut into a full and a half
+    tile) against the oracle: every move's index, decision and
+    dE (volume moves included), then positions, energies, box lengths and
+    heights, step sizes and counters bit for bit."""
+    x, l, h, bc = _neon_box(kind, 2, 5, ne13)
+    N, R = len(x), 24
+    p = P_ATM if ens == "npt" else 0.0
+    ensemble = pt.NPT(N, P_ATM) if ens == "npt" else pt.NVT(N)
+    temp = pt.TempGrid(10.0, 40.0, R)
+    mc = pt.MCParams(100, R, N, n_adjust=10)
+    eng = PTEngine(pt.ELJPotentialEven(ne13["coeff"]), ensemble, bc, mc,
+                   pt.EngineOptions(precision="fp64", exchange="even_odd", seed=61, workgroup_lanes=64,
+                                    lanes_per_replica=4))
+    run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=p, run_key=2)
+    eng.prepare()
+    assert eng.plan_info()["tiles"] == 2 and eng.lanes_of(run) == 4
+    print(f"fp64 box spread vs oracle {kind} {ens}", flush=True)
+    orc = oracle.OracleRun(N, temp.beta_grid, x, pot=oracle.make_pot("elj_even", ne13["coeff"]), bc=kind, box=l,
+                           box_h=h, ensemble=ens, pressure=p, mode=oracle.engine_mode(4), exchange_mode=2,
+                           n_adjust=10, seed=61, run_key=2)
+    assert np.array_equal(eng.energies(run), orc.en_tot)
+    eng.set_trace(25)
+    eng.start_phase(False)
+    eng.cycles(25)
+    eng.sync()
+    de, acc, idx = eng.trace(run, 25)
+    orc.start_phase(False)
+    tr = orc.cycles(25, trace=True)
+    assert np.array_equal(idx, tr["idx"])
+    assert np.array_equal(acc, tr["acc"])
+    assert np.array_equal(de, tr["de"])
+    eng.cycles(45)
+    eng.sync()
+    orc.cycles(45)
+    xd, en, box = eng.state(run)
+    ctr = eng.counters(run)
+    assert np.array_equal(xd, orc.x.reshape(xd.shape))
+    assert np.array_equal(en, orc.en_tot)
+    if kind == "rect":
+        assert np.array_equal(box[:, 0], orc.box) and np.array_equal(box[:, 1], orc.box_h)
+    else:
+        assert np.array_equal(box, orc.box)
+    for k in ("max_displ", "count_atom", "count_vol", "count_exc"):
+        assert np.array_equal(ctr[k], getattr(orc, k)), k
+    if ens == "npt":
+        assert orc.count_vol[:, 0].sum() > 0
+    eng.close()
+
+
 def test_spread_arguments():
-    """Tile widths other than 0, 64, 128, 256, and potentials or boundaries the
-    spread layout does not cover, are rejected at prepare()."""
+    """Tile widths other than 0, 64, 128, 256, and potentials, boundaries or
+    ensembles the spread layout does not cover (rhombic boxes, separated
+    volume moves), are rejected at prepare()."""
     x, rad, beta = _lj_runs(13, 4)
-    for wg, bc, pot in [(96, pt.SphericalBC(rad), pt.ELJPotentialEven(LJ_COEFF)),
-                        (64, pt.CubicBC(10.0), pt.ELJPotentialEven(LJ_COEFF))]:
-        eng = PTEngine(pot, pt.NVT(13), bc, pt.MCParams(10, 4, 13),
+    for wg, bc, pot, ens in [(96, pt.SphericalBC(rad), pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13)),
+                             (64, pt.RhombicBC(10.0, 10.0), pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13)),
+                             (64, pt.RectangularBC(10.0, 12.0), pt.ELJPotentialEven(LJ_COEFF),
+                              pt.NPT(13, P_ATM, separated_volume=True))]:
+        eng = PTEngine(pot, ens, bc, pt.MCParams(10, 4, 13),
                        pt.EngineOptions(precision="fp32", workgroup_lanes=wg))
-        eng.add_run(13, beta, x[None], bc)
+        eng.add_run(13, beta, x[None], bc, pressure=P_ATM if isinstance(ens, pt.NPT) else 0.0)
         with pytest.raises(Exception):
             eng.prepare()
         eng.close()
