@@ -239,10 +239,11 @@ def _cpu_rate(make_run, moves_per_cycle, target_s):
                 cpu_kind="port", cpu_one_core_moves_per_s=moves_per_cycle * max(1, ncyc // 4) / dt1)
 
 
-def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
+def bench_ne13(pt, n_ladders, warmup, steps, cpu_s, layout=None):
     """Ne13 extended-LJ cluster in FP64: icosahedral start, 5.32 A container,
     32 temperatures geometric 4-16 K, exchange every cycle; lanes per
-    configuration tuned on the workload."""
+    configuration tuned on the workload, or layout = (workgroup_lanes,
+    lanes per configuration)."""
     from paralleltemperingmontecarlo.autotune import tune_lanes
 
     d = json.loads(NE13_SETUP.read_text())
@@ -253,10 +254,14 @@ def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
 
     def make():
         return pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(13), pt.SphericalBC(rad), mc,
-                           pt.EngineOptions(precision="fp64", exchange="even_odd", seed=1234, recompute_every=1000))
+                           pt.EngineOptions(precision="fp64", exchange="even_odd", seed=1234, recompute_every=1000,
+                                            workgroup_lanes=0 if layout is None else layout[0]))
 
     runs = [dict(n_atoms=13, beta=temp.beta_grid, x0=pos, bc=pt.SphericalBC(rad), key=r) for r in range(n_ladders)]
-    table = tune_lanes(make, runs, cycles=max(500, 20000 // n_ladders), adapt_cycles=500)
+    if layout is None:
+        table = tune_lanes(make, runs, cycles=max(500, 20000 // n_ladders), adapt_cycles=500)
+    else:
+        table = {13: layout[1]}
     eng = make()
     for r in range(n_ladders):
         eng.add_run(13, temp.beta_grid, pos[None], pt.SphericalBC(rad), run_key=r, lanes=table.get(13, 0))
@@ -264,6 +269,7 @@ def bench_ne13(pt, n_ladders, warmup, steps, cpu_s):
     dt = _production(eng, n_ladders, warmup, steps)
     out = dict(workload=f"Ne13 ELJ FP64, {n_ladders} ladder(s) x 32 temperatures 4-16 K, sphere 5.32 A",
                moves_per_s=n_ladders * 32 * 13 * steps / dt, steps=steps, dtype="f64",
+               workgroup_lanes=0 if layout is None else layout[0], workgroups=eng.plan_info()["tiles"],
                lanes_per_configuration=eng.lanes_of(0),
                acceptance=_acceptance(eng, n_ladders, 13, steps),
                max_drift=max(float(np.max(eng.drift(i))) for i in range(n_ladders)))
@@ -295,10 +301,12 @@ def _bulk_start(n_atoms, vol_per_atom):
     return np.array(pts) - L / 2 + a / 4, L
 
 
-def bench_bulk(pt, n_atoms, ensemble, warmup, steps, cpu_s=0.0):
+def bench_bulk(pt, n_atoms, ensemble, warmup, steps, cpu_s=0.0, layout=(0, 0)):
     """Periodic neon (cubic box, minimum image) at the density of a 32-atom
     fcc cell of edge 8.7674 A, 48 temperatures 10-40 K, binary32 pair sums
-    with FP64 totals; NPT adds one isotropic volume move per cycle at 1 atm."""
+    with FP64 totals; NPT adds one isotropic volume move per cycle at 1 atm.
+    layout = (workgroup_lanes, lanes per configuration); (0, 0) is one
+    workgroup for the ladder with the engine's default lanes."""
     d = json.loads(NE13_SETUP.read_text())
     coeff = np.array(d["coeff"])
     x, L = _bulk_start(n_atoms, (8.7674 * A2B) ** 3 / 32)
@@ -307,12 +315,16 @@ def bench_bulk(pt, n_atoms, ensemble, warmup, steps, cpu_s=0.0):
     ens = pt.NPT(n_atoms, P_ATM) if ensemble == "npt" else pt.NVT(n_atoms)
     bc = pt.CubicBC(L)
     eng = pt.PTEngine(pt.ELJPotentialEven(coeff), ens, bc, mc,
-                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000))
-    eng.add_run(n_atoms, temp.beta_grid, x[None], bc, pressure=P_ATM if ensemble == "npt" else 0.0, run_key=0)
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
+                                       workgroup_lanes=layout[0]))
+    eng.add_run(n_atoms, temp.beta_grid, x[None], bc, pressure=P_ATM if ensemble == "npt" else 0.0, run_key=0,
+                lanes=layout[1])
     eng.prepare()
     dt = _production(eng, 1, warmup, steps)
     out = dict(workload=f"bulk Ne{n_atoms} {ensemble.upper()} cubic box, 48 temperatures 10-40 K",
                moves_per_s=48 * n_atoms * steps / dt, steps=steps, dtype="fp32",
+               workgroup_lanes=layout[0], lanes_per_configuration=eng.lanes_of(0),
+               workgroups=eng.plan_info()["tiles"],
                acceptance=_acceptance(eng, 1, n_atoms, steps), max_fp32_vs_fp64_drift=float(np.max(eng.drift(0))))
     eng.close()
     if cpu_s > 0:
@@ -485,7 +497,10 @@ def other_configs(pt, args, only=None):
     todo = {
         "ne13": lambda: [bench_ne13(pt, 1, 500, 20000, cpu)],
         "ne13_batched": lambda: [bench_ne13(pt, 512, 500, 2000, 0.0)],
-        "bulk": lambda: [bench_bulk(pt, n, ens, 200, 1000, cpu / 6 if n == 108 else 0.0)
+        # one 48-temperature ladder: 64 lanes per configuration on 64-lane
+        # workgroups spreads the ladder over 48 CUs (the fastest of the
+        # one-workgroup layout and 16 / 32 / 64 lanes on 64- and 128-lane tiles)
+        "bulk": lambda: [bench_bulk(pt, n, ens, 600, 1000, cpu / 6 if n == 108 else 0.0, layout=(64, 64))
                          for n in (108, 216, 256) for ens in ("nvt", "npt")],
         "nnp": lambda: [bench_nnp(pt, 1, 50, 50, cpu)],
         "nnp_batched": lambda: [bench_nnp(pt, 64, 50, 50, 0.0)],

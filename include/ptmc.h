@@ -58,7 +58,8 @@ typedef struct {
     int32_t n_bin;             /* MCParams.n_bin (default 100) */
     double min_acc, max_acc;   /* MCParams.min_acc/max_acc (0.4, 0.6) */
     int32_t recompute_every;   /* full FP64 energy recompute every this many cycles (0 = never) */
-    int32_t lanes_per_replica; /* 0: chosen per atom count; else 1, 2, 4, 8 or 16 */
+    int32_t lanes_per_replica; /* 0: chosen per atom count; else 1, 2, 4, 8 or 16 (32 or 64 in the
+                                  spread layout, workgroup_lanes > 0) */
     int32_t cycles_per_launch; /* cycles per kernel launch (0 = 256) */
     int32_t device;            /* HIP device ordinal */
     uint64_t seed;             /* Philox key */

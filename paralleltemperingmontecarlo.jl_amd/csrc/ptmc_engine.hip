@@ -381,8 +381,9 @@ int plan_spread(ptmc_engine *h)
         const int kl = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
                        : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
                                                       : default_lanes(N, prec);
-        if (kl < 1 || kl > 16 || (kl & (kl - 1)) != 0 || kl > W)
-            return fail(h, PTMC_E_ARG, "run %d: lanes per configuration must be 1, 2, 4, 8 or 16 (at most %d)", r, W);
+        if (kl < 1 || kl > 64 || (kl & (kl - 1)) != 0 || kl > W)
+            return fail(h, PTMC_E_ARG, "run %d: lanes per configuration must be a power of two up to 64 (at most %d)",
+                        r, W);
         if (rh.d.n_temps > SPREAD_MAX_TEMPS)
             return fail(h, PTMC_E_UNSUPPORTED, "run %d: at most %d temperatures in the spread layout", r,
                         SPREAD_MAX_TEMPS);
@@ -555,6 +556,9 @@ int plan(ptmc_engine *h)
         const int req = rh.d.lanes_per_replica > 0 ? rh.d.lanes_per_replica
                         : h->cfg.lanes_per_replica > 0 ? h->cfg.lanes_per_replica
                                                        : default_lanes(N, prec);
+        if (req > 16)
+            return fail(h, PTMC_E_UNSUPPORTED, "run %d: %d lanes per configuration need the spread layout "
+                        "(workgroup_lanes > 0)", r, req);
         // smallest built workgroup width that holds the whole ladder; more
         // lanes per configuration when the slab does not fit LDS
         int kl = -1, lanes = 0;
@@ -1454,9 +1458,10 @@ int ptmc_add_run(ptmc_handle h, const ptmc_run_desc *d, int32_t *run_index)
     if ((h->cfg.boundary == BC_RECT || h->cfg.boundary == BC_RHOMB) && !(d->box_height > 0))
         return fail(h, PTMC_E_ARG, "box height must be positive");
     if (d->n_temps > (1 << 20)) return fail(h, PTMC_E_ARG, "too many temperatures");
-    if (d->lanes_per_replica != 0 && d->lanes_per_replica != 1 && d->lanes_per_replica != 2 &&
-        d->lanes_per_replica != 4 && d->lanes_per_replica != 8 && d->lanes_per_replica != 16)
-        return fail(h, PTMC_E_ARG, "lanes_per_replica must be 1, 2, 4, 8 or 16");
+    // 32 and 64 lanes per configuration exist in the spread layout only; the
+    // ladder layout's planner refuses them as unsupported
+    if (d->lanes_per_replica < 0 || d->lanes_per_replica > 64 || (d->lanes_per_replica & (d->lanes_per_replica - 1)))
+        return fail(h, PTMC_E_ARG, "lanes_per_replica must be 0 or a power of two up to 64");
     RunHost rh;
     rh.d = *d;
     rh.beta.assign(d->beta, d->beta + d->n_temps);

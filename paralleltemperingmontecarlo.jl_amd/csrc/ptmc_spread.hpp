@@ -20,8 +20,40 @@
 
 namespace ptmc {
 
+// Value of the lane whose index differs in bit 4 (ROW = 16) or bit 5 (ROW =
+// 32), from gfx950's row swaps: v_permlane16_swap(a, b) exchanges the odd
+// 16-lane rows of a with the even rows of b, v_permlane32_swap(a, b) the upper
+// half of a with the lower half of b.  With a = b = v, the returned a holds
+// the even (lower) rows twice and b the odd (upper) rows twice, so a lane of
+// an even row finds its partner in b and a lane of an odd row in a.
+template <int ROW>
+__device__ __forceinline__ uint32_t row_xor_u32(uint32_t v)
+{
+    const bool odd = (threadIdx.x & ROW) != 0;
+    if constexpr (ROW == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return odd ? r[0] : r[1];
+    } else {
+        static_assert(ROW == 32, "row swap of 16 or 32 lanes");
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return odd ? r[0] : r[1];
+    }
+}
+template <int ROW>
+__device__ __forceinline__ float row_xor(float v) { return __uint_as_float(row_xor_u32<ROW>(__float_as_uint(v))); }
+template <int ROW>
+__device__ __forceinline__ double row_xor(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = row_xor_u32<ROW>((uint32_t)b), hi = row_xor_u32<ROW>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // group_sum for a cluster width kl known at run time (uniform across the
-// workgroup, so the branches are scalar); the same steps as group_sum<KL>.
+// workgroup, so the branches are scalar); the same steps as group_sum<KL>,
+// then for clusters of 32 and 64 lanes the sums of the two 16-lane rows and
+// of the two halves of the wave.  Every step adds the same two operands on
+// both lanes of a pair (an xor butterfly), so all lanes end with equal bits.
 template <typename T>
 __device__ __forceinline__ T group_sum_rt(T v, int kl)
 {
@@ -29,6 +61,8 @@ __device__ __forceinline__ T group_sum_rt(T v, int kl)
     if (kl >= 4) v = v + dpp_perm<0x4E>(v);
     if (kl >= 8) v = v + dpp_perm<0x141>(v);
     if (kl >= 16) v = v + dpp_perm<0x140>(v);
+    if (kl >= 32) v = v + row_xor<16>(v);
+    if (kl >= 64) v = v + row_xor<32>(v);
     return v;
 }
 

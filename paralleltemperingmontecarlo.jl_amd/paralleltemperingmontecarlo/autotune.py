@@ -43,7 +43,8 @@ from typing import Callable
 
 from . import _lib
 
-LANE_CHOICES = (1, 2, 4, 8, 16)
+# 32 and 64 lanes per configuration run in the spread layout only
+LANE_CHOICES = (1, 2, 4, 8, 16, 32, 64)
 # (workgroup_lanes, spread_streams): one workgroup per ladder, or the spread
 # layout at three widths with one to four run groups
 LAYOUTS = ((0, 0), (64, 1), (64, 2), (64, 3), (64, 4), (128, 2), (128, 3), (256, 2), (256, 3))
@@ -143,7 +144,7 @@ def _descend(make_engine, runs, table, best, sizes, choices, warmup, cycles, sta
         improved = False
         for n in sizes:
             for kl in choices:
-                if table.get(n, 0) == kl:
+                if table.get(n, 0) == kl or kl >= 2 * n:  # at least half the lanes of a cluster hold atoms
                     continue
                 trial = dict(table)
                 trial[n] = kl

@@ -166,11 +166,13 @@ def test_fp64_cubic_bit_exact(oracle, ne13, ensemble):
         eng.close()
 
 
-@pytest.mark.parametrize("n_cells,ensemble", [(3, "nvt"), (4, "npt")])
-def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble):
+@pytest.mark.parametrize("n_cells,ensemble,wg", [(3, "nvt", 0), (4, "npt", 0), (3, "nvt", 64), (4, "npt", 64),
+                                                (3, "npt", 128)])
+def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble, wg):
     """A 48-temperature ladder of a periodic neon box (fcc 108 / 256 atoms at
     the scripts/pbc32-short.jl density) in binary32 takes one 768-lane
-    workgroup.  Replaying the device decisions on the oracle reproduces every
+    workgroup, or (spread layout) 64 lanes per configuration on 48 tiles of
+    64 lanes or 24 of 128.  Replaying the device decisions on the oracle reproduces every
     move, every dE to 2e-5 of the atom energy scale, and the positions bit for
     bit (NPT: with the volume moves).  The lattice is jittered by up to 0.05
     bohr: in the perfect crystal many pairs sit exactly at the NPT cut-off
@@ -184,10 +186,15 @@ def test_fp32_bulk_48_temperatures(oracle, ne13, n_cells, ensemble):
     bc = pt.CubicBC(L)
     p = P_ATM if ensemble == "npt" else 0.0
     ens = pt.NPT(N, P_ATM) if ensemble == "npt" else pt.NVT(N)
-    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), ens, bc, mc, precision="fp32", exchange="even_odd", seed=31)
-    run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=p, run_key=4)
+    print(f"fp32 bulk N={N} {ensemble} wg={wg}", flush=True)
+    eng = _engine(pt.ELJPotentialEven(ne13["coeff"]), ens, bc, mc, precision="fp32", exchange="even_odd", seed=31,
+                  workgroup_lanes=wg)
+    run = eng.add_run(N, temp.beta_grid, x[None], bc, pressure=p, run_key=4, lanes=64 if wg else 0)
     eng.prepare()
-    assert eng.group_info(eng.group_of(run))["workgroup_lanes"] == 768
+    if wg:
+        assert eng.lanes_of(run) == 64 and eng.plan_info()["tiles"] == R * 64 // wg
+    else:
+        assert eng.group_info(eng.group_of(run))["workgroup_lanes"] == 768
     ncyc = 6
     eng.set_trace(ncyc)
     eng.start_phase(False)

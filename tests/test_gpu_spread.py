@@ -317,6 +317,119 @@ def test_fp64_box_spread_bit_exact_against_oracle(oracle, ne13, kind, ens):
     eng.close()
 
 
+@pytest.mark.parametrize("kind,ens,lanes,wg", [("sphere", "nvt", 32, 64), ("sphere", "nvt", 64, 128),
+                                               ("cubic", "nvt", 64, 64), ("cubic", "npt", 32, 128),
+                                               ("rect", "npt", 64, 64)])
+def test_fp64_wide_clusters_bit_exact_against_oracle(oracle, ne13, kind, ens, lanes, wg):
+    """32 and 64 lanes per configuration (the partner sums close with the
+    swaps of 16-lane rows and of wave halves): FP64 ladders of a 55-atom LJ
+    cluster in a sphere or of 108 periodic neon atoms (NPT: the volume
+    move's full energy summed by the cluster too) against the oracle's
+    32 / 64-wide butterfly: every move's index, decision and dE, then the
+    final state, bit for bit."""
+    R = 12
+    if kind == "sphere":
+        x, rad, beta = _lj_runs(55, R, seed=lanes)
+        pot, N, p, bc = pt.ELJPotentialEven(LJ_COEFF), 55, 0.0, pt.SphericalBC(rad)
+        ensemble = pt.NVT(N)
+        orc_geom = dict(radius=rad)
+    else:
+        x, l, h, bc = _neon_box(kind, 3, lanes, ne13)
+        N = len(x)
+        beta = pt.TempGrid(10.0, 40.0, R).beta_grid
+        pot = pt.ELJPotentialEven(ne13["coeff"])
+        p = P_ATM if ens == "npt" else 0.0
+        ensemble = pt.NPT(N, P_ATM) if ens == "npt" else pt.NVT(N)
+        orc_geom = dict(bc=kind, box=l, box_h=h, ensemble=ens, pressure=p)
+    mc = pt.MCParams(100, R, N, n_adjust=5)
+    eng = PTEngine(pot, ensemble, bc, mc,
+                   pt.EngineOptions(precision="fp64", exchange="even_odd", seed=73, workgroup_lanes=wg,
+                                    lanes_per_replica=lanes))
+    run = eng.add_run(N, beta, x[None], bc, pressure=p, run_key=6)
+    eng.prepare()
+    assert eng.lanes_of(run) == lanes and eng.plan_info()["tiles"] == R * lanes // wg
+    print(f"fp64 wide clusters {kind} {ens} N={N} lanes={lanes} wg={wg}", flush=True)
+    coeff = LJ_COEFF if kind == "sphere" else ne13["coeff"]
+    orc = oracle.OracleRun(N, beta, x, pot=oracle.make_pot("elj_even", coeff), mode=oracle.engine_mode(lanes),
+                           exchange_mode=2, n_adjust=5, seed=73, run_key=6, **orc_geom)
+    assert np.array_equal(eng.energies(run), orc.en_tot)
+    ncyc = 12
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, trace=True)
+    assert np.array_equal(idx, tr["idx"])
+    assert np.array_equal(acc, tr["acc"])
+    assert np.array_equal(de, tr["de"])
+    assert 0 < acc.mean() < 1
+    eng.cycles(8)
+    eng.sync()
+    orc.cycles(8)
+    xd, en, box = eng.state(run)
+    ctr = eng.counters(run)
+    assert np.array_equal(xd, orc.x.reshape(xd.shape))
+    assert np.array_equal(en, orc.en_tot)
+    if kind == "rect":
+        assert np.array_equal(box[:, 0], orc.box) and np.array_equal(box[:, 1], orc.box_h)
+    elif kind == "cubic":
+        assert np.array_equal(box, orc.box)
+    for k in ("max_displ", "count_atom", "count_exc") + (("count_vol",) if ens == "npt" else ()):
+        assert np.array_equal(ctr[k], getattr(orc, k)), k
+    if ens == "npt":
+        assert orc.count_vol[:, 0].sum() > 0
+    eng.close()
+
+
+@pytest.mark.parametrize("lanes,wg", [(32, 64), (64, 256)])
+def test_binary32_wide_cluster_replay(oracle, lanes, wg):
+    """binary32 LJ98 at 32 and 64 lanes per configuration: the oracle replays
+    the device's decisions; dE within 2e-5 of the interaction scale, every
+    decision equal to the oracle's own FP64 Metropolis decision outside the
+    near-tie band, positions bit for bit."""
+    n_atoms, n_temps = 98, 24
+    x, rad, beta = _lj_runs(n_atoms, n_temps, seed=lanes + 1)
+    mc = pt.MCParams(100, n_temps, n_atoms, n_adjust=20)
+    bc = pt.SphericalBC(radius=rad)
+    eng = PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(n_atoms), bc, mc,
+                   pt.EngineOptions(precision="fp32", exchange="even_odd", seed=12, workgroup_lanes=wg,
+                                    lanes_per_replica=lanes))
+    run = eng.add_run(n_atoms, beta, x[None], bc, run_key=9)
+    eng.prepare()
+    assert eng.lanes_of(run) == lanes and eng.plan_info()["tiles"] == n_temps * lanes // wg
+    ncyc = 12
+    eng.set_trace(ncyc)
+    eng.start_phase(False)
+    eng.cycles(ncyc)
+    eng.sync()
+    de, acc, idx = eng.trace(run, ncyc)
+    orc = oracle.OracleRun(n_atoms, beta, x, pot=oracle.make_pot("elj_even", LJ_COEFF), radius=rad,
+                           mode=oracle.engine_mode(1), fp32=True, exchange_mode=2, n_adjust=20, seed=12, run_key=9)
+    orc.start_phase(False)
+    tr = orc.cycles(ncyc, replay_acc=acc)
+    assert np.array_equal(idx, tr["idx"])
+    eps = 2e-5 * np.maximum(6.0, np.abs(tr["de"]))
+    assert np.all(np.abs(de - tr["de"]) <= eps), np.max(np.abs(de - tr["de"]) / eps)
+    assert_decisions(tr, acc, beta, eps, min_compared=acc.size)
+    x_dev, en_dev, _ = eng.state(run)
+    assert np.array_equal(x_dev, orc.x.reshape(x_dev.shape))
+    eng.close()
+
+
+def test_wide_clusters_need_the_spread_layout():
+    """32 or 64 lanes per configuration in the one-workgroup-per-ladder layout
+    are refused as unsupported at prepare()."""
+    x, rad, beta = _lj_runs(55, 4)
+    eng = PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(55), pt.SphericalBC(rad), pt.MCParams(10, 4, 55),
+                   pt.EngineOptions(precision="fp32"))
+    eng.add_run(55, beta, x[None], pt.SphericalBC(rad), lanes=32)
+    with pytest.raises(Exception):
+        eng.prepare()
+    eng.close()
+
+
 def test_spread_arguments():
     """Tile widths other than 0, 64, 128, 256, and potentials, boundaries or
     ensembles the spread layout does not cover (rhombic boxes, separated
