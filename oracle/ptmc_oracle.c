@@ -10,7 +10,9 @@
  *     restates the Julia code operation by operation;
  *   - engine form (pair_form = 1, dist_cache = 0, lanes = k, math_lib = 0)
  *     restates what the device engine computes (Horner pair energy, distances
- *     recomputed from positions, k-wide butterfly sums, deterministic exp/log)
+ *     recomputed from positions, k-wide butterfly sums, deterministic exp/log,
+ *     and for FP64 pair-potential atom moves the Metropolis test in threshold
+ *     form dE <= -ln(u) / beta, equivalent to min(1, e^{-beta dE}) >= u)
  *     so device results can be compared bit for bit.
  */
 #include "ptmc_oracle.h"
@@ -320,6 +322,16 @@ static double tree_sum(double *v, int k)
 }
 
 static inline int dist_form(const orc_params *p) { return p->pair_form == 0 ? 0 : 1; }
+
+/* The engine's FP64 pair-potential sweeps decide atom moves by the threshold
+ * form of src/Exchange.jl:53-57: accept iff dE <= -ln(u) / beta, which equals
+ * min(1, e^{-beta dE}) >= u up to the rounding of the two sides. */
+static inline int metro_threshold(const orc_params *p, const orc_pot *pot)
+{
+    return !p->math_lib && !p->fp32 &&
+           (pot->kind == ORC_POT_ELJ_EVEN || pot->kind == ORC_POT_ELJ || pot->kind == ORC_POT_ELJB ||
+            pot->kind == ORC_POT_LUT);
+}
 
 /* Full configuration energy, src/EnergyEvaluation.jl:378-474: sum over i<j
  * (NPT: pairs inside r_cut, plus the box's lrc).  en_atom optional.  With
@@ -778,6 +790,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         double prob = xexp(p, -dE * s->beta[r]);
                         if (prob > 1.0) prob = 1.0;
                         accept = prob >= d0.umet;
+                        if (metro_threshold(p, pot)) accept = dE <= -orc_log(d0.umet) / s->beta[r];
                         if (s->trace_prob) s->trace_prob[tix] = prob;
                         if (s->trace_u) s->trace_u[tix] = d0.umet;
                         if (p->plant_decision) {
@@ -836,6 +849,7 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
                         double prob = xexp(p, -dE * s->beta[r]);
                         if (prob > 1.0) prob = 1.0;
                         accept = prob >= d0.umet;
+                        if (metro_threshold(p, pot)) accept = dE <= -orc_log(d0.umet) / s->beta[r];
                         if (s->trace_prob) s->trace_prob[tix] = prob;
                         if (s->trace_u) s->trace_u[tix] = d0.umet;
                         if (p->fp32 && s->replay_acc) accept = s->replay_acc[tix];

@@ -562,7 +562,7 @@ int plan(ptmc_engine *h)
         // smallest built workgroup width that holds the whole ladder; more
         // lanes per configuration when the slab does not fit LDS
         int kl = -1, lanes = 0;
-        for (int w : {LANES_SMALL, LANES_MID, LANES_LARGE}) {
+        for (int w : {LANES_SMALL, LANES_HALF, LANES_MID, LANES_LARGE}) {
             if (eam && w != LANES_SMALL) continue;
             for (int k = (w != LANES_SMALL && req < 2) ? 2 : req; k <= 16 && R * k <= w; k *= 2) {
                 if (tile_lds(rsize, N, k, w, eam) <= (size_t)max_lds &&

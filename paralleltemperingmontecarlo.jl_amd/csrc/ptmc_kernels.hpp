@@ -17,6 +17,7 @@ struct KernelPair {
 
 // Workgroup widths the kernels are built for (lanes = threads per workgroup).
 constexpr int LANES_SMALL = 256;
+constexpr int LANES_HALF = 512;
 constexpr int LANES_MID = 768;
 constexpr int LANES_LARGE = 1024;
 
@@ -69,6 +70,17 @@ KernelPair kl_switch(int kl, int lanes)
             switch (kl) {
             case 8: return PTMC_PAIR(8, 768);
             case 16: return PTMC_PAIR(16, 768);
+            default: return {};
+            }
+        }
+    }
+    if constexpr (sizeof(Real) == 8 && BC == BC_SPHERE && ENS == ENS_NVT) {
+        // a 32-temperature FP64 ladder at 16 lanes per configuration: two
+        // waves per SIMD hide the latency of the moves' dependent FP64 chains
+        if (lanes == 512) {
+            switch (kl) {
+            case 8: return PTMC_PAIR(8, 512);
+            case 16: return PTMC_PAIR(16, 512);
             default: return {};
             }
         }

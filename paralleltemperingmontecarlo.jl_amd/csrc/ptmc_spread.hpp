@@ -288,22 +288,32 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             int nacc = 0, nvacc = 0;
             u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
             u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
+            double mthr = 0.0, nthr = 0.0;  // FP64: Metropolis thresholds of the lane's and the next move
             if (KL > 1) {
-                if ((uint32_t)q < nsteps) move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                if ((uint32_t)q < nsteps) {
+                    move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                    if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                }
                 fetch_words<Real>(mA, mB, gb, nA, nB);
+                if constexpr (!F32) nthr = lane_get_d(mthr, gb);
             }
             for (uint32_t m = 0; m < nsteps; ++m) {
                 // draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
                 u32x4 cur = nA, curB = nB;
+                double thr = nthr;
                 if (KL == 1) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m, cur, curB);
+                    if constexpr (!F32) thr = metro_thr(u53(curB.z, curB.w), beta);
                 } else {
                     const uint32_t m1 = m + 1;
                     if (m1 < nsteps) {
                         const uint32_t k1 = m1 & (uint32_t)(KL - 1);
-                        if (k1 == 0 && m1 + q < nsteps)
+                        if (k1 == 0 && m1 + q < nsteps) {
                             move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m1 + q, mA, mB);
+                            if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                        }
                         fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
+                        if constexpr (!F32) nthr = lane_get_d(mthr, gb + (int)k1);
                     }
                 }
                 const MoveDraw<Real> d0 = decode_move<Real>(cur, curB, nsteps);
@@ -432,7 +442,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         const double new_en = (sn - so) + E;
                         const double dd = new_en - E;
                         dE_used = dd;
-                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                        accept = dd <= thr;
                         if (accept) {
                             E = new_en;
                             nacc += 1;

@@ -803,6 +803,20 @@ __device__ __forceinline__ void move_words(uint64_t seed, uint32_t run, uint32_t
     if constexpr (sizeof(Real) == 8) B = draw(seed, run, RNG_MOVE, 1u, temp, sweep, mm);
 }
 
+// FP64 Metropolis threshold of a move (src/Exchange.jl:53-57, src/MCRun.jl:
+// 122-132): min(1, e^{-beta dE}) >= u  <=>  dE <= -ln(u) / beta (u < 1, beta
+// > 0; u = 0 gives +inf, always accepted).  The lane that draws a move's
+// words also forms its threshold, so the move itself only compares.
+__device__ __forceinline__ double metro_thr(double umet, double beta) { return -det_log(umet) / beta; }
+
+// value of lane `src` of a double
+__device__ __forceinline__ double lane_get_d(double v, int src)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = lane_get((uint32_t)b, src), hi = lane_get((uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 template <typename Real>
 __device__ __forceinline__ void fetch_words(const u32x4 &mA, const u32x4 &mB, int src, u32x4 &A, u32x4 &B)
 {
@@ -913,23 +927,34 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             // the current move runs.
             u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
             u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
+            double mthr = 0.0, nthr = 0.0;  // FP64: Metropolis thresholds of the lane's and the next move
             if constexpr (KL > 1) {
-                if ((uint32_t)q < nsteps) move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                if ((uint32_t)q < nsteps) {
+                    move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
+                    if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                }
                 fetch_words<Real>(mA, mB, gb, nA, nB);
+                if constexpr (!F32) nthr = lane_get_d(mthr, gb);
             }
             for (uint32_t m = 0; m < nsteps; ++m) {
                 // ---- draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
                 MoveDraw<Real> d0;
+                double thr;
                 if constexpr (KL == 1) {
                     d0 = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, 0u, nsteps);
+                    thr = F32 ? 0.0 : metro_thr((double)d0.umet, beta);
                 } else {
                     d0 = decode_move<Real>(nA, nB, nsteps);
+                    thr = nthr;
                     const uint32_t m1 = m + 1;
                     if (m1 < nsteps) {
                         const uint32_t k1 = m1 & (KL - 1);
-                        if (k1 == 0 && m1 + q < nsteps)
+                        if (k1 == 0 && m1 + q < nsteps) {
                             move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m1 + q, mA, mB);
+                            if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                        }
                         fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
+                        if constexpr (!F32) nthr = lane_get_d(mthr, gb + (int)k1);
                     }
                 }
                 const int i = (int)d0.index;
@@ -1059,7 +1084,7 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         const double new_en = delta_en + E;
                         const double dd = new_en - E;
                         dE_used = dd;
-                        accept = (dd <= 0.0) || (det_exp(-dd * beta) >= d0.umet);
+                        accept = dd <= thr;
                         if (accept) {
                             E = new_en;
                             nacc += 1;

@@ -78,6 +78,31 @@ def test_det_log_matches_libm(oracle):
     assert oracle.lib().orc_log_det(1.0) == 0.0
 
 
+def test_metropolis_threshold_form(oracle):
+    """The FP64 pair-potential sweeps decide min(1, e^{-beta dE}) >= u
+    (src/Exchange.jl:53-57) as dE <= -ln(u) / beta.  Over random moves near
+    and far from the boundary the two forms agree except where e^{-beta dE}
+    and u are within rounding of each other; a threshold with beta
+    multiplied instead of divided, or with the sign of ln u flipped, does not."""
+    rng = np.random.default_rng(5)
+    L = oracle.lib()
+    n = 20000
+    beta = 10 ** rng.uniform(1, 5, n)
+    u = rng.uniform(0, 1, n)
+    # half the moves right at the boundary dE = -ln(u)/beta (times 1 +- 1e-9), half spread
+    de = np.where(np.arange(n) % 2 == 0, -np.log(u) / beta * (1 + rng.uniform(-1e-9, 1e-9, n)),
+                  rng.uniform(-5, 5, n) / beta)
+    exp_form = np.array([min(1.0, L.orc_exp_det(float(-b * d))) >= x for b, d, x in zip(beta, de, u)])
+    lnu = np.array([L.orc_log_det(float(x)) for x in u])
+    thr_form = de <= -lnu / beta
+    p = np.array([min(1.0, L.orc_exp_det(float(-b * d))) for b, d in zip(beta, de)])
+    tie = np.abs(p - u) <= 1e-13 * u
+    assert np.all(thr_form[~tie] == exp_form[~tie])
+    assert tie.mean() < 0.01 and exp_form.mean() > 0.2
+    assert np.any((de <= -lnu * beta)[~tie] != exp_form[~tie])
+    assert np.any((de <= lnu / beta)[~tie] != exp_form[~tie])
+
+
 # ------------------------------------------------------ reference KATs
 
 
