@@ -392,6 +392,50 @@ def bench_nnp(pt, n_ladders, warmup, steps, cpu_s):
     return out
 
 
+def bench_nnp_cu78(pt, n_ladders, warmup, steps, cpu_s):
+    """The reference's Cu78 run (scripts/run_Cu78.jl: its start
+    configuration, 20 temperatures 700-1100 K, 14 A container) with the
+    reference RuNNer potential (weights.029, scaling.data; 5 radial + 26
+    angular active functions, 88-20-20-1): 78 atoms take 128-atom slots, one
+    workgroup per CU; binary32 features and layers, FP64 totals."""
+    from paralleltemperingmontecarlo.nnp_setup import reference_cu55_potential
+
+    d = json.loads((ROOT / "tests" / "golden" / "cu78_start.json").read_text())
+    x0 = np.array(d["pos_angstrom"]) * A2B
+    x0 = x0 - x0.mean(0)
+    pot = reference_cu55_potential(NNP_DIR / "scaling.data", NNP_DIR / "weights.029.data")
+    R = d["n_traj"]
+    temp = pt.TempGrid(d["t_range"][0], d["t_range"][1], R)
+    rad = d["radius_angstrom"] * A2B
+    mc = pt.MCParams(steps, R, 78, n_adjust=10, mc_sample=1, n_bin=100)
+    eng = pt.PTEngine(pot, pt.NVT(78), pt.SphericalBC(rad), mc,
+                      pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=50))
+    for r in range(n_ladders):
+        eng.add_run(78, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r)
+    eng.prepare()
+    dt = _production(eng, n_ladders, warmup, steps)
+    out = dict(workload=f"NNP Cu78 (reference potential), {n_ladders} ladder(s) x {R} temperatures "
+                        f"{d['t_range'][0]:.0f}-{d['t_range'][1]:.0f} K",
+               moves_per_s=n_ladders * R * 78 * steps / dt, steps=steps, dtype="fp32",
+               acceptance=_acceptance(eng, n_ladders, 78, steps),
+               max_fp32_vs_fp64_drift=max(float(np.max(eng.drift(i))) for i in range(n_ladders)))
+    eng.close()
+    if cpu_s > 0:
+        from oracle import oracle as orc
+
+        orc.lib()
+        rt, at = pot.tables()
+        nn = orc.NNP(rt, at, pot.r_cut, pot.nnp.num_nodes, pot.nnp.activation_functions, pot.nnp.parameters)
+        opot = orc.make_pot("nnp", nnp=nn)
+        t2 = pt.TempGrid(d["t_range"][0], d["t_range"][1], 2)
+        out.update(_cpu_rate(lambda k: orc.OracleRun(78, t2.beta_grid, x0, pot=opot, radius=rad,
+                                                     mode=orc.REFERENCE, exchange_mode=2, seed=1234, run_key=k),
+                             2 * 78, cpu_s))
+        out["cpu_sample"] = "2-temperature ladders 700-1100 K, one per thread"
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_moves_per_s"]
+    return out
+
+
 def bench_nnvt(pt, n_ladders, warmup, steps, cpu_s, precision="fp64"):
     """Brass cluster in the NNVT ensemble (scripts/brass55.jl: 50 Cu + 5 Zn,
     NNVT([50, 5]; natomswaps=2), 10 temperatures 250-900 K, 14 A container,
@@ -504,6 +548,7 @@ def other_configs(pt, args, only=None):
                          for n in (108, 216, 256) for ens in ("nvt", "npt")],
         "nnp": lambda: [bench_nnp(pt, 1, 50, 50, cpu)],
         "nnp_batched": lambda: [bench_nnp(pt, 64, 50, 50, 0.0)],
+        "nnp_cu78": lambda: [bench_nnp_cu78(pt, 1, 20, 30, cpu), bench_nnp_cu78(pt, 64, 20, 30, 0.0)],
         # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
         # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
         "delayed_acceptance": lambda: [bench_delayed_acceptance(pt, 256, 100, 4, 8, 30)],
@@ -541,7 +586,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--config", type=str, default="",
-                    help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, "
+                    help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, nnp_cu78, "
                          "delayed_acceptance, nnvt, nnvt_batched, nnvt_f32, nnvt_f32_batched) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -134,7 +134,9 @@ int ptmc_set_potential_lookup(ptmc_handle h, int32_t n_dist, int32_t n_angle, do
  * network is [n_rad + n_ang, h1, h2, 1] with activations [*, 2 (tanh),
  * 2 (tanh), 1 (linear)] and RuNNer parameter order (NeuralNetworkPotential,
  * ForwardPass.jl:23-39).  Binary32 features and network, FP64 totals;
- * NVT in a spherical container, at most 64 atoms and 128 functions, zeta a
+ * NVT in a spherical container, at most 128 atoms (runs of up to 64 atoms
+ * share a CU two workgroups at a time; larger ones take a CU each; the
+ * delayed-acceptance surrogate covers up to 64) and 128 functions, zeta a
  * power of two up to 16, lambda = +-1, at most 32 groups of functions with
  * equal eta (one function per (lambda, zeta) in a group). */
 int ptmc_set_potential_nnp(ptmc_handle h, int32_t n_rad, int32_t n_ang, const double *rad, const double *ang,

@@ -22,9 +22,9 @@
 using namespace ptmc;
 
 namespace ptmc {
-SweepFn nnp_sweep_fn();
+SweepFn nnp_sweep_fn(int np);
 SweepFn nnp_sweep_da_fn();
-SweepFn nnp_init_fn();
+SweepFn nnp_init_fn(int np);
 SweepFn nnp2_sweep_fn();
 SweepFn nnp2_init_fn();
 SweepFn nnp2_sweep_fn_f32();
@@ -166,6 +166,7 @@ struct ptmc_engine {
     bool plug_nnp_up = false;
     NnpTables nnp_dev{};
     size_t nnp_lds = 0;
+    int nnp_np = NNP_NP;  // atoms per slot of the network-potential engine (64 or 128)
 
     // two-species network potential, NNVT (one workgroup per slot, FP64)
     bool nnvt = false;
@@ -777,7 +778,9 @@ bool is_nnp(const ptmc_engine *h) { return h->pot.kind == POT_NNP; }
 bool is_nnp2(const ptmc_engine *h) { return h->pot.kind == POT_NNP2; }
 // potentials run one workgroup per slot (label exchange between cycles)
 bool is_slot(const ptmc_engine *h) { return is_nnp(h) || is_nnp2(h); }
-// slot engines keeping binary32 positions [slot][3][NNP_NP]
+// atom stride of a slot engine's positions, features and atom energies
+int slot_np(const ptmc_engine *h) { return is_nnp(h) ? h->nnp_np : NNP_NP; }
+// slot engines keeping binary32 positions [slot][3][slot_np]
 bool slot_f32(const ptmc_engine *h) { return is_nnp(h) || (is_nnp2(h) && h->cfg.precision == 1); }
 
 // Slots of the two-species engine: as prepare_nnp, FP64 positions
@@ -864,14 +867,21 @@ int prepare_nnp(ptmc_engine *h)
     int dev = 0, max_lds = 0;
     HIPCHK(h, hipGetDevice(&dev));
     HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    h->nnp_lds = nnp_lds_floats(h->nnp.nf, h->nnp.h1) * sizeof(float);
-    if ((long long)h->nnp_lds > max_lds) return fail(h, PTMC_E_UNSUPPORTED, "network tables need %zu B of LDS", h->nnp_lds);
-    const void *fs = (const void *)nnp_sweep_fn(), *fi = (const void *)nnp_init_fn();
+    // slots of 64 atoms, or 128 (one workgroup per CU) when a run is larger
+    int n_max = 0;
+    for (auto &rh : h->runs) n_max = std::max(n_max, rh.d.n_atoms);
+    if (n_max > NNP_NP_WIDE)
+        return fail(h, PTMC_E_UNSUPPORTED, "at most %d atoms with the network potential", NNP_NP_WIDE);
+    h->nnp_np = n_max > NNP_NP ? NNP_NP_WIDE : NNP_NP;
+    h->nnp_lds = nnp_lds_floats(h->nnp.nf, h->nnp.h1, h->nnp_np) * sizeof(float);
+    if ((long long)h->nnp_lds > max_lds)
+        return fail(h, PTMC_E_UNSUPPORTED, "network tables for %d-atom slots need %zu B of LDS (%d available)", h->nnp_np,
+                    h->nnp_lds, max_lds);
+    const void *fs = (const void *)nnp_sweep_fn(h->nnp_np), *fi = (const void *)nnp_init_fn(h->nnp_np);
     HIPCHK(h, hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp_lds));
     HIPCHK(h, hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->nnp_lds));
     int slot = 0;
     for (auto &rh : h->runs) {
-        if (rh.d.n_atoms > NNP_NP) return fail(h, PTMC_E_UNSUPPORTED, "at most %d atoms with the network potential", NNP_NP);
         rh.info.slot_base = slot;
         rh.info.local_base = 0;
         rh.kl = NNP_NT;
@@ -890,7 +900,7 @@ int prepare_nnp(ptmc_engine *h)
     L.kl = NNP_NT;
     L.lds = h->nnp_lds;
     h->launches.push_back(L);
-    h->n_coord = (int64_t)slot * 3 * NNP_NP;
+    h->n_coord = (int64_t)slot * 3 * h->nnp_np;
     h->n_comp = 0;
     int rc;
     auto &n = h->nnp;
@@ -900,8 +910,8 @@ int prepare_nnp(ptmc_engine *h)
         (rc = upload(h, h->d_nnp_abuild, n.abuild)) || (rc = upload(h, h->d_nnp_gdesc, n.gdesc)) ||
         (rc = upload(h, h->d_nnp_wgroups, n.wgroups)))
         return rc;
-    if ((rc = dalloc(h, h->d_feat, (size_t)slot * n.nf * NNP_NP))) return rc;
-    if ((rc = dalloc(h, h->d_eatom, (size_t)slot * NNP_NP))) return rc;
+    if ((rc = dalloc(h, h->d_feat, (size_t)slot * n.nf * h->nnp_np))) return rc;
+    if ((rc = dalloc(h, h->d_eatom, (size_t)slot * h->nnp_np))) return rc;
     NnpTables &t = h->nnp_dev;
     t.n_rad = n.n_rad;
     t.n_ang = n.n_ang;
@@ -933,13 +943,14 @@ template <typename Real>
 void scatter_coords_nnp(ptmc_engine *h, std::vector<Real> &buf)
 {
     buf.assign((size_t)h->n_coord, (Real)0);
+    const int np = slot_np(h);
     for (auto &rh : h->runs) {
         const int N = rh.d.n_atoms;
         for (int t = 0; t < rh.d.n_temps; ++t) {
             const double *x = &rh.x0[(size_t)(t % rh.d.n_configs) * N * 3];
-            Real *dst = &buf[(size_t)(rh.info.slot_base + t) * 3 * NNP_NP];
+            Real *dst = &buf[(size_t)(rh.info.slot_base + t) * 3 * np];
             for (int j = 0; j < N; ++j)
-                for (int c = 0; c < 3; ++c) dst[c * NNP_NP + j] = (Real)x[3 * j + c];
+                for (int c = 0; c < 3; ++c) dst[c * np + j] = (Real)x[3 * j + c];
         }
     }
 }
@@ -1857,13 +1868,14 @@ int ptmc_get_state(ptmc_handle h, int32_t run, double *x, double *en, double *bo
         }
     }
     if (x && slot_f32(h)) {
-        std::vector<float> blk((size_t)R * 3 * NNP_NP);
-        HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float),
+        const int np = slot_np(h);
+        std::vector<float> blk((size_t)R * 3 * np);
+        HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * np * sizeof(float),
                             blk.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (int s = 0; s < R; ++s)
             for (int j = 0; j < N; ++j)
                 for (int c = 0; c < 3; ++c)
-                    x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * NNP_NP + j];
+                    x[((size_t)temp[s] * N + j) * 3 + c] = blk[((size_t)s * 3 + c) * np + j];
     } else if (x && is_nnp2(h)) {
         std::vector<double> blk((size_t)R * 3 * NNP_NP);
         HIPCHK(h, hipMemcpy(blk.data(), h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(double),
@@ -2052,12 +2064,13 @@ int ptmc_set_state(ptmc_handle h, int32_t run, const double *x, const double *bo
             HIPCHK(h, hipMemcpy(h->d_temp_slot.p + rh.info.temp_base, ts.data(), R * 4, hipMemcpyHostToDevice));
         }
         if (x && slot_f32(h)) {
-            std::vector<float> blk((size_t)R * 3 * NNP_NP, 0.0f);
+            const int np = slot_np(h);
+            std::vector<float> blk((size_t)R * 3 * np, 0.0f);
             for (int s = 0; s < R; ++s)
                 for (int j = 0; j < N; ++j)
                     for (int c = 0; c < 3; ++c)
-                        blk[((size_t)s * 3 + c) * NNP_NP + j] = (float)x[((size_t)s * N + j) * 3 + c];
-            HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * NNP_NP * sizeof(float), blk.data(),
+                        blk[((size_t)s * 3 + c) * np + j] = (float)x[((size_t)s * N + j) * 3 + c];
+            HIPCHK(h, hipMemcpy(h->d_coords.p + (size_t)rh.info.slot_base * 3 * np * sizeof(float), blk.data(),
                                 blk.size() * sizeof(float), hipMemcpyHostToDevice));
         } else if (x && is_nnp2(h)) {
             std::vector<double> blk((size_t)R * 3 * NNP_NP, 0.0);
@@ -2244,6 +2257,8 @@ int ptmc_surrogate_init(ptmc_handle h, int32_t capacity, uint64_t seed)
     int rc = ptmc_prepare(h);
     if (rc) return rc;
     if (!is_nnp(h)) return fail(h, PTMC_E_UNSUPPORTED, "the surrogate filters the network potential");
+    if (h->nnp_np != NNP_NP)
+        return fail(h, PTMC_E_UNSUPPORTED, "the surrogate covers configurations of at most %d atoms", NNP_NP);
     if (capacity < 1) return fail(h, PTMC_E_ARG, "sample capacity must be positive");
     HIPCHK(h, hipSetDevice(h->cfg.device));
     int dev = 0, max_lds = 0;
@@ -2311,7 +2326,7 @@ int ptmc_surrogate_enable(ptmc_handle h, int32_t on)
     if (rc) return rc;
     h->sur.da = on != 0;
     Launch &L = h->launches[0];
-    L.fn = h->sur.da ? nnp_sweep_da_fn() : nnp_sweep_fn();
+    L.fn = h->sur.da ? nnp_sweep_da_fn() : nnp_sweep_fn(h->nnp_np);
     L.lds = h->nnp_lds + (h->sur.da ? sur_lds_floats() * sizeof(float) : 0);
     return PTMC_OK;
 }

@@ -32,21 +32,22 @@ struct NnpLds {
     __shared__ __align__(16) int32_t s_gdesc[NNP_MAXG * NNP_GD];              \
     __shared__ int32_t s_wgrp[(NNP_NT / 64) * (NNP_WG + 1)]
 
+template <int NP>
 __device__ inline NnpLds nnp_lds(float *base, int nf, int h1)
 {
     NnpLds l;
     l.X = base;
-    l.G = l.X + 3 * NNP_NP;
-    l.D = l.G + nf * NNP_NP;
-    l.W1 = l.D + nf * NNP_NP;
+    l.G = l.X + 3 * NP;
+    l.D = l.G + nf * NP;
+    l.W1 = l.D + nf * NP;
     l.H1 = l.W1 + nf * h1;
-    l.H2 = l.H1 + NNP_NP * NNP_HS;
-    l.rn2 = l.H2 + NNP_NP * NNP_HS;
-    l.fn = l.rn2 + NNP_NP;
-    l.ro2 = l.fn + NNP_NP;
-    l.fo = l.ro2 + NNP_NP;
-    l.eat = l.fo + NNP_NP;
-    l.eatn = l.eat + NNP_NP;
+    l.H2 = l.H1 + NP * NNP_HS;
+    l.rn2 = l.H2 + NP * NNP_HS;
+    l.fn = l.rn2 + NP;
+    l.ro2 = l.fn + NP;
+    l.fo = l.ro2 + NP;
+    l.eat = l.fo + NP;
+    l.eatn = l.eat + NP;
     l.rad = l.W2 = l.B = nullptr;
     l.gd = l.wg = nullptr;
     return l;
@@ -214,9 +215,11 @@ __device__ __forceinline__ void nnp_visit(const NnpGroup &G, float rsn, float rs
 // Stores of one group: the 2^lp lanes of a target (an aligned block)
 // combine their parts by quad exchanges and the half-row mirror, the first
 // writes the target's entry; the wave's sum of the terms centred on i goes
-// to the moved atom's column.  All lanes call.
+// to the moved atom's column (added to the earlier chunks' sum when the
+// targets come in several chunks of 64).  All lanes call.
+template <int NP>
 __device__ __forceinline__ void nnp_store(const NnpGroup &G, int lp, bool writer, int j, int i, int lane, float *Dp,
-                                          float *aj, float *ai)
+                                          float *aj, float *ai, bool first)
 {
 #pragma unroll
     for (int q = 0; q < NNP_GMAX; ++q) {
@@ -228,20 +231,21 @@ __device__ __forceinline__ void nnp_store(const NnpGroup &G, int lp, bool writer
             v = lp >= 2 ? v + t2 : v;
             const float t3 = dpp_f<0x141>(v);
             v = lp >= 3 ? v + t3 : v;
-            float *Dr = Dp + (G.row0 + G.rk[q]) * NNP_NP;
+            float *Dr = Dp + (G.row0 + G.rk[q]) * NP;
             if (writer) Dr[j] = v * G.tpz[q];
             const float vi = wave_sum_last(ai[q]);
-            if (lane == 63) Dr[i] = vi * G.tpz[q];
+            if (lane == 63) Dr[i] = first ? vi * G.tpz[q] : Dr[i] + vi * G.tpz[q];
         }
     }
 }
 
 // Full feature matrix from positions into Gout (total_symm_calc,
 // SymmFunc.jl:494-510; radial :259-278, angular :280-312).  All threads call.
+template <int NP>
 __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout, int N)
 {
     const int tid = threadIdx.x;
-    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Gout[e] = 0.0f;
+    for (int e = tid; e < T.nf * NP; e += NNP_NT) Gout[e] = 0.0f;
     __syncthreads();
     // radial: one (function, atom) per item
     for (int e = tid; e < T.n_rad_active * N; e += NNP_NT) {
@@ -251,13 +255,12 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout,
         float s = 0.0f;
         for (int b = 0; b < N; ++b) {
             if (b == a) continue;
-            const float dx = L.X[a] - L.X[b], dy = L.X[NNP_NP + a] - L.X[NNP_NP + b],
-                        dz = L.X[2 * NNP_NP + a] - L.X[2 * NNP_NP + b];
+            const float dx = L.X[a] - L.X[b], dy = L.X[NP + a] - L.X[NP + b], dz = L.X[2 * NP + a] - L.X[2 * NP + b];
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float fc = nnp_cutoff(sqrtf(r2), T.r_cut);
             if (fc != 0.0f && fc != 1.0f) s += fc * expf(-eta * r2) * norm;
         }
-        Gout[row * NNP_NP + a] = s;
+        Gout[row * NP + a] = s;
     }
     // angular: one triangle a < b < c per item, all functions
     const int ntri = N * (N - 1) * (N - 2) / 6;
@@ -274,9 +277,9 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout,
             ++b;
         }
         const int c = b + 1 + rem;
-        const float pa[3] = {L.X[a], L.X[NNP_NP + a], L.X[2 * NNP_NP + a]};
-        const float pb[3] = {L.X[b], L.X[NNP_NP + b], L.X[2 * NNP_NP + b]};
-        const float pc[3] = {L.X[c], L.X[NNP_NP + c], L.X[2 * NNP_NP + c]};
+        const float pa[3] = {L.X[a], L.X[NP + a], L.X[2 * NP + a]};
+        const float pb[3] = {L.X[b], L.X[NP + b], L.X[2 * NP + b]};
+        const float pc[3] = {L.X[c], L.X[NP + c], L.X[2 * NP + c]};
         auto d2 = [](const float *u, const float *v) {
             const float x = u[0] - v[0], y = u[1] - v[1], z = u[2] - v[2];
             return x * x + y * y + z * z;
@@ -298,16 +301,16 @@ __device__ void nnp_full_build(const NnpTables &T, const NnpLds &L, float *Gout,
                 const int row = (int)T.afn[4 * f + 3];
                 float pa_, pb_, pc_;
                 nnp_theta3(lam, zc, ta, tb, tc, pa_, pb_, pc_);
-                atomicAdd(&Gout[row * NNP_NP + a], ex * pa_ * tpz);
-                atomicAdd(&Gout[row * NNP_NP + b], ex * pb_ * tpz);
-                atomicAdd(&Gout[row * NNP_NP + c], ex * pc_ * tpz);
+                atomicAdd(&Gout[row * NP + a], ex * pa_ * tpz);
+                atomicAdd(&Gout[row * NP + b], ex * pb_ * tpz);
+                atomicAdd(&Gout[row * NP + c], ex * pc_ * tpz);
             }
         }
     }
     __syncthreads();
     for (int e = tid; e < T.nf * N; e += NNP_NT) {
         const int q = e / N, a = e % N;
-        Gout[q * NNP_NP + a] += T.row_off[q];
+        Gout[q * NP + a] += T.row_off[q];
     }
     __syncthreads();
 }
@@ -324,19 +327,24 @@ __device__ __forceinline__ float nnp_tanh(float x)
 
 // Per-atom network energies of the features Gin into out[N]
 // (n_features -> tanh h1 -> tanh h2 -> linear 1, ForwardPass.jl:23-39).
-// The two hidden layers are 64 x 32 products on the matrix cores in binary32
-// (v_mfma_f32_32x32x2_f32; waves 0 and 1 take atoms 0-31 and 32-63, two
+// The two hidden layers are NP x 32 products on the matrix cores in binary32
+// (v_mfma_f32_32x32x2_f32; wave w takes atoms 32w to 32w + 31, two
 // accumulators over alternating k pairs); lane layout: A[row l%32][k l/32],
 // B[k l/32][col l%32], output value v of lane l at row 8(v/4) + 4(l/32) + v%4,
-// column l%32.  The output layer is one dot product per atom.  All threads call.
+// column l%32.  The output layer is one dot product per atom; the FP64 total
+// is a butterfly over each wave of 64 atoms, then the waves' sums in order.
+// All threads call.
+template <int NP>
 __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gin, int N, float *out,
                             double *total = nullptr)
 {
+    constexpr int NW = NP / 64;
+    __shared__ double s_fpart[NNP_NP_WIDE / 64];
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lr = l & 31, lh = l >> 5;
     const int nf = T.nf, h1 = T.h1, h2 = T.h2;
-    if (w < 2) {
+    if (w < NP / 32) {
         const int r0 = 32 * w;
         const int c1 = lr < h1 ? lr : 0;
         const float m1 = lr < h1 ? 1.0f : 0.0f;
@@ -345,7 +353,7 @@ __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gi
             const int ka = kk + lh, kb = kk + 2 + lh;
             const int kac = ka < nf ? ka : nf - 1, kbc = kb < nf ? kb : nf - 1;
             const float ma = ka < nf ? 1.0f : 0.0f, mb = kb < nf ? 1.0f : 0.0f;
-            const float xa = Gin[kac * NNP_NP + r0 + lr] * ma, xb = Gin[kbc * NNP_NP + r0 + lr] * mb;
+            const float xa = Gin[kac * NP + r0 + lr] * ma, xb = Gin[kbc * NP + r0 + lr] * mb;
             const float wa = L.W1[kac * h1 + c1] * m1, wb = L.W1[kbc * h1 + c1] * m1;
             za = __builtin_amdgcn_mfma_f32_32x32x2f32(xa, wa, za, 0, 0, 0);
             zb = __builtin_amdgcn_mfma_f32_32x32x2f32(xb, wb, zb, 0, 0, 0);
@@ -360,7 +368,7 @@ __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gi
         }
     }
     __syncthreads();
-    if (w < 2) {
+    if (w < NP / 32) {
         const int r0 = 32 * w;
         const int c2 = lr < h2 ? lr : 0;
         const float m2 = lr < h2 ? 1.0f : 0.0f;
@@ -385,30 +393,41 @@ __device__ void nnp_forward(const NnpTables &T, const NnpLds &L, const float *Gi
         }
     }
     __syncthreads();
-    if (w == 0) {
+    if (w < NW) {
+        const int at = 64 * w + l;
         float y = 0.0f;
-        if (l < N) {
-            for (int k = 0; k < h2; ++k) y += L.H2[l * NNP_HS + k];
+        if (at < N) {
+            for (int k = 0; k < h2; ++k) y += L.H2[at * NNP_HS + k];
             y += T.b3;
-            out[l] = y;
+            out[at] = y;
         }
         // FP64 total of the atomic energies (a butterfly over the wave)
         if (total != nullptr) {
-            double t = l < N ? (double)y : 0.0;
+            double t = at < N ? (double)y : 0.0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-            if (l == 0) *total = t;
+            if (NW == 1 && l == 0) *total = t;
+            if (NW > 1 && l == 0) s_fpart[w] = t;
+        }
+    }
+    if (NW > 1 && total != nullptr) {
+        __syncthreads();
+        if (tid == 0) {
+            double t = s_fpart[0];
+            for (int v = 1; v < NW; ++v) t += s_fpart[v];
+            *total = t;
         }
     }
     __syncthreads();
 }
 
+template <int NP>
 __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, int slot)
 {
     const NnpTables &T = a.nnp;
-    const float *gX = reinterpret_cast<const float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
+    const float *gX = reinterpret_cast<const float *>(a.coords) + (size_t)slot * 3 * NP;
     const int tid = threadIdx.x;
-    for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) L.X[e] = gX[e];
+    for (int e = tid; e < 3 * NP; e += NNP_NT) L.X[e] = gX[e];
     for (int e = tid; e < T.nf * T.h1; e += NNP_NT) L.W1[e] = T.w1[e];
     for (int e = tid; e < 4 * T.n_rad_active; e += NNP_NT) L.rad[e] = T.rad[e];
     for (int e = tid; e < NNP_GD * T.n_groups; e += NNP_NT) L.gd[e] = T.gdesc[e];
@@ -430,6 +449,7 @@ __device__ __forceinline__ void nnp_load(const SweepArgs &a, const NnpLds &L, in
 
 // set_variables + initialise_energy (src/EnergyEvaluation.jl:1812-1822,
 // :1735-1747): features, per-atom energies and the FP64 total of each slot.
+template <int NP>
 __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -437,30 +457,37 @@ __global__ void __launch_bounds__(NNP_NT) nnp_init_kernel(SweepArgs a)
     const int slot = blockIdx.x;
     if (a.init_run >= 0 && a.slot_run[slot] != a.init_run) return;  // whole workgroup
     NNP_TABLES_LDS;
-    NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NnpLds L = nnp_lds<NP>(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
     NNP_BIND_TABLES(L);
     const int N = a.runs[a.slot_run[slot]].n_atoms;
-    nnp_load(a, L, slot);
+    nnp_load<NP>(a, L, slot);
     __syncthreads();
-    nnp_full_build(T, L, L.G, N);
+    nnp_full_build<NP>(T, L, L.G, N);
     __shared__ double s_tot;
-    nnp_forward(T, L, L.G, N, L.eat, &s_tot);
-    float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
-    for (int e = threadIdx.x; e < T.nf * NNP_NP; e += NNP_NT) gG[e] = L.G[e];
-    for (int e = threadIdx.x; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = e < N ? L.eat[e] : 0.0f;
+    nnp_forward<NP>(T, L, L.G, N, L.eat, &s_tot);
+    float *gG = a.feat + (size_t)slot * T.nf * NP;
+    for (int e = threadIdx.x; e < T.nf * NP; e += NNP_NT) gG[e] = L.G[e];
+    for (int e = threadIdx.x; e < NP; e += NNP_NT) a.eatom[(size_t)slot * NP + e] = e < N ? L.eat[e] : 0.0f;
     if (threadIdx.x == 0) a.slot_en[slot] = s_tot;
 }
 
 // One Monte-Carlo cycle (N moves) of one configuration per workgroup
 // (mc_step!, src/MCRun.jl:169-182, with energy_update!(RuNNerPotential)).
 // DA: delayed acceptance with the learned surrogate (one workgroup per CU by
-// LDS, so the register budget is relaxed for that variant).
-template <bool DA>
-__global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA ? 2 : 4))) nnp_sweep_kernel(SweepArgs a)
+// LDS, so the register budget is relaxed for that variant).  NP: atoms of a
+// slot, 64 (two workgroups per CU) or 128 (one; the near atoms of a move are
+// then up to 127, their partner masks two words, and the angular targets are
+// taken in chunks of 64 lanes).
+template <bool DA, int NP>
+__global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu((DA || NP > NNP_NP) ? 2 : 4)))
+nnp_sweep_kernel(SweepArgs a)
 {
+    static_assert(NP == NNP_NP || (NP == NNP_NP_WIDE && !DA), "slots of 64 atoms, or 128 without the surrogate");
+    constexpr int NW = NP / 64;  // 64-bit words of an atom mask
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ int s_flag, s_err, s_nnear;
-    __shared__ uint8_t s_near[NNP_NP];
+    __shared__ uint64_t s_wmask[NW];
+    __shared__ uint8_t s_near[NP];
     __shared__ double s_en;
     __shared__ float s_trial[3], s_old[3];
     const NnpTables &T = a.nnp;
@@ -468,7 +495,7 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     NNP_TABLES_LDS;
-    NnpLds L = nnp_lds(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
+    NnpLds L = nnp_lds<NP>(reinterpret_cast<float *>(lds_raw), T.nf, T.h1);
     NNP_BIND_TABLES(L);
     const RunInfo &ri = a.runs[a.slot_run[slot]];
     const int N = ri.n_atoms;
@@ -480,10 +507,10 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     const float delta_r = (float)a.t_displ[2 * Tg];
     const float radius2 = (float)ri.radius2;
 
-    nnp_load(a, L, slot);
-    const float *gG = a.feat + (size_t)slot * T.nf * NNP_NP;
-    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) L.G[e] = gG[e];
-    for (int e = tid; e < NNP_NP; e += NNP_NT) L.eat[e] = a.eatom[(size_t)slot * NNP_NP + e];
+    nnp_load<NP>(a, L, slot);
+    const float *gG = a.feat + (size_t)slot * T.nf * NP;
+    for (int e = tid; e < T.nf * NP; e += NNP_NT) L.G[e] = gG[e];
+    for (int e = tid; e < NP; e += NNP_NT) L.eat[e] = a.eatom[(size_t)slot * NP + e];
     double E = a.slot_en[slot];
     if (tid == 0) s_err = 0;
     // delayed acceptance: surrogate descriptors and atom energies in LDS
@@ -499,19 +526,19 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     if (da) {
         float *base = reinterpret_cast<float *>(lds_raw) + nnp_lds_floats(T.nf, T.h1);
         Sd = base;
-        Sn = Sd + NNP_NP * SUR_SS;
-        net.W1 = reinterpret_cast<short *>(Sn + NNP_NP * SUR_SS);
+        Sn = Sd + NP * SUR_SS;
+        net.W1 = reinterpret_cast<short *>(Sn + NP * SUR_SS);
         net.W2 = net.W1 + SUR_K * SUR_H;
         net.Hb = net.W2 + SUR_H * SUR_H;
-        sat = reinterpret_cast<float *>(net.Hb + NNP_NP * SUR_HS);
-        satn = sat + NNP_NP;
-        net.b1 = satn + NNP_NP;
+        sat = reinterpret_cast<float *>(net.Hb + NP * SUR_HS);
+        satn = sat + NP;
+        net.b1 = satn + NP;
         net.b2 = net.b1 + SUR_H;
         net.w3 = net.b2 + SUR_H;
         sgeo = net.w3 + SUR_H;  // per atom: r_new, r_old, f_c(r_new), f_c(r_old)
         sur_load_net(net, a.sur_w, NNP_NT);
         sur_descriptors(sg, L.X, N, Sd, SUR_SS, NNP_NT);
-        for (int e = tid; e < NNP_NP * SUR_SS; e += NNP_NT) Sn[e] = 0.0f;
+        for (int e = tid; e < NP * SUR_SS; e += NNP_NT) Sn[e] = 0.0f;
         __syncthreads();
         sur_atom_energies(net, Sd, N, L.H1, sat, &s_ehat);
         Ehat = s_ehat;
@@ -522,14 +549,14 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
     // accepted move swaps the two (swap_vars!(NNPVariables), src/swap_config.jl:213-221)
     float *Gp = L.G, *Dp = L.D;
     float4 *nb_a = reinterpret_cast<float4 *>((reinterpret_cast<uintptr_t>(L.H2) + 15) & ~(uintptr_t)15);
-    float4 *nb_b = nb_a + NNP_NP;
-    float *nb_fo = reinterpret_cast<float *>(nb_b + NNP_NP);
+    float4 *nb_b = nb_a + NP;
+    float *nb_fo = reinterpret_cast<float *>(nb_b + NP);
 
     for (uint32_t m = 0; m < nsteps; ++m) {
         // draws and trial position (every thread the same), src/MCMoves.jl:115-135
         MoveDraw<float> d0 = move_draw<float>(a.seed, ri.run_key, (uint32_t)t, sweep, m, 0u, nsteps);
         const int i = (int)d0.index;
-        float xi = L.X[i], yi = L.X[NNP_NP + i], zi = L.X[2 * NNP_NP + i];
+        float xi = L.X[i], yi = L.X[NP + i], zi = L.X[2 * NP + i];
         float tx, ty, tz;
         make_trial<float, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, 0.0f, 0.0f, tx, ty, tz);
         if (outside_sphere<float>(tx, ty, tz, radius2)) {
@@ -542,19 +569,19 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                     if (tid == 0) {
                         double c[3] = {0.0, 0.0, 0.0};
                         for (int b = 0; b < N; ++b)
-                            for (int k = 0; k < 3; ++k) c[k] += (double)L.X[k * NNP_NP + b];
+                            for (int k = 0; k < 3; ++k) c[k] += (double)L.X[k * NP + b];
                         for (int k = 0; k < 3; ++k) c[k] /= (double)N;
                         for (int b = 0; b < N; ++b)
                             for (int k = 0; k < 3; ++k)
-                                L.X[k * NNP_NP + b] = (float)((double)L.X[k * NNP_NP + b] - c[k]);
+                                L.X[k * NP + b] = (float)((double)L.X[k * NP + b] - c[k]);
                     }
                     __syncthreads();
                 } else {
                     attempt += 1;
                     MoveDraw<float> dd = move_draw<float>(a.seed, ri.run_key, (uint32_t)t, sweep, m, attempt, nsteps);
                     xi = L.X[i];
-                    yi = L.X[NNP_NP + i];
-                    zi = L.X[2 * NNP_NP + i];
+                    yi = L.X[NP + i];
+                    zi = L.X[2 * NP + i];
                     make_trial<float, BC_SPHERE, false>(xi, yi, zi, dd, delta_r, 0.0f, 0.0f, tx, ty, tz);
                     if (count == 100) {
                         if (tid == 0) s_err = 1;
@@ -563,20 +590,20 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 }
             }
             xi = L.X[i];
-            yi = L.X[NNP_NP + i];
-            zi = L.X[2 * NNP_NP + i];
+            yi = L.X[NP + i];
+            zi = L.X[2 * NP + i];
         }
         const float pt[3] = {tx, ty, tz}, po[3] = {xi, yi, zi};
         // distances and cutoffs to the new and old position; hard-core test
-        // (wave 0, one lane per atom) and the list of the atoms within the
-        // cut-off of either position, the only partners of angular terms
+        // (one lane per atom) and the list of the atoms within the cut-off of
+        // either position, the only partners of angular terms, in atom order
         if (tid == 0) s_flag = 0;
         __syncthreads();
-        if (tid < 64) {
-            const int j = lane;
+        if (tid < NP) {
+            const int j = tid;
             bool near = false;
             if (j < N) {
-                const float px = L.X[j], py = L.X[NNP_NP + j], pz = L.X[2 * NNP_NP + j];
+                const float px = L.X[j], py = L.X[NP + j], pz = L.X[2 * NP + j];
                 float dx = tx - px, dy = ty - py, dz = tz - pz;
                 const float rn = j == i ? 0.0f : dx * dx + dy * dy + dz * dz;
                 dx = xi - px;
@@ -592,18 +619,44 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 near = j != i && (fnj != 0.0f || foj != 0.0f);
             }
             const uint64_t b = __ballot(near);
-            if (near) {
-                // packed per near atom for the angular visits (in the H2 scratch):
-                // {x, y, z, f_new}, {r2_new, r2_old, r_new, r_old}, f_old
-                const int q = mbcnt64(b);
-                s_near[q] = (uint8_t)j;
-                nb_a[q] = make_float4(L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j], L.fn[j]);
-                nb_b[q] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
-                nb_fo[q] = L.fo[j];
+            if constexpr (NW == 1) {
+                if (near) {
+                    // packed per near atom for the angular visits (in the H2 scratch):
+                    // {x, y, z, f_new}, {r2_new, r2_old, r_new, r_old}, f_old
+                    const int q = mbcnt64(b);
+                    s_near[q] = (uint8_t)j;
+                    nb_a[q] = make_float4(L.X[j], L.X[NP + j], L.X[2 * NP + j], L.fn[j]);
+                    nb_b[q] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
+                    nb_fo[q] = L.fo[j];
+                }
+                if (lane == 0) s_nnear = __popcll(b);
+            } else {
+                if (lane == 0) s_wmask[tid >> 6] = b;
             }
-            if (lane == 0) s_nnear = __popcll(b);
         }
-        for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] = 0.0f;
+        if constexpr (NW > 1) {
+            // the near atoms of lower waves come first: list positions from
+            // the waves' masks once every wave has written its own
+            __syncthreads();
+            if (tid < NP) {
+                const int j = tid;
+                const uint64_t b = s_wmask[tid >> 6];
+                if ((b >> lane) & 1ull) {
+                    int q = mbcnt64(b);
+                    for (int v = 0; v < (tid >> 6); ++v) q += __popcll(s_wmask[v]);
+                    s_near[q] = (uint8_t)j;
+                    nb_a[q] = make_float4(L.X[j], L.X[NP + j], L.X[2 * NP + j], L.fn[j]);
+                    nb_b[q] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
+                    nb_fo[q] = L.fo[j];
+                }
+            }
+            if (tid == 0) {
+                int t = 0;
+                for (int v = 0; v < NW; ++v) t += __popcll(s_wmask[v]);
+                s_nnear = t;
+            }
+        }
+        for (int e = tid; e < T.nf * NP; e += NNP_NT) Dp[e] = 0.0f;
         if (da && tid < SUR_K) s_acci[tid] = 0.0f;
         __syncthreads();
         // delayed acceptance, stage 1: Metropolis on the surrogate energy change
@@ -614,9 +667,9 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             if (tid < N) {
                 const float rn = sqrtf(L.rn2[tid]), ro = sqrtf(L.ro2[tid]);
                 sgeo[tid] = rn;
-                sgeo[NNP_NP + tid] = ro;
-                sgeo[2 * NNP_NP + tid] = tid == i ? 0.0f : sur_cutoff(rn, sg.r_cut);
-                sgeo[3 * NNP_NP + tid] = tid == i ? 0.0f : sur_cutoff(ro, sg.r_cut);
+                sgeo[NP + tid] = ro;
+                sgeo[2 * NP + tid] = tid == i ? 0.0f : sur_cutoff(rn, sg.r_cut);
+                sgeo[3 * NP + tid] = tid == i ? 0.0f : sur_cutoff(ro, sg.r_cut);
             }
             __syncthreads();
             float accq = 0.0f;
@@ -624,9 +677,9 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             const float mu = sur_mu(sg, kq);
             for (int j = tid / SUR_K; j < N; j += NNP_NT / SUR_K) {
                 if (j == i) continue;
-                const float fcn = sgeo[2 * NNP_NP + j], fco = sgeo[3 * NNP_NP + j];
+                const float fcn = sgeo[2 * NP + j], fco = sgeo[3 * NP + j];
                 const float pn = sur_phi_mu(sg, sgeo[j], fcn, mu);
-                const float pq = sur_phi_mu(sg, sgeo[NNP_NP + j], fco, mu);
+                const float pq = sur_phi_mu(sg, sgeo[NP + j], fco, mu);
                 Sn[j * SUR_SS + kq] = __builtin_fmaf(SUR_SIN, pn - pq, Sd[j * SUR_SS + kq]);
                 accq += pn;
             }
@@ -648,22 +701,29 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 // the surrogate's layer scratch covered the packed near-atom data
                 if (tid < s_nnear) {
                     const int j = s_near[tid];
-                    nb_a[tid] = make_float4(L.X[j], L.X[NNP_NP + j], L.X[2 * NNP_NP + j], L.fn[j]);
+                    nb_a[tid] = make_float4(L.X[j], L.X[NP + j], L.X[2 * NP + j], L.fn[j]);
                     nb_b[tid] = make_float4(L.rn2[j], L.ro2[j], sqrtf(L.rn2[j]), sqrtf(L.ro2[j]));
                     nb_fo[tid] = L.fo[j];
                 }
                 __syncthreads();
             }
-            // radial changes, DeltaMatrix.jl:794-822: one wave per function
+            // radial changes, DeltaMatrix.jl:794-822: one wave per function,
+            // one lane per atom (NW atoms per lane in the wide layout)
             for (int q = tid >> 6; q < T.n_rad_active; q += NNP_NT / 64) {
                 const float eta = L.rad[4 * q], norm = L.rad[4 * q + 1];
                 const int row = (int)L.rad[4 * q + 2];
-                float d = 0.0f;
-                if (lane < N && lane != i)
-                    d = expf(-eta * L.rn2[lane]) * (L.fn[lane] * norm) + expf(-eta * L.ro2[lane]) * (-L.fo[lane] * norm);
-                if (lane < N && lane != i) Dp[row * NNP_NP + lane] = d;
-                const float s = wave_sum(d);
-                if (lane == 0) Dp[row * NNP_NP + i] = s;
+                float dsum = 0.0f;
+#pragma unroll
+                for (int v = 0; v < NW; ++v) {
+                    const int at = 64 * v + lane;
+                    float d = 0.0f;
+                    if (at < N && at != i)
+                        d = expf(-eta * L.rn2[at]) * (L.fn[at] * norm) + expf(-eta * L.ro2[at]) * (-L.fo[at] * norm);
+                    if (at < N && at != i) Dp[row * NP + at] = d;
+                    dsum += d;
+                }
+                const float s = wave_sum(dsum);
+                if (lane == 0) Dp[row * NP + i] = s;
             }
             // angular changes, DeltaMatrix.jl:827-872.  Every partner pair
             // (j, k) of the moved atom i whose triangle lies inside the cut-off
@@ -675,70 +735,114 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             // feature entry is written by one lane with a plain store: no LDS
             // atomics (float LDS adds run far below the store rate).
             const int n_near = s_nnear;
-            uint64_t *s_mask = reinterpret_cast<uint64_t *>(L.H1);  // scratch until the network pass
+            // partner masks [near atom][NW words] (scratch until the network pass)
+            uint64_t *s_mask = reinterpret_cast<uint64_t *>(L.H1);
             const float rc2 = T.r_cut * T.r_cut * 1.0001f;  // loose: the cut-off function decides below
             for (int a = tid >> 6; a < n_near; a += NNP_NT / 64) {
                 const float4 pa = nb_a[a];
                 const float foa = nb_fo[a];
-                bool act = false;
-                if (lane < n_near && lane != a) {
-                    const float4 pb = nb_a[lane];
-                    if (pa.w * pb.w != 0.0f || foa * nb_fo[lane] != 0.0f) {
-                        const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-                        act = dx * dx + dy * dy + dz * dz < rc2;
+#pragma unroll
+                for (int v = 0; v < NW; ++v) {
+                    const int b = 64 * v + lane;
+                    bool act = false;
+                    if (b < n_near && b != a) {
+                        const float4 pb = nb_a[b];
+                        if (pa.w * pb.w != 0.0f || foa * nb_fo[b] != 0.0f) {
+                            const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+                            act = dx * dx + dy * dy + dz * dz < rc2;
+                        }
                     }
+                    const uint64_t m = __ballot(act);
+                    if (lane == 0) s_mask[a * NW + v] = m;
                 }
-                const uint64_t m = __ballot(act);
-                if (lane == 0) s_mask[a] = m;
             }
             __syncthreads();
-            // lanes per target: target a gets P_a = 1, 2, 4 or 8 lanes, the
-            // fewest that bring its share of visits down to the load T, T the
-            // smallest load for which the shares fit 64 lanes (bisection, the
-            // lane count summed by ballots).  Blocks of P lanes are laid out
-            // by P (8, 4, 2, 1), so each is aligned for the quad and half-row
-            // exchanges that combine them.  Every wave builds the same table
-            // in its own LDS slice (no barrier needed within a wave).
-            const int La = lane < n_near ? __popcll(s_mask[lane]) : 0;
-            auto lanes_for = [&](int load) {
-                if (lane >= n_near) return 0;
-                const int q = (La + load - 1) / load;
-                return q <= 1 ? 1 : (q <= 2 ? 2 : (q <= 4 ? 4 : 8));
-            };
-            auto lanes_used = [&](int pa) {
-                int t = 0;
+            // Targets in chunks of 64 near atoms; per chunk, lanes per target:
+            // target a gets P_a = 1, 2, 4 or 8 lanes, the fewest that bring its
+            // share of visits down to the load T, T the smallest load for which
+            // the shares fit 64 lanes (bisection, the lane count summed by
+            // ballots).  Blocks of P lanes are laid out by P (8, 4, 2, 1), so
+            // each is aligned for the quad and half-row exchanges that combine
+            // them.  Every wave builds the same tables in its own LDS slice (no
+            // barrier needed within a wave).
+            uint16_t *s_lt = reinterpret_cast<uint16_t *>(L.H1 + 2 * NW * NP) + NP * (tid >> 6);
+            const int nch = (n_near + 63) >> 6;
+            int used_c[NW];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) t += __popcll(__ballot(pa == (1 << c))) << c;
-                return t;
-            };
-            int lo = 1, hi = 64;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (lanes_used(lanes_for(mid)) <= 64) hi = mid;
-                else lo = mid + 1;
-            }
-            const int Pa = lanes_for(lo);
-            uint16_t *s_lt = reinterpret_cast<uint16_t *>(L.H1 + 2 * NNP_NP) + NNP_NP * (tid >> 6);
-            int used = 0;
+            for (int c = 0; c < NW; ++c) {
+                used_c[c] = 0;
+                if (c >= nch) continue;
+                const int nc = n_near - 64 * c < 64 ? n_near - 64 * c : 64;
+                int La = 0;
+                if (lane < nc) {
 #pragma unroll
-            for (int c = 3; c >= 0; --c) {
-                const bool in = Pa == (1 << c);
-                const uint64_t b = __ballot(in);
-                if (in)
-                    for (int q = 0; q < Pa; ++q) s_lt[used + (mbcnt64(b) << c) + q] = (uint16_t)(lane | (q << 6) | (c << 9));
-                used += __popcll(b) << c;
+                    for (int v = 0; v < NW; ++v) La += __popcll(s_mask[(64 * c + lane) * NW + v]);
+                }
+                auto lanes_for = [&](int load) {
+                    if (lane >= nc) return 0;
+                    const int q = (La + load - 1) / load;
+                    return q <= 1 ? 1 : (q <= 2 ? 2 : (q <= 4 ? 4 : 8));
+                };
+                auto lanes_used = [&](int pa) {
+                    int t = 0;
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc) t += __popcll(__ballot(pa == (1 << cc))) << cc;
+                    return t;
+                };
+                int lo = 1, hi = 64 * NW;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (lanes_used(lanes_for(mid)) <= 64) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int Pa = lanes_for(lo);
+                int used = 0;
+#pragma unroll
+                for (int cc = 3; cc >= 0; --cc) {
+                    const bool in = Pa == (1 << cc);
+                    const uint64_t b = __ballot(in);
+                    if (in)
+                        for (int q = 0; q < Pa; ++q)
+                            s_lt[64 * c + used + (mbcnt64(b) << cc) + q] = (uint16_t)(lane | (q << 6) | (cc << 9));
+                    used += __popcll(b) << cc;
+                }
+                used_c[c] = used;
             }
-            const int ent = lane < used ? (int)s_lt[lane] : 0;
-            const int lp = ent >> 9, P = 1 << lp;
-            const int ta = ent & 63, part = (ent >> 6) & 7;
-            const bool tact = lane < used;
-            const int j = tact ? (int)s_near[ta] : 0;
-            const uint64_t mask = tact ? s_mask[ta] : 0ull;
-            const uint64_t after = ta < 63 ? ~((2ull << ta) - 1ull) : 0ull;  // partners of larger index
-            const float4 ja = nb_a[ta], jb = nb_b[ta];
-            const float jfo = nb_fo[ta];
-            const float pj3[3] = {ja.x, ja.y, ja.z};
-            const float dijn = jb.z, dijo = jb.w;
+            // this lane's target in chunk c: lanes, part, atom, partner masks
+            // (and those of larger list index, the visits that also carry the
+            // terms centred on i), packed data
+            struct Target {
+                int lp, P, part, j;
+                bool tact;
+                uint64_t m0, m1, after0, after1;
+                float4 ja, jb;
+                float jfo;
+            };
+            auto target = [&](int c) {
+                Target g;
+                const int ent = lane < used_c[c] ? (int)s_lt[64 * c + lane] : 0;
+                g.lp = ent >> 9;
+                g.P = 1 << g.lp;
+                const int ta = 64 * c + (ent & 63);
+                g.part = (ent >> 6) & 7;
+                g.tact = lane < used_c[c];
+                g.j = g.tact ? (int)s_near[ta] : 0;
+                g.m0 = g.tact ? s_mask[ta * NW] : 0ull;
+                g.m1 = 0ull;
+                g.after0 = ta < 63 ? ~((2ull << ta) - 1ull) : 0ull;
+                g.after1 = 0ull;
+                if constexpr (NW > 1) {
+                    g.m1 = g.tact ? s_mask[ta * NW + 1] : 0ull;
+                    const int sh = ta - 64;
+                    g.after1 = sh < 0 ? ~0ull : (sh < 63 ? ~((2ull << sh) - 1ull) : 0ull);
+                }
+                g.ja = nb_a[ta];
+                g.jb = nb_b[ta];
+                g.jfo = nb_fo[ta];
+                return g;
+            };
+            Target t1{};
+            if constexpr (NW == 1) t1 = target(0);  // one chunk: read once for all group passes
             const int w = tid >> 6;
             const int ngw = L.wg[w * (NNP_WG + 1)];
             for (int gi = 0; gi < ngw; gi += 2) {
@@ -749,46 +853,67 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
                 nnp_group(L.gd, ga, G0);
                 if (gb >= 0) nnp_group(L.gd, gb, G1);
                 else G1.none();
-                float aj0[NNP_GMAX], ai0[NNP_GMAX], aj1[NNP_GMAX], ai1[NNP_GMAX];
 #pragma unroll
-                for (int q = 0; q < NNP_GMAX; ++q) aj0[q] = ai0[q] = aj1[q] = ai1[q] = 0.0f;
-                uint64_t m = mask;
-                for (int t = 0; t < part; ++t) m &= m - 1ull;  // this lane's first partner
-                while (m != 0ull) {
-                    const int bk = __builtin_ctzll(m);
-                    for (int t = 0; t < P; ++t) m &= m - 1ull;  // next partner of this lane
-                    const bool own = (after >> bk) & 1ull;
-                    const float4 ka = nb_a[bk], kb = nb_b[bk];
-                    const float kfo = nb_fo[bk];
-                    const float pk3[3] = {ka.x, ka.y, ka.z};
-                    const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
-                    const float rjk = dx * dx + dy * dy + dz * dz;
-                    const float djk = sqrtf(rjk);
-                    const float fjk = nnp_cutoff(djk, T.r_cut);
-                    const float fpn = ja.w * ka.w * fjk, fpo = jfo * kfo * fjk;
-                    if (fpn == 0.0f && fpo == 0.0f) continue;
-                    const float rsn = jb.x + kb.x + rjk, rso = jb.y + kb.y + rjk;
-                    // cosines at j (and at i), new and old
-                    float cjn = 0.0f, cjo = 0.0f, cin = 0.0f, cio = 0.0f;
-                    const float dikn = kb.z, diko = kb.w;
-                    if (fpn != 0.0f) {
-                        cjn = nnp_cos(pj3, pt, pk3, dijn, djk);
-                        if (own) cin = nnp_cos(pt, pj3, pk3, dijn, dikn);
+                for (int c = 0; c < NW; ++c) {
+                    if (c >= nch) continue;
+                    const Target tg = NW == 1 ? t1 : target(c);
+                    const int lp = tg.lp, P = tg.P, part = tg.part, j = tg.j;
+                    const bool tact = tg.tact;
+                    uint64_t m0 = tg.m0, m1 = tg.m1;
+                    const uint64_t after0 = tg.after0, after1 = tg.after1;
+                    const float4 ja = tg.ja, jb = tg.jb;
+                    const float jfo = tg.jfo;
+                    const float pj3[3] = {ja.x, ja.y, ja.z};
+                    const float dijn = jb.z, dijo = jb.w;
+                    float aj0[NNP_GMAX], ai0[NNP_GMAX], aj1[NNP_GMAX], ai1[NNP_GMAX];
+#pragma unroll
+                    for (int q = 0; q < NNP_GMAX; ++q) aj0[q] = ai0[q] = aj1[q] = ai1[q] = 0.0f;
+                    auto pop = [&]() {  // drop the lowest partner
+                        if constexpr (NW == 1) {
+                            m0 &= m0 - 1ull;
+                        } else {
+                            if (m0 != 0ull) m0 &= m0 - 1ull;
+                            else m1 &= m1 - 1ull;
+                        }
+                    };
+                    for (int t = 0; t < part; ++t) pop();  // this lane's first partner
+                    while (NW == 1 ? m0 != 0ull : (m0 | m1) != 0ull) {
+                        const int bk = (NW == 1 || m0 != 0ull) ? __builtin_ctzll(m0) : 64 + __builtin_ctzll(m1);
+                        for (int t = 0; t < P; ++t) pop();  // next partner of this lane
+                        const bool own = (NW == 1 || bk < 64) ? ((after0 >> bk) & 1ull) != 0ull
+                                                              : ((after1 >> (bk - 64)) & 1ull) != 0ull;
+                        const float4 ka = nb_a[bk], kb = nb_b[bk];
+                        const float kfo = nb_fo[bk];
+                        const float pk3[3] = {ka.x, ka.y, ka.z};
+                        const float dx = pj3[0] - pk3[0], dy = pj3[1] - pk3[1], dz = pj3[2] - pk3[2];
+                        const float rjk = dx * dx + dy * dy + dz * dz;
+                        const float djk = sqrtf(rjk);
+                        const float fjk = nnp_cutoff(djk, T.r_cut);
+                        const float fpn = ja.w * ka.w * fjk, fpo = jfo * kfo * fjk;
+                        if (fpn == 0.0f && fpo == 0.0f) continue;
+                        const float rsn = jb.x + kb.x + rjk, rso = jb.y + kb.y + rjk;
+                        // cosines at j (and at i), new and old
+                        float cjn = 0.0f, cjo = 0.0f, cin = 0.0f, cio = 0.0f;
+                        const float dikn = kb.z, diko = kb.w;
+                        if (fpn != 0.0f) {
+                            cjn = nnp_cos(pj3, pt, pk3, dijn, djk);
+                            if (own) cin = nnp_cos(pt, pj3, pk3, dijn, dikn);
+                        }
+                        if (fpo != 0.0f) {
+                            cjo = nnp_cos(pj3, po, pk3, dijo, djk);
+                            if (own) cio = nnp_cos(po, pj3, pk3, dijo, diko);
+                        }
+                        nnp_visit(G0, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj0, ai0);
+                        if (gb >= 0) nnp_visit(G1, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj1, ai1);
                     }
-                    if (fpo != 0.0f) {
-                        cjo = nnp_cos(pj3, po, pk3, dijo, djk);
-                        if (own) cio = nnp_cos(po, pj3, pk3, dijo, diko);
-                    }
-                    nnp_visit(G0, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj0, ai0);
-                    if (gb >= 0) nnp_visit(G1, rsn, rso, fpn, fpo, cjn, cjo, cin, cio, own, aj1, ai1);
+                    nnp_store<NP>(G0, lp, tact && part == 0, j, i, lane, Dp, aj0, ai0, c == 0);
+                    if (gb >= 0) nnp_store<NP>(G1, lp, tact && part == 0, j, i, lane, Dp, aj1, ai1, c == 0);
                 }
-                nnp_store(G0, lp, tact && part == 0, j, i, lane, Dp, aj0, ai0);
-                if (gb >= 0) nnp_store(G1, lp, tact && part == 0, j, i, lane, Dp, aj1, ai1);
             }
             __syncthreads();
-            for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) Dp[e] += Gp[e];
+            for (int e = tid; e < T.nf * NP; e += NNP_NT) Dp[e] += Gp[e];
             __syncthreads();
-            nnp_forward(T, L, Dp, N, L.eatn, &s_en);
+            nnp_forward<NP>(T, L, Dp, N, L.eatn, &s_en);
             new_en = s_en;
         }
         // Metropolis, src/MCRun.jl:122-132 and src/Exchange.jl:53-57; with
@@ -829,8 +954,8 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
             }
             if (tid == 0) {
                 L.X[i] = tx;
-                L.X[NNP_NP + i] = ty;
-                L.X[2 * NNP_NP + i] = tz;
+                L.X[NP + i] = ty;
+                L.X[2 * NP + i] = tz;
             }
             E = new_en;
             nacc += 1;
@@ -840,18 +965,18 @@ __global__ void __launch_bounds__(NNP_NT) __attribute__((amdgpu_waves_per_eu(DA 
 
     // periodic full rebuild bounds the drift of the incremental features
     if (a.recompute_every > 0 && ((long long)sweep + 1) % a.recompute_every == 0) {
-        nnp_full_build(T, L, Gp, N);
-        nnp_forward(T, L, Gp, N, L.eat, &s_en);
+        nnp_full_build<NP>(T, L, Gp, N);
+        nnp_forward<NP>(T, L, Gp, N, L.eat, &s_en);
         const double dr = __builtin_fabs(E - s_en);
         if (tid == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
         E = s_en;
     }
 
-    float *gX = reinterpret_cast<float *>(a.coords) + (size_t)slot * 3 * NNP_NP;
-    float *gGw = a.feat + (size_t)slot * T.nf * NNP_NP;
-    for (int e = tid; e < 3 * NNP_NP; e += NNP_NT) gX[e] = L.X[e];
-    for (int e = tid; e < T.nf * NNP_NP; e += NNP_NT) gGw[e] = Gp[e];
-    for (int e = tid; e < NNP_NP; e += NNP_NT) a.eatom[(size_t)slot * NNP_NP + e] = L.eat[e];
+    float *gX = reinterpret_cast<float *>(a.coords) + (size_t)slot * 3 * NP;
+    float *gGw = a.feat + (size_t)slot * T.nf * NP;
+    for (int e = tid; e < 3 * NP; e += NNP_NT) gX[e] = L.X[e];
+    for (int e = tid; e < T.nf * NP; e += NNP_NT) gGw[e] = Gp[e];
+    for (int e = tid; e < NP; e += NNP_NT) a.eatom[(size_t)slot * NP + e] = L.eat[e];
     if (tid == 0) {
         a.slot_en[slot] = E;
         a.t_count_atom[2 * Tg] += nacc;

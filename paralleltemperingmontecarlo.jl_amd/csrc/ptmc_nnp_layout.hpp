@@ -5,7 +5,8 @@
 
 namespace ptmc {
 
-constexpr int NNP_NP = 64;     // atoms per configuration (maximum)
+constexpr int NNP_NP = 64;     // atoms per configuration of the narrow slot layout
+constexpr int NNP_NP_WIDE = 128;  // atoms per configuration of the wide layout (one workgroup per CU)
 constexpr int NNP_NT = 512;    // threads per workgroup
 constexpr int NNP_MAXH = 32;   // hidden width (maximum)
 constexpr int NNP_HS = NNP_MAXH + 1;  // padded hidden row stride
@@ -20,12 +21,12 @@ constexpr int NNP_WG = 4;     // angular groups per wave in the move update (max
 static_assert(NNP_GD % 4 == 0, "group descriptors are read in 16-byte words");
 constexpr int NNP_MAXF = 128;  // symmetry functions (maximum)
 
-// LDS layout in floats: X[3][NP] | G[nf][NP] | D[nf][NP] | W1[nf][h1] |
-// H1[NP][HS] | H2[NP][HS] | rn2, fn, ro2, fo, eat, eatn [NP each]
-__host__ __device__ inline size_t nnp_lds_floats(int nf, int h1)
+// LDS layout in floats for slots of np atoms (64 or 128): X[3][np] |
+// G[nf][np] | D[nf][np] | W1[nf][h1] | H1[np][HS] | H2[np][HS] | rn2, fn,
+// ro2, fo, eat, eatn [np each]
+__host__ __device__ inline size_t nnp_lds_floats(int nf, int h1, int np = NNP_NP)
 {
-    return (size_t)3 * NNP_NP + 2 * (size_t)nf * NNP_NP + (size_t)nf * h1 + 2 * (size_t)NNP_NP * NNP_HS +
-           6 * NNP_NP;
+    return (size_t)3 * np + 2 * (size_t)nf * np + (size_t)nf * h1 + 2 * (size_t)np * NNP_HS + 6 * (size_t)np;
 }
 
 }  // namespace ptmc

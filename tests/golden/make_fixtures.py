@@ -234,6 +234,25 @@ def known_answers(ref: pathlib.Path):
     (HERE / "known_answers.json").write_text(json.dumps(out, indent=1))
 
 
+def cu78_start(ref: pathlib.Path):
+    src = (ref / "scripts" / "run_Cu78.jl").read_text()
+    blk = src[src.index("pos_cu78 = ["): src.index("\n]", src.index("pos_cu78 = ["))]
+    pos = _floats(blk)
+    rad = float(re.search(r"SphericalBC\(;\s*radius\s*=\s*([0-9.]+)\s*\*\s*AtoBohr", src).group(1))
+    t = re.search(r"ti\s*=\s*([0-9.]+)\s*\n\s*tf\s*=\s*([0-9.]+)", src)
+    n_traj = int(re.search(r"n_traj\s*=\s*(\d+)", src).group(1))
+    out = dict(
+        source="scripts/run_Cu78.jl:48-127 (positions, Angstrom, centred at :140-149), :155 (container), "
+               ":17-21 (temperatures)",
+        pos_angstrom=[pos[i: i + 3] for i in range(0, len(pos), 3)],
+        radius_angstrom=rad,
+        t_range=[float(t.group(1)), float(t.group(2))],
+        n_traj=n_traj,
+    )
+    assert len(out["pos_angstrom"]) == 78
+    (HERE / "cu78_start.json").write_text(json.dumps(out, indent=1))
+
+
 def main():
     ref = pathlib.Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     ne13_setup(ref)
@@ -243,6 +262,7 @@ def main():
     nnp2_potentialfile(ref)
     nnvt_states(ref)
     known_answers(ref)
+    cu78_start(ref)
     print("fixtures written to", HERE)
 
 
