@@ -579,6 +579,9 @@ def main():
                     help="spread layout: independent run groups on their own streams")
     ap.add_argument("--workgroup-lanes", type=int, default=0,
                     help="0: each ladder in one workgroup; 64/128/256: ladders spread over workgroups of this width")
+    ap.add_argument("--exchange-every", type=int, default=1,
+                    help="replica exchange after every k-th cycle (1: every cycle, the workload's cadence); the "
+                         "spread layout runs the cycles between exchanges in one launch")
     ap.add_argument("--adapt", type=int, default=3000,
                     help="untimed step-size adaptation cycles before --warmup (n_adjust = 100)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -612,7 +615,8 @@ def main():
     def make_engine(workgroup_lanes=args.workgroup_lanes, spread_streams=args.spread_streams):
         opts = pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=1000,
                                 lanes_per_replica=args.lanes, cycles_per_launch=args.cycles_per_launch,
-                                workgroup_lanes=workgroup_lanes, spread_streams=spread_streams)
+                                workgroup_lanes=workgroup_lanes, spread_streams=spread_streams,
+                                exchange_every=args.exchange_every)
         return pt.PTEngine(pt.ELJPotentialEven(LJ_COEFF), pt.NVT(13), pt.SphericalBC(1.0), mc, opts)
 
     t_tune = 0.0
@@ -711,7 +715,7 @@ def main():
     for i, r in enumerate(runs):
         lanes_used.setdefault(str(r["n_atoms"]), eng.lanes_of(i))
     eng.close()
-    cyc_launch = 1 if args.workgroup_lanes else min(args.steps, args.cycles_per_launch)
+    cyc_launch = args.exchange_every if args.workgroup_lanes else min(args.steps, args.cycles_per_launch)
 
     out = {
         "metric": METRIC,
@@ -729,13 +733,15 @@ def main():
         "config": {
             "workload": f"batched LJ parallel tempering: {len(runs)} runs, N in {{13,38,55,75,98}}, 24-64 "
                         "temperatures (geometric 0.05-0.5 eps/k), spherical container 1.2 N^(1/3) sigma, "
-                        "binary32 pair sums + FP64 totals, FP64 recompute every 1000 cycles, even/odd exchange",
+                        "binary32 pair sums + FP64 totals, FP64 recompute every 1000 cycles, even/odd exchange"
+                        + (f" every {args.exchange_every} cycles" if args.exchange_every > 1 else ""),
             "replicas": sum(r["n_temps"] for r in runs),
             "moves_per_step": moves_per_cycle(runs),
             "workgroups": plan["tiles"],
             "launch_groups": plan["launch_groups"],
-            "layout": ("spread: ladders over %d-lane workgroups, one cycle per launch, %d run group(s) on their "
-                       "own streams" % (args.workgroup_lanes, max(args.spread_streams, 1))) if spread
+            "layout": ("spread: ladders over %d-lane workgroups, %d cycle(s) per launch, %d run group(s) on their "
+                       "own streams" % (args.workgroup_lanes, args.exchange_every, max(args.spread_streams, 1)))
+                      if spread
                       else "one workgroup per ladder group, %d cycles per launch" % args.cycles_per_launch,
             "max_fp32_vs_fp64_energy_drift": drift,
             "drift_note": "largest |E_running - E_FP64| (epsilon units) over all configurations and all FP64 "

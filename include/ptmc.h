@@ -72,6 +72,11 @@ typedef struct {
     int32_t spread_streams;    /* spread layout: the runs are cut into this many independent
                                   groups launched on their own streams, so that one group's
                                   cycle overlaps another's (0 or 1: one group); same results */
+    int32_t exchange_every;    /* exchange cadence: replica exchange is attempted at the end of
+                                  the cycles with (cycle + 1) % exchange_every == 0 only, as
+                                  exchange_mode says (0 or 1: every cycle, the reference's
+                                  src/MCRun.jl:203); the spread layout then runs the cycles
+                                  between exchanges in one launch */
 } ptmc_config;
 
 /* One PT run.  Counterparts: TempGrid (src/InputParams.jl:77-95), the start

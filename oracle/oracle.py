@@ -35,7 +35,7 @@ class orc_params(C.Structure):
         ("seed", C.c_uint64), ("run", C.c_uint32),
         ("sweep0", C.c_int64), ("cycle0", C.c_int64),
         ("en_min", C.c_double), ("delta_en_hist", C.c_double), ("v_min", C.c_double), ("delta_v_hist", C.c_double),
-        ("separated", C.c_int32), ("plant_decision", C.c_int32),
+        ("separated", C.c_int32), ("plant_decision", C.c_int32), ("exchange_every", C.c_int32),
     ]
 
 
@@ -390,7 +390,8 @@ class OracleRun:
     """One PT run advanced by the CPU restatement."""
 
     def __init__(self, n_atoms, beta, x0, *, pot: orc_pot, bc="sphere", radius=0.0, box=0.0, ensemble="nvt",
-                 pressure=0.0, mode=None, fp32=False, exchange_mode=0, n_adjust=100, mc_sample=1, n_bin=100,
+                 pressure=0.0, mode=None, fp32=False, exchange_mode=0, exchange_every=1, n_adjust=100, mc_sample=1,
+                 n_bin=100,
                  min_acc=0.4, max_acc=0.6, seed=1234, run_key=0, max_displ=(0.1, 0.1), trace_cycles=0,
                  box_h=None, separated=False, max_displ_sep=(0.1, 0.1)):
         mode = dict(REFERENCE if mode is None else mode)
@@ -406,6 +407,7 @@ class OracleRun:
                                                           mode["math_lib"])
         p.fp32 = 1 if fp32 else 0
         p.exchange_mode = exchange_mode
+        p.exchange_every = exchange_every
         p.n_adjust, p.mc_sample, p.n_bin = n_adjust, mc_sample, n_bin
         p.min_acc, p.max_acc = min_acc, max_acc
         p.radius2 = radius * radius

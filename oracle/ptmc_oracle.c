@@ -1010,8 +1010,8 @@ int orc_run_cycles(const orc_params *p, const orc_pot *pot, orc_state *s, int64_
             }
         }
 
-        /* exchange, src/MCRun.jl:203-205 */
-        if (R > 1) {
+        /* exchange, src/MCRun.jl:203-205 (attempted every exchange_every cycles) */
+        if (R > 1 && (p->exchange_every <= 1 || (sweep + 1) % (uint32_t)p->exchange_every == 0)) {
             if (p->exchange_mode == ORC_EXC_EVEN_ODD) {
                 for (int n = (int)(sweep & 1u); n < R - 1; n += 2) try_exchange(p, pot, s, n, sweep);
             } else {

@@ -48,6 +48,9 @@ typedef struct {
      * show a decision check catches it: 0 none, 1 beta doubled, 2 exponent
      * sign flipped, 3 uniform taken as 1 - u */
     int32_t plant_decision;
+    /* exchange cadence: attempts only at the end of cycles with
+     * (sweep + 1) % exchange_every == 0 (0 or 1: every cycle) */
+    int32_t exchange_every;
 } orc_params;
 
 /* RuNNer-style neural-network potential (src/EnergyEvaluation.jl:1182-1275,

@@ -248,8 +248,9 @@ __global__ void __launch_bounds__(LANES) eam_kernel(SweepArgs a)
         }
         __syncthreads();
 
-        // replica exchange, src/Exchange.jl:103-155 and src/MCRun.jl:203-205
-        if (active && q == 0 && R > 1) {
+        // replica exchange, src/Exchange.jl:103-155 and src/MCRun.jl:203-205,
+        // attempted every exchange_every cycles
+        if (active && q == 0 && R > 1 && (uint32_t)(sweep + 1u) % (uint32_t)a.exchange_every == 0u) {
             int n = -1;
             if (a.exchange_mode == EXC_EVEN_ODD) {
                 if ((t & 1) == (int)(sweep & 1u) && t + 1 < R) n = t;

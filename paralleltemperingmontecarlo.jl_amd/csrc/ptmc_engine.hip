@@ -755,6 +755,7 @@ SweepArgs base_args(ptmc_engine *h)
     a.seed = h->cfg.seed;
     a.ensemble = h->cfg.ensemble;
     a.exchange_mode = h->cfg.exchange_mode;
+    a.exchange_every = h->cfg.exchange_every > 1 ? h->cfg.exchange_every : 1;
     a.n_adjust = h->cfg.n_adjust;
     a.mc_sample = h->cfg.mc_sample;
     a.n_bin = h->cfg.n_bin;
@@ -1677,9 +1678,10 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
         done = n_cycles;
     }
     if (h->spread) {
-        // one launch of the tiles per cycle, then the per-ladder exchange,
-        // adaptation and sampling; a last launch without moves applies a
-        // recompute due after the final cycle
+        // one launch of the tiles runs the cycles up to the next exchange
+        // (exchange_every; one cycle with the default), then the per-ladder
+        // exchange, adaptation and sampling of that cycle; a last launch
+        // without moves applies a recompute due after the final cycle
         const bool tracing = h->trace_pending > 0;
         if (tracing && n_cycles > h->trace_cycles)
             return fail(h, PTMC_E_STATE, "traced cycles must be run in one ptmc_cycles call of at most %d cycles",
@@ -1692,13 +1694,17 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
             HIPCHK(h, hipEventRecord(h->fork, h->stream));
             for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamWaitEvent(stream_of(g), h->fork, 0));
         }
-        for (int64_t c = 0; c < n_cycles; ++c) {
+        const int64_t k = h->cfg.exchange_every > 1 ? h->cfg.exchange_every : 1;
+        for (int64_t c = 0; c < n_cycles;) {
+            // cycles up to and including the next with (sweep + 1) % k == 0
+            const int64_t n = std::min<int64_t>(k - h->sweep_counter % k, n_cycles - c);
+            const bool exch = (h->sweep_counter + n) % k == 0;
             for (size_t g = 0; g < ng; ++g) {
                 const Launch &L = h->launches[g];
                 SweepArgs a = base_args(h);
                 a.sweep0 = h->sweep_counter;
                 a.cycle0 = h->phase_counter;
-                a.n_cycles = 1;
+                a.n_cycles = (int32_t)n;
                 a.sampling = h->production;
                 a.init_run = -1;
                 a.run_list = h->d_run_list.p + L.run0;
@@ -1711,13 +1717,19 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                     a.trace_cycle = (int32_t)c;
                 }
                 if ((rc = launch_one(h, g, a, stream_of(g), false))) return rc;
-                void *args[] = {&a};
-                HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(L.nruns), dim3(64), args, 0, stream_of(g)));
+                if (exch) {
+                    // the exchange cycle is the launch's last
+                    a.sweep0 = h->sweep_counter + n - 1;
+                    a.cycle0 = h->phase_counter + n - 1;
+                    void *args[] = {&a};
+                    HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(L.nruns), dim3(64), args, 0, stream_of(g)));
+                }
             }
             // bound the event pool on long calls: the oldest launches are long done
             if (h->pending.size() >= 512 && (rc = collect_timings(h, 256))) return rc;
-            h->sweep_counter += 1;
-            h->phase_counter += 1;
+            h->sweep_counter += n;
+            h->phase_counter += n;
+            c += n;
         }
         if (n_cycles > 0 && h->cfg.recompute_every > 0 && h->sweep_counter % h->cfg.recompute_every == 0) {
             for (size_t g = 0; g < ng; ++g) {

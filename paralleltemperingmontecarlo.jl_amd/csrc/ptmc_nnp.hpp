@@ -1020,7 +1020,7 @@ __global__ void nnp_cycle_kernel(SweepArgs a, int n_runs)
             a.temp_slot[tb + n + 1] = sa;
         }
     };
-    if (R > 1) {
+    if (R > 1 && (sweep + 1u) % (uint32_t)a.exchange_every == 0u) {
         if (a.exchange_mode == EXC_EVEN_ODD) {
             for (int n = (int)(sweep & 1u); n + 1 < R; n += 2) try_pair(n);
         } else {

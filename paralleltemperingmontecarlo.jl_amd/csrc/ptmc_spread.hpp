@@ -259,7 +259,23 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             box = a.slot_box[slot];
             boxh = a.slot_boxh[slot];
         }
-        if (!a.recompute_done && a.recompute_every > 0 && a.sweep0 > 0 && a.sweep0 % a.recompute_every == 0) {
+        // The temperature label stays until the exchange after the launch's
+        // last cycle: its step sizes, window counters and energy sums live in
+        // registers (every lane of the cluster holds the same values) and go
+        // back to memory at the end.
+        const int T = tb + t;
+        const double beta = a.t_beta[T];
+        double delta = a.t_displ[2 * T];
+        double dvol = ENS == ENS_NPT ? a.t_displ[2 * T + 1] : 0.0;
+        long long win_atom = a.t_count_atom[2 * T + 1], add_atom = 0;
+        long long win_vol = ENS == ENS_NPT ? a.t_count_vol[2 * T + 1] : 0, add_vol = 0;
+        double ham0 = a.t_ham[2 * T], ham1 = a.t_ham[2 * T + 1];
+        const int ncyc = a.n_cycles > 0 ? a.n_cycles : 1;  // n_cycles = 0: the pending recompute only
+        for (int cc = 0; cc < ncyc; ++cc) {
+          const uint32_t sweep = (uint32_t)(a.sweep0 + cc);
+          // FP64 recompute due after the previous cycle (after its sampling)
+          if ((cc > 0 || !a.recompute_done) && a.recompute_every > 0 && sweep > 0u &&
+              sweep % (uint32_t)a.recompute_every == 0u) {
             const double rc64 = PBC ? box_rcut(kind, box, boxh) : 0.0;
             double ef = spread_pair_sum64<Real, PK, KC, BC, ENS, LANES>(X, gb, q, N, KL, kls,
                                                                        make_box<double>(box, boxh), 1.0, 1.0, 1.0,
@@ -268,11 +284,8 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             const double dr = __builtin_fabs(E - ef);
             if (q == 0 && dr > a.slot_drift[slot]) a.slot_drift[slot] = dr;
             E = ef;
-        }
-        if (a.n_cycles > 0) {
-            const uint32_t sweep = (uint32_t)a.sweep0;
-            const double beta = a.t_beta[tb + t];
-            const double delta = a.t_displ[2 * (tb + t)];
+          }
+          if (a.n_cycles > 0) {
             const Real delta_r = (Real)delta;
             const float beta_l2e = (float)beta * 1.4426950408889634f;
             // binary32 displacement (u - 1/2) delta as one fma of the 24-bit
@@ -459,10 +472,9 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 if constexpr (ENS == ENS_NPT) {
                     // uniform volume move: r^2 of the current positions scaled by s^2
                     const bool vm = i >= (uint32_t)N;
-                    const int T = tb + t;
                     double sx = 1.0;
                     if (vm) {
-                        const double y = ((double)d0.ux - 0.5) * a.t_displ[2 * T + 1];
+                        const double y = ((double)d0.ux - 0.5) * dvol;
                         double scale = det_exp(y / 3.0);
                         if (box >= a.t_maxbox[T] && scale > 1.0) scale = 1.0;
                         sx = scale;
@@ -520,27 +532,77 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         }
                     }
                 }
-                if (a.trace_de != nullptr && q == 0 && a.trace_cycle < a.trace_cycles) {
-                    const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle) * nsteps + m;
+                if (a.trace_de != nullptr && q == 0 && a.trace_cycle + cc < a.trace_cycles) {
+                    const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle + cc) * nsteps + m;
                     a.trace_de[ti] = dE_used;
                     a.trace_acc[ti] = (int8_t)accept;
                     a.trace_idx[ti] = (int32_t)i;
                 }
             }
-            if (q == 0) {
-                a.t_count_atom[2 * (tb + t)] += nacc;
-                a.t_count_atom[2 * (tb + t) + 1] += nacc;
-                if (ENS == ENS_NPT) {
-                    a.t_count_vol[2 * (tb + t)] += nvacc;
-                    a.t_count_vol[2 * (tb + t) + 1] += nvacc;
+            win_atom += nacc;
+            add_atom += nacc;
+            win_vol += nvacc;
+            add_vol += nvacc;
+            if ((sweep + 1u) % (uint32_t)a.exchange_every != 0u) {
+                // no exchange after this cycle: its step-size adaptation and
+                // sampling here (src/Exchange.jl:198-263, src/Sampling.jl:29-125,
+                // :258-280, :330-371), as spread_cycle_kernel makes them
+                const long long pidx = a.cycle0 + cc + 1;
+                if (pidx % a.n_adjust == 0) {
+                    const double ar = (double)win_atom / (double)((long long)a.n_adjust * N);
+                    if (ar < a.min_acc) delta *= 0.9;
+                    else if (ar > a.max_acc) delta *= 1.1;
+                    win_atom = 0;
+                    if (ENS == ENS_NPT) {
+                        const double av = (double)win_vol / (double)a.n_adjust;
+                        if (av < a.min_acc) dvol *= 0.9;
+                        else if (av > a.max_acc) dvol *= 1.1;
+                        win_vol = 0;
+                    }
+                }
+                if (a.sampling && pidx % a.mc_sample == 0) {
+                    if (ENS == ENS_NPT) {
+                        const double h = E + ri.pressure * box_volume(kind, box, boxh);
+                        ham0 += h;
+                        ham1 += h * h;
+                        const double ie = __builtin_floor((E - ri.en_min) / ri.delta_en) + 1.0;
+                        const double iv =
+                            __builtin_floor((box_hist_volume(kind, box, boxh) - ri.v_min) / ri.delta_v) + 1.0;
+                        const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
+                        const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
+                        if (q == 0)
+                            atomicAdd(&a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)],
+                                      1u);
+                    } else {
+                        ham0 += E;
+                        ham1 += E * E;
+                        const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
+                        const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
+                        if (q == 0) atomicAdd(&a.t_hist[(size_t)T * a.hist_stride + (h - 1)], 1u);
+                    }
                 }
             }
+          }
         }
         if (q == 0) {
             a.slot_en[slot] = E;
             if constexpr (PBC) {
                 a.slot_box[slot] = box;
                 a.slot_boxh[slot] = boxh;
+            }
+            if (a.n_cycles > 0) {
+                a.t_displ[2 * T] = delta;
+                a.t_count_atom[2 * T] += add_atom;
+                a.t_count_atom[2 * T + 1] = win_atom;
+                if (ENS == ENS_NPT) {
+                    a.t_displ[2 * T + 1] = dvol;
+                    a.t_count_vol[2 * T] += add_vol;
+                    a.t_count_vol[2 * T + 1] = win_vol;
+                }
+                if (a.sampling) {
+                    a.t_ham[2 * T] = ham0;
+                    a.t_ham[2 * T + 1] = ham1;
+                }
             }
         }
     }
@@ -586,7 +648,7 @@ __global__ void __launch_bounds__(LANES) spread_init_kernel(SweepArgs a)
 
 // Exchange, step-size adaptation and sampling of one cycle of every ladder
 // (src/MCRun.jl:203-212, :248-250), one 64-lane workgroup per run, after the
-// cycle's moves.  The ladder's energies (NPT: and volumes) and slot map are
+// moves of a cycle that ends with an exchange (every cycle by default).  The ladder's energies (NPT: and volumes) and slot map are
 // staged in LDS; the even/odd pairs of a cycle are disjoint, so lanes take
 // them in parallel.  Draws, tests and counters are those of the exchange in
 // sweep_kernel (src/Exchange.jl:103-155, :198-263; src/Sampling.jl:29-125,
@@ -639,7 +701,7 @@ __global__ void __launch_bounds__(64) spread_cycle_kernel(SweepArgs a)
             s_v[n + 1] = v0;
         }
     };
-    if (R > 1) {
+    if (R > 1 && (sweep + 1u) % (uint32_t)a.exchange_every == 0u) {
         if (a.exchange_mode == EXC_EVEN_ODD) {
             for (int n = (int)(sweep & 1u) + 2 * lane; n + 1 < R; n += 128) try_pair(n);
         } else if (lane == 0) {

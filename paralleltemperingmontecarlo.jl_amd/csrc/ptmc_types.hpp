@@ -149,6 +149,7 @@ struct SweepArgs {
     int32_t n_cycles;
     int32_t ensemble;
     int32_t exchange_mode;
+    int32_t exchange_every;    // exchange attempted after cycles with (sweep + 1) % exchange_every == 0
     int32_t n_adjust;
     int32_t mc_sample;
     int32_t n_bin;

@@ -58,6 +58,7 @@ class ptmc_config(C.Structure):
         ("seed", C.c_uint64),
         ("workgroup_lanes", C.c_int32),
         ("spread_streams", C.c_int32),
+        ("exchange_every", C.c_int32),
     ]
 
 

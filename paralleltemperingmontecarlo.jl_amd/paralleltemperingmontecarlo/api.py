@@ -550,7 +550,10 @@ class EngineOptions:
     Horner form in a spherical container in NVT, or in a cubic or
     rectangular box in NVT or NPT with uniform volume moves), with the same
     results; spread_streams > 1
-    cuts the runs into that many groups launched on their own streams."""
+    cuts the runs into that many groups launched on their own streams.
+    exchange_every: replica exchange is attempted (as `exchange` says) only
+    after the cycles with (cycle + 1) % exchange_every == 0; 1 (default)
+    attempts it after every cycle as src/MCRun.jl:203 does."""
 
     precision: str = "fp64"
     exchange: str = "reference"
@@ -562,6 +565,7 @@ class EngineOptions:
     device: int = 0
     workgroup_lanes: int = 0
     spread_streams: int = 0
+    exchange_every: int = 1
     max_displ: tuple = (0.1, 0.1)
 
 
@@ -643,6 +647,7 @@ def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _l
     cfg.cycles_per_launch = opts.cycles_per_launch
     cfg.workgroup_lanes = opts.workgroup_lanes
     cfg.spread_streams = opts.spread_streams
+    cfg.exchange_every = opts.exchange_every
     cfg.device = opts.device
     cfg.seed = opts.seed & 0xFFFFFFFFFFFFFFFF
     return cfg

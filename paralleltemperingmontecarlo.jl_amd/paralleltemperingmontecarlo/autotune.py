@@ -15,9 +15,12 @@ of runs, so it is measured:
 - for the spread layout, the number of independent run groups launched on
   their own streams (one group's cycle tail overlaps another group's work).
 
-The exchange cadence is not a throughput choice here: the exchange costs well
-under one per cent of a cycle in both layouts, so it is left to the physics
-(the reference's p = 0.1 per cycle or even/odd pairs).
+- the exchange cadence (exchange after every k-th cycle): fewer exchanges
+  let the spread layout run k cycles per launch, but mix the ladder less, so
+  throughput alone cannot choose it.  ``tune`` takes the candidate cadences
+  the user allows and returns the most frequent exchange whose throughput is
+  within ``cadence_tolerance`` (3 %) of the fastest candidate's: exchanges
+  are kept as often as they are nearly free.
 
 Search: the layouts are timed with the starting lanes table, then a
 coordinate descent over the distinct atom counts (largest first) runs in the
@@ -30,9 +33,9 @@ scratch engine, from the state reached after an equilibration phase.
 
     best = tune(make_engine, runs)
     # runs: list of dicts with n_atoms, beta, x0, bc (and optionally pressure, key)
-    # make_engine(workgroup_lanes, spread_streams): a fresh PTEngine
+    # make_engine(workgroup_lanes, spread_streams[, exchange_every]): a fresh PTEngine
     # best: dict(table={n_atoms: lanes}, workgroup_lanes=..., spread_streams=...,
-    #            lanes_used={n_atoms: lanes}, rate=moves/s)
+    #            exchange_every=..., lanes_used={n_atoms: lanes}, rate=moves/s)
 
 ``tune_lanes`` is the lanes-only descent in a fixed layout.
 """
@@ -187,14 +190,24 @@ def tune_lanes(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500
     return table
 
 
+def choose_cadence(rates: dict, tolerance: float = 0.03) -> int:
+    """The most frequent exchange (smallest k) whose rate is within
+    `tolerance` of the fastest: rates = {k: moves per second}."""
+    best = max(rates.values())
+    return min(k for k, r in rates.items() if r >= (1.0 - tolerance) * best)
+
+
 def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, rounds: int = 1,
          choices=LANE_CHOICES, layouts=LAYOUTS, start: dict | None = None, adapt_cycles: int = 1000,
-         log: Callable | None = None) -> dict:
+         log: Callable | None = None, cadences=(1,), cadence_tolerance: float = 0.03) -> dict:
     """Layout and lanes per configuration: layouts timed with the starting
     table, lanes descent in the best layout, layouts timed again with the
-    tuned table.  make_engine(workgroup_lanes, spread_streams) builds a fresh
-    engine.  Returns the best candidate as a dict (table, workgroup_lanes,
-    spread_streams, lanes_used, rate)."""
+    tuned table; then, if more than one exchange cadence is allowed, each
+    timed in the chosen layout (``choose_cadence``).  make_engine(
+    workgroup_lanes, spread_streams) builds a fresh engine (with cadences
+    besides (1,): make_engine(workgroup_lanes, spread_streams,
+    exchange_every)).  Returns the best candidate as a dict (table,
+    workgroup_lanes, spread_streams, exchange_every, lanes_used, rate)."""
     sizes = sorted({r["n_atoms"] for r in runs}, reverse=True)
     table = dict(start if start is not None else start_table(sizes))
     state = _adapted_state(make_engine, runs, table, adapt_cycles, layouts[0]) if adapt_cycles > 0 else None
@@ -222,5 +235,15 @@ def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, roun
     again, _, used = _rate(make_engine, runs, table, warmup, cycles, state, layout)
     if log:
         log(f"best layout {layout} lanes {table}: {rate:.4g} / {again:.4g} moves/s")
-    return dict(table=table, workgroup_lanes=layout[0], spread_streams=layout[1], lanes_used=used,
-                rate=0.5 * (rate + again))
+    rate = 0.5 * (rate + again)
+    every = 1
+    if tuple(cadences) != (1,):
+        by_k = {}
+        for k in sorted(set(cadences)):
+            by_k[k], _, _ = _rate(make_engine, runs, table, warmup, cycles, state, tuple(layout) + (k,))
+            if log:
+                log(f"  exchange every {k}: {by_k[k]:.4g} moves/s")
+        every = choose_cadence(by_k, cadence_tolerance)
+        rate = by_k[every]
+    return dict(table=table, workgroup_lanes=layout[0], spread_streams=layout[1], exchange_every=every,
+                lanes_used=used, rate=rate)

@@ -16,6 +16,15 @@ def test_start_table():
     assert start_table([64, 63]) == {64: 16, 63: 8}
 
 
+def test_choose_cadence():
+    """The most frequent exchange within the tolerance of the fastest cadence."""
+    from paralleltemperingmontecarlo.autotune import choose_cadence
+
+    assert choose_cadence({1: 100.0, 2: 110.0, 4: 112.0, 8: 113.0}) == 2
+    assert choose_cadence({1: 100.0, 4: 101.0}) == 1
+    assert choose_cadence({1: 50.0, 10: 100.0}, tolerance=0.1) == 10
+
+
 def _runs(seed=3):
     rng = np.random.default_rng(seed)
     out = []
@@ -93,3 +102,33 @@ def test_layout_tuner_returns_a_runnable_plan():
         assert eng.lanes_of(i) == best["lanes_used"][r["n_atoms"]]
     eng.close()
     assert set(LAYOUTS) >= {(0, 0), (64, 2), (128, 3)}
+
+
+@pytest.mark.gpu
+def test_cadence_tuner_returns_an_allowed_cadence():
+    """tune() with exchange cadences 1 and 4 in the spread layout: the
+    returned cadence is one of them, and the engine it describes runs."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    import paralleltemperingmontecarlo as pt
+    from paralleltemperingmontecarlo.autotune import tune
+
+    runs = [dict(r, bc=pt.SphericalBC(r["radius"])) for r in _runs()]
+    mc = pt.MCParams(100, 1, 1, n_adjust=20)
+
+    def make(wg=0, streams=0, every=1):
+        return pt.PTEngine(pt.ELJPotentialEven([-4.0, 0.0, 0.0, 4.0]), pt.NVT(13), pt.SphericalBC(1.0), mc,
+                           pt.EngineOptions(precision="fp32", exchange="even_odd", seed=5, workgroup_lanes=wg,
+                                            spread_streams=streams, exchange_every=every))
+
+    best = tune(make, runs, warmup=5, cycles=40, rounds=1, adapt_cycles=40, layouts=((64, 1),), cadences=(1, 4))
+    assert best["exchange_every"] in (1, 4) and best["rate"] > 0
+    eng = make(best["workgroup_lanes"], best["spread_streams"], best["exchange_every"])
+    for r in runs:
+        eng.add_run(r["n_atoms"], r["beta"], r["x0"][None], r["bc"], run_key=r["key"],
+                    lanes=best["table"].get(r["n_atoms"], 0))
+    eng.prepare()
+    eng.start_phase(False)
+    eng.cycles(9)
+    eng.sync()
+    eng.close()

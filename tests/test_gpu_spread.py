@@ -40,14 +40,14 @@ def _lj_runs(n_atoms, n_temps, seed=7):
     return np.array(pts), rad, beta
 
 
-def _run_layout(specs, wg, precision, pot, exchange, cycles, recompute=0, seed=99, streams=0):
+def _run_layout(specs, wg, precision, pot, exchange, cycles, recompute=0, seed=99, streams=0, every=1):
     """specs: (N, R, lanes, x0, radius, beta, key).  Equilibration with step
     adaptation, histogram bounds, production with sampling; returns per-run
     (state, counters, histograms, drift) and the engine's plan."""
     mc = pt.MCParams(cycles, 1, 1, n_adjust=10, mc_sample=1, n_bin=50)
     eng = PTEngine(pot, pt.NVT(specs[0][0]), pt.SphericalBC(1.0), mc,
                    pt.EngineOptions(precision=precision, exchange=exchange, seed=seed, workgroup_lanes=wg,
-                                    recompute_every=recompute, spread_streams=streams))
+                                    recompute_every=recompute, spread_streams=streams, exchange_every=every))
     for (n, R, lanes, x0, rad, beta, key) in specs:
         eng.add_run(n, beta, x0[None], pt.SphericalBC(rad), run_key=key, lanes=lanes)
     eng.prepare()
@@ -110,6 +110,77 @@ def test_fp64_spread_equals_one_workgroup_layout(ne13, exchange):
     b, plan = _run_layout(specs, 64, "fp64", pot, exchange, 80)
     assert plan["tiles"] == 12
     _same(a, b)
+
+
+@pytest.mark.parametrize("wg,streams,every", [(64, 0, 3), (128, 2, 4), (64, 3, 7)])
+def test_exchange_cadence_layouts_agree(wg, streams, every):
+    """Exchange every 3, 4 or 7 cycles: the spread layout runs the cycles
+    between exchanges in one launch (adaptation every 10 cycles and sampling
+    inside it), the one-workgroup layout skips the exchange in the kernel; 61
+    cycles in two calls of 30 and 31 (so calls end between exchanges), with
+    FP64 recomputes every 20 cycles: identical results bit for bit."""
+    specs = []
+    for key, (n, R, lanes) in enumerate([(13, 24, 2), (38, 40, 4), (55, 33, 8), (98, 44, 16), (13, 7, 1)]):
+        x, rad, beta = _lj_runs(n, R, seed=key + 11)
+        specs.append((n, R, lanes, x, rad, beta, key))
+    pot = pt.ELJPotentialEven(LJ_COEFF)
+    a, _ = _run_layout(specs, 0, "fp32", pot, "even_odd", 61, recompute=20, every=every)
+    b, plan = _run_layout(specs, wg, "fp32", pot, "even_odd", 61, recompute=20, streams=streams, every=every)
+    _same(a, b)
+    c, _ = _run_layout(specs, wg, "fp32", pot, "even_odd", 61, recompute=20, streams=streams, every=1)
+    # fewer exchanges than with the default cadence
+    assert sum(ca["count_exc"][:, 0].sum() for (_, ca, _, _) in a) < sum(cc["count_exc"][:, 0].sum()
+                                                                            for (_, cc, _, _) in c)
+    assert sum(h.sum() for (_, _, h, _) in b) > 0
+
+
+@pytest.mark.parametrize("wg,every", [(0, 4), (64, 4), (64, 5)])
+def test_fp64_exchange_cadence_against_oracle(oracle, ne13, wg, every):
+    """One Ne13 FP64 ladder exchanging every 4 or 5 cycles, in one workgroup
+    or on 64-lane tiles, against the oracle: every move's index, decision and
+    dE over 25 traced cycles, then positions, energies, step sizes, counters,
+    energy sums and histograms bit for bit."""
+    temp = pt.TempGrid(4.0, 16.0, 16)
+    mc = pt.MCParams(100, 16, 13, n_adjust=10, n_bin=30)
+    bc = pt.SphericalBC(radius=ne13["radius"])
+    eng = PTEngine(pt.ELJPotentialEven(ne13["coeff"]), pt.NVT(13), bc, mc,
+                   pt.EngineOptions(precision="fp64", exchange="even_odd", seed=2468, workgroup_lanes=wg,
+                                    lanes_per_replica=4, exchange_every=every))
+    run = eng.add_run(13, temp.beta_grid, ne13["pos"][None], bc, run_key=8)
+    eng.prepare()
+    print(f"fp64 exchange cadence wg={wg} every={every}", flush=True)
+    orc = oracle.OracleRun(13, temp.beta_grid, ne13["pos"], pot=oracle.make_pot("elj_even", ne13["coeff"]),
+                           radius=ne13["radius"], mode=oracle.engine_mode(4), exchange_mode=2, exchange_every=every,
+                           n_adjust=10, n_bin=30, seed=2468, run_key=8)
+    eng.set_trace(25)
+    eng.start_phase(False)
+    eng.cycles(25)
+    eng.sync()
+    de, acc, idx = eng.trace(run, 25)
+    orc.start_phase(False)
+    tr = orc.cycles(25, trace=True)
+    assert np.array_equal(idx, tr["idx"])
+    assert np.array_equal(acc, tr["acc"])
+    assert np.array_equal(de, tr["de"])
+    e = eng.energies(run)
+    eng.set_histogram(run, float(e.min()), float(e.max()))
+    orc.set_histogram(float(e.min()), float(e.max()))
+    eng.reset_counters()
+    orc.reset_counters()
+    eng.start_phase(True)
+    orc.start_phase(True)
+    eng.cycles(37)
+    eng.sync()
+    orc.cycles(37)
+    x, en, _ = eng.state(run)
+    ctr = eng.counters(run)
+    assert np.array_equal(x, orc.x.reshape(x.shape))
+    assert np.array_equal(en, orc.en_tot)
+    for k in ("max_displ", "count_atom", "count_exc", "ham"):
+        assert np.array_equal(ctr[k], getattr(orc, k)), k
+    assert np.array_equal(eng.histograms(run), orc.hist)
+    assert ctr["count_exc"][:, 0].sum() > 0
+    eng.close()
 
 
 def test_fp64_spread_bit_exact_against_oracle(oracle, ne13):

@@ -278,6 +278,21 @@ def test_exchange_modes(oracle, ne13):
     assert list(run2.count_exc[:, 0]) == [50, 100, 100, 100, 100, 100, 100, 50]
 
 
+def test_exchange_cadence(oracle, ne13):
+    """exchange_every = 3: attempts only after the cycles with (cycle + 1) % 3
+    == 0 (cycles 2, 5, 8, ...), the even/odd pairs by that cycle's parity;
+    between them no temperature changes hands, and cadence 1 is the default."""
+    run = _ne13_run(oracle, ne13, oracle.engine_mode(1), exchange_mode=2, exchange_every=3)
+    run.cycles(30)
+    # cycles 2, 8, 14, 20, 26 (even): pairs (0,1),(2,3),(4,5),(6,7); 5, 11, 17, 23, 29 (odd): (1,2),(3,4),(5,6)
+    assert list(run.count_exc[:, 0]) == [5, 10, 10, 10, 10, 10, 10, 5]
+    a = _ne13_run(oracle, ne13, oracle.engine_mode(1), exchange_mode=1)
+    b = _ne13_run(oracle, ne13, oracle.engine_mode(1), exchange_mode=1, exchange_every=1)
+    a.cycles(40)
+    b.cycles(40)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.count_exc, b.count_exc)
+
+
 def test_step_adaptation(oracle, ne13):
     """update_max_stepsize!: after n_adjust cycles delta is scaled by exactly 0.9 or 1.1 (or kept)."""
     run = _ne13_run(oracle, ne13, oracle.engine_mode(1), n_adjust=50)
