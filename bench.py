@@ -78,12 +78,14 @@ def moves_per_cycle(runs):
     return sum(r["n_temps"] * r["n_atoms"] for r in runs)
 
 
-def kernel_symbol(kl, wgs):
+def kernel_symbol(kl, wgs, multi_cycle=False):
     """Demangled name of the sweep kernel instantiation a launch group runs
     (binary32, Lennard-Jones pair form, spherical container, NVT); kl = 0:
-    the spread layout's kernel (lanes per configuration read per tile)."""
+    the spread layout's kernel (lanes per configuration read per tile; the
+    multi-cycle variant when exchanges come every k > 1 cycles)."""
     if kl == 0:
-        return f"void ptmc::spread_sweep_kernel<float, 4, 4, 0, 0, {wgs}>(ptmc::SweepArgs)"
+        mc = "true" if multi_cycle else "false"
+        return f"void ptmc::spread_sweep_kernel<float, 4, 4, 0, 0, {wgs}, {mc}>(ptmc::SweepArgs)"
     return f"void ptmc::sweep_kernel<float, 4, 4, 0, 0, {kl}, {wgs}>(ptmc::SweepArgs)"
 
 
@@ -631,8 +633,11 @@ def main():
         from paralleltemperingmontecarlo.autotune import tune
 
         t0 = time.perf_counter()
+        # candidates timed as calls of the timed region's length when it is
+        # shorter than the tuning sample (its per-call costs then count)
         tuned = tune(make_engine, [dict(r, bc=pt.SphericalBC(r["radius"])) for r in runs], cycles=1500,
-                     log=lambda m: print("tune:", m, file=sys.stderr, flush=True))
+                     log=lambda m: print("tune:", m, file=sys.stderr, flush=True),
+                     call_cycles=args.steps if args.steps < 1500 else None)
         t_tune = time.perf_counter() - t0
         table = tuned["table"]
         args.workgroup_lanes, args.spread_streams = tuned["workgroup_lanes"], tuned["spread_streams"]
@@ -767,11 +772,12 @@ def main():
             "peak": PEAK_FP32,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "traffic": traffic_per_launch(kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
+            "traffic": traffic_per_launch(kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"],
+                                                        args.exchange_every > 1),
                                           cyc_launch, plan["tiles"] / max(plan["launch_groups"], 1) if spread
                                           else None),
             "cycles_per_launch": cyc_launch,
-            "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"]),
+            "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"], args.exchange_every > 1),
             "kernel_flops_per_launch": dom_flops / max(gi["launches"], 1),
             "kernel_avg_launch_ms": gi["ms"] / max(gi["launches"], 1),
             "kernel_launches": gi["launches"],

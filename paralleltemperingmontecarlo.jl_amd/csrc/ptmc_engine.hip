@@ -368,7 +368,8 @@ int plan_spread(ptmc_engine *h)
                     "cubic / rectangular box (NVT, NPT with uniform volume moves)");
     if (h->cfg.spread_streams < 0 || h->cfg.spread_streams > 4)
         return fail(h, PTMC_E_ARG, "spread_streams must be 0 to 4");
-    const KernelPair kp = lookup_spread(prec, h->pk, h->kc, W, sphere ? BC_SPHERE : BC_CUBIC, h->cfg.ensemble);
+    const KernelPair kp = lookup_spread(prec, h->pk, h->kc, W, sphere ? BC_SPHERE : BC_CUBIC, h->cfg.ensemble,
+                                        h->cfg.exchange_every > 1);
     if (!kp.sweep) return fail(h, PTMC_E_UNSUPPORTED, "no spread kernel for pair form %d/%d", h->pk, h->kc);
     int dev = 0;
     HIPCHK(h, hipGetDevice(&dev));

@@ -38,7 +38,8 @@ KernelPair lookup_eam(int precision, int bc, int kl, int lanes);
 // spread layout (spherical container in NVT; cubic / rectangular boxes, bc
 // BC_CUBIC, in NVT and NPT): 64-, 128- or 256-lane workgroups, lanes per
 // configuration read from each tile; and its per-ladder cycle kernel
-KernelPair lookup_spread(int precision, int pk, int kc, int lanes, int bc, int ens);
+// multi_cycle: the variant that runs several cycles between exchanges
+KernelPair lookup_spread(int precision, int pk, int kc, int lanes, int bc, int ens, bool multi_cycle);
 const void *spread_cycle_fn();
 
 }  // namespace ptmc

@@ -207,7 +207,14 @@ __device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls
 // adds the uniform volume move as the cycle's (N + 1)-th move
 // (src/MCMoves.jl:169-177, :277-287; energy src/EnergyEvaluation.jl:397-474;
 // acceptance src/Exchange.jl:58-71).
-template <typename Real, int PK, int KC, int BC, int ENS, int LANES>
+//
+// MC (exchange after every k-th cycle, k > 1): one launch runs the cycles up
+// to the next exchange; the temperature label of a configuration stays in
+// between, so its step sizes, window counters and energy sums live in
+// registers and the cycles without an exchange adapt and sample here.  MC =
+// false compiles the one-cycle launch of the default cadence without that
+// state (fewer registers, more waves per SIMD).
+template <typename Real, int PK, int KC, int BC, int ENS, int LANES, bool MC>
 __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
 {
     static_assert((BC == BC_SPHERE && ENS == ENS_NVT) || BC == BC_CUBIC,
@@ -260,17 +267,23 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             boxh = a.slot_boxh[slot];
         }
         // The temperature label stays until the exchange after the launch's
-        // last cycle: its step sizes, window counters and energy sums live in
-        // registers (every lane of the cluster holds the same values) and go
-        // back to memory at the end.
+        // last cycle (MC: its step sizes, window counters and energy sums in
+        // registers, every lane of the cluster holding the same values, back
+        // to memory at the end).
         const int T = tb + t;
         const double beta = a.t_beta[T];
         double delta = a.t_displ[2 * T];
         double dvol = ENS == ENS_NPT ? a.t_displ[2 * T + 1] : 0.0;
-        long long win_atom = a.t_count_atom[2 * T + 1], add_atom = 0;
-        long long win_vol = ENS == ENS_NPT ? a.t_count_vol[2 * T + 1] : 0, add_vol = 0;
-        double ham0 = a.t_ham[2 * T], ham1 = a.t_ham[2 * T + 1];
-        const int ncyc = a.n_cycles > 0 ? a.n_cycles : 1;  // n_cycles = 0: the pending recompute only
+        long long win_atom = 0, add_atom = 0, win_vol = 0, add_vol = 0;
+        double ham0 = 0.0, ham1 = 0.0;
+        if constexpr (MC) {
+            win_atom = a.t_count_atom[2 * T + 1];
+            if (ENS == ENS_NPT) win_vol = a.t_count_vol[2 * T + 1];
+            ham0 = a.t_ham[2 * T];
+            ham1 = a.t_ham[2 * T + 1];
+        }
+        // n_cycles = 0: the pending recompute only
+        const int ncyc = MC ? (a.n_cycles > 0 ? a.n_cycles : 1) : 1;
         for (int cc = 0; cc < ncyc; ++cc) {
           const uint32_t sweep = (uint32_t)(a.sweep0 + cc);
           // FP64 recompute due after the previous cycle (after its sampling)
@@ -539,6 +552,16 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     a.trace_idx[ti] = (int32_t)i;
                 }
             }
+            if constexpr (!MC) {
+                if (q == 0) {
+                    a.t_count_atom[2 * T] += nacc;
+                    a.t_count_atom[2 * T + 1] += nacc;
+                    if (ENS == ENS_NPT) {
+                        a.t_count_vol[2 * T] += nvacc;
+                        a.t_count_vol[2 * T + 1] += nvacc;
+                    }
+                }
+            } else {
             win_atom += nacc;
             add_atom += nacc;
             win_vol += nvacc;
@@ -582,6 +605,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     }
                 }
             }
+            }
           }
         }
         if (q == 0) {
@@ -590,7 +614,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 a.slot_box[slot] = box;
                 a.slot_boxh[slot] = boxh;
             }
-            if (a.n_cycles > 0) {
+            if (MC && a.n_cycles > 0) {
                 a.t_displ[2 * T] = delta;
                 a.t_count_atom[2 * T] += add_atom;
                 a.t_count_atom[2 * T + 1] = win_atom;

@@ -94,9 +94,13 @@ def _adapted_state(make_engine: Callable, runs, table, cycles: int, layout=None)
         eng.close()
 
 
-def _rate(make_engine: Callable, runs, table, warmup: int, cycles: int, state=None, layout=None, seen=None):
+def _rate(make_engine: Callable, runs, table, warmup: int, cycles: int, state=None, layout=None, seen=None,
+          call_cycles=None):
     """(moves per second, plan key, lanes used) of one candidate; raises
-    _Unplannable, or returns None when the same plan was timed before."""
+    _Unplannable, or returns None when the same plan was timed before.
+    call_cycles: time the cycles as calls of that many cycles (each call
+    ends with a device synchronisation, so its fixed costs are counted as a
+    run of that length pays them)."""
     eng = _engine(make_engine, runs, table, layout)
     try:
         key, used = _plan_key(eng, runs, layout)
@@ -113,13 +117,15 @@ def _rate(make_engine: Callable, runs, table, warmup: int, cycles: int, state=No
             e = eng.energies(i)
             eng.set_histogram(i, float(e.min()), float(e.max()))
         eng.start_phase(True)
+        n_calls, per_call = (max(1, cycles // call_cycles), call_cycles) if call_cycles else (1, cycles)
         t0 = time.perf_counter()
-        eng.cycles(cycles)
+        for _ in range(n_calls):
+            eng.cycles(per_call)
         eng.sync()
         dt = time.perf_counter() - t0
     finally:
         eng.close()
-    moves = sum(len(r["beta"]) * r["n_atoms"] for r in runs) * cycles
+    moves = sum(len(r["beta"]) * r["n_atoms"] for r in runs) * n_calls * per_call
     return moves / dt, key, used
 
 
@@ -135,12 +141,13 @@ def start_table(sizes) -> dict:
     return out
 
 
-def _descend(make_engine, runs, table, best, sizes, choices, warmup, cycles, state, layout, rounds, log):
+def _descend(make_engine, runs, table, best, sizes, choices, warmup, cycles, state, layout, rounds, log,
+             call_cycles=None):
     """Coordinate descent over {n_atoms: lanes} in one layout; a candidate
     more than 3 % faster is timed again beside the current table and taken
     only if the two timings together still say so."""
     seen = set()
-    cur_rate, key, _ = _rate(make_engine, runs, table, warmup, cycles, state, layout)
+    cur_rate, key, _ = _rate(make_engine, runs, table, warmup, cycles, state, layout, call_cycles=call_cycles)
     seen.add(key)
     best = max(best, cur_rate) if best else cur_rate
     for _ in range(rounds):
@@ -152,7 +159,8 @@ def _descend(make_engine, runs, table, best, sizes, choices, warmup, cycles, sta
                 trial = dict(table)
                 trial[n] = kl
                 try:
-                    rate, key, used = _rate(make_engine, runs, trial, warmup, cycles, state, layout, seen)
+                    rate, key, used = _rate(make_engine, runs, trial, warmup, cycles, state, layout, seen,
+                                            call_cycles=call_cycles)
                 except _Unplannable:
                     continue
                 if rate is None:  # the engine planned this candidate as one already timed
@@ -161,8 +169,8 @@ def _descend(make_engine, runs, table, best, sizes, choices, warmup, cycles, sta
                 if log:
                     log(f"  N={n} lanes={kl} (used {used[n]}): {rate:.4g} moves/s")
                 if rate > best * 1.03:  # changes inside the timing noise are not taken
-                    again, _, _ = _rate(make_engine, runs, trial, warmup, cycles, state, layout)
-                    cur, _, _ = _rate(make_engine, runs, table, warmup, cycles, state, layout)
+                    again, _, _ = _rate(make_engine, runs, trial, warmup, cycles, state, layout, call_cycles=call_cycles)
+                    cur, _, _ = _rate(make_engine, runs, table, warmup, cycles, state, layout, call_cycles=call_cycles)
                     if log:
                         log(f"  confirm N={n} lanes={kl}: {again:.4g} vs current {cur:.4g} moves/s")
                     if rate + again > (best + cur) * 1.03:
@@ -199,14 +207,18 @@ def choose_cadence(rates: dict, tolerance: float = 0.03) -> int:
 
 def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, rounds: int = 1,
          choices=LANE_CHOICES, layouts=LAYOUTS, start: dict | None = None, adapt_cycles: int = 1000,
-         log: Callable | None = None, cadences=(1,), cadence_tolerance: float = 0.03) -> dict:
+         log: Callable | None = None, cadences=(1,), cadence_tolerance: float = 0.03,
+         call_cycles: int | None = None) -> dict:
     """Layout and lanes per configuration: layouts timed with the starting
     table, lanes descent in the best layout, layouts timed again with the
     tuned table; then, if more than one exchange cadence is allowed, each
     timed in the chosen layout (``choose_cadence``).  make_engine(
     workgroup_lanes, spread_streams) builds a fresh engine (with cadences
     besides (1,): make_engine(workgroup_lanes, spread_streams,
-    exchange_every)).  Returns the best candidate as a dict (table,
+    exchange_every)).  call_cycles: time every candidate as calls of that
+    many cycles (the length of the run being tuned for), so per-call costs
+    (stream fork and join, first-launch latency, the final synchronisation)
+    weigh as they will.  Returns the best candidate as a dict (table,
     workgroup_lanes, spread_streams, exchange_every, lanes_used, rate)."""
     sizes = sorted({r["n_atoms"] for r in runs}, reverse=True)
     table = dict(start if start is not None else start_table(sizes))
@@ -217,7 +229,7 @@ def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, roun
         res = []
         for lay in layouts:
             try:
-                rate, _, used = _rate(make_engine, runs, tab, warmup, cycles, state, lay)
+                rate, _, used = _rate(make_engine, runs, tab, warmup, cycles, state, lay, call_cycles=call_cycles)
             except _Unplannable:
                 continue
             if log:
@@ -229,10 +241,10 @@ def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, roun
     if log:
         log(f"layout {layout} with {table}: {rate:.4g} moves/s")
     table, rate = _descend(make_engine, runs, table, rate, sizes, choices, warmup, cycles, state, layout, rounds,
-                           log)
+                           log, call_cycles=call_cycles)
     rate, layout, used = sweep_layouts(table)
     # the winner timed once more, so the reported rate is not the luckiest of many
-    again, _, used = _rate(make_engine, runs, table, warmup, cycles, state, layout)
+    again, _, used = _rate(make_engine, runs, table, warmup, cycles, state, layout, call_cycles=call_cycles)
     if log:
         log(f"best layout {layout} lanes {table}: {rate:.4g} / {again:.4g} moves/s")
     rate = 0.5 * (rate + again)
@@ -240,7 +252,8 @@ def tune(make_engine: Callable, runs, warmup: int = 20, cycles: int = 1500, roun
     if tuple(cadences) != (1,):
         by_k = {}
         for k in sorted(set(cadences)):
-            by_k[k], _, _ = _rate(make_engine, runs, table, warmup, cycles, state, tuple(layout) + (k,))
+            by_k[k], _, _ = _rate(make_engine, runs, table, warmup, cycles, state, tuple(layout) + (k,),
+                                  call_cycles=call_cycles)
             if log:
                 log(f"  exchange every {k}: {by_k[k]:.4g} moves/s")
         every = choose_cadence(by_k, cadence_tolerance)
