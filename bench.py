@@ -538,6 +538,85 @@ def bench_delayed_acceptance(pt, n_ladders, warmup, steps, collect, epochs):
     return out
 
 
+def bench_delayed_acceptance_ne38(pt, n_ladders, warmup, steps, collect, epochs):
+    """Delayed acceptance on ELJ Ne38 (the start of scripts/ne38_B.jl with the
+    neon coefficients of scripts/ne13.jl, 32 temperatures 10-20 K, 6.6 A
+    container): the ladders run plain in pair slots (one workgroup per
+    configuration, the surrogate's layout), collect samples, train the
+    surrogate and run it as stage 1.  The plain rate of the tile layout (64
+    lanes per tile, lanes per configuration 8) is reported beside it: for a
+    pair potential the exact move costs less than the surrogate stage."""
+    d = json.loads((ROOT / "tests" / "golden" / "ne38_start.json").read_text())
+    coeff = json.loads(NE13_SETUP.read_text())["coeff"]
+    x0 = np.array(d["pos_angstrom"]) * A2B
+    rad = d["radius_angstrom"] * A2B
+    R = d["n_traj"]
+    temp = pt.TempGrid(d["t_range"][0], d["t_range"][1], R)
+    mc = pt.MCParams(steps, R, 38, n_adjust=10, mc_sample=1, n_bin=100)
+
+    def engine(**kw):
+        eng = pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(38), pt.SphericalBC(rad), mc,
+                          pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=50, **kw))
+        for r in range(n_ladders):
+            eng.add_run(38, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r, lanes=kw.get("lanes", 0))
+        eng.prepare()
+        return eng
+
+    tile = pt.PTEngine(pt.ELJPotentialEven(coeff), pt.NVT(38), pt.SphericalBC(rad), mc,
+                       pt.EngineOptions(precision="fp32", exchange="even_odd", seed=1234, recompute_every=50,
+                                        workgroup_lanes=64))
+    for r in range(n_ladders):
+        tile.add_run(38, temp.beta_grid, x0[None], pt.SphericalBC(rad), run_key=r, lanes=8)
+    tile.prepare()
+    tile.start_phase(False)
+    tile.cycles(warmup)
+    tile.sync()
+    t0 = time.perf_counter()
+    tile.cycles(steps)
+    tile.sync()
+    tile_rate = n_ladders * R * 38 * steps / (time.perf_counter() - t0)
+    tile.close()
+
+    eng = engine(pair_slots=True)
+    eng.surrogate_init(capacity=100000, seed=5)
+    eng.start_phase(False)
+    eng.cycles(warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.cycles(steps)
+    eng.sync()
+    plain = n_ladders * R * 38 * steps / (time.perf_counter() - t0)
+    eng.surrogate_collect(1)
+    eng.cycles(collect)
+    eng.surrogate_collect(0)
+    _, rmse0 = eng.train_surrogate(0)
+    t0 = time.perf_counter()
+    rmse_tr, rmse_va = eng.train_surrogate(epochs, batch=128, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    eng.delayed_acceptance(True)
+    eng.cycles(2)
+    eng.sync()
+    before = [eng.surrogate_stats(r) for r in range(n_ladders)]
+    t0 = time.perf_counter()
+    eng.cycles(steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    after = [eng.surrogate_stats(r) for r in range(n_ladders)]
+    dd = {k: sum(a[k] - b[k] for a, b in zip(after, before)) for k in ("proposals", "stage1", "accepted")}
+    out = dict(workload=f"delayed acceptance, ELJ Ne38, {n_ladders} ladders x {R} temperatures "
+                        f"{d['t_range'][0]:.0f}-{d['t_range'][1]:.0f} K",
+               moves_per_s=dd["proposals"] / dt, plain_pair_slot_moves_per_s=plain,
+               speedup_vs_plain_pair_slots=dd["proposals"] / dt / plain, plain_tile_moves_per_s=tile_rate,
+               speedup_vs_plain_tiles=dd["proposals"] / dt / tile_rate,
+               exact_evaluations_per_proposal=dd["stage1"] / dd["proposals"],
+               acceptance=dd["accepted"] / dd["proposals"], stage2_acceptance=dd["accepted"] / max(dd["stage1"], 1),
+               samples=eng.surrogate_samples(), train_epochs=epochs, train_s=t_train,
+               surrogate_rmse_per_atom=dict(untrained=rmse0, train=rmse_tr, valid=rmse_va), steps=steps,
+               dtype="fp32 pair sums, FP64 totals, bf16 surrogate (matrix cores)")
+    eng.close()
+    return out
+
+
 def other_configs(pt, args, only=None):
     cpu = 0.0 if args.no_cpu else args.cpu_seconds / 3
     todo = {
@@ -554,6 +633,7 @@ def other_configs(pt, args, only=None):
         # 100 warm-up cycles: step adaptation (every 10 cycles) reaches the
         # 0.4-0.6 acceptance window, where the stage-1 filter has work to do
         "delayed_acceptance": lambda: [bench_delayed_acceptance(pt, 256, 100, 4, 8, 30)],
+        "delayed_acceptance_ne38": lambda: [bench_delayed_acceptance_ne38(pt, 256, 100, 20, 12, 30)],
         "nnvt": lambda: [bench_nnvt(pt, 1, 5, 20, cpu)],
         "nnvt_batched": lambda: [bench_nnvt(pt, 25, 5, 20, 0.0)],
         "nnvt_f32": lambda: [bench_nnvt(pt, 1, 5, 20, 0.0, "fp32")],
@@ -592,7 +672,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--config", type=str, default="",
                     help="measure only this other configuration (ne13, ne13_batched, bulk, nnp, nnp_batched, nnp_cu78, "
-                         "delayed_acceptance, nnvt, nnvt_batched, nnvt_f32, nnvt_f32_batched) and print its JSON")
+                         "delayed_acceptance, delayed_acceptance_ne38, nnvt, nnvt_batched, nnvt_f32, nnvt_f32_batched) and print its "
+                         "JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent PT runs would "

@@ -77,6 +77,10 @@ typedef struct {
                                   exchange_mode says (0 or 1: every cycle, the reference's
                                   src/MCRun.jl:203); the spread layout then runs the cycles
                                   between exchanges in one launch */
+    int32_t pair_slots;        /* 1: pair potentials (LJ, even extended LJ) one workgroup per
+                                  configuration of up to 64 atoms in binary32 with FP64 totals,
+                                  spherical container, NVT: the layout of delayed acceptance
+                                  with the learned surrogate (ptmc_surrogate_init) */
 } ptmc_config;
 
 /* One PT run.  Counterparts: TempGrid (src/InputParams.jl:77-95), the start
@@ -368,13 +372,15 @@ int ptmc_energy_update(ptmc_handle h, int32_t n_conf, int32_t n_atoms, const dou
                        const double *vars, double *new_en, double *new_en_atom, double *new_vars);
 
 /* Learned surrogate and delayed-acceptance Metropolis for the network
- * potential.  No counterpart in the reference: these entry points serve the
- * engine's own acceleration (Christen & Fox 2005).  The surrogate is a
- * per-atom MLP on 64 Gaussian radial descriptors (64 -> 64 -> 64 SiLU -> 1),
+ * potential, or for a pair potential in pair slots (ptmc_config.pair_slots).
+ * No counterpart in the reference: these entry points serve the engine's own
+ * acceleration (Christen & Fox 2005).  The surrogate is a per-atom MLP on 64
+ * Gaussian radial descriptors between 2 bohr and the cutoff (the network
+ * potential's; 11.338 bohr for pair potentials) (64 -> 64 -> 64 SiLU -> 1),
  * trained on the device on (configuration, exact energy) samples the run
  * collects.  With delayed acceptance on, a move is first tested on the
  * surrogate energy change and only a surviving move is evaluated with the
- * network potential and accepted with min(1, exp(-beta (dE - dE_hat))), so
+ * exact potential and accepted with min(1, exp(-beta (dE - dE_hat))), so
  * the sampled distribution is the exact one.  Parameters are 8385 values:
  * W1 [64][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3. */
 int ptmc_surrogate_init(ptmc_handle h, int32_t sample_capacity, uint64_t seed);

@@ -24,6 +24,8 @@ using namespace ptmc;
 namespace ptmc {
 SweepFn nnp_sweep_fn(int np);
 SweepFn nnp_sweep_da_fn();
+bool pair_slot_fns(int pk, int kc, bool da, SweepFn *sweep, SweepFn *init);
+size_t pair_slot_lds_bytes(bool da);
 SweepFn nnp_init_fn(int np);
 SweepFn nnp2_sweep_fn();
 SweepFn nnp2_init_fn();
@@ -723,6 +725,13 @@ int launch_wgs(ptmc_engine *h, int tile)
     return 256;
 }
 
+bool is_pair_slot(const ptmc_engine *h);
+
+// Cutoff of the surrogate's radial descriptors: the network potential's own;
+// for pair potentials 6 A (11.338 bohr), the descriptors' grid starting at
+// 2 bohr (atomic units, as the neon and copper set-ups of the reference).
+double sur_cutoff_of(const ptmc_engine *h) { return is_pair_slot(h) ? 11.338 : h->nnp.r_cut; }
+
 SweepArgs base_args(ptmc_engine *h)
 {
     SweepArgs a{};
@@ -771,7 +780,7 @@ SweepArgs base_args(ptmc_engine *h)
     a.temp_slot = h->d_temp_slot.p;
     a.sur_w = h->d_sur_w.p;
     a.da = h->sur.da ? 1 : 0;
-    a.sur_rcut = (float)h->nnp.r_cut;
+    a.sur_rcut = (float)sur_cutoff_of(h);
     a.da_stats = h->d_da_stats.p;
     return a;
 }
@@ -779,11 +788,54 @@ SweepArgs base_args(ptmc_engine *h)
 bool is_nnp(const ptmc_engine *h) { return h->pot.kind == POT_NNP; }
 bool is_nnp2(const ptmc_engine *h) { return h->pot.kind == POT_NNP2; }
 // potentials run one workgroup per slot (label exchange between cycles)
-bool is_slot(const ptmc_engine *h) { return is_nnp(h) || is_nnp2(h); }
+// pair potentials one workgroup per configuration (cfg.pair_slots)
+bool is_pair_slot(const ptmc_engine *h) { return h->cfg.pair_slots != 0 && !is_nnp(h) && !is_nnp2(h); }
+bool is_slot(const ptmc_engine *h) { return is_nnp(h) || is_nnp2(h) || is_pair_slot(h); }
 // atom stride of a slot engine's positions, features and atom energies
 int slot_np(const ptmc_engine *h) { return is_nnp(h) ? h->nnp_np : NNP_NP; }
 // slot engines keeping binary32 positions [slot][3][slot_np]
-bool slot_f32(const ptmc_engine *h) { return is_nnp(h) || (is_nnp2(h) && h->cfg.precision == 1); }
+bool slot_f32(const ptmc_engine *h)
+{
+    return is_nnp(h) || is_pair_slot(h) || (is_nnp2(h) && h->cfg.precision == 1);
+}
+
+// Pair slots: one workgroup per configuration, positions [slot][3][NNP_NP]
+// in binary32, the surrogate's layout for delayed acceptance.
+int prepare_pair_slot(ptmc_engine *h)
+{
+    if (h->cfg.precision != 1) return fail(h, PTMC_E_UNSUPPORTED, "pair slots run in binary32 (precision 1)");
+    if (h->cfg.ensemble != ENS_NVT || h->cfg.boundary != BC_SPHERE)
+        return fail(h, PTMC_E_UNSUPPORTED, "pair slots run NVT in a spherical container");
+    SweepFn fs = nullptr, fi = nullptr;
+    if (h->pot.kind == POT_EAM || !pair_slot_fns(h->pk, h->kc, false, &fs, &fi))
+        return fail(h, PTMC_E_UNSUPPORTED, "pair slots take the Lennard-Jones or even extended-LJ (up to 8 "
+                    "coefficients) pair forms");
+    h->nnp_lds = pair_slot_lds_bytes(false);
+    int slot = 0;
+    for (auto &rh : h->runs) {
+        if (rh.d.n_atoms > NNP_NP) return fail(h, PTMC_E_UNSUPPORTED, "at most %d atoms in pair slots", NNP_NP);
+        rh.info.slot_base = slot;
+        rh.info.local_base = 0;
+        rh.kl = PSL_NT;
+        rh.lanes = PSL_NT;
+        rh.tile = -1;
+        slot += rh.d.n_temps;
+    }
+    h->n_slots = slot;
+    h->tiles.clear();
+    h->launches.clear();
+    Launch L;
+    L.fn = fs;
+    L.init = fi;
+    L.ntiles = slot;
+    L.wgs = PSL_NT;
+    L.kl = PSL_NT;
+    L.lds = h->nnp_lds;
+    h->launches.push_back(L);
+    h->n_coord = (int64_t)slot * 3 * NNP_NP;
+    h->n_comp = 0;
+    return PTMC_OK;
+}
 
 // Slots of the two-species engine: as prepare_nnp, FP64 positions
 // [slot][3][NNP_NP], features [slot][NNP_NP][ng] and atom energies in HBM.
@@ -1497,6 +1549,7 @@ int ptmc_prepare(ptmc_handle h)
         return fail(h, PTMC_E_UNSUPPORTED, "the NNVT ensemble and the two-species network potential go together");
     int rc = is_nnp(h) ? prepare_nnp(h) : (is_nnp2(h) ? prepare_nnp2(h) : choose_pair(h));
     if (rc) return rc;
+    if (is_pair_slot(h) && (rc = prepare_pair_slot(h))) return rc;
     if (!is_slot(h)) rc = plan(h);
     if (rc) return rc;
     if (h->launches.size() > 1) {
@@ -2257,6 +2310,15 @@ int ptmc_run_group(ptmc_handle h, int32_t run)
 
 // ------------------------------------------------------------------ surrogate
 
+// the delayed-acceptance sweep kernel of the engine's potential
+static SweepFn da_sweep_fn(ptmc_engine *h)
+{
+    if (!is_pair_slot(h)) return nnp_sweep_da_fn();
+    SweepFn fs = nullptr, fi = nullptr;
+    pair_slot_fns(h->pk, h->kc, true, &fs, &fi);
+    return fs;
+}
+
 static int sur_check(ptmc_engine *h)
 {
     if (!h) return PTMC_E_ARG;
@@ -2269,7 +2331,9 @@ int ptmc_surrogate_init(ptmc_handle h, int32_t capacity, uint64_t seed)
     if (!h) return PTMC_E_ARG;
     int rc = ptmc_prepare(h);
     if (rc) return rc;
-    if (!is_nnp(h)) return fail(h, PTMC_E_UNSUPPORTED, "the surrogate filters the network potential");
+    if (!is_nnp(h) && !is_pair_slot(h))
+        return fail(h, PTMC_E_UNSUPPORTED, "the surrogate filters the network potential or a pair potential in pair "
+                    "slots (ptmc_config.pair_slots)");
     if (h->nnp_np != NNP_NP)
         return fail(h, PTMC_E_UNSUPPORTED, "the surrogate covers configurations of at most %d atoms", NNP_NP);
     if (capacity < 1) return fail(h, PTMC_E_ARG, "sample capacity must be positive");
@@ -2277,9 +2341,9 @@ int ptmc_surrogate_init(ptmc_handle h, int32_t capacity, uint64_t seed)
     int dev = 0, max_lds = 0;
     HIPCHK(h, hipGetDevice(&dev));
     HIPCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t lds = h->nnp_lds + sur_lds_floats() * sizeof(float);
+    const size_t lds = is_pair_slot(h) ? pair_slot_lds_bytes(true) : h->nnp_lds + sur_lds_floats() * sizeof(float);
     if ((long long)lds > max_lds) return fail(h, PTMC_E_UNSUPPORTED, "delayed acceptance needs %zu B of LDS", lds);
-    HIPCHK(h, hipFuncSetAttribute((const void *)nnp_sweep_da_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIPCHK(h, hipFuncSetAttribute((const void *)da_sweep_fn(h), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     h->sur.cap = capacity;
     h->sur.seed = seed;
     h->sur.n_samples = 0;
@@ -2339,6 +2403,13 @@ int ptmc_surrogate_enable(ptmc_handle h, int32_t on)
     if (rc) return rc;
     h->sur.da = on != 0;
     Launch &L = h->launches[0];
+    if (is_pair_slot(h)) {
+        SweepFn fs = nullptr, fi = nullptr;
+        pair_slot_fns(h->pk, h->kc, h->sur.da, &fs, &fi);
+        L.fn = fs;
+        L.lds = pair_slot_lds_bytes(h->sur.da);
+        return PTMC_OK;
+    }
     L.fn = h->sur.da ? nnp_sweep_da_fn() : nnp_sweep_fn(h->nnp_np);
     L.lds = h->nnp_lds + (h->sur.da ? sur_lds_floats() * sizeof(float) : 0);
     return PTMC_OK;
@@ -2385,7 +2456,7 @@ static int sur_upload_configs(ptmc_engine *h, int32_t n, int32_t n_atoms, const 
     std::vector<int32_t> nv(n, n_atoms);
     if ((rc = upload(h, Nb, nv))) return rc;
     if ((rc = dalloc(h, D, (size_t)n * NNP_NP * SUR_K))) return rc;
-    HIPCHK(h, sur_launch_desc(h->d_sur_x.p, Nb.p, D.p, (float)h->nnp.r_cut, n, h->stream));
+    HIPCHK(h, sur_launch_desc(h->d_sur_x.p, Nb.p, D.p, (float)sur_cutoff_of(h), n, h->stream));
     return PTMC_OK;
 }
 

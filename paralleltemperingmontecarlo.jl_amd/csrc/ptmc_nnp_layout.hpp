@@ -8,6 +8,7 @@ namespace ptmc {
 constexpr int NNP_NP = 64;     // atoms per configuration of the narrow slot layout
 constexpr int NNP_NP_WIDE = 128;  // atoms per configuration of the wide layout (one workgroup per CU)
 constexpr int NNP_NT = 512;    // threads per workgroup
+constexpr int PSL_NT = 256;    // threads per workgroup of the pair slots (the surrogate network uses waves 0-3)
 constexpr int NNP_MAXH = 32;   // hidden width (maximum)
 constexpr int NNP_HS = NNP_MAXH + 1;  // padded hidden row stride
 constexpr int NNP_KSLOT = 5;   // zeta = 2^k, k = 0..4 (zeta up to 16)

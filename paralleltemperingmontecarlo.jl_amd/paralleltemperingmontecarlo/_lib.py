@@ -59,6 +59,7 @@ class ptmc_config(C.Structure):
         ("workgroup_lanes", C.c_int32),
         ("spread_streams", C.c_int32),
         ("exchange_every", C.c_int32),
+        ("pair_slots", C.c_int32),
     ]
 
 

@@ -553,7 +553,10 @@ class EngineOptions:
     cuts the runs into that many groups launched on their own streams.
     exchange_every: replica exchange is attempted (as `exchange` says) only
     after the cycles with (cycle + 1) % exchange_every == 0; 1 (default)
-    attempts it after every cycle as src/MCRun.jl:203 does."""
+    attempts it after every cycle as src/MCRun.jl:203 does.  pair_slots: pair
+    potentials one workgroup per configuration (binary32, spherical container,
+    NVT, up to 64 atoms), the layout in which the learned surrogate filters
+    their moves (delayed acceptance)."""
 
     precision: str = "fp64"
     exchange: str = "reference"
@@ -566,6 +569,7 @@ class EngineOptions:
     workgroup_lanes: int = 0
     spread_streams: int = 0
     exchange_every: int = 1
+    pair_slots: bool = False
     max_displ: tuple = (0.1, 0.1)
 
 
@@ -648,6 +652,7 @@ def _engine_config(mc_params: MCParams, ensemble, bc, opts: EngineOptions) -> _l
     cfg.workgroup_lanes = opts.workgroup_lanes
     cfg.spread_streams = opts.spread_streams
     cfg.exchange_every = opts.exchange_every
+    cfg.pair_slots = 1 if opts.pair_slots else 0
     cfg.device = opts.device
     cfg.seed = opts.seed & 0xFFFFFFFFFFFFFFFF
     return cfg

@@ -52,3 +52,33 @@ def assert_decisions(tr, acc_dev, beta, eps, valid=None, min_compared=1):
     assert n >= min_compared, (n, min_compared)
     assert outside == 0, f"{outside} of {n} binary32 decisions differ from the FP64 Metropolis test outside the near-tie band"
     return inside
+
+
+def assert_da_decisions(oracle, tr, de, acc, aux, beta, seed, run_key, eps):
+    """Both stages of every traced delayed-acceptance move against FP64
+    Metropolis tests: stage 1 accepts iff min(1, e^{-beta dEhat}) >= u with the
+    move's own uniform u (the oracle's trace gives u; dEhat is the device's
+    surrogate change); stage 2 accepts iff min(1, e^{-beta (dE - dEhat)}) >= u2,
+    dE the oracle's FP64 change on the replayed positions and u2 the 24-bit
+    uniform of the move's Philox block 255.  Stage-2 decisions may differ only
+    inside the near-tie band of the dE bound eps."""
+    have = np.isfinite(de)
+    assert np.isfinite(aux).all()
+    b = np.asarray(beta).reshape(-1, 1, 1)
+    p1 = np.minimum(1.0, np.exp(-b * aux))
+    s1 = p1 >= tr["u"]
+    tie1 = np.abs(p1 - tr["u"]) <= 1e-12 * p1
+    assert np.array_equal(s1[~tie1], have[~tie1])
+    R, nc, ns = de.shape
+    u2 = np.zeros(de.shape)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for t in range(R):
+        for c in range(nc):
+            for m in range(ns):
+                w = oracle.philox4x32([m, c, (255 << 20) | t, run_key], key)
+                u2[t, c, m] = (w[0] >> 8) * 2.0**-24
+    x2 = tr["de"] - aux
+    p2 = np.minimum(1.0, np.exp(-b * x2))
+    n, inside, outside = decision_mismatches(dict(prob=p2, u=u2, de=x2), acc, beta, eps, valid=have)
+    assert n == have.sum() and outside == 0, (n, inside, outside)
+    assert not acc[~have].any()

@@ -253,6 +253,24 @@ def cu78_start(ref: pathlib.Path):
     (HERE / "cu78_start.json").write_text(json.dumps(out, indent=1))
 
 
+def ne38_start(ref: pathlib.Path):
+    src = (ref / "scripts" / "ne38_B.jl").read_text()
+    blk = src[src.index("pos_ne38 = ["): src.index("\n]", src.index("pos_ne38 = ["))]
+    pos = _floats(blk)
+    rad = float(re.search(r"SphericalBC\(;\s*radius\s*=\s*([0-9.]+)\)", src).group(1))
+    t = re.search(r"ti\s*=\s*([0-9.]+)\s*\n\s*tf\s*=\s*([0-9.]+)", src)
+    n_traj = int(re.search(r"n_traj\s*=\s*(\d+)", src).group(1))
+    out = dict(
+        source="scripts/ne38_B.jl:48-87 (positions, Angstrom), :97 (container, Angstrom), :12-14 (temperatures)",
+        pos_angstrom=[pos[i: i + 3] for i in range(0, len(pos), 3)],
+        radius_angstrom=rad,
+        t_range=[float(t.group(1)), float(t.group(2))],
+        n_traj=n_traj,
+    )
+    assert len(out["pos_angstrom"]) == 38
+    (HERE / "ne38_start.json").write_text(json.dumps(out, indent=1))
+
+
 def main():
     ref = pathlib.Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     ne13_setup(ref)
@@ -263,6 +281,7 @@ def main():
     nnvt_states(ref)
     known_answers(ref)
     cu78_start(ref)
+    ne38_start(ref)
     print("fixtures written to", HERE)
 
 
