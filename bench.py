@@ -620,7 +620,7 @@ def bench_delayed_acceptance_ne38(pt, n_ladders, warmup, steps, collect, epochs)
 def other_configs(pt, args, only=None):
     cpu = 0.0 if args.no_cpu else args.cpu_seconds / 3
     todo = {
-        "ne13": lambda: [bench_ne13(pt, 1, 500, 20000, cpu)],
+        "ne13": lambda: [bench_ne13(pt, 1, 500, 20000, cpu, layout=(0, args.ne13_lanes) if args.ne13_lanes else None)],
         "ne13_batched": lambda: [bench_ne13(pt, 512, 500, 2000, 0.0)],
         # one 48-temperature ladder: 64 lanes per configuration on 64-lane
         # workgroups spreads the ladder over 48 CUs (the fastest of the
@@ -666,6 +666,9 @@ def main():
                          "spread layout runs the cycles between exchanges in one launch")
     ap.add_argument("--adapt", type=int, default=3000,
                     help="untimed step-size adaptation cycles before --warmup (n_adjust = 100)")
+    ap.add_argument("--ne13-lanes", type=int, default=0,
+                    help="single Ne13 ladder: this many lanes per configuration in one workgroup instead of the "
+                         "tuned choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, help="run only a device copy of this many bytes")

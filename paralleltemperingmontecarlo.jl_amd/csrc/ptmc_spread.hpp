@@ -315,7 +315,14 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
             u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
             double mthr = 0.0, nthr = 0.0;  // FP64: Metropolis thresholds of the lane's and the next move
-            if (KL > 1) {
+            // binary32 in the spherical container: the lane that draws a move's
+            // words also decodes them (LDS offset of the atom, the displacement,
+            // the Metropolis uniform), so per move the cluster's lanes fetch five
+            // ready values instead of decoding the words each
+            constexpr bool PRE = F32 && !PBC;
+            uint32_t p_off = 0u;
+            float p_dx = 0.0f, p_dy = 0.0f, p_dz = 0.0f, p_um = 0.0f;
+            if (!PRE && KL > 1) {
                 if ((uint32_t)q < nsteps) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
                     if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
@@ -327,7 +334,32 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 // draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
                 u32x4 cur = nA, curB = nB;
                 double thr = nthr;
-                if (KL == 1) {
+                MoveDraw<Real> d0{};
+                uint32_t i = 0u;  // PRE: formed from xo where needed
+                int xo = 0;       // LDS offset of the moved atom's x
+                float pdx = 0.0f, pdy = 0.0f, pdz = 0.0f, umet = 0.0f;
+                if constexpr (PRE) {
+                    const uint32_t k = m & (uint32_t)(KL - 1);
+                    if (k == 0u && m + (uint32_t)q < nsteps) {
+                        u32x4 w, wb;
+                        move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m + (uint32_t)q, w, wb);
+                        const MoveDraw<Real> dw = decode_move<Real>(w, w, nsteps);
+                        p_off = (uint32_t)xoff<LANES>((int)(dw.index >> kls), 0,
+                                                      gb + (int)(dw.index & (uint32_t)(KL - 1)));
+                        p_dx = __builtin_fmaf((float)(w.y >> 8), dsc, dof);
+                        p_dy = __builtin_fmaf((float)(w.z >> 8), dsc, dof);
+                        p_dz = __builtin_fmaf((float)(w.w >> 8), dsc, dof);
+                        p_um = (float)dw.umet;
+                    }
+                    // (also with one lane per configuration: LDS-pipe permutes
+                    // in place of register copies on the VALU)
+                    const int src = gb + (int)k;
+                    xo = (int)lane_get(p_off, src);
+                    pdx = __uint_as_float(lane_get(__float_as_uint(p_dx), src));
+                    pdy = __uint_as_float(lane_get(__float_as_uint(p_dy), src));
+                    pdz = __uint_as_float(lane_get(__float_as_uint(p_dz), src));
+                    umet = __uint_as_float(lane_get(__float_as_uint(p_um), src));
+                } else if (KL == 1) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m, cur, curB);
                     if constexpr (!F32) thr = metro_thr(u53(curB.z, curB.w), beta);
                 } else {
@@ -342,28 +374,33 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         if constexpr (!F32) nthr = lane_get_d(mthr, gb + (int)k1);
                     }
                 }
-                const MoveDraw<Real> d0 = decode_move<Real>(cur, curB, nsteps);
-                const uint32_t i = d0.index;
+                if constexpr (!PRE) {
+                    d0 = decode_move<Real>(cur, curB, nsteps);
+                    i = d0.index;
+                    if (i < (uint32_t)N) xo = xoff<LANES>((int)(i >> kls), 0, gb + (int)(i & (uint32_t)(KL - 1)));
+                }
                 bool accept = false;
                 double dE_used = 0.0;
-                if (i < (uint32_t)N) {
-                    const int si = (int)(i >> kls), li = gb + (int)(i & (uint32_t)(KL - 1));
-                    Real xi = X[xoff<LANES>(si, 0, li)];
-                    Real yi = X[xoff<LANES>(si, 1, li)];
-                    Real zi = X[xoff<LANES>(si, 2, li)];
+                // NVT: every index is an atom; NPT: index N is the volume move
+                if (PRE || i < (uint32_t)N) {
+                    const int li = xo & (LANES - 1);
+                    Real xi = X[xo];
+                    Real yi = X[xo + LANES];
+                    Real zi = X[xo + 2 * LANES];
                     Real tx, ty, tz;
                     if constexpr (PBC) {
                         box_trial<Real, BC, false>(xi, yi, zi, d0, delta_r, bxr, tx, ty, tz);
                     } else if constexpr (F32) {
-                        tx = xi + __builtin_fmaf((float)(cur.y >> 8), dsc, dof);
-                        ty = yi + __builtin_fmaf((float)(cur.z >> 8), dsc, dof);
-                        tz = zi + __builtin_fmaf((float)(cur.w >> 8), dsc, dof);
+                        tx = xi + pdx;
+                        ty = yi + pdy;
+                        tz = zi + pdz;
                     } else {
                         make_trial<Real, BC_SPHERE, false>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
                     }
                     if constexpr (!PBC) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
                         if (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                            const int si = xo / (3 * LANES);
                             if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, si, li, radius2, delta_r,
                                                                a.seed, run_key, (uint32_t)t, sweep, m, nsteps, xi, yi,
                                                                zi, tx, ty, tz) &&
@@ -372,9 +409,9 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         }
                         // park the moved atom so its self pair contributes zero
                         if (tid == li) {
-                            X[xoff<LANES>(si, 0, li)] = far_point<Real>();
-                            X[xoff<LANES>(si, 1, li)] = far_point<Real>();
-                            X[xoff<LANES>(si, 2, li)] = far_point<Real>();
+                            X[xo] = far_point<Real>();
+                            X[xo + LANES] = far_point<Real>();
+                            X[xo + 2 * LANES] = far_point<Real>();
                         }
                         __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     }
@@ -457,7 +494,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         const float dEf = group_sum_rt((float)sn, KL);
                         dE_used = (double)dEf;
                         const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
-                        accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                        accept = (dEf <= 0.0f) || (pr >= (PRE ? umet : (float)d0.umet));
                         if (accept) {
                             E = E + (double)dEf;
                             nacc += 1;
@@ -476,9 +513,9 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     }
                     // commit (src/swap_config.jl:40-51) or restore the parked atom
                     if (tid == li) {
-                        X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
-                        X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
-                        X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
+                        X[xo] = accept ? tx : xi;
+                        X[xo + LANES] = accept ? ty : yi;
+                        X[xo + 2 * LANES] = accept ? tz : zi;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 }
@@ -549,7 +586,8 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle + cc) * nsteps + m;
                     a.trace_de[ti] = dE_used;
                     a.trace_acc[ti] = (int8_t)accept;
-                    a.trace_idx[ti] = (int32_t)i;
+                    a.trace_idx[ti] = PRE ? (int32_t)(((xo / (3 * LANES)) << kls) + ((xo & (LANES - 1)) - gb))
+                                          : (int32_t)i;
                 }
             }
             if constexpr (!MC) {
