@@ -145,7 +145,7 @@ __device__ double wave_pair_sum64_rt(const Real *X, int cl, int N, int kl, int k
 
 // sphere_redraw for a run-time cluster width (src/MCMoves.jl:115-135).
 template <typename Real, int LANES>
-__device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls, int A, int si, int li,
+__device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls, int A, int xo,
                                  Real radius2, Real delta_r, uint64_t seed, uint32_t run_key, uint32_t temp,
                                  uint32_t sweep, uint32_t m, uint32_t nsteps, Real &xi, Real &yi, Real &zi, Real &tx,
                                  Real &ty, Real &tz)
@@ -179,9 +179,9 @@ __device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls
         } else {
             attempt += 1;
             MoveDraw<Real> dd = move_draw<Real>(seed, run_key, temp, sweep, m, attempt, nsteps);
-            xi = X[xoff<LANES>(si, 0, li)];
-            yi = X[xoff<LANES>(si, 1, li)];
-            zi = X[xoff<LANES>(si, 2, li)];
+            xi = X[xo];
+            yi = X[xo + LANES];
+            zi = X[xo + 2 * LANES];
             make_trial<Real, BC_SPHERE, false>(xi, yi, zi, dd, delta_r, Real(0), Real(0), tx, ty, tz);
             if (count == 100) {
                 ok = false;
@@ -189,9 +189,9 @@ __device__ bool sphere_redraw_rt(Real *X, int tid, int q, int N, int kl, int kls
             }
         }
     }
-    xi = X[xoff<LANES>(si, 0, li)];
-    yi = X[xoff<LANES>(si, 1, li)];
-    zi = X[xoff<LANES>(si, 2, li)];
+    xi = X[xo];
+    yi = X[xo + LANES];
+    zi = X[xo + 2 * LANES];
     return ok;
 }
 
@@ -400,8 +400,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     if constexpr (!PBC) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
                         if (outside_sphere<Real>(tx, ty, tz, radius2)) {
-                            const int si = xo / (3 * LANES);
-                            if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, si, li, radius2, delta_r,
+                            if (!sphere_redraw_rt<Real, LANES>(X, tid, q, N, KL, kls, A, xo, radius2, delta_r,
                                                                a.seed, run_key, (uint32_t)t, sweep, m, nsteps, xi, yi,
                                                                zi, tx, ty, tz) &&
                                 q == 0)
@@ -447,6 +446,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                             if constexpr (F32 && PK == PK_LJ) {
                                 // e(u) = u^3 (c12 u^3 + c6): e_new - e_old = (a - b)(c12 (a + b) + c6),
                                 // a = u_new^3, b = u_old^3; one reciprocal serves both pairs
+                                // (a reciprocal issues at a fraction of the multiply rate)
                                 const float ip = __builtin_amdgcn_rcpf(r2n * r2o);
                                 const float un = ip * r2o, uo = ip * r2n;
                                 const float an = un * un * un, bo = uo * uo * uo;

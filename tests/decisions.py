@@ -82,3 +82,51 @@ def assert_da_decisions(oracle, tr, de, acc, aux, beta, seed, run_key, eps):
     n, inside, outside = decision_mismatches(dict(prob=p2, u=u2, de=x2), acc, beta, eps, valid=have)
     assert n == have.sum() and outside == 0, (n, inside, outside)
     assert not acc[~have].any()
+
+
+def exchange_uniform(oracle, seed, run_key, sweep, n):
+    """Uniform of the exchange of temperatures n, n + 1 after cycle `sweep`:
+    53 bits of the pair's Philox block (counter n + 1, sweep, stream 1 << 28,
+    run key)."""
+    w = oracle.philox4x32([1 + n, sweep, 1 << 28, run_key], [seed & 0xFFFFFFFF, seed >> 32])
+    return float(((int(w[0]) << 21) | (int(w[1]) >> 11))) * 2.0**-53
+
+
+def assert_same_configurations(oracle, x_dev, en_dev, orc, beta, *, seed, run_key, sweep, e_tol, max_ties=2):
+    """Device and oracle configurations per temperature bit for bit after a
+    binary32 replay, except adjacent temperatures n, n + 1 whose exchange after
+    the last cycle (`sweep`) went the other way on a near tie.  The device
+    tests min(1, e^{(beta_n - beta_n+1)(E_n - E_n+1)}) > u with its running
+    energies, which differ from the oracle's FP64 ones by at most e_tol
+    (asserted here on the matched configurations), so the two decisions can
+    differ only where u (the exchange's uniform, Philox block of the pair,
+    src/Exchange.jl:103-155) lies within p (e^{2 |dbeta| e_tol} - 1) of p.
+    Returns the number of such near ties."""
+    x_o = orc.x.reshape(x_dev.shape)
+    e_o = np.asarray(orc.en_tot, dtype=np.float64)
+    R = len(beta)
+    perm = np.arange(R)
+    ties = 0
+    t = 0
+    while t < R:
+        if np.array_equal(x_dev[t], x_o[t]):
+            t += 1
+            continue
+        n = t
+        assert n + 1 < R and np.array_equal(x_dev[n], x_o[n + 1]) and np.array_equal(x_dev[n + 1], x_o[n]), \
+            f"temperature {t}: the device configuration is not the oracle's"
+        u = exchange_uniform(oracle, seed, run_key, sweep, n)
+        db = float(beta[n] - beta[n + 1])
+        band = np.expm1(2.0 * abs(db) * e_tol) + 1e-12
+        # the oracle's energies before its last exchange: its pair kept or swapped
+        near = False
+        for e1, e2 in ((e_o[n], e_o[n + 1]), (e_o[n + 1], e_o[n])):
+            p = min(1.0, float(np.exp(db * (e1 - e2))))
+            near |= abs(p - u) <= band * p
+        assert near, f"temperatures {n}, {n + 1}: exchange differs outside the near-tie band"
+        perm[n], perm[n + 1] = n + 1, n
+        ties += 1
+        t += 2
+    assert ties <= max_ties, ties
+    assert np.max(np.abs(np.asarray(en_dev) - e_o[perm])) <= e_tol
+    return ties

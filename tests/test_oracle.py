@@ -293,6 +293,65 @@ def test_exchange_cadence(oracle, ne13):
     assert np.array_equal(a.x, b.x) and np.array_equal(a.count_exc, b.count_exc)
 
 
+def test_exchange_uniform_and_near_tie_check(oracle, ne13):
+    """The replay tests' exchange uniform (tests/decisions.py) is the
+    oracle's: over 40 even/odd cycles every attempted pair was accepted iff
+    min(1, e^{dbeta dE}) > u with the energies before the exchange.  The
+    configuration check accepts a swapped pair only on a near tie: planted
+    swaps with energies that put p on u pass, far from u they fail, and a
+    configuration that matches no oracle one fails."""
+    from decisions import assert_same_configurations, exchange_uniform
+
+    from paralleltemperingmontecarlo.api import TempGrid
+
+    beta = TempGrid(4.0, 16.0, 8).beta_grid
+    R = len(beta)
+    run = _ne13_run(oracle, ne13, oracle.engine_mode(1), exchange_mode=2, seed=2468, run_key=8)
+    n_ok = 0
+    prev = run.count_exc[:, 1].copy()
+    for c in range(40):
+        run.cycles(1)
+        acc = run.count_exc[:, 1] - prev
+        prev = run.count_exc[:, 1].copy()
+        e = run.en_tot
+        for n in range(c & 1, R - 1, 2):
+            swapped = acc[n] == 1
+            e1, e2 = (e[n + 1], e[n]) if swapped else (e[n], e[n + 1])
+            p = min(1.0, math.exp((beta[n] - beta[n + 1]) * (e1 - e2)))
+            assert (p > exchange_uniform(oracle, 2468, 8, c, n)) == swapped
+            n_ok += 1
+    assert n_ok == 20 * 3 + 20 * 4
+
+    class Fake:
+        pass
+
+    rng = np.random.default_rng(1)
+    R, N = 6, 5
+    x = rng.normal(size=(R, N, 3))
+    seed, key, sweep = 99, 4, 7
+    b = np.linspace(2.0, 1.0, R)
+    u = exchange_uniform(oracle, seed, key, sweep, 2)
+    e = rng.normal(size=R)
+    # energies of the pair (2, 3) with p = u: dbeta (e2 - e3) = ln u
+    e[2] = e[3] + math.log(u) / (b[2] - b[3])
+    orc = Fake()
+    orc.x, orc.en_tot = x.copy(), e.copy()
+    xs = x.copy()
+    xs[[2, 3]] = xs[[3, 2]]
+    es = e.copy()
+    es[[2, 3]] = es[[3, 2]]
+    assert assert_same_configurations(oracle, x, e, orc, b, seed=seed, run_key=key, sweep=sweep, e_tol=1e-6) == 0
+    assert assert_same_configurations(oracle, xs, es, orc, b, seed=seed, run_key=key, sweep=sweep, e_tol=1e-6) == 1
+    orc.en_tot = e + np.where(np.arange(R) == 2, 0.5, 0.0)  # p far from u
+    with pytest.raises(AssertionError):
+        assert_same_configurations(oracle, xs, es, orc, b, seed=seed, run_key=key, sweep=sweep, e_tol=1e-6)
+    orc.en_tot = e.copy()
+    xb = x.copy()
+    xb[4, 0, 0] += 1e-12
+    with pytest.raises(AssertionError):
+        assert_same_configurations(oracle, xb, e, orc, b, seed=seed, run_key=key, sweep=sweep, e_tol=1e-6)
+
+
 def test_step_adaptation(oracle, ne13):
     """update_max_stepsize!: after n_adjust cycles delta is scaled by exactly 0.9 or 1.1 (or kept)."""
     run = _ne13_run(oracle, ne13, oracle.engine_mode(1), n_adjust=50)
