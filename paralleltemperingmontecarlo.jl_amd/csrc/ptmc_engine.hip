@@ -1744,7 +1744,9 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
         const size_t ng = h->launches.size();
         // groups on their own streams (one group: the engine stream), joined at the end
         auto stream_of = [&](size_t g) { return ng == 1 ? h->stream : h->side[g % h->side.size()]; };
-        if (ng > 1) {
+        // the side streams start after the engine stream's pending work (set-up
+        // copies, counter resets); nothing to wait for when it is idle
+        if (ng > 1 && hipStreamQuery(h->stream) != hipSuccess) {
             HIPCHK(h, hipEventRecord(h->fork, h->stream));
             for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamWaitEvent(stream_of(g), h->fork, 0));
         }
@@ -1753,7 +1755,7 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
             // cycles up to and including the next with (sweep + 1) % k == 0
             const int64_t n = std::min<int64_t>(k - h->sweep_counter % k, n_cycles - c);
             const bool exch = (h->sweep_counter + n) % k == 0;
-            for (size_t g = 0; g < ng; ++g) {
+            auto args_of = [&](size_t g) {
                 const Launch &L = h->launches[g];
                 SweepArgs a = base_args(h);
                 a.sweep0 = h->sweep_counter;
@@ -1770,15 +1772,21 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
                     a.trace_cycles = h->trace_cycles;
                     a.trace_cycle = (int32_t)c;
                 }
-                if ((rc = launch_one(h, g, a, stream_of(g), false))) return rc;
-                if (exch) {
+                return a;
+            };
+            // every group's moves first, so the groups start together
+            for (size_t g = 0; g < ng; ++g)
+                if ((rc = launch_one(h, g, args_of(g), stream_of(g), false))) return rc;
+            if (exch)
+                for (size_t g = 0; g < ng; ++g) {
                     // the exchange cycle is the launch's last
+                    SweepArgs a = args_of(g);
                     a.sweep0 = h->sweep_counter + n - 1;
                     a.cycle0 = h->phase_counter + n - 1;
                     void *args[] = {&a};
-                    HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(L.nruns), dim3(64), args, 0, stream_of(g)));
+                    HIPCHK(h, hipLaunchKernel(spread_cycle_fn(), dim3(h->launches[g].nruns), dim3(64), args, 0,
+                                              stream_of(g)));
                 }
-            }
             // bound the event pool on long calls: the oldest launches are long done
             if (h->pending.size() >= 512 && (rc = collect_timings(h, 256))) return rc;
             h->sweep_counter += n;
