@@ -41,6 +41,26 @@ __device__ __forceinline__ uint32_t row_xor_u32(uint32_t v)
 }
 template <int ROW>
 __device__ __forceinline__ float row_xor(float v) { return __uint_as_float(row_xor_u32<ROW>(__float_as_uint(v))); }
+
+// v + (v of the lane across the row boundary), from one row swap of v with
+// a copy of itself: after v_permlane16_swap(a = v, b = v) (or the 32-lane
+// swap) a lane of an even row holds its own value in a and its partner's in
+// b, a lane of an odd row the reverse, so a + b is the pair sum on every lane
+// (the same bits as v + partner: the two operands are the same, in either
+// order).
+template <int ROW>
+__device__ __forceinline__ float row_pair_sum(float v)
+{
+    const uint32_t u = __float_as_uint(v);
+    if constexpr (ROW == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    } else {
+        static_assert(ROW == 32, "row swap of 16 or 32 lanes");
+        const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+}
 template <int ROW>
 __device__ __forceinline__ double row_xor(double v)
 {
@@ -61,8 +81,13 @@ __device__ __forceinline__ T group_sum_rt(T v, int kl)
     if (kl >= 4) v = v + dpp_perm<0x4E>(v);
     if (kl >= 8) v = v + dpp_perm<0x141>(v);
     if (kl >= 16) v = v + dpp_perm<0x140>(v);
-    if (kl >= 32) v = v + row_xor<16>(v);
-    if (kl >= 64) v = v + row_xor<32>(v);
+    if constexpr (sizeof(T) == 4) {
+        if (kl >= 32) v = row_pair_sum<16>(v);
+        if (kl >= 64) v = row_pair_sum<32>(v);
+    } else {
+        if (kl >= 32) v = v + row_xor<16>(v);
+        if (kl >= 64) v = v + row_xor<32>(v);
+    }
     return v;
 }
 
@@ -320,6 +345,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
             // the Metropolis uniform), so per move the cluster's lanes fetch five
             // ready values instead of decoding the words each
             constexpr bool PRE = F32 && !PBC;
+            const int gb4 = (gb & 63) << 2;  // byte address of the cluster's first lane for ds_bpermute
             uint32_t p_off = 0u;
             float p_dx = 0.0f, p_dy = 0.0f, p_dz = 0.0f, p_um = 0.0f;
             if (!PRE && KL > 1) {
@@ -352,13 +378,14 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         p_um = (float)dw.umet;
                     }
                     // (also with one lane per configuration: LDS-pipe permutes
-                    // in place of register copies on the VALU)
-                    const int src = gb + (int)k;
-                    xo = (int)lane_get(p_off, src);
-                    pdx = __uint_as_float(lane_get(__float_as_uint(p_dx), src));
-                    pdy = __uint_as_float(lane_get(__float_as_uint(p_dy), src));
-                    pdz = __uint_as_float(lane_get(__float_as_uint(p_dz), src));
-                    umet = __uint_as_float(lane_get(__float_as_uint(p_um), src));
+                    // in place of register copies on the VALU; the cluster lies
+                    // inside one wave, so the source never wraps)
+                    const int src4 = gb4 + (int)(k << 2);
+                    xo = __builtin_amdgcn_ds_bpermute(src4, (int)p_off);
+                    pdx = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dx)));
+                    pdy = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dy)));
+                    pdz = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dz)));
+                    umet = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_um)));
                 } else if (KL == 1) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m, cur, curB);
                     if constexpr (!F32) thr = metro_thr(u53(curB.z, curB.w), beta);
@@ -381,6 +408,8 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 }
                 bool accept = false;
                 double dE_used = 0.0;
+                float dE_f32 = 0.0f;  // binary32 atom moves: dE, widened only where traced
+                bool atom_f32 = false;
                 // NVT: every index is an atom; NPT: index N is the volume move
                 if (PRE || i < (uint32_t)N) {
                     const int li = xo & (LANES - 1);
@@ -492,13 +521,17 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     if constexpr (F32) {
                         // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
                         const float dEf = group_sum_rt((float)sn, KL);
-                        dE_used = (double)dEf;
+                        dE_f32 = dEf;
+                        atom_f32 = true;
                         const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
                         accept = (dEf <= 0.0f) || (pr >= (PRE ? umet : (float)d0.umet));
-                        if (accept) {
-                            E = E + (double)dEf;
-                            nacc += 1;
-                        }
+                        // E + 0 = E: the same bits as adding only on accept; the select on
+                        // the binary32 value (the empty asm keeps the compiler from moving
+                        // the widening before it: one select instead of two)
+                        float dacc = accept ? dEf : 0.0f;
+                        __asm__("" : "+v"(dacc));
+                        E = E + (double)dacc;
+                        nacc += (int)accept;
                     } else {
                         sn = group_sum_rt(sn, KL);
                         so = group_sum_rt(so, KL);
@@ -584,7 +617,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 }
                 if (a.trace_de != nullptr && q == 0 && a.trace_cycle + cc < a.trace_cycles) {
                     const size_t ti = ((size_t)(tb + t) * a.trace_cycles + a.trace_cycle + cc) * nsteps + m;
-                    a.trace_de[ti] = dE_used;
+                    a.trace_de[ti] = atom_f32 ? (double)dE_f32 : dE_used;
                     a.trace_acc[ti] = (int8_t)accept;
                     a.trace_idx[ti] = PRE ? (int32_t)(((xo / (3 * LANES)) << kls) + ((xo & (LANES - 1)) - gb))
                                           : (int32_t)i;
