@@ -928,23 +928,57 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
             u32x4 mA = {0, 0, 0, 0}, mB = {0, 0, 0, 0};
             u32x4 nA = {0, 0, 0, 0}, nB = {0, 0, 0, 0};
             double mthr = 0.0, nthr = 0.0;  // FP64: Metropolis thresholds of the lane's and the next move
+            // FP64 in the spherical container: the lane that draws a move's words
+            // also decodes them (LDS offset and lane of the atom, displacement
+            // (u - 1/2) delta), so the cluster fetches ready values instead of
+            // every lane decoding every move on the move's serial chain
+            constexpr bool PRE = !F32 && BC == BC_SPHERE && KL > 1;
+            int m_off = 0, m_li = 0, n_off = 0, n_li = 0;
+            double m_dx = 0.0, m_dy = 0.0, m_dz = 0.0, n_dx = 0.0, n_dy = 0.0, n_dz = 0.0;
+            auto pre_decode = [&]() {
+                const MoveDraw<Real> dw = decode_move<Real>(mA, mB, nsteps);
+                m_li = gb + (int)(dw.index % KL);
+                m_off = xoff<LANES>((int)(dw.index / KL), 0, m_li);
+                m_dx = (dw.ux - 0.5) * delta_r;
+                m_dy = (dw.uy - 0.5) * delta_r;
+                m_dz = (dw.uz - 0.5) * delta_r;
+            };
+            auto pre_fetch = [&](int src) {
+                n_off = (int)lane_get((uint32_t)m_off, src);
+                n_li = (int)lane_get((uint32_t)m_li, src);
+                n_dx = lane_get_d(m_dx, src);
+                n_dy = lane_get_d(m_dy, src);
+                n_dz = lane_get_d(m_dz, src);
+            };
             if constexpr (KL > 1) {
                 if ((uint32_t)q < nsteps) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, (uint32_t)q, mA, mB);
                     if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                    if constexpr (PRE) pre_decode();
                 }
-                fetch_words<Real>(mA, mB, gb, nA, nB);
+                if constexpr (PRE) pre_fetch(gb);
+                else fetch_words<Real>(mA, mB, gb, nA, nB);
                 if constexpr (!F32) nthr = lane_get_d(mthr, gb);
             }
             for (uint32_t m = 0; m < nsteps; ++m) {
                 // ---- draws of this move (src/MCRun.jl:145, src/MCMoves.jl:34-38, src/MCRun.jl:123)
-                MoveDraw<Real> d0;
+                MoveDraw<Real> d0{};
                 double thr;
+                int xo = 0, li = 0;  // LDS offset of the moved atom's x, the lane holding it
+                double pdx = 0.0, pdy = 0.0, pdz = 0.0;
                 if constexpr (KL == 1) {
                     d0 = move_draw<Real>(a.seed, run_key, (uint32_t)t, sweep, m, 0u, nsteps);
                     thr = F32 ? 0.0 : metro_thr((double)d0.umet, beta);
                 } else {
-                    d0 = decode_move<Real>(nA, nB, nsteps);
+                    if constexpr (PRE) {
+                        xo = n_off;
+                        li = n_li;
+                        pdx = n_dx;
+                        pdy = n_dy;
+                        pdz = n_dz;
+                    } else {
+                        d0 = decode_move<Real>(nA, nB, nsteps);
+                    }
                     thr = nthr;
                     const uint32_t m1 = m + 1;
                     if (m1 < nsteps) {
@@ -952,25 +986,41 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         if (k1 == 0 && m1 + q < nsteps) {
                             move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m1 + q, mA, mB);
                             if constexpr (!F32) mthr = metro_thr(u53(mB.z, mB.w), beta);
+                            if constexpr (PRE) pre_decode();
                         }
-                        fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
+                        if constexpr (PRE) pre_fetch(gb + (int)k1);
+                        else fetch_words<Real>(mA, mB, gb + (int)k1, nA, nB);
                         if constexpr (!F32) nthr = lane_get_d(mthr, gb + (int)k1);
                     }
                 }
-                const int i = (int)d0.index;
+                // NVT in the sphere: every index is an atom; NPT: index N is the volume move
+                const int i = PRE ? -1 : (int)d0.index;
+                if constexpr (!PRE) {
+                    if (i < N) {
+                        li = gb + (i % KL);
+                        xo = xoff<LANES>(i / KL, 0, li);
+                    }
+                }
                 bool accept = false;
                 double dE_used = 0.0;
-                if (i < N) {
-                    const int si = i / KL, li = gb + (i % KL);
-                    Real xi = X[xoff<LANES>(si, 0, li)];
-                    Real yi = X[xoff<LANES>(si, 1, li)];
-                    Real zi = X[xoff<LANES>(si, 2, li)];
+                if (PRE || i < N) {
+                    Real xi = X[xo];
+                    Real yi = X[xo + LANES];
+                    Real zi = X[xo + 2 * LANES];
                     Real tx, ty, tz;
-                    if constexpr (PBC) box_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, bxr, tx, ty, tz);
-                    else make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
+                    if constexpr (PBC) {
+                        box_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, bxr, tx, ty, tz);
+                    } else if constexpr (PRE) {
+                        tx = xi + pdx;
+                        ty = yi + pdy;
+                        tz = zi + pdz;
+                    } else {
+                        make_trial<Real, BC, DIV>(xi, yi, zi, d0, delta_r, Real(1), Real(1), tx, ty, tz);
+                    }
                     if constexpr (BC == BC_SPHERE) {
                         // src/MCMoves.jl:115-135: redraw until inside, recentre at 50, error at 100
                         if (outside_sphere<Real>(tx, ty, tz, radius2)) {
+                            const int si = xo / (3 * LANES);
                             if (!sphere_redraw<Real, KL, LANES>(X, tid, q, N, A, si, li, radius2, delta_r, a.seed, run_key,
                                                                 (uint32_t)t, sweep, m, nsteps, xi, yi, zi, tx, ty, tz) &&
                                 q == 0)
@@ -978,9 +1028,9 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         }
                         // park the moved atom so its self pair contributes zero
                         if (tid == li) {
-                            X[xoff<LANES>(si, 0, li)] = far_point<Real>();
-                            X[xoff<LANES>(si, 1, li)] = far_point<Real>();
-                            X[xoff<LANES>(si, 2, li)] = far_point<Real>();
+                            X[xo] = far_point<Real>();
+                            X[xo + LANES] = far_point<Real>();
+                            X[xo + 2 * LANES] = far_point<Real>();
                         }
                         __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     }
@@ -1092,9 +1142,9 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     }
                     // commit (src/swap_config.jl:40-51) or restore the parked atom
                     if (tid == li) {
-                        X[xoff<LANES>(si, 0, li)] = accept ? tx : xi;
-                        X[xoff<LANES>(si, 1, li)] = accept ? ty : yi;
-                        X[xoff<LANES>(si, 2, li)] = accept ? tz : zi;
+                        X[xo] = accept ? tx : xi;
+                        X[xo + LANES] = accept ? ty : yi;
+                        X[xo + 2 * LANES] = accept ? tz : zi;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 }
@@ -1204,7 +1254,7 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     const size_t ti = ((size_t)(tb + t) * a.trace_cycles + cyc) * nsteps + m;
                     a.trace_de[ti] = dE_used;
                     a.trace_acc[ti] = (int8_t)accept;
-                    a.trace_idx[ti] = i;
+                    a.trace_idx[ti] = PRE ? (xo / (3 * LANES)) * KL + (li - gb) : i;
                 }
                 if constexpr (ENS == ENS_NPT) {
                     // an accepted volume move changes the box for the remaining moves
