@@ -280,7 +280,11 @@ KernelPair lookup(int precision, int bc, int ens, int pot, int pk, int kc, int k
 size_t tile_lds(size_t rsize, int n_atoms, int kl, int lanes, bool eam)
 {
     const int A = (n_atoms + kl - 1) / kl;
-    return rsize * (eam ? 7 : 3) * A * lanes + (size_t)(lanes / kl) * (8 + 8 + 4 + 4);
+    // positions; per slot: exchange energy, volume, label, slot map; FP64 pair
+    // sweeps also 64 B of per-temperature state (four counters, two sums,
+    // step size, beta)
+    const bool rs = !eam && rsize == sizeof(double);
+    return rsize * (eam ? 7 : 3) * A * lanes + (size_t)(lanes / kl) * (8 + 8 + 4 + 4 + (rs ? 64 : 0));
 }
 
 int check_run(ptmc_engine *h, int32_t run)

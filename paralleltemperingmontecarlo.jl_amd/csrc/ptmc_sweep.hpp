@@ -854,6 +854,34 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
     double *exV = exE + NSLOT;
     int *exT = reinterpret_cast<int *>(exV + NSLOT);
     int *exS = exT + NSLOT;
+    // FP64: per-temperature state of the tile's ladders for the launch,
+    // indexed by the local temperature slot lb + t (each temperature belongs
+    // to one tile): read at the start, updated here, written back at the end,
+    // so the per-cycle bookkeeping waits on no global read-modify-write (the
+    // latency-bound single ladders).  Binary32 tiles, whose LDS is the tight
+    // resource, keep it in global memory.
+    constexpr bool RS = !F32;
+    long long *rsC = reinterpret_cast<long long *>(exS + NSLOT);  // atom moves (total, window), exchanges (attempted, accepted)
+    double *rsH = reinterpret_cast<double *>(rsC + 4 * NSLOT);    // sums of E (NPT: H) and its square
+    double *rsD = rsH + 2 * NSLOT;                                 // atom step size
+    double *rsB = rsD + NSLOT;                                     // beta
+    // k: 0, 1 atom moves (total, window), 2, 3 exchanges (attempted, accepted)
+    auto count = [&](int lt, int T, int k) -> long long & {
+        if constexpr (RS) return rsC[4 * lt + k];
+        else return k < 2 ? a.t_count_atom[2 * T + k] : a.t_count_exc[2 * T + k - 2];
+    };
+    auto ham = [&](int lt, int T, int k) -> double & {
+        if constexpr (RS) return rsH[2 * lt + k];
+        else return a.t_ham[2 * T + k];
+    };
+    auto displ = [&](int lt, int T) -> double & {
+        if constexpr (RS) return rsD[lt];
+        else return a.t_displ[2 * T];
+    };
+    auto beta_of = [&](int lt, int T) -> double {
+        if constexpr (RS) return rsB[lt];
+        else return a.t_beta[T];
+    };
 
     // coefficients: Real copy for the move path, FP64 copy for totals
     Real c[KC];
@@ -903,7 +931,28 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         if (q == 0) {
             exT[lg] = t;
             exS[lb + t] = lg;
+            const int lt = lb + t, T0 = tb + t;
+            if constexpr (RS) {
+            rsC[4 * lt] = a.t_count_atom[2 * T0];
+            rsC[4 * lt + 1] = a.t_count_atom[2 * T0 + 1];
+            rsC[4 * lt + 2] = a.t_count_exc[2 * T0];
+            rsC[4 * lt + 3] = a.t_count_exc[2 * T0 + 1];
+            rsH[2 * lt] = a.t_ham[2 * T0];
+            rsH[2 * lt + 1] = a.t_ham[2 * T0 + 1];
+            rsD[lt] = a.t_displ[2 * T0];
+            rsB[lt] = a.t_beta[T0];
+            }
         }
+    }
+    // the run's histogram bounds and pressure for the sampling
+    double en_min = 0.0, delta_en = 1.0, v_min = 0.0, delta_v = 1.0, pressure = 0.0;
+    if (active) {
+        const RunInfo &ri = a.runs[a.slot_run[slot]];
+        en_min = ri.en_min;
+        delta_en = ri.delta_en;
+        v_min = ri.v_min;
+        delta_v = ri.delta_v;
+        pressure = ri.pressure;
     }
     __syncthreads();
 
@@ -911,8 +960,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         const uint32_t sweep = (uint32_t)(a.sweep0 + cyc);
         const long long pidx = a.cycle0 + cyc + 1;
         if (active) {
-            const double beta = a.t_beta[tb + t];
-            const double delta = a.t_displ[2 * (tb + t)];
+            const double beta = beta_of(lb + t, tb + t);
+            const double delta = displ(lb + t, tb + t);
             const Real delta_r = (Real)delta;
             const float beta_l2e = (float)beta * 1.4426950408889634f;
             BoxC<Real> bxr{};
@@ -1265,8 +1314,8 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                 }
             }
             if (q == 0) {
-                a.t_count_atom[2 * (tb + t)] += nacc;
-                a.t_count_atom[2 * (tb + t) + 1] += nacc;
+                count(lb + t, tb + t, 0) += nacc;
+                count(lb + t, tb + t, 1) += nacc;
                 if (ENS == ENS_NPT) {
                     a.t_count_vol[2 * (tb + t)] += nvacc;
                     a.t_count_vol[2 * (tb + t) + 1] += nvacc;
@@ -1305,14 +1354,14 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     e2 = e2 + pressure * exV[sb];
                 }
                 const double de = e1 - e2;
-                const double db = a.t_beta[tb + n] - a.t_beta[tb + n + 1];
+                const double db = beta_of(lb + n, tb + n) - beta_of(lb + n + 1, tb + n + 1);
                 double acc = det_exp(db * de);
                 if (acc > 1.0) acc = 1.0;
-                a.t_count_exc[2 * (tb + n)] += 1;
-                a.t_count_exc[2 * (tb + n + 1)] += 1;
+                count(lb + n, tb + n, 2) += 1;
+                count(lb + n + 1, tb + n + 1, 2) += 1;
                 if (acc > u) {
-                    a.t_count_exc[2 * (tb + n) + 1] += 1;
-                    a.t_count_exc[2 * (tb + n + 1) + 1] += 1;
+                    count(lb + n, tb + n, 3) += 1;
+                    count(lb + n + 1, tb + n + 1, 3) += 1;
                     exT[sa] = n + 1;
                     exT[sb] = n;
                     exS[lb + n] = sb;
@@ -1325,13 +1374,13 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         if (active) {
             t = exT[lg];
             if (q == 0) {
-                const int T = tb + t;
+                const int T = tb + t, lt = lb + t;
                 // update_max_stepsize!, src/Exchange.jl:198-263
                 if (pidx % a.n_adjust == 0) {
-                    const double ar = (double)a.t_count_atom[2 * T + 1] / (double)((long long)a.n_adjust * N);
-                    if (ar < a.min_acc) a.t_displ[2 * T] *= 0.9;
-                    else if (ar > a.max_acc) a.t_displ[2 * T] *= 1.1;
-                    a.t_count_atom[2 * T + 1] = 0;
+                    const double ar = (double)count(lt, T, 1) / (double)((long long)a.n_adjust * N);
+                    if (ar < a.min_acc) displ(lt, T) *= 0.9;
+                    else if (ar > a.max_acc) displ(lt, T) *= 1.1;
+                    count(lt, T, 1) = 0;
                     if (ENS == ENS_NPT && !a.separated) {
                         const double av = (double)a.t_count_vol[2 * T + 1] / (double)a.n_adjust;
                         if (av < a.min_acc) a.t_displ[2 * T + 1] *= 0.9;
@@ -1354,24 +1403,24 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                     }
                 }
                 // sampling_step!, src/Sampling.jl:29-66, :258-280, :330-371
+                // (histogram counts by integer atomics, which the wave does not wait for)
                 if (a.sampling && pidx % a.mc_sample == 0) {
-                    const RunInfo &ri = a.runs[a.slot_run[slot]];
                     if (ENS == ENS_NPT) {
                         const double v = box_volume(kind, box, boxh);
-                        const double h = E + ri.pressure * v;
-                        a.t_ham[2 * T] += h;
-                        a.t_ham[2 * T + 1] += h * h;
-                        const double ie = __builtin_floor((E - ri.en_min) / ri.delta_en) + 1.0;
-                        const double iv = __builtin_floor((box_hist_volume(kind, box, boxh) - ri.v_min) / ri.delta_v) + 1.0;
+                        const double h = E + pressure * v;
+                        ham(lt, T, 0) += h;
+                        ham(lt, T, 1) += h * h;
+                        const double ie = __builtin_floor((E - en_min) / delta_en) + 1.0;
+                        const double iv = __builtin_floor((box_hist_volume(kind, box, boxh) - v_min) / delta_v) + 1.0;
                         const int he = ie < 1.0 ? 1 : (ie > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(ie) + 1);
                         const int hv = iv < 1.0 ? 1 : (iv > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(iv) + 1);
-                        a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)] += 1u;
+                        atomicAdd(&a.t_hist[(size_t)T * a.hist_stride + (size_t)(he - 1) * (a.n_bin + 2) + (hv - 1)], 1u);
                     } else {
-                        a.t_ham[2 * T] += E;
-                        a.t_ham[2 * T + 1] += E * E;
-                        const double hi = (E - ri.en_min) / ri.delta_en + 1.0;
+                        ham(lt, T, 0) += E;
+                        ham(lt, T, 1) += E * E;
+                        const double hi = (E - en_min) / delta_en + 1.0;
                         const int h = hi < 1.0 ? 1 : (hi > (double)a.n_bin ? a.n_bin + 2 : (int)__builtin_floor(hi) + 1);
-                        a.t_hist[(size_t)T * a.hist_stride + (h - 1)] += 1u;
+                        atomicAdd(&a.t_hist[(size_t)T * a.hist_stride + (h - 1)], 1u);
                     }
                 }
             }
@@ -1396,6 +1445,17 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
         if constexpr (PBC) {
             a.slot_box[slot] = box;
             a.slot_boxh[slot] = boxh;
+        }
+        // the per-temperature state of the temperature this slot holds now
+        const int lt = lb + t, T = tb + t;
+        if constexpr (RS) {
+        a.t_count_atom[2 * T] = rsC[4 * lt];
+        a.t_count_atom[2 * T + 1] = rsC[4 * lt + 1];
+        a.t_count_exc[2 * T] = rsC[4 * lt + 2];
+        a.t_count_exc[2 * T + 1] = rsC[4 * lt + 3];
+        a.t_ham[2 * T] = rsH[2 * lt];
+        a.t_ham[2 * T + 1] = rsH[2 * lt + 1];
+        a.t_displ[2 * T] = rsD[lt];
         }
     }
 }
