@@ -136,32 +136,18 @@ __device__ double wave_pair_sum64_rt(const Real *X, int cl, int N, int kl, int k
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
     const int nl = __popcll(live);
-    const int h = N / 2;
-    const bool even = (N & 1) == 0;
     const int cb = (threadIdx.x & ~63) + cl;
-    int j = rank % N, d = rank / N + 1;
-    double part = 0.0;
-    while (d <= h) {
-        int l = j + d;
-        if (l >= N) l -= N;
-        if (!(even && d == h && j >= h)) {
-            const int sj = j >> kls, lj = cb + (j & (kl - 1));
-            const int sl = l >> kls, ll = cb + (l & (kl - 1));
-            const double xj = (double)X[xoff<LANES>(sj, 0, lj)] * 1.0;
-            const double yj = (double)X[xoff<LANES>(sj, 1, lj)] * 1.0;
-            const double zj = (double)X[xoff<LANES>(sj, 2, lj)] * 1.0;
-            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * 1.0;
-            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * 1.0;
-            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * 1.0;
-            const double r2 = box_d2<double, BC, false>(xj, yj, zj, xl, yl, zl, bx) * m2;
-            if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
-        }
-        j += nl;
-        while (j >= N) {
-            j -= N;
-            ++d;
-        }
-    }
+    auto pos = [&](int j, double &x, double &y, double &z) {
+        const int sj = j >> kls, lj = cb + (j & (kl - 1));
+        x = (double)X[xoff<LANES>(sj, 0, lj)] * 1.0;
+        y = (double)X[xoff<LANES>(sj, 1, lj)] * 1.0;
+        z = (double)X[xoff<LANES>(sj, 2, lj)] * 1.0;
+    };
+    auto energy = [&](double xj, double yj, double zj, double xl, double yl, double zl) -> double {
+        const double r2 = box_d2<double, BC, false>(xj, yj, zj, xl, yl, zl, bx) * m2;
+        return r2 <= rc ? pair_energy64<PK, KC>(r2, c64) : 0.0;
+    };
+    double part = wave_pair_walk(N, rank, nl, pos, energy);
     part = group_sum_rt(part, kl);
     double tot = 0.0;
     for (uint64_t b = leaders; b != 0; b &= b - 1) tot += readlane_d(part, __builtin_ctzll(b));

@@ -735,6 +735,55 @@ __device__ __forceinline__ double readlane_d(double v, int src)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// The pair walk of the wave sums: the N(N-1)/2 pairs as (j, j + d mod N)
+// for d = 1..N/2 (at d = N/2 of an even N only j < N/2), pair slot
+// p = (d - 1) N + j dealt round-robin over the nl live lanes (this lane's
+// rank first).  Four slots per step, each into its own partial sum, so the
+// four pair chains (positions, minimum image, division, polynomial) overlap
+// instead of running one after another; the partial sums are added as
+// (s0 + s1) + (s2 + s3).  `pos(j, x, y, z)` reads atom j, `energy` the pair
+// energy (0 beyond the cut-off).
+template <typename Pos, typename En>
+__device__ __forceinline__ double wave_pair_walk(int N, int rank, int nl, Pos &&pos, En &&energy)
+{
+    const int h = N / 2;
+    const bool even = (N & 1) == 0;
+    const int P = h * N;
+    const int step = 4 * nl;
+    // slot advance of a step as (d, j) increments
+    const int dj = step % N, dd = step / N;
+    int j[4], d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int p = rank + u * nl;
+        j[u] = p % N;
+        d[u] = p / N + 1;
+    }
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int p0 = rank; p0 < P; p0 += step) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ju = j[u], du = d[u];
+            if (p0 + u * nl < P && !(even && du == h && ju >= h)) {
+                int l = ju + du;
+                if (l >= N) l -= N;
+                double xj, yj, zj, xl, yl, zl;
+                pos(ju, xj, yj, zj);
+                pos(l, xl, yl, zl);
+                s[u] += energy(xj, yj, zj, xl, yl, zl);
+            }
+            int jn = ju + dj, dn = du + dd;
+            if (jn >= N) {
+                jn -= N;
+                dn += 1;
+            }
+            j[u] = jn;
+            d[u] = dn;
+        }
+    }
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 // The pair sum of pair_sum64 (NPT form, cut-off rc) for the configuration
 // held by the cluster whose first lane is wave lane `cl`, computed by every
 // executing lane of the wave together (volume-move energies of the binary32
@@ -743,7 +792,7 @@ __device__ __forceinline__ double readlane_d(double v, int src)
 // over the `live` lanes; cluster butterflies, then the cluster totals of the
 // `leaders` (first lanes) in lane order.  Every lane returns the same value.
 // Pairs and cut-off test are those of pair_sum64; only the summation order
-// differs.
+// differs (wave_pair_walk).
 template <int PK, int KC, int BC, int KL, int LANES, typename Real>
 __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<double> &bx, double px, double pz,
                                   double m2, double rc, const double (&c64)[KC], const double (&ca64)[KC],
@@ -753,41 +802,26 @@ __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<doubl
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
     const int nl = __popcll(live);
-    const int h = N / 2;
-    const bool even = (N & 1) == 0;
     const int cb = (threadIdx.x & ~63) + cl;  // block lane of the cluster's first atom column
-    int j = rank % N, d = rank / N + 1;
-    double part = 0.0;
-    while (d <= h) {
-        int l = j + d;
-        if (l >= N) l -= N;
-        if (!(even && d == h && j >= h)) {
-            const int sj = j / KL, lj = cb + (j % KL);
-            const int sl = l / KL, ll = cb + (l % KL);
-            const double xj = (double)X[xoff<LANES>(sj, 0, lj)] * px;
-            const double yj = (double)X[xoff<LANES>(sj, 1, lj)] * px;
-            const double zj = (double)X[xoff<LANES>(sj, 2, lj)] * pz;
-            const double xl = (double)X[xoff<LANES>(sl, 0, ll)] * px;
-            const double yl = (double)X[xoff<LANES>(sl, 1, ll)] * px;
-            const double zl = (double)X[xoff<LANES>(sl, 2, ll)] * pz;
-            if constexpr (PK == PK_ELJB || PK == PK_LUT) {
-                double dxy2, dz;
-                const double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz) * m2;
-                if (r2 <= rc) {
-                    if constexpr (PK == PK_ELJB) part += pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
-                    else part += pair_energy_lut<double>(r2, dxy2, dz, pp);
-                }
-            } else {
-                const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
-                if (r2 <= rc) part += pair_energy64<PK, KC>(r2, c64);
-            }
+    auto pos = [&](int j, double &x, double &y, double &z) {
+        const int sj = j / KL, lj = cb + (j % KL);
+        x = (double)X[xoff<LANES>(sj, 0, lj)] * px;
+        y = (double)X[xoff<LANES>(sj, 1, lj)] * px;
+        z = (double)X[xoff<LANES>(sj, 2, lj)] * pz;
+    };
+    auto energy = [&](double xj, double yj, double zj, double xl, double yl, double zl) -> double {
+        if constexpr (PK == PK_ELJB || PK == PK_LUT) {
+            double dxy2, dz;
+            const double r2 = box_d2c<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx, dxy2, dz) * m2;
+            if (r2 > rc) return 0.0;
+            if constexpr (PK == PK_ELJB) return pair_energy_b<double, KC>(r2, dxy2, dz, c64, ca64, cb64);
+            else return pair_energy_lut<double>(r2, dxy2, dz, pp);
+        } else {
+            const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
+            return r2 <= rc ? pair_energy64<PK, KC>(r2, c64) : 0.0;
         }
-        j += nl;
-        while (j >= N) {
-            j -= N;
-            ++d;
-        }
-    }
+    };
+    double part = wave_pair_walk(N, rank, nl, pos, energy);
     part = group_sum<KL>(part);
     double tot = 0.0;
     for (uint64_t b = leaders; b != 0; b &= b - 1) tot += readlane_d(part, __builtin_ctzll(b));
