@@ -145,7 +145,7 @@ __device__ double wave_pair_sum64_rt(const Real *X, int cl, int N, int kl, int k
     };
     auto energy = [&](double xj, double yj, double zj, double xl, double yl, double zl) -> double {
         const double r2 = box_d2<double, BC, false>(xj, yj, zj, xl, yl, zl, bx) * m2;
-        return r2 <= rc ? pair_energy64<PK, KC>(r2, c64) : 0.0;
+        return r2 <= rc ? pair_energy64_vol<PK, KC>(r2, c64) : 0.0;
     };
     double part = wave_pair_walk(N, rank, nl, pos, energy);
     part = group_sum_rt(part, kl);

@@ -114,6 +114,26 @@ __device__ __forceinline__ double pair_energy64(double r2, const double (&c)[KC]
         return pair_energy<double, PK_EVEN_H, KC>(r2, c);
 }
 
+// The same for the volume-move energies of the binary32 paths (the wave pair
+// sums): 1/r^2 from the hardware reciprocal and two Newton steps, within an
+// ulp of the division and without its scaling and fix-up passes (these
+// paths' running totals carry binary32 errors; their FP64 recomputes use
+// pair_energy64).
+template <int PK, int KC>
+__device__ __forceinline__ double pair_energy64_vol(double r2, const double (&c)[KC])
+{
+    if constexpr (PK == PK_EVEN_R || PK == PK_ODD_R || PK == PK_ODD_H) {
+        return pair_energy<double, PK, KC>(r2, c);
+    } else {
+        double u = __builtin_amdgcn_rcp(r2);
+        double e = __builtin_fma(-r2, u, 1.0);
+        u = __builtin_fma(u, e, u);
+        e = __builtin_fma(-r2, u, 1.0);
+        u = __builtin_fma(u, e, u);
+        return pair_energy_u<double, PK_EVEN_H, KC>(u, c);
+    }
+}
+
 // ELJPotentialB pair energy (src/EnergyEvaluation.jl:943-958) from r^2 and the
 // pair vector's squared xy extent dxy2 and z extent dz: c_k r^-p_k (1 + a_k
 // cos 2t + b_k cos 4t), p = 6, 8, 9, 10, ..., t the angle to z; 0.1 inside
@@ -818,7 +838,7 @@ __device__ double wave_pair_sum64(const Real *X, int cl, int N, const BoxC<doubl
             else return pair_energy_lut<double>(r2, dxy2, dz, pp);
         } else {
             const double r2 = box_d2<double, BC, DIV>(xj, yj, zj, xl, yl, zl, bx) * m2;
-            return r2 <= rc ? pair_energy64<PK, KC>(r2, c64) : 0.0;
+            return r2 <= rc ? pair_energy64_vol<PK, KC>(r2, c64) : 0.0;
         }
     };
     double part = wave_pair_walk(N, rank, nl, pos, energy);
