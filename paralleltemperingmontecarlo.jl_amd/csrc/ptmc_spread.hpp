@@ -538,8 +538,9 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 }
-                if constexpr (ENS == ENS_NPT) {
-                    // uniform volume move: r^2 of the current positions scaled by s^2
+                // uniform volume move: r^2 of the current positions scaled by s^2
+                // (binary32: skipped by a wave none of whose clusters makes one)
+                if (ENS == ENS_NPT && (!F32 || __ballot(i >= (uint32_t)N) != 0)) {
                     const bool vm = i >= (uint32_t)N;
                     double sx = 1.0;
                     if (vm) {
