@@ -862,9 +862,12 @@ def main():
                                           else None),
             "cycles_per_launch": cyc_launch,
             "kernel": kernel_symbol(gi["lanes_per_replica"], gi["workgroup_lanes"], args.exchange_every > 1),
-            "kernel_flops_per_launch": dom_flops / max(gi["launches"], 1),
+            # spread layout: HIP events bracket one cycle's launches in four
+            # (kernel_launches_timed); every launch carries the same work
+            "kernel_flops_per_launch": dom_flops / max(nlaunch if spread else gi["launches"], 1),
             "kernel_avg_launch_ms": gi["ms"] / max(gi["launches"], 1),
-            "kernel_launches": gi["launches"],
+            "kernel_launches": nlaunch if spread else gi["launches"],
+            "kernel_launches_timed": gi["launches"],
             "kernel_workgroups": gi["tiles"],
             "aggregate_achieved": aggregate,
             "step_kernel_ms": kms / args.steps,

@@ -248,7 +248,9 @@ int ptmc_run_lanes(ptmc_handle h, int32_t run);
 int ptmc_kernel_time(ptmc_handle h, double *total_ms, int64_t *n_launches, int32_t reset);
 /* Launch groups (one kernel instantiation each; groups run concurrently):
  * lanes per configuration, workgroup width, workgroups, and the summed
- * duration of the group's own launches (events on the group's stream).
+ * duration and count of the group's timed launches (events on the group's
+ * stream; every launch in the ladder layout, one cycle's launches in four in
+ * the spread layout).  ptmc_kernel_time's n_launches counts every launch.
  * ptmc_run_group returns the group a run was planned into (-1 on error). */
 int ptmc_group_info(ptmc_handle h, int32_t group, int32_t *lanes_per_replica, int32_t *workgroup_lanes,
                     int32_t *n_tiles, double *total_ms, int64_t *n_launches);
