@@ -1708,6 +1708,10 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
     if (rc) return rc;
     HIPCHK(h, hipSetDevice(h->cfg.device));
     int64_t done = 0;
+    // spread layout: whether the side streams must wait for work pending on
+    // the engine stream (set-up copies, counter resets), asked before the
+    // start event goes onto it
+    const bool engine_busy = h->spread && h->launches.size() > 1 && hipStreamQuery(h->stream) != hipSuccess;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     if (is_slot(h)) {
         // one launch per cycle, then exchange / adaptation / sampling
@@ -1760,9 +1764,9 @@ int ptmc_cycles(ptmc_handle h, int64_t n_cycles)
         const size_t ng = h->launches.size();
         // groups on their own streams (one group: the engine stream), joined at the end
         auto stream_of = [&](size_t g) { return ng == 1 ? h->stream : h->side[g % h->side.size()]; };
-        // the side streams start after the engine stream's pending work (set-up
-        // copies, counter resets); nothing to wait for when it is idle
-        if (ng > 1 && hipStreamQuery(h->stream) != hipSuccess) {
+        // the side streams start after the engine stream's pending work;
+        // nothing to wait for when it was idle
+        if (ng > 1 && engine_busy) {
             HIPCHK(h, hipEventRecord(h->fork, h->stream));
             for (size_t g = 0; g < ng; ++g) HIPCHK(h, hipStreamWaitEvent(stream_of(g), h->fork, 0));
         }
