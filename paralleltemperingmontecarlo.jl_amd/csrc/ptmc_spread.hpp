@@ -312,6 +312,9 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
           if (a.n_cycles > 0) {
             const Real delta_r = (Real)delta;
             const float beta_l2e = (float)beta * 1.4426950408889634f;
+            // decoded draws carry the Metropolis threshold -log2(u) / (beta log2 e):
+            // exp2(-beta log2e dE) >= u  <=>  dE <= threshold
+            const float inv_bl2e = 1.0f / beta_l2e;
             // binary32 displacement (u - 1/2) delta as one fma of the 24-bit
             // integer of u: u - 1/2 and the products by powers of two are
             // exact, so the result is the rounding of the same exact value
@@ -349,7 +352,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                 MoveDraw<Real> d0{};
                 uint32_t i = 0u;  // PRE: formed from xo where needed
                 int xo = 0;       // LDS offset of the moved atom's x
-                float pdx = 0.0f, pdy = 0.0f, pdz = 0.0f, umet = 0.0f;
+                float pdx = 0.0f, pdy = 0.0f, pdz = 0.0f, fthr = 0.0f;
                 if constexpr (PRE) {
                     const uint32_t k = m & (uint32_t)(KL - 1);
                     if (k == 0u && m + (uint32_t)q < nsteps) {
@@ -361,7 +364,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         p_dx = __builtin_fmaf((float)(w.y >> 8), dsc, dof);
                         p_dy = __builtin_fmaf((float)(w.z >> 8), dsc, dof);
                         p_dz = __builtin_fmaf((float)(w.w >> 8), dsc, dof);
-                        p_um = (float)dw.umet;
+                        p_um = -__builtin_amdgcn_logf((float)dw.umet) * inv_bl2e;
                     }
                     // (also with one lane per configuration: LDS-pipe permutes
                     // in place of register copies on the VALU; the cluster lies
@@ -371,7 +374,7 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                     pdx = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dx)));
                     pdy = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dy)));
                     pdz = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_dz)));
-                    umet = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_um)));
+                    fthr = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(p_um)));
                 } else if (KL == 1) {
                     move_words<Real>(a.seed, run_key, (uint32_t)t, sweep, m, cur, curB);
                     if constexpr (!F32) thr = metro_thr(u53(curB.z, curB.w), beta);
@@ -509,8 +512,12 @@ __global__ void __launch_bounds__(LANES) spread_sweep_kernel(SweepArgs a)
                         const float dEf = group_sum_rt((float)sn, KL);
                         dE_f32 = dEf;
                         atom_f32 = true;
-                        const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
-                        accept = (dEf <= 0.0f) || (pr >= (PRE ? umet : (float)d0.umet));
+                        if constexpr (PRE) {
+                            accept = dEf <= fthr;  // the threshold is >= 0: dE <= 0 accepts
+                        } else {
+                            const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
+                            accept = (dEf <= 0.0f) || (pr >= (float)d0.umet);
+                        }
                         // E + 0 = E: the same bits as adding only on accept; the select on
                         // the binary32 value (the empty asm keeps the compiler from moving
                         // the widening before it: one select instead of two)

@@ -1224,8 +1224,14 @@ __global__ void __launch_bounds__(LANES) sweep_kernel(SweepArgs a)
                         // Metropolis in binary32 (src/Exchange.jl:53-57, src/MCRun.jl:122-132)
                         const float dEf = group_sum<KL>((float)sn);
                         dE_used = (double)dEf;
-                        const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
-                        accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                        if constexpr (!PBC) {
+                            // exp2(-beta log2e dE) >= u  <=>  dE <= -log2(u) / (beta log2 e),
+                            // the rule (and the rounding) of the spread layout's decoded draws
+                            accept = dEf <= -__builtin_amdgcn_logf(d0.umet) * (1.0f / beta_l2e);
+                        } else {
+                            const float pr = __builtin_amdgcn_exp2f(-dEf * beta_l2e);
+                            accept = (dEf <= 0.0f) || (pr >= d0.umet);
+                        }
                         if (accept) {
                             E = E + (double)dEf;
                             nacc += 1;
